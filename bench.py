@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""MAC-projection benchmark on one MI355X.
+
+Workload: 3D 256^3 plane-parallel MAC projection, 64 boxes of 64^3, x/y
+periodic, z slip walls, isothermal beta0 with 1e4 contrast, rho = beta0 (1 +
+0.01 N(0,1)), U ~ U[-1,1], solved to 1e-10 relative residual.  One step is
+one complete ``macproject`` call (beta0 scaling, divergence, coefficients,
+solve, face correction, ghost fill) on device-resident per-box arrays, each
+step on its own fresh copy of the inputs (staged in HBM before timing).
+
+``value`` is the mixed-precision path (FP32 V-cycle inside FP64 iterative
+refinement) in solves/s; the plain FP64 V-cycle is measured the same way
+and reported beside it.  ``roofline`` covers the dominant kernel of the
+headline path (finest-level GSRB colour pass), timed with HIP events on the
+library's own stream during the timed steps.  ``cpu_baseline`` is the FP64
+oracle (C/OpenMP restatement) on the same inputs, on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def gsrb_launch_bytes(ncell: int, s: int) -> float:
+    """Algorithmic bytes of one GSRB colour launch: every face weight once
+    (3s per cell), the rhs of the updated colour (s/2), the other colour's
+    values (s/2) and the updated colour written (s/2) -> 4.5 s per cell."""
+    return 4.5 * s * ncell
+
+
+def stage(per_box, copies, torch):
+    """Device copies of the per-box inputs: ``copies`` independent sets of
+    umac/phi (modified in place by a call) and one shared rho."""
+    boxes, umac, phi, rho, _, _ = per_box
+    sets = []
+    for _ in range(copies):
+        du = [[torch.from_numpy(a).cuda() for a in box] for box in umac]
+        dp = [torch.from_numpy(a).cuda() for a in phi]
+        sets.append((du, dp))
+    dr = [torch.from_numpy(a).cuda() for a in rho]
+    return sets, dr
+
+
+def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch):
+    plan.set_opts(precision=prec, device_ptrs=1)
+    stats = []
+    for w in range(warmup):
+        du, dp = sets[w]
+        stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        du, dp = sets[warmup + k]
+        stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = plan.profile_read()
+    plan.profile(False)
+    return (t1 - t0), stats[warmup:], prof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--box", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    per_box_inputs = BX.split_workload
+
+    n = (args.n,) * 3
+    t_gen = time.perf_counter()
+    wl = W.make_workload(2, n=n, box=args.box)
+    pb = per_box_inputs(wl)
+    print(f"# inputs ready in {time.perf_counter() - t_gen:.1f}s", file=sys.stderr, flush=True)
+    copies = args.warmup + args.steps
+    sets, dr = stage(pb, copies, torch)
+    b0 = torch.from_numpy(np.ascontiguousarray(wl["beta0"])).cuda()
+    b0e = torch.from_numpy(np.ascontiguousarray(wl["beta0_edge"])).cuda()
+    lay = mac.Layout.uniform(n, args.box, wl["dx"], wl["bc"])
+    plan = mac.MacPlan(lay, mac.default_opts(device_ptrs=1))
+    ncell = n[0] * n[1] * n[2]
+
+    results = {}
+    for name, prec in (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED)):
+        # refresh the staged inputs from the host copies (outside timing)
+        boxes, umac, phi, _, _, _ = pb
+        for du, dp in sets:
+            for box_d, box_h in zip(du, umac):
+                for a, h in zip(box_d, box_h):
+                    a.copy_(torch.from_numpy(h))
+            for a, h in zip(dp, phi):
+                a.copy_(torch.from_numpy(h))
+        torch.cuda.synchronize()
+        dt, st, prof = run_precision(plan, sets, dr, b0, b0e, args.warmup, args.steps, prec, torch)
+        results[name] = {"seconds": dt, "stats": st, "prof": prof,
+                         "umac": sets[-1][0], "phi": sets[-1][1]}
+        print(f"# {name}: {args.steps / dt:.2f} solves/s, {1e3 * dt / args.steps:.2f} ms/solve, "
+              f"iters {[s['iters'] for s in st]}, res/res0 {st[-1]['res'] / st[-1]['res0']:.2e}",
+              file=sys.stderr, flush=True)
+
+    mixed, fp64 = results["mixed"], results["fp64"]
+    value = args.steps / mixed["seconds"]
+    cnt, ms = mixed["prof"]["gsrb32"]
+    avg_ms = ms / max(cnt, 1)
+    achieved = gsrb_launch_bytes(ncell, 4) / (avg_ms * 1e-3) / 1e9
+    cnt64, ms64 = fp64["prof"]["gsrb64"]
+    avg64 = ms64 / max(cnt64, 1)
+    ach64 = gsrb_launch_bytes(ncell, 8) / (avg64 * 1e-3) / 1e9
+
+    # accuracy of the mixed path against the FP64 GPU path and the oracle
+    dim = 3
+    nn = list(n)
+    boxes = pb[0]
+
+    def gather(res):
+        faces = [BX.gather_faces([u[d].cpu().numpy() for u in res["umac"]], boxes, d, 1, nn, dim)
+                 for d in range(dim)]
+        ph = BX.gather_cells([p.cpu().numpy() for p in res["phi"]], boxes, 1, nn, dim)
+        return faces, ph
+
+    acc = {}
+    if not args.no_check:
+        fm, pm = gather(mixed)
+        f6, p6 = gather(fp64)
+
+        def errs(fa, pa, fb, pb_):
+            pa0, pb0 = pa - pa.mean(), pb_ - pb_.mean()
+            return {
+                "phi_max": float(np.abs(pa0 - pb0).max() / np.abs(pb0).max()),
+                "phi_rel": float(np.linalg.norm(pa0 - pb0) / np.linalg.norm(pb0)),
+                "u_max": float(max(np.abs(fa[d] - fb[d]).max() / np.abs(fb[d]).max() for d in range(dim))),
+                "u_rel": float(max(np.linalg.norm(fa[d] - fb[d]) / np.linalg.norm(fb[d]) for d in range(dim))),
+            }
+        acc["mixed_vs_fp64_gpu"] = errs(fm, pm, f6, p6)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        O.build()
+        g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
+        t = time.perf_counter()
+        ref = O.macproject(g, wl["faces"], wl["phi"], wl["rho"], beta0=wl["beta0"],
+                           beta0_edge=wl["beta0_edge"], opts=O.default_opts())
+        tc = time.perf_counter() - t
+        cpu = {"value": 1.0 / tc, "unit": "solves/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"one full 256^3 FP64 solve of the same inputs by the C/OpenMP oracle "
+                         f"({ref['stats'].iters} V-cycles, {tc:.1f} s)"}
+        if not args.no_check:
+            fm, pm = gather(mixed)
+            f6, p6 = gather(fp64)
+            po = ref["phi"] - ref["phi"].mean()
+            for tag, (fa, pa) in (("mixed_vs_oracle", (fm, pm)), ("fp64_vs_oracle", (f6, p6))):
+                pa0 = pa - pa.mean()
+                acc[tag] = {
+                    "phi_max": float(np.abs(pa0 - po).max() / np.abs(po).max()),
+                    "phi_rel": float(np.linalg.norm(pa0 - po) / np.linalg.norm(po)),
+                    "u_max": float(max(np.abs(fa[d] - ref["faces"][d]).max() /
+                                       np.abs(ref["faces"][d]).max() for d in range(dim))),
+                    "u_rel": float(max(np.linalg.norm(fa[d] - ref["faces"][d]) /
+                                       np.linalg.norm(ref["faces"][d]) for d in range(dim))),
+                }
+        print(f"# cpu oracle: {tc:.2f} s/solve on {O.num_threads()} threads", file=sys.stderr)
+
+    line = {
+        "metric": "MAC solves/s at 256^3 to 1e-10 resid: FP32-MG+FP64 IR vs FP64; HBM GB/s",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["seconds"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycle + f64 refinement",
+        "data": "synthetic (seeded, BASELINE config 2 inputs)",
+        "config": {"workload": "3D 256^3 plane-parallel MAC projection, 64 boxes of 64^3, "
+                               "x/y periodic, z walls, beta0 contrast 1e4, mixed FP32-MG + FP64 IR, "
+                               "rel residual 1e-10",
+                   "cells": ncell, "boxes": len(boxes), "parallelism": "single GPU"},
+        "fp64_solves_per_s": round(args.steps / fp64["seconds"], 3),
+        "fp64_ms_per_step": round(1e3 * fp64["seconds"] / args.steps, 3),
+        "mixed_over_fp64": round(fp64["seconds"] / mixed["seconds"], 3),
+        "iters": {"fp64_vcycles": fp64["stats"][-1]["iters"],
+                  "mixed_ir_steps": mixed["stats"][-1]["iters"],
+                  "mixed_fp32_vcycles": mixed["stats"][-1]["inner_cycles"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_gsrb<3,float> (finest level, one colour)",
+                     "avg_launch_us": round(1e3 * avg_ms, 2), "launches": cnt,
+                     "bytes_per_launch": gsrb_launch_bytes(ncell, 4)},
+        "roofline_fp64": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
+                          "kernel": "k_gsrb<3,double> (finest level, one colour)",
+                          "avg_launch_us": round(1e3 * avg64, 2), "launches": cnt64},
+        "kernel_ms_per_solve": {
+            p: {k: round(v[1] / args.steps, 3) for k, v in results[p]["prof"].items()}
+            for p in ("fp64", "mixed")},
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

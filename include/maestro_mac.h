@@ -1,0 +1,208 @@
+/*
+ * maestro_mac.h -- C ABI of the MI355X (gfx950) MAC-projection library
+ * (libmaestro_mac.so).
+ *
+ * This library stands in for MAESTRO's single-level MAC projection:
+ *   macproject(mla, umac, phi, rho, dx, the_bc_tower
+ *              [, macrhs][, beta0_1d, beta0_1d_edge][, beta0_cart_edge])
+ *     Source/macproject.f90:51-76
+ * together with the FBoxLib cell-centred multigrid solve that
+ *   mac_multigrid(mla, rh, phi, fine_flx, alpha, beta, dx, bc_tower, bc_comp,
+ *                 stencil_order, rel_eps, abs_eps)
+ *     Source/mac_multigrid.f90:18-19,56-67
+ * hands it to.  The Fortran module maestro_amd/fortran/macproject_hip.f90
+ * binds these entry points through iso_c_binding under the reference's own
+ * name and argument list; maestro_amd/mac.py binds them with ctypes.
+ *
+ * Data layout (per box, as FBoxLib's fab storage, architecture.tex:617-644):
+ * every array is Fortran column-major over global 0-based indices,
+ *   cell array  a(lo1-ng:hi1+ng, lo2-ng:hi2+ng[, lo3-ng:hi3+ng])
+ *   face array of direction d: the same, with hi_d+1+ng in direction d.
+ * The pointer handed in for a box addresses element (lo-ng, ...).  Arrays
+ * of box pointers are ordered box-major: umac[b*dim + d].
+ *
+ * Single level only.  The boxes must tile the domain exactly.
+ * Return codes: MAC_OK, or one of MAC_ERR_* below.
+ */
+#ifndef MAESTRO_MAC_H
+#define MAESTRO_MAC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physical boundary codes (FBoxLib bc_module) */
+#define MAC_PERIODIC (-1)
+#define MAC_INTERIOR 0
+#define MAC_INLET 11
+#define MAC_OUTLET 12
+#define MAC_SYMMETRY 13
+#define MAC_SLIP_WALL 14
+#define MAC_NO_SLIP_WALL 15
+
+/* return codes */
+#define MAC_OK 0
+#define MAC_ERR_INVALID 1        /* invalid argument or option combination */
+#define MAC_ERR_LAYOUT 2         /* boxes do not tile the domain, bad ghosts */
+#define MAC_ERR_NONFINITE 3      /* NaN/Inf in rho, umac or the right side */
+#define MAC_ERR_NOT_CONVERGED 4  /* max_iter reached; stats are filled */
+#define MAC_ERR_HIP 5            /* a HIP runtime call failed */
+#define MAC_ERR_NOMEM 6
+
+/* precision of the V-cycle */
+#define MAC_PREC_FP64 0          /* plain FP64 V-cycle (the reference) */
+#define MAC_PREC_MIXED 1         /* FP32 V-cycle inside FP64 refinement */
+
+typedef struct mac_layout {
+    int dim;                /* 2 or 3 */
+    int nboxes;
+    const int *box_lo;      /* [nboxes*3] inclusive lower cell index (z unused in 2D) */
+    const int *box_hi;      /* [nboxes*3] inclusive upper cell index */
+    int domain_lo[3];       /* problem domain, inclusive */
+    int domain_hi[3];
+    double dx[3];
+    int phys_bc[3][2];      /* [direction][lo, hi] physical BC code */
+    int ng_umac;            /* ghost widths of the caller's arrays */
+    int ng_phi;
+    int ng_rho;             /* >= 1: rho ghosts feed the boundary coefficients */
+    int ng_macrhs;
+    int ng_beta0_cart;
+} mac_layout;
+
+typedef struct mac_opts {
+    double eps_rel;          /* relative residual target; default 1e-10
+                                (min(eps_mac*10^(nlevs-1), eps_mac_max),
+                                Source/macproject.f90:217, mg_eps.f90:22-28) */
+    double eps_abs;          /* absolute target; <= 0 disables (:209)       */
+    int max_iter;            /* V-cycles (or refinement steps); default 200 */
+    int nu1, nu2;            /* GSRB sweeps down / up; default 2, 2         */
+    int nub;                 /* sweeps around the bottom solve; default 10
+                                (mg_bottom_nu, Source/_parameters:392)       */
+    double bottom_eps;       /* bottom CG relative tolerance; default 1e-3
+                                (Source/mac_multigrid.f90:61)                */
+    int bottom_max_iter;     /* default 1000 */
+    int max_levels;          /* 0: coarsen down to 2 cells per direction     */
+    int use_alt_energy_fix;  /* 1: beta = beta0^2/rho (reference default,
+                                Source/_parameters:552-554); 0: beta0/rho    */
+    int precision;           /* MAC_PREC_FP64 or MAC_PREC_MIXED              */
+    int inner_cycles;        /* FP32 V-cycles per refinement step (mixed)    */
+    int device_ptrs;         /* 1: every array pointer is a device pointer;
+                                0: host pointers (copied in and out)         */
+    int verbose;
+} mac_opts;
+
+typedef struct mac_stats {
+    int iters;               /* V-cycles (FP64) or refinement steps (mixed) */
+    int inner_cycles;        /* FP32 V-cycles in total (mixed)             */
+    int converged;
+    int nlevels;
+    double rhs_norm;         /* max |rh| after mean removal               */
+    double res0;             /* max |rh - A phi0|  ("error0")             */
+    double res;              /* final max |rh - A phi|                     */
+    double t_h2d_ms, t_setup_ms, t_solve_ms, t_correct_ms, t_d2h_ms, t_total_ms;
+} mac_stats;
+
+typedef struct mac_plan mac_plan;
+
+/* Fill *o with the defaults listed above. */
+int mac_default_opts(mac_opts *o);
+
+/* Build a plan: validates the layout, allocates the device MG hierarchy and
+ * scratch.  One plan serves any number of calls on the same layout (the two
+ * MAC projections of a time step, Source/advance.f90:466,937).  The plan
+ * owns one HIP stream; all work of a call runs on it and the call returns
+ * after it completes. */
+int mac_plan_create(const mac_layout *layout, const mac_opts *opts, mac_plan **plan);
+int mac_plan_destroy(mac_plan *plan);
+
+/* Change solver options of an existing plan (tolerances, precision,
+ * device_ptrs, ...).  Layout-dependent fields are ignored. */
+int mac_plan_set_opts(mac_plan *plan, const mac_opts *opts);
+
+/* macproject, Source/macproject.f90:51-328 (single level):
+ *   umac  [nboxes*dim]  in/out edge velocities (ghosts filled on return:
+ *                       box/periodic images, then physical edge BCs)
+ *   phi   [nboxes]      in/out, warm start (ng_phi ghosts, not read)
+ *   rho   [nboxes]      cell density with >= 1 filled ghost layer
+ *   macrhs[nboxes]      optional (NULL): rh = macrhs - D(beta0 U)
+ *   beta0_1d[nr], beta0_1d_edge[nr+1]   optional planar beta0 along the
+ *                       last direction (nr = domain cells in it)
+ *   beta0_cart_edge[nboxes*dim]  optional face beta0 (spherical); at most
+ *                       one of the two beta0 forms may be given (:99-109)
+ *   stats               optional output */
+int mac_project(mac_plan *plan, double *const *umac, double *const *phi,
+                const double *const *rho, const double *const *macrhs,
+                const double *beta0_1d, const double *beta0_1d_edge,
+                const double *const *beta0_cart_edge, mac_stats *stats);
+
+/* ---- whole-domain device entry points (used by the autograd wrapper) ----
+ * Arrays are single device buffers over the whole domain, C order with x
+ * fastest: cells (nz,ny,nx); x faces (nz,ny,nx+1); y faces (nz,ny+1,nx);
+ * z faces (nz+1,ny,nx).  beta_* are the solver face coefficients (already
+ * including beta0).  They require opts.device_ptrs = 1. */
+
+/* Forward: rh = macrhs - D(U); solve (-D beta G) phi = rh; U -= beta G phi.
+ * Same operator, boundary handling and stopping rule as mac_project. */
+int mac_project_level(mac_plan *plan, double *ux, double *uy, double *uz,
+                      double *phi, const double *beta_x, const double *beta_y,
+                      const double *beta_z, const double *macrhs, mac_stats *stats);
+
+/* Adjoint of mac_project_level for L(phi, U_proj): given the forward phi
+ * (mean-removed where singular), dL/dphi (g_phi, cells) and dL/dU_proj
+ * (g_ux.., faces), solve A lambda = g_phi - G^T(beta g_U) with the same
+ * self-adjoint operator and return dL/dU (out_ux..) and dL/dbeta
+ * (out_bx..) per face, and lambda (cells). */
+int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
+                      const double *beta_y, const double *beta_z,
+                      const double *g_phi, const double *g_ux, const double *g_uy,
+                      const double *g_uz, double *lambda, double *out_ux,
+                      double *out_uy, double *out_uz, double *out_bx,
+                      double *out_by, double *out_bz, mac_stats *stats);
+
+/* ---- diagnostics (no counterpart in the reference) ----------------------
+ * Read or overwrite an internal FP64 array of MG level `level` (0 = finest)
+ * through host memory, and run single solver operators on it, so tests can
+ * compare each kernel with a CPU restatement.  `what`: MAC_ARR_* below;
+ * sizes are the level's cell count, or the slab sizes ny*nz, nx*nz, nx*ny
+ * for the high-face weights.  `op`: MAC_OP_* below, repeated `count` times. */
+#define MAC_ARR_PHI 0
+#define MAC_ARR_RHS 1
+#define MAC_ARR_BX 2
+#define MAC_ARR_BY 3
+#define MAC_ARR_BZ 4
+#define MAC_ARR_BXH 5
+#define MAC_ARR_BYH 6
+#define MAC_ARR_BZH 7
+#define MAC_ARR_RES 8
+#define MAC_OP_GSRB 0        /* one red-black sweep on `level`             */
+#define MAC_OP_RESIDUAL 1    /* res = rhs - A phi on `level`              */
+#define MAC_OP_RESTRICT 2    /* rhs(level+1) = average of res(level)      */
+#define MAC_OP_PROLONG 3     /* phi(level) += P phi(level+1)              */
+#define MAC_OP_COARSEN 4     /* weights(level+1) from weights(level)      */
+#define MAC_OP_VCYCLE 5      /* one V-cycle from `level` (0 only)         */
+int mac_debug_level_dims(mac_plan *plan, int level, int dims[3]);
+int mac_debug_get(mac_plan *plan, int level, int what, double *out);
+int mac_debug_set(mac_plan *plan, int level, int what, const double *in);
+int mac_debug_op(mac_plan *plan, int level, int op, int count);
+
+/* Kernel timing on the plan's stream (no reference counterpart): when
+ * enabled, launches of the finest-level kernels are bracketed by HIP
+ * events; mac_plan_profile_read returns the number of timed launches and
+ * their summed duration for one class.  Enabling resets the counters. */
+#define MAC_PROF_GSRB64 0     /* finest-level smoothing launches, FP64 */
+#define MAC_PROF_GSRB32 1     /* finest-level smoothing launches, FP32 */
+#define MAC_PROF_RESID64 2    /* finest-level FP64 residual/norm launches */
+#define MAC_PROF_SETUP 3      /* fused setup from the caller's boxes */
+#define MAC_PROF_CORRECT 4    /* face correction + ghost fill */
+#define MAC_PROF_COARSE 5     /* single-workgroup coarse-level cycle */
+#define MAC_PROF_NCLASS 6
+int mac_plan_profile(mac_plan *plan, int enable);
+int mac_plan_profile_read(mac_plan *plan, int cls, long *count, double *total_ms);
+
+/* Library version string. */
+const char *mac_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

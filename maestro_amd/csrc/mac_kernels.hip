@@ -1,0 +1,1163 @@
+// mac_kernels.hip -- gfx950 kernels of the MAC projection (libmaestro_mac.so).
+//
+// Reference routines each kernel replaces (Source/macproject.f90 unless noted):
+//   k_setup            mult_edge_by_1d_coeff (:577-609), divumac_3d (:457-480),
+//                      mk_mac_coeffs_3d (:698-738), the beta0 scaling of the
+//                      coefficients (:185-203) and the copy of phi into the
+//                      solver, fused into one pass over the caller's boxes
+//   k_mean_*           solvability: compensated FP64 mean of rh (singular BCs)
+//   k_coarsen          coarse-level face coefficients (FBoxLib stencil fill,
+//                      Docs/mg/mg.tex:333-336; averaging rule of this build)
+//   k_gsrb             red-black Gauss-Seidel (MG_SMOOTHER_GS_RB, mg.tex:201)
+//   k_residual         compute_defect (mg.tex:351)
+//   k_restrict         conservative-average restriction (mg.tex:353-354)
+//   k_prolong          linear prolongation + correction (mg.tex:363-365)
+//   k_coarse_cycle     the V-cycle on the coarse levels plus the FP64 CG
+//                      bottom solve (mg.tex:221-262) in one workgroup
+//   k_correct_valid    mkumac_3d (:908-975) fused with the division by beta0
+//                      (:232-236)
+//   k_fill_ghost       multifab_fill_boundary of umac (:245-251)
+//   k_physbc           multifab_physbc_edgevel (:254; rules of the in-tree
+//                      sample Exec/SCIENCE/flame/multifab_physbc_edgevel.f90:
+//                      211-321)
+#include "mac_launch.h"
+
+namespace mac {
+
+static constexpr int TPB = 256;
+
+// ------------------------------------------------------ per-box indexing
+__device__ __forceinline__ long boxoff(const BoxDev &B, int ng, int fd, int dim, int i, int j,
+                                       int k)
+{
+    long ex = B.hi[0] - B.lo[0] + 1 + 2 * ng + (fd == 0);
+    long ey = B.hi[1] - B.lo[1] + 1 + 2 * ng + (fd == 1);
+    long li = i - B.lo[0] + ng, lj = j - B.lo[1] + ng;
+    long lk = (dim == 3) ? (k - B.lo[2] + ng) : 0;
+    return li + ex * (lj + ey * lk);
+}
+
+__device__ __forceinline__ double fold_bc(double beta, int kind)
+{
+    if (kind == ELL_NEU) return 0.0;
+    if (kind == ELL_DIR) return 2.0 * beta;
+    return beta;
+}
+
+// beta0 of the face (dir d, global face index g[]) -- mult_edge_by_1d_coeff
+// uses beta0(r) on tangential faces and beta0_edge(r) on radial faces
+__device__ __forceinline__ double beta0_face(const SetupArgs &A, int b, int d, int i, int j,
+                                             int k, bool &has)
+{
+    if (A.beta0) {
+        has = true;
+        int rd = A.dim - 1;
+        int r = (rd == 2 ? k : j) - A.dlo[rd];
+        return (d == rd) ? A.beta0e[r] : A.beta0[r];
+    }
+    if (A.cart) {
+        has = true;
+        return A.cart[b * A.dim + d][boxoff(A.boxes[b], A.ng_cart, d, A.dim, i, j, k)];
+    }
+    has = false;
+    return 1.0;
+}
+
+// ------------------------------------------------------------- k_setup
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
+{
+    const int b = blockIdx.y;
+    const BoxDev B = A.boxes[b];
+    const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
+    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    double rh = 0.0;
+    bool bad = false;
+    if (q < (long)ex * ey * ez) {
+        const int i = B.lo[0] + (int)(q % ex);
+        const int j = B.lo[1] + (int)((q / ex) % ey);
+        const int k = (DIM == 3) ? B.lo[2] + (int)(q / ((long)ex * ey)) : 0;
+        const int I = i - A.dlo[0], J = j - A.dlo[1], K = (DIM == 3) ? k - A.dlo[2] : 0;
+        const int nx = A.n[0], ny = A.n[1];
+        const long c = I + (long)nx * (J + (long)ny * K);
+        const double *rho = A.rho[b];
+        auto R = [&](int ii, int jj, int kk) { return rho[boxoff(B, A.ng_rho, -1, DIM, ii, jj, kk)]; };
+        // ---- beta0-scaled face velocities and divergence (divumac)
+        double uf[DIM][2];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double *u = A.umac[b * DIM + d];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                int fi = i + (d == 0 ? s : 0), fj = j + (d == 1 ? s : 0), fk = k + (d == 2 ? s : 0);
+                double v = u[boxoff(B, A.ng_u, d, DIM, fi, fj, fk)];
+                bool has;
+                double b0 = beta0_face(A, b, d, fi, fj, fk, has);
+                if (has) v = v * b0;
+                uf[d][s] = v;
+            }
+        }
+        double D = (uf[0][1] - uf[0][0]) / A.dx[0] + (uf[1][1] - uf[1][0]) / A.dx[1];
+        if (DIM == 3) D = D + (uf[2][1] - uf[2][0]) / A.dx[2];
+        if (A.macrhs) {
+            double m = A.macrhs[b][boxoff(B, A.ng_rhs, -1, DIM, i, j, k)];
+            rh = -(D - m);
+        } else {
+            rh = -D;
+        }
+        A.L.rhs[c] = rh;
+        bad |= !isfinite(rh);
+        // ---- face coefficients (mk_mac_coeffs, beta0 scaling, BC folding)
+        const double rc = R(i, j, k);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            int di = (d == 0), dj = (d == 1), dk = (d == 2);
+            int Id = (d == 0) ? I : (d == 1 ? J : K);
+            double beta = 2.0 / (rc + R(i - di, j - dj, k - dk));
+            bool has;
+            double b0 = beta0_face(A, b, d, i, j, k, has);
+            if (has) {
+                beta = beta * b0;
+                if (A.alt) beta = beta * b0;
+            }
+            if (Id == 0) beta = fold_bc(beta, A.bc.k[d][0]);
+            bad |= !isfinite(beta);
+            double *bl = (d == 0) ? A.L.bx : (d == 1 ? A.L.by : A.L.bz);
+            bl[c] = beta;
+            if (Id == A.n[d] - 1) {
+                double bh = 2.0 / (R(i + di, j + dj, k + dk) + rc);
+                double b0h = beta0_face(A, b, d, i + di, j + dj, k + dk, has);
+                if (has) {
+                    bh = bh * b0h;
+                    if (A.alt) bh = bh * b0h;
+                }
+                bh = fold_bc(bh, A.bc.k[d][1]);
+                bad |= !isfinite(bh);
+                if (d == 0) A.L.bxh[J + (long)ny * K] = bh;
+                else if (d == 1) A.L.byh[I + (long)nx * K] = bh;
+                else A.L.bzh[I + (long)nx * J] = bh;
+            }
+        }
+        // ---- warm start
+        double p0 = A.phi[b][boxoff(B, A.ng_phi, -1, DIM, i, j, k)];
+        bad |= !isfinite(p0);
+        A.L.phi[c] = p0;
+    }
+    if (bad) atomicOr(A.err, 1);
+    // ---- block compensated partial sum of rh (solvability mean)
+    double s = rh, cmp = 0.0;
+    wave_comp_sum(s, cmp);
+    __shared__ double ss[TPB / 64], sc[TPB / 64];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[wv] = s; sc[wv] = cmp; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ss[0], Cc = sc[0];
+        for (int w = 1; w < TPB / 64; ++w) comp_add(S, Cc, ss[w], sc[w]);
+        long pid = (long)b * gridDim.x + blockIdx.x;
+        A.part[2 * pid] = S;
+        A.part[2 * pid + 1] = Cc;
+    }
+}
+
+// ------------------------------------ compensated mean of partial sums
+__global__ void __launch_bounds__(1024) k_mean_finalize(const double *part, long nparts, double ncell,
+                                                        double *mean_out)
+{
+    double s = 0.0, c = 0.0;
+    for (long p = threadIdx.x; p < nparts; p += blockDim.x) comp_add(s, c, part[2 * p], part[2 * p + 1]);
+    wave_comp_sum(s, c);
+    __shared__ double ss[16], sc[16];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[wv] = s; sc[wv] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ss[0], C = sc[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) comp_add(S, C, ss[w], sc[w]);
+        *mean_out = (S + C) / ncell;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_sub_mean(T *x, long n, const double *mean)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) x[q] = (T)((double)x[q] - *mean);
+}
+
+// ------------------------------------------------- block partial sums
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_partial_sum(const T *x, long n, double *part)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    double s = (q < n) ? (double)x[q] : 0.0, c = 0.0;
+    wave_comp_sum(s, c);
+    __shared__ double ss[TPB / 64], sc[TPB / 64];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[wv] = s; sc[wv] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ss[0], C = sc[0];
+        for (int w = 1; w < TPB / 64; ++w) comp_add(S, C, ss[w], sc[w]);
+        part[2 * blockIdx.x] = S;
+        part[2 * blockIdx.x + 1] = C;
+    }
+}
+
+// ------------------------------------------------ coarse coefficients
+// coarse face = average of the 2^(dim-1) fine faces covering it
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_coarsen(Lev<T> F, Lev<T> C)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long nc = (long)C.nx * C.ny * C.nz;
+    if (q >= nc) return;
+    int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), K = (int)(q / ((long)C.nx * C.ny));
+    int i = 2 * I, j = 2 * J, k = (DIM == 3) ? 2 * K : 0;
+    auto fc = [&](int ii, int jj, int kk) { return ii + (long)F.nx * (jj + (long)F.ny * kk); };
+    if (DIM == 3) {
+        const T q4 = T(0.25);
+        C.bx[q] = q4 * ((F.bx[fc(i, j, k)] + F.bx[fc(i, j + 1, k)]) +
+                        (F.bx[fc(i, j, k + 1)] + F.bx[fc(i, j + 1, k + 1)]));
+        C.by[q] = q4 * ((F.by[fc(i, j, k)] + F.by[fc(i + 1, j, k)]) +
+                        (F.by[fc(i, j, k + 1)] + F.by[fc(i + 1, j, k + 1)]));
+        C.bz[q] = q4 * ((F.bz[fc(i, j, k)] + F.bz[fc(i + 1, j, k)]) +
+                        (F.bz[fc(i, j + 1, k)] + F.bz[fc(i + 1, j + 1, k)]));
+        if (I == C.nx - 1)
+            C.bxh[J + (long)C.ny * K] =
+                q4 * ((F.bxh[j + (long)F.ny * k] + F.bxh[j + 1 + (long)F.ny * k]) +
+                      (F.bxh[j + (long)F.ny * (k + 1)] + F.bxh[j + 1 + (long)F.ny * (k + 1)]));
+        if (J == C.ny - 1)
+            C.byh[I + (long)C.nx * K] =
+                q4 * ((F.byh[i + (long)F.nx * k] + F.byh[i + 1 + (long)F.nx * k]) +
+                      (F.byh[i + (long)F.nx * (k + 1)] + F.byh[i + 1 + (long)F.nx * (k + 1)]));
+        if (K == C.nz - 1)
+            C.bzh[I + (long)C.nx * J] =
+                q4 * ((F.bzh[i + (long)F.nx * j] + F.bzh[i + 1 + (long)F.nx * j]) +
+                      (F.bzh[i + (long)F.nx * (j + 1)] + F.bzh[i + 1 + (long)F.nx * (j + 1)]));
+    } else {
+        const T h = T(0.5);
+        C.bx[q] = h * (F.bx[fc(i, j, 0)] + F.bx[fc(i, j + 1, 0)]);
+        C.by[q] = h * (F.by[fc(i, j, 0)] + F.by[fc(i + 1, j, 0)]);
+        if (I == C.nx - 1) C.bxh[J] = h * (F.bxh[j] + F.bxh[j + 1]);
+        if (J == C.ny - 1) C.byh[I] = h * (F.byh[i] + F.byh[i + 1]);
+    }
+}
+
+// FP64 -> FP32 copy of a level's face weights (mixed precision hierarchy)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_weights_to_f32(Lev<double> S, Lev<float> D)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long n = (long)S.nx * S.ny * S.nz;
+    if (q >= n) return;
+    D.bx[q] = (float)S.bx[q];
+    D.by[q] = (float)S.by[q];
+    if (DIM == 3) D.bz[q] = (float)S.bz[q];
+    int I = (int)(q % S.nx), J = (int)((q / S.nx) % S.ny), K = (int)(q / ((long)S.nx * S.ny));
+    if (I == 0) D.bxh[J + (long)S.ny * K] = (float)S.bxh[J + (long)S.ny * K];
+    if (J == 0) D.byh[I + (long)S.nx * K] = (float)S.byh[I + (long)S.nx * K];
+    if (DIM == 3 && K == 0) D.bzh[I + (long)S.nx * J] = (float)S.bzh[I + (long)S.nx * J];
+}
+
+// --------------------------------------------------------------- GSRB
+// one colour of a red-black sweep; thread per (i-pair, j, k)
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
+{
+    const int hx = (L.nx + 1) >> 1;
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long nq = (long)hx * L.ny * L.nz;
+    if (q >= nq) return;
+    int m = (int)(q % hx);
+    int j = (int)((q / hx) % L.ny);
+    int k = (int)(q / ((long)hx * L.ny));
+    int i = 2 * m + ((j + k + color) & 1);
+    if (i >= L.nx) return;
+    long c = i + (long)L.nx * (j + (long)L.ny * k);
+    T w[6], p[6];
+    stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
+    L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
+}
+
+// ------------------------------------------- residual (+ max norm)
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_residual(Lev<T> L, T *out, unsigned long long *nrm)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long n = (long)L.nx * L.ny * L.nz;
+    double a = 0.0;
+    if (q < n) {
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        T w[6], p[6];
+        stencil<DIM, T>(L, L.phi, i, j, k, q, w, p);
+        T r = L.rhs[q] - apply_value<DIM, T>(L, w, p, L.phi[q]);
+        if (out) out[q] = r;
+        a = fabs((double)r);
+    }
+    if (nrm) {
+        a = wave_max(a);
+        __shared__ double sm[TPB / 64];
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = sm[0];
+            for (int w = 1; w < TPB / 64; ++w) m = fmax(m, sm[w]);
+            atomic_max_nonneg(nrm, m);
+        }
+    }
+}
+
+// FP64 residual of the refinement loop, written as FP32 right side of the
+// FP32 V-cycle, with its max norm
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
+                                                    unsigned long long *nrm)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long n = (long)L.nx * L.ny * L.nz;
+    double a = 0.0;
+    if (q < n) {
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        double w[6], p[6];
+        stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
+        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, L.phi[q]);
+        r32[q] = (float)r;
+        a = fabs(r);
+    }
+    a = wave_max(a);
+    __shared__ double sm[TPB / 64];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = sm[0];
+        for (int w = 1; w < TPB / 64; ++w) m = fmax(m, sm[w]);
+        atomic_max_nonneg(nrm, m);
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_phi_add_f32(double *phi, const float *e, long n)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) phi[q] = phi[q] + (double)e[q];
+}
+
+// ------------------------------------------------------- restriction
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_restrict(const T *r, int fnx, int fny, Lev<T> C)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long nc = (long)C.nx * C.ny * C.nz;
+    if (q >= nc) return;
+    int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), K = (int)(q / ((long)C.nx * C.ny));
+    int i = 2 * I, j = 2 * J;
+    auto f = [&](int ii, int jj, int kk) { return r[ii + (long)fnx * (jj + (long)fny * kk)]; };
+    T v;
+    if (DIM == 3) {
+        int k = 2 * K;
+        v = ((f(i, j, k) + f(i + 1, j, k)) + (f(i, j + 1, k) + f(i + 1, j + 1, k))) +
+            ((f(i, j, k + 1) + f(i + 1, j, k + 1)) + (f(i, j + 1, k + 1) + f(i + 1, j + 1, k + 1)));
+        C.rhs[q] = T(0.125) * v;
+    } else {
+        v = (f(i, j, 0) + f(i + 1, j, 0)) + (f(i, j + 1, 0) + f(i + 1, j + 1, 0));
+        C.rhs[q] = T(0.25) * v;
+    }
+}
+
+// ------------------------------------------------------ prolongation
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_prolong(Lev<T> F, const T *e, int cnx, int cny, int cnz,
+                                                 EllBC bc)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    long n = (long)F.nx * F.ny * F.nz;
+    if (q >= n) return;
+    int i = (int)(q % F.nx), j = (int)((q / F.nx) % F.ny), k = (int)(q / ((long)F.nx * F.ny));
+    F.phi[q] = F.phi[q] + prolong_value<DIM, T>(e, cnx, cny, cnz, bc, i, j, k);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fill(T *x, long n, T v)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) x[q] = v;
+}
+
+// -------------------------------------------- single-workgroup coarse cycle
+// V-cycle over levels lc..nlev-1 and the FP64 CG bottom solve, all inside
+// one 1024-thread workgroup (these levels are latency bound; one launch
+// replaces dozens).  Level data lives in global memory (L2 resident).
+static constexpr int CTPB = 1024;
+
+__device__ __forceinline__ double blk_sum(double v, double *sh)
+{
+    // fixed-order block sum (deterministic)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double blk_max(double v, double *sh)
+{
+    v = wave_max(v);
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s = fmax(s, sh[w]);
+    __syncthreads();
+    return s;
+}
+
+template <int DIM, typename T>
+__device__ void blk_gsrb(const Lev<T> &L)
+{
+    const int hx = (L.nx + 1) >> 1;
+    long nq = (long)hx * L.ny * L.nz;
+    for (int color = 0; color < 2; ++color) {
+        for (long q = threadIdx.x; q < nq; q += blockDim.x) {
+            int m = (int)(q % hx);
+            int j = (int)((q / hx) % L.ny);
+            int k = (int)(q / ((long)hx * L.ny));
+            int i = 2 * m + ((j + k + color) & 1);
+            if (i >= L.nx) continue;
+            long c = i + (long)L.nx * (j + (long)L.ny * k);
+            T w[6], p[6];
+            stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
+            L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
+        }
+        __syncthreads();
+    }
+}
+
+// A x in FP64 with the level's (T) weights; x, out are FP64 arrays
+template <int DIM, typename T>
+__device__ void blk_apply64(const Lev<T> &L, const double *x, double *out)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        Lev<double> D;
+        D.nx = L.nx; D.ny = L.ny; D.nz = L.nz;
+        D.perx = L.perx; D.pery = L.pery; D.perz = L.perz;
+        D.ih2x = L.ih2x; D.ih2y = L.ih2y; D.ih2z = L.ih2z;
+        T wt[6], pt[6];
+        (void)pt;
+        // weights from the level, neighbour values from x
+        wt[0] = L.bx[q];
+        wt[1] = (i < L.nx - 1) ? L.bx[q + 1] : L.bxh[j + (long)L.ny * k];
+        wt[2] = L.by[q];
+        wt[3] = (j < L.ny - 1) ? L.by[q + L.nx] : L.byh[i + (long)L.nx * k];
+        if (DIM == 3) {
+            long sz = (long)L.nx * L.ny;
+            wt[4] = L.bz[q];
+            wt[5] = (k < L.nz - 1) ? L.bz[q + sz] : L.bzh[i + (long)L.nx * j];
+        }
+        double w[6], p[6];
+        const long sy = L.nx, sz = (long)L.nx * L.ny;
+        for (int s = 0; s < 2 * DIM; ++s) w[s] = (double)wt[s];
+        p[0] = (i > 0) ? x[q - 1] : (L.perx ? x[q + L.nx - 1] : 0.0);
+        p[1] = (i < L.nx - 1) ? x[q + 1] : (L.perx ? x[q - (L.nx - 1)] : 0.0);
+        p[2] = (j > 0) ? x[q - sy] : (L.pery ? x[q + (long)(L.ny - 1) * sy] : 0.0);
+        p[3] = (j < L.ny - 1) ? x[q + sy] : (L.pery ? x[q - (long)(L.ny - 1) * sy] : 0.0);
+        if (DIM == 3) {
+            p[4] = (k > 0) ? x[q - sz] : (L.perz ? x[q + (long)(L.nz - 1) * sz] : 0.0);
+            p[5] = (k < L.nz - 1) ? x[q + sz] : (L.perz ? x[q - (long)(L.nz - 1) * sz] : 0.0);
+        } else {
+            w[4] = w[5] = p[4] = p[5] = 0.0;
+        }
+        out[q] = apply_value<DIM, double>(D, w, p, x[q]);
+    }
+    __syncthreads();
+}
+
+template <int DIM, typename T>
+__device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit, int singular,
+                              double *sh)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    double *x = cg, *r = cg + n, *p = cg + 2 * n, *Ap = cg + 3 * n;
+    // rhs (mean removed on singular levels) and initial guess
+    double m = 0.0;
+    if (singular) {
+        double s = 0.0;
+        for (long q = threadIdx.x; q < n; q += blockDim.x) s += (double)L.rhs[q];
+        m = blk_sum(s, sh) / (double)n;
+    }
+    for (long q = threadIdx.x; q < n; q += blockDim.x) x[q] = (double)L.phi[q];
+    __syncthreads();
+    blk_apply64<DIM, T>(L, x, Ap);
+    double rs = 0.0;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+        r[q] = ((double)L.rhs[q] - m) - Ap[q];
+        rs += r[q];
+    }
+    if (singular) {
+        double rm = blk_sum(rs, sh) / (double)n;
+        for (long q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+    }
+    __syncthreads();
+    double a = 0.0, rr = 0.0;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+        a = fmax(a, fabs(r[q]));
+        rr += r[q] * r[q];
+        p[q] = r[q];
+    }
+    double r0 = blk_max(a, sh);
+    rr = blk_sum(rr, sh);
+    if (r0 > 0.0) {
+        for (int it = 0; it < maxit; ++it) {
+            blk_apply64<DIM, T>(L, p, Ap);
+            double pap = 0.0;
+            for (long q = threadIdx.x; q < n; q += blockDim.x) pap += p[q] * Ap[q];
+            pap = blk_sum(pap, sh);
+            if (!(pap > 0.0)) break;
+            double alpha = rr / pap;
+            double rsum = 0.0;
+            for (long q = threadIdx.x; q < n; q += blockDim.x) {
+                x[q] += alpha * p[q];
+                r[q] -= alpha * Ap[q];
+                rsum += r[q];
+            }
+            if (singular) {
+                double rm = blk_sum(rsum, sh) / (double)n;
+                for (long q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+            }
+            __syncthreads();
+            double am = 0.0, rr1 = 0.0;
+            for (long q = threadIdx.x; q < n; q += blockDim.x) {
+                am = fmax(am, fabs(r[q]));
+                rr1 += r[q] * r[q];
+            }
+            am = blk_max(am, sh);
+            rr1 = blk_sum(rr1, sh);
+            if (am <= eps * r0) break;
+            double beta = rr1 / rr;
+            rr = rr1;
+            for (long q = threadIdx.x; q < n; q += blockDim.x) p[q] = r[q] + beta * p[q];
+            __syncthreads();
+        }
+    }
+    for (long q = threadIdx.x; q < n; q += blockDim.x) L.phi[q] = (T)x[q];
+    __syncthreads();
+}
+
+template <int DIM, typename T>
+__device__ void blk_resid_restrict(const Lev<T> &F, const Lev<T> &C)
+{
+    long nc = (long)C.nx * C.ny * C.nz;
+    for (long q = threadIdx.x; q < nc; q += blockDim.x) {
+        int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), K = (int)(q / ((long)C.nx * C.ny));
+        T rv[2][2][2];
+        for (int c = 0; c < (DIM == 3 ? 2 : 1); ++c)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) {
+                    int i = 2 * I + a, j = 2 * J + b, k = (DIM == 3) ? 2 * K + c : 0;
+                    long f = i + (long)F.nx * (j + (long)F.ny * k);
+                    T w[6], p[6];
+                    stencil<DIM, T>(F, F.phi, i, j, k, f, w, p);
+                    rv[c][b][a] = F.rhs[f] - apply_value<DIM, T>(F, w, p, F.phi[f]);
+                }
+        T v;
+        if (DIM == 3) {
+            v = ((rv[0][0][0] + rv[0][0][1]) + (rv[0][1][0] + rv[0][1][1])) +
+                ((rv[1][0][0] + rv[1][0][1]) + (rv[1][1][0] + rv[1][1][1]));
+            C.rhs[q] = T(0.125) * v;
+        } else {
+            v = (rv[0][0][0] + rv[0][0][1]) + (rv[0][1][0] + rv[0][1][1]);
+            C.rhs[q] = T(0.25) * v;
+        }
+        C.phi[q] = T(0);
+    }
+    __syncthreads();
+}
+
+template <int DIM, typename T>
+__device__ void blk_prolong(const Lev<T> &F, const Lev<T> &C, const EllBC &bc)
+{
+    long n = (long)F.nx * F.ny * F.nz;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+        int i = (int)(q % F.nx), j = (int)((q / F.nx) % F.ny), k = (int)(q / ((long)F.nx * F.ny));
+        F.phi[q] = F.phi[q] + prolong_value<DIM, T>(C.phi, C.nx, C.ny, C.nz, bc, i, j, k);
+    }
+    __syncthreads();
+}
+
+template <int DIM, typename T>
+__global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict__ levs, int lc,
+                                                       int nlev, CoarseOpts o, double *cg)
+{
+    __shared__ double sh[CTPB / 64];
+    const int bot = nlev - 1;
+    for (int l = lc; l < bot; ++l) {
+        const Lev<T> L = levs[l];
+        for (int s = 0; s < o.nu1; ++s) blk_gsrb<DIM, T>(L);
+        blk_resid_restrict<DIM, T>(L, levs[l + 1]);
+    }
+    {
+        const Lev<T> L = levs[bot];
+        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+        blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
+        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+    }
+    for (int l = bot - 1; l >= lc; --l) {
+        const Lev<T> L = levs[l];
+        blk_prolong<DIM, T>(L, levs[l + 1], o.bc);
+        for (int s = 0; s < o.nu2; ++s) blk_gsrb<DIM, T>(L);
+    }
+}
+
+// ------------------------------------------------------ face correction
+// valid faces of every box: U = (beta0 U - w (phi_hi - phi_lo)/dx) / beta0
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
+{
+    const int b = blockIdx.y, d = blockIdx.z;
+    const BoxDev B = A.boxes[b];
+    const int ex = B.hi[0] - B.lo[0] + 1 + (d == 0), ey = B.hi[1] - B.lo[1] + 1 + (d == 1);
+    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 + (d == 2) : 1;
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    const int i = B.lo[0] + (int)(q % ex), j = B.lo[1] + (int)((q / ex) % ey);
+    const int k = (DIM == 3) ? B.lo[2] + (int)(q / ((long)ex * ey)) : 0;
+    int G[3] = {i - A.dlo[0], j - A.dlo[1], (DIM == 3) ? k - A.dlo[2] : 0};
+    const int nd = A.n[d];
+    // effective weight of the face and the phi values on both sides
+    int f = G[d];
+    int per = (A.bc.k[d][0] == ELL_PER);
+    if (per && f == nd) f = 0;
+    int Cu[3] = {G[0], G[1], G[2]};
+    Cu[d] = f;
+    const long sx = 1, sy = A.n[0], sz = (long)A.n[0] * A.n[1];
+    const long s[3] = {sx, sy, sz};
+    double w;
+    double phi_hi, phi_lo;
+    if (f < nd) {
+        long cu = Cu[0] + sy * Cu[1] + sz * Cu[2];
+        w = (d == 0 ? A.bx : (d == 1 ? A.by : A.bz))[cu];
+        phi_hi = A.phi[cu];
+        if (f > 0) phi_lo = A.phi[cu - s[d]];
+        else phi_lo = per ? A.phi[cu + (long)(nd - 1) * s[d]] : 0.0;
+    } else {  // domain high face, non-periodic
+        long cl = Cu[0] + sy * Cu[1] + sz * Cu[2] - s[d];
+        if (d == 0) w = A.bxh[G[1] + (long)A.n[1] * G[2]];
+        else if (d == 1) w = A.byh[G[0] + (long)A.n[0] * G[2]];
+        else w = A.bzh[G[0] + (long)A.n[0] * G[1]];
+        phi_hi = 0.0;
+        phi_lo = A.phi[cl];
+    }
+    double *u = A.umac[b * DIM + d];
+    long o = boxoff(B, A.ng_u, d, DIM, i, j, k);
+    double b0 = 1.0;
+    bool has = false;
+    if (A.beta0) {
+        int rd = DIM - 1;
+        int r = (rd == 2 ? k : j) - A.dlo[rd];
+        b0 = (d == rd) ? A.beta0e[r] : A.beta0[r];
+        has = true;
+    } else if (A.cart) {
+        b0 = A.cart[b * DIM + d][boxoff(B, A.ng_cart, d, DIM, i, j, k)];
+        has = true;
+    }
+    double t = u[o];
+    if (has) t = t * b0;
+    double gphi = (phi_hi - phi_lo) / A.dx[d];
+    t = t - w * gphi;
+    if (has) t = t / b0;
+    u[o] = t;
+}
+
+// ghost faces that lie inside the (periodic) domain take the value of the
+// owning box's valid face (multifab_fill_boundary semantics)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_fill_ghost(CorrectArgs A)
+{
+    const int b = blockIdx.y, d = blockIdx.z;
+    const BoxDev B = A.boxes[b];
+    const int ng = A.ng_u;
+    const int ex = B.hi[0] - B.lo[0] + 1 + 2 * ng + (d == 0);
+    const int ey = B.hi[1] - B.lo[1] + 1 + 2 * ng + (d == 1);
+    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 + 2 * ng + (d == 2) : 1;
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    int g[3] = {B.lo[0] - ng + (int)(q % ex), B.lo[1] - ng + (int)((q / ex) % ey),
+                (DIM == 3) ? B.lo[2] - ng + (int)(q / ((long)ex * ey)) : 0};
+    // inside this box's valid region?  then nothing to do
+    bool valid = true;
+    for (int e = 0; e < DIM; ++e) {
+        int hi = B.hi[e] + (e == d);
+        if (g[e] < B.lo[e] || g[e] > hi) valid = false;
+    }
+    if (valid) return;
+    // map to the domain (periodic wrap), give up outside non-periodic sides
+    int G[3];
+    for (int e = 0; e < DIM; ++e) {
+        int x = g[e] - A.dlo[e];
+        int ne = A.n[e];
+        int top = ne - 1 + (e == d);   // last valid index (faces include n)
+        if (x < 0 || x > top) {
+            if (A.bc.k[e][0] != ELL_PER) return;
+            x = ((x % ne) + ne) % ne;
+        }
+        G[e] = x;
+    }
+    if (DIM == 2) G[2] = 0;
+    // owner: box whose cell contains the face's cell index (n -> n-1)
+    int cell[3];
+    for (int e = 0; e < 3; ++e) cell[e] = G[e];
+    if (cell[d] == A.n[d]) cell[d] -= 1;
+    int mi = cell[0] / A.mapg[0], mj = cell[1] / A.mapg[1], mk = (DIM == 3) ? cell[2] / A.mapg[2] : 0;
+    int ob = A.boxmap[mi + A.mapn[0] * (mj + A.mapn[1] * mk)];
+    const BoxDev OB = A.boxes[ob];
+    int gi = G[0] + A.dlo[0], gj = G[1] + A.dlo[1], gk = (DIM == 3) ? G[2] + A.dlo[2] : 0;
+    double v = A.umac[ob * DIM + d][boxoff(OB, ng, d, DIM, gi, gj, gk)];
+    A.umac[b * DIM + d][boxoff(B, ng, d, DIM, g[0], g[1], g[2])] = v;
+}
+
+// physical edge-velocity BCs on the first ghost layer of one side
+// (Exec/SCIENCE/flame/multifab_physbc_edgevel.f90:211-321)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_physbc(CorrectArgs A, int dir, int side, int phys)
+{
+    const int b = blockIdx.y, comp = blockIdx.z;   // comp = face direction
+    const BoxDev B = A.boxes[b];
+    if (side == 0 && B.lo[dir] != A.dlo[dir]) return;
+    if (side == 1 && B.hi[dir] != A.dlo[dir] + A.n[dir] - 1) return;
+    const int ng = A.ng_u;
+    if (ng < 1) return;
+    // extents of the face array of direction comp (with ghosts)
+    int ext[3];
+    for (int e = 0; e < 3; ++e) ext[e] = (e < DIM) ? B.hi[e] - B.lo[e] + 1 + 2 * ng + (e == comp) : 1;
+    // transverse plane of the array across 'dir'
+    int t0 = (dir == 0) ? 1 : 0, t1 = (dir == 2) ? 1 : 2;
+    long nt = (long)ext[t0] * (DIM == 3 ? ext[t1] : 1);
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q >= nt) return;
+    int a0 = (int)(q % ext[t0]), a1 = (DIM == 3) ? (int)(q / ext[t0]) : 0;
+    bool normal = (comp == dir);
+    int lo = B.lo[dir], hi = B.hi[dir];
+    // index along dir of the target and the source (or -1 for none)
+    int tgt, src = -1000000;
+    double sign = 1.0, setv = 0.0;
+    bool set_zero = false;
+    if (side == 0) {
+        if (phys == MAC_SLIP_WALL || phys == MAC_NO_SLIP_WALL || phys == MAC_INLET) {
+            if (normal) return;   // normal ghost untouched (INLET_VEL is problem data)
+            tgt = lo - 1;
+            set_zero = true;
+        } else if (phys == MAC_OUTLET) {
+            tgt = lo - 1;
+            src = lo;
+        } else if (phys == MAC_SYMMETRY) {
+            tgt = lo - 1;
+            if (normal) { src = lo + 1; sign = -1.0; }
+            else src = lo;
+        } else {
+            return;
+        }
+    } else {
+        if (phys == MAC_SLIP_WALL || phys == MAC_NO_SLIP_WALL || phys == MAC_INLET) {
+            if (normal) return;
+            tgt = hi + 1;
+            set_zero = true;
+        } else if (phys == MAC_OUTLET) {
+            if (normal) { tgt = hi + 2; src = hi + 1; }
+            else { tgt = hi + 1; src = hi; }
+        } else if (phys == MAC_SYMMETRY) {
+            if (normal) { tgt = hi + 2; src = hi; sign = -1.0; }
+            else { tgt = hi + 1; src = hi; }
+        } else {
+            return;
+        }
+    }
+    (void)setv;
+    int gt[3], gs[3];
+    gt[t0] = gs[t0] = B.lo[t0] - ng + a0;
+    if (DIM == 3) gt[t1] = gs[t1] = B.lo[t1] - ng + a1;
+    else gt[2] = gs[2] = 0;
+    gt[dir] = tgt;
+    gs[dir] = src;
+    double *u = A.umac[b * DIM + comp];
+    long ot = boxoff(B, ng, comp, DIM, gt[0], gt[1], gt[2]);
+    if (set_zero) u[ot] = 0.0;
+    else u[ot] = sign * u[boxoff(B, ng, comp, DIM, gs[0], gs[1], gs[2])];
+}
+
+// ------------------------------- whole-domain (level) API kernels
+// natural-layout faces -> folded level weights, rh = macrhs - D(U), phi copy
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_level_setup(LevelIO A, Lev<double> L, double *part)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    long n = (long)nx * ny * nz;
+    double rh = 0.0;
+    if (q < n) {
+        int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long)nx * ny));
+        long fx = i + (long)(nx + 1) * (j + (long)ny * k);
+        long fy = i + (long)nx * (j + (long)(ny + 1) * k);
+        long fz = q;
+        double D = (A.ux[fx + 1] - A.ux[fx]) / A.dx[0] + (A.uy[fy + nx] - A.uy[fy]) / A.dx[1];
+        if (DIM == 3) D = D + (A.uz[fz + (long)nx * ny] - A.uz[fz]) / A.dx[2];
+        rh = A.macrhs ? -(D - A.macrhs[q]) : -D;
+        L.rhs[q] = rh;
+        L.phi[q] = A.phi[q];
+        L.bx[q] = (i == 0) ? fold_bc(A.bx[fx], A.bc.k[0][0]) : A.bx[fx];
+        L.by[q] = (j == 0) ? fold_bc(A.by[fy], A.bc.k[1][0]) : A.by[fy];
+        if (i == nx - 1) L.bxh[j + (long)ny * k] = fold_bc(A.bx[fx + 1], A.bc.k[0][1]);
+        if (j == ny - 1) L.byh[i + (long)nx * k] = fold_bc(A.by[fy + nx], A.bc.k[1][1]);
+        if (DIM == 3) {
+            L.bz[q] = (k == 0) ? fold_bc(A.bz[fz], A.bc.k[2][0]) : A.bz[fz];
+            if (k == nz - 1) L.bzh[i + (long)nx * j] = fold_bc(A.bz[fz + (long)nx * ny], A.bc.k[2][1]);
+        }
+    }
+    double s = rh, c = 0.0;
+    wave_comp_sum(s, c);
+    __shared__ double ss[TPB / 64], sc[TPB / 64];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[wv] = s; sc[wv] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ss[0], C = sc[0];
+        for (int w = 1; w < TPB / 64; ++w) comp_add(S, C, ss[w], sc[w]);
+        part[2 * blockIdx.x] = S;
+        part[2 * blockIdx.x + 1] = C;
+    }
+}
+
+// gradient of a cell field across every face of direction d, with the
+// folded weight w:  returns w and (v_hi - v_lo)/dx (periodic, Neumann w=0,
+// Dirichlet neighbour 0)
+template <int DIM>
+__device__ __forceinline__ void face_grad(const Lev<double> &L, const double *v, int d, int i, int j,
+                                          int k, double dxd, int per, double &w, double &g)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int n[3] = {nx, ny, nz};
+    int f = (d == 0) ? i : (d == 1 ? j : k);
+    int nd = n[d];
+    const long s[3] = {1, nx, (long)nx * ny};
+    int cc[3] = {i, j, k};
+    if (per && f == nd) f = 0;
+    cc[d] = f;
+    double hi, lo;
+    if (f < nd) {
+        long cu = cc[0] + s[1] * cc[1] + s[2] * cc[2];
+        w = (d == 0 ? L.bx : (d == 1 ? L.by : L.bz))[cu];
+        hi = v[cu];
+        lo = (f > 0) ? v[cu - s[d]] : (per ? v[cu + (long)(nd - 1) * s[d]] : 0.0);
+    } else {
+        long cl = cc[0] + s[1] * cc[1] + s[2] * cc[2] - s[d];
+        if (d == 0) w = L.bxh[j + (long)ny * k];
+        else if (d == 1) w = L.byh[i + (long)nx * k];
+        else w = L.bzh[i + (long)nx * j];
+        hi = 0.0;
+        lo = v[cl];
+    }
+    g = (hi - lo) / dxd;
+}
+
+// U -= w G phi on every face (natural layout)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_level_correct(LevelIO A, Lev<double> L, int d)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int ex = nx + (d == 0), ey = ny + (d == 1), ez = nz + (d == 2);
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    int i = (int)(q % ex), j = (int)((q / ex) % ey), k = (int)(q / ((long)ex * ey));
+    double w, g;
+    face_grad<DIM>(L, L.phi, d, i, j, k, A.dx[d], A.bc.k[d][0] == ELL_PER, w, g);
+    double *u = (d == 0) ? A.ux : (d == 1 ? A.uy : A.uz);
+    u[q] = u[q] - w * g;
+}
+
+// weight of the domain's high face in the row of cell (i,j,k) along d
+__device__ __forceinline__ double whi_row(const Lev<double> &L, int d, int i, int j, int k)
+{
+    if (d == 0) return L.bxh[j + (long)L.ny * k];
+    if (d == 1) return L.byh[i + (long)L.nx * k];
+    return L.bzh[i + (long)L.nx * j];
+}
+
+// adjoint right side: a = g_phi - G^T(w g_U)  (cells, natural-layout g_U)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_adj_rhs(AdjIO A, Lev<double> L, double *part)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    long n = (long)nx * ny * nz;
+    double a = 0.0;
+    if (q < n) {
+        int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((long)nx * ny));
+        // G^T(w gU) at cell c = sum over faces of dG_f/dphi_c * w_f gU_f
+        // face f = lo face of c: G_f = (phi_c - phi_lo)/dx  -> +1/dx
+        // face f = hi face of c: G_f = (phi_hi - phi_c)/dx  -> -1/dx
+        double acc = 0.0;
+        for (int d = 0; d < DIM; ++d) {
+            const double *gu = (d == 0) ? A.gux : (d == 1 ? A.guy : A.guz);
+            long flo, fhi;
+            if (d == 0) { flo = i + (long)(nx + 1) * (j + (long)ny * k); fhi = flo + 1; }
+            else if (d == 1) { flo = i + (long)nx * (j + (long)(ny + 1) * k); fhi = flo + nx; }
+            else { flo = q; fhi = q + (long)nx * ny; }
+            double wlo = (d == 0 ? L.bx : (d == 1 ? L.by : L.bz))[q];
+            double whi;
+            int ii = (d == 0) ? i : (d == 1 ? j : k), nd = (d == 0) ? nx : (d == 1 ? ny : nz);
+            if (ii < nd - 1) whi = (d == 0 ? L.bx : (d == 1 ? L.by : L.bz))[q + (d == 0 ? 1 : (d == 1 ? (long)nx : (long)nx * ny))];
+            else if (d == 0) whi = L.bxh[j + (long)ny * k];
+            else if (d == 1) whi = L.byh[i + (long)nx * k];
+            else whi = L.bzh[i + (long)nx * j];
+            double glo = gu[flo], ghi = gu[fhi];
+            double t = wlo * glo - whi * ghi;
+            // periodic rows: face n carries the same wrapped gradient as face 0,
+            // so it also touches cell 0 (as its high side) and face 0 also
+            // touches cell n-1 (as its low side)
+            if (A.bc.k[d][0] == ELL_PER) {
+                long row_f0 = flo - (long)ii * (d == 0 ? 1 : (d == 1 ? (long)nx : (long)nx * ny));
+                long row_fn = row_f0 + (long)nd * (d == 0 ? 1 : (d == 1 ? (long)nx : (long)nx * ny));
+                long row_c0 = q - (long)ii * (d == 0 ? 1 : (d == 1 ? (long)nx : (long)nx * ny));
+                if (ii == 0) t += whi_row(L, d, i, j, k) * gu[row_fn];
+                if (ii == nd - 1) t -= (d == 0 ? L.bx : (d == 1 ? L.by : L.bz))[row_c0] * gu[row_f0];
+            }
+            acc += t / A.dx[d];
+        }
+        a = A.gphi[q] - acc;
+        A.out_rhs[q] = a;
+    }
+    double s = a, c = 0.0;
+    wave_comp_sum(s, c);
+    __shared__ double ss[TPB / 64], sc[TPB / 64];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { ss[wv] = s; sc[wv] = c; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = ss[0], C = sc[0];
+        for (int w = 1; w < TPB / 64; ++w) comp_add(S, C, ss[w], sc[w]);
+        part[2 * blockIdx.x] = S;
+        part[2 * blockIdx.x + 1] = C;
+    }
+}
+
+// per-face gradients:  dL/dU = gU - D^T lambda,  dL/dbeta = -G(phi) (G(lambda) + gU)
+// (interior and periodic faces; Neumann faces carry no beta dependence;
+// Dirichlet faces use the folded one-sided gradients)
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_adj_faces(AdjIO A, Lev<double> L, const double *phi,
+                                                   const double *lam, int d)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int ex = nx + (d == 0), ey = ny + (d == 1), ez = nz + (d == 2);
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    int i = (int)(q % ex), j = (int)((q / ex) % ey), k = (int)(q / ((long)ex * ey));
+    int per = A.bc.k[d][0] == ELL_PER;
+    double w, gp, gl;
+    face_grad<DIM>(L, phi, d, i, j, k, A.dx[d], per, w, gp);
+    face_grad<DIM>(L, lam, d, i, j, k, A.dx[d], per, w, gl);
+    const double *gu = (d == 0) ? A.gux : (d == 1 ? A.guy : A.guz);
+    double *ou = (d == 0) ? A.out_ux : (d == 1 ? A.out_uy : A.out_uz);
+    double *ob = (d == 0) ? A.out_bx : (d == 1 ? A.out_by : A.out_bz);
+    // D^T lambda at face f = (lambda_lo - lambda_hi)/dx over the cells that
+    // exist (rh = macrhs - D U, so dL/dU through rh is -D^T lambda)
+    int n[3] = {nx, ny, nz};
+    int f = (d == 0) ? i : (d == 1 ? j : k);
+    const long s[3] = {1, nx, (long)nx * ny};
+    int cc[3] = {i, j, k};
+    double lhi = 0.0, llo = 0.0;
+    if (f < n[d]) { cc[d] = f; lhi = lam[cc[0] + s[1] * cc[1] + s[2] * cc[2]]; }
+    if (f > 0) { cc[d] = f - 1; llo = lam[cc[0] + s[1] * cc[1] + s[2] * cc[2]]; }
+    double dt = (llo - lhi) / A.dx[d];
+    ou[q] = gu[q] - dt;
+    // dL/dw_f = G(phi)_f ((D^T lambda)_f - gU_f); on interior and Dirichlet
+    // faces D^T lambda = -G(lambda), on the duplicated periodic faces it is
+    // not.  The folded Dirichlet weight is 2 beta (chain factor 2); Neumann
+    // faces do not depend on beta.
+    (void)gl;
+    int kind = (f == 0) ? A.bc.k[d][0] : ((f == n[d]) ? A.bc.k[d][1] : ELL_PER);
+    double fac = (kind == ELL_NEU) ? 0.0 : (kind == ELL_DIR ? 2.0 : 1.0);
+    ob[q] = fac * gp * (dt - gu[q]);
+}
+
+// --------------------------------------------------------- launchers
+static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+#define DISPATCH_DIM(dim, ...)            \
+    do {                                  \
+        if ((dim) == 3) {                 \
+            constexpr int D_ = 3;         \
+            __VA_ARGS__;                  \
+        } else {                          \
+            constexpr int D_ = 2;         \
+            __VA_ARGS__;                  \
+        }                                 \
+    } while (0)
+
+void launch_setup(const SetupArgs &A, int maxcells, hipStream_t s)
+{
+    dim3 g(nblk(maxcells), A.nboxes, 1);
+    DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_setup<D_>, g, dim3(TPB), 0, s, A));
+}
+
+long setup_nparts(int maxcells, int nboxes) { return (long)nblk(maxcells) * nboxes; }
+
+void launch_mean(const double *part, long nparts, double ncell, double *mean, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mean_finalize, dim3(1), dim3(1024), 0, s, part, nparts, ncell, mean);
+}
+
+template <typename T>
+void launch_sub_mean(T *x, long n, const double *mean, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_mean<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, mean);
+}
+
+template <typename T>
+long launch_partial_sum(const T *x, long n, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_partial_sum<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, part);
+    return nblk(n);
+}
+
+template <typename T>
+void launch_coarsen(int dim, const Lev<T> &F, const Lev<T> &C, hipStream_t s)
+{
+    long n = (long)C.nx * C.ny * C.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarsen<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, F, C));
+}
+
+void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, hipStream_t s)
+{
+    long n = (long)S.nx * S.ny * S.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_weights_to_f32<D_>, dim3(nblk(n)), dim3(TPB), 0, s, S, D));
+}
+
+template <typename T>
+void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
+{
+    long nq = (long)((L.nx + 1) >> 1) * L.ny * L.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
+}
+
+template <typename T>
+void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_residual<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, L, out, nrm));
+}
+
+void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
+                        hipStream_t s)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(nblk(n)), dim3(TPB), 0, s, L, r32, nrm));
+}
+
+void launch_phi_add_f32(double *phi, const float *e, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_phi_add_f32, dim3(nblk(n)), dim3(TPB), 0, s, phi, e, n);
+}
+
+template <typename T>
+void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hipStream_t s)
+{
+    long n = (long)C.nx * C.ny * C.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_restrict<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, r, fnx, fny, C));
+}
+
+template <typename T>
+void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s)
+{
+    long n = (long)F.nx * F.ny * F.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_prolong<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, F,
+                                         (const T *)C.phi, C.nx, C.ny, C.nz, bc));
+}
+
+template <typename T>
+void launch_fill(T *x, long n, T v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, v);
+}
+
+template <typename T>
+void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, int lc, int nlev, const CoarseOpts &o,
+                         double *cg, hipStream_t s)
+{
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T>), dim3(1), dim3(CTPB), 0, s, levs_dev,
+                                         lc, nlev, o, cg));
+}
+
+void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s)
+{
+    DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_correct_valid<D_>, dim3(nblk(maxfaces), A.nboxes, A.dim),
+                                           dim3(TPB), 0, s, A));
+    if (A.ng_u > 0) {
+        DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_fill_ghost<D_>, dim3(nblk(maxghost), A.nboxes, A.dim),
+                                               dim3(TPB), 0, s, A));
+        for (int d = 0; d < A.dim; ++d)
+            for (int side = 0; side < 2; ++side) {
+                int phys = A.phys[d][side];
+                if (phys == MAC_PERIODIC || phys == MAC_INTERIOR) continue;
+                DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_physbc<D_>, dim3(nblk(A.maxplane), A.nboxes, A.dim),
+                                                       dim3(TPB), 0, s, A, d, side, phys));
+            }
+    }
+}
+
+void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_level_setup<D_>, dim3(nblk(n)), dim3(TPB), 0, s, A, L, part));
+}
+
+void launch_level_correct(int dim, const LevelIO &A, const Lev<double> &L, hipStream_t s)
+{
+    for (int d = 0; d < dim; ++d) {
+        long n = (long)(L.nx + (d == 0)) * (L.ny + (d == 1)) * (L.nz + (d == 2));
+        DISPATCH_DIM(dim, hipLaunchKernelGGL(k_level_correct<D_>, dim3(nblk(n)), dim3(TPB), 0, s, A, L, d));
+    }
+}
+
+void launch_adj_rhs(int dim, const AdjIO &A, const Lev<double> &L, double *part, hipStream_t s)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_adj_rhs<D_>, dim3(nblk(n)), dim3(TPB), 0, s, A, L, part));
+}
+
+void launch_adj_faces(int dim, const AdjIO &A, const Lev<double> &L, const double *phi,
+                      const double *lam, hipStream_t s)
+{
+    for (int d = 0; d < dim; ++d) {
+        long n = (long)(L.nx + (d == 0)) * (L.ny + (d == 1)) * (L.nz + (d == 2));
+        DISPATCH_DIM(dim, hipLaunchKernelGGL(k_adj_faces<D_>, dim3(nblk(n)), dim3(TPB), 0, s, A, L, phi,
+                                             lam, d));
+    }
+}
+
+// explicit instantiations
+template void launch_sub_mean<double>(double *, long, const double *, hipStream_t);
+template long launch_partial_sum<double>(const double *, long, double *, hipStream_t);
+template void launch_coarsen<double>(int, const Lev<double> &, const Lev<double> &, hipStream_t);
+template void launch_coarsen<float>(int, const Lev<float> &, const Lev<float> &, hipStream_t);
+template void launch_gsrb<double>(int, const Lev<double> &, int, hipStream_t);
+template void launch_gsrb<float>(int, const Lev<float> &, int, hipStream_t);
+template void launch_residual<double>(int, const Lev<double> &, double *, unsigned long long *, hipStream_t);
+template void launch_residual<float>(int, const Lev<float> &, float *, unsigned long long *, hipStream_t);
+template void launch_restrict<double>(int, const double *, int, int, const Lev<double> &, hipStream_t);
+template void launch_restrict<float>(int, const float *, int, int, const Lev<float> &, hipStream_t);
+template void launch_prolong<double>(int, const Lev<double> &, const Lev<double> &, const EllBC &, hipStream_t);
+template void launch_prolong<float>(int, const Lev<float> &, const Lev<float> &, const EllBC &, hipStream_t);
+template void launch_fill<double>(double *, long, double, hipStream_t);
+template void launch_fill<float>(float *, long, float, hipStream_t);
+template void launch_coarse_cycle<double>(int, const Lev<double> *, int, int, const CoarseOpts &, double *, hipStream_t);
+template void launch_coarse_cycle<float>(int, const Lev<float> *, int, int, const CoarseOpts &, double *, hipStream_t);
+
+}  // namespace mac

@@ -1,0 +1,101 @@
+// mac_launch.h -- kernel argument blocks and host launchers of
+// libmaestro_mac.so (internal; the public ABI is include/maestro_mac.h).
+#pragma once
+#include "mac_internal.h"
+#include "../../include/maestro_mac.h"
+
+namespace mac {
+
+// fused setup from the caller's boxes (see k_setup)
+struct SetupArgs {
+    int dim, nboxes;
+    const BoxDev *boxes;
+    int dlo[3], n[3];
+    double dx[3];
+    EllBC bc;
+    int ng_u, ng_phi, ng_rho, ng_rhs, ng_cart;
+    double *const *umac;
+    double *const *phi;
+    const double *const *rho;
+    const double *const *macrhs;
+    const double *beta0, *beta0e;
+    const double *const *cart;
+    int alt;
+    Lev<double> L;
+    double *part;
+    int *err;
+};
+
+struct CorrectArgs {
+    int dim, nboxes;
+    const BoxDev *boxes;
+    int dlo[3], n[3];
+    double dx[3];
+    EllBC bc;
+    int phys[3][2];
+    int ng_u, ng_cart;
+    double *const *umac;
+    const double *beta0, *beta0e;
+    const double *const *cart;
+    const double *phi;
+    const double *bx, *by, *bz, *bxh, *byh, *bzh;
+    const int *boxmap;
+    int mapg[3], mapn[3];
+    long maxplane;
+};
+
+struct CoarseOpts {
+    int nu1, nu2, nub;
+    double bottom_eps;
+    int bottom_maxit;
+    int singular;
+    EllBC bc;
+};
+
+// whole-domain (natural layout) forward arguments
+struct LevelIO {
+    double *ux, *uy, *uz;
+    const double *bx, *by, *bz;
+    const double *macrhs;
+    const double *phi;
+    double dx[3];
+    EllBC bc;
+};
+
+struct AdjIO {
+    const double *gphi, *gux, *guy, *guz;
+    double *out_rhs;
+    double *out_ux, *out_uy, *out_uz, *out_bx, *out_by, *out_bz;
+    double dx[3];
+    EllBC bc;
+};
+
+void launch_setup(const SetupArgs &A, int maxcells, hipStream_t s);
+long setup_nparts(int maxcells, int nboxes);
+void launch_mean(const double *part, long nparts, double ncell, double *mean, hipStream_t s);
+template <typename T> void launch_sub_mean(T *x, long n, const double *mean, hipStream_t s);
+template <typename T> long launch_partial_sum(const T *x, long n, double *part, hipStream_t s);
+template <typename T> void launch_coarsen(int dim, const Lev<T> &F, const Lev<T> &C, hipStream_t s);
+void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, hipStream_t s);
+template <typename T> void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s);
+template <typename T>
+void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
+void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
+                        hipStream_t s);
+void launch_phi_add_f32(double *phi, const float *e, long n, hipStream_t s);
+template <typename T>
+void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hipStream_t s);
+template <typename T>
+void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s);
+template <typename T> void launch_fill(T *x, long n, T v, hipStream_t s);
+template <typename T>
+void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, int lc, int nlev, const CoarseOpts &o,
+                         double *cg, hipStream_t s);
+void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s);
+void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s);
+void launch_level_correct(int dim, const LevelIO &A, const Lev<double> &L, hipStream_t s);
+void launch_adj_rhs(int dim, const AdjIO &A, const Lev<double> &L, double *part, hipStream_t s);
+void launch_adj_faces(int dim, const AdjIO &A, const Lev<double> &L, const double *phi,
+                      const double *lam, hipStream_t s);
+
+}  // namespace mac

@@ -1,0 +1,1066 @@
+// mac_plan.hip -- C ABI of libmaestro_mac.so: plans, macproject, solver
+// driver (V-cycle, FP32-in-FP64 refinement) and the adjoint.
+//
+// Reference interfaces replaced:
+//   macproject      Source/macproject.f90:51-328
+//   mac_multigrid   Source/mac_multigrid.f90:18-71 (-> FBoxLib ml_cc_solve)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "mac_launch.h"
+
+using namespace mac;
+
+#define HIPCHK(x)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) {                                                       \
+            if (getenv("MAC_DEBUG"))                                                  \
+                fprintf(stderr, "maestro_mac: %s failed: %s (%s:%d)\n", #x,           \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                   \
+            return MAC_ERR_HIP;                                                       \
+        }                                                                             \
+    } while (0)
+
+namespace {
+
+template <typename T>
+struct Hier {
+    int nlev = 0, lc = 0;
+    std::vector<Lev<T>> lev;
+    Lev<T> *lev_dev = nullptr;
+    std::vector<void *> allocs;
+    bool built = false;
+};
+
+struct HostStage {
+    // device mirrors of the caller's per-box host arrays
+    std::vector<double *> umac, phi, rho, rhs, cart;
+    std::vector<size_t> umac_n, cell_n_phi, cell_n_rho, cell_n_rhs, cart_n;
+    bool rhs_ready = false, cart_ready = false, base_ready = false;
+};
+
+}  // namespace
+
+struct mac_plan {
+    int dim = 3, nboxes = 0;
+    int n[3] = {1, 1, 1}, dlo[3] = {0, 0, 0};
+    double dx[3] = {1, 1, 1};
+    EllBC bc;
+    int phys[3][2];
+    bool singular = true;
+    int ng_u = 0, ng_phi = 0, ng_rho = 0, ng_rhs = 0, ng_cart = 0;
+    mac_opts opts;
+    std::vector<BoxDev> boxes;
+    BoxDev *boxes_dev = nullptr;
+    int maxcells = 0, maxfaces = 0, maxghost = 0;
+    long maxplane = 0;
+    int *boxmap_dev = nullptr;
+    int mapg[3] = {1, 1, 1}, mapn[3] = {1, 1, 1};
+    Hier<double> H64;
+    Hier<float> H32;
+    double *part = nullptr;
+    long part_cap = 0;
+    double *mean_dev = nullptr;
+    unsigned long long *nrm_dev = nullptr;
+    unsigned long long *nrm_host = nullptr;
+    int *err_dev = nullptr;
+    double *cg = nullptr;
+    long cg_cap = 0;
+    double **tab_dev = nullptr;   // pointer tables
+    double *b0_dev = nullptr, *b0e_dev = nullptr;
+    int nr = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8];
+    HostStage hs;
+    double *scratch_cells = nullptr;   // one FP64 cell array (adjoint)
+    int device = 0;
+    // kernel timing (mac_plan_profile)
+    bool prof_on = false;
+    std::vector<hipEvent_t> prof_ev;   // pairs
+    std::vector<int> prof_cls;
+    int prof_used = 0;
+    long prof_cnt[MAC_PROF_NCLASS] = {0};
+    double prof_ms[MAC_PROF_NCLASS] = {0};
+};
+
+static constexpr int PROF_PAIRS = 512;
+
+static void prof_flush(mac_plan *P)
+{
+    if (!P->prof_used) return;
+    (void)hipStreamSynchronize(P->stream);
+    for (int q = 0; q < P->prof_used; ++q) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, P->prof_ev[2 * q], P->prof_ev[2 * q + 1]) == hipSuccess) {
+            P->prof_ms[P->prof_cls[q]] += ms;
+            P->prof_cnt[P->prof_cls[q]] += 1;
+        }
+    }
+    P->prof_used = 0;
+}
+
+// returns the slot to close with prof_end, or -1 when profiling is off
+static int prof_begin(mac_plan *P, int cls)
+{
+    if (!P->prof_on) return -1;
+    if (P->prof_used == PROF_PAIRS) prof_flush(P);
+    int q = P->prof_used++;
+    P->prof_cls[q] = cls;
+    (void)hipEventRecord(P->prof_ev[2 * q], P->stream);
+    return q;
+}
+
+static void prof_end(mac_plan *P, int q)
+{
+    if (q >= 0) (void)hipEventRecord(P->prof_ev[2 * q + 1], P->stream);
+}
+
+static long ncell(const int *n) { return (long)n[0] * n[1] * n[2]; }
+
+static int ell_kind(int phys)
+{
+    if (phys == MAC_PERIODIC) return ELL_PER;
+    if (phys == MAC_OUTLET) return ELL_DIR;
+    return ELL_NEU;
+}
+
+extern "C" int mac_default_opts(mac_opts *o)
+{
+    if (!o) return MAC_ERR_INVALID;
+    memset(o, 0, sizeof(*o));
+    o->eps_rel = 1e-10;
+    o->eps_abs = -1.0;
+    o->max_iter = 200;
+    o->nu1 = 2;
+    o->nu2 = 2;
+    o->nub = 10;
+    o->bottom_eps = 1e-3;
+    o->bottom_max_iter = 1000;
+    o->max_levels = 0;
+    o->use_alt_energy_fix = 1;
+    o->precision = MAC_PREC_FP64;
+    o->inner_cycles = 1;
+    o->device_ptrs = 0;
+    o->verbose = 0;
+    return MAC_OK;
+}
+
+extern "C" const char *mac_version(void) { return "maestro_mac 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------ hierarchy
+template <typename T>
+static int alloc_dev(Hier<T> &H, T **p, long n)
+{
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, sizeof(T) * (size_t)std::max(n, 1L)));
+    H.allocs.push_back(q);
+    *p = (T *)q;
+    return MAC_OK;
+}
+
+static int coarse_threshold(int dim)
+{
+    const char *e = getenv("MAC_COARSE_CELLS");
+    if (e) return atoi(e);
+    return dim == 3 ? 512 : 1024;
+}
+
+template <typename T>
+static int build_hier(mac_plan *P, Hier<T> &H)
+{
+    if (H.built) return MAC_OK;
+    int n[3] = {P->n[0], P->n[1], P->n[2]};
+    double h[3] = {P->dx[0], P->dx[1], P->dx[2]};
+    int nl = 1;
+    {
+        int m[3] = {n[0], n[1], n[2]};
+        for (;;) {
+            if (P->opts.max_levels > 0 && nl >= P->opts.max_levels) break;
+            bool ok = true;
+            for (int d = 0; d < P->dim; ++d)
+                if ((m[d] & 1) || m[d] / 2 < 2) ok = false;
+            if (!ok) break;
+            for (int d = 0; d < P->dim; ++d) m[d] /= 2;
+            ++nl;
+        }
+    }
+    H.nlev = nl;
+    H.lev.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        Lev<T> &L = H.lev[l];
+        L.nx = n[0]; L.ny = n[1]; L.nz = n[2];
+        L.perx = P->bc.k[0][0] == ELL_PER;
+        L.pery = P->bc.k[1][0] == ELL_PER;
+        L.perz = (P->dim == 3) && P->bc.k[2][0] == ELL_PER;
+        L.ih2x = (T)(1.0 / (h[0] * h[0]));
+        L.ih2y = (T)(1.0 / (h[1] * h[1]));
+        L.ih2z = (P->dim == 3) ? (T)(1.0 / (h[2] * h[2])) : T(0);
+        long nc = ncell(n);
+        int rc;
+        if ((rc = alloc_dev(H, &L.phi, nc))) return rc;
+        L.phi2 = nullptr;
+        if ((rc = alloc_dev(H, &L.rhs, nc))) return rc;
+        if ((rc = alloc_dev(H, &L.res, nc))) return rc;
+        if ((rc = alloc_dev(H, &L.bx, nc))) return rc;
+        if ((rc = alloc_dev(H, &L.by, nc))) return rc;
+        if ((rc = alloc_dev(H, &L.bxh, (long)n[1] * n[2]))) return rc;
+        if ((rc = alloc_dev(H, &L.byh, (long)n[0] * n[2]))) return rc;
+        if (P->dim == 3) {
+            if ((rc = alloc_dev(H, &L.bz, nc))) return rc;
+            if ((rc = alloc_dev(H, &L.bzh, (long)n[0] * n[1]))) return rc;
+        } else {
+            L.bz = L.bzh = nullptr;
+        }
+        for (int d = 0; d < P->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
+    }
+    int thr = coarse_threshold(P->dim);
+    H.lc = nl - 1;
+    for (int l = 0; l < nl; ++l)
+        if ((long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz <= thr) { H.lc = l; break; }
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, sizeof(Lev<T>) * nl));
+    H.allocs.push_back(q);
+    H.lev_dev = (Lev<T> *)q;
+    HIPCHK(hipMemcpy(H.lev_dev, H.lev.data(), sizeof(Lev<T>) * nl, hipMemcpyHostToDevice));
+    // bottom CG scratch
+    const Lev<T> &B = H.lev[nl - 1];
+    long nb = (long)B.nx * B.ny * B.nz;
+    if (4 * nb > P->cg_cap) {
+        if (P->cg) (void)hipFree(P->cg);
+        HIPCHK(hipMalloc(&P->cg, sizeof(double) * 4 * nb));
+        P->cg_cap = 4 * nb;
+    }
+    H.built = true;
+    return MAC_OK;
+}
+
+template <typename T>
+static void free_hier(Hier<T> &H)
+{
+    for (void *p : H.allocs) (void)hipFree(p);
+    H.allocs.clear();
+    H.lev.clear();
+    H.built = false;
+}
+
+template <typename T>
+static void coarsen_all(mac_plan *P, Hier<T> &H)
+{
+    for (int l = 1; l < H.nlev; ++l) launch_coarsen<T>(P->dim, H.lev[l - 1], H.lev[l], P->stream);
+}
+
+template <typename T>
+static void smooth(mac_plan *P, const Lev<T> &L, int level, int sweeps)
+{
+    const int cls = sizeof(T) == 8 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32;
+    for (int it = 0; it < sweeps; ++it)
+        for (int color = 0; color < 2; ++color) {
+            int q = (level == 0) ? prof_begin(P, cls) : -1;
+            launch_gsrb<T>(P->dim, L, color, P->stream);
+            prof_end(P, q);
+        }
+}
+
+template <typename T>
+static void vcycle(mac_plan *P, Hier<T> &H)
+{
+    hipStream_t s = P->stream;
+    const int dim = P->dim;
+    const mac_opts &o = P->opts;
+    for (int l = 0; l < H.lc; ++l) {
+        const Lev<T> &L = H.lev[l];
+        if (l > 0) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
+        smooth<T>(P, L, l, o.nu1);
+        launch_residual<T>(dim, L, L.res, nullptr, s);
+        launch_restrict<T>(dim, L.res, L.nx, L.ny, H.lev[l + 1], s);
+    }
+    {
+        const Lev<T> &L = H.lev[H.lc];
+        if (H.lc > 0) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
+        CoarseOpts co;
+        co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
+        co.bottom_eps = o.bottom_eps;
+        co.bottom_maxit = o.bottom_max_iter;
+        co.singular = P->singular;
+        co.bc = P->bc;
+        int q = prof_begin(P, MAC_PROF_COARSE);
+        launch_coarse_cycle<T>(dim, H.lev_dev, H.lc, H.nlev, co, P->cg, s);
+        prof_end(P, q);
+    }
+    for (int l = H.lc - 1; l >= 0; --l) {
+        const Lev<T> &L = H.lev[l];
+        launch_prolong<T>(dim, L, H.lev[l + 1], P->bc, s);
+        smooth<T>(P, L, l, o.nu2);
+    }
+}
+
+static int read_norm(mac_plan *P, double *out)
+{
+    HIPCHK(hipMemcpyAsync(P->nrm_host, P->nrm_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    unsigned long long u = *P->nrm_host;
+    double v;
+    memcpy(&v, &u, sizeof v);
+    *out = v;
+    return MAC_OK;
+}
+
+static int fp64_norm(mac_plan *P, double *out)
+{
+    const Lev<double> &L = P->H64.lev[0];
+    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
+    int q = prof_begin(P, MAC_PROF_RESID64);
+    launch_residual<double>(P->dim, L, (double *)nullptr, P->nrm_dev, P->stream);
+    prof_end(P, q);
+    return read_norm(P, out);
+}
+
+static int ir_norm(mac_plan *P, double *out)
+{
+    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
+    int q = prof_begin(P, MAC_PROF_RESID64);
+    launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, P->nrm_dev, P->stream);
+    prof_end(P, q);
+    return read_norm(P, out);
+}
+
+static bool converged(const mac_opts &o, double r, double r0)
+{
+    if (r0 == 0.0) return true;
+    if (r <= o.eps_rel * r0) return true;
+    if (o.eps_abs > 0.0 && r <= o.eps_abs) return true;
+    return false;
+}
+
+// Solve A phi = rhs on level 0 of H64 (phi holds the warm start).
+static int solve(mac_plan *P, mac_stats *st)
+{
+    int rc;
+    const mac_opts &o = P->opts;
+    double r0 = 0, r = 0;
+    int it = 0, inner = 0;
+    if (o.precision == MAC_PREC_MIXED) {
+        if ((rc = build_hier<float>(P, P->H32))) return rc;
+        launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
+        coarsen_all<float>(P, P->H32);
+        if ((rc = ir_norm(P, &r0))) return rc;
+        r = r0;
+        const Lev<float> &E = P->H32.lev[0];
+        long n0 = (long)E.nx * E.ny * E.nz;
+        while (!converged(o, r, r0) && it < o.max_iter) {
+            launch_fill<float>(E.phi, n0, 0.0f, P->stream);
+            for (int c = 0; c < std::max(1, o.inner_cycles); ++c) {
+                vcycle<float>(P, P->H32);
+                ++inner;
+            }
+            launch_phi_add_f32(P->H64.lev[0].phi, E.phi, n0, P->stream);
+            if ((rc = ir_norm(P, &r))) return rc;
+            ++it;
+            if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
+        }
+    } else {
+        if ((rc = fp64_norm(P, &r0))) return rc;
+        r = r0;
+        while (!converged(o, r, r0) && it < o.max_iter) {
+            vcycle<double>(P, P->H64);
+            if ((rc = fp64_norm(P, &r))) return rc;
+            ++it;
+            if (o.verbose) fprintf(stderr, "maestro_mac: V-cycle %d  res/res0 = %.3e\n", it, r / r0);
+        }
+    }
+    if (st) {
+        st->iters = it;
+        st->inner_cycles = inner;
+        st->res0 = r0;
+        st->res = r;
+        st->converged = converged(o, r, r0);
+        st->nlevels = P->H64.nlev;
+    }
+    return converged(o, r, r0) ? MAC_OK : MAC_ERR_NOT_CONVERGED;
+}
+
+static int ensure_part(mac_plan *P, long nparts)
+{
+    if (nparts > P->part_cap) {
+        if (P->part) (void)hipFree(P->part);
+        HIPCHK(hipMalloc(&P->part, sizeof(double) * 2 * nparts));
+        P->part_cap = nparts;
+    }
+    return MAC_OK;
+}
+
+// subtract the compensated mean of x (n cells) on the plan stream
+static int remove_mean(mac_plan *P, double *x, long n)
+{
+    long np = (n + 255) / 256;
+    int rc;
+    if ((rc = ensure_part(P, np))) return rc;
+    launch_partial_sum<double>(x, n, P->part, P->stream);
+    launch_mean(P->part, np, (double)n, P->mean_dev, P->stream);
+    launch_sub_mean<double>(x, n, P->mean_dev, P->stream);
+    return MAC_OK;
+}
+
+// ---------------------------------------------------------------- plan
+extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_plan **out)
+{
+    if (!lay || !out) return MAC_ERR_INVALID;
+    *out = nullptr;
+    if (lay->dim != 2 && lay->dim != 3) return MAC_ERR_INVALID;
+    if (lay->nboxes <= 0 || !lay->box_lo || !lay->box_hi) return MAC_ERR_LAYOUT;
+    if (lay->ng_rho < 1 || lay->ng_umac < 0 || lay->ng_phi < 0 || lay->ng_macrhs < 0 ||
+        lay->ng_beta0_cart < 0)
+        return MAC_ERR_LAYOUT;
+    mac_plan *P = new mac_plan();
+    P->dim = lay->dim;
+    P->nboxes = lay->nboxes;
+    if (opts) P->opts = *opts;
+    else mac_default_opts(&P->opts);
+    for (int d = 0; d < 3; ++d) {
+        if (d < lay->dim) {
+            P->dlo[d] = lay->domain_lo[d];
+            P->n[d] = lay->domain_hi[d] - lay->domain_lo[d] + 1;
+            P->dx[d] = lay->dx[d];
+            for (int s = 0; s < 2; ++s) {
+                P->phys[d][s] = lay->phys_bc[d][s];
+                P->bc.k[d][s] = ell_kind(lay->phys_bc[d][s]);
+            }
+            if ((P->bc.k[d][0] == ELL_PER) != (P->bc.k[d][1] == ELL_PER)) { delete P; return MAC_ERR_INVALID; }
+            if (P->n[d] < 2 || !(P->dx[d] > 0)) { delete P; return MAC_ERR_LAYOUT; }
+        } else {
+            P->dlo[d] = 0; P->n[d] = 1; P->dx[d] = 1.0;
+            P->phys[d][0] = P->phys[d][1] = MAC_PERIODIC;
+            P->bc.k[d][0] = P->bc.k[d][1] = ELL_PER;
+        }
+    }
+    P->singular = true;
+    for (int d = 0; d < P->dim; ++d)
+        for (int s = 0; s < 2; ++s)
+            if (P->bc.k[d][s] == ELL_DIR) P->singular = false;
+    P->ng_u = lay->ng_umac; P->ng_phi = lay->ng_phi; P->ng_rho = lay->ng_rho;
+    P->ng_rhs = lay->ng_macrhs; P->ng_cart = lay->ng_beta0_cart;
+    // boxes: inside the domain, disjoint, covering it exactly
+    long covered = 0;
+    P->boxes.resize(P->nboxes);
+    int g[3] = {0, 0, 0};
+    for (int b = 0; b < P->nboxes; ++b) {
+        BoxDev B;
+        for (int d = 0; d < 3; ++d) {
+            B.lo[d] = (d < P->dim) ? lay->box_lo[3 * b + d] : 0;
+            B.hi[d] = (d < P->dim) ? lay->box_hi[3 * b + d] : 0;
+        }
+        long v = 1;
+        for (int d = 0; d < P->dim; ++d) {
+            if (B.lo[d] < P->dlo[d] || B.hi[d] > P->dlo[d] + P->n[d] - 1 || B.hi[d] < B.lo[d]) {
+                delete P;
+                return MAC_ERR_LAYOUT;
+            }
+            v *= (B.hi[d] - B.lo[d] + 1);
+            g[d] = std::gcd(g[d], std::gcd(B.lo[d] - P->dlo[d], B.hi[d] - B.lo[d] + 1));
+        }
+        covered += v;
+        P->boxes[b] = B;
+    }
+    if (covered != ncell(P->n)) { delete P; return MAC_ERR_LAYOUT; }
+    // owner map at the gcd resolution; also detects overlaps
+    for (int d = 0; d < 3; ++d) {
+        if (d >= P->dim || g[d] == 0) g[d] = 1;
+        P->mapg[d] = (d < P->dim) ? g[d] : 1;
+        P->mapn[d] = (d < P->dim) ? P->n[d] / P->mapg[d] : 1;
+    }
+    std::vector<int> map((size_t)P->mapn[0] * P->mapn[1] * P->mapn[2], -1);
+    for (int b = 0; b < P->nboxes; ++b) {
+        const BoxDev &B = P->boxes[b];
+        int l0 = (B.lo[0] - P->dlo[0]) / P->mapg[0], h0 = (B.hi[0] - P->dlo[0]) / P->mapg[0];
+        int l1 = (B.lo[1] - P->dlo[1]) / P->mapg[1], h1 = (B.hi[1] - P->dlo[1]) / P->mapg[1];
+        int l2 = (P->dim == 3) ? (B.lo[2] - P->dlo[2]) / P->mapg[2] : 0;
+        int h2 = (P->dim == 3) ? (B.hi[2] - P->dlo[2]) / P->mapg[2] : 0;
+        for (int k = l2; k <= h2; ++k)
+            for (int j = l1; j <= h1; ++j)
+                for (int i = l0; i <= h0; ++i) {
+                    int &m = map[i + (size_t)P->mapn[0] * (j + (size_t)P->mapn[1] * k)];
+                    if (m >= 0) { delete P; return MAC_ERR_LAYOUT; }
+                    m = b;
+                }
+    }
+    // extents for launches
+    for (const BoxDev &B : P->boxes) {
+        int e[3];
+        for (int d = 0; d < 3; ++d) e[d] = (d < P->dim) ? B.hi[d] - B.lo[d] + 1 : 1;
+        P->maxcells = std::max(P->maxcells, e[0] * e[1] * e[2]);
+        for (int d = 0; d < P->dim; ++d) {
+            int f[3] = {e[0], e[1], e[2]};
+            f[d] += 1;
+            P->maxfaces = std::max(P->maxfaces, f[0] * f[1] * f[2]);
+            int gg[3];
+            for (int q = 0; q < 3; ++q) gg[q] = (q < P->dim) ? f[q] + 2 * P->ng_u : 1;
+            P->maxghost = std::max(P->maxghost, gg[0] * gg[1] * gg[2]);
+            for (int dir = 0; dir < P->dim; ++dir) {
+                long plane = 1;
+                for (int q = 0; q < P->dim; ++q)
+                    if (q != dir) plane *= gg[q];
+                P->maxplane = std::max(P->maxplane, plane);
+            }
+        }
+    }
+    P->nr = P->n[P->dim - 1];
+    int rc = MAC_OK;
+    auto fail = [&](int r) { mac_plan_destroy(P); return r; };
+    if (hipGetDevice(&P->device) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) return fail(MAC_ERR_HIP);
+    for (int e = 0; e < 8; ++e)
+        if (hipEventCreate(&P->ev[e]) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->boxes_dev, sizeof(BoxDev) * P->nboxes) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMemcpy(P->boxes_dev, P->boxes.data(), sizeof(BoxDev) * P->nboxes, hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->boxmap_dev, sizeof(int) * map.size()) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMemcpy(P->boxmap_dev, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->mean_dev, sizeof(double)) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->nrm_dev, sizeof(unsigned long long) * 4) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipHostMalloc(&P->nrm_host, sizeof(unsigned long long) * 4, 0) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->err_dev, sizeof(int)) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->tab_dev, sizeof(double *) * (size_t)P->nboxes * (2 * P->dim + 3)) != hipSuccess)
+        return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->b0_dev, sizeof(double) * (P->nr + 1)) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMalloc(&P->b0e_dev, sizeof(double) * (P->nr + 1)) != hipSuccess) return fail(MAC_ERR_HIP);
+    if ((rc = build_hier<double>(P, P->H64))) return fail(rc);
+    *out = P;
+    return MAC_OK;
+}
+
+static void free_stage(mac_plan *P)
+{
+    auto fr = [](std::vector<double *> &v) {
+        for (double *p : v)
+            if (p) (void)hipFree(p);
+        v.clear();
+    };
+    fr(P->hs.umac); fr(P->hs.phi); fr(P->hs.rho); fr(P->hs.rhs); fr(P->hs.cart);
+    P->hs.base_ready = P->hs.rhs_ready = P->hs.cart_ready = false;
+}
+
+extern "C" int mac_plan_profile(mac_plan *P, int enable)
+{
+    if (!P) return MAC_ERR_INVALID;
+    prof_flush(P);
+    if (enable && P->prof_ev.empty()) {
+        P->prof_ev.resize(2 * PROF_PAIRS);
+        P->prof_cls.resize(PROF_PAIRS);
+        for (auto &e : P->prof_ev) HIPCHK(hipEventCreate(&e));
+    }
+    for (int c = 0; c < MAC_PROF_NCLASS; ++c) { P->prof_cnt[c] = 0; P->prof_ms[c] = 0.0; }
+    P->prof_on = enable != 0;
+    return MAC_OK;
+}
+
+extern "C" int mac_plan_profile_read(mac_plan *P, int cls, long *count, double *total_ms)
+{
+    if (!P || cls < 0 || cls >= MAC_PROF_NCLASS) return MAC_ERR_INVALID;
+    prof_flush(P);
+    if (count) *count = P->prof_cnt[cls];
+    if (total_ms) *total_ms = P->prof_ms[cls];
+    return MAC_OK;
+}
+
+extern "C" int mac_plan_destroy(mac_plan *P)
+{
+    if (!P) return MAC_OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (auto &e : P->prof_ev)
+        if (e) (void)hipEventDestroy(e);
+    free_hier(P->H64);
+    free_hier(P->H32);
+    free_stage(P);
+    if (P->boxes_dev) (void)hipFree(P->boxes_dev);
+    if (P->boxmap_dev) (void)hipFree(P->boxmap_dev);
+    if (P->part) (void)hipFree(P->part);
+    if (P->mean_dev) (void)hipFree(P->mean_dev);
+    if (P->nrm_dev) (void)hipFree(P->nrm_dev);
+    if (P->nrm_host) (void)hipHostFree(P->nrm_host);
+    if (P->err_dev) (void)hipFree(P->err_dev);
+    if (P->cg) (void)hipFree(P->cg);
+    if (P->tab_dev) (void)hipFree(P->tab_dev);
+    if (P->b0_dev) (void)hipFree(P->b0_dev);
+    if (P->b0e_dev) (void)hipFree(P->b0e_dev);
+    if (P->scratch_cells) (void)hipFree(P->scratch_cells);
+    for (int e = 0; e < 8; ++e)
+        if (P->ev[e]) (void)hipEventDestroy(P->ev[e]);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return MAC_OK;
+}
+
+extern "C" int mac_plan_set_opts(mac_plan *P, const mac_opts *o)
+{
+    if (!P || !o) return MAC_ERR_INVALID;
+    int old_levels = P->opts.max_levels;
+    P->opts = *o;
+    if (o->max_levels != old_levels) {
+        free_hier(P->H64);
+        free_hier(P->H32);
+        int rc = build_hier<double>(P, P->H64);
+        if (rc) return rc;
+    }
+    return MAC_OK;
+}
+
+// --------------------------------------------------------- per-box sizes
+static size_t box_elems(const mac_plan *P, int b, int ng, int fd)
+{
+    const BoxDev &B = P->boxes[b];
+    size_t v = 1;
+    for (int d = 0; d < P->dim; ++d) v *= (size_t)(B.hi[d] - B.lo[d] + 1 + 2 * ng + (d == fd));
+    return v;
+}
+
+static int stage_in(mac_plan *P, double *const *umac, double *const *phi, const double *const *rho,
+                    const double *const *macrhs, const double *const *cart)
+{
+    HostStage &S = P->hs;
+    const int nb = P->nboxes, dim = P->dim;
+    if (!S.base_ready) {
+        S.umac.assign((size_t)nb * dim, nullptr);
+        S.phi.assign(nb, nullptr);
+        S.rho.assign(nb, nullptr);
+        for (int b = 0; b < nb; ++b) {
+            for (int d = 0; d < dim; ++d)
+                HIPCHK(hipMalloc(&S.umac[b * dim + d], sizeof(double) * box_elems(P, b, P->ng_u, d)));
+            HIPCHK(hipMalloc(&S.phi[b], sizeof(double) * box_elems(P, b, P->ng_phi, -1)));
+            HIPCHK(hipMalloc(&S.rho[b], sizeof(double) * box_elems(P, b, P->ng_rho, -1)));
+        }
+        S.base_ready = true;
+    }
+    if (macrhs && !S.rhs_ready) {
+        S.rhs.assign(nb, nullptr);
+        for (int b = 0; b < nb; ++b)
+            HIPCHK(hipMalloc(&S.rhs[b], sizeof(double) * box_elems(P, b, P->ng_rhs, -1)));
+        S.rhs_ready = true;
+    }
+    if (cart && !S.cart_ready) {
+        S.cart.assign((size_t)nb * dim, nullptr);
+        for (int b = 0; b < nb; ++b)
+            for (int d = 0; d < dim; ++d)
+                HIPCHK(hipMalloc(&S.cart[b * dim + d], sizeof(double) * box_elems(P, b, P->ng_cart, d)));
+        S.cart_ready = true;
+    }
+    for (int b = 0; b < nb; ++b) {
+        for (int d = 0; d < dim; ++d)
+            HIPCHK(hipMemcpyAsync(S.umac[b * dim + d], umac[b * dim + d],
+                                  sizeof(double) * box_elems(P, b, P->ng_u, d), hipMemcpyHostToDevice,
+                                  P->stream));
+        HIPCHK(hipMemcpyAsync(S.phi[b], phi[b], sizeof(double) * box_elems(P, b, P->ng_phi, -1),
+                              hipMemcpyHostToDevice, P->stream));
+        HIPCHK(hipMemcpyAsync(S.rho[b], rho[b], sizeof(double) * box_elems(P, b, P->ng_rho, -1),
+                              hipMemcpyHostToDevice, P->stream));
+        if (macrhs)
+            HIPCHK(hipMemcpyAsync(S.rhs[b], macrhs[b], sizeof(double) * box_elems(P, b, P->ng_rhs, -1),
+                                  hipMemcpyHostToDevice, P->stream));
+        if (cart)
+            for (int d = 0; d < dim; ++d)
+                HIPCHK(hipMemcpyAsync(S.cart[b * dim + d], cart[b * dim + d],
+                                      sizeof(double) * box_elems(P, b, P->ng_cart, d),
+                                      hipMemcpyHostToDevice, P->stream));
+    }
+    return MAC_OK;
+}
+
+static int stage_out(mac_plan *P, double *const *umac, double *const *phi)
+{
+    HostStage &S = P->hs;
+    for (int b = 0; b < P->nboxes; ++b) {
+        for (int d = 0; d < P->dim; ++d)
+            HIPCHK(hipMemcpyAsync(umac[b * P->dim + d], S.umac[b * P->dim + d],
+                                  sizeof(double) * box_elems(P, b, P->ng_u, d), hipMemcpyDeviceToHost,
+                                  P->stream));
+        HIPCHK(hipMemcpyAsync(phi[b], S.phi[b], sizeof(double) * box_elems(P, b, P->ng_phi, -1),
+                              hipMemcpyDeviceToHost, P->stream));
+    }
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}
+
+// copy the level-0 solution back into the caller's per-box phi
+__global__ void k_phi_out(const BoxDev *boxes, double *const *phi, int ng, int dim, const double *src,
+                          int dlo0, int dlo1, int dlo2, int nx, int ny)
+{
+    const int b = blockIdx.y;
+    const BoxDev B = boxes[b];
+    const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
+    const int ez = (dim == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    int li = (int)(q % ex), lj = (int)((q / ex) % ey), lk = (int)(q / ((long)ex * ey));
+    int i = B.lo[0] + li, j = B.lo[1] + lj, k = (dim == 3) ? B.lo[2] + lk : 0;
+    long c = (i - dlo0) + (long)nx * ((j - dlo1) + (long)ny * ((dim == 3) ? k - dlo2 : 0));
+    long ax = ex + 2 * ng, ay = ey + 2 * ng;
+    long o = (li + ng) + ax * ((lj + ng) + ay * ((dim == 3) ? lk + ng : 0));
+    phi[b][o] = src[c];
+}
+
+static float ev_ms(mac_plan *P, int a, int b)
+{
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, P->ev[a], P->ev[b]) != hipSuccess) return 0.f;
+    return ms;
+}
+
+// --------------------------------------------------------- mac_project
+extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
+                           const double *const *rho, const double *const *macrhs,
+                           const double *beta0_1d, const double *beta0_1d_edge,
+                           const double *const *beta0_cart_edge, mac_stats *st)
+{
+    if (!P || !umac || !phi || !rho) return MAC_ERR_INVALID;
+    if (beta0_1d && beta0_cart_edge) return MAC_ERR_INVALID;      // macproject.f90:99-101
+    if (beta0_1d && !beta0_1d_edge) return MAC_ERR_INVALID;
+    if (!beta0_1d && beta0_1d_edge) return MAC_ERR_INVALID;
+    if (P->opts.precision != MAC_PREC_FP64 && P->opts.precision != MAC_PREC_MIXED) return MAC_ERR_INVALID;
+    if (st) memset(st, 0, sizeof(*st));
+    hipStream_t s = P->stream;
+    const int nb = P->nboxes, dim = P->dim;
+    int rc;
+    HIPCHK(hipSetDevice(P->device));
+    HIPCHK(hipEventRecord(P->ev[0], s));
+    // resolve device pointers
+    std::vector<double *> tab((size_t)nb * (2 * dim + 3), nullptr);
+    double **t_umac = tab.data(), **t_phi = t_umac + nb * dim, **t_rho = t_phi + nb,
+           **t_rhs = t_rho + nb, **t_cart = t_rhs + nb;
+    if (P->opts.device_ptrs) {
+        for (int q = 0; q < nb * dim; ++q) t_umac[q] = umac[q];
+        for (int b = 0; b < nb; ++b) {
+            t_phi[b] = phi[b];
+            t_rho[b] = (double *)rho[b];
+            if (macrhs) t_rhs[b] = (double *)macrhs[b];
+        }
+        if (beta0_cart_edge)
+            for (int q = 0; q < nb * dim; ++q) t_cart[q] = (double *)beta0_cart_edge[q];
+    } else {
+        if ((rc = stage_in(P, umac, phi, rho, macrhs, beta0_cart_edge))) return rc;
+        for (int q = 0; q < nb * dim; ++q) t_umac[q] = P->hs.umac[q];
+        for (int b = 0; b < nb; ++b) {
+            t_phi[b] = P->hs.phi[b];
+            t_rho[b] = P->hs.rho[b];
+            if (macrhs) t_rhs[b] = P->hs.rhs[b];
+        }
+        if (beta0_cart_edge)
+            for (int q = 0; q < nb * dim; ++q) t_cart[q] = P->hs.cart[q];
+    }
+    HIPCHK(hipMemcpyAsync(P->tab_dev, tab.data(), sizeof(double *) * tab.size(), hipMemcpyHostToDevice, s));
+    if (beta0_1d) {
+        HIPCHK(hipMemcpyAsync(P->b0_dev, beta0_1d, sizeof(double) * P->nr,
+                              P->opts.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(P->b0e_dev, beta0_1d_edge, sizeof(double) * (P->nr + 1),
+                              P->opts.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    }
+    HIPCHK(hipEventRecord(P->ev[1], s));
+    // ---- setup: beta0 U, rh, coefficients, warm start
+    SetupArgs A;
+    A.dim = dim; A.nboxes = nb; A.boxes = P->boxes_dev;
+    for (int d = 0; d < 3; ++d) { A.dlo[d] = P->dlo[d]; A.n[d] = P->n[d]; A.dx[d] = P->dx[d]; }
+    A.bc = P->bc;
+    A.ng_u = P->ng_u; A.ng_phi = P->ng_phi; A.ng_rho = P->ng_rho; A.ng_rhs = P->ng_rhs; A.ng_cart = P->ng_cart;
+    double **td = P->tab_dev;
+    A.umac = td; A.phi = td + nb * dim; A.rho = td + nb * dim + nb;
+    A.macrhs = macrhs ? td + nb * dim + 2 * nb : nullptr;
+    A.cart = beta0_cart_edge ? td + nb * dim + 3 * nb : nullptr;
+    A.beta0 = beta0_1d ? P->b0_dev : nullptr;
+    A.beta0e = beta0_1d ? P->b0e_dev : nullptr;
+    A.alt = P->opts.use_alt_energy_fix;
+    A.L = P->H64.lev[0];
+    long np = setup_nparts(P->maxcells, nb);
+    if ((rc = ensure_part(P, np))) return rc;
+    A.part = P->part;
+    A.err = P->err_dev;
+    HIPCHK(hipMemsetAsync(P->err_dev, 0, sizeof(int), s));
+    int pq = prof_begin(P, MAC_PROF_SETUP);
+    launch_setup(A, P->maxcells, s);
+    prof_end(P, pq);
+    const long n0 = ncell(P->n);
+    if (P->singular) {
+        launch_mean(P->part, np, (double)n0, P->mean_dev, s);
+        launch_sub_mean<double>(P->H64.lev[0].rhs, n0, P->mean_dev, s);
+    }
+    coarsen_all<double>(P, P->H64);
+    int err_host = 0;
+    HIPCHK(hipMemcpyAsync(&err_host, P->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(P->ev[2], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err_host) return MAC_ERR_NONFINITE;
+    // ---- solve
+    mac_stats local;
+    memset(&local, 0, sizeof local);
+    int src = solve(P, &local);
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    HIPCHK(hipEventRecord(P->ev[3], s));
+    // ---- correction, ghost fill, phi out
+    CorrectArgs C;
+    C.dim = dim; C.nboxes = nb; C.boxes = P->boxes_dev;
+    for (int d = 0; d < 3; ++d) {
+        C.dlo[d] = P->dlo[d]; C.n[d] = P->n[d]; C.dx[d] = P->dx[d];
+        C.mapg[d] = P->mapg[d]; C.mapn[d] = P->mapn[d];
+        for (int q = 0; q < 2; ++q) C.phys[d][q] = P->phys[d][q];
+    }
+    C.bc = P->bc;
+    C.ng_u = P->ng_u; C.ng_cart = P->ng_cart;
+    C.umac = td; C.beta0 = A.beta0; C.beta0e = A.beta0e; C.cart = A.cart;
+    const Lev<double> &L0 = P->H64.lev[0];
+    C.phi = L0.phi;
+    C.bx = L0.bx; C.by = L0.by; C.bz = L0.bz; C.bxh = L0.bxh; C.byh = L0.byh; C.bzh = L0.bzh;
+    C.boxmap = P->boxmap_dev;
+    C.maxplane = P->maxplane;
+    pq = prof_begin(P, MAC_PROF_CORRECT);
+    launch_correct(C, P->maxfaces, P->maxghost, s);
+    prof_end(P, pq);
+    hipLaunchKernelGGL(k_phi_out, dim3((P->maxcells + 255) / 256, nb), dim3(256), 0, s,
+                       (const BoxDev *)P->boxes_dev, (double *const *)(td + nb * dim), P->ng_phi, dim,
+                       (const double *)L0.phi, P->dlo[0], P->dlo[1], P->dlo[2], P->n[0], P->n[1]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P->ev[4], s));
+    if (!P->opts.device_ptrs) {
+        if ((rc = stage_out(P, umac, phi))) return rc;
+    }
+    HIPCHK(hipEventRecord(P->ev[5], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        *st = local;
+        st->t_h2d_ms = ev_ms(P, 0, 1);
+        st->t_setup_ms = ev_ms(P, 1, 2);
+        st->t_solve_ms = ev_ms(P, 2, 3);
+        st->t_correct_ms = ev_ms(P, 3, 4);
+        st->t_d2h_ms = ev_ms(P, 4, 5);
+        st->t_total_ms = ev_ms(P, 0, 5);
+    }
+    return src;
+}
+
+// -------------------------------------------------- whole-domain entries
+static int level_weights_and_rhs(mac_plan *P, LevelIO &IO)
+{
+    const Lev<double> &L0 = P->H64.lev[0];
+    long n0 = ncell(P->n);
+    long np = (n0 + 255) / 256;
+    int rc;
+    if ((rc = ensure_part(P, np))) return rc;
+    launch_level_setup(P->dim, IO, L0, P->part, P->stream);
+    if (P->singular) {
+        launch_mean(P->part, np, (double)n0, P->mean_dev, P->stream);
+        launch_sub_mean<double>(L0.rhs, n0, P->mean_dev, P->stream);
+    }
+    coarsen_all<double>(P, P->H64);
+    return MAC_OK;
+}
+
+extern "C" int mac_project_level(mac_plan *P, double *ux, double *uy, double *uz, double *phi,
+                                 const double *bx, const double *by, const double *bz,
+                                 const double *macrhs, mac_stats *st)
+{
+    if (!P || !ux || !uy || !phi || !bx || !by) return MAC_ERR_INVALID;
+    if (P->dim == 3 && (!uz || !bz)) return MAC_ERR_INVALID;
+    if (!P->opts.device_ptrs) return MAC_ERR_INVALID;
+    if (st) memset(st, 0, sizeof(*st));
+    HIPCHK(hipSetDevice(P->device));
+    hipStream_t s = P->stream;
+    HIPCHK(hipEventRecord(P->ev[0], s));
+    LevelIO IO;
+    IO.ux = ux; IO.uy = uy; IO.uz = uz;
+    IO.bx = bx; IO.by = by; IO.bz = bz;
+    IO.macrhs = macrhs; IO.phi = phi;
+    for (int d = 0; d < 3; ++d) IO.dx[d] = P->dx[d];
+    IO.bc = P->bc;
+    int rc;
+    if ((rc = level_weights_and_rhs(P, IO))) return rc;
+    HIPCHK(hipEventRecord(P->ev[1], s));
+    mac_stats local;
+    memset(&local, 0, sizeof local);
+    int src = solve(P, &local);
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    HIPCHK(hipEventRecord(P->ev[2], s));
+    const Lev<double> &L0 = P->H64.lev[0];
+    launch_level_correct(P->dim, IO, L0, s);
+    HIPCHK(hipMemcpyAsync(phi, L0.phi, sizeof(double) * ncell(P->n), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipEventRecord(P->ev[3], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        *st = local;
+        st->t_setup_ms = ev_ms(P, 0, 1);
+        st->t_solve_ms = ev_ms(P, 1, 2);
+        st->t_correct_ms = ev_ms(P, 2, 3);
+        st->t_total_ms = ev_ms(P, 0, 3);
+    }
+    return src;
+}
+
+extern "C" int mac_adjoint_level(mac_plan *P, const double *phi, const double *bx, const double *by,
+                                 const double *bz, const double *g_phi, const double *g_ux,
+                                 const double *g_uy, const double *g_uz, double *lambda, double *out_ux,
+                                 double *out_uy, double *out_uz, double *out_bx, double *out_by,
+                                 double *out_bz, mac_stats *st)
+{
+    if (!P || !phi || !bx || !by || !g_phi || !g_ux || !g_uy || !lambda || !out_ux || !out_uy ||
+        !out_bx || !out_by)
+        return MAC_ERR_INVALID;
+    if (P->dim == 3 && (!bz || !g_uz || !out_uz || !out_bz)) return MAC_ERR_INVALID;
+    if (!P->opts.device_ptrs) return MAC_ERR_INVALID;
+    if (st) memset(st, 0, sizeof(*st));
+    HIPCHK(hipSetDevice(P->device));
+    hipStream_t s = P->stream;
+    const long n0 = ncell(P->n);
+    const Lev<double> &L0 = P->H64.lev[0];
+    HIPCHK(hipEventRecord(P->ev[0], s));
+    // weights from beta (no velocity: rh is replaced below), lambda guess 0
+    if (!P->scratch_cells) HIPCHK(hipMalloc(&P->scratch_cells, sizeof(double) * n0));
+    launch_fill<double>(P->scratch_cells, n0, 0.0, s);
+    LevelIO IO;
+    memset(&IO, 0, sizeof IO);
+    // zero faces for the divergence: reuse the output buffers, cleared
+    long nfx = (long)(P->n[0] + 1) * P->n[1] * P->n[2];
+    long nfy = (long)P->n[0] * (P->n[1] + 1) * P->n[2];
+    long nfz = (long)P->n[0] * P->n[1] * (P->n[2] + 1);
+    launch_fill<double>(out_ux, nfx, 0.0, s);
+    launch_fill<double>(out_uy, nfy, 0.0, s);
+    if (P->dim == 3) launch_fill<double>(out_uz, nfz, 0.0, s);
+    IO.ux = out_ux; IO.uy = out_uy; IO.uz = out_uz;
+    IO.bx = bx; IO.by = by; IO.bz = bz;
+    IO.macrhs = nullptr;
+    IO.phi = P->scratch_cells;
+    for (int d = 0; d < 3; ++d) IO.dx[d] = P->dx[d];
+    IO.bc = P->bc;
+    int rc;
+    if ((rc = level_weights_and_rhs(P, IO))) return rc;
+    // adjoint right side a = g_phi - G^T(w g_U), mean-free on singular problems
+    AdjIO J;
+    J.gphi = g_phi; J.gux = g_ux; J.guy = g_uy; J.guz = g_uz;
+    J.out_rhs = L0.rhs;
+    J.out_ux = out_ux; J.out_uy = out_uy; J.out_uz = out_uz;
+    J.out_bx = out_bx; J.out_by = out_by; J.out_bz = out_bz;
+    for (int d = 0; d < 3; ++d) J.dx[d] = P->dx[d];
+    J.bc = P->bc;
+    long np = (n0 + 255) / 256;
+    launch_adj_rhs(P->dim, J, L0, P->part, s);
+    if (P->singular) {
+        launch_mean(P->part, np, (double)n0, P->mean_dev, s);
+        launch_sub_mean<double>(L0.rhs, n0, P->mean_dev, s);
+    }
+    HIPCHK(hipEventRecord(P->ev[1], s));
+    mac_stats local;
+    memset(&local, 0, sizeof local);
+    int src = solve(P, &local);
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    if (P->singular)
+        if ((rc = remove_mean(P, L0.phi, n0))) return rc;
+    HIPCHK(hipEventRecord(P->ev[2], s));
+    HIPCHK(hipMemcpyAsync(lambda, L0.phi, sizeof(double) * n0, hipMemcpyDeviceToDevice, s));
+    launch_adj_faces(P->dim, J, L0, phi, L0.phi, s);
+    HIPCHK(hipEventRecord(P->ev[3], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        *st = local;
+        st->t_setup_ms = ev_ms(P, 0, 1);
+        st->t_solve_ms = ev_ms(P, 1, 2);
+        st->t_correct_ms = ev_ms(P, 2, 3);
+        st->t_total_ms = ev_ms(P, 0, 3);
+    }
+    return src;
+}
+
+// ------------------------------------------------------------ diagnostics
+static int debug_array(mac_plan *P, int level, int what, double **ptr, long *n)
+{
+    if (!P || level < 0 || level >= P->H64.nlev) return MAC_ERR_INVALID;
+    const Lev<double> &L = P->H64.lev[level];
+    long nc = (long)L.nx * L.ny * L.nz;
+    switch (what) {
+    case MAC_ARR_PHI: *ptr = L.phi; *n = nc; break;
+    case MAC_ARR_RHS: *ptr = L.rhs; *n = nc; break;
+    case MAC_ARR_RES: *ptr = L.res; *n = nc; break;
+    case MAC_ARR_BX: *ptr = L.bx; *n = nc; break;
+    case MAC_ARR_BY: *ptr = L.by; *n = nc; break;
+    case MAC_ARR_BZ: *ptr = L.bz; *n = nc; break;
+    case MAC_ARR_BXH: *ptr = L.bxh; *n = (long)L.ny * L.nz; break;
+    case MAC_ARR_BYH: *ptr = L.byh; *n = (long)L.nx * L.nz; break;
+    case MAC_ARR_BZH: *ptr = L.bzh; *n = (long)L.nx * L.ny; break;
+    default: return MAC_ERR_INVALID;
+    }
+    if (!*ptr) return MAC_ERR_INVALID;
+    return MAC_OK;
+}
+
+extern "C" int mac_debug_level_dims(mac_plan *P, int level, int dims[3])
+{
+    if (!P || !dims || level < 0 || level >= P->H64.nlev) return MAC_ERR_INVALID;
+    dims[0] = P->H64.lev[level].nx;
+    dims[1] = P->H64.lev[level].ny;
+    dims[2] = P->H64.lev[level].nz;
+    return MAC_OK;
+}
+
+extern "C" int mac_debug_get(mac_plan *P, int level, int what, double *out)
+{
+    double *p;
+    long n;
+    int rc = debug_array(P, level, what, &p, &n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, p, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}
+
+extern "C" int mac_debug_set(mac_plan *P, int level, int what, const double *in)
+{
+    double *p;
+    long n;
+    int rc = debug_array(P, level, what, &p, &n);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(p, in, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}
+
+extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
+{
+    if (!P || level < 0 || level >= P->H64.nlev || count < 0) return MAC_ERR_INVALID;
+    Hier<double> &H = P->H64;
+    const Lev<double> &L = H.lev[level];
+    hipStream_t s = P->stream;
+    bool has_next = level + 1 < H.nlev;
+    for (int c = 0; c < count; ++c) {
+        switch (op) {
+        case MAC_OP_GSRB:
+            launch_gsrb<double>(P->dim, L, 0, s);
+            launch_gsrb<double>(P->dim, L, 1, s);
+            break;
+        case MAC_OP_RESIDUAL: launch_residual<double>(P->dim, L, L.res, nullptr, s); break;
+        case MAC_OP_RESTRICT:
+            if (!has_next) return MAC_ERR_INVALID;
+            launch_restrict<double>(P->dim, L.res, L.nx, L.ny, H.lev[level + 1], s);
+            break;
+        case MAC_OP_PROLONG:
+            if (!has_next) return MAC_ERR_INVALID;
+            launch_prolong<double>(P->dim, L, H.lev[level + 1], P->bc, s);
+            break;
+        case MAC_OP_COARSEN:
+            if (!has_next) return MAC_ERR_INVALID;
+            launch_coarsen<double>(P->dim, L, H.lev[level + 1], s);
+            break;
+        case MAC_OP_VCYCLE:
+            if (level != 0) return MAC_ERR_INVALID;
+            vcycle<double>(P, H);
+            break;
+        default: return MAC_ERR_INVALID;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return MAC_OK;
+}

@@ -1,0 +1,341 @@
+"""Python binding of ``libmaestro_mac.so`` (the gfx950 MAC projection).
+
+Mirrors the reference's MAC-projection interface,
+``macproject(mla, umac, phi, rho, dx, the_bc_tower[, macrhs][, beta0_1d,
+beta0_1d_edge][, beta0_cart_edge])`` (Source/macproject.f90:51-76), over the
+per-box arrays of :mod:`maestro_amd.boxes`:
+
+* ``Layout`` plays the role of ``mla`` + ``dx`` + ``the_bc_tower``: the boxes,
+  the problem domain, the cell size and the physical BC codes.
+* :func:`macproject` takes per-box ``umac`` (list of ``dim`` arrays per box),
+  ``phi`` and ``rho`` (lists of arrays), modifies ``umac`` and ``phi`` in
+  place and returns the solver statistics.  Invalid option combinations raise
+  :class:`MacError` as the reference calls ``bl_error`` (:99-109).
+
+Arrays may be numpy (host) or torch tensors on the GPU; a call uses one kind
+throughout.  There is no CPU fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmaestro_mac.so")
+
+PERIODIC, INTERIOR, INLET, OUTLET, SYMMETRY, SLIP_WALL, NO_SLIP_WALL = -1, 0, 11, 12, 13, 14, 15
+PREC_FP64, PREC_MIXED = 0, 1
+ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
+          3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory"}
+SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
+           "mac_project", "mac_project_level", "mac_adjoint_level", "mac_version",
+           "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
+           "mac_plan_profile", "mac_plan_profile_read")
+PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5}
+ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
+OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5}
+
+
+class MacError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__(f"{what}: {ERRORS.get(code, 'error')} (code {code})")
+
+
+class CLayout(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nboxes", C.c_int), ("box_lo", C.POINTER(C.c_int)),
+                ("box_hi", C.POINTER(C.c_int)), ("domain_lo", C.c_int * 3),
+                ("domain_hi", C.c_int * 3), ("dx", C.c_double * 3),
+                ("phys_bc", (C.c_int * 2) * 3), ("ng_umac", C.c_int), ("ng_phi", C.c_int),
+                ("ng_rho", C.c_int), ("ng_macrhs", C.c_int), ("ng_beta0_cart", C.c_int)]
+
+
+class COpts(C.Structure):
+    _fields_ = [("eps_rel", C.c_double), ("eps_abs", C.c_double), ("max_iter", C.c_int),
+                ("nu1", C.c_int), ("nu2", C.c_int), ("nub", C.c_int),
+                ("bottom_eps", C.c_double), ("bottom_max_iter", C.c_int),
+                ("max_levels", C.c_int), ("use_alt_energy_fix", C.c_int),
+                ("precision", C.c_int), ("inner_cycles", C.c_int), ("device_ptrs", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class CStats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("inner_cycles", C.c_int), ("converged", C.c_int),
+                ("nlevels", C.c_int), ("rhs_norm", C.c_double), ("res0", C.c_double),
+                ("res", C.c_double), ("t_h2d_ms", C.c_double), ("t_setup_ms", C.c_double),
+                ("t_solve_ms", C.c_double), ("t_correct_ms", C.c_double),
+                ("t_d2h_ms", C.c_double), ("t_total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libmaestro_mac.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with maestro_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.c_int
+        L.mac_default_opts.argtypes = [C.POINTER(COpts)]
+        L.mac_plan_create.argtypes = [C.POINTER(CLayout), C.POINTER(COpts), C.POINTER(vp)]
+        L.mac_plan_destroy.argtypes = [vp]
+        L.mac_plan_set_opts.argtypes = [vp, C.POINTER(COpts)]
+        L.mac_project.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
+        L.mac_project_level.argtypes = [vp] * 9 + [C.POINTER(CStats)]
+        L.mac_adjoint_level.argtypes = [vp] * 17 + [C.POINTER(CStats)]
+        L.mac_version.restype = C.c_char_p
+        L.mac_debug_level_dims.argtypes = [vp, C.POINTER(C.c_int)]
+        L.mac_debug_get.argtypes = [vp, ip, ip, vp]
+        L.mac_debug_set.argtypes = [vp, ip, ip, vp]
+        L.mac_debug_op.argtypes = [vp, ip, ip, ip]
+        L.mac_plan_profile.argtypes = [vp, ip]
+        L.mac_plan_profile_read.argtypes = [vp, ip, C.POINTER(C.c_long), C.POINTER(C.c_double)]
+        for f in ("mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
+                  "mac_plan_profile", "mac_plan_profile_read"):
+            getattr(L, f).restype = ip
+        for f in ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
+                  "mac_project", "mac_project_level", "mac_adjoint_level"):
+            getattr(L, f).restype = ip
+        _lib = L
+    return _lib
+
+
+def default_opts(**kw) -> COpts:
+    o = COpts()
+    lib().mac_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class Layout:
+    """Boxes, domain, cell size and physical BCs of one level (``mla`` +
+    ``dx`` + ``the_bc_tower`` of the reference call)."""
+    dim: int
+    boxes: list            # [(lo, hi)] inclusive cell indices
+    domain_lo: tuple
+    domain_hi: tuple
+    dx: tuple
+    phys_bc: list          # [[lo, hi]] per direction
+    ng_umac: int = 1
+    ng_phi: int = 1
+    ng_rho: int = 1
+    ng_macrhs: int = 0
+    ng_beta0_cart: int = 1
+    _keep: list = field(default_factory=list, repr=False)
+
+    def to_c(self) -> CLayout:
+        nb = len(self.boxes)
+        lo = (C.c_int * (3 * nb))()
+        hi = (C.c_int * (3 * nb))()
+        for b, (l, h) in enumerate(self.boxes):
+            for d in range(3):
+                lo[3 * b + d] = int(l[d]) if d < self.dim else 0
+                hi[3 * b + d] = int(h[d]) if d < self.dim else 0
+        self._keep[:] = [lo, hi]
+        c = CLayout()
+        c.dim, c.nboxes = self.dim, nb
+        c.box_lo = C.cast(lo, C.POINTER(C.c_int))
+        c.box_hi = C.cast(hi, C.POINTER(C.c_int))
+        for d in range(3):
+            c.domain_lo[d] = int(self.domain_lo[d]) if d < self.dim else 0
+            c.domain_hi[d] = int(self.domain_hi[d]) if d < self.dim else 0
+            c.dx[d] = float(self.dx[d]) if d < self.dim else 1.0
+            for s in range(2):
+                c.phys_bc[d][s] = int(self.phys_bc[d][s]) if d < self.dim else PERIODIC
+        c.ng_umac, c.ng_phi, c.ng_rho = self.ng_umac, self.ng_phi, self.ng_rho
+        c.ng_macrhs, c.ng_beta0_cart = self.ng_macrhs, self.ng_beta0_cart
+        return c
+
+    @staticmethod
+    def uniform(n, box, dx, phys_bc, **ng):
+        from .workloads import box_list
+        dim = len(n)
+        return Layout(dim=dim, boxes=box_list(n, box), domain_lo=(0,) * dim,
+                      domain_hi=tuple(x - 1 for x in n), dx=tuple(dx), phys_bc=phys_bc, **ng)
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        if not x.is_contiguous() or x.dtype.itemsize != 8:
+            raise ValueError("device arrays must be contiguous float64 tensors")
+        return C.c_void_p(x.data_ptr())
+    if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+        raise ValueError("host arrays must be C-contiguous float64 numpy arrays")
+    return C.c_void_p(x.ctypes.data)
+
+
+def _ptr_array(xs):
+    if xs is None:
+        return None
+    arr = (C.c_void_p * len(xs))(*[_ptr(x).value for x in xs])
+    return arr
+
+
+class MacPlan:
+    """A plan: validated layout + device MG hierarchy (``mac_plan_create``)."""
+
+    def __init__(self, layout: Layout, opts: COpts | None = None, **kw):
+        self.layout = layout
+        self.opts = opts or default_opts(**kw)
+        self._c_layout = layout.to_c()
+        h = C.c_void_p()
+        rc = lib().mac_plan_create(C.byref(self._c_layout), C.byref(self.opts), C.byref(h))
+        if rc:
+            raise MacError(rc, "mac_plan_create")
+        self._h = h
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        rc = lib().mac_plan_set_opts(self._h, C.byref(self.opts))
+        if rc:
+            raise MacError(rc, "mac_plan_set_opts")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mac_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _sync_torch(self, xs):
+        if any(_is_torch(x) for x in xs if x is not None):
+            import torch
+            torch.cuda.current_stream().synchronize()
+
+    def project(self, umac, phi, rho, macrhs=None, beta0_1d=None, beta0_1d_edge=None,
+                beta0_cart_edge=None, check=True):
+        """macproject on per-box arrays; ``umac`` is a list (per box) of ``dim``
+        face arrays.  Returns the stats dict."""
+        dim = self.layout.dim
+        flat_u = [u for box in umac for u in box]
+        flat_c = None if beta0_cart_edge is None else [c for box in beta0_cart_edge for c in box]
+        dev = _is_torch(flat_u[0])
+        if bool(self.opts.device_ptrs) != dev:
+            self.set_opts(device_ptrs=int(dev))
+        self._sync_torch(flat_u + list(phi) + list(rho))
+        assert all(len(box) == dim for box in umac)
+        keep = [_ptr_array(flat_u), _ptr_array(list(phi)), _ptr_array(list(rho)),
+                _ptr_array(macrhs), _ptr_array(flat_c)]
+        b0 = _ptr(beta0_1d) if beta0_1d is not None else None
+        b0e = _ptr(beta0_1d_edge) if beta0_1d_edge is not None else None
+        st = CStats()
+        rc = lib().mac_project(self._h, keep[0], keep[1], keep[2], keep[3], b0, b0e, keep[4],
+                               C.byref(st))
+        if rc and (check or rc != 4):
+            raise MacError(rc, "mac_project")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    # ---- kernel timing on the plan's stream
+    def profile(self, enable=True):
+        rc = lib().mac_plan_profile(self._h, int(enable))
+        if rc:
+            raise MacError(rc, "mac_plan_profile")
+
+    def profile_read(self):
+        out = {}
+        for name, cls in PROF.items():
+            cnt, ms = C.c_long(), C.c_double()
+            rc = lib().mac_plan_profile_read(self._h, cls, C.byref(cnt), C.byref(ms))
+            if rc:
+                raise MacError(rc, "mac_plan_profile_read")
+            out[name] = (cnt.value, ms.value)
+        return out
+
+    # ---- diagnostics: internal FP64 level arrays and single operators
+    def level_dims(self, level):
+        d = (C.c_int * 3)()
+        rc = lib().mac_debug_level_dims(self._h, level, d)
+        if rc:
+            raise MacError(rc, "mac_debug_level_dims")
+        return tuple(d)
+
+    def _arr_size(self, level, what):
+        nx, ny, nz = self.level_dims(level)
+        return {"bxh": ny * nz, "byh": nx * nz, "bzh": nx * ny}.get(what, nx * ny * nz)
+
+    def debug_get(self, level, what):
+        out = np.zeros(self._arr_size(level, what))
+        rc = lib().mac_debug_get(self._h, level, ARR[what], C.c_void_p(out.ctypes.data))
+        if rc:
+            raise MacError(rc, "mac_debug_get")
+        return out
+
+    def debug_set(self, level, what, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        assert a.size == self._arr_size(level, what)
+        rc = lib().mac_debug_set(self._h, level, ARR[what], C.c_void_p(a.ctypes.data))
+        if rc:
+            raise MacError(rc, "mac_debug_set")
+
+    def debug_op(self, level, op, count=1):
+        rc = lib().mac_debug_op(self._h, level, OP[op], count)
+        if rc:
+            raise MacError(rc, "mac_debug_op")
+
+    def project_level(self, ux, uy, uz, phi, bx, by, bz, macrhs=None, check=True):
+        """Whole-domain device entry (natural face layout, torch tensors)."""
+        if not self.opts.device_ptrs:
+            self.set_opts(device_ptrs=1)
+        self._sync_torch([ux])
+        st = CStats()
+        rc = lib().mac_project_level(self._h, _ptr(ux), _ptr(uy), _ptr(uz), _ptr(phi), _ptr(bx),
+                                     _ptr(by), _ptr(bz), _ptr(macrhs), C.byref(st))
+        if rc and (check or rc != 4):
+            raise MacError(rc, "mac_project_level")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+    def adjoint_level(self, phi, bx, by, bz, gphi, gux, guy, guz, lam, oux, ouy, ouz, obx, oby,
+                      obz, check=True):
+        if not self.opts.device_ptrs:
+            self.set_opts(device_ptrs=1)
+        self._sync_torch([phi])
+        st = CStats()
+        args = [_ptr(a) for a in (phi, bx, by, bz, gphi, gux, guy, guz, lam, oux, ouy, ouz, obx,
+                                  oby, obz)]
+        rc = lib().mac_adjoint_level(self._h, *args, C.byref(st))
+        if rc and (check or rc != 4):
+            raise MacError(rc, "mac_adjoint_level")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+
+def macproject(layout: Layout, umac, phi, rho, macrhs=None, beta0_1d=None, beta0_1d_edge=None,
+               beta0_cart_edge=None, opts: COpts | None = None, plan: MacPlan | None = None, **kw):
+    """``macproject`` (Source/macproject.f90:51-328) on per-box arrays.
+    ``umac`` and ``phi`` are updated in place; returns the solver stats."""
+    if beta0_1d is not None and beta0_cart_edge is not None:
+        raise MacError(1, "macproject: CANT HAVE 1D and 3D DIV_COEFF IN MACPROJECT")
+    p = plan or MacPlan(layout, opts, **kw)
+    try:
+        return p.project(umac, phi, rho, macrhs, beta0_1d, beta0_1d_edge, beta0_cart_edge)
+    finally:
+        if plan is None:
+            p.close()

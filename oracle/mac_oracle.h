@@ -1,0 +1,108 @@
+/*
+ * mac_oracle.h -- FP64 CPU restatement of MAESTRO's MAC projection.
+ * TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU
+ * baseline of bench.py as the checker; never linked by the product library.
+ */
+#ifndef MAC_ORACLE_H
+#define MAC_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* physical BC codes (FBoxLib bc_module values; OUTLET=12 and SYMMETRY=13
+ * also appear in Exec/SCIENCE/sub_chandra/paper_II/PyRoot/subchandra.py) */
+enum {
+    ORC_PERIODIC = -1,
+    ORC_INTERIOR = 0,
+    ORC_INLET = 11,
+    ORC_OUTLET = 12,
+    ORC_SYMMETRY = 13,
+    ORC_SLIP_WALL = 14,
+    ORC_NO_SLIP_WALL = 15
+};
+enum { ORC_ELL_PER = -1, ORC_ELL_DIR = 1, ORC_ELL_NEU = 2 };
+enum { ORC_OK = 0, ORC_ERR_INVALID = 1, ORC_ERR_NOT_CONVERGED = 4 };
+
+#define ORC_MAX_HIST 256
+
+typedef struct {
+    int dim;          /* 2 or 3 */
+    int n[3];         /* cells per direction; n[2] = 1 in 2D */
+    double dx[3];
+    int bc[3][2];     /* physical BC code per direction and side (lo, hi) */
+} orc_grid;
+
+typedef struct {
+    double eps_rel, eps_abs;
+    int max_iter, nu1, nu2, nub;
+    double bottom_eps;
+    int bottom_maxit;
+    int max_levels;   /* 0 = coarsen as far as possible */
+    int use_alt_energy_fix;
+} orc_opts;
+
+typedef struct {
+    int iters, converged, nlev;
+    double res0, res;
+    double hist[ORC_MAX_HIST];
+} orc_stats;
+
+typedef struct {
+    int n[3];
+    double h[3];
+    double *b[3];     /* raw face coefficients, natural (n+1)-in-d layout */
+    double *phi, *rhs, *res;
+} orc_level;
+
+typedef struct {
+    orc_grid g;
+    orc_opts opts;
+    int nlev;
+    orc_level *lev;
+} orc_mg;
+
+void orc_default_opts(orc_opts *o);
+int orc_num_threads(void);
+int orc_ell_bc(int phys);
+int orc_is_singular(const orc_grid *g);
+double orc_sum(const double *x, long n);
+
+void orc_mk_mac_coeffs(const orc_grid *g, const double *rho, double *bx,
+                       double *by, double *bz);
+void orc_scale_faces_1d(const orc_grid *g, double *fx, double *fy, double *fz,
+                        const double *beta0, const double *beta0_edge, int do_mult);
+void orc_scale_faces_cart(const orc_grid *g, double *fx, double *fy, double *fz,
+                          const double *cx, const double *cy, const double *cz,
+                          int do_mult);
+void orc_divumac(const orc_grid *g, const double *u, const double *v,
+                 const double *w, const double *macrhs, double *rh);
+
+void orc_apply(const orc_grid *g, const orc_level *lev, const double *phi,
+               double *out);
+double orc_residual(const orc_grid *g, const orc_level *lev, const double *phi,
+                    const double *rhs, double *res);
+void orc_gsrb(const orc_grid *g, orc_level *lev);
+
+int orc_mg_nlevels(const orc_grid *g, int max_levels);
+orc_mg *orc_mg_create(const orc_grid *g, const double *bx, const double *by,
+                      const double *bz, const orc_opts *o);
+void orc_mg_destroy(orc_mg *mg);
+int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st);
+
+void orc_mkumac(const orc_grid *g, double *u, double *v, double *w,
+                const double *phi, const double *bx, const double *by,
+                const double *bz);
+
+int orc_macproject(const orc_grid *g, double *u, double *v, double *w,
+                   double *phi, const double *rho, const double *macrhs,
+                   const double *beta0, const double *beta0_edge,
+                   const double *cart_x, const double *cart_y,
+                   const double *cart_z, const orc_opts *o, orc_stats *st,
+                   double *bx_out, double *by_out, double *bz_out,
+                   double *rh_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,252 @@
+"""ctypes wrapper of the FP64 CPU oracle (``liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke``
+and the CPU-baseline leg of ``bench.py`` as the checker; the product package
+``maestro_amd`` never imports it.
+
+Arrays follow the oracle layout (C order, x fastest):
+cells ``(nz, ny, nx)``; rho with one ghost layer ``(nz+2gz, ny+2, nx+2)``;
+faces ``x:(nz,ny,nx+1)``, ``y:(nz,ny+1,nx)``, ``z:(nz+1,ny,nx)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+PERIODIC, INTERIOR, INLET, OUTLET, SYMMETRY, SLIP_WALL, NO_SLIP_WALL = -1, 0, 11, 12, 13, 14, 15
+ELL_PER, ELL_DIR, ELL_NEU = -1, 1, 2
+MAX_HIST = 256
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("dx", C.c_double * 3),
+                ("bc", (C.c_int * 2) * 3)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("eps_rel", C.c_double), ("eps_abs", C.c_double),
+                ("max_iter", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int),
+                ("nub", C.c_int), ("bottom_eps", C.c_double),
+                ("bottom_maxit", C.c_int), ("max_levels", C.c_int),
+                ("use_alt_energy_fix", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("nlev", C.c_int),
+                ("res0", C.c_double), ("res", C.c_double),
+                ("hist", C.c_double * MAX_HIST)]
+
+
+class Level(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("h", C.c_double * 3),
+                ("b", C.POINTER(C.c_double) * 3),
+                ("phi", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
+                ("res", C.POINTER(C.c_double))]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        L.orc_default_opts.argtypes = [C.POINTER(Opts)]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_sum.restype = C.c_double
+        L.orc_sum.argtypes = [dp, C.c_long]
+        L.orc_mk_mac_coeffs.argtypes = [C.POINTER(Grid), dp, dp, dp, dp]
+        L.orc_scale_faces_1d.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, C.c_int]
+        L.orc_scale_faces_cart.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, C.c_int]
+        L.orc_divumac.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp]
+        L.orc_apply.argtypes = [C.POINTER(Grid), C.POINTER(Level), dp, dp]
+        L.orc_residual.restype = C.c_double
+        L.orc_residual.argtypes = [C.POINTER(Grid), C.POINTER(Level), dp, dp, dp]
+        L.orc_gsrb.argtypes = [C.POINTER(Grid), C.POINTER(Level)]
+        L.orc_mg_nlevels.restype = C.c_int
+        L.orc_mg_nlevels.argtypes = [C.POINTER(Grid), C.c_int]
+        L.orc_mg_create.restype = C.c_void_p
+        L.orc_mg_create.argtypes = [C.POINTER(Grid), dp, dp, dp, C.POINTER(Opts)]
+        L.orc_mg_destroy.argtypes = [C.c_void_p]
+        L.orc_mg_solve.restype = C.c_int
+        L.orc_mg_solve.argtypes = [C.c_void_p, dp, dp, C.POINTER(Stats)]
+        L.orc_mkumac.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp]
+        L.orc_macproject.restype = C.c_int
+        L.orc_macproject.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp, dp,
+                                     dp, dp, dp, C.POINTER(Opts), C.POINTER(Stats),
+                                     dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_grid(n, dx, bc, dim=None) -> Grid:
+    dim = dim or len(n)
+    g = Grid()
+    g.dim = dim
+    nn = list(n) + [1] * (3 - len(n))
+    dd = list(dx) + [1.0] * (3 - len(dx))
+    for d in range(3):
+        g.n[d] = int(nn[d])
+        g.dx[d] = float(dd[d])
+        for s in range(2):
+            g.bc[d][s] = int(bc[d][s]) if d < dim else PERIODIC
+    return g
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().orc_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def cell_shape(g):
+    return (g.n[2], g.n[1], g.n[0])
+
+
+def face_shape(g, d):
+    n = [g.n[0], g.n[1], g.n[2]]
+    n[d] += 1
+    return (n[2], n[1], n[0])
+
+
+def rho_shape(g):
+    gz = 1 if g.dim == 3 else 0
+    return (g.n[2] + 2 * gz, g.n[1] + 2, g.n[0] + 2)
+
+
+def mk_mac_coeffs(g, rho):
+    b = [np.zeros(face_shape(g, d)) for d in range(3)]
+    lib().orc_mk_mac_coeffs(C.byref(g), _p(np.ascontiguousarray(rho)), _p(b[0]), _p(b[1]),
+                            _p(b[2]) if g.dim == 3 else None)
+    return b[:g.dim]
+
+
+def divumac(g, u, v, w=None, macrhs=None):
+    rh = np.zeros(cell_shape(g))
+    lib().orc_divumac(C.byref(g), _p(u), _p(v), _p(w), _p(macrhs), _p(rh))
+    return rh
+
+
+def scale_faces_1d(g, faces, beta0, beta0_edge, do_mult):
+    f = [np.array(x, dtype=np.float64, copy=True) for x in faces] + [None] * (3 - len(faces))
+    lib().orc_scale_faces_1d(C.byref(g), _p(f[0]), _p(f[1]), _p(f[2]),
+                             _p(np.ascontiguousarray(beta0, dtype=np.float64)),
+                             _p(np.ascontiguousarray(beta0_edge, dtype=np.float64)),
+                             int(do_mult))
+    return f[:g.dim]
+
+
+class LevelView:
+    """Keeps numpy buffers alive behind an oracle ``orc_level``."""
+
+    def __init__(self, g, b, phi, rhs, h=None):
+        self.g = g
+        self.b = [np.ascontiguousarray(x, dtype=np.float64) for x in b]
+        self.phi = np.ascontiguousarray(phi, dtype=np.float64)
+        self.rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        self.res = np.zeros_like(self.phi)
+        L = Level()
+        for d in range(3):
+            L.n[d] = g.n[d]
+            L.h[d] = (h[d] if h is not None else g.dx[d]) if d < g.dim else 1.0
+            L.b[d] = _p(self.b[d]) if d < g.dim else C.POINTER(C.c_double)()
+        L.phi = _p(self.phi)
+        L.rhs = _p(self.rhs)
+        L.res = _p(self.res)
+        self.L = L
+
+
+def apply(g, b, phi):
+    lv = LevelView(g, b, phi, np.zeros(cell_shape(g)))
+    out = np.zeros(cell_shape(g))
+    lib().orc_apply(C.byref(g), C.byref(lv.L), _p(lv.phi), _p(out))
+    return out
+
+
+def residual(g, b, phi, rhs):
+    lv = LevelView(g, b, phi, rhs)
+    res = np.zeros(cell_shape(g))
+    m = lib().orc_residual(C.byref(g), C.byref(lv.L), _p(lv.phi), _p(lv.rhs), _p(res))
+    return res, m
+
+
+def gsrb(g, b, phi, rhs, sweeps=1):
+    lv = LevelView(g, b, np.array(phi, dtype=np.float64, copy=True), rhs)
+    for _ in range(sweeps):
+        lib().orc_gsrb(C.byref(g), C.byref(lv.L))
+    return lv.phi
+
+
+def mg_solve(g, b, phi, rh, opts=None):
+    """FP64 MG solve of A phi = rh; returns (phi, rh_used, stats)."""
+    opts = opts or default_opts()
+    phi = np.array(phi, dtype=np.float64, copy=True)
+    rh = np.array(rh, dtype=np.float64, copy=True)
+    bb = [np.ascontiguousarray(x, dtype=np.float64) for x in b] + [None] * (3 - len(b))
+    mg = lib().orc_mg_create(C.byref(g), _p(bb[0]), _p(bb[1]), _p(bb[2]), C.byref(opts))
+    st = Stats()
+    try:
+        rc = lib().orc_mg_solve(mg, _p(phi), _p(rh), C.byref(st))
+    finally:
+        lib().orc_mg_destroy(mg)
+    return phi, rh, st, rc
+
+
+def mkumac(g, faces, phi, b):
+    f = [np.array(x, dtype=np.float64, copy=True) for x in faces] + [None] * (3 - len(faces))
+    bb = list(b) + [None] * (3 - len(b))
+    lib().orc_mkumac(C.byref(g), _p(f[0]), _p(f[1]), _p(f[2]), _p(phi), _p(bb[0]), _p(bb[1]),
+                     _p(bb[2]))
+    return f[:g.dim]
+
+
+def macproject(g, faces, phi, rho, macrhs=None, beta0=None, beta0_edge=None,
+               cart=None, opts=None):
+    """Whole-domain oracle macproject.  Returns dict with the projected faces,
+    phi, face coefficients, rh and stats."""
+    opts = opts or default_opts()
+    f = [np.array(x, dtype=np.float64, copy=True) for x in faces] + [None] * (3 - len(faces))
+    phi = np.array(phi, dtype=np.float64, copy=True)
+    b = [np.zeros(face_shape(g, d)) for d in range(g.dim)] + [None] * (3 - g.dim)
+    rh = np.zeros(cell_shape(g))
+    cc = list(cart) + [None] * (3 - len(cart)) if cart is not None else [None] * 3
+    st = Stats()
+    rc = lib().orc_macproject(
+        C.byref(g), _p(f[0]), _p(f[1]), _p(f[2]), _p(phi),
+        _p(np.ascontiguousarray(rho, dtype=np.float64)),
+        _p(None if macrhs is None else np.ascontiguousarray(macrhs, dtype=np.float64)),
+        _p(None if beta0 is None else np.ascontiguousarray(beta0, dtype=np.float64)),
+        _p(None if beta0_edge is None else np.ascontiguousarray(beta0_edge, dtype=np.float64)),
+        _p(cc[0]), _p(cc[1]), _p(cc[2]), C.byref(opts), C.byref(st),
+        _p(b[0]), _p(b[1]), _p(b[2]), _p(rh))
+    return {"faces": f[:g.dim], "phi": phi, "beta": b[:g.dim], "rh": rh, "stats": st, "rc": rc}
+
+
+def num_threads() -> int:
+    return lib().orc_num_threads()

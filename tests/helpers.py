@@ -1,0 +1,74 @@
+"""Shared helpers of the test-suite: per-box inputs from a workload, the
+HIP path through the C ABI, the oracle on the same inputs, error norms."""
+from __future__ import annotations
+
+import numpy as np
+
+from maestro_amd import boxes as BX
+from maestro_amd import workloads as W
+
+
+def per_box_inputs(wl, ng_u=1, ng_phi=1, ng_rho=1, ng_rhs=0, ng_cart=1, macrhs=None):
+    return BX.split_workload(wl, ng_u=ng_u, ng_phi=ng_phi, ng_rho=ng_rho, ng_rhs=ng_rhs,
+                             ng_cart=ng_cart, macrhs=macrhs)
+
+
+def layout_for(wl, **ng):
+    from maestro_amd.mac import Layout
+    return Layout.uniform(wl["n"], wl["box"], wl["dx"], wl["bc"], **ng)
+
+
+def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, **kw):
+    """Run macproject through the C ABI on per-box arrays; returns the
+    whole-domain projected faces, phi and the stats."""
+    from maestro_amd import mac
+    boxes, umac, phi, rho, rhs, cart = per_box_inputs(wl, ng_u=ng_u, macrhs=macrhs)
+    lay = layout_for(wl, ng_umac=ng_u, ng_macrhs=0)
+    p = plan or mac.MacPlan(lay, opts or mac.default_opts(**kw))
+    if device:
+        import torch
+        cv = lambda a: torch.from_numpy(a).cuda()
+        du = [[cv(a) for a in box] for box in umac]
+        dp = [cv(a) for a in phi]
+        dr = [cv(a) for a in rho]
+        drh = None if rhs is None else [cv(a) for a in rhs]
+        dc = None if cart is None else [[cv(a) for a in box] for box in cart]
+        b0 = None if wl["beta0"] is None else cv(np.ascontiguousarray(wl["beta0"]))
+        b0e = None if wl["beta0_edge"] is None else cv(np.ascontiguousarray(wl["beta0_edge"]))
+        st = p.project(du, dp, dr, drh, b0, b0e, dc)
+        umac = [[a.cpu().numpy() for a in box] for box in du]
+        phi = [a.cpu().numpy() for a in dp]
+    else:
+        st = p.project(umac, phi, rho, rhs,
+                       None if wl["beta0"] is None else np.ascontiguousarray(wl["beta0"]),
+                       None if wl["beta0_edge"] is None else np.ascontiguousarray(wl["beta0_edge"]),
+                       cart)
+    dim, n = wl["dim"], wl["n"]
+    nn = list(n) + [1] * (3 - dim)
+    faces = [BX.gather_faces([u[d] for u in umac], boxes, d, ng_u, nn, dim) for d in range(dim)]
+    phig = BX.gather_cells(phi, boxes, 1, nn, dim)
+    if plan is None:
+        p.close()
+    return {"faces": faces, "phi": phig, "stats": st, "umac_boxes": umac, "boxes": boxes}
+
+
+def run_oracle(wl, eps=1e-13, alt=1, macrhs=None, **kw):
+    from oracle import oracle as O
+    g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
+    opts = O.default_opts(eps_rel=eps, use_alt_energy_fix=alt, **kw)
+    return O.macproject(g, wl["faces"], wl["phi"], wl["rho"], macrhs=macrhs, beta0=wl["beta0"],
+                        beta0_edge=wl["beta0_edge"], cart=wl["cart"], opts=opts)
+
+
+def rel_err(a, b, remove_mean=False):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if remove_mean:
+        a = a - a.mean()
+        b = b - b.mean()
+    nb = np.linalg.norm(b.ravel())
+    return float(np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0))
+
+
+def max_rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
