@@ -180,6 +180,10 @@ int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
 #define MAC_OP_PROLONG 3     /* phi(level) += P phi(level+1)              */
 #define MAC_OP_COARSEN 4     /* weights(level+1) from weights(level)      */
 #define MAC_OP_VCYCLE 5      /* one V-cycle from `level` (0 only)         */
+#define MAC_OP_RESID_RESTRICT 6  /* rhs(level+1) = average of (rhs - A phi)(level),
+                                    phi(level+1) = 0, one fused pass       */
+#define MAC_OP_PROLONG_SWEEP 7   /* phi(level) += P phi(level+1), then one
+                                    sweep, as the V-cycle's first upward sweep */
 int mac_debug_level_dims(mac_plan *plan, int level, int dims[3]);
 int mac_debug_get(mac_plan *plan, int level, int what, double *out);
 int mac_debug_set(mac_plan *plan, int level, int what, const double *in);

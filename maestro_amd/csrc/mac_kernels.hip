@@ -20,6 +20,8 @@
 //   k_physbc           multifab_physbc_edgevel (:254; rules of the in-tree
 //                      sample Exec/SCIENCE/flame/multifab_physbc_edgevel.f90:
 //                      211-321)
+#include <algorithm>
+
 #include "mac_launch.h"
 
 namespace mac {
@@ -281,51 +283,232 @@ __global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
     L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
 }
 
-// ------------------------------------------- residual (+ max norm)
-template <int DIM, typename T>
-__global__ void __launch_bounds__(TPB) k_residual(Lev<T> L, T *out, unsigned long long *nrm)
+// ------------------------------------------- fused red-black sweep (3D)
+// One launch performs a whole GSRB sweep (red then black).  A 512-thread
+// block owns an FX x FY output tile of columns in x-y plus a one-column ring
+// and marches through a z-chunk.  At step k it updates red cells of plane k
+// (ring included: their red values are what the tile's black cells need) and
+// black cells of plane k-1, whose red neighbours are then all final.  The
+// sweep reads pin and writes pout, so overlapping tiles never observe each
+// other's writes.  Plane values rotate through three LDS buffers, one
+// barrier per plane.  Results equal the two-pass red/black update exactly.
+//   MODE 0: plain sweep; 1: previous iterate is zero (pin not read);
+//   MODE 2: pin + P e (first sweep after the coarse correction, e coarse).
+namespace fz {
+constexpr int EX = 64, EY = 8, FX = EX - 2, FY = EY - 2;
+constexpr int PX = EX + 2, PY = EY + 2;
+constexpr int NT = EX * EY;
+}
+
+struct ProlongSrc {
+    const void *e;
+    int cnx, cny, cnz;
+    EllBC bc;
+};
+
+__device__ __forceinline__ void wrap_coord(int x, int n, int per, int &xw, bool &ok)
 {
-    long q = (long)blockIdx.x * TPB + threadIdx.x;
-    long n = (long)L.nx * L.ny * L.nz;
-    double a = 0.0;
-    if (q < n) {
-        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
-        T w[6], p[6];
-        stencil<DIM, T>(L, L.phi, i, j, k, q, w, p);
-        T r = L.rhs[q] - apply_value<DIM, T>(L, w, p, L.phi[q]);
-        if (out) out[q] = r;
-        a = fabs((double)r);
+    if (x >= 0 && x < n) { xw = x; ok = true; return; }
+    if (per) { xw = ((x % n) + n) % n; ok = true; return; }
+    xw = 0;
+    ok = false;
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(fz::NT) k_gsrb_fused(Lev<T> L, const T *__restrict__ pin,
+                                                       T *__restrict__ pout, FusedGrid G, ProlongSrc PS)
+{
+    using namespace fz;
+    const int b = blockIdx.x;
+    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
+    if (idx >= G.total) return;
+    const int ntile = G.ntx * G.nty;
+    const int chunk = idx / ntile, tile = idx - chunk * ntile;
+    const int tyi = tile / G.ntx, txi = tile - tyi * G.ntx;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long plane = (long)nx * ny;
+    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+    const int tx = threadIdx.x & (EX - 1), ty = threadIdx.x >> 6;
+    const int X = txi * FX - 1 + tx, Y = tyi * FY - 1 + ty;
+    int Xw, Yw;
+    bool okx, oky;
+    wrap_coord(X, nx, L.perx, Xw, okx);
+    wrap_coord(Y, ny, L.pery, Yw, oky);
+    const bool active = okx && oky;
+    const bool outcol = tx >= 1 && tx <= FX && ty >= 1 && ty <= FY && X < nx && Y < ny;
+    const long col = Xw + (long)nx * Yw;
+    const int pos = (tx + 1) + PX * (ty + 1);
+    // ring-2 loads: a second ring of old values, neighbours of ring red cells
+    int r2a = -1, r2b = -1;
+    long c2a = 0, c2b = 0;
+    int x2a = 0, y2a = 0, x2b = 0, y2b = 0;   // ring-2 cell coordinates
+    if (ty == 0 || ty == EY - 1) {
+        int yy = (ty == 0) ? Y - 1 : Y + 1, yw;
+        bool ok;
+        wrap_coord(yy, ny, L.pery, yw, ok);
+        if (ok && okx) { r2a = (tx + 1) + PX * (ty == 0 ? 0 : EY + 1); c2a = Xw + (long)nx * yw; x2a = Xw; y2a = yw; }
     }
-    if (nrm) {
-        a = wave_max(a);
-        __shared__ double sm[TPB / 64];
-        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double m = sm[0];
-            for (int w = 1; w < TPB / 64; ++w) m = fmax(m, sm[w]);
-            atomic_max_nonneg(nrm, m);
+    if (tx == 0 || tx == EX - 1) {
+        int xx = (tx == 0) ? X - 1 : X + 1, xw;
+        bool ok;
+        wrap_coord(xx, nx, L.perx, xw, ok);
+        if (ok && oky) { r2b = (tx == 0 ? 0 : EX + 1) + PX * (ty + 1); c2b = xw + (long)nx * Yw; x2b = xw; y2b = Yw; }
+    }
+    __shared__ T tile_s[3][PX * PY];
+    // the ghost positions (outside non-periodic sides) stay zero
+    for (int q = threadIdx.x; q < 3 * PX * PY; q += NT) (&tile_s[0][0])[q] = T(0);
+
+    // plane index within the domain (periodic image) or false outside a
+    // non-periodic side; k stays within a few planes of [0, nz)
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (L.perz) {
+            while (k < 0) k += nz;
+            while (k >= nz) k -= nz;
+            kw = k;
+            return true;
         }
+        kw = 0;
+        return false;
+    };
+    // old value of cell (column c, xy parity pxy, plane k); black cells get
+    // the coarse correction in MODE 2
+    auto load_old = [&](long c, int pxy, int xw, int yw, int k) -> T {
+        int kw;
+        if (!kwrap(k, kw)) return T(0);
+        if (MODE == 1) return T(0);
+        T v = pin[c + plane * kw];
+        if (MODE == 2 && ((pxy + kw) & 1))
+            v = v + prolong_value<3, T>((const T *)PS.e, PS.cnx, PS.cny, PS.cnz, PS.bc, xw, yw, kw);
+        return v;
+    };
+    const int pxy = Xw + Yw;
+    const int kb = z0 - 1, ke = z1;
+    T pm2 = T(0), pm1 = T(0), p0 = T(0), pp1 = T(0), pp2 = T(0);
+    if (active) {
+        pm1 = load_old(col, pxy, Xw, Yw, kb - 1);
+        p0 = load_old(col, pxy, Xw, Yw, kb);
+        pp1 = load_old(col, pxy, Xw, Yw, kb + 1);
+        pp2 = load_old(col, pxy, Xw, Yw, kb + 2);
+    }
+    // per-plane inputs of this thread: its six face weights, rhs and the
+    // ring-2 values it publishes.  They are fetched one plane ahead (nw..)
+    // so their latency overlaps the previous plane's work.
+    // two planes of look-ahead: n1 (plane k+1) and n2 (plane k+2)
+    T cw[6], crhs = T(0), pw[6], prhs = T(0);
+    T n1w[6], n1rhs = T(0), n1a = T(0), n1b = T(0);
+    T n2w[6], n2rhs = T(0), n2a = T(0), n2b = T(0);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) { cw[s] = T(0); pw[s] = T(0); n1w[s] = T(0); n2w[s] = T(0); }
+    auto fetch = [&](int k, T *nw, T &nrhs, T &nr2a, T &nr2b) {
+        int kw;
+        const bool kok = kwrap(k, kw);
+        if (active && kok) {
+            const long c = col + plane * kw;
+            nw[0] = L.bx[c];
+            nw[1] = (Xw < nx - 1) ? L.bx[c + 1] : L.bxh[Yw + (long)ny * kw];
+            nw[2] = L.by[c];
+            nw[3] = (Yw < ny - 1) ? L.by[c + nx] : L.byh[Xw + (long)nx * kw];
+            nw[4] = L.bz[c];
+            nw[5] = (kw < nz - 1) ? L.bz[c + plane] : L.bzh[col];
+            nrhs = L.rhs[c];
+        }
+        nr2a = (r2a >= 0) ? load_old(c2a, x2a + y2a, x2a, y2a, k) : T(0);
+        nr2b = (r2b >= 0) ? load_old(c2b, x2b + y2b, x2b, y2b, k) : T(0);
+    };
+    fetch(kb, n1w, n1rhs, n1a, n1b);
+    if (kb + 1 <= ke) fetch(kb + 1, n2w, n2rhs, n2a, n2b);
+    __syncthreads();
+    T *buf = tile_s[0], *pbuf = tile_s[2], *qbuf = tile_s[1];   // planes k, k-1, k-2
+    for (int k = kb; k <= ke; ++k) {
+        if (k > kb) {   // rotate: (k-2, k-1, k) <- (k-1, k, k-2)
+            T *t = qbuf;
+            qbuf = pbuf;
+            pbuf = buf;
+            buf = t;
+        }
+        int kw;
+        const bool kok = kwrap(k, kw);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) { pw[s] = cw[s]; cw[s] = n1w[s]; n1w[s] = n2w[s]; }
+        prhs = crhs;
+        crhs = n1rhs;
+        n1rhs = n2rhs;
+        const T r2va = n1a, r2vb = n1b;
+        n1a = n2a;
+        n1b = n2b;
+        if (k + 2 <= ke) fetch(k + 2, n2w, n2rhs, n2a, n2b);
+        // (a) publish the old values of plane k
+        if (active) buf[pos] = kok ? p0 : T(0);
+        if (r2a >= 0) buf[r2a] = r2va;
+        if (r2b >= 0) buf[r2b] = r2vb;
+        __syncthreads();
+        const bool red = kok && ((pxy + kw) & 1) == 0;
+        // (c) red cells of plane k (tile and ring)
+        if (active && red) {
+            T p[6] = {buf[pos - 1], buf[pos + 1], buf[pos - PX], buf[pos + PX], pm1, pp1};
+            T v = gs_value<3, T>(L, cw, p, crhs);
+            buf[pos] = v;
+            p0 = v;
+        }
+        // (d) black cells of plane k-1 (tile only), (e) write plane k-1
+        const int km = k - 1;
+        if (outcol && km >= z0 && km < z1) {
+            if ((pxy + km) & 1) {   // cell (col, k-1) is black
+                T p[6] = {pbuf[pos - 1], pbuf[pos + 1], pbuf[pos - PX], pbuf[pos + PX], pm2, p0};
+                pm1 = gs_value<3, T>(L, pw, p, prhs);
+            }
+            pout[col + plane * km] = pm1;
+        }
+        // rotate the column registers and fetch plane k+3
+        pm2 = pm1;
+        pm1 = p0;
+        p0 = pp1;
+        pp1 = pp2;
+        pp2 = (active && k + 3 <= ke + 1) ? load_old(col, pxy, Xw, Yw, k + 3) : T(0);
     }
 }
 
-// FP64 residual of the refinement loop, written as FP32 right side of the
-// FP32 V-cycle, with its max norm
-template <int DIM>
-__global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
-                                                    unsigned long long *nrm)
+// residual of the fine level restricted onto the coarse level in one pass
+// (the fine residual is never stored); also clears the coarse iterate.
+// Block: 64 coarse cells in x times the 2 (y) x 2 (z) fine offsets.
+template <int DIM, typename T>
+__global__ void __launch_bounds__(256) k_resid_restrict(Lev<T> F, const T *__restrict__ phi, Lev<T> C)
 {
-    long q = (long)blockIdx.x * TPB + threadIdx.x;
-    long n = (long)L.nx * L.ny * L.nz;
-    double a = 0.0;
-    if (q < n) {
-        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
-        double w[6], p[6];
-        stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
-        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, L.phi[q]);
-        r32[q] = (float)r;
-        a = fabs(r);
+    const int tx = threadIdx.x & 63, b = (threadIdx.x >> 6) & 1, c = threadIdx.x >> 7;
+    const int I = blockIdx.x * 64 + tx, J = blockIdx.y, K = blockIdx.z;
+    __shared__ T s[4][64];
+    T pair = T(0);
+    if (I < C.nx && (DIM == 3 || c == 0)) {
+        const int j = 2 * J + b, k = (DIM == 3) ? 2 * K + c : 0;
+        T r[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int i = 2 * I + a;
+            const long f = i + (long)F.nx * (j + (long)F.ny * k);
+            T w[6], p[6];
+            stencil<DIM, T>(F, phi, i, j, k, f, w, p);
+            r[a] = F.rhs[f] - apply_value<DIM, T>(F, w, p, phi[f]);
+        }
+        pair = r[0] + r[1];
     }
+    s[b + 2 * c][tx] = pair;
+    __syncthreads();
+    if (threadIdx.x < 64 && I < C.nx) {
+        const long q = I + (long)C.nx * (J + (long)C.ny * K);
+        if (DIM == 3)
+            C.rhs[q] = T(0.125) * ((s[0][tx] + s[1][tx]) + (s[2][tx] + s[3][tx]));
+        else
+            C.rhs[q] = T(0.25) * (s[0][tx] + s[1][tx]);
+        C.phi[q] = T(0);
+    }
+}
+
+// ------------------------------------------- residual (+ max norm)
+// block max -> one atomic per block (max of non-negative doubles compares
+// like their bit patterns, so the result is order independent)
+__device__ __forceinline__ void block_max_atomic(double a, unsigned long long *nrm)
+{
     a = wave_max(a);
     __shared__ double sm[TPB / 64];
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = a;
@@ -335,6 +518,43 @@ __global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
         for (int w = 1; w < TPB / 64; ++w) m = fmax(m, sm[w]);
         atomic_max_nonneg(nrm, m);
     }
+}
+
+// residual array and/or its max norm; grid-stride so the norm costs at most
+// one atomic per block of a fixed-size grid
+template <int DIM, typename T>
+__global__ void __launch_bounds__(TPB) k_residual(Lev<T> L, T *out, unsigned long long *nrm)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    double a = 0.0;
+    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB) {
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        T w[6], p[6];
+        stencil<DIM, T>(L, L.phi, i, j, k, q, w, p);
+        T r = L.rhs[q] - apply_value<DIM, T>(L, w, p, L.phi[q]);
+        if (out) out[q] = r;
+        a = fmax(a, fabs((double)r));
+    }
+    if (nrm) block_max_atomic(a, nrm);
+}
+
+// FP64 residual of the refinement loop, written as FP32 right side of the
+// FP32 V-cycle, with its max norm
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
+                                                    unsigned long long *nrm)
+{
+    long n = (long)L.nx * L.ny * L.nz;
+    double a = 0.0;
+    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB) {
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+        double w[6], p[6];
+        stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
+        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, L.phi[q]);
+        r32[q] = (float)r;
+        a = fmax(a, fabs(r));
+    }
+    block_max_atomic(a, nrm);
 }
 
 __global__ void __launch_bounds__(TPB) k_phi_add_f32(double *phi, const float *e, long n)
@@ -1048,17 +1268,69 @@ void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
 }
 
 template <typename T>
+void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
+                       const EllBC &bc, hipStream_t s)
+{
+    FusedGrid G;
+    G.ntx = (L.nx + fz::FX - 1) / fz::FX;
+    G.nty = (L.ny + fz::FY - 1) / fz::FY;
+    const int tiles = G.ntx * G.nty;
+    int zc = (2048 + tiles - 1) / tiles;            // aim at >= 2048 blocks
+    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
+    G.tz = (L.nz + zc - 1) / zc;
+    zc = (L.nz + G.tz - 1) / G.tz;
+    G.total = tiles * zc;
+    G.per_xcd = (G.total + 7) / 8;
+    ProlongSrc PS;
+    PS.e = C ? (const void *)C->phi : nullptr;
+    PS.cnx = C ? C->nx : 1;
+    PS.cny = C ? C->ny : 1;
+    PS.cnz = C ? C->nz : 1;
+    PS.bc = bc;
+    dim3 g(8 * G.per_xcd), b(fz::NT);
+    if (mode == SWEEP_ZERO)
+        hipLaunchKernelGGL((k_gsrb_fused<T, 1>), g, b, 0, s, L, pin, pout, G, PS);
+    else if (mode == SWEEP_PROLONG)
+        hipLaunchKernelGGL((k_gsrb_fused<T, 2>), g, b, 0, s, L, pin, pout, G, PS);
+    else
+        hipLaunchKernelGGL((k_gsrb_fused<T, 0>), g, b, 0, s, L, pin, pout, G, PS);
+}
+
+template <typename T>
+void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s)
+{
+    dim3 g((C.nx + 63) / 64, C.ny, C.nz);
+    if (dim == 3)
+        hipLaunchKernelGGL((k_resid_restrict<3, T>), g, dim3(256), 0, s, F, phi, C);
+    else
+        hipLaunchKernelGGL((k_resid_restrict<2, T>), g, dim3(128), 0, s, F, phi, C);
+}
+
+template void launch_gsrb_fused<double>(const Lev<double> &, const double *, double *, int,
+                                        const Lev<double> *, const EllBC &, hipStream_t);
+template void launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int,
+                                       const Lev<float> *, const EllBC &, hipStream_t);
+template void launch_resid_restrict<double>(int, const Lev<double> &, const double *,
+                                            const Lev<double> &, hipStream_t);
+template void launch_resid_restrict<float>(int, const Lev<float> &, const float *,
+                                           const Lev<float> &, hipStream_t);
+
+static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
+
+template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
 {
     long n = (long)L.nx * L.ny * L.nz;
-    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_residual<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, L, out, nrm));
+    unsigned g = nrm ? std::min(nblk(n), NORM_GRID) : nblk(n);
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_residual<D_, T>), dim3(g), dim3(TPB), 0, s, L, out, nrm));
 }
 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s)
 {
     long n = (long)L.nx * L.ny * L.nz;
-    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(nblk(n)), dim3(TPB), 0, s, L, r32, nrm));
+    unsigned g = std::min(nblk(n), NORM_GRID);
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
 
 void launch_phi_add_f32(double *phi, const float *e, long n, hipStream_t s)

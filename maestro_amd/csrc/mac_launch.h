@@ -44,6 +44,13 @@ struct CorrectArgs {
     long maxplane;
 };
 
+// tiling of the fused 3D sweep (k_gsrb_fused)
+struct FusedGrid {
+    int ntx, nty, tz, total, per_xcd;
+};
+
+enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2 };
+
 struct CoarseOpts {
     int nu1, nu2, nub;
     double bottom_eps;
@@ -78,6 +85,14 @@ template <typename T> long launch_partial_sum(const T *x, long n, double *part, 
 template <typename T> void launch_coarsen(int dim, const Lev<T> &F, const Lev<T> &C, hipStream_t s);
 void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, hipStream_t s);
 template <typename T> void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s);
+// one full red-black sweep of a 3D level, pin -> pout; mode SWEEP_*; for
+// SWEEP_PROLONG the coarse level C supplies the correction
+template <typename T>
+void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
+                       const EllBC &bc, hipStream_t s);
+// fine residual (of phi) restricted to C.rhs; C.phi cleared
+template <typename T>
+void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s);
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,

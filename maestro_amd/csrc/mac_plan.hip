@@ -18,6 +18,7 @@
 
 using namespace mac;
 
+#define HIPCHK_V(x) (void)(x)
 #define HIPCHK(x)                                                                     \
     do {                                                                              \
         hipError_t e_ = (x);                                                          \
@@ -207,6 +208,8 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         int rc;
         if ((rc = alloc_dev(H, &L.phi, nc))) return rc;
         L.phi2 = nullptr;
+        if (P->dim == 3)
+            if ((rc = alloc_dev(H, &L.phi2, nc))) return rc;
         if ((rc = alloc_dev(H, &L.rhs, nc))) return rc;
         if ((rc = alloc_dev(H, &L.res, nc))) return rc;
         if ((rc = alloc_dev(H, &L.bx, nc))) return rc;
@@ -269,19 +272,97 @@ static void smooth(mac_plan *P, const Lev<T> &L, int level, int sweeps)
         }
 }
 
+static bool use_fused(const mac_plan *P)
+{
+    static int off = -1;
+    if (off < 0) off = getenv("MAC_NO_FUSED") ? 1 : 0;
+    return P->dim == 3 && !off;
+}
+
+// nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
+// (cur tracks which holds it).  first_mode applies to the first sweep.
 template <typename T>
-static void vcycle(mac_plan *P, Hier<T> &H)
+static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur)
+{
+    const int cls = sizeof(T) == 8 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32;
+    Lev<T> &L = H.lev[l];
+    static int fuse_prolong = -1;
+    if (fuse_prolong < 0) fuse_prolong = getenv("MAC_FUSED_PROLONG") ? 1 : 0;
+    if (first_mode == SWEEP_PROLONG && !fuse_prolong) {
+        // the interpolation inside the sweep stalls its plane pipeline; a
+        // separate streaming pass (phi += P e, 17 B/cell) is cheaper
+        Lev<T> Lc = L;
+        Lc.phi = cur ? L.phi2 : L.phi;
+        launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
+        first_mode = SWEEP_PLAIN;
+    }
+    for (int it = 0; it < nsweep; ++it) {
+        T *pin = cur ? L.phi2 : L.phi;
+        T *pout = cur ? L.phi : L.phi2;
+        int mode = it == 0 ? first_mode : SWEEP_PLAIN;
+        int q = (l == 0) ? prof_begin(P, cls) : -1;
+        launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
+                             P->bc, P->stream);
+        prof_end(P, q);
+        cur ^= 1;
+    }
+}
+
+// zero_first: the level-0 iterate is known to be zero (refinement step)
+template <typename T>
+static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
 {
     hipStream_t s = P->stream;
     const int dim = P->dim;
     const mac_opts &o = P->opts;
+    if (use_fused(P)) {
+        std::vector<int> cur(H.nlev, 0);
+        for (int l = 0; l < H.lc; ++l) {
+            const bool zero = (l > 0) || zero_first;
+            if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur[l]);
+            else if (zero) launch_fill<T>(H.lev[l].phi, (long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz, T(0), s);
+            const Lev<T> &L = H.lev[l];
+            launch_resid_restrict<T>(dim, L, cur[l] ? L.phi2 : L.phi, H.lev[l + 1], s);
+        }
+        {
+            CoarseOpts co;
+            co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
+            co.bottom_eps = o.bottom_eps;
+            co.bottom_maxit = o.bottom_max_iter;
+            co.singular = P->singular;
+            co.bc = P->bc;
+            if (H.lc == 0 && zero_first)
+                launch_fill<T>(H.lev[0].phi, (long)H.lev[0].nx * H.lev[0].ny * H.lev[0].nz, T(0), s);
+            int q = prof_begin(P, MAC_PROF_COARSE);
+            launch_coarse_cycle<T>(dim, H.lev_dev, H.lc, H.nlev, co, P->cg, s);
+            prof_end(P, q);
+        }
+        for (int l = H.lc - 1; l >= 0; --l) {
+            Lev<T> &L = H.lev[l];
+            if (o.nu2 > 0) {
+                fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur[l]);
+            } else {
+                Lev<T> Lc = L;
+                Lc.phi = cur[l] ? L.phi2 : L.phi;
+                launch_prolong<T>(dim, Lc, H.lev[l + 1], P->bc, s);
+            }
+            if (cur[l]) {   // bring the iterate back to phi
+                HIPCHK_V(hipMemcpyAsync(L.phi, L.phi2, sizeof(T) * (size_t)L.nx * L.ny * L.nz,
+                                        hipMemcpyDeviceToDevice, s));
+                cur[l] = 0;
+            }
+        }
+        return;
+    }
     for (int l = 0; l < H.lc; ++l) {
         const Lev<T> &L = H.lev[l];
-        if (l > 0) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
+        if (l > 0 || zero_first) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
         smooth<T>(P, L, l, o.nu1);
         launch_residual<T>(dim, L, L.res, nullptr, s);
         launch_restrict<T>(dim, L.res, L.nx, L.ny, H.lev[l + 1], s);
     }
+    if (H.lc == 0 && zero_first)
+        launch_fill<T>(H.lev[0].phi, (long)H.lev[0].nx * H.lev[0].ny * H.lev[0].nz, T(0), s);
     {
         const Lev<T> &L = H.lev[H.lc];
         if (H.lc > 0) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
@@ -357,9 +438,8 @@ static int solve(mac_plan *P, mac_stats *st)
         const Lev<float> &E = P->H32.lev[0];
         long n0 = (long)E.nx * E.ny * E.nz;
         while (!converged(o, r, r0) && it < o.max_iter) {
-            launch_fill<float>(E.phi, n0, 0.0f, P->stream);
             for (int c = 0; c < std::max(1, o.inner_cycles); ++c) {
-                vcycle<float>(P, P->H32);
+                vcycle<float>(P, P->H32, c == 0);
                 ++inner;
             }
             launch_phi_add_f32(P->H64.lev[0].phi, E.phi, n0, P->stream);
@@ -436,6 +516,8 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
             }
             if ((P->bc.k[d][0] == ELL_PER) != (P->bc.k[d][1] == ELL_PER)) { delete P; return MAC_ERR_INVALID; }
             if (P->n[d] < 2 || !(P->dx[d] > 0)) { delete P; return MAC_ERR_LAYOUT; }
+            // red-black colouring needs an even period across a periodic seam
+            if (P->bc.k[d][0] == ELL_PER && (P->n[d] & 1)) { delete P; return MAC_ERR_LAYOUT; }
         } else {
             P->dlo[d] = 0; P->n[d] = 1; P->dx[d] = 1.0;
             P->phys[d][0] = P->phys[d][1] = MAC_PERIODIC;
@@ -1034,11 +1116,26 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
     const Lev<double> &L = H.lev[level];
     hipStream_t s = P->stream;
     bool has_next = level + 1 < H.nlev;
+    if (op == MAC_OP_GSRB && use_fused(P)) {   // sweeps alternate phi <-> phi2
+        for (int c = 0; c < count; ++c)
+            launch_gsrb_fused<double>(L, (c & 1) ? L.phi2 : L.phi, (c & 1) ? L.phi : L.phi2,
+                                      SWEEP_PLAIN, nullptr, P->bc, s);
+        if (count & 1)
+            HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
+                                  hipMemcpyDeviceToDevice, s));
+        count = 0;
+    }
     for (int c = 0; c < count; ++c) {
         switch (op) {
         case MAC_OP_GSRB:
-            launch_gsrb<double>(P->dim, L, 0, s);
-            launch_gsrb<double>(P->dim, L, 1, s);
+            if (use_fused(P)) {
+                launch_gsrb_fused<double>(L, L.phi, L.phi2, SWEEP_PLAIN, nullptr, P->bc, s);
+                HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
+                                      hipMemcpyDeviceToDevice, s));
+            } else {
+                launch_gsrb<double>(P->dim, L, 0, s);
+                launch_gsrb<double>(P->dim, L, 1, s);
+            }
             break;
         case MAC_OP_RESIDUAL: launch_residual<double>(P->dim, L, L.res, nullptr, s); break;
         case MAC_OP_RESTRICT:
@@ -1056,6 +1153,23 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         case MAC_OP_VCYCLE:
             if (level != 0) return MAC_ERR_INVALID;
             vcycle<double>(P, H);
+            break;
+        case MAC_OP_RESID_RESTRICT:
+            if (!has_next) return MAC_ERR_INVALID;
+            launch_resid_restrict<double>(P->dim, L, L.phi, H.lev[level + 1], s);
+            break;
+        case MAC_OP_PROLONG_SWEEP:
+            if (!has_next) return MAC_ERR_INVALID;
+            if (use_fused(P)) {
+                int cur = 0;
+                fused_sweeps<double>(P, H, level, 1, SWEEP_PROLONG, cur);
+                HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
+                                      hipMemcpyDeviceToDevice, s));
+            } else {
+                launch_prolong<double>(P->dim, L, H.lev[level + 1], P->bc, s);
+                launch_gsrb<double>(P->dim, L, 0, s);
+                launch_gsrb<double>(P->dim, L, 1, s);
+            }
             break;
         default: return MAC_ERR_INVALID;
         }

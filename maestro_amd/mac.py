@@ -36,7 +36,8 @@ SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_
            "mac_plan_profile", "mac_plan_profile_read")
 PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5}
 ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
-OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5}
+OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5,
+      "resid_restrict": 6, "prolong_sweep": 7}
 
 
 class MacError(RuntimeError):
@@ -92,7 +93,7 @@ def lib():
         L.mac_project_level.argtypes = [vp] * 9 + [C.POINTER(CStats)]
         L.mac_adjoint_level.argtypes = [vp] * 17 + [C.POINTER(CStats)]
         L.mac_version.restype = C.c_char_p
-        L.mac_debug_level_dims.argtypes = [vp, C.POINTER(C.c_int)]
+        L.mac_debug_level_dims.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.mac_debug_get.argtypes = [vp, ip, ip, vp]
         L.mac_debug_set.argtypes = [vp, ip, ip, vp]
         L.mac_debug_op.argtypes = [vp, ip, ip, ip]

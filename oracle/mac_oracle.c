@@ -626,6 +626,25 @@ static void vcycle(orc_mg *mg, int l)
     for (int s = 0; s < mg->opts.nu2; ++s) orc_gsrb(g, L);
 }
 
+/* single operators on caller-built levels (used by the kernel tests) */
+void orc_level_restrict(const orc_grid *g, const orc_level *f, orc_level *c)
+{
+    restrict_defect(g->dim, f, c);
+}
+
+void orc_level_prolong(const orc_grid *g, const orc_level *c, orc_level *f)
+{
+    prolong_add(g, c, f);
+}
+
+void orc_level_coarsen(const orc_grid *g, const orc_level *f, orc_level *c)
+{
+    average_coeffs(g->dim, f, c);
+}
+
+/* one V-cycle on the hierarchy, level 0 holding phi/rhs already */
+void orc_mg_vcycle(orc_mg *mg) { vcycle(mg, 0); }
+
 /* MG solve of A phi = rh on the finest level, phi in/out (warm start).
  * Stops when max|rh - A phi| <= eps_rel * max|rh - A phi0| (the
  * "error/error0" test printed by F90mg, see the reference run log

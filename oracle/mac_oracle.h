@@ -84,6 +84,11 @@ double orc_residual(const orc_grid *g, const orc_level *lev, const double *phi,
                     const double *rhs, double *res);
 void orc_gsrb(const orc_grid *g, orc_level *lev);
 
+void orc_level_restrict(const orc_grid *g, const orc_level *f, orc_level *c);
+void orc_level_prolong(const orc_grid *g, const orc_level *c, orc_level *f);
+void orc_level_coarsen(const orc_grid *g, const orc_level *f, orc_level *c);
+void orc_mg_vcycle(orc_mg *mg);
+
 int orc_mg_nlevels(const orc_grid *g, int max_levels);
 orc_mg *orc_mg_create(const orc_grid *g, const double *bx, const double *by,
                       const double *bz, const orc_opts *o);

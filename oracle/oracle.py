@@ -88,6 +88,9 @@ def lib():
         L.orc_mg_solve.restype = C.c_int
         L.orc_mg_solve.argtypes = [C.c_void_p, dp, dp, C.POINTER(Stats)]
         L.orc_mkumac.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp]
+        for f in ("orc_level_restrict", "orc_level_prolong", "orc_level_coarsen"):
+            getattr(L, f).argtypes = [C.POINTER(Grid), C.POINTER(Level), C.POINTER(Level)]
+        L.orc_mg_vcycle.argtypes = [C.c_void_p]
         L.orc_macproject.restype = C.c_int
         L.orc_macproject.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp, dp,
                                      dp, dp, dp, C.POINTER(Opts), C.POINTER(Stats),
@@ -163,23 +166,39 @@ def scale_faces_1d(g, faces, beta0, beta0_edge, do_mult):
 
 
 class LevelView:
-    """Keeps numpy buffers alive behind an oracle ``orc_level``."""
+    """Keeps numpy buffers alive behind an oracle ``orc_level`` (extents
+    ``n`` and spacing ``h`` default to the grid's)."""
 
-    def __init__(self, g, b, phi, rhs, h=None):
+    def __init__(self, g, b, phi, rhs, h=None, n=None, res=None):
         self.g = g
         self.b = [np.ascontiguousarray(x, dtype=np.float64) for x in b]
         self.phi = np.ascontiguousarray(phi, dtype=np.float64)
         self.rhs = np.ascontiguousarray(rhs, dtype=np.float64)
-        self.res = np.zeros_like(self.phi)
+        self.res = np.zeros_like(self.phi) if res is None else np.ascontiguousarray(res, dtype=np.float64)
         L = Level()
         for d in range(3):
-            L.n[d] = g.n[d]
+            L.n[d] = (n[d] if n is not None else g.n[d])
             L.h[d] = (h[d] if h is not None else g.dx[d]) if d < g.dim else 1.0
             L.b[d] = _p(self.b[d]) if d < g.dim else C.POINTER(C.c_double)()
         L.phi = _p(self.phi)
         L.rhs = _p(self.rhs)
         L.res = _p(self.res)
         self.L = L
+
+
+def level_restrict(g, fine: LevelView, coarse: LevelView):
+    """coarse.rhs = conservative average of fine.res"""
+    lib().orc_level_restrict(C.byref(g), C.byref(fine.L), C.byref(coarse.L))
+
+
+def level_prolong(g, coarse: LevelView, fine: LevelView):
+    """fine.phi += P coarse.phi"""
+    lib().orc_level_prolong(C.byref(g), C.byref(coarse.L), C.byref(fine.L))
+
+
+def level_coarsen(g, fine: LevelView, coarse: LevelView):
+    """coarse.b = average of the fine faces covering each coarse face"""
+    lib().orc_level_coarsen(C.byref(g), C.byref(fine.L), C.byref(coarse.L))
 
 
 def apply(g, b, phi):

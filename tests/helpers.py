@@ -18,7 +18,7 @@ def layout_for(wl, **ng):
     return Layout.uniform(wl["n"], wl["box"], wl["dx"], wl["bc"], **ng)
 
 
-def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, **kw):
+def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, check=True, **kw):
     """Run macproject through the C ABI on per-box arrays; returns the
     whole-domain projected faces, phi and the stats."""
     from maestro_amd import mac
@@ -35,14 +35,14 @@ def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, **kw):
         dc = None if cart is None else [[cv(a) for a in box] for box in cart]
         b0 = None if wl["beta0"] is None else cv(np.ascontiguousarray(wl["beta0"]))
         b0e = None if wl["beta0_edge"] is None else cv(np.ascontiguousarray(wl["beta0_edge"]))
-        st = p.project(du, dp, dr, drh, b0, b0e, dc)
+        st = p.project(du, dp, dr, drh, b0, b0e, dc, check=check)
         umac = [[a.cpu().numpy() for a in box] for box in du]
         phi = [a.cpu().numpy() for a in dp]
     else:
         st = p.project(umac, phi, rho, rhs,
                        None if wl["beta0"] is None else np.ascontiguousarray(wl["beta0"]),
                        None if wl["beta0_edge"] is None else np.ascontiguousarray(wl["beta0_edge"]),
-                       cart)
+                       cart, check=check)
     dim, n = wl["dim"], wl["n"]
     nn = list(n) + [1] * (3 - dim)
     faces = [BX.gather_faces([u[d] for u in umac], boxes, d, ng_u, nn, dim) for d in range(dim)]

@@ -1,0 +1,91 @@
+"""CPU checks of the C ABI: libmaestro_mac.so loads, exports every function
+include/maestro_mac.h declares, and validates layouts and options before any
+device work (so these run without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from maestro_amd import mac
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "maestro_mac.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mac_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_function():
+    L = mac.lib()
+    names = header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(mac.SYMBOLS)
+    assert L.mac_version().startswith(b"maestro_mac")
+
+
+def test_default_opts_match_reference_settings():
+    o = mac.default_opts()
+    assert o.eps_rel == 1e-10          # eps_mac, Source/mg_eps.f90:22
+    assert o.eps_abs == -1.0           # Source/macproject.f90:209
+    assert o.bottom_eps == 1e-3        # Source/mac_multigrid.f90:61
+    assert o.nub == 10                 # mg_bottom_nu, Source/_parameters:392
+    assert (o.nu1, o.nu2) == (2, 2)
+    assert o.use_alt_energy_fix == 1   # Source/_parameters:552-554
+    assert o.precision == mac.PREC_FP64
+
+
+def _create(layout):
+    c = layout.to_c()
+    h = C.c_void_p()
+    rc = mac.lib().mac_plan_create(C.byref(c), C.byref(mac.default_opts()), C.byref(h))
+    return rc, h
+
+
+P, W = mac.PERIODIC, mac.SLIP_WALL
+
+
+@pytest.mark.parametrize("boxes,code", [
+    ([((0, 0, 0), (7, 7, 7))], 2),                             # does not cover the domain
+    ([((0, 0, 0), (15, 15, 15)), ((0, 0, 0), (7, 7, 7))], 2),   # overlap
+    ([((0, 0, 0), (16, 15, 15))], 2),                          # outside the domain
+])
+def test_bad_layouts_rejected(boxes, code):
+    lay = mac.Layout(dim=3, boxes=boxes, domain_lo=(0, 0, 0), domain_hi=(15, 15, 15),
+                     dx=(1 / 16,) * 3, phys_bc=[[P, P], [P, P], [W, W]])
+    rc, h = _create(lay)
+    assert rc == code and not h.value
+
+
+def test_bad_bc_and_ghosts_rejected():
+    box = [((0, 0, 0), (15, 15, 15))]
+    lay = mac.Layout(dim=3, boxes=box, domain_lo=(0, 0, 0), domain_hi=(15, 15, 15),
+                     dx=(1 / 16,) * 3, phys_bc=[[P, W], [P, P], [W, W]])   # half-periodic
+    assert _create(lay)[0] == 1      # a side periodic only on one end
+    lay = mac.Layout(dim=3, boxes=box, domain_lo=(0, 0, 0), domain_hi=(15, 15, 15),
+                     dx=(1 / 16,) * 3, phys_bc=[[P, P], [P, P], [W, W]], ng_rho=0)
+    assert _create(lay)[0] == 2      # rho needs a ghost layer for the boundary faces
+    odd = [((0, 0, 0), (14, 15, 15))]
+    lay = mac.Layout(dim=3, boxes=odd, domain_lo=(0, 0, 0), domain_hi=(14, 15, 15),
+                     dx=(1 / 15,) * 3, phys_bc=[[P, P], [P, P], [W, W]])
+    assert _create(lay)[0] == 2      # odd periodic extent breaks red-black colouring
+
+
+def test_dim_validation():
+    lay = mac.Layout(dim=1, boxes=[((0,), (7,))], domain_lo=(0,), domain_hi=(7,), dx=(0.1,),
+                     phys_bc=[[P, P]])
+    assert _create(lay)[0] == 1
+
+
+def test_python_mirror_raises_on_invalid_combination():
+    """macproject with both beta0 forms is an error, as in the reference
+    (Source/macproject.f90:99-101)."""
+    lay = mac.Layout.uniform((16, 16, 16), 8, (1 / 16,) * 3, [[P, P], [P, P], [W, W]])
+    with pytest.raises(mac.MacError) as e:
+        mac.macproject(lay, [], [], [], beta0_1d=object(), beta0_cart_edge=object())
+    assert e.value.code == 1

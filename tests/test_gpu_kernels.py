@@ -1,0 +1,243 @@
+"""Kernel-level parity: each solver operator of the HIP library, driven
+through the diagnostic C entry points, against the oracle on identical
+inputs.  The stencils evaluate in the same order with explicit fma on both
+sides, so these comparisons are bitwise unless stated otherwise."""
+import numpy as np
+import pytest
+
+from maestro_amd import mac
+from maestro_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+P, WALL, OUT = W.PERIODIC, W.SLIP_WALL, W.OUTLET
+
+BCS_3D = [
+    [[P, P], [P, P], [WALL, WALL]],
+    [[OUT, OUT], [OUT, OUT], [OUT, OUT]],
+    [[P, P], [WALL, OUT], [OUT, WALL]],
+    [[WALL, WALL], [WALL, WALL], [WALL, WALL]],
+]
+SIZES_3D = [(32, 16, 16), (64, 48, 40), (128, 12, 8)]
+
+
+def ell(phys):
+    return -1 if phys == P else (1 if phys == OUT else 2)
+
+
+def fold(bc, n, b_raw, dim):
+    """Oracle raw face coefficients -> the library's folded low-face arrays
+    and high-face slabs (Neumann 0, Dirichlet 2 beta)."""
+    out = {}
+    names = ["bx", "by", "bz"]
+    for d in range(dim):
+        ax = 2 - d
+        raw = b_raw[d]
+        lo = np.take(raw, np.arange(n[d]), axis=ax).copy()
+        hi = np.take(raw, [n[d]], axis=ax).copy()
+        k0, k1 = ell(bc[d][0]), ell(bc[d][1])
+        sl = [slice(None)] * 3
+        sl[ax] = 0
+        if k0 == 2:
+            lo[tuple(sl)] = 0.0
+        elif k0 == 1:
+            lo[tuple(sl)] *= 2.0
+        if k1 == 2:
+            hi[...] = 0.0
+        elif k1 == 1:
+            hi *= 2.0
+        out[names[d]] = lo.ravel()
+        out[names[d] + "h"] = np.squeeze(hi, axis=ax).ravel()
+    return out
+
+
+def make_plan(n, bc, dx=None, **opts):
+    dim = len(n)
+    dx = dx or [1.0 / n[0]] * dim
+    lay = mac.Layout(dim=dim, boxes=[((0,) * dim, tuple(x - 1 for x in n))],
+                     domain_lo=(0,) * dim, domain_hi=tuple(x - 1 for x in n), dx=tuple(dx),
+                     phys_bc=bc)
+    return mac.MacPlan(lay, mac.default_opts(**opts))
+
+
+def rand_level(g, n, dim, rng, O):
+    b = [rng.uniform(0.5, 2.0, size=O.face_shape(g, d)) for d in range(dim)]
+    phi = rng.standard_normal(O.cell_shape(g))
+    rhs = rng.standard_normal(O.cell_shape(g))
+    return b, phi, rhs
+
+
+def push_level(plan, level, bc, n, b, phi, rhs, dim):
+    for k, v in fold(bc, n, b, dim).items():
+        plan.debug_set(level, k, v)
+    plan.debug_set(level, "phi", phi.ravel())
+    plan.debug_set(level, "rhs", rhs.ravel())
+
+
+@pytest.mark.parametrize("n", SIZES_3D)
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_gsrb_sweep_bitwise(gpu, oracle_lib, n, bc):
+    O = oracle_lib
+    rng = np.random.default_rng(7)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_op(0, "gsrb", 2)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    ref = O.gsrb(g, b, phi, rhs, sweeps=2)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL]], [[OUT, WALL], [P, P]]])
+def test_gsrb_sweep_bitwise_2d(gpu, oracle_lib, bc):
+    O = oracle_lib
+    n = (48, 32)
+    rng = np.random.default_rng(3)
+    g = O.make_grid(n, [1.0 / n[0]] * 2, bc)
+    b, phi, rhs = rand_level(g, n, 2, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 2)
+    plan.debug_op(0, "gsrb", 1)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    ref = O.gsrb(g, b, phi, rhs, sweeps=1)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40)])
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_residual_restrict_bitwise(gpu, oracle_lib, n, bc):
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_set(1, "phi", np.ones(int(np.prod([x // 2 for x in n]))))
+    plan.debug_op(0, "resid_restrict")
+    got = plan.debug_get(1, "rhs")
+    res, _ = O.residual(g, b, phi, rhs)
+    cn = [x // 2 for x in n]
+    fine = O.LevelView(g, b, phi, rhs, res=res)
+    coarse = O.LevelView(g, [np.zeros(1)] * 3, np.zeros(cn[::-1]), np.zeros(cn[::-1]), n=cn)
+    O.level_restrict(g, fine, coarse)
+    assert np.array_equal(got, coarse.rhs.ravel())
+    assert not plan.debug_get(1, "phi").any()
+
+
+@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40)])
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_prolong_then_sweep_bitwise(gpu, oracle_lib, n, bc):
+    O = oracle_lib
+    rng = np.random.default_rng(5)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    cn = [x // 2 for x in n]
+    e = rng.standard_normal(cn[::-1])
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_set(1, "phi", e.ravel())
+    plan.debug_op(0, "prolong_sweep")
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    fine = O.LevelView(g, b, phi.copy(), rhs)
+    coarse = O.LevelView(g, [np.zeros(1)] * 3, e, np.zeros_like(e), n=cn)
+    O.level_prolong(g, coarse, fine)
+    ref = O.gsrb(g, b, fine.phi, rhs, sweeps=1)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_coarsen_bitwise(gpu, oracle_lib, bc):
+    O = oracle_lib
+    n = (32, 16, 16)
+    rng = np.random.default_rng(2)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_op(0, "coarsen")
+    cn = [x // 2 for x in n]
+    gc = O.make_grid(cn, [2.0 / n[0]] * 3, bc)
+    fine = O.LevelView(g, b, phi, rhs)
+    cb = [np.zeros(O.face_shape(gc, d)) for d in range(3)]
+    coarse = O.LevelView(gc, cb, np.zeros(cn[::-1]), np.zeros(cn[::-1]))
+    O.level_coarsen(g, fine, coarse)
+    want = fold(bc, cn, coarse.b, 3)
+    for k, v in want.items():
+        assert np.array_equal(plan.debug_get(1, k), v), k
+
+
+@pytest.mark.parametrize("bc", [BCS_3D[1], BCS_3D[2]])
+def test_vcycle_matches_oracle(gpu, oracle_lib, bc):
+    """One full V-cycle (non-singular BCs): equal up to the bottom solver's
+    reduction order."""
+    O = oracle_lib
+    n = (64, 32, 32)
+    rng = np.random.default_rng(9)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc, bottom_eps=1e-15, bottom_max_iter=200)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    nlev = 0
+    while True:
+        try:
+            plan.level_dims(nlev + 1)
+            plan.debug_op(nlev, "coarsen")
+            nlev += 1
+        except mac.MacError:
+            break
+    plan.debug_op(0, "vcycle")
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    ref, _, st, _ = O.mg_solve(g, b, phi, rhs, O.default_opts(max_iter=1, eps_rel=1e-300,
+                                                                   bottom_eps=1e-15, bottom_maxit=200))
+    assert st.iters == 1
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+
+
+@pytest.mark.parametrize("cfg,n,box,alt", [(1, (32, 32, 32), 16, 1), (3, (32, 32, 32), 16, 1),
+                                           (0, (64, 64), 16, 0)])
+def test_setup_and_correction_match_oracle(gpu, oracle_lib, cfg, n, box, alt):
+    """macproject with no V-cycle: the setup kernel's coefficients and rh and
+    the face correction with the warm-start phi, against the oracle."""
+    O = oracle_lib
+    from .helpers import run_gpu
+    wl = W.make_workload(cfg, n=n, box=box)
+    rng = np.random.default_rng(4)
+    wl["phi"] = rng.standard_normal(wl["phi"].shape)
+    dim = wl["dim"]
+    lay_opts = mac.default_opts(max_iter=0, use_alt_energy_fix=alt)
+    from .helpers import layout_for
+    plan = mac.MacPlan(layout_for(wl), lay_opts)
+    out = run_gpu(wl, plan=plan, check=False)
+    g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
+    b = O.mk_mac_coeffs(g, wl["rho"])
+    faces = [f.copy() for f in wl["faces"]]
+    if wl["beta0"] is not None:
+        b = O.scale_faces_1d(g, b, wl["beta0"], wl["beta0_edge"], 1)
+        if alt:
+            b = O.scale_faces_1d(g, b, wl["beta0"], wl["beta0_edge"], 1)
+        faces = O.scale_faces_1d(g, faces, wl["beta0"], wl["beta0_edge"], 1)
+    else:
+        cart = wl["cart"]
+        b = [b[d] * cart[d] for d in range(dim)]
+        if alt:
+            b = [b[d] * cart[d] for d in range(dim)]
+        faces = [faces[d] * cart[d] for d in range(dim)]
+    nn = list(wl["n"]) + [1] * (3 - dim)
+    for k, v in fold(wl["bc"], nn, b, dim).items():
+        assert np.array_equal(plan.debug_get(0, k), v), k
+    rh = O.divumac(g, *faces)
+    singular = all(x[0] != OUT for x in wl["bc"])
+    if singular:
+        rh = rh - rh.mean()
+    got_rh = plan.debug_get(0, "rhs")
+    assert np.abs(got_rh - rh.ravel()).max() <= 1e-13 * np.abs(rh).max()
+    proj = O.mkumac(g, faces, wl["phi"], b)
+    if wl["beta0"] is not None:
+        proj = O.scale_faces_1d(g, proj, wl["beta0"], wl["beta0_edge"], 0)
+    else:
+        proj = [proj[d] / wl["cart"][d] for d in range(dim)]
+    for d in range(dim):
+        assert np.array_equal(out["faces"][d], proj[d]), d
+    plan.close()

@@ -1,0 +1,298 @@
+"""CPU tests that pin the FP64 oracle: golden fixtures from an independent
+sparse-matrix construction, the reference unit test's known answers
+(Exec/UNIT_TESTS/test_projection), closed forms and invariants."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from maestro_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P, WALL, OUT = W.PERIODIC, W.SLIP_WALL, W.OUTLET
+GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+
+
+def oracle_project(O, wl, alt, macrhs=None, eps=1e-13):
+    g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
+    return O.macproject(g, wl["faces"], wl["phi"], wl["rho"], macrhs=macrhs, beta0=wl["beta0"],
+                        beta0_edge=wl["beta0_edge"], cart=wl["cart"],
+                        opts=O.default_opts(eps_rel=eps, use_alt_energy_fix=alt))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_oracle_matches_golden(oracle_lib, path):
+    O = oracle_lib
+    gold = np.load(path)
+    cfg, n, alt = int(gold["cfg"]), tuple(int(x) for x in gold["n"]), int(gold["alt"])
+    wl = W.make_workload(cfg, n=n, box=n[0] // 2)
+    macrhs = gold["macrhs"] if "macrhs" in gold else None
+    r = oracle_project(O, wl, alt, macrhs)
+    dim = wl["dim"]
+    for d in range(dim):
+        b = gold[f"beta{d}"]
+        assert np.allclose(r["beta"][d], b, rtol=1e-15, atol=0), d
+    singular = all(b[0] != OUT for b in wl["bc"])
+    rh = gold["rh"] - (gold["rh"].mean() if singular else 0.0)
+    assert np.abs(r["rh"] - rh).max() <= 1e-13 * np.abs(rh).max()
+    ph, gp = r["phi"], gold["phi"]
+    if singular:
+        ph, gp = ph - ph.mean(), gp - gp.mean()
+    assert np.linalg.norm(ph - gp) <= 1e-10 * np.linalg.norm(gp)
+    for d in range(dim):
+        ref = gold[f"proj{d}"]
+        assert np.linalg.norm(r["faces"][d] - ref) <= 1e-10 * np.linalg.norm(ref), d
+
+
+def test_coefficient_closed_forms(oracle_lib):
+    O = oracle_lib
+    n = (6, 5, 4)
+    g = O.make_grid(n, (0.1, 0.2, 0.3), [[P, P], [WALL, WALL], [OUT, OUT]])
+    rho = np.full(O.rho_shape(g), 4.0)
+    for b in O.mk_mac_coeffs(g, rho):
+        assert np.all(b == 0.25)     # constant rho: beta = 1/rho
+    rng = np.random.default_rng(0)
+    rho = rng.uniform(1, 2, O.rho_shape(g))
+    bx = O.mk_mac_coeffs(g, rho)[0]
+    # beta depends only on the two cells beside the face, symmetrically
+    flip = O.mk_mac_coeffs(g, rho[:, :, ::-1].copy())[0][:, :, ::-1]
+    assert np.allclose(bx, flip, rtol=0, atol=0)
+
+
+def test_divergence_closed_form(oracle_lib):
+    O = oracle_lib
+    n, dx = (8, 6, 4), (0.5, 0.25, 0.125)
+    g = O.make_grid(n, dx, [[WALL, WALL]] * 3)
+    # u = a x, v = b y, w = c z on faces -> D = a + b + c exactly
+    a, b, c = 3.0, -2.0, 0.5
+    u = a * np.arange(n[0] + 1)[None, None, :] * dx[0] * np.ones((n[2], n[1], n[0] + 1))
+    v = b * np.arange(n[1] + 1)[None, :, None] * dx[1] * np.ones((n[2], n[1] + 1, n[0]))
+    w = c * np.arange(n[2] + 1)[:, None, None] * dx[2] * np.ones((n[2] + 1, n[1], n[0]))
+    rh = O.divumac(g, u, v, w)
+    assert np.allclose(rh, -(a + b + c), rtol=1e-13)
+    m = np.full(O.cell_shape(g), 2.0)
+    assert np.allclose(O.divumac(g, u, v, w, macrhs=m), 2.0 - (a + b + c), rtol=1e-13)
+
+
+def test_periodic_laplacian_eigenvalue(oracle_lib):
+    """beta = 1, periodic: cos modes are eigenvectors with
+    lambda = sum_d (2 - 2 cos(2 pi m_d / n_d)) / h_d^2."""
+    O = oracle_lib
+    n, h = (16, 8, 12), (0.1, 0.3, 0.2)
+    g = O.make_grid(n, h, [[P, P]] * 3)
+    m = (3, 1, 2)
+    x = [np.arange(n[d]) for d in range(3)]
+    phi = (np.cos(2 * np.pi * m[2] * x[2] / n[2])[:, None, None] *
+           np.cos(2 * np.pi * m[1] * x[1] / n[1])[None, :, None] *
+           np.cos(2 * np.pi * m[0] * x[0] / n[0])[None, None, :])
+    b = [np.ones(O.face_shape(g, d)) for d in range(3)]
+    lam = sum((2 - 2 * np.cos(2 * np.pi * m[d] / n[d])) / h[d] ** 2 for d in range(3))
+    assert np.allclose(O.apply(g, b, phi), lam * phi, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL], [OUT, OUT]],
+                                [[OUT, WALL], [P, P], [WALL, OUT]],
+                                [[WALL, WALL]] * 3])
+def test_operator_symmetric_and_null_space(oracle_lib, bc):
+    O = oracle_lib
+    n = (8, 6, 4)
+    g = O.make_grid(n, (0.1, 0.2, 0.15), bc)
+    rng = np.random.default_rng(1)
+    b = [rng.uniform(0.5, 2, O.face_shape(g, d)) for d in range(3)]
+    for d in range(3):   # one value per physical face: periodic duplicates equal
+        if bc[d][0] == P:
+            ax = 2 - d
+            sl_lo, sl_hi = [slice(None)] * 3, [slice(None)] * 3
+            sl_lo[ax], sl_hi[ax] = 0, -1
+            b[d][tuple(sl_hi)] = b[d][tuple(sl_lo)]
+    x, y = rng.standard_normal(O.cell_shape(g)), rng.standard_normal(O.cell_shape(g))
+    Ax, Ay = O.apply(g, b, x), O.apply(g, b, y)
+    assert abs((Ax * y).sum() - (x * Ay).sum()) <= 1e-12 * abs((Ax * y).sum())
+    assert (x * Ax).sum() > 0          # positive (semi)definite
+    if all(s[0] != OUT for s in bc):
+        assert np.abs(O.apply(g, b, np.ones(O.cell_shape(g)))).max() < 1e-10
+
+
+def test_gsrb_fixed_point_and_black_residual(oracle_lib):
+    O = oracle_lib
+    n = (8, 8, 8)
+    g = O.make_grid(n, (0.125,) * 3, [[OUT, OUT]] * 3)
+    rng = np.random.default_rng(2)
+    b = [rng.uniform(0.5, 2, O.face_shape(g, d)) for d in range(3)]
+    rhs = rng.standard_normal(O.cell_shape(g))
+    phi, _, st, rc = O.mg_solve(g, b, np.zeros_like(rhs), rhs, O.default_opts(eps_rel=1e-15))
+    # the exact solution is a fixed point of the smoother
+    assert np.abs(O.gsrb(g, b, phi, rhs) - phi).max() <= 1e-12 * np.abs(phi).max()
+    # after one sweep from any iterate the black (odd) cells have zero residual
+    p1 = O.gsrb(g, b, rng.standard_normal(rhs.shape), rhs)
+    r, _ = O.residual(g, b, p1, rhs)
+    k, j, i = np.indices(rhs.shape)
+    black = ((i + j + k) & 1) == 1
+    assert np.abs(r[black]).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_restriction_conserves_and_prolongation_exact(oracle_lib):
+    O = oracle_lib
+    n = (8, 8, 8)
+    cn = [4, 4, 4]
+    for bc in ([[P, P]] * 3, [[WALL, WALL]] * 3):
+        g = O.make_grid(n, (0.125,) * 3, bc)
+        rng = np.random.default_rng(3)
+        res = rng.standard_normal(O.cell_shape(g))
+        fine = O.LevelView(g, [np.zeros(1)] * 3, np.zeros_like(res), np.zeros_like(res), res=res)
+        coarse = O.LevelView(g, [np.zeros(1)] * 3, np.zeros(cn), np.zeros(cn), n=cn)
+        O.level_restrict(g, fine, coarse)
+        assert abs(8 * coarse.rhs.sum() - res.sum()) < 1e-12
+        # constants are reproduced exactly (periodic and Neumann)
+        coarse.phi[...] = 2.5
+        fine.phi[...] = 0.0
+        O.level_prolong(g, coarse, fine)
+        assert np.all(fine.phi == 2.5)
+    # linear functions are reproduced in the interior (periodic wrap aside)
+    g = O.make_grid(n, (0.125,) * 3, [[WALL, WALL]] * 3)
+    xc = (np.arange(4) + 0.5) * 2
+    xf = np.arange(8) + 0.5
+    coarse = O.LevelView(g, [np.zeros(1)] * 3, np.broadcast_to(xc, (4, 4, 4)).copy(),
+                         np.zeros((4, 4, 4)), n=cn)
+    fine = O.LevelView(g, [np.zeros(1)] * 3, np.zeros((8, 8, 8)), np.zeros((8, 8, 8)))
+    O.level_prolong(g, coarse, fine)
+    assert np.allclose(fine.phi[:, :, 1:-1], np.broadcast_to(xf, (8, 8, 8))[:, :, 1:-1])
+
+
+@pytest.mark.parametrize("cfg,n", [(0, (64, 64)), (1, (32, 32, 32)), (3, (32, 32, 32))])
+def test_vcycle_convergence_rate(oracle_lib, cfg, n):
+    O = oracle_lib
+    wl = W.make_workload(cfg, n=n)
+    r = oracle_project(O, wl, alt=0, eps=1e-10)
+    st = r["stats"]
+    rate = (st.res / st.res0) ** (1.0 / st.iters)
+    assert st.converged and rate < 0.15, rate
+
+
+def test_test_projection_2d_known_answers(oracle_lib):
+    """Exec/UNIT_TESTS/test_projection: the divergence-free field plus the
+    gradient of a scalar projects back onto the field (test_projection.f90:
+    221-261, 714-780)."""
+    O = oracle_lib
+    n = 64
+    dx = 1.0 / n
+    xf = np.arange(n + 1) * dx
+    xc = (np.arange(n) + 0.5) * dx
+    u = (-np.sin(np.pi * xf[None, :]) ** 2 * np.sin(2 * np.pi * xc[:, None]))[None]
+    v = (np.sin(np.pi * xf[:, None]) ** 2 * np.sin(2 * np.pi * xc[None, :]))[None]
+    for bc, kind in (([[P, P], [P, P]], "per"), ([[WALL, WALL], [WALL, WALL]], "wall")):
+        if kind == "per":
+            xg = (np.arange(-1, n + 1) + 0.5) * dx
+            ph = 0.1 * np.cos(2 * np.pi * xg[None, :]) * np.cos(2 * np.pi * xg[:, None])
+            gx = (ph[1:-1, 1:] - ph[1:-1, :-1]) / dx
+            gy = (ph[1:, 1:-1] - ph[:-1, 1:-1]) / dx
+        else:
+            gx = np.broadcast_to(4 * xf * (1 - xf), (n, n + 1)).copy()
+            gy = np.broadcast_to((4 * xf * (1 - xf))[:, None], (n + 1, n)).copy()
+        g = O.make_grid((n, n), (dx, dx), bc)
+        r = O.macproject(g, [u + gx[None], v + gy[None]], np.zeros((1, n, n)),
+                         np.ones((1, n + 2, n + 2)), opts=O.default_opts())
+        assert r["stats"].converged
+        assert max(np.abs(r["faces"][0] - u).max(), np.abs(r["faces"][1] - v).max()) < 1e-9
+
+
+def test_test_projection_3d_second_order(oracle_lib):
+    """3D periodic case (test_projection.f90:264-344, 920-1000): the curl
+    field is divergence-free to O(h^2); the recovered field converges at
+    second order."""
+    O = oracle_lib
+    errs = []
+    for n in (16, 32):
+        dx = 1.0 / n
+        f = np.arange(n + 1) * dx
+        c = (np.arange(n) + 0.5) * dx
+        tp, fp = 2 * np.pi, 4 * np.pi
+        Z, Y, X = np.meshgrid(c, c, f, indexing="ij")
+        u = tp * np.sin(fp * X) * np.cos(tp * Y) - fp * np.sin(tp * X) * np.cos(fp * Z)
+        Z, Y, X = np.meshgrid(c, f, c, indexing="ij")
+        v = tp * np.sin(fp * Y) * np.cos(tp * Z) - fp * np.cos(fp * X) * np.sin(tp * Y)
+        Z, Y, X = np.meshgrid(f, c, c, indexing="ij")
+        w = tp * np.cos(tp * X) * np.sin(fp * Z) - fp * np.cos(fp * Y) * np.sin(tp * Z)
+        cg = (np.arange(-1, n + 1) + 0.5) * dx
+        Z, Y, X = np.meshgrid(cg, cg, cg, indexing="ij")
+        ph = 5.0 * np.cos(tp * Y) * np.cos(tp * X) * np.cos(tp * Z)
+        gx = (ph[1:-1, 1:-1, 1:] - ph[1:-1, 1:-1, :-1]) / dx
+        gy = (ph[1:-1, 1:, 1:-1] - ph[1:-1, :-1, 1:-1]) / dx
+        gz = (ph[1:, 1:-1, 1:-1] - ph[:-1, 1:-1, 1:-1]) / dx
+        g = O.make_grid((n, n, n), (dx,) * 3, [[P, P]] * 3)
+        r = O.macproject(g, [u + gx, v + gy, w + gz], np.zeros((n, n, n)),
+                         np.ones((n + 2,) * 3), opts=O.default_opts())
+        errs.append(max(np.abs(r["faces"][0] - u).max(), np.abs(r["faces"][1] - v).max(),
+                        np.abs(r["faces"][2] - w).max()))
+        # projecting u itself gives the same result (the pollution is a
+        # discrete gradient)
+        r2 = O.macproject(g, [u, v, w], np.zeros((n, n, n)), np.ones((n + 2,) * 3),
+                          opts=O.default_opts())
+        assert np.abs(r2["faces"][0] - r["faces"][0]).max() < 1e-8
+    assert errs[1] < errs[0] / 3.5     # ~4x per halving
+
+
+@pytest.mark.parametrize("cfg,n", [(0, (32, 32)), (1, (16, 16, 16)), (3, (16, 16, 16))])
+def test_projection_invariants(oracle_lib, cfg, n):
+    """D(beta0 U_proj) = macrhs (the constraint), projecting twice changes
+    nothing beyond the solver tolerance, and a constant beta0 leaves U_proj
+    unchanged."""
+    O = oracle_lib
+    wl = W.make_workload(cfg, n=n)
+    g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
+    rng = np.random.default_rng(6)
+    macrhs = rng.standard_normal(O.cell_shape(g))
+    if all(b[0] != OUT for b in wl["bc"]):
+        macrhs -= macrhs.mean()
+    r = oracle_project(O, wl, alt=1, macrhs=macrhs)
+    f = r["faces"]
+    if wl["beta0"] is not None:
+        fs = O.scale_faces_1d(g, f, wl["beta0"], wl["beta0_edge"], 1)
+    else:
+        fs = [f[d] * wl["cart"][d] for d in range(wl["dim"])]
+    div = O.divumac(g, *fs)            # = -D(beta0 U)
+    assert np.abs(-div - macrhs).max() <= 1e-9 * np.abs(macrhs).max() + 1e-9
+    wl2 = dict(wl, faces=f)
+    r2 = oracle_project(O, wl2, alt=1, macrhs=macrhs)
+    for d in range(wl["dim"]):
+        assert np.abs(r2["faces"][d] - f[d]).max() <= 1e-8 * np.abs(f[d]).max()
+    if wl["beta0"] is not None:
+        wl3 = dict(wl, beta0=np.full_like(wl["beta0"], 3.0),
+                   beta0_edge=np.full_like(wl["beta0_edge"], 3.0))
+        wl1 = dict(wl, beta0=np.ones_like(wl["beta0"]), beta0_edge=np.ones_like(wl["beta0_edge"]))
+        a = oracle_project(O, wl3, alt=0)
+        b = oracle_project(O, wl1, alt=0)
+        for d in range(wl["dim"]):
+            assert np.allclose(a["faces"][d], b["faces"][d], rtol=1e-10, atol=1e-12)
+
+
+def test_cell_to_edge_rules():
+    """Source/cell_to_edge.f90:36-61 on a linear profile (the limited
+    fourth-order formula is exact for linear data)."""
+    b0 = 1.0 + 0.5 * np.arange(10)
+    e = W.cell_to_edge(b0)
+    assert e[0] == b0[0] and e[-1] == b0[-1]
+    assert e[1] == 0.5 * (b0[0] + b0[1]) and e[9] == 0.5 * (b0[8] + b0[9])
+    assert np.allclose(e[2:9], 0.5 * (b0[1:8] + b0[2:9]))
+    # limiter: a jump keeps edges within the neighbouring cell values
+    b0 = np.array([1.0, 1, 1, 1, 10, 10, 10, 10])
+    e = W.cell_to_edge(b0)
+    for r in range(1, 8):
+        assert min(b0[r - 1], b0[r]) <= e[r] <= max(b0[r - 1], b0[r])
+
+
+@pytest.mark.parametrize("cfg,n", [(0, (32, 32)), (1, (16, 16, 16)), (3, (16, 16, 16))])
+def test_workload_compatibility(cfg, n):
+    """Inputs obey the solvability rules: one value per physical face, equal
+    periodic duplicates, zero wall-normal velocity."""
+    wl = W.make_workload(cfg, n=n)
+    for d in range(wl["dim"]):
+        f = wl["faces"][d]
+        ax = 2 - d
+        lo = np.take(f, 0, axis=ax)
+        hi = np.take(f, -1, axis=ax)
+        if wl["bc"][d][0] == P:
+            assert np.array_equal(lo, hi)
+        elif wl["bc"][d][0] == WALL:
+            assert not lo.any() and not hi.any()
