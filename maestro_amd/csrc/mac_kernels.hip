@@ -469,6 +469,204 @@ __global__ void __launch_bounds__(fz::NT) k_gsrb_fused(Lev<T> L, const T *__rest
     }
 }
 
+// ------------------------------------- fused red-black sweep, x-pairs (3D)
+// Same algorithm as k_gsrb_fused, but each thread owns two neighbouring
+// columns (an x-pair: one red and one black cell on every plane), so every
+// lane does one red update of plane k and one black update of plane k-1 per
+// step; indexing is 32-bit and incremental.  Block: 32 pair-lanes x EY rows;
+// the tile covers 64 x EY columns and writes the inner 62 x (EY-2).
+namespace fp {
+constexpr int NPX = 32, EX = 64, FX = EX - 2;
+constexpr int PX = EX + 4;   // LDS pitch: ring-2 at 1 and EX+2, columns at 2..EX+1
+}
+
+template <typename T, int MODE, int EY>
+__global__ void __launch_bounds__(fp::NPX *EY) k_gsrb_pair(Lev<T> L, const T *__restrict__ pin,
+                                                          T *__restrict__ pout, FusedGrid G)
+{
+    using namespace fp;
+    constexpr int FY = EY - 2, PY = EY + 2, NT = NPX * EY;
+    const int b = blockIdx.x;
+    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
+    if (idx >= G.total) return;
+    const int ntile = G.ntx * G.nty;
+    const int chunk = idx / ntile, tile = idx - chunk * ntile;
+    const int tyi = tile / G.ntx, txi = tile - tyi * G.ntx;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int plane = nx * ny;
+    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+    const int p = threadIdx.x & (NPX - 1), ty = threadIdx.x / NPX;
+    const int Y = tyi * FY - 1 + ty;
+    const int X0 = txi * FX - 1 + 2 * p;    // global x of the pair's columns X0, X0+1
+    int Yw, Xw[2];
+    bool oky, okx[2];
+    wrap_coord(Y, ny, L.pery, Yw, oky);
+    wrap_coord(X0, nx, L.perx, Xw[0], okx[0]);
+    wrap_coord(X0 + 1, nx, L.perx, Xw[1], okx[1]);
+    bool act[2] = {okx[0] && oky, okx[1] && oky};
+    const bool outrow = ty >= 1 && ty <= FY && Y < ny;
+    bool out[2] = {outrow && (2 * p) >= 1 && X0 < nx, outrow && (2 * p + 1) <= FX && X0 + 1 < nx};
+    const int col[2] = {Xw[0] + nx * Yw, Xw[1] + nx * Yw};
+    const int pos0 = (2 * p + 2) + PX * (ty + 1);   // LDS position of column 0 (column 1 at +1)
+    // colour parity from unwrapped coordinates (periodic extents are even, so
+    // wrapping never changes it; ghost columns keep their natural parity)
+    const int par0 = (X0 + Y) & 1;
+    // ring-2 roles: rows -1/EY for the tile's 64 columns (two per thread), and
+    // columns -1/EX of every row (first/last pair-lane)
+    int r2pos[2] = {-1, -1}, r2col[2] = {0, 0};
+    if (ty == 0 || ty == EY - 1) {
+        int yw;
+        bool ok;
+        wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, L.pery, yw, ok);
+        for (int c = 0; c < 2; ++c)
+            if (ok && okx[c]) { r2pos[c] = pos0 + c + (ty == 0 ? -PX : PX); r2col[c] = Xw[c] + nx * yw; }
+    }
+    int r2xpos = -1, r2xcol = 0;
+    if (p == 0 || p == NPX - 1) {
+        int xw;
+        bool ok;
+        wrap_coord(p == 0 ? X0 - 1 : X0 + 2, nx, L.perx, xw, ok);
+        if (ok && oky) { r2xpos = (p == 0 ? pos0 - 1 : pos0 + 2); r2xcol = xw + nx * Yw; }
+    }
+    __shared__ T tile_s[3][PX * PY];
+    for (int q = threadIdx.x; q < 3 * PX * PY; q += NT) (&tile_s[0][0])[q] = T(0);
+
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (L.perz) {
+            while (k < 0) k += nz;
+            while (k >= nz) k -= nz;
+            kw = k;
+            return true;
+        }
+        kw = 0;
+        return false;
+    };
+    auto ld = [&](int c, int k) -> T {   // old iterate, zero outside / in MODE 1
+        int kw;
+        if (MODE == 1 || !kwrap(k, kw)) return T(0);
+        return pin[c + plane * kw];
+    };
+    const int kb = z0 - 1, ke = z1;
+    // per column: final/old values at planes k-2 .. k+2
+    T pm2[2], pm1[2], p0[2], pp1[2], pp2[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        pm2[c] = T(0);
+        pm1[c] = act[c] ? ld(col[c], kb - 1) : T(0);
+        p0[c] = act[c] ? ld(col[c], kb) : T(0);
+        pp1[c] = act[c] ? ld(col[c], kb + 1) : T(0);
+        pp2[c] = act[c] ? ld(col[c], kb + 2) : T(0);
+    }
+    // face weights and rhs of both columns: current plane (red), previous
+    // plane (black) and the next plane (fetched one step ahead)
+    T cw[2][6], crhs[2], pw[2][6], prhs[2], nw[2][6], nrhs[2], n2[3];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        crhs[c] = prhs[c] = nrhs[c] = T(0);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) cw[c][s] = pw[c][s] = nw[c][s] = T(0);
+    }
+    n2[0] = n2[1] = n2[2] = T(0);
+    auto fetch = [&](int k) {
+        int kw;
+        const bool kok = kwrap(k, kw);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (act[c] && kok) {
+                const int q = col[c] + plane * kw;
+                nw[c][0] = L.bx[q];
+                nw[c][1] = (Xw[c] < nx - 1) ? L.bx[q + 1] : L.bxh[Yw + ny * kw];
+                nw[c][2] = L.by[q];
+                nw[c][3] = (Yw < ny - 1) ? L.by[q + nx] : L.byh[Xw[c] + nx * kw];
+                nw[c][4] = L.bz[q];
+                nw[c][5] = (kw < nz - 1) ? L.bz[q + plane] : L.bzh[col[c]];
+                nrhs[c] = L.rhs[q];
+            }
+        n2[0] = (r2pos[0] >= 0) ? ld(r2col[0], k) : T(0);
+        n2[1] = (r2pos[1] >= 0) ? ld(r2col[1], k) : T(0);
+        n2[2] = (r2xpos >= 0) ? ld(r2xcol, k) : T(0);
+    };
+    fetch(kb);
+    __syncthreads();
+    T *buf = tile_s[0], *pbuf = tile_s[2], *qbuf = tile_s[1];
+    for (int k = kb; k <= ke; ++k) {
+        if (k > kb) {
+            T *t = qbuf;
+            qbuf = pbuf;
+            pbuf = buf;
+            buf = t;
+        }
+        int kw;
+        const bool kok = kwrap(k, kw);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) { pw[c][s] = cw[c][s]; cw[c][s] = nw[c][s]; }
+            prhs[c] = crhs[c];
+            crhs[c] = nrhs[c];
+        }
+        const T r2v0 = n2[0], r2v1 = n2[1], r2vx = n2[2];
+        if (k < ke) fetch(k + 1);
+        // (a) publish the old values of plane k
+        if (act[0]) buf[pos0] = kok ? p0[0] : T(0);
+        if (act[1]) buf[pos0 + 1] = kok ? p0[1] : T(0);
+        if (r2pos[0] >= 0) buf[r2pos[0]] = r2v0;
+        if (r2pos[1] >= 0) buf[r2pos[1]] = r2v1;
+        if (r2xpos >= 0) buf[r2xpos] = r2vx;
+        __syncthreads();
+        // (c) the red cell of the pair on plane k: column r (0 when X0+Y+k even)
+        const int r = (par0 + k) & 1;
+        const bool ractive = kok && (r == 0 ? act[0] : act[1]);
+        if (ractive) {
+            const int pr = pos0 + r;
+            T w[6], nb[6];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) w[s] = r ? cw[1][s] : cw[0][s];
+            nb[0] = r ? p0[0] : buf[pr - 1];          // x- (own black column or LDS)
+            nb[1] = r ? buf[pr + 1] : p0[1];          // x+
+            nb[2] = buf[pr - PX];
+            nb[3] = buf[pr + PX];
+            nb[4] = r ? pm1[1] : pm1[0];
+            nb[5] = r ? pp1[1] : pp1[0];
+            T v = gs_value<3, T>(L, w, nb, r ? crhs[1] : crhs[0]);
+            buf[pr] = v;
+            if (r) p0[1] = v;
+            else p0[0] = v;
+        }
+        // (d) the black cell of plane k-1 is the same column r; (e) write k-1
+        const int km = k - 1;
+        if (km >= z0 && km < z1) {
+            const bool bout = r ? out[1] : out[0];
+            if (bout) {
+                const int pr = pos0 + r;
+                T w[6], nb[6];
+#pragma unroll
+                for (int s = 0; s < 6; ++s) w[s] = r ? pw[1][s] : pw[0][s];
+                nb[0] = r ? pm1[0] : pbuf[pr - 1];    // the other column is red at k-1
+                nb[1] = r ? pbuf[pr + 1] : pm1[1];
+                nb[2] = pbuf[pr - PX];
+                nb[3] = pbuf[pr + PX];
+                nb[4] = r ? pm2[1] : pm2[0];
+                nb[5] = r ? p0[1] : p0[0];
+                T v = gs_value<3, T>(L, w, nb, r ? prhs[1] : prhs[0]);
+                if (r) pm1[1] = v;
+                else pm1[0] = v;
+            }
+            if (out[0]) pout[col[0] + plane * km] = pm1[0];
+            if (out[1]) pout[col[1] + plane * km] = pm1[1];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            pm2[c] = pm1[c];
+            pm1[c] = p0[c];
+            p0[c] = pp1[c];
+            pp1[c] = pp2[c];
+            pp2[c] = (act[c] && k + 3 <= ke + 1) ? ld(col[c], k + 3) : T(0);
+        }
+    }
+}
+
 // residual of the fine level restricted onto the coarse level in one pass
 // (the fine residual is never stored); also clears the coarse iterate.
 // Block: 64 coarse cells in x times the 2 (y) x 2 (z) fine offsets.
@@ -1267,10 +1465,37 @@ void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
 }
 
+template <typename T, int EY>
+static void launch_pair(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
+{
+    FusedGrid G;
+    G.ntx = (L.nx + fp::FX - 1) / fp::FX;
+    G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
+    const int tiles = G.ntx * G.nty;
+    int zc = (2048 + tiles - 1) / tiles;
+    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
+    G.tz = (L.nz + zc - 1) / zc;
+    zc = (L.nz + G.tz - 1) / G.tz;
+    G.total = tiles * zc;
+    G.per_xcd = (G.total + 7) / 8;
+    dim3 g(8 * G.per_xcd), b(fp::NPX * EY);
+    if (mode == SWEEP_ZERO)
+        hipLaunchKernelGGL((k_gsrb_pair<T, 1, EY>), g, b, 0, s, L, pin, pout, G);
+    else
+        hipLaunchKernelGGL((k_gsrb_pair<T, 0, EY>), g, b, 0, s, L, pin, pout, G);
+}
+
 template <typename T>
 void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
                        const EllBC &bc, hipStream_t s)
 {
+    static int variant = -1;   // 0: one column per thread, 8/16: x-pairs with 8/16 rows
+    if (variant < 0) {
+        const char *e = getenv("MAC_SWEEP");
+        variant = e ? atoi(e) : 8;
+    }
+    if (mode != SWEEP_PROLONG && variant == 8) { launch_pair<T, 8>(L, pin, pout, mode, s); return; }
+    if (mode != SWEEP_PROLONG && variant == 16) { launch_pair<T, 16>(L, pin, pout, mode, s); return; }
     FusedGrid G;
     G.ntx = (L.nx + fz::FX - 1) / fz::FX;
     G.nty = (L.ny + fz::FY - 1) / fz::FY;

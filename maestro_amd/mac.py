@@ -91,7 +91,7 @@ def lib():
         L.mac_plan_set_opts.argtypes = [vp, C.POINTER(COpts)]
         L.mac_project.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
         L.mac_project_level.argtypes = [vp] * 9 + [C.POINTER(CStats)]
-        L.mac_adjoint_level.argtypes = [vp] * 17 + [C.POINTER(CStats)]
+        L.mac_adjoint_level.argtypes = [vp] * 16 + [C.POINTER(CStats)]
         L.mac_version.restype = C.c_char_p
         L.mac_debug_level_dims.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.mac_debug_get.argtypes = [vp, ip, ip, vp]
