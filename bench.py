@@ -11,9 +11,12 @@ step on its own fresh copy of the inputs (staged in HBM before timing).
 ``value`` is the mixed-precision path (FP32 V-cycle inside FP64 iterative
 refinement) in solves/s; the plain FP64 V-cycle is measured the same way
 and reported beside it.  ``roofline`` covers the dominant kernel of the
-headline path (finest-level GSRB colour pass), timed with HIP events on the
-library's own stream during the timed steps.  ``cpu_baseline`` is the FP64
-oracle (C/OpenMP restatement) on the same inputs, on the host cores.
+headline path: the finest-level fused red-black sweep (one full sweep per
+launch, the launches that read the old iterate), timed with HIP events on
+the library's own stream during the timed steps.  Its ``traffic`` figure is
+read from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+(profiles/pmc_gsrb_sweep_256.txt) when present.  ``cpu_baseline`` is the
+FP64 oracle (C/OpenMP restatement) on the same inputs, on the host cores.
 """
 from __future__ import annotations
 
@@ -29,6 +32,32 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def sweep_kernel_name(s: int) -> str:
+    """Name of the finest-level sweep kernel the library launches (the
+    MAC_SWEEP environment variable selects the variant)."""
+    t = "double" if s == 8 else "float"
+    v = os.environ.get("MAC_SWEEP", "8")
+    if v == "0":
+        return f"k_gsrb_fused<{t}, 0>"
+    return f"k_gsrb_pair<{t}, 0, {int(v)}>"
+
+
+def measured_traffic(name: str):
+    """HBM bytes per launch of ``name`` from the committed PMC summary:
+    lines 'bench-traffic <kernel> <bytes>' (FETCH_SIZE x2 + WRITE_SIZE x1,
+    calibrated on a plain copy)."""
+    path = os.path.join(ROOT, "profiles", "pmc_gsrb_sweep_256.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("bench-traffic "):
+                rest = line[len("bench-traffic "):].rsplit(None, 1)
+                if len(rest) == 2 and rest[0].strip() == name:
+                    return float(rest[1])
+    return None
 
 
 def gsrb_launch_bytes(ncell: int, s: int) -> float:
@@ -51,8 +80,8 @@ def stage(per_box, copies, torch):
     return sets, dr
 
 
-def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch):
-    plan.set_opts(precision=prec, device_ptrs=1)
+def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1):
+    plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner)
     stats = []
     for w in range(warmup):
         du, dp = sets[w]
@@ -79,6 +108,11 @@ def main():
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
+    ap.add_argument("--inner-cycles", type=int, default=1,
+                    help="FP32 V-cycles per refinement step of the mixed path")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (the PMC byte-counter "
+                         "calibration: 1 GiB read + 1 GiB written per launch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -86,6 +120,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 8,
+                          "bytes_written": a.numel() * 8, "launches": 5}))
+        return
     from maestro_amd import mac
     from maestro_amd import workloads as W
     from maestro_amd import boxes as BX
@@ -115,7 +158,8 @@ def main():
             for a, h in zip(dp, phi):
                 a.copy_(torch.from_numpy(h))
         torch.cuda.synchronize()
-        dt, st, prof = run_precision(plan, sets, dr, b0, b0e, args.warmup, args.steps, prec, torch)
+        dt, st, prof = run_precision(plan, sets, dr, b0, b0e, args.warmup, args.steps, prec, torch,
+                                     args.inner_cycles)
         # host snapshot of the last step's outputs (the staged buffers are reused)
         results[name] = {"seconds": dt, "stats": st, "prof": prof,
                          "umac": [[a.cpu().numpy() for a in box] for box in sets[-1][0]],
@@ -198,8 +242,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 V-cycle + f64 refinement",
-        "data": "synthetic (seeded, BASELINE config 2 inputs)",
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded isothermal stratification, 1% density noise, U~U[-1,1])",
         "config": {"workload": "3D 256^3 plane-parallel MAC projection, 64 boxes of 64^3, "
                                "x/y periodic, z walls, beta0 contrast 1e4, mixed FP32-MG + FP64 IR, "
                                "rel residual 1e-10",
@@ -211,13 +255,15 @@ def main():
                   "mixed_ir_steps": mixed["stats"][-1]["iters"],
                   "mixed_fp32_vcycles": mixed["stats"][-1]["inner_cycles"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_gsrb_fused<float,0> (finest level, one full red-black sweep)",
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(sweep_kernel_name(4)),
+                     "kernel": sweep_kernel_name(4) + " (finest level, one full red-black sweep)",
                      "avg_launch_us": round(1e3 * avg_ms, 2), "launches": cnt,
                      "bytes_per_launch": gsrb_launch_bytes(ncell, 4)},
         "roofline_fp64": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                          "kernel": "k_gsrb_fused<double,*> (finest level, one full red-black sweep)",
+                          "traffic": measured_traffic(sweep_kernel_name(8)),
+                          "kernel": sweep_kernel_name(8) + " (finest level, one full red-black sweep)",
                           "avg_launch_us": round(1e3 * avg64, 2), "launches": cnt64},
         "kernel_ms_per_solve": {
             p: {k: round(v[1] / args.steps, 3) for k, v in results[p]["prof"].items()}

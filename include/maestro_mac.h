@@ -193,13 +193,20 @@ int mac_debug_op(mac_plan *plan, int level, int op, int count);
  * enabled, launches of the finest-level kernels are bracketed by HIP
  * events; mac_plan_profile_read returns the number of timed launches and
  * their summed duration for one class.  Enabling resets the counters. */
-#define MAC_PROF_GSRB64 0     /* finest-level smoothing launches, FP64 */
-#define MAC_PROF_GSRB32 1     /* finest-level smoothing launches, FP32 */
+#define MAC_PROF_GSRB64 0     /* finest-level smoothing launches reading phi, FP64 */
+#define MAC_PROF_GSRB32 1     /* finest-level smoothing launches reading phi, FP32 */
 #define MAC_PROF_RESID64 2    /* finest-level FP64 residual/norm launches */
 #define MAC_PROF_SETUP 3      /* fused setup from the caller's boxes */
 #define MAC_PROF_CORRECT 4    /* face correction + ghost fill */
 #define MAC_PROF_COARSE 5     /* single-workgroup coarse-level cycle */
-#define MAC_PROF_NCLASS 6
+#define MAC_PROF_GSRB64_ZERO 6   /* finest-level first sweeps from a zero iterate, FP64 */
+#define MAC_PROF_GSRB32_ZERO 7   /* finest-level first sweeps from a zero iterate, FP32 */
+#define MAC_PROF_RESTRICT64 8    /* finest-level residual + restriction, FP64 */
+#define MAC_PROF_RESTRICT32 9    /* finest-level residual + restriction, FP32 */
+#define MAC_PROF_PROLONG64 10    /* finest-level prolongation, FP64 */
+#define MAC_PROF_PROLONG32 11    /* finest-level prolongation, FP32 */
+#define MAC_PROF_IR_UPDATE 12    /* phi64 += e32 of the refinement */
+#define MAC_PROF_NCLASS 13
 int mac_plan_profile(mac_plan *plan, int enable);
 int mac_plan_profile_read(mac_plan *plan, int cls, long *count, double *total_ms);
 

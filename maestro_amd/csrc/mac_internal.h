@@ -71,6 +71,26 @@ __device__ __forceinline__ void stencil(const Lev<T> &L, const T *__restrict__ p
     }
 }
 
+// The six neighbour values of cell (i,j,k) of an array x laid out like
+// level L, with the same periodic wrap and zero outside as stencil()
+template <int DIM, typename S, typename T>
+__device__ __forceinline__ void nbr_values(const Lev<S> &L, const T *__restrict__ x, int i, int j, int k,
+                                           long c, T p[6])
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const long sy = nx, sz = (long)nx * ny;
+    p[0] = (i > 0) ? x[c - 1] : (L.perx ? x[c + (nx - 1)] : T(0));
+    p[1] = (i < nx - 1) ? x[c + 1] : (L.perx ? x[c - (nx - 1)] : T(0));
+    p[2] = (j > 0) ? x[c - sy] : (L.pery ? x[c + (long)(ny - 1) * sy] : T(0));
+    p[3] = (j < ny - 1) ? x[c + sy] : (L.pery ? x[c - (long)(ny - 1) * sy] : T(0));
+    if (DIM == 3) {
+        p[4] = (k > 0) ? x[c - sz] : (L.perz ? x[c + (long)(nz - 1) * sz] : T(0));
+        p[5] = (k < nz - 1) ? x[c + sz] : (L.perz ? x[c - (long)(nz - 1) * sz] : T(0));
+    } else {
+        p[4] = p[5] = T(0);
+    }
+}
+
 // Gauss-Seidel point value (fixed evaluation order, shared with the oracle)
 template <int DIM, typename T>
 __device__ __forceinline__ T gs_value(const Lev<T> &L, const T w[6], const T p[6], T rhs)

@@ -718,21 +718,43 @@ __device__ __forceinline__ void block_max_atomic(double a, unsigned long long *n
     }
 }
 
+// Grid-stride walk over the x-rows (j,k) of an nx x ny x nz level: each
+// wave takes whole rows and its lanes stride x.  The row index is
+// wave-uniform, so its division into (j,k) runs once per wave and row on the
+// scalar unit instead of a 64-bit divide per cell.
+static constexpr int ROW_WAVES = TPB / 64;
+
+template <typename F>
+__device__ __forceinline__ void for_rows(int nx, int ny, int nz, F &&f)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nrow = ny * nz;
+    for (int row = blockIdx.x * ROW_WAVES + wv; row < nrow; row += gridDim.x * ROW_WAVES) {
+        const int k = row / ny, j = row - k * ny;
+        const long base = (long)row * nx;
+        for (int i = lane; i < nx; i += 64) f(i, j, k, base + i);
+    }
+}
+
+static inline unsigned row_blocks(int ny, int nz)
+{
+    return (unsigned)(((long)ny * nz + ROW_WAVES - 1) / ROW_WAVES);
+}
+
 // residual array and/or its max norm; grid-stride so the norm costs at most
 // one atomic per block of a fixed-size grid
 template <int DIM, typename T>
 __global__ void __launch_bounds__(TPB) k_residual(Lev<T> L, T *out, unsigned long long *nrm)
 {
-    long n = (long)L.nx * L.ny * L.nz;
     double a = 0.0;
-    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB) {
-        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+    for_rows(L.nx, L.ny, L.nz, [&](int i, int j, int k, long q) {
         T w[6], p[6];
         stencil<DIM, T>(L, L.phi, i, j, k, q, w, p);
         T r = L.rhs[q] - apply_value<DIM, T>(L, w, p, L.phi[q]);
         if (out) out[q] = r;
         a = fmax(a, fabs((double)r));
-    }
+    });
     if (nrm) block_max_atomic(a, nrm);
 }
 
@@ -742,23 +764,51 @@ template <int DIM>
 __global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
                                                     unsigned long long *nrm)
 {
-    long n = (long)L.nx * L.ny * L.nz;
     double a = 0.0;
-    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB) {
-        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+    for_rows(L.nx, L.ny, L.nz, [&](int i, int j, int k, long q) {
         double w[6], p[6];
         stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
         double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, L.phi[q]);
         r32[q] = (float)r;
         a = fmax(a, fabs(r));
-    }
+    });
     block_max_atomic(a, nrm);
 }
 
-__global__ void __launch_bounds__(TPB) k_phi_add_f32(double *phi, const float *e, long n)
+// Refinement update fused with the next FP64 residual: phi_new = phi + e
+// (e the FP32 correction) is formed on the fly for the cell and its six
+// neighbours, written to phi_out, and r = rhs - A phi_new goes out as the
+// FP32 right side of the next V-cycle, with its max norm.  phi_new equals
+// what a separate phi += e pass stores, so the iteration is unchanged.
+template <int DIM>
+__global__ void __launch_bounds__(TPB) k_ir_update_residual(Lev<double> L, const float *__restrict__ e,
+                                                           double *__restrict__ phi_out, float *r32,
+                                                           unsigned long long *nrm)
 {
-    long q = (long)blockIdx.x * TPB + threadIdx.x;
-    if (q < n) phi[q] = phi[q] + (double)e[q];
+    double a = 0.0;
+    for_rows(L.nx, L.ny, L.nz, [&](int i, int j, int k, long q) {
+        double w[6], p[6];
+        float pe[6];
+        stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
+        nbr_values<DIM>(L, e, i, j, k, q, pe);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) p[s] = p[s] + (double)pe[s];
+        const double pc = L.phi[q] + (double)e[q];
+        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, pc);
+        phi_out[q] = pc;
+        r32[q] = (float)r;
+        a = fmax(a, fabs(r));
+    });
+    block_max_atomic(a, nrm);
+}
+
+// max |x| over n values (the norm of the solver's right side)
+__global__ void __launch_bounds__(TPB) k_absmax(const double *x, long n, unsigned long long *nrm)
+{
+    double a = 0.0;
+    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB)
+        a = fmax(a, fabs(x[q]));
+    block_max_atomic(a, nrm);
 }
 
 // ------------------------------------------------------- restriction
@@ -788,11 +838,9 @@ template <int DIM, typename T>
 __global__ void __launch_bounds__(TPB) k_prolong(Lev<T> F, const T *e, int cnx, int cny, int cnz,
                                                  EllBC bc)
 {
-    long q = (long)blockIdx.x * TPB + threadIdx.x;
-    long n = (long)F.nx * F.ny * F.nz;
-    if (q >= n) return;
-    int i = (int)(q % F.nx), j = (int)((q / F.nx) % F.ny), k = (int)(q / ((long)F.nx * F.ny));
-    F.phi[q] = F.phi[q] + prolong_value<DIM, T>(e, cnx, cny, cnz, bc, i, j, k);
+    for_rows(F.nx, F.ny, F.nz, [&](int i, int j, int k, long q) {
+        F.phi[q] = F.phi[q] + prolong_value<DIM, T>(e, cnx, cny, cnz, bc, i, j, k);
+    });
 }
 
 template <typename T>
@@ -804,9 +852,11 @@ __global__ void __launch_bounds__(TPB) k_fill(T *x, long n, T v)
 
 // -------------------------------------------- single-workgroup coarse cycle
 // V-cycle over levels lc..nlev-1 and the FP64 CG bottom solve, all inside
-// one 1024-thread workgroup (these levels are latency bound; one launch
-// replaces dozens).  Level data lives in global memory (L2 resident).
-static constexpr int CTPB = 1024;
+// one 512-thread workgroup (these levels are latency bound; one launch
+// replaces dozens).  When they fit, the levels are staged in LDS for the
+// whole cycle; otherwise they stay in global memory (L2 resident).  512
+// threads leave 256 VGPRs per lane, so the inlined cycle does not spill.
+static constexpr int CTPB = 512;
 
 __device__ __forceinline__ double blk_sum(double v, double *sh)
 {
@@ -840,15 +890,14 @@ template <int DIM, typename T>
 __device__ void blk_gsrb(const Lev<T> &L)
 {
     const int hx = (L.nx + 1) >> 1;
-    long nq = (long)hx * L.ny * L.nz;
+    const int nq = hx * L.ny * L.nz;
     for (int color = 0; color < 2; ++color) {
-        for (long q = threadIdx.x; q < nq; q += blockDim.x) {
-            int m = (int)(q % hx);
-            int j = (int)((q / hx) % L.ny);
-            int k = (int)(q / ((long)hx * L.ny));
+        for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+            const int r = q / hx, m = q - r * hx;
+            const int k = r / L.ny, j = r - k * L.ny;
             int i = 2 * m + ((j + k + color) & 1);
             if (i >= L.nx) continue;
-            long c = i + (long)L.nx * (j + (long)L.ny * k);
+            const int c = i + L.nx * (j + L.ny * k);
             T w[6], p[6];
             stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
             L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
@@ -861,9 +910,10 @@ __device__ void blk_gsrb(const Lev<T> &L)
 template <int DIM, typename T>
 __device__ void blk_apply64(const Lev<T> &L, const double *x, double *out)
 {
-    long n = (long)L.nx * L.ny * L.nz;
-    for (long q = threadIdx.x; q < n; q += blockDim.x) {
-        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long)L.nx * L.ny));
+    const int n = L.nx * L.ny * L.nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int r = q / L.nx, i = q - r * L.nx;
+        const int k = r / L.ny, j = r - k * L.ny;
         Lev<double> D;
         D.nx = L.nx; D.ny = L.ny; D.nz = L.nz;
         D.perx = L.perx; D.pery = L.pery; D.perz = L.perz;
@@ -872,24 +922,24 @@ __device__ void blk_apply64(const Lev<T> &L, const double *x, double *out)
         (void)pt;
         // weights from the level, neighbour values from x
         wt[0] = L.bx[q];
-        wt[1] = (i < L.nx - 1) ? L.bx[q + 1] : L.bxh[j + (long)L.ny * k];
+        wt[1] = (i < L.nx - 1) ? L.bx[q + 1] : L.bxh[j + L.ny * k];
         wt[2] = L.by[q];
-        wt[3] = (j < L.ny - 1) ? L.by[q + L.nx] : L.byh[i + (long)L.nx * k];
+        wt[3] = (j < L.ny - 1) ? L.by[q + L.nx] : L.byh[i + L.nx * k];
         if (DIM == 3) {
-            long sz = (long)L.nx * L.ny;
+            const int sz = L.nx * L.ny;
             wt[4] = L.bz[q];
-            wt[5] = (k < L.nz - 1) ? L.bz[q + sz] : L.bzh[i + (long)L.nx * j];
+            wt[5] = (k < L.nz - 1) ? L.bz[q + sz] : L.bzh[i + L.nx * j];
         }
         double w[6], p[6];
-        const long sy = L.nx, sz = (long)L.nx * L.ny;
+        const int sy = L.nx, sz = L.nx * L.ny;
         for (int s = 0; s < 2 * DIM; ++s) w[s] = (double)wt[s];
         p[0] = (i > 0) ? x[q - 1] : (L.perx ? x[q + L.nx - 1] : 0.0);
         p[1] = (i < L.nx - 1) ? x[q + 1] : (L.perx ? x[q - (L.nx - 1)] : 0.0);
-        p[2] = (j > 0) ? x[q - sy] : (L.pery ? x[q + (long)(L.ny - 1) * sy] : 0.0);
-        p[3] = (j < L.ny - 1) ? x[q + sy] : (L.pery ? x[q - (long)(L.ny - 1) * sy] : 0.0);
+        p[2] = (j > 0) ? x[q - sy] : (L.pery ? x[q + (L.ny - 1) * sy] : 0.0);
+        p[3] = (j < L.ny - 1) ? x[q + sy] : (L.pery ? x[q - (L.ny - 1) * sy] : 0.0);
         if (DIM == 3) {
-            p[4] = (k > 0) ? x[q - sz] : (L.perz ? x[q + (long)(L.nz - 1) * sz] : 0.0);
-            p[5] = (k < L.nz - 1) ? x[q + sz] : (L.perz ? x[q - (long)(L.nz - 1) * sz] : 0.0);
+            p[4] = (k > 0) ? x[q - sz] : (L.perz ? x[q + (L.nz - 1) * sz] : 0.0);
+            p[5] = (k < L.nz - 1) ? x[q + sz] : (L.perz ? x[q - (L.nz - 1) * sz] : 0.0);
         } else {
             w[4] = w[5] = p[4] = p[5] = 0.0;
         }
@@ -902,30 +952,30 @@ template <int DIM, typename T>
 __device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit, int singular,
                               double *sh)
 {
-    long n = (long)L.nx * L.ny * L.nz;
+    const int n = L.nx * L.ny * L.nz;
     double *x = cg, *r = cg + n, *p = cg + 2 * n, *Ap = cg + 3 * n;
     // rhs (mean removed on singular levels) and initial guess
     double m = 0.0;
     if (singular) {
         double s = 0.0;
-        for (long q = threadIdx.x; q < n; q += blockDim.x) s += (double)L.rhs[q];
+        for (int q = threadIdx.x; q < n; q += blockDim.x) s += (double)L.rhs[q];
         m = blk_sum(s, sh) / (double)n;
     }
-    for (long q = threadIdx.x; q < n; q += blockDim.x) x[q] = (double)L.phi[q];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] = (double)L.phi[q];
     __syncthreads();
     blk_apply64<DIM, T>(L, x, Ap);
     double rs = 0.0;
-    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
         r[q] = ((double)L.rhs[q] - m) - Ap[q];
         rs += r[q];
     }
     if (singular) {
         double rm = blk_sum(rs, sh) / (double)n;
-        for (long q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
     }
     __syncthreads();
     double a = 0.0, rr = 0.0;
-    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
         a = fmax(a, fabs(r[q]));
         rr += r[q] * r[q];
         p[q] = r[q];
@@ -936,23 +986,23 @@ __device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit
         for (int it = 0; it < maxit; ++it) {
             blk_apply64<DIM, T>(L, p, Ap);
             double pap = 0.0;
-            for (long q = threadIdx.x; q < n; q += blockDim.x) pap += p[q] * Ap[q];
+            for (int q = threadIdx.x; q < n; q += blockDim.x) pap += p[q] * Ap[q];
             pap = blk_sum(pap, sh);
             if (!(pap > 0.0)) break;
             double alpha = rr / pap;
             double rsum = 0.0;
-            for (long q = threadIdx.x; q < n; q += blockDim.x) {
+            for (int q = threadIdx.x; q < n; q += blockDim.x) {
                 x[q] += alpha * p[q];
                 r[q] -= alpha * Ap[q];
                 rsum += r[q];
             }
             if (singular) {
                 double rm = blk_sum(rsum, sh) / (double)n;
-                for (long q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+                for (int q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
             }
             __syncthreads();
             double am = 0.0, rr1 = 0.0;
-            for (long q = threadIdx.x; q < n; q += blockDim.x) {
+            for (int q = threadIdx.x; q < n; q += blockDim.x) {
                 am = fmax(am, fabs(r[q]));
                 rr1 += r[q] * r[q];
             }
@@ -961,26 +1011,27 @@ __device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit
             if (am <= eps * r0) break;
             double beta = rr1 / rr;
             rr = rr1;
-            for (long q = threadIdx.x; q < n; q += blockDim.x) p[q] = r[q] + beta * p[q];
+            for (int q = threadIdx.x; q < n; q += blockDim.x) p[q] = r[q] + beta * p[q];
             __syncthreads();
         }
     }
-    for (long q = threadIdx.x; q < n; q += blockDim.x) L.phi[q] = (T)x[q];
+    for (int q = threadIdx.x; q < n; q += blockDim.x) L.phi[q] = (T)x[q];
     __syncthreads();
 }
 
 template <int DIM, typename T>
 __device__ void blk_resid_restrict(const Lev<T> &F, const Lev<T> &C)
 {
-    long nc = (long)C.nx * C.ny * C.nz;
-    for (long q = threadIdx.x; q < nc; q += blockDim.x) {
-        int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), K = (int)(q / ((long)C.nx * C.ny));
+    const int nc = C.nx * C.ny * C.nz;
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) {
+        const int rq = q / C.nx, I = q - rq * C.nx;
+        const int K = rq / C.ny, J = rq - K * C.ny;
         T rv[2][2][2];
         for (int c = 0; c < (DIM == 3 ? 2 : 1); ++c)
             for (int b = 0; b < 2; ++b)
                 for (int a = 0; a < 2; ++a) {
                     int i = 2 * I + a, j = 2 * J + b, k = (DIM == 3) ? 2 * K + c : 0;
-                    long f = i + (long)F.nx * (j + (long)F.ny * k);
+                    const int f = i + F.nx * (j + F.ny * k);
                     T w[6], p[6];
                     stencil<DIM, T>(F, F.phi, i, j, k, f, w, p);
                     rv[c][b][a] = F.rhs[f] - apply_value<DIM, T>(F, w, p, F.phi[f]);
@@ -1002,35 +1053,117 @@ __device__ void blk_resid_restrict(const Lev<T> &F, const Lev<T> &C)
 template <int DIM, typename T>
 __device__ void blk_prolong(const Lev<T> &F, const Lev<T> &C, const EllBC &bc)
 {
-    long n = (long)F.nx * F.ny * F.nz;
-    for (long q = threadIdx.x; q < n; q += blockDim.x) {
-        int i = (int)(q % F.nx), j = (int)((q / F.nx) % F.ny), k = (int)(q / ((long)F.nx * F.ny));
+    const int n = F.nx * F.ny * F.nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int r = q / F.nx, i = q - r * F.nx;
+        const int k = r / F.ny, j = r - k * F.ny;
         F.phi[q] = F.phi[q] + prolong_value<DIM, T>(C.phi, C.nx, C.ny, C.nz, bc, i, j, k);
     }
     __syncthreads();
 }
 
+// LDS image of the coarse levels: per level phi, rhs, bx, by, (bz), then the
+// high-face slabs, each rounded to 16 bytes; after the last level the four
+// FP64 CG vectors.  The host sizes the allocation with the same function.
+__host__ __device__ inline int lds_round(int bytes) { return (bytes + 15) & ~15; }
+
 template <int DIM, typename T>
+__host__ __device__ inline int lds_level_bytes(int nx, int ny, int nz)
+{
+    const int n = nx * ny * nz, s = (int)sizeof(T);
+    int b = 4 * lds_round(n * s) + lds_round(ny * nz * s) + lds_round(nx * nz * s);
+    if (DIM == 3) b += lds_round(n * s) + lds_round(nx * ny * s);
+    return b;
+}
+
+template <int DIM, typename T>
+__device__ Lev<T> lds_view(const Lev<T> &G, char *p)
+{
+    Lev<T> S = G;
+    const int n = G.nx * G.ny * G.nz;
+    auto carve = [&](int cnt) {
+        T *r = (T *)p;
+        p += lds_round(cnt * (int)sizeof(T));
+        return r;
+    };
+    S.phi = carve(n);
+    S.rhs = carve(n);
+    S.bx = carve(n);
+    S.by = carve(n);
+    S.bxh = carve(G.ny * G.nz);
+    S.byh = carve(G.nx * G.nz);
+    if (DIM == 3) {
+        S.bz = carve(n);
+        S.bzh = carve(G.nx * G.ny);
+    }
+    S.res = S.phi2 = nullptr;
+    return S;
+}
+
+// Level l of the coarse cycle: the global level itself, or its LDS image.
+template <int DIM, typename T, bool LDS>
+__device__ Lev<T> coarse_level(const Lev<T> *levs, int lc, int l, char *smem)
+{
+    if (!LDS) return levs[l];
+    char *p = smem;
+    for (int m = lc; m < l; ++m) p += lds_level_bytes<DIM, T>(levs[m].nx, levs[m].ny, levs[m].nz);
+    return lds_view<DIM, T>(levs[l], p);
+}
+
+template <int DIM, typename T, bool LDS>
 __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict__ levs, int lc,
                                                        int nlev, CoarseOpts o, double *cg)
 {
+    extern __shared__ __align__(16) char smem[];
     __shared__ double sh[CTPB / 64];
     const int bot = nlev - 1;
+    if (LDS) {
+        // stage the weights of every coarse level, and phi/rhs of the first
+        // (deeper phi/rhs are written by the restriction before use)
+        for (int l = lc; l <= bot; ++l) {
+            const Lev<T> G = levs[l];
+            const Lev<T> S = coarse_level<DIM, T, true>(levs, lc, l, smem);
+            const int n = G.nx * G.ny * G.nz;
+            for (int q = threadIdx.x; q < n; q += blockDim.x) {
+                S.bx[q] = G.bx[q];
+                S.by[q] = G.by[q];
+                if (DIM == 3) S.bz[q] = G.bz[q];
+                if (l == lc) {
+                    S.phi[q] = G.phi[q];
+                    S.rhs[q] = G.rhs[q];
+                }
+            }
+            for (int q = threadIdx.x; q < G.ny * G.nz; q += blockDim.x) S.bxh[q] = G.bxh[q];
+            for (int q = threadIdx.x; q < G.nx * G.nz; q += blockDim.x) S.byh[q] = G.byh[q];
+            if (DIM == 3)
+                for (int q = threadIdx.x; q < G.nx * G.ny; q += blockDim.x) S.bzh[q] = G.bzh[q];
+        }
+        char *p = smem;
+        for (int m = lc; m <= bot; ++m) p += lds_level_bytes<DIM, T>(levs[m].nx, levs[m].ny, levs[m].nz);
+        cg = (double *)p;
+        __syncthreads();
+    }
     for (int l = lc; l < bot; ++l) {
-        const Lev<T> L = levs[l];
+        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
         for (int s = 0; s < o.nu1; ++s) blk_gsrb<DIM, T>(L);
-        blk_resid_restrict<DIM, T>(L, levs[l + 1]);
+        blk_resid_restrict<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem));
     }
     {
-        const Lev<T> L = levs[bot];
+        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, bot, smem);
         for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
         blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
         for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
     }
     for (int l = bot - 1; l >= lc; --l) {
-        const Lev<T> L = levs[l];
-        blk_prolong<DIM, T>(L, levs[l + 1], o.bc);
+        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
+        blk_prolong<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem), o.bc);
         for (int s = 0; s < o.nu2; ++s) blk_gsrb<DIM, T>(L);
+    }
+    if (LDS) {
+        const Lev<T> G = levs[lc];
+        const Lev<T> S = coarse_level<DIM, T, true>(levs, lc, lc, smem);
+        const int n = G.nx * G.ny * G.nz;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) G.phi[q] = S.phi[q];
     }
 }
 
@@ -1545,22 +1678,30 @@ static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
 {
-    long n = (long)L.nx * L.ny * L.nz;
-    unsigned g = nrm ? std::min(nblk(n), NORM_GRID) : nblk(n);
+    const unsigned rb = row_blocks(L.ny, L.nz);
+    unsigned g = nrm ? std::min(rb, NORM_GRID) : rb;
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_residual<D_, T>), dim3(g), dim3(TPB), 0, s, L, out, nrm));
 }
 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s)
 {
-    long n = (long)L.nx * L.ny * L.nz;
-    unsigned g = std::min(nblk(n), NORM_GRID);
+    unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
 
-void launch_phi_add_f32(double *phi, const float *e, long n, hipStream_t s)
+void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
+                               unsigned long long *nrm, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_phi_add_f32, dim3(nblk(n)), dim3(TPB), 0, s, phi, e, n);
+    unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
+    DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_update_residual<D_>, dim3(g), dim3(TPB), 0, s, L, e, phi_out,
+                                         r32, nrm));
+}
+
+void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s)
+{
+    unsigned g = std::min(nblk(n), NORM_GRID);
+    hipLaunchKernelGGL(k_absmax, dim3(g), dim3(TPB), 0, s, x, n, nrm);
 }
 
 template <typename T>
@@ -1573,9 +1714,8 @@ void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hip
 template <typename T>
 void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s)
 {
-    long n = (long)F.nx * F.ny * F.nz;
-    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_prolong<D_, T>), dim3(nblk(n)), dim3(TPB), 0, s, F,
-                                         (const T *)C.phi, C.nx, C.ny, C.nz, bc));
+    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_prolong<D_, T>), dim3(row_blocks(F.ny, F.nz)), dim3(TPB), 0, s,
+                                         F, (const T *)C.phi, C.nx, C.ny, C.nz, bc));
 }
 
 template <typename T>
@@ -1584,12 +1724,36 @@ void launch_fill(T *x, long n, T v, hipStream_t s)
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, v);
 }
 
+// LDS bytes the coarse cycle needs to keep levels lc..nlev-1 and the bottom
+// CG vectors on chip
 template <typename T>
-void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, int lc, int nlev, const CoarseOpts &o,
-                         double *cg, hipStream_t s)
+static long coarse_lds_bytes(int dim, const Lev<T> *levs_host, int lc, int nlev)
 {
-    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T>), dim3(1), dim3(CTPB), 0, s, levs_dev,
-                                         lc, nlev, o, cg));
+    long b = 0;
+    for (int l = lc; l < nlev; ++l) {
+        const Lev<T> &L = levs_host[l];
+        b += dim == 3 ? lds_level_bytes<3, T>(L.nx, L.ny, L.nz) : lds_level_bytes<2, T>(L.nx, L.ny, 1);
+    }
+    const Lev<T> &B = levs_host[nlev - 1];
+    return b + 4L * B.nx * B.ny * B.nz * (long)sizeof(double);
+}
+
+static constexpr long COARSE_LDS_MAX = 64 * 1024;
+
+template <typename T>
+void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_host, int lc, int nlev,
+                         const CoarseOpts &o, double *cg, hipStream_t s)
+{
+    static int global_only = -1;   // MAC_COARSE_GLOBAL: keep the levels in global memory
+    if (global_only < 0) global_only = getenv("MAC_COARSE_GLOBAL") ? 1 : 0;
+    const long lds = coarse_lds_bytes<T>(dim, levs_host, lc, nlev);
+    if (!global_only && lds <= COARSE_LDS_MAX) {
+        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, true>), dim3(1), dim3(CTPB),
+                                             (unsigned)lds, s, levs_dev, lc, nlev, o, cg));
+    } else {
+        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, false>), dim3(1), dim3(CTPB), 0, s,
+                                             levs_dev, lc, nlev, o, cg));
+    }
 }
 
 void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s)
@@ -1654,7 +1818,7 @@ template void launch_prolong<double>(int, const Lev<double> &, const Lev<double>
 template void launch_prolong<float>(int, const Lev<float> &, const Lev<float> &, const EllBC &, hipStream_t);
 template void launch_fill<double>(double *, long, double, hipStream_t);
 template void launch_fill<float>(float *, long, float, hipStream_t);
-template void launch_coarse_cycle<double>(int, const Lev<double> *, int, int, const CoarseOpts &, double *, hipStream_t);
-template void launch_coarse_cycle<float>(int, const Lev<float> *, int, int, const CoarseOpts &, double *, hipStream_t);
+template void launch_coarse_cycle<double>(int, const Lev<double> *, const Lev<double> *, int, int, const CoarseOpts &, double *, hipStream_t);
+template void launch_coarse_cycle<float>(int, const Lev<float> *, const Lev<float> *, int, int, const CoarseOpts &, double *, hipStream_t);
 
 }  // namespace mac

@@ -95,17 +95,19 @@ template <typename T>
 void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s);
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
+void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
+                               unsigned long long *nrm, hipStream_t s);
+void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s);
-void launch_phi_add_f32(double *phi, const float *e, long n, hipStream_t s);
 template <typename T>
 void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hipStream_t s);
 template <typename T>
 void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s);
 template <typename T> void launch_fill(T *x, long n, T v, hipStream_t s);
 template <typename T>
-void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, int lc, int nlev, const CoarseOpts &o,
-                         double *cg, hipStream_t s);
+void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_host, int lc, int nlev,
+                         const CoarseOpts &o, double *cg, hipStream_t s);
 void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s);
 void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s);
 void launch_level_correct(int dim, const LevelIO &A, const Lev<double> &L, hipStream_t s);

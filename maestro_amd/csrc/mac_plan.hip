@@ -208,7 +208,9 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         int rc;
         if ((rc = alloc_dev(H, &L.phi, nc))) return rc;
         L.phi2 = nullptr;
-        if (P->dim == 3)
+        // second iterate: ping-pong of the fused 3D sweep, and on level 0
+        // the output of the fused refinement update
+        if (P->dim == 3 || l == 0)
             if ((rc = alloc_dev(H, &L.phi2, nc))) return rc;
         if ((rc = alloc_dev(H, &L.rhs, nc))) return rc;
         if ((rc = alloc_dev(H, &L.res, nc))) return rc;
@@ -284,7 +286,7 @@ static bool use_fused(const mac_plan *P)
 template <typename T>
 static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur)
 {
-    const int cls = sizeof(T) == 8 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32;
+    const bool f64 = sizeof(T) == 8;
     Lev<T> &L = H.lev[l];
     static int fuse_prolong = -1;
     if (fuse_prolong < 0) fuse_prolong = getenv("MAC_FUSED_PROLONG") ? 1 : 0;
@@ -293,13 +295,17 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         // separate streaming pass (phi += P e, 17 B/cell) is cheaper
         Lev<T> Lc = L;
         Lc.phi = cur ? L.phi2 : L.phi;
+        int q = (l == 0) ? prof_begin(P, f64 ? MAC_PROF_PROLONG64 : MAC_PROF_PROLONG32) : -1;
         launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
+        prof_end(P, q);
         first_mode = SWEEP_PLAIN;
     }
     for (int it = 0; it < nsweep; ++it) {
         T *pin = cur ? L.phi2 : L.phi;
         T *pout = cur ? L.phi : L.phi2;
         int mode = it == 0 ? first_mode : SWEEP_PLAIN;
+        const int cls = mode == SWEEP_ZERO ? (f64 ? MAC_PROF_GSRB64_ZERO : MAC_PROF_GSRB32_ZERO)
+                                           : (f64 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32);
         int q = (l == 0) ? prof_begin(P, cls) : -1;
         launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
                              P->bc, P->stream);
@@ -322,7 +328,9 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
             if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur[l]);
             else if (zero) launch_fill<T>(H.lev[l].phi, (long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz, T(0), s);
             const Lev<T> &L = H.lev[l];
+            int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_RESTRICT64 : MAC_PROF_RESTRICT32) : -1;
             launch_resid_restrict<T>(dim, L, cur[l] ? L.phi2 : L.phi, H.lev[l + 1], s);
+            prof_end(P, q);
         }
         {
             CoarseOpts co;
@@ -334,7 +342,7 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
             if (H.lc == 0 && zero_first)
                 launch_fill<T>(H.lev[0].phi, (long)H.lev[0].nx * H.lev[0].ny * H.lev[0].nz, T(0), s);
             int q = prof_begin(P, MAC_PROF_COARSE);
-            launch_coarse_cycle<T>(dim, H.lev_dev, H.lc, H.nlev, co, P->cg, s);
+            launch_coarse_cycle<T>(dim, H.lev_dev, H.lev.data(), H.lc, H.nlev, co, P->cg, s);
             prof_end(P, q);
         }
         for (int l = H.lc - 1; l >= 0; --l) {
@@ -373,7 +381,7 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
         co.singular = P->singular;
         co.bc = P->bc;
         int q = prof_begin(P, MAC_PROF_COARSE);
-        launch_coarse_cycle<T>(dim, H.lev_dev, H.lc, H.nlev, co, P->cg, s);
+        launch_coarse_cycle<T>(dim, H.lev_dev, H.lev.data(), H.lc, H.nlev, co, P->cg, s);
         prof_end(P, q);
     }
     for (int l = H.lc - 1; l >= 0; --l) {
@@ -436,14 +444,23 @@ static int solve(mac_plan *P, mac_stats *st)
         if ((rc = ir_norm(P, &r0))) return rc;
         r = r0;
         const Lev<float> &E = P->H32.lev[0];
-        long n0 = (long)E.nx * E.ny * E.nz;
         while (!converged(o, r, r0) && it < o.max_iter) {
             for (int c = 0; c < std::max(1, o.inner_cycles); ++c) {
                 vcycle<float>(P, P->H32, c == 0);
                 ++inner;
             }
-            launch_phi_add_f32(P->H64.lev[0].phi, E.phi, n0, P->stream);
-            if ((rc = ir_norm(P, &r))) return rc;
+            // phi64 + e32 lands in phi2 together with the next residual;
+            // the two buffers then trade names (and the device copy of the
+            // level table follows)
+            Lev<double> &L0 = P->H64.lev[0];
+            HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
+            int q = prof_begin(P, MAC_PROF_IR_UPDATE);
+            launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, P->nrm_dev, P->stream);
+            prof_end(P, q);
+            std::swap(L0.phi, L0.phi2);
+            HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &L0, sizeof(Lev<double>), hipMemcpyHostToDevice,
+                                  P->stream));
+            if ((rc = read_norm(P, &r))) return rc;
             ++it;
             if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
         }
@@ -874,8 +891,13 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
         launch_sub_mean<double>(P->H64.lev[0].rhs, n0, P->mean_dev, s);
     }
     coarsen_all<double>(P, P->H64);
+    // max |rh| (the "Initial rhs" figure of the solver log)
+    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
+    launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
     int err_host = 0;
+    unsigned long long rhs_bits = 0;
     HIPCHK(hipMemcpyAsync(&err_host, P->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&rhs_bits, P->nrm_dev, sizeof rhs_bits, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(P->ev[2], s));
     HIPCHK(hipStreamSynchronize(s));
     if (err_host) return MAC_ERR_NONFINITE;
@@ -883,6 +905,7 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     mac_stats local;
     memset(&local, 0, sizeof local);
     int src = solve(P, &local);
+    memcpy(&local.rhs_norm, &rhs_bits, sizeof(double));
     if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
     HIPCHK(hipEventRecord(P->ev[3], s));
     // ---- correction, ghost fill, phi out

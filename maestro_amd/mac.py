@@ -34,7 +34,9 @@ SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_
            "mac_project", "mac_project_level", "mac_adjoint_level", "mac_version",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
            "mac_plan_profile", "mac_plan_profile_read")
-PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5}
+PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5,
+        "gsrb64_zero": 6, "gsrb32_zero": 7, "restrict64": 8, "restrict32": 9, "prolong64": 10,
+        "prolong32": 11, "ir_update": 12}
 ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
 OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5,
       "resid_restrict": 6, "prolong_sweep": 7}
