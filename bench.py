@@ -38,10 +38,16 @@ def sweep_kernel_name(s: int) -> str:
     """Name of the finest-level sweep kernel the library launches (the
     MAC_SWEEP environment variable selects the variant)."""
     t = "double" if s == 8 else "float"
-    v = os.environ.get("MAC_SWEEP", "8")
-    if v == "0":
+    v = int(os.environ.get("MAC_SWEEP", "1082"))
+    if v >= 1000 and s == 8:
+        v = 208   # the full-row form is FP32 only; FP64 falls back to the deep-prefetch form
+    if v == 0:
         return f"k_gsrb_fused<{t}, 0>"
-    return f"k_gsrb_pair<{t}, 0, {int(v)}>"
+    if v in (8, 16):
+        return f"k_gsrb_pair<{t}, 0, {v}>"
+    if v >= 1000:
+        return f"k_gsrb_rows<{t}, 0, {(v // 10) % 100}>"
+    return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}>"
 
 
 def measured_traffic(name: str):

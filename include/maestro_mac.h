@@ -184,6 +184,9 @@ int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
                                     phi(level+1) = 0, one fused pass       */
 #define MAC_OP_PROLONG_SWEEP 7   /* phi(level) += P phi(level+1), then one
                                     sweep, as the V-cycle's first upward sweep */
+#define MAC_OP_GSRB32 8          /* `count` sweeps in FP32 with the kernels of the
+                                    mixed path: weights, rhs and phi of the level
+                                    rounded to FP32, the result widened into phi */
 int mac_debug_level_dims(mac_plan *plan, int level, int dims[3]);
 int mac_debug_get(mac_plan *plan, int level, int what, double *out);
 int mac_debug_set(mac_plan *plan, int level, int what, const double *in);

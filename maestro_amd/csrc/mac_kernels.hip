@@ -21,6 +21,7 @@
 //                      sample Exec/SCIENCE/flame/multifab_physbc_edgevel.f90:
 //                      211-321)
 #include <algorithm>
+#include <type_traits>
 
 #include "mac_launch.h"
 
@@ -263,6 +264,13 @@ __global__ void __launch_bounds__(TPB) k_weights_to_f32(Lev<double> S, Lev<float
     if (DIM == 3 && K == 0) D.bzh[I + (long)S.nx * J] = (float)S.bzh[I + (long)S.nx * J];
 }
 
+template <typename A, typename B>
+__global__ void __launch_bounds__(TPB) k_convert(const A *__restrict__ in, B *__restrict__ out, long n)
+{
+    long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) out[q] = (B)in[q];
+}
+
 // --------------------------------------------------------------- GSRB
 // one colour of a red-black sweep; thread per (i-pair, j, k)
 template <int DIM, typename T>
@@ -468,6 +476,498 @@ __global__ void __launch_bounds__(fz::NT) k_gsrb_fused(Lev<T> L, const T *__rest
         pp2 = (active && k + 3 <= ke + 1) ? load_old(col, pxy, Xw, Yw, k + 3) : T(0);
     }
 }
+
+// ------------------------------- fused red-black sweep, deep prefetch (3D)
+// The z-march of k_gsrb_fused with its global loads issued PD planes ahead
+// (face weights, rhs, ring-2 values) and the old iterate PD+1 planes ahead,
+// held in register queues.  Each step of the march waits only for loads
+// issued PD steps earlier, so a block keeps PD planes of traffic in flight.
+// One column per thread (64 x EY tile, inner 62 x (EY-2) written), 32-bit
+// indexing (levels below 2^31 cells).  MODE 0: plain; 1: zero old iterate.
+template <typename T, int MODE, int EY, int PD>
+__global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__restrict__ pin,
+                                                      T *__restrict__ pout, FusedGrid G)
+{
+    constexpr int EX = 64, FX = EX - 2, FY = EY - 2, PX = EX + 2, PY = EY + 2, NT = EX * EY;
+    constexpr int QF = PD;   // old-iterate queue: planes k+2 .. k+1+QF
+    const int b = blockIdx.x;
+    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
+    if (idx >= G.total) return;
+    const int ntile = G.ntx * G.nty;
+    const int chunk = idx / ntile, tile = idx - chunk * ntile;
+    const int tyi = tile / G.ntx, txi = tile - tyi * G.ntx;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int plane = nx * ny;
+    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+    const int tx = threadIdx.x & (EX - 1), ty = threadIdx.x >> 6;
+    const int X = txi * FX - 1 + tx, Y = tyi * FY - 1 + ty;
+    int Xw, Yw;
+    bool okx, oky;
+    wrap_coord(X, nx, L.perx, Xw, okx);
+    wrap_coord(Y, ny, L.pery, Yw, oky);
+    const bool active = okx && oky;
+    const bool outcol = tx >= 1 && tx <= FX && ty >= 1 && ty <= FY && X < nx && Y < ny;
+    const int col = Xw + nx * Yw;
+    const int pos = (tx + 1) + PX * (ty + 1);
+    int r2a = -1, r2b = -1, c2a = 0, c2b = 0;
+    if (ty == 0 || ty == EY - 1) {
+        int yw;
+        bool ok;
+        wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, L.pery, yw, ok);
+        if (ok && okx) { r2a = (tx + 1) + PX * (ty == 0 ? 0 : EY + 1); c2a = Xw + nx * yw; }
+    }
+    if (tx == 0 || tx == EX - 1) {
+        int xw;
+        bool ok;
+        wrap_coord(tx == 0 ? X - 1 : X + 1, nx, L.perx, xw, ok);
+        if (ok && oky) { r2b = (tx == 0 ? 0 : EX + 1) + PX * (ty + 1); c2b = xw + nx * Yw; }
+    }
+    __shared__ T tile_s[3][PX * PY];
+    for (int q = threadIdx.x; q < 3 * PX * PY; q += NT) (&tile_s[0][0])[q] = T(0);
+
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (L.perz) {
+            while (k < 0) k += nz;
+            while (k >= nz) k -= nz;
+            kw = k;
+            return true;
+        }
+        kw = 0;
+        return false;
+    };
+    auto ld = [&](int c, int k) -> T {   // old iterate, zero outside / in MODE 1
+        int kw;
+        if (MODE == 1 || !kwrap(k, kw)) return T(0);
+        return pin[c + plane * kw];
+    };
+    const int pxy = Xw + Yw;
+    const int kb = z0 - 1, ke = z1;
+    // old iterate of this column: pm2, pm1 (updated), p0 (plane k), pp1, then
+    // the queue fq[i] = plane k+2+i
+    T pm2 = T(0), pm1 = T(0), p0 = T(0), pp1 = T(0), fq[QF];
+    if (active) {
+        pm1 = ld(col, kb - 1);
+        p0 = ld(col, kb);
+        pp1 = ld(col, kb + 1);
+    }
+#pragma unroll
+    for (int i = 0; i < QF; ++i) fq[i] = active ? ld(col, kb + 2 + i) : T(0);
+    // per-plane inputs queue: sw/srhs/sa/sb[i] = plane k+1+i at step k
+    T cw[6], crhs = T(0), pw[6], prhs = T(0);
+    T sw[PD][6], srhs[PD], sa[PD], sb[PD];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) { cw[s] = T(0); pw[s] = T(0); }
+#pragma unroll
+    for (int i = 0; i < PD; ++i) {
+        srhs[i] = sa[i] = sb[i] = T(0);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) sw[i][s] = T(0);
+    }
+    auto fetch = [&](int k, T *nw, T &nrhs, T &na, T &nb) {
+        int kw;
+        const bool kok = kwrap(k, kw);
+        if (active && kok) {
+            const int c = col + plane * kw;
+            nw[0] = L.bx[c];
+            nw[1] = (Xw < nx - 1) ? L.bx[c + 1] : L.bxh[Yw + ny * kw];
+            nw[2] = L.by[c];
+            nw[3] = (Yw < ny - 1) ? L.by[c + nx] : L.byh[Xw + nx * kw];
+            nw[4] = L.bz[c];
+            nw[5] = (kw < nz - 1) ? L.bz[c + plane] : L.bzh[col];
+            nrhs = L.rhs[c];
+        }
+        na = (r2a >= 0) ? ld(c2a, k) : T(0);
+        nb = (r2b >= 0) ? ld(c2b, k) : T(0);
+    };
+    // prologue: plane kb goes to (cw, crhs) via the queue head
+#pragma unroll
+    for (int i = 0; i < PD; ++i)
+        if (kb + i <= ke) fetch(kb + i, sw[i], srhs[i], sa[i], sb[i]);
+    __syncthreads();
+    T *buf = tile_s[0], *pbuf = tile_s[2], *qbuf = tile_s[1];
+    for (int k = kb; k <= ke; ++k) {
+        if (k > kb) {
+            T *t = qbuf;
+            qbuf = pbuf;
+            pbuf = buf;
+            buf = t;
+        }
+        int kw;
+        const bool kok = kwrap(k, kw);
+        // advance the input queue: head -> current plane, fetch plane k+PD
+#pragma unroll
+        for (int s = 0; s < 6; ++s) { pw[s] = cw[s]; cw[s] = sw[0][s]; }
+        prhs = crhs;
+        crhs = srhs[0];
+        const T r2va = sa[0], r2vb = sb[0];
+#pragma unroll
+        for (int i = 0; i + 1 < PD; ++i) {
+#pragma unroll
+            for (int s = 0; s < 6; ++s) sw[i][s] = sw[i + 1][s];
+            srhs[i] = srhs[i + 1];
+            sa[i] = sa[i + 1];
+            sb[i] = sb[i + 1];
+        }
+        if (k + PD <= ke) fetch(k + PD, sw[PD - 1], srhs[PD - 1], sa[PD - 1], sb[PD - 1]);
+        // (a) publish the old values of plane k
+        if (active) buf[pos] = kok ? p0 : T(0);
+        if (r2a >= 0) buf[r2a] = r2va;
+        if (r2b >= 0) buf[r2b] = r2vb;
+        __syncthreads();
+        const bool red = kok && ((pxy + kw) & 1) == 0;
+        // (b) red cells of plane k (tile and ring)
+        if (active && red) {
+            T p[6] = {buf[pos - 1], buf[pos + 1], buf[pos - PX], buf[pos + PX], pm1, pp1};
+            T v = gs_value<3, T>(L, cw, p, crhs);
+            buf[pos] = v;
+            p0 = v;
+        }
+        // (c) black cells of plane k-1 (tile only), (d) write plane k-1
+        const int km = k - 1;
+        if (outcol && km >= z0 && km < z1) {
+            if ((pxy + km) & 1) {
+                T p[6] = {pbuf[pos - 1], pbuf[pos + 1], pbuf[pos - PX], pbuf[pos + PX], pm2, p0};
+                pm1 = gs_value<3, T>(L, pw, p, prhs);
+            }
+            pout[col + plane * km] = pm1;
+        }
+        // rotate the column and fetch the old iterate of plane k+2+QF
+        pm2 = pm1;
+        pm1 = p0;
+        p0 = pp1;
+        pp1 = fq[0];
+#pragma unroll
+        for (int i = 0; i + 1 < QF; ++i) fq[i] = fq[i + 1];
+        fq[QF - 1] = (active && k + 2 + QF <= ke + 1) ? ld(col, k + 2 + QF) : T(0);
+    }
+}
+
+// ------------------------------------------ fused red-black sweep, full rows
+// Each lane owns two neighbouring columns x0, x0+1 (x0 = 2*(lane + 64*w)), and
+// W waves cover a whole x-row (nx = 128*W), so there is no x-ring and every
+// lane has one red and one black cell per plane (no idle lanes in either
+// phase).  The LDS rows carry a pad column on each side: zero outside a
+// non-periodic side, the periodic image otherwise (mirrored by the end lanes).
+// EY rows per block, the first and last being the y-ring (red only, not
+// written); z-march as k_gsrb_deep with PD planes of prefetch.  Loads and
+// stores are two-element vectors.  MODE 0: plain; 1: zero old iterate.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+struct RowSweepGrid {
+    int W;         // waves per x-row
+    int nty;       // y-tiles
+    int tz;        // planes per z-chunk
+    int total;     // y-tiles x z-chunks
+    int per_xcd;   // ceil(total / 8)
+};
+
+// Load/store at a wave-uniform base plus a per-lane 32-bit byte offset,
+// through a raw buffer resource built from the base (the resource sits in
+// SGPRs and the offset is one VGPR; a plain pointer form lets the compiler
+// hoist a 64-bit per-lane address per array out of the march, which costs
+// two VGPRs per array).  The range is not limited (num_records = 2^32-1);
+// every offset used stays inside its array.
+static constexpr int BUF_CFG = 0x00020000;   // gfx9-family raw buffer dword 3
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)0xffffffff, BUF_CFG);
+}
+template <typename V>
+__device__ __forceinline__ V ld_off(const void *base, unsigned boff)
+{
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(base);
+    if constexpr (sizeof(V) == 4)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
+    else if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, boff, 0, 0));
+    else
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, boff, 0, 0));
+}
+template <typename V>
+__device__ __forceinline__ void st_off(void *base, unsigned boff, const V &v)
+{
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(base);
+    if constexpr (sizeof(V) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                              r, boff, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
+                                               r, boff, 0, 0);
+}
+
+template <typename T, int MODE, int EY>
+__global__ void __launch_bounds__(1024) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
+                                                    T *__restrict__ pout, RowSweepGrid G)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int FY = EY - 2, PY = EY + 2, NI = 13;
+    const int b = blockIdx.x;
+    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
+    if (idx >= G.total) return;
+    const int chunk = idx / G.nty, tyi = idx - chunk * G.nty;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, plane = nx * ny;
+    const int PXL = nx + 4;   // LDS row: 2 pad (x = -1 at 1), nx columns from 2, pad (x = nx)
+    const int PS = PY * PXL;                            // LDS plane
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = wv % G.W, ty = wv / G.W;              // wave-uniform
+    const int x0 = 2 * (lane + 64 * w);
+    const int Y = tyi * FY - 1 + ty;
+    int Yw;
+    bool oky;
+    wrap_coord(Y, ny, L.pery, Yw, oky);
+    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+    const bool outrow = oky && ty >= 1 && ty <= FY && Y < ny;
+    const unsigned col = (unsigned)(x0 + nx * Yw);      // per-lane offset within a plane
+    const bool ring = (ty == 0) || (ty == EY - 1);
+    int y2w = 0;
+    bool ok2 = false;
+    if (ring) wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, L.pery, y2w, ok2);
+    const unsigned col2 = (unsigned)(x0 + nx * y2w);
+    const unsigned colyn = (Yw < ny - 1) ? col + nx : (unsigned)x0;   // y-high faces: by or byh
+    const T *const byn_base = (Yw < ny - 1) ? L.by : L.byh;
+    const unsigned byn_plane = (Yw < ny - 1) ? (unsigned)plane : (unsigned)nx;
+    constexpr unsigned ES = sizeof(T);
+    const unsigned bcol = col * ES, bcol2 = col2 * ES, bcolyn = colyn * ES;   // byte offsets
+    const int lpos = (ty + 1) * PXL + 2 + x0;           // LDS index of column x0 (even)
+    const int lpos2 = (ty == 0 ? 0 : EY + 1) * PXL + 2 + x0;
+    const bool xbeg = (x0 == 0), xend = (x0 + 2 == nx), mirror = L.perx != 0;
+    extern __shared__ __align__(16) unsigned char rows_smem[];
+    T *lds = (T *)rows_smem;
+    for (int q = threadIdx.x; q < 4 * PS; q += blockDim.x) lds[q] = T(0);
+
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (L.perz) {
+            while (k < 0) k += nz;
+            while (k >= nz) k -= nz;
+            kw = k;
+            return true;
+        }
+        kw = 0;
+        return false;
+    };
+    // zero pairs are built from immediates (a zero object in memory would
+    // turn "cond ? load : zero" into a flat load through a selected pointer)
+    auto z2 = []() {
+        T2 v;
+        v.x = T(0);
+        v.y = T(0);
+        return v;
+    };
+    // old iterate of the column pair on plane k (zero outside / in MODE 1)
+    auto ldphi = [&](int k) -> T2 {
+        int kw;
+        if (MODE == 0 && oky && kwrap(k, kw)) return ld_off<T2>(pin + (size_t)plane * kw, bcol);
+        return z2();
+    };
+    // per-plane inputs: [0..2] x faces at x0, x0+1, x0+2; [3,4] y-low faces;
+    // [5,6] y-high faces; [7,8] z-low faces; [9,10] rhs; [11,12] ring-2 old
+    // values (ring waves).  The z-high faces of plane k are the z-low faces of
+    // plane k+1 (next slot of the ring), except on the domain's top plane,
+    // whose high faces come from the bzh slab.  Every entry is assigned and every load
+    // is made from an in-range address (plane 0 stands in for a plane outside
+    // a non-periodic side, row 0 for a row outside; those values are never
+    // used), so the queue stays in registers.  Addresses are a wave-uniform
+    // plane base plus the lane's 32-bit offset.
+    struct In {
+        T v[NI];
+    };
+    auto fetch = [&](int k) -> In {
+        In o;
+        int kw;
+        const bool kok = kwrap(k, kw);
+        const size_t pk = (size_t)plane * kw;
+        const T2 vx = ld_off<T2>(L.bx + pk, bcol);
+        o.v[0] = vx.x;
+        o.v[1] = vx.y;
+        o.v[2] = xend ? L.bxh[Yw + ny * kw] : ld_off<T>(L.bx + pk + 2, bcol);
+        const T2 vy = ld_off<T2>(L.by + pk, bcol);
+        o.v[3] = vy.x;
+        o.v[4] = vy.y;
+        const T2 vyn = ld_off<T2>(byn_base + (size_t)byn_plane * kw, bcolyn);
+        o.v[5] = vyn.x;
+        o.v[6] = vyn.y;
+        const T2 vz = ld_off<T2>(L.bz + pk, bcol);
+        o.v[7] = vz.x;
+        o.v[8] = vz.y;
+        const T2 vr = ld_off<T2>(L.rhs + pk, bcol);
+        o.v[9] = vr.x;
+        o.v[10] = vr.y;
+        T2 va = z2();
+        if (MODE == 0 && ring) va = ld_off<T2>(pin + pk, bcol2);
+        const bool use2 = kok && ok2;
+        o.v[11] = use2 ? va.x : T(0);
+        o.v[12] = use2 ? va.y : T(0);
+        return o;
+    };
+    const int kb = z0 - 1, ke = z1;
+    const int nsteps = ((ke - kb + 1) + 7) & ~7;   // a multiple of the unroll
+    // rings indexed by the step m = k - kb: inputs of plane m in wq[m & 3],
+    // old/updated iterate of plane m in ph[m & 7], LDS plane m in buffer m & 3
+    In wq[4];
+    struct Ring {
+        T v[16];
+    } phr;
+#define PH(slot, c) phr.v[2 * (slot) + (c)]
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < NI; ++t) wq[s].v[t] = T(0);
+    // planes kb-1 .. kb+5 into slots 7, 0 .. 5 (written out: a loop here may
+    // stay rolled and turn the ring into a stack array)
+    auto put = [&](auto SC, const T2 &e) __attribute__((always_inline)) {
+        constexpr int S = decltype(SC)::value;
+        PH(S, 0) = e.x;
+        PH(S, 1) = e.y;
+    };
+    put(std::integral_constant<int, 7>{}, ldphi(kb - 1));
+    put(std::integral_constant<int, 0>{}, ldphi(kb));
+    put(std::integral_constant<int, 1>{}, ldphi(kb + 1));
+    put(std::integral_constant<int, 2>{}, ldphi(kb + 2));
+    put(std::integral_constant<int, 3>{}, ldphi(kb + 3));
+    put(std::integral_constant<int, 4>{}, ldphi(kb + 4));
+    put(std::integral_constant<int, 5>{}, ldphi(kb + 5));
+    PH(6, 0) = PH(6, 1) = T(0);
+    wq[0] = fetch(kb);
+    wq[1] = fetch(min(kb + 1, ke + 1));
+    __syncthreads();
+
+    auto step = [&](auto SC, int m) __attribute__((always_inline)) {
+        constexpr int S = decltype(SC)::value;
+        constexpr int C0 = S & 3, CP = (S + 3) & 3, C1 = (S + 1) & 3, CN = (S + 2) & 3;   // m, m-1, m+1, m+2
+        constexpr int F0 = S, FM1 = (S + 7) & 7, FM2 = (S + 6) & 7, FP1 = (S + 1) & 7;
+        const int k = kb + m;
+        T *buf = lds + C0 * PS, *pbuf = lds + CP * PS;
+        int kw;
+        const bool kok = kwrap(k, kw) && k <= ke;
+        wq[CN] = fetch(min(k + 2, ke + 1));
+        // (a) publish the old values of plane k (and the ring-2 row)
+        if (oky) {
+            T2 v;
+            v.x = kok ? PH(F0, 0) : T(0);
+            v.y = kok ? PH(F0, 1) : T(0);
+            *(T2 *)(buf + lpos) = v;
+            if (mirror && xbeg) buf[lpos + nx] = v.x;        // pad x = nx <- column 0
+            if (mirror && xend) buf[lpos - nx + 1] = v.y;    // pad x = -1 <- column nx-1
+        }
+        if (ring) {
+            T2 v;
+            v.x = kok ? wq[C0].v[11] : T(0);
+            v.y = kok ? wq[C0].v[12] : T(0);
+            *(T2 *)(buf + lpos2) = v;
+            if (mirror && xbeg) buf[lpos2 + nx] = v.x;
+            if (mirror && xend) buf[lpos2 - nx + 1] = v.y;
+        }
+        __syncthreads();
+        // r is uniform over the wave (one row, one plane): each colour case
+        // is its own straight-line code behind a scalar branch.  The parity
+        // uses k itself (a plane outside a non-periodic side has kw = 0; a
+        // periodic extent is even, so kw and k agree in parity).
+        const int r = (Yw + k) & 1;
+        const int km = k - 1;
+        const bool do_red = kok && oky, do_black = outrow && km >= z0 && km < z1;
+        // z-high faces of plane k (zh) and plane k-1 (zhb): the next plane's
+        // z-low faces, or the bzh slab on the domain's top plane
+        T zh0, zh1, zb0, zb1;
+        if (kw == nz - 1) {
+            const T2 t = ld_off<T2>(L.bzh, bcol);
+            zh0 = t.x;
+            zh1 = t.y;
+        } else {
+            zh0 = wq[C1].v[7];
+            zh1 = wq[C1].v[8];
+        }
+        if (km == nz - 1) {
+            const T2 t = ld_off<T2>(L.bzh, bcol);
+            zb0 = t.x;
+            zb1 = t.y;
+        } else {
+            zb0 = wq[C0].v[7];
+            zb1 = wq[C0].v[8];
+        }
+        // the ring values this step reads, as values: the colour branches
+        // below then touch no ring slot, so the ring stays in registers (a
+        // slot chosen per branch would be merged into a pointer select)
+        const T f0a = PH(F0, 0), f0b = PH(F0, 1), m1a = PH(FM1, 0), m1b = PH(FM1, 1);
+        const T p1a = PH(FP1, 0), p1b = PH(FP1, 1), m2a = PH(FM2, 0), m2b = PH(FM2, 1);
+        T red_v = T(0), blk_v = T(0);
+        auto update = [&](auto RC) __attribute__((always_inline)) {
+            constexpr int R = decltype(RC)::value;
+            const In &cw = wq[C0], &pw = wq[CP];
+            T f0R = R ? f0b : f0a;   // column R of plane k, after the red update
+            if (do_red) {
+                // (b) the red cell of plane k: column R
+                const T wt[6] = {cw.v[R], cw.v[R + 1], cw.v[3 + R], cw.v[5 + R], cw.v[7 + R], R ? zh1 : zh0};
+                T xm, xp;
+                if constexpr (R == 1) {
+                    xm = f0a;
+                    xp = buf[lpos + 2];
+                } else {
+                    xm = buf[lpos - 1];
+                    xp = f0b;
+                }
+                const T nb[6] = {xm, xp, buf[lpos - PXL + R], buf[lpos + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
+                red_v = gs_value<3, T>(L, wt, nb, cw.v[9 + R]);
+                buf[lpos + R] = red_v;
+                f0R = red_v;
+                if (R == 0 && mirror && xbeg) buf[lpos + nx] = red_v;
+                if (R == 1 && mirror && xend) buf[lpos - nx + 1] = red_v;
+            }
+            if (do_black) {
+                // (c) the black cell of plane k-1 (same column R); the other
+                // column is red there; (d) write plane k-1
+                const T wt[6] = {pw.v[R], pw.v[R + 1], pw.v[3 + R], pw.v[5 + R], pw.v[7 + R], R ? zb1 : zb0};
+                T xm, xp;
+                if constexpr (R == 1) {
+                    xm = m1a;
+                    xp = pbuf[lpos + 2];
+                } else {
+                    xm = pbuf[lpos - 1];
+                    xp = m1b;
+                }
+                const T nb[6] = {xm, xp, pbuf[lpos - PXL + R], pbuf[lpos + PXL + R], R ? m2b : m2a, f0R};
+                blk_v = gs_value<3, T>(L, wt, nb, pw.v[9 + R]);
+                T2 o;
+                o.x = R ? m1a : blk_v;
+                o.y = R ? blk_v : m1b;
+                st_off<T2>(pout + (size_t)plane * km, bcol, o);
+            }
+        };
+        if (r) update(std::integral_constant<int, 1>{});
+        else update(std::integral_constant<int, 0>{});
+        // write the updated values back to their (fixed) ring slots
+        if (do_red) {
+            PH(F0, 0) = r ? f0a : red_v;
+            PH(F0, 1) = r ? red_v : f0b;
+        }
+        if (do_black) {
+            PH(FM1, 0) = r ? m1a : blk_v;
+            PH(FM1, 1) = r ? blk_v : m1b;
+        }
+        // the slot of plane k-2 takes the old iterate of plane k+6
+        {
+            const T2 e = (k + 6 <= ke + 1) ? ldphi(k + 6) : z2();
+            PH(FM2, 0) = e.x;
+            PH(FM2, 1) = e.y;
+        }
+    };
+    for (int m = 0; m < nsteps; m += 8) {
+        step(std::integral_constant<int, 0>{}, m);
+        step(std::integral_constant<int, 1>{}, m + 1);
+        step(std::integral_constant<int, 2>{}, m + 2);
+        step(std::integral_constant<int, 3>{}, m + 3);
+        step(std::integral_constant<int, 4>{}, m + 4);
+        step(std::integral_constant<int, 5>{}, m + 5);
+        step(std::integral_constant<int, 6>{}, m + 6);
+        step(std::integral_constant<int, 7>{}, m + 7);
+    }
+}
+#undef PH
 
 // ------------------------------------- fused red-black sweep, x-pairs (3D)
 // Same algorithm as k_gsrb_fused, but each thread owns two neighbouring
@@ -721,7 +1221,11 @@ __device__ __forceinline__ void block_max_atomic(double a, unsigned long long *n
 // Grid-stride walk over the x-rows (j,k) of an nx x ny x nz level: each
 // wave takes whole rows and its lanes stride x.  The row index is
 // wave-uniform, so its division into (j,k) runs once per wave and row on the
-// scalar unit instead of a 64-bit divide per cell.
+// scalar unit instead of a 64-bit divide per cell.  Consecutive blocks take
+// consecutive row groups; with the round-robin block-to-XCD dispatch the z
+// neighbour rows (a multiple of 8 blocks away for ny a multiple of 32) land
+// on the same XCD.  (Giving each XCD one contiguous slab of rows measured
+// 16% slower on the 256^3 residual.)
 static constexpr int ROW_WAVES = TPB / 64;
 
 template <typename F>
@@ -737,6 +1241,7 @@ __device__ __forceinline__ void for_rows(int nx, int ny, int nz, F &&f)
     }
 }
 
+// grid for for_rows: one wave per row
 static inline unsigned row_blocks(int ny, int nz)
 {
     return (unsigned)(((long)ny * nz + ROW_WAVES - 1) / ROW_WAVES);
@@ -1591,6 +2096,14 @@ void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, h
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_weights_to_f32<D_>, dim3(nblk(n)), dim3(TPB), 0, s, S, D));
 }
 
+template <typename A, typename B>
+void launch_convert(const A *in, B *out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert<A, B>), dim3(nblk(n)), dim3(TPB), 0, s, in, out, n);
+}
+template void launch_convert<double, float>(const double *, float *, long, hipStream_t);
+template void launch_convert<float, double>(const float *, double *, long, hipStream_t);
+
 template <typename T>
 void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
 {
@@ -1618,17 +2131,126 @@ static void launch_pair(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
         hipLaunchKernelGGL((k_gsrb_pair<T, 0, EY>), g, b, 0, s, L, pin, pout, G);
 }
 
+template <typename T, int EY, int PD>
+static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
+{
+    FusedGrid G;
+    G.ntx = (L.nx + 62 - 1) / 62;
+    G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
+    const int tiles = G.ntx * G.nty;
+    int zc = (2048 + tiles - 1) / tiles;
+    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
+    G.tz = (L.nz + zc - 1) / zc;
+    zc = (L.nz + G.tz - 1) / G.tz;
+    G.total = tiles * zc;
+    G.per_xcd = (G.total + 7) / 8;
+    dim3 g(8 * G.per_xcd), b(64 * EY);
+    if (mode == SWEEP_ZERO)
+        hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD>), g, b, 0, s, L, pin, pout, G);
+    else
+        hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD>), g, b, 0, s, L, pin, pout, G);
+}
+
+// Full-row sweep: usable when nx = 128 W and a block of EY rows (64 W EY
+// threads) fits in 1024.  The z-chunk count is chosen so the number of
+// blocks fills whole rounds of the resident block slots (CUs x blocks per
+// CU) with the least z-ring overhead.
+template <typename T, int EY>
+static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
+{
+    // FP32 only: the FP64 form needs more than the 128 VGPRs a 1024-thread
+    // block allows and spills
+    if constexpr (sizeof(T) == 8) {
+        return false;
+    } else {
+    if (L.nx % 128 != 0) return false;
+    const int W = L.nx / 128;
+    if (64 * W * EY > 1024 || L.nz < 2) return false;
+    const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
+    if (lds > 64 * 1024) return false;
+    const int threads = 64 * W * EY;
+    static int slots_cache[2][17] = {{0}};   // [mode][W] resident blocks per device
+    int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0][std::min(W, 16)];
+    if (slots == 0) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return false;
+        hipError_t e = (mode == SWEEP_ZERO)
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY>, threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY>, threads, lds);
+        if (e != hipSuccess || per < 1) return false;
+        slots = cus * per;
+    }
+    RowSweepGrid G;
+    G.W = W;
+    G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
+    static int zc_env = -2;
+    if (zc_env == -2) {
+        const char *e = getenv("MAC_ROWS_ZC");
+        zc_env = e ? atoi(e) : -1;
+    }
+    int best_zc = 1;
+    double best = -1.0;
+    for (int zc = 1; zc <= std::max(1, L.nz / 4); ++zc) {
+        const int tz = (L.nz + zc - 1) / zc;
+        const int nzc = (L.nz + tz - 1) / tz;
+        const long blocks = (long)G.nty * nzc;
+        const long rounds = (blocks + slots - 1) / slots;
+        const int steps = ((tz + 2) + 7) & ~7;   // the march runs in groups of 8 planes
+        const double eff = (double)blocks / (double)(rounds * slots) * (double)tz / (double)steps;
+        if (eff > best + 1e-9) { best = eff; best_zc = zc; }
+    }
+    if (zc_env > 0) best_zc = std::min(zc_env, L.nz);
+    G.tz = (L.nz + best_zc - 1) / best_zc;
+    const int nzc = (L.nz + G.tz - 1) / G.tz;
+    G.total = G.nty * nzc;
+    G.per_xcd = (G.total + 7) / 8;
+    dim3 g(8 * G.per_xcd), b(threads);
+    if (mode == SWEEP_ZERO)
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY>), g, b, lds, s, L, pin, pout, G);
+    else
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY>), g, b, lds, s, L, pin, pout, G);
+    return true;
+    }
+}
+
 template <typename T>
 void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
                        const EllBC &bc, hipStream_t s)
 {
-    static int variant = -1;   // 0: one column per thread, 8/16: x-pairs with 8/16 rows
+    // 0: one column per thread; 8/16: x-pairs with 8/16 rows; 100*PD + EY:
+    // one column per thread with PD planes of prefetch and EY rows;
+    // 1000 + 10*EY + PD: full rows (falls back to 208 where it does not fit)
+    static int variant = -1;
     if (variant < 0) {
         const char *e = getenv("MAC_SWEEP");
-        variant = e ? atoi(e) : 8;
+        variant = e ? atoi(e) : 1082;
     }
-    if (mode != SWEEP_PROLONG && variant == 8) { launch_pair<T, 8>(L, pin, pout, mode, s); return; }
-    if (mode != SWEEP_PROLONG && variant == 16) { launch_pair<T, 16>(L, pin, pout, mode, s); return; }
+    if (mode != SWEEP_PROLONG && variant >= 1000) {
+        bool done = false;
+        switch (variant) {
+        case 1082: done = launch_rows<T, 8>(L, pin, pout, mode, s); break;
+        case 1042: done = launch_rows<T, 4>(L, pin, pout, mode, s); break;
+        default: break;
+        }
+        if (done) return;
+        launch_deep<T, 8, 2>(L, pin, pout, mode, s);
+        return;
+    }
+    if (mode != SWEEP_PROLONG) {
+        switch (variant) {
+        case 8: launch_pair<T, 8>(L, pin, pout, mode, s); return;
+        case 16: launch_pair<T, 16>(L, pin, pout, mode, s); return;
+        case 208: launch_deep<T, 8, 2>(L, pin, pout, mode, s); return;
+        case 308: launch_deep<T, 8, 3>(L, pin, pout, mode, s); return;
+        case 408: launch_deep<T, 8, 4>(L, pin, pout, mode, s); return;
+        case 304: launch_deep<T, 4, 3>(L, pin, pout, mode, s); return;
+        case 404: launch_deep<T, 4, 4>(L, pin, pout, mode, s); return;
+        case 604: launch_deep<T, 4, 6>(L, pin, pout, mode, s); return;
+        default: break;
+        }
+    }
     FusedGrid G;
     G.ntx = (L.nx + fz::FX - 1) / fz::FX;
     G.nty = (L.ny + fz::FY - 1) / fz::FY;

@@ -97,6 +97,7 @@ template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
                                unsigned long long *nrm, hipStream_t s);
+template <typename A, typename B> void launch_convert(const A *in, B *out, long n, hipStream_t s);
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s);

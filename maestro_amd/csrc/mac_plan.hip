@@ -1139,6 +1139,30 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
     const Lev<double> &L = H.lev[level];
     hipStream_t s = P->stream;
     bool has_next = level + 1 < H.nlev;
+    if (op == MAC_OP_GSRB32) {
+        int rc;
+        if ((rc = build_hier<float>(P, P->H32))) return rc;
+        const Lev<float> &F = P->H32.lev[level];
+        const long n = (long)L.nx * L.ny * L.nz;
+        launch_weights_to_f32(P->dim, L, F, s);
+        launch_convert<double, float>(L.phi, F.phi, n, s);
+        launch_convert<double, float>(L.rhs, F.rhs, n, s);
+        int cur = 0;
+        for (int c = 0; c < count; ++c) {
+            if (use_fused(P)) {
+                launch_gsrb_fused<float>(F, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, nullptr,
+                                         P->bc, s);
+                cur ^= 1;
+            } else {
+                launch_gsrb<float>(P->dim, F, 0, s);
+                launch_gsrb<float>(P->dim, F, 1, s);
+            }
+        }
+        launch_convert<float, double>(cur ? F.phi2 : F.phi, L.phi, n, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        return MAC_OK;
+    }
     if (op == MAC_OP_GSRB && use_fused(P)) {   // sweeps alternate phi <-> phi2
         for (int c = 0; c < count; ++c)
             launch_gsrb_fused<double>(L, (c & 1) ? L.phi2 : L.phi, (c & 1) ? L.phi : L.phi2,

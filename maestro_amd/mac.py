@@ -39,7 +39,7 @@ PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coars
         "prolong32": 11, "ir_update": 12}
 ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
 OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5,
-      "resid_restrict": 6, "prolong_sweep": 7}
+      "resid_restrict": 6, "prolong_sweep": 7, "gsrb32": 8}
 
 
 class MacError(RuntimeError):

@@ -814,3 +814,56 @@ int orc_num_threads(void)
     return 1;
 #endif
 }
+
+/* ------------------------------------------ FP32 sweep, folded weights */
+
+/* One red-black Gauss-Seidel sweep in single precision on the solver's
+ * folded-weight form of a level (the form the GPU library stores): bx, by,
+ * bz hold the low face of every cell with the boundary conditions already
+ * folded in (Neumann 0, Dirichlet 2 beta); bxh (ny*nz), byh (nx*nz) and bzh
+ * (nx*ny) hold the domain's high faces; a neighbour outside a non-periodic
+ * side has value 0, a periodic one is the image.  Evaluation order is that
+ * of cell_gs, in fmaf.  It checks the FP32 smoother kernels of the mixed
+ * path; its own pin is the FP64 sweep orc_gsrb (agreement to FP32 rounding
+ * on the same level, tests/test_oracle.py). */
+void orc_gsrb_folded_f32(int dim, const int *n, const int *per, const float *ih2,
+                         const float *bx, const float *by, const float *bz,
+                         const float *bxh, const float *byh, const float *bzh,
+                         float *phi, const float *rhs)
+{
+    const int nx = n[0], ny = n[1], nz = (dim == 3) ? n[2] : 1;
+    const long sy = nx, sz = (long)nx * ny;
+    for (int color = 0; color < 2; ++color) {
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = (j + k + color) & 1; i < nx; i += 2) {
+                    const long c = i + sy * j + sz * k;
+                    float w[6], p[6];
+                    w[0] = bx[c];
+                    w[1] = (i < nx - 1) ? bx[c + 1] : bxh[j + (long)ny * k];
+                    w[2] = by[c];
+                    w[3] = (j < ny - 1) ? by[c + sy] : byh[i + (long)nx * k];
+                    p[0] = (i > 0) ? phi[c - 1] : (per[0] ? phi[c + nx - 1] : 0.0f);
+                    p[1] = (i < nx - 1) ? phi[c + 1] : (per[0] ? phi[c - (nx - 1)] : 0.0f);
+                    p[2] = (j > 0) ? phi[c - sy] : (per[1] ? phi[c + (long)(ny - 1) * sy] : 0.0f);
+                    p[3] = (j < ny - 1) ? phi[c + sy] : (per[1] ? phi[c - (long)(ny - 1) * sy] : 0.0f);
+                    float tx = fmaf(w[0], p[0], w[1] * p[1]), sx = w[0] + w[1];
+                    float ty = fmaf(w[2], p[2], w[3] * p[3]), sy2 = w[2] + w[3];
+                    float num, den;
+                    if (dim == 3) {
+                        w[4] = bz[c];
+                        w[5] = (k < nz - 1) ? bz[c + sz] : bzh[i + (long)nx * j];
+                        p[4] = (k > 0) ? phi[c - sz] : (per[2] ? phi[c + (long)(nz - 1) * sz] : 0.0f);
+                        p[5] = (k < nz - 1) ? phi[c + sz] : (per[2] ? phi[c - (long)(nz - 1) * sz] : 0.0f);
+                        float tz = fmaf(w[4], p[4], w[5] * p[5]), szz = w[4] + w[5];
+                        num = fmaf(ih2[0], tx, fmaf(ih2[1], ty, fmaf(ih2[2], tz, rhs[c])));
+                        den = fmaf(ih2[0], sx, fmaf(ih2[1], sy2, ih2[2] * szz));
+                    } else {
+                        num = fmaf(ih2[0], tx, fmaf(ih2[1], ty, rhs[c]));
+                        den = fmaf(ih2[0], sx, ih2[1] * sy2);
+                    }
+                    phi[c] = num / den;
+                }
+    }
+}

@@ -83,6 +83,10 @@ void orc_apply(const orc_grid *g, const orc_level *lev, const double *phi,
 double orc_residual(const orc_grid *g, const orc_level *lev, const double *phi,
                     const double *rhs, double *res);
 void orc_gsrb(const orc_grid *g, orc_level *lev);
+void orc_gsrb_folded_f32(int dim, const int *n, const int *per, const float *ih2,
+                         const float *bx, const float *by, const float *bz,
+                         const float *bxh, const float *byh, const float *bzh,
+                         float *phi, const float *rhs);
 
 void orc_level_restrict(const orc_grid *g, const orc_level *f, orc_level *c);
 void orc_level_prolong(const orc_grid *g, const orc_level *c, orc_level *f);

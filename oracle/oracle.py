@@ -80,6 +80,9 @@ def lib():
         L.orc_residual.restype = C.c_double
         L.orc_residual.argtypes = [C.POINTER(Grid), C.POINTER(Level), dp, dp, dp]
         L.orc_gsrb.argtypes = [C.POINTER(Grid), C.POINTER(Level)]
+        fp = C.POINTER(C.c_float)
+        ip = C.POINTER(C.c_int)
+        L.orc_gsrb_folded_f32.argtypes = [C.c_int, ip, ip, fp, fp, fp, fp, fp, fp, fp, fp, fp]
         L.orc_mg_nlevels.restype = C.c_int
         L.orc_mg_nlevels.argtypes = [C.POINTER(Grid), C.c_int]
         L.orc_mg_create.restype = C.c_void_p
@@ -220,6 +223,27 @@ def gsrb(g, b, phi, rhs, sweeps=1):
     for _ in range(sweeps):
         lib().orc_gsrb(C.byref(g), C.byref(lv.L))
     return lv.phi
+
+
+def gsrb_folded_f32(dim, n, per, ih2, w, phi, rhs, sweeps=1):
+    """Red-black sweeps in FP32 on folded weights ``w`` (dict of bx, by, bz,
+    bxh, byh, bzh float32 arrays as the GPU library stores a level); returns
+    the new phi (float32, shape of ``phi``)."""
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    fptr = lambda a: None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+    nn = (C.c_int * 3)(*(list(n) + [1] * (3 - len(n))))
+    pp = (C.c_int * 3)(*(list(per) + [0] * (3 - len(per))))
+    hh = (C.c_float * 3)(*(list(ih2) + [0.0] * (3 - len(ih2))))
+    arrs = {k: f32(w[k]) for k in ("bx", "by", "bxh", "byh")}
+    for k in ("bz", "bzh"):
+        arrs[k] = f32(w[k]) if dim == 3 else None
+    out = np.array(phi, dtype=np.float32, copy=True, order="C")
+    r = f32(rhs)
+    for _ in range(sweeps):
+        lib().orc_gsrb_folded_f32(dim, nn, pp, hh, fptr(arrs["bx"]), fptr(arrs["by"]), fptr(arrs["bz"]),
+                                  fptr(arrs["bxh"]), fptr(arrs["byh"]), fptr(arrs["bzh"]), fptr(out),
+                                  fptr(r))
+    return out
 
 
 def mg_solve(g, b, phi, rh, opts=None):

@@ -72,3 +72,33 @@ def rel_err(a, b, remove_mean=False):
 
 def max_rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def ell(phys):
+    return -1 if phys == W.PERIODIC else (1 if phys == W.OUTLET else 2)
+
+
+def fold(bc, n, b_raw, dim):
+    """Oracle raw face coefficients -> the library's folded low-face arrays
+    and high-face slabs (Neumann 0, Dirichlet 2 beta)."""
+    out = {}
+    names = ["bx", "by", "bz"]
+    for d in range(dim):
+        ax = 2 - d
+        raw = b_raw[d]
+        lo = np.take(raw, np.arange(n[d]), axis=ax).copy()
+        hi = np.take(raw, [n[d]], axis=ax).copy()
+        k0, k1 = ell(bc[d][0]), ell(bc[d][1])
+        sl = [slice(None)] * 3
+        sl[ax] = 0
+        if k0 == 2:
+            lo[tuple(sl)] = 0.0
+        elif k0 == 1:
+            lo[tuple(sl)] *= 2.0
+        if k1 == 2:
+            hi[...] = 0.0
+        elif k1 == 1:
+            hi *= 2.0
+        out[names[d]] = lo.ravel()
+        out[names[d] + "h"] = np.squeeze(hi, axis=ax).ravel()
+    return out

@@ -8,6 +8,8 @@ import pytest
 from maestro_amd import mac
 from maestro_amd import workloads as W
 
+from .helpers import ell, fold
+
 pytestmark = pytest.mark.gpu
 
 P, WALL, OUT = W.PERIODIC, W.SLIP_WALL, W.OUTLET
@@ -18,37 +20,7 @@ BCS_3D = [
     [[P, P], [WALL, OUT], [OUT, WALL]],
     [[WALL, WALL], [WALL, WALL], [WALL, WALL]],
 ]
-SIZES_3D = [(32, 16, 16), (64, 48, 40), (128, 12, 8)]
-
-
-def ell(phys):
-    return -1 if phys == P else (1 if phys == OUT else 2)
-
-
-def fold(bc, n, b_raw, dim):
-    """Oracle raw face coefficients -> the library's folded low-face arrays
-    and high-face slabs (Neumann 0, Dirichlet 2 beta)."""
-    out = {}
-    names = ["bx", "by", "bz"]
-    for d in range(dim):
-        ax = 2 - d
-        raw = b_raw[d]
-        lo = np.take(raw, np.arange(n[d]), axis=ax).copy()
-        hi = np.take(raw, [n[d]], axis=ax).copy()
-        k0, k1 = ell(bc[d][0]), ell(bc[d][1])
-        sl = [slice(None)] * 3
-        sl[ax] = 0
-        if k0 == 2:
-            lo[tuple(sl)] = 0.0
-        elif k0 == 1:
-            lo[tuple(sl)] *= 2.0
-        if k1 == 2:
-            hi[...] = 0.0
-        elif k1 == 1:
-            hi *= 2.0
-        out[names[d]] = lo.ravel()
-        out[names[d] + "h"] = np.squeeze(hi, axis=ax).ravel()
-    return out
+SIZES_3D = [(32, 16, 16), (64, 48, 40), (128, 12, 8), (128, 12, 9)]
 
 
 def make_plan(n, bc, dx=None, **opts):
@@ -101,6 +73,52 @@ def test_gsrb_sweep_bitwise_2d(gpu, oracle_lib, bc):
     plan.debug_op(0, "gsrb", 1)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
     ref = O.gsrb(g, b, phi, rhs, sweeps=1)
+    assert np.array_equal(got, ref)
+
+
+def fp32_reference(O, n, bc, b, phi, rhs, sweeps):
+    """The oracle's FP32 sweep on the level rounded to FP32 as the library
+    rounds it (weights, rhs, phi and 1/h^2 each cast from FP64)."""
+    dim = len(n)
+    w = {k: np.asarray(v, dtype=np.float32) for k, v in fold(bc, n, b, dim).items()}
+    ih2 = [np.float32(1.0 / (1.0 / n[0]) ** 2)] * dim
+    per = [1 if bc[d][0] == P else 0 for d in range(dim)]
+    out = O.gsrb_folded_f32(dim, n, per, ih2, w, phi.astype(np.float32), rhs.astype(np.float32),
+                            sweeps=sweeps)
+    return out.astype(np.float64)
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (64, 48, 40), (128, 30, 6), (128, 12, 9)])
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_gsrb32_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
+    """The FP32 smoother of the mixed path (full-row kernel where nx is a
+    multiple of 128, the one-column kernel elsewhere) against the oracle's
+    FP32 sweep, bit for bit."""
+    O = oracle_lib
+    rng = np.random.default_rng(21)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_op(0, "gsrb32", sweeps)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL]], [[OUT, WALL], [P, P]]])
+def test_gsrb32_sweep_bitwise_2d(gpu, oracle_lib, bc):
+    O = oracle_lib
+    n = (48, 32)
+    rng = np.random.default_rng(23)
+    g = O.make_grid(n, [1.0 / n[0]] * 2, bc)
+    b, phi, rhs = rand_level(g, n, 2, rng, O)
+    plan = make_plan(n, bc)
+    push_level(plan, 0, bc, n, b, phi, rhs, 2)
+    plan.debug_op(0, "gsrb32", 1)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    ref = fp32_reference(O, n, bc, b, phi, rhs, 1)
     assert np.array_equal(got, ref)
 
 

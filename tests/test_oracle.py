@@ -132,6 +132,39 @@ def test_gsrb_fixed_point_and_black_residual(oracle_lib):
     assert np.abs(r[black]).max() <= 1e-12 * np.abs(r).max()
 
 
+@pytest.mark.parametrize("n,bc", [
+    ((16, 12, 10), [[P, P], [P, P], [WALL, WALL]]),
+    ((12, 10, 8), [[OUT, OUT], [WALL, OUT], [P, P]]),
+    ((16, 12), [[P, P], [WALL, OUT]]),
+])
+def test_fp32_folded_sweep_matches_fp64_sweep(oracle_lib, n, bc):
+    """The FP32 folded-weight sweep (the checker of the FP32 kernels) is the
+    FP64 sweep up to single-precision rounding: same colours, neighbours,
+    boundary folding and wrap.  A wrong neighbour, weight or boundary value
+    would show as an O(1) difference."""
+    from .helpers import fold
+    O = oracle_lib
+    dim = len(n)
+    g = O.make_grid(n, (1.0 / n[0],) * dim, bc)
+    rng = np.random.default_rng(8)
+    b = [rng.uniform(0.5, 2, O.face_shape(g, d)) for d in range(dim)]
+    phi = rng.standard_normal(O.cell_shape(g))
+    rhs = rng.standard_normal(O.cell_shape(g))
+    ref = O.gsrb(g, b, phi, rhs, sweeps=2)
+    w = fold(bc, n, b, dim)
+    per = [1 if bc[d][0] == P else 0 for d in range(dim)]
+    ih2 = [1.0 / (1.0 / n[0]) ** 2] * dim
+    got = O.gsrb_folded_f32(dim, n, per, ih2, w, phi, rhs, sweeps=2).astype(np.float64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 2e-6, err
+    # a planted mistake in the checker's input (one x-face weight) is seen
+    w_bad = dict(w)
+    w_bad["bx"] = w["bx"].copy()
+    w_bad["bx"][len(w_bad["bx"]) // 2 + 1] *= 1.5
+    bad = O.gsrb_folded_f32(dim, n, per, ih2, w_bad, phi, rhs, sweeps=2).astype(np.float64)
+    assert np.abs(bad - ref).max() / np.abs(ref).max() > 1e-4
+
+
 def test_restriction_conserves_and_prolongation_exact(oracle_lib):
     O = oracle_lib
     n = (8, 8, 8)
