@@ -1307,6 +1307,200 @@ __global__ void __launch_bounds__(TPB) k_ir_update_residual(Lev<double> L, const
     block_max_atomic(a, nrm);
 }
 
+// ---------------------------------------- FP64 residual passes, row march
+// The finest-level FP64 residual passes as one wave per (x-row segment,
+// z-chunk): each lane owns two neighbouring columns and marches z, keeping
+// the plane below, the plane and the plane above in registers, so every
+// array is read from HBM once; the y- and x-neighbour values come from the
+// neighbouring rows and lanes' loads (L1/L2).  Loads are two-element vectors
+// at a uniform plane base plus a lane offset.  Needs 3D and nx = 128 W.
+//   MODE 0: max |rhs - A phi|
+//   MODE 1: r = rhs - A phi written as FP32 (the refinement's right side)
+//   MODE 2: phi' = phi + e (e FP32) formed on the fly for the cell and its
+//           neighbours, phi' written to phi_out, r = rhs - A phi' as FP32
+// The arithmetic per cell is that of k_residual / k_ir_residual /
+// k_ir_update_residual (same operand order), so results are identical.
+struct RowMarchGrid {
+    int W;         // waves per x-row
+    int tz;        // planes per z-chunk
+    int total;     // waves in all: ny * W * chunks
+    int per_xcd;   // blocks per XCD
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *__restrict__ e,
+                                                   double *__restrict__ phi_out, float *__restrict__ r32,
+                                                   unsigned long long *nrm, RowMarchGrid G)
+{
+    using D2 = double2;
+    using F2 = float2;
+    constexpr bool HAS_E = MODE == 2;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int bidx = (blockIdx.x & 7) * G.per_xcd + (blockIdx.x >> 3);   // XCD-contiguous
+    const int gw = bidx * 4 + wv;
+    double a = 0.0;
+    if (gw < G.total) {
+        const int nx = L.nx, ny = L.ny, nz = L.nz, plane = nx * ny;
+        const int seg = gw % G.W, t = gw / G.W;
+        const int j = t % ny, chunk = t / ny;
+        const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+        const int x0 = 2 * (lane + 64 * seg);
+        // neighbour rows / columns: index in range, plus whether it exists
+        const bool okym = j > 0 || L.pery, okyp = j < ny - 1 || L.pery;
+        const int jm = j > 0 ? j - 1 : ny - 1, jp = j < ny - 1 ? j + 1 : 0;
+        const bool okxm = x0 > 0 || L.perx, okxp = x0 + 2 < nx || L.perx;
+        const int xm = x0 > 0 ? x0 - 1 : nx - 1, xp = x0 + 2 < nx ? x0 + 2 : 0;
+        const unsigned oc = (unsigned)(x0 + nx * j), oym = (unsigned)(x0 + nx * jm), oyp = (unsigned)(x0 + nx * jp);
+        const unsigned oxm = (unsigned)(xm + nx * j), oxp = (unsigned)(xp + nx * j);
+        const unsigned obyh = (unsigned)x0;   // byh row offset within its plane slab
+        auto kwrap = [&](int k, int &kw) -> bool {
+            if (k >= 0 && k < nz) { kw = k; return true; }
+            if (L.perz) { kw = k < 0 ? k + nz : k - nz; return true; }
+            kw = 0;
+            return false;
+        };
+        struct Z {   // column-pair values of one plane (old iterate, correction, z-low faces)
+            double p0, p1, b0, b1;
+            float e0, e1;
+        };
+        auto loadz = [&](int k) -> Z {
+            Z z;
+            int kw;
+            const bool ok = kwrap(k, kw);
+            const size_t pk = (size_t)plane * kw;
+            const D2 p = ld_off<D2>(L.phi + pk, oc * 8u);
+            const D2 bz = ld_off<D2>(L.bz + pk, oc * 8u);
+            z.p0 = ok ? p.x : 0.0;
+            z.p1 = ok ? p.y : 0.0;
+            z.b0 = bz.x;
+            z.b1 = bz.y;
+            if constexpr (HAS_E) {
+                const F2 v = ld_off<F2>(e + pk, oc * 4u);
+                z.e0 = ok ? v.x : 0.0f;
+                z.e1 = ok ? v.y : 0.0f;
+            } else {
+                z.e0 = z.e1 = 0.0f;
+            }
+            return z;
+        };
+        struct P {   // in-plane inputs of one plane
+            double ym0, ym1, yp0, yp1, xm, xp;
+            float eym0, eym1, eyp0, eyp1, exm, exp;
+            double bx0, bx1, bx2, by0, by1, byn0, byn1, r0, r1;
+        };
+        auto loadp = [&](int k) -> P {
+            P q;
+            const size_t pk = (size_t)plane * k;
+            const double *ph = L.phi + pk;
+            const D2 ym = ld_off<D2>(ph, oym * 8u), yp = ld_off<D2>(ph, oyp * 8u);
+            const double vxm = ld_off<double>(ph, oxm * 8u), vxp = ld_off<double>(ph, oxp * 8u);
+            q.ym0 = okym ? ym.x : 0.0;
+            q.ym1 = okym ? ym.y : 0.0;
+            q.yp0 = okyp ? yp.x : 0.0;
+            q.yp1 = okyp ? yp.y : 0.0;
+            q.xm = okxm ? vxm : 0.0;
+            q.xp = okxp ? vxp : 0.0;
+            if constexpr (HAS_E) {
+                const float *ek = e + pk;
+                const F2 eym = ld_off<F2>(ek, oym * 4u), eyp = ld_off<F2>(ek, oyp * 4u);
+                const float exm = ld_off<float>(ek, oxm * 4u), exp = ld_off<float>(ek, oxp * 4u);
+                q.eym0 = okym ? eym.x : 0.0f;
+                q.eym1 = okym ? eym.y : 0.0f;
+                q.eyp0 = okyp ? eyp.x : 0.0f;
+                q.eyp1 = okyp ? eyp.y : 0.0f;
+                q.exm = okxm ? exm : 0.0f;
+                q.exp = okxp ? exp : 0.0f;
+            } else {
+                q.eym0 = q.eym1 = q.eyp0 = q.eyp1 = q.exm = q.exp = 0.0f;
+            }
+            const D2 bx = ld_off<D2>(L.bx + pk, oc * 8u);
+            q.bx0 = bx.x;
+            q.bx1 = bx.y;
+            q.bx2 = (x0 + 2 < nx) ? ld_off<double>(L.bx + pk, (oc + 2u) * 8u) : L.bxh[j + ny * k];
+            const D2 by = ld_off<D2>(L.by + pk, oc * 8u);
+            q.by0 = by.x;
+            q.by1 = by.y;
+            const D2 byn = (j < ny - 1) ? ld_off<D2>(L.by + pk, (oc + nx) * 8u)
+                                        : ld_off<D2>(L.byh + (size_t)nx * k, obyh * 8u);
+            q.byn0 = byn.x;
+            q.byn1 = byn.y;
+            const D2 r = ld_off<D2>(L.rhs + pk, oc * 8u);
+            q.r0 = r.x;
+            q.r1 = r.y;
+            return q;
+        };
+        Z zm = loadz(z0 - 1), zc = loadz(z0), zp = loadz(z0 + 1);
+        P pc = loadp(z0);
+        for (int k = z0; k < z1; ++k) {
+            // next plane's inputs go out first; their latency hides behind this plane's work
+            const Z zn = loadz(k + 2);
+            const P pn = loadp(min(k + 1, z1 - 1));
+            // z-high faces: the next plane's z-low faces, or the bzh slab on top
+            double bzn0, bzn1;
+            if (k < nz - 1) {
+                bzn0 = zp.b0;
+                bzn1 = zp.b1;
+            } else {
+                const D2 t2 = ld_off<D2>(L.bzh, oc * 8u);
+                bzn0 = t2.x;
+                bzn1 = t2.y;
+            }
+            double res[2], pcen[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double w[6], p[6];
+                float pe[6];
+                w[0] = c ? pc.bx1 : pc.bx0;
+                w[1] = c ? pc.bx2 : pc.bx1;
+                w[2] = c ? pc.by1 : pc.by0;
+                w[3] = c ? pc.byn1 : pc.byn0;
+                w[4] = c ? zc.b1 : zc.b0;
+                w[5] = c ? bzn1 : bzn0;
+                p[0] = c ? zc.p0 : pc.xm;
+                p[1] = c ? pc.xp : zc.p1;
+                p[2] = c ? pc.ym1 : pc.ym0;
+                p[3] = c ? pc.yp1 : pc.yp0;
+                p[4] = c ? zm.p1 : zm.p0;
+                p[5] = c ? zp.p1 : zp.p0;
+                double pcc = c ? zc.p1 : zc.p0;
+                if constexpr (HAS_E) {
+                    pe[0] = c ? zc.e0 : pc.exm;
+                    pe[1] = c ? pc.exp : zc.e1;
+                    pe[2] = c ? pc.eym1 : pc.eym0;
+                    pe[3] = c ? pc.eyp1 : pc.eyp0;
+                    pe[4] = c ? zm.e1 : zm.e0;
+                    pe[5] = c ? zp.e1 : zp.e0;
+#pragma unroll
+                    for (int s = 0; s < 6; ++s) p[s] = p[s] + (double)pe[s];
+                    pcc = pcc + (double)(c ? zc.e1 : zc.e0);
+                }
+                res[c] = (c ? pc.r1 : pc.r0) - apply_value<3, double>(L, w, p, pcc);
+                pcen[c] = pcc;
+                a = fmax(a, fabs(res[c]));
+            }
+            const size_t pk = (size_t)plane * k;
+            if constexpr (MODE >= 1) {
+                F2 o;
+                o.x = (float)res[0];
+                o.y = (float)res[1];
+                st_off<F2>(r32 + pk, oc * 4u, o);
+            }
+            if constexpr (MODE == 2) {
+                D2 o;
+                o.x = pcen[0];
+                o.y = pcen[1];
+                st_off<D2>(phi_out + pk, oc * 8u, o);
+            }
+            zm = zc;
+            zc = zp;
+            zp = zn;
+            pc = pn;
+        }
+    }
+    block_max_atomic(a, nrm);
+}
+
 // max |x| over n values (the norm of the solver's right side)
 __global__ void __launch_bounds__(TPB) k_absmax(const double *x, long n, unsigned long long *nrm)
 {
@@ -2297,9 +2491,41 @@ template void launch_resid_restrict<float>(int, const Lev<float> &, const float 
 
 static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
 
+// Row-march form of the FP64 residual passes where it applies (3D, nx a
+// multiple of 128); false otherwise.  MAC_RES_ROWS=0 keeps the row-per-wave
+// kernels.  The z-chunk length is 32 planes (MAC_RES_TZ overrides).
+template <int MODE>
+static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_out, float *r32,
+                              unsigned long long *nrm, hipStream_t s)
+{
+    static int on = -1, tz_env = 0;
+    if (on < 0) {
+        const char *v = getenv("MAC_RES_ROWS");
+        on = v ? atoi(v) : 1;
+        const char *t = getenv("MAC_RES_TZ");
+        tz_env = t ? atoi(t) : 0;
+    }
+    if (!on || L.nx % 128 != 0 || L.nz < 2) return false;
+    RowMarchGrid G;
+    G.W = L.nx / 128;
+    G.tz = tz_env > 0 ? tz_env : 32;
+    G.tz = std::min(G.tz, L.nz);
+    const int chunks = (L.nz + G.tz - 1) / G.tz;
+    G.total = L.ny * G.W * chunks;
+    const int blocks = (G.total + 3) / 4;
+    G.per_xcd = (blocks + 7) / 8;
+    hipLaunchKernelGGL(k_resid_rows<MODE>, dim3(8 * G.per_xcd), dim3(256), 0, s, L, e, phi_out, r32, nrm, G);
+    return true;
+}
+
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 8) {
+        if (dim == 3 && out == nullptr && nrm != nullptr &&
+            launch_resid_rows<0>(L, nullptr, nullptr, nullptr, nrm, s))
+            return;
+    }
     const unsigned rb = row_blocks(L.ny, L.nz);
     unsigned g = nrm ? std::min(rb, NORM_GRID) : rb;
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_residual<D_, T>), dim3(g), dim3(TPB), 0, s, L, out, nrm));
@@ -2308,6 +2534,7 @@ void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s)
 {
+    if (dim == 3 && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
@@ -2315,6 +2542,7 @@ void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
                                unsigned long long *nrm, hipStream_t s)
 {
+    if (dim == 3 && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_update_residual<D_>, dim3(g), dim3(TPB), 0, s, L, e, phi_out,
                                          r32, nrm));
