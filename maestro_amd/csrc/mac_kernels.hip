@@ -680,6 +680,7 @@ template <typename V>
 __device__ __forceinline__ V ld_off(const void *base, unsigned boff)
 {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(base);
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16, "buffer load width");
     if constexpr (sizeof(V) == 4)
         return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, boff, 0, 0));
     else if constexpr (sizeof(V) == 8)
@@ -691,7 +692,10 @@ template <typename V>
 __device__ __forceinline__ void st_off(void *base, unsigned boff, const V &v)
 {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(base);
-    if constexpr (sizeof(V) == 8)
+    static_assert(sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16, "buffer store width");
+    if constexpr (sizeof(V) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, boff, 0, 0);
+    else if constexpr (sizeof(V) == 8)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
                                               r, boff, 0, 0);
     else
@@ -1499,6 +1503,139 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
         }
     }
     block_max_atomic(a, nrm);
+}
+
+// ------------------------------------- residual + restriction, row march
+// The fine residual averaged onto the coarse level (and the coarse iterate
+// cleared) as one wave per (coarse x-row segment, coarse z-chunk): a lane
+// owns one coarse column, i.e. the fine column pair 2I, 2I+1 of the fine row
+// pair 2J, 2J+1, and marches fine planes with the planes below and above in
+// registers.  Per-cell arithmetic and the summation order of the eight fine
+// residuals are those of k_resid_restrict.  3D, fine nx = 128 W.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__ phi, Lev<T> C,
+                                                RowMarchGrid G)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr unsigned ES = sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int bidx = (blockIdx.x & 7) * G.per_xcd + (blockIdx.x >> 3);
+    const int gw = bidx * 4 + wv;
+    if (gw >= G.total) return;
+    const int nx = F.nx, ny = F.ny, nz = F.nz, plane = nx * ny;
+    const int seg = gw % G.W, t = gw / G.W;
+    const int J = t % C.ny, chunk = t / C.ny;
+    const int K0 = chunk * G.tz, K1 = min(K0 + G.tz, C.nz);
+    const int I = lane + 64 * seg, x0 = 2 * I;
+    const int j0 = 2 * J, j1 = j0 + 1;
+    // rows j0-1 and j1+1 (y-neighbours outside the pair), x-neighbour columns
+    const bool okym = j0 > 0 || F.pery, okyp = j1 < ny - 1 || F.pery;
+    const int jm = j0 > 0 ? j0 - 1 : ny - 1, jp = j1 < ny - 1 ? j1 + 1 : 0;
+    const bool okxm = x0 > 0 || F.perx, okxp = x0 + 2 < nx || F.perx;
+    const int xm = x0 > 0 ? x0 - 1 : nx - 1, xp = x0 + 2 < nx ? x0 + 2 : 0;
+    const unsigned o0 = (unsigned)(x0 + nx * j0), o1 = o0 + nx;
+    const unsigned oym = (unsigned)(x0 + nx * jm), oyp = (unsigned)(x0 + nx * jp);
+    const unsigned oxm0 = (unsigned)(xm + nx * j0), oxp0 = (unsigned)(xp + nx * j0);
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (F.perz) { kw = k < 0 ? k + nz : k - nz; return true; }
+        kw = 0;
+        return false;
+    };
+    struct Z {   // own two rows on one plane: old iterate and z-low faces
+        T p00, p01, p10, p11, b00, b01, b10, b11;   // [row][column]
+    };
+    auto loadz = [&](int k) -> Z {
+        Z z;
+        int kw;
+        const bool ok = kwrap(k, kw);
+        const size_t pk = (size_t)plane * kw;
+        const T2 a = ld_off<T2>(phi + pk, o0 * ES), c = ld_off<T2>(phi + pk, o1 * ES);
+        const T2 ba = ld_off<T2>(F.bz + pk, o0 * ES), bc = ld_off<T2>(F.bz + pk, o1 * ES);
+        z.p00 = ok ? a.x : T(0);
+        z.p01 = ok ? a.y : T(0);
+        z.p10 = ok ? c.x : T(0);
+        z.p11 = ok ? c.y : T(0);
+        z.b00 = ba.x;
+        z.b01 = ba.y;
+        z.b10 = bc.x;
+        z.b11 = bc.y;
+        return z;
+    };
+    // residuals of the four fine cells of plane k (k in range)
+    auto plane_res = [&](int k, const Z &zm, const Z &zc, const Z &zp, T r[2][2]) {
+        const size_t pk = (size_t)plane * k;
+        const T *ph = phi + pk;
+        const T2 ym = ld_off<T2>(ph, oym * ES), yp = ld_off<T2>(ph, oyp * ES);
+        const T xm0 = ld_off<T>(ph, oxm0 * ES), xp0 = ld_off<T>(ph, oxp0 * ES);
+        const T xm1 = ld_off<T>(ph, (oxm0 + nx) * ES), xp1 = ld_off<T>(ph, (oxp0 + nx) * ES);
+        const T2 bx0 = ld_off<T2>(F.bx + pk, o0 * ES), bx1 = ld_off<T2>(F.bx + pk, o1 * ES);
+        const T bxe0 = (x0 + 2 < nx) ? ld_off<T>(F.bx + pk, (o0 + 2u) * ES) : F.bxh[j0 + ny * k];
+        const T bxe1 = (x0 + 2 < nx) ? ld_off<T>(F.bx + pk, (o1 + 2u) * ES) : F.bxh[j1 + ny * k];
+        const T2 by0 = ld_off<T2>(F.by + pk, o0 * ES), by1 = ld_off<T2>(F.by + pk, o1 * ES);
+        const T2 byn = (j1 < ny - 1) ? ld_off<T2>(F.by + pk, (o1 + nx) * ES)
+                                     : ld_off<T2>(F.byh + (size_t)nx * k, (unsigned)x0 * ES);
+        const T2 rh0 = ld_off<T2>(F.rhs + pk, o0 * ES), rh1 = ld_off<T2>(F.rhs + pk, o1 * ES);
+        T bzn00, bzn01, bzn10, bzn11;
+        if (k < nz - 1) {
+            bzn00 = zp.b00;
+            bzn01 = zp.b01;
+            bzn10 = zp.b10;
+            bzn11 = zp.b11;
+        } else {
+            const T2 ha = ld_off<T2>(F.bzh, o0 * ES), hb = ld_off<T2>(F.bzh, o1 * ES);
+            bzn00 = ha.x;
+            bzn01 = ha.y;
+            bzn10 = hb.x;
+            bzn11 = hb.y;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                T w[6], p[6];
+                const T2 &bxr = b ? bx1 : bx0;
+                const T2 &byr = b ? by1 : by0;
+                w[0] = a ? bxr.y : bxr.x;
+                w[1] = a ? (b ? bxe1 : bxe0) : bxr.y;
+                w[2] = a ? byr.y : byr.x;
+                w[3] = b ? (a ? byn.y : byn.x) : (a ? by1.y : by1.x);
+                w[4] = b ? (a ? zc.b11 : zc.b10) : (a ? zc.b01 : zc.b00);
+                w[5] = b ? (a ? bzn11 : bzn10) : (a ? bzn01 : bzn00);
+                const T pr0 = b ? zc.p10 : zc.p00, pr1 = b ? zc.p11 : zc.p01;   // this row
+                p[0] = a ? pr0 : (okxm ? (b ? xm1 : xm0) : T(0));
+                p[1] = a ? (okxp ? (b ? xp1 : xp0) : T(0)) : pr1;
+                p[2] = b ? (a ? zc.p01 : zc.p00) : (okym ? (a ? ym.y : ym.x) : T(0));
+                p[3] = b ? (okyp ? (a ? yp.y : yp.x) : T(0)) : (a ? zc.p11 : zc.p10);
+                p[4] = b ? (a ? zm.p11 : zm.p10) : (a ? zm.p01 : zm.p00);
+                p[5] = b ? (a ? zp.p11 : zp.p10) : (a ? zp.p01 : zp.p00);
+                const T pcc = a ? pr1 : pr0;
+                const T rh = b ? (a ? rh1.y : rh1.x) : (a ? rh0.y : rh0.x);
+                r[b][a] = rh - apply_value<3, T>(F, w, p, pcc);
+            }
+    };
+    Z zm = loadz(2 * K0 - 1), zc = loadz(2 * K0), zp = loadz(2 * K0 + 1);
+    const unsigned oc = (unsigned)(I + C.nx * J);
+    for (int K = K0; K < K1; ++K) {
+        T s[2][2];   // [c][b]: pair sums of fine plane 2K+c, row b
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int k = 2 * K + c;
+            const Z zn = loadz(k + 2);
+            T r[2][2];
+            plane_res(k, zm, zc, zp, r);
+            s[c][0] = r[0][0] + r[0][1];
+            s[c][1] = r[1][0] + r[1][1];
+            zm = zc;
+            zc = zp;
+            zp = zn;
+        }
+        const T v = T(0.125) * ((s[0][0] + s[0][1]) + (s[1][0] + s[1][1]));
+        const size_t cp = (size_t)C.nx * C.ny * K;
+        st_off<T>(C.rhs + cp, oc * ES, v);
+        st_off<T>(C.phi + cp, oc * ES, T(0));
+    }
 }
 
 // max |x| over n values (the norm of the solver's right side)
@@ -2333,7 +2470,11 @@ static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
     const int tiles = G.ntx * G.nty;
     int zc = (2048 + tiles - 1) / tiles;
-    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
+    // small levels are latency bound: their march is kept short (2 planes and
+    // more chunks) even though each chunk re-reads its two ring planes
+    const long cells = (long)L.nx * L.ny * L.nz;
+    const int min_tz = cells <= (1L << 18) ? 2 : 8;
+    zc = std::max(1, std::min(zc, std::max(1, L.nz / min_tz)));
     G.tz = (L.nz + zc - 1) / zc;
     zc = (L.nz + G.tz - 1) / G.tz;
     G.total = tiles * zc;
@@ -2473,6 +2614,22 @@ void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
 template <typename T>
 void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s)
 {
+    static int rows = -1;   // MAC_RR_ROWS=0 keeps the block-per-coarse-row kernel
+    if (rows < 0) {
+        const char *v = getenv("MAC_RR_ROWS");
+        rows = v ? atoi(v) : 1;
+    }
+    if (rows && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
+        RowMarchGrid G;
+        G.W = F.nx / 128;
+        G.tz = std::min(16, C.nz);   // coarse planes per chunk (32 fine planes)
+        const int chunks = (C.nz + G.tz - 1) / G.tz;
+        G.total = C.ny * G.W * chunks;
+        const int blocks = (G.total + 3) / 4;
+        G.per_xcd = (blocks + 7) / 8;
+        hipLaunchKernelGGL((k_rr_rows<T>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G);
+        return;
+    }
     dim3 g((C.nx + 63) / 64, C.ny, C.nz);
     if (dim == 3)
         hipLaunchKernelGGL((k_resid_restrict<3, T>), g, dim3(256), 0, s, F, phi, C);
