@@ -86,13 +86,13 @@ def stage(per_box, copies, torch):
     return sets, dr
 
 
-def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1):
+def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1, profile=True):
     plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner)
     stats = []
     for w in range(warmup):
         du, dp = sets[w]
         stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
-    plan.profile(True)
+    plan.profile(profile)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time the steps without the per-kernel HIP events (no roofline figures)")
     ap.add_argument("--inner-cycles", type=int, default=1,
                     help="FP32 V-cycles per refinement step of the mixed path")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -165,7 +167,7 @@ def main():
                 a.copy_(torch.from_numpy(h))
         torch.cuda.synchronize()
         dt, st, prof = run_precision(plan, sets, dr, b0, b0e, args.warmup, args.steps, prec, torch,
-                                     args.inner_cycles)
+                                     args.inner_cycles, profile=not args.no_profile)
         # host snapshot of the last step's outputs (the staged buffers are reused)
         results[name] = {"seconds": dt, "stats": st, "prof": prof,
                          "umac": [[a.cpu().numpy() for a in box] for box in sets[-1][0]],
@@ -177,10 +179,12 @@ def main():
     mixed, fp64 = results["mixed"], results["fp64"]
     value = args.steps / mixed["seconds"]
     cnt, ms = mixed["prof"]["gsrb32"]
-    avg_ms = ms / max(cnt, 1)
-    achieved = gsrb_launch_bytes(ncell, 4) / (avg_ms * 1e-3) / 1e9
     cnt64, ms64 = fp64["prof"]["gsrb64"]
-    avg64 = ms64 / max(cnt64, 1)
+    if cnt == 0 or cnt64 == 0:   # --no-profile: no per-kernel timing
+        avg_ms = avg64 = float("inf")
+    else:
+        avg_ms, avg64 = ms / cnt, ms64 / cnt64
+    achieved = gsrb_launch_bytes(ncell, 4) / (avg_ms * 1e-3) / 1e9
     ach64 = gsrb_launch_bytes(ncell, 8) / (avg64 * 1e-3) / 1e9
 
     # accuracy of the mixed path against the FP64 GPU path and the oracle

@@ -2622,7 +2622,11 @@ void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
     if (rows && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
         RowMarchGrid G;
         G.W = F.nx / 128;
-        G.tz = std::min(16, C.nz);   // coarse planes per chunk (32 fine planes)
+        // coarse planes per chunk: at most 16, fewer on small levels so that
+        // there are some 4096 waves (16 per CU) to hide the march's latency
+        const long segs = (long)C.ny * G.W;
+        G.tz = (int)std::max(1L, std::min(16L, segs * C.nz / 4096));
+        G.tz = std::min(G.tz, C.nz);
         const int chunks = (C.nz + G.tz - 1) / G.tz;
         G.total = C.ny * G.W * chunks;
         const int blocks = (G.total + 3) / 4;
@@ -2665,7 +2669,9 @@ static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_
     if (!on || L.nx % 128 != 0 || L.nz < 2) return false;
     RowMarchGrid G;
     G.W = L.nx / 128;
-    G.tz = tz_env > 0 ? tz_env : 32;
+    // planes per chunk: 32, fewer on small levels (some 4096 waves in all)
+    const long segs = (long)L.ny * G.W;
+    G.tz = tz_env > 0 ? tz_env : (int)std::max(2L, std::min(32L, segs * L.nz / 4096));
     G.tz = std::min(G.tz, L.nz);
     const int chunks = (L.nz + G.tz - 1) / G.tz;
     G.total = L.ny * G.W * chunks;

@@ -617,7 +617,7 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
     if (hipGetDevice(&P->device) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess) return fail(MAC_ERR_HIP);
     for (int e = 0; e < 8; ++e)
-        if (hipEventCreate(&P->ev[e]) != hipSuccess) return fail(MAC_ERR_HIP);
+        if (hipEventCreateWithFlags(&P->ev[e], hipEventDisableSystemFence) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMalloc(&P->boxes_dev, sizeof(BoxDev) * P->nboxes) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMemcpy(P->boxes_dev, P->boxes.data(), sizeof(BoxDev) * P->nboxes, hipMemcpyHostToDevice) !=
         hipSuccess)
@@ -656,7 +656,7 @@ extern "C" int mac_plan_profile(mac_plan *P, int enable)
     if (enable && P->prof_ev.empty()) {
         P->prof_ev.resize(2 * PROF_PAIRS);
         P->prof_cls.resize(PROF_PAIRS);
-        for (auto &e : P->prof_ev) HIPCHK(hipEventCreate(&e));
+        for (auto &e : P->prof_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     }
     for (int c = 0; c < MAC_PROF_NCLASS; ++c) { P->prof_cnt[c] = 0; P->prof_ms[c] = 0.0; }
     P->prof_on = enable != 0;
