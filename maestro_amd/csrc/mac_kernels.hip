@@ -2074,11 +2074,34 @@ __global__ void __launch_bounds__(TPB) k_fill_ghost(CorrectArgs A)
     const int ex = B.hi[0] - B.lo[0] + 1 + 2 * ng + (d == 0);
     const int ey = B.hi[1] - B.lo[1] + 1 + 2 * ng + (d == 1);
     const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 + 2 * ng + (d == 2) : 1;
-    const long q = (long)blockIdx.x * TPB + threadIdx.x;
-    if (q >= (long)ex * ey * ez) return;
-    int g[3] = {B.lo[0] - ng + (int)(q % ex), B.lo[1] - ng + (int)((q / ex) % ey),
-                (DIM == 3) ? B.lo[2] - ng + (int)(q / ((long)ex * ey)) : 0};
-    // inside this box's valid region?  then nothing to do
+    // the thread's ghost face: the grown box minus the valid faces is the
+    // z-slabs (full x-y), then the y-slabs of the inner z range, then the
+    // x-slabs of the inner y-z range, each ng thick on both sides
+    const int ngz = (DIM == 3) ? ng : 0;
+    const int iy = ey - 2 * ng, iz = ez - 2 * ngz;
+    const int nA = 2 * ngz * ex * ey, nB = 2 * ng * ex * iz, nC = 2 * ng * iy * iz;
+    int q = blockIdx.x * TPB + threadIdx.x;
+    if (q >= nA + nB + nC) return;
+    int x, y, z;
+    if (q < nA) {
+        const int sl = q / (ex * ey), r = q - sl * ex * ey;
+        z = sl < ngz ? sl : ez - 2 * ngz + sl;
+        y = r / ex;
+        x = r - y * ex;
+    } else if ((q -= nA) < nB) {
+        const int sl = q / (ex * iz), r = q - sl * ex * iz;
+        y = sl < ng ? sl : ey - 2 * ng + sl;
+        z = ngz + r / ex;
+        x = r - (r / ex) * ex;
+    } else {
+        q -= nB;
+        const int sl = q / (iy * iz), r = q - sl * iy * iz;
+        x = sl < ng ? sl : ex - 2 * ng + sl;
+        y = ng + r % iy;
+        z = ngz + r / iy;
+    }
+    int g[3] = {B.lo[0] - ng + x, B.lo[1] - ng + y, (DIM == 3) ? B.lo[2] - ng + z : 0};
+    // inside this box's valid region?  then nothing to do (not reached)
     bool valid = true;
     for (int e = 0; e < DIM; ++e) {
         int hi = B.hi[e] + (e == d);

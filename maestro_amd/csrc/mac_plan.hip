@@ -602,7 +602,8 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
             P->maxfaces = std::max(P->maxfaces, f[0] * f[1] * f[2]);
             int gg[3];
             for (int q = 0; q < 3; ++q) gg[q] = (q < P->dim) ? f[q] + 2 * P->ng_u : 1;
-            P->maxghost = std::max(P->maxghost, gg[0] * gg[1] * gg[2]);
+            // ghost faces only: the grown face box minus the valid faces
+            P->maxghost = std::max(P->maxghost, gg[0] * gg[1] * gg[2] - f[0] * f[1] * f[2]);
             for (int dir = 0; dir < P->dim; ++dir) {
                 long plane = 1;
                 for (int q = 0; q < P->dim; ++q)

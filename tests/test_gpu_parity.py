@@ -98,3 +98,40 @@ def test_initial_residual_norm_bitwise(gpu, cfg, n, box, prec):
         assert abs(got - want) <= 1e-12 * want, (got, want)
     else:
         assert got == want, (got, want)
+
+
+@pytest.mark.parametrize("ng", [1, 2])
+def test_ghost_faces_filled_from_neighbours(gpu, ng):
+    """After the call, every ghost face of every box that lies inside the
+    domain or across a periodic side holds the valid face of the box that
+    owns it (multifab_fill_boundary); ghost faces outside non-periodic sides
+    belong to the physical edge-velocity rules and are not checked here."""
+    wl = W.make_workload(1, n=(32, 32, 32), box=16)
+    out = run_gpu(wl, ng_u=ng)
+    n = wl["n"]
+    per = [b[0] == W.PERIODIC for b in wl["bc"]]
+    checked = 0
+    for b, (lo, hi) in enumerate(out["boxes"]):
+        for d in range(3):
+            arr = out["umac_boxes"][b][d]
+            ez, ey, ex = arr.shape
+            k, j, i = np.meshgrid(np.arange(ez), np.arange(ey), np.arange(ex), indexing="ij")
+            g = [lo[0] - ng + i, lo[1] - ng + j, lo[2] - ng + k]
+            valid = np.ones(arr.shape, dtype=bool)
+            inside = np.ones(arr.shape, dtype=bool)
+            G = []
+            for e in range(3):
+                top_box = hi[e] + (1 if e == d else 0)
+                valid &= (g[e] >= lo[e]) & (g[e] <= top_box)
+                top = n[e] - 1 + (1 if e == d else 0)
+                out_of = (g[e] < 0) | (g[e] > top)
+                if per[e]:
+                    G.append(np.mod(g[e], n[e]))
+                else:
+                    inside &= ~out_of
+                    G.append(np.clip(g[e], 0, top))
+            sel = ~valid & inside
+            want = out["faces"][d][G[2][sel], G[1][sel], G[0][sel]]
+            assert np.array_equal(arr[sel], want), (b, d)
+            checked += int(sel.sum())
+    assert checked > 0
