@@ -39,14 +39,14 @@ def sweep_kernel_name(s: int) -> str:
     MAC_SWEEP environment variable selects the variant)."""
     t = "double" if s == 8 else "float"
     v = int(os.environ.get("MAC_SWEEP", "1082"))
-    if v >= 1000 and s == 8:
-        v = 208   # the full-row form is FP32 only; FP64 falls back to the deep-prefetch form
     if v == 0:
         return f"k_gsrb_fused<{t}, 0>"
     if v in (8, 16):
         return f"k_gsrb_pair<{t}, 0, {v}>"
-    if v >= 1000:
-        return f"k_gsrb_rows<{t}, 0, {(v // 10) % 100}>"
+    if v == 1082:   # full rows: 8 rows x 1024 threads (FP32), 6 rows x 768 threads (FP64)
+        return f"k_gsrb_rows<{t}, 0, 8, 1024>" if s == 4 else f"k_gsrb_rows<{t}, 0, 6, 768>"
+    if v == 1042:
+        return f"k_gsrb_rows<{t}, 0, 4, 512>"
     return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}>"
 
 

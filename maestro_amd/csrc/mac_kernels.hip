@@ -703,8 +703,8 @@ __device__ __forceinline__ void st_off(void *base, unsigned boff, const V &v)
                                                r, boff, 0, 0);
 }
 
-template <typename T, int MODE, int EY>
-__global__ void __launch_bounds__(1024) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
+template <typename T, int MODE, int EY, int NTH>
+__global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
                                                     T *__restrict__ pout, RowSweepGrid G)
 {
     using T2 = typename Vec2<T>::type;
@@ -742,7 +742,7 @@ __global__ void __launch_bounds__(1024) k_gsrb_rows(Lev<T> L, const T *__restric
     const bool xbeg = (x0 == 0), xend = (x0 + 2 == nx), mirror = L.perx != 0;
     extern __shared__ __align__(16) unsigned char rows_smem[];
     T *lds = (T *)rows_smem;
-    for (int q = threadIdx.x; q < 4 * PS; q += blockDim.x) lds[q] = T(0);
+    for (int q = threadIdx.x; q < 3 * PS; q += blockDim.x) lds[q] = T(0);
 
     auto kwrap = [&](int k, int &kw) -> bool {
         if (k >= 0 && k < nz) { kw = k; return true; }
@@ -812,7 +812,7 @@ __global__ void __launch_bounds__(1024) k_gsrb_rows(Lev<T> L, const T *__restric
     const int kb = z0 - 1, ke = z1;
     const int nsteps = ((ke - kb + 1) + 7) & ~7;   // a multiple of the unroll
     // rings indexed by the step m = k - kb: inputs of plane m in wq[m & 3],
-    // old/updated iterate of plane m in ph[m & 7], LDS plane m in buffer m & 3
+    // old/updated iterate of plane m in ph[m & 7], LDS plane m in buffer m % 3
     In wq[4];
     struct Ring {
         T v[16];
@@ -846,7 +846,9 @@ __global__ void __launch_bounds__(1024) k_gsrb_rows(Lev<T> L, const T *__restric
         constexpr int C0 = S & 3, CP = (S + 3) & 3, C1 = (S + 1) & 3, CN = (S + 2) & 3;   // m, m-1, m+1, m+2
         constexpr int F0 = S, FM1 = (S + 7) & 7, FM2 = (S + 6) & 7, FP1 = (S + 1) & 7;
         const int k = kb + m;
-        T *buf = lds + C0 * PS, *pbuf = lds + CP * PS;
+        // LDS planes rotate through three buffers (plane m in buffer m mod 3)
+        const int b3 = m % 3;
+        T *buf = lds + b3 * PS, *pbuf = lds + (b3 == 0 ? 2 : b3 - 1) * PS;
         int kw;
         const bool kok = kwrap(k, kw) && k <= ke;
         wq[CN] = fetch(min(k + 2, ke + 1));
@@ -2510,21 +2512,19 @@ static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
 }
 
 // Full-row sweep: usable when nx = 128 W and a block of EY rows (64 W EY
-// threads) fits in 1024.  The z-chunk count is chosen so the number of
-// blocks fills whole rounds of the resident block slots (CUs x blocks per
-// CU) with the least z-ring overhead.
-template <typename T, int EY>
+// threads) fits in NTH.  FP32 runs 8 rows in blocks of up to 1024 threads
+// (at most 128 VGPRs per lane); FP64, whose register rings are twice as wide,
+// runs 6 rows in blocks of up to 768 (168 VGPRs).  The z-chunk count is
+// chosen so the number of blocks fills whole rounds of the resident block
+// slots (CUs x blocks per CU) with the least z-ring overhead.
+template <typename T, int EY, int NTH>
 static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
 {
-    // FP32 only: the FP64 form needs more than the 128 VGPRs a 1024-thread
-    // block allows and spills
-    if constexpr (sizeof(T) == 8) {
-        return false;
-    } else {
+    {
     if (L.nx % 128 != 0) return false;
     const int W = L.nx / 128;
-    if (64 * W * EY > 1024 || L.nz < 2) return false;
-    const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
+    if (64 * W * EY > NTH || L.nz < 2) return false;
+    const size_t lds = sizeof(T) * 3 * (size_t)(EY + 2) * (L.nx + 4);
     if (lds > 64 * 1024) return false;
     const int threads = 64 * W * EY;
     static int slots_cache[2][17] = {{0}};   // [mode][W] resident blocks per device
@@ -2535,8 +2535,8 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return false;
         hipError_t e = (mode == SWEEP_ZERO)
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY>, threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY>, threads, lds);
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH>, threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH>, threads, lds);
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
     }
@@ -2566,9 +2566,9 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY>), g, b, lds, s, L, pin, pout, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH>), g, b, lds, s, L, pin, pout, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY>), g, b, lds, s, L, pin, pout, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH>), g, b, lds, s, L, pin, pout, G);
     return true;
     }
 }
@@ -2588,8 +2588,11 @@ void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
     if (mode != SWEEP_PROLONG && variant >= 1000) {
         bool done = false;
         switch (variant) {
-        case 1082: done = launch_rows<T, 8>(L, pin, pout, mode, s); break;
-        case 1042: done = launch_rows<T, 4>(L, pin, pout, mode, s); break;
+        case 1082:
+            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768>(L, pin, pout, mode, s);
+            else done = launch_rows<T, 8, 1024>(L, pin, pout, mode, s);
+            break;
+        case 1042: done = launch_rows<T, 4, 512>(L, pin, pout, mode, s); break;
         default: break;
         }
         if (done) return;
