@@ -1681,6 +1681,55 @@ __global__ void __launch_bounds__(TPB) k_prolong(Lev<T> F, const T *e, int cnx, 
     });
 }
 
+// The same prolongation with one wave per fine (x-row segment, plane): the
+// row's coarse rows/planes and weights are wave-uniform, each lane owns the
+// fine column pair 2I, 2I+1 (coarse column I) and reads the coarse values at
+// I-1, I, I+1; the fine pair is loaded and stored as a vector.  Evaluation
+// order is that of prolong_value.  3D, fine nx = 128 W.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_rows(Lev<T> F, const T *__restrict__ e, int cnx, int cny,
+                                                     int cnz, EllBC bc, int W, int total)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr unsigned ES = sizeof(T);
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (gw >= total) return;
+    const int seg = gw % W, t = gw / W;
+    const int j = t % F.ny, k = t / F.ny;
+    const int I = lane + 64 * seg, x0 = 2 * I;
+    // coarse x neighbours of the pair: I-1 for the even fine cell, I+1 for the odd
+    int xa0, xa1, xb0, xb1;
+    T wa0, wa1, wb0, wb1;
+    lin_w<T>(x0, cnx, bc.k[0][0], bc.k[0][1], xa0, xa1, wa0, wa1);
+    lin_w<T>(x0 + 1, cnx, bc.k[0][0], bc.k[0][1], xb0, xb1, wb0, wb1);
+    int yi0, yi1, zi0, zi1;
+    T wy0, wy1, wz0, wz1;
+    lin_w<T>(j, cny, bc.k[1][0], bc.k[1][1], yi0, yi1, wy0, wy1);
+    lin_w<T>(k, cnz, bc.k[2][0], bc.k[2][1], zi0, zi1, wz0, wz1);
+    const int yy[2] = {yi0, yi1}, zz[2] = {zi0, zi1};
+    T va[2], vb[2];
+#pragma unroll
+    for (int cz = 0; cz < 2; ++cz) {
+        T ya[2], yb[2];
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy) {
+            const T *row = e + (size_t)cnx * (yy[cy] + (size_t)cny * zz[cz]);
+            const T c0 = ld_off<T>(row, (unsigned)xa0 * ES);   // = coarse I
+            ya[cy] = dfma(wa1, ld_off<T>(row, (unsigned)xa1 * ES), wa0 * c0);
+            yb[cy] = dfma(wb1, ld_off<T>(row, (unsigned)xb1 * ES), wb0 * c0);
+        }
+        va[cz] = dfma(wy1, ya[1], wy0 * ya[0]);
+        vb[cz] = dfma(wy1, yb[1], wy0 * yb[0]);
+    }
+    const unsigned off = (unsigned)(x0 + F.nx * j) * ES;
+    T *fp = F.phi + (size_t)F.nx * F.ny * k;
+    T2 p = ld_off<T2>(fp, off);
+    p.x = p.x + dfma(wz1, va[1], wz0 * va[0]);
+    p.y = p.y + dfma(wz1, vb[1], wz0 * vb[0]);
+    st_off<T2>(fp, off, p);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_fill(T *x, long n, T v)
 {
@@ -2753,6 +2802,13 @@ void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hip
 template <typename T>
 void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s)
 {
+    if (dim == 3 && F.nx % 128 == 0) {
+        const int W = F.nx / 128;
+        const int total = F.ny * F.nz * W;
+        hipLaunchKernelGGL((k_prolong_rows<T>), dim3((total + 3) / 4), dim3(256), 0, s, F, (const T *)C.phi,
+                           C.nx, C.ny, C.nz, bc, W, total);
+        return;
+    }
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_prolong<D_, T>), dim3(row_blocks(F.ny, F.nz)), dim3(TPB), 0, s,
                                          F, (const T *)C.phi, C.nx, C.ny, C.nz, bc));
 }

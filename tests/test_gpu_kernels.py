@@ -143,7 +143,7 @@ def test_residual_restrict_bitwise(gpu, oracle_lib, n, bc):
     assert not plan.debug_get(1, "phi").any()
 
 
-@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40)])
+@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40), (128, 12, 8), (256, 20, 12), (128, 36, 10)])
 @pytest.mark.parametrize("bc", BCS_3D)
 def test_prolong_then_sweep_bitwise(gpu, oracle_lib, n, bc):
     O = oracle_lib
