@@ -1906,6 +1906,121 @@ __device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit
     __syncthreads();
 }
 
+// sums over the 64 lanes of a wave in a fixed order (butterfly), so every
+// lane holds the same, deterministic result
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// The bottom solve of a level of at most 64 cells inside one wave: nub
+// red-black sweeps, the FP64 CG of blk_bottom_cg (mean projected out on
+// singular problems), nub sweeps.  Lane q owns cell q; neighbour values come
+// by lane shuffles (zero outside a non-periodic side), so the wave needs no
+// barrier.  Same updates and stopping rule as the block form; only the order
+// of the CG's sums differs.
+template <int DIM, typename T>
+__device__ void wave_bottom(const Lev<T> &L, const CoarseOpts &o)
+{
+    const int q = threadIdx.x & 63;
+    const int nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz, plane = nx * ny;
+    const bool live = q < n;
+    const int qc = live ? q : 0;
+    const int i = qc % nx, j = (qc / nx) % ny, k = qc / plane;
+    // neighbour lanes (-1: outside a non-periodic side) and face weights
+    int nb[6];
+    nb[0] = i > 0 ? qc - 1 : (L.perx ? qc + nx - 1 : -1);
+    nb[1] = i < nx - 1 ? qc + 1 : (L.perx ? qc - (nx - 1) : -1);
+    nb[2] = j > 0 ? qc - nx : (L.pery ? qc + (ny - 1) * nx : -1);
+    nb[3] = j < ny - 1 ? qc + nx : (L.pery ? qc - (ny - 1) * nx : -1);
+    if (DIM == 3) {
+        nb[4] = k > 0 ? qc - plane : (L.perz ? qc + (nz - 1) * plane : -1);
+        nb[5] = k < nz - 1 ? qc + plane : (L.perz ? qc - (nz - 1) * plane : -1);
+    } else {
+        nb[4] = nb[5] = -1;
+    }
+    T w[6];
+    w[0] = L.bx[qc];
+    w[1] = (i < nx - 1) ? L.bx[qc + 1] : L.bxh[j + ny * k];
+    w[2] = L.by[qc];
+    w[3] = (j < ny - 1) ? L.by[qc + nx] : L.byh[i + nx * k];
+    if (DIM == 3) {
+        w[4] = L.bz[qc];
+        w[5] = (k < nz - 1) ? L.bz[qc + plane] : L.bzh[i + nx * j];
+    } else {
+        w[4] = w[5] = T(0);
+    }
+    const int colour = (i + j + k) & 1;
+    T phi = live ? L.phi[qc] : T(0);
+    const T rhs = live ? L.rhs[qc] : T(0);
+    auto gather = [&](auto v, auto *p) {   // neighbour values of v (all lanes shuffle)
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const auto t = __shfl(v, nb[s] < 0 ? q : nb[s], 64);
+            p[s] = nb[s] < 0 ? decltype(v)(0) : t;
+        }
+    };
+    auto sweeps = [&](int count) {
+        for (int it = 0; it < count; ++it)
+            for (int c = 0; c < 2; ++c) {
+                T p[6];
+                gather(phi, p);
+                const T v = gs_value<DIM, T>(L, w, p, rhs);
+                if (live && colour == c) phi = v;
+            }
+    };
+    sweeps(o.nub);
+    // ---- CG in FP64 on A x = rhs (- mean)
+    Lev<double> D;
+    D.nx = nx; D.ny = ny; D.nz = nz;
+    D.perx = L.perx; D.pery = L.pery; D.perz = L.perz;
+    D.ih2x = L.ih2x; D.ih2y = L.ih2y; D.ih2z = L.ih2z;
+    double wd[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) wd[s] = (double)w[s];
+    auto apply = [&](double x) -> double {
+        double p[6];
+        gather(x, p);
+        return live ? apply_value<DIM, double>(D, wd, p, x) : 0.0;
+    };
+    const double nd = (double)n;
+    const double m = o.singular ? wave_sum(live ? (double)rhs : 0.0) / nd : 0.0;
+    double x = live ? (double)phi : 0.0;
+    double r = live ? ((double)rhs - m) - apply(x) : 0.0;
+    if (o.singular) {
+        const double rm = wave_sum(r) / nd;
+        if (live) r -= rm;
+    }
+    double p = r;
+    const double r0 = wave_max(fabs(r));
+    double rr = wave_sum(r * r);
+    if (r0 > 0.0) {
+        for (int it = 0; it < o.bottom_maxit; ++it) {
+            const double Ap = apply(p);
+            const double pap = wave_sum(p * Ap);
+            if (!(pap > 0.0)) break;
+            const double alpha = rr / pap;
+            x += alpha * p;
+            r -= alpha * Ap;
+            if (o.singular) {
+                const double rm = wave_sum(r) / nd;
+                if (live) r -= rm;
+            }
+            const double am = wave_max(fabs(r));
+            const double rr1 = wave_sum(r * r);
+            if (am <= o.bottom_eps * r0) break;
+            const double beta = rr1 / rr;
+            rr = rr1;
+            p = r + beta * p;
+        }
+    }
+    phi = live ? (T)x : T(0);
+    sweeps(o.nub);
+    if (live) L.phi[qc] = phi;
+}
+
 template <int DIM, typename T>
 __device__ void blk_resid_restrict(const Lev<T> &F, const Lev<T> &C)
 {
@@ -2037,9 +2152,16 @@ __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict_
     }
     {
         const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, bot, smem);
-        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
-        blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
-        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+        if (L.nx * L.ny * L.nz <= 64) {
+            // a bottom level of at most 64 cells is solved by the first wave
+            // alone, one cell per lane in registers (no block barriers)
+            if (threadIdx.x < 64) wave_bottom<DIM, T>(L, o);
+            __syncthreads();
+        } else {
+            for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+            blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
+            for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+        }
     }
     for (int l = bot - 1; l >= lc; --l) {
         const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
