@@ -92,10 +92,17 @@ def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1, 
     for w in range(warmup):
         du, dp = sets[w]
         stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
-    plan.profile(profile)
+    # per-kernel HIP events bracket the launches of the last timed step only:
+    # each bracket stalls the stream for a few microseconds, so instrumenting
+    # every step would slow the measured throughput itself
+    if profile:   # create the event pool outside the timed region
+        plan.profile(True)
+    plan.profile(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
+        if profile and k == steps - 1:
+            plan.profile(True)
         du, dp = sets[warmup + k]
         stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
     torch.cuda.synchronize()
@@ -276,7 +283,7 @@ def main():
                           "kernel": sweep_kernel_name(8) + " (finest level, one full red-black sweep)",
                           "avg_launch_us": round(1e3 * avg64, 2), "launches": cnt64},
         "kernel_ms_per_solve": {
-            p: {k: round(v[1] / args.steps, 3) for k, v in results[p]["prof"].items()}
+            p: {k: round(v[1], 3) for k, v in results[p]["prof"].items()}
             for p in ("fp64", "mixed")},
         "accuracy": acc,
         "cpu_baseline": cpu,
