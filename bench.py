@@ -34,19 +34,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def sweep_kernel_name(s: int) -> str:
+def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
     """Name of the finest-level sweep kernel the library launches (the
-    MAC_SWEEP environment variable selects the variant)."""
+    MAC_SWEEP environment variable selects the variant; b16: the compact
+    bfloat16-storage form of the FP32 full-row kernel)."""
     t = "double" if s == 8 else "float"
     v = int(os.environ.get("MAC_SWEEP", "1082"))
     if v == 0:
         return f"k_gsrb_fused<{t}, 0>"
     if v in (8, 16):
         return f"k_gsrb_pair<{t}, 0, {v}>"
-    if v == 1082:   # full rows: 8 rows x 1024 threads (FP32), 6 rows x 768 threads (FP64)
-        return f"k_gsrb_rows<{t}, 0, 8, 1024>" if s == 4 else f"k_gsrb_rows<{t}, 0, 6, 768>"
-    if v == 1042:
-        return f"k_gsrb_rows<{t}, 0, 4, 512>"
+    c = "true" if b16 else "false"
+    if v == 1082:   # full rows: 8 rows x 1024 threads (FP32), 6 rows x 768 threads (FP64); W = nx / 128
+        w = max(1, nx // 128)
+        return (f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}>" if s == 4
+                else f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}>")
     return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}>"
 
 
@@ -66,11 +68,13 @@ def measured_traffic(name: str):
     return None
 
 
-def gsrb_launch_bytes(ncell: int, s: int) -> float:
+def gsrb_launch_bytes(ncell: int, s: int, ws: int = None) -> float:
     """Algorithmic bytes of one launch of the fused 3D red-black sweep (one
-    full sweep): three face weights, rhs, the old iterate read and the new
-    one written, each once per cell -> 6 s per cell (s = element size)."""
-    return 6.0 * s * ncell
+    full sweep): three face weights and rhs (element size ws), the old
+    iterate read and the new one written (element size s), each once per
+    cell -> 2 s + 4 ws per cell."""
+    ws = s if ws is None else ws
+    return (2.0 * s + 4.0 * ws) * ncell
 
 
 def stage(per_box, copies, torch):
@@ -123,6 +127,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
     ap.add_argument("--no-profile", action="store_true",
                     help="time the steps without the per-kernel HIP events (no roofline figures)")
+    ap.add_argument("--headline", default="mixed_bf16", choices=("mixed", "mixed_bf16"),
+                    help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=1,
                     help="FP32 V-cycles per refinement step of the mixed path")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -163,7 +169,8 @@ def main():
     ncell = n[0] * n[1] * n[2]
 
     results = {}
-    for name, prec in (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED)):
+    paths = (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED), ("mixed_bf16", mac.PREC_MIXED_BF16))
+    for name, prec in paths:
         # refresh the staged inputs from the host copies (outside timing)
         boxes, umac, phi, _, _, _ = pb
         for du, dp in sets:
@@ -183,16 +190,24 @@ def main():
               f"iters {[s['iters'] for s in st]}, res/res0 {st[-1]['res'] / st[-1]['res0']:.2e}",
               file=sys.stderr, flush=True)
 
-    mixed, fp64 = results["mixed"], results["fp64"]
+    def sweep_roofline(path):
+        """Roofline of the finest-level sweep of one path (the launches that
+        read the old iterate)."""
+        s_, ws, b16, cls = {"fp64": (8, 8, False, "gsrb64"), "mixed": (4, 4, False, "gsrb32"),
+                            "mixed_bf16": (4, 2, True, "gsrb32")}[path]
+        cnt_, ms_ = results[path]["prof"][cls]
+        avg = ms_ / cnt_ if cnt_ else float("inf")   # --no-profile: no per-kernel timing
+        nbytes = gsrb_launch_bytes(ncell, s_, ws)
+        ach = nbytes / (avg * 1e-3) / 1e9
+        name = sweep_kernel_name(s_, b16, n[0])
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
+                "kernel": name + " (finest level, one full red-black sweep)",
+                "avg_launch_us": round(1e3 * avg, 2), "launches": cnt_, "bytes_per_launch": nbytes}
+
+    head = args.headline
+    mixed, fp64 = results[head], results["fp64"]
     value = args.steps / mixed["seconds"]
-    cnt, ms = mixed["prof"]["gsrb32"]
-    cnt64, ms64 = fp64["prof"]["gsrb64"]
-    if cnt == 0 or cnt64 == 0:   # --no-profile: no per-kernel timing
-        avg_ms = avg64 = float("inf")
-    else:
-        avg_ms, avg64 = ms / cnt, ms64 / cnt64
-    achieved = gsrb_launch_bytes(ncell, 4) / (avg_ms * 1e-3) / 1e9
-    ach64 = gsrb_launch_bytes(ncell, 8) / (avg64 * 1e-3) / 1e9
 
     # accuracy of the mixed path against the FP64 GPU path and the oracle
     dim = 3
@@ -207,7 +222,6 @@ def main():
 
     acc = {}
     if not args.no_check:
-        fm, pm = gather(mixed)
         f6, p6 = gather(fp64)
 
         def errs(fa, pa, fb, pb_):
@@ -218,7 +232,9 @@ def main():
                 "u_max": float(max(np.abs(fa[d] - fb[d]).max() / np.abs(fb[d]).max() for d in range(dim))),
                 "u_rel": float(max(np.linalg.norm(fa[d] - fb[d]) / np.linalg.norm(fb[d]) for d in range(dim))),
             }
-        acc["mixed_vs_fp64_gpu"] = errs(fm, pm, f6, p6)
+        for path in ("mixed", "mixed_bf16"):
+            fm, pm = gather(results[path])
+            acc[f"{path}_vs_fp64_gpu"] = errs(fm, pm, f6, p6)
 
     cpu = None
     if not args.no_cpu:
@@ -233,10 +249,10 @@ def main():
                "sample": f"one full 256^3 FP64 solve of the same inputs by the C/OpenMP oracle "
                          f"({ref['stats'].iters} V-cycles, {tc:.1f} s)"}
         if not args.no_check:
-            fm, pm = gather(mixed)
-            f6, p6 = gather(fp64)
             po = ref["phi"] - ref["phi"].mean()
-            for tag, (fa, pa) in (("mixed_vs_oracle", (fm, pm)), ("fp64_vs_oracle", (f6, p6))):
+            for path in ("fp64", "mixed", "mixed_bf16"):
+                tag = f"{path}_vs_oracle"
+                fa, pa = gather(results[path])
                 pa0 = pa - pa.mean()
                 acc[tag] = {
                     "phi_max": float(np.abs(pa0 - po).max() / np.abs(po).max()),
@@ -248,6 +264,8 @@ def main():
                 }
         print(f"# cpu oracle: {tc:.2f} s/solve on {O.num_threads()} threads", file=sys.stderr)
 
+    storage = {"mixed": "FP32 V-cycle (FP32 storage) inside FP64 IR",
+               "mixed_bf16": "FP32 V-cycle inside FP64 IR, finest-level weights and rhs stored as bfloat16"}
     line = {
         "metric": "MAC solves/s at 256^3 to 1e-10 resid: FP32-MG+FP64 IR vs FP64; HBM GB/s",
         "value": round(value, 3),
@@ -262,29 +280,19 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic (seeded isothermal stratification, 1% density noise, U~U[-1,1])",
         "config": {"workload": "3D 256^3 plane-parallel MAC projection, 64 boxes of 64^3, "
-                               "x/y periodic, z walls, beta0 contrast 1e4, mixed FP32-MG + FP64 IR, "
-                               "rel residual 1e-10",
+                               "x/y periodic, z walls, beta0 contrast 1e4, rel residual 1e-10",
+                   "path": storage[head],
                    "cells": ncell, "boxes": len(boxes), "parallelism": "single GPU"},
         "fp64_solves_per_s": round(args.steps / fp64["seconds"], 3),
         "fp64_ms_per_step": round(1e3 * fp64["seconds"] / args.steps, 3),
+        "solves_per_s": {p: round(args.steps / results[p]["seconds"], 3) for p in results},
         "mixed_over_fp64": round(fp64["seconds"] / mixed["seconds"], 3),
         "iters": {"fp64_vcycles": fp64["stats"][-1]["iters"],
-                  "mixed_ir_steps": mixed["stats"][-1]["iters"],
-                  "mixed_fp32_vcycles": mixed["stats"][-1]["inner_cycles"]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(sweep_kernel_name(4)),
-                     "kernel": sweep_kernel_name(4) + " (finest level, one full red-black sweep)",
-                     "avg_launch_us": round(1e3 * avg_ms, 2), "launches": cnt,
-                     "bytes_per_launch": gsrb_launch_bytes(ncell, 4)},
-        "roofline_fp64": {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": round(ach64 / HBM_PEAK_GBS, 4),
-                          "traffic": measured_traffic(sweep_kernel_name(8)),
-                          "kernel": sweep_kernel_name(8) + " (finest level, one full red-black sweep)",
-                          "avg_launch_us": round(1e3 * avg64, 2), "launches": cnt64},
+                  **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"] for p in ("mixed", "mixed_bf16")}},
+        "roofline": sweep_roofline(head),
+        "roofline_other": {p: sweep_roofline(p) for p in results if p != head},
         "kernel_ms_per_solve": {
-            p: {k: round(v[1], 3) for k, v in results[p]["prof"].items()}
-            for p in ("fp64", "mixed")},
+            p: {k: round(v[1], 3) for k, v in results[p]["prof"].items()} for p in results},
         "accuracy": acc,
         "cpu_baseline": cpu,
     }

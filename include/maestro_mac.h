@@ -52,6 +52,11 @@ extern "C" {
 /* precision of the V-cycle */
 #define MAC_PREC_FP64 0          /* plain FP64 V-cycle (the reference) */
 #define MAC_PREC_MIXED 1         /* FP32 V-cycle inside FP64 refinement */
+#define MAC_PREC_MIXED_BF16 2    /* as MAC_PREC_MIXED, with the finest level's
+                                    face weights and right side stored as
+                                    bfloat16 (iterate and arithmetic FP32) where
+                                    the 3D full-row kernels apply (nx a multiple
+                                    of 128, nx <= 256); FP32 storage elsewhere */
 
 typedef struct mac_layout {
     int dim;                /* 2 or 3 */
@@ -84,7 +89,8 @@ typedef struct mac_opts {
     int max_levels;          /* 0: coarsen down to 2 cells per direction     */
     int use_alt_energy_fix;  /* 1: beta = beta0^2/rho (reference default,
                                 Source/_parameters:552-554); 0: beta0/rho    */
-    int precision;           /* MAC_PREC_FP64 or MAC_PREC_MIXED              */
+    int precision;           /* MAC_PREC_FP64, MAC_PREC_MIXED or
+                                MAC_PREC_MIXED_BF16                          */
     int inner_cycles;        /* FP32 V-cycles per refinement step (mixed)    */
     int device_ptrs;         /* 1: every array pointer is a device pointer;
                                 0: host pointers (copied in and out)         */
@@ -186,7 +192,10 @@ int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
                                     sweep, as the V-cycle's first upward sweep */
 #define MAC_OP_GSRB32 8          /* `count` sweeps in FP32 with the kernels of the
                                     mixed path: weights, rhs and phi of the level
-                                    rounded to FP32, the result widened into phi */
+                                    rounded to FP32 (weights and rhs further to
+                                    bfloat16 on level 0 of a MAC_PREC_MIXED_BF16
+                                    plan where it stores them so), the result
+                                    widened into phi */
 int mac_debug_level_dims(mac_plan *plan, int level, int dims[3]);
 int mac_debug_get(mac_plan *plan, int level, int what, double *out);
 int mac_debug_set(mac_plan *plan, int level, int what, const double *in);

@@ -41,6 +41,30 @@ struct BoxDev {
     int hi[3];
 };
 
+// Level 0 of the FP32 hierarchy in compact storage: the face weights (same
+// layout as Lev::bx.. including the high-face slabs) and the right side as
+// bfloat16 bit patterns.  The iterate stays FP32 and all arithmetic is FP32;
+// a bfloat16 value widens to FP32 exactly (its bits in the high half).
+struct Compact16 {
+    unsigned short *bx, *by, *bz;
+    unsigned short *bxh, *byh, *bzh;
+    unsigned short *rhs;
+};
+
+// FP32 -> bfloat16, round to nearest even (finite inputs)
+__host__ __device__ __forceinline__ unsigned short f32_to_bf16(float f)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, f);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+// the two bfloat16 values packed in a 32-bit word (low half first)
+__device__ __forceinline__ float bf16_lo(unsigned u) { return __uint_as_float(u << 16); }
+__device__ __forceinline__ float bf16_hi(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
+__device__ __forceinline__ unsigned bf16_pack(float lo, float hi)
+{
+    return (unsigned)f32_to_bf16(lo) | ((unsigned)f32_to_bf16(hi) << 16);
+}
+
 // ------------------------------------------------------------ math helpers
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __fma_rn(a, b, c); }
 __device__ __forceinline__ float dfma(float a, float b, float c) { return __fmaf_rn(a, b, c); }

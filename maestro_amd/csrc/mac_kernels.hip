@@ -264,6 +264,13 @@ __global__ void __launch_bounds__(TPB) k_weights_to_f32(Lev<double> S, Lev<float
     if (DIM == 3 && K == 0) D.bzh[I + (long)S.nx * J] = (float)S.bzh[I + (long)S.nx * J];
 }
 
+__global__ void __launch_bounds__(TPB) k_to_bf16(const float *__restrict__ in, unsigned short *__restrict__ out,
+                                                 long n)
+{
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    if (q < n) out[q] = f32_to_bf16(in[q]);
+}
+
 template <typename A, typename B>
 __global__ void __launch_bounds__(TPB) k_convert(const A *__restrict__ in, B *__restrict__ out, long n)
 {
@@ -703,22 +710,66 @@ __device__ __forceinline__ void st_off(void *base, unsigned boff, const V &v)
                                                r, boff, 0, 0);
 }
 
-template <typename T, int MODE, int EY, int NTH>
-__global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
-                                                    T *__restrict__ pout, RowSweepGrid G)
+__device__ __forceinline__ unsigned short ld_u16(const void *base, unsigned boff)
 {
+    return __builtin_amdgcn_raw_buffer_load_b16(buf_rsrc(base), boff, 0, 0);
+}
+
+template <typename V>
+__device__ __forceinline__ V ld_rs(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    static_assert(sizeof(V) == 2 || sizeof(V) == 4 || sizeof(V) == 8 || sizeof(V) == 16, "buffer load width");
+    if constexpr (sizeof(V) == 2)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+    else if constexpr (sizeof(V) == 4)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else if constexpr (sizeof(V) == 8)
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+template <typename V>
+__device__ __forceinline__ void st_rs(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const V &v)
+{
+    static_assert(sizeof(V) == 8 || sizeof(V) == 16, "buffer store width");
+    if constexpr (sizeof(V) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                              r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
+                                               r, voff, soff, 0);
+}
+
+// Full-row sweep, nx = 128 W (W a template parameter, so every LDS offset is
+// an immediate).  A wave covers 128 columns of one row, two per lane (one red
+// and one black cell on every plane).  Each array's buffer resource is built
+// once; a plane's byte offset goes in the scalar offset of the load (the
+// arrays are below 4 GiB).  The LDS rows carry a zero pad column on each
+// side; an x-periodic neighbour across the row end is read from the opposite
+// end of the same LDS row (per-lane offsets fixed for the whole march), so no
+// pad is ever written after the initial clear.  Plane values rotate through
+// four LDS buffers (plane m in buffer m & 3, a compile-time slot of the
+// 8-step unroll), one barrier per plane.
+// B16: the face weights and right side come from compact (bfloat16) storage
+// K instead of L; the arithmetic is the same FP32 code on the widened values.
+template <typename T, int MODE, int EY, int NTH, bool B16, int W>
+__global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
+                                                    T *__restrict__ pout, RowSweepGrid G, Compact16 K)
+{
+    static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
     using T2 = typename Vec2<T>::type;
-    constexpr int FY = EY - 2, PY = EY + 2, NI = 13;
+    constexpr int NX = 128 * W, FY = EY - 2, PY = EY + 2, NI = 13;
+    constexpr int PXL = NX + 4;      // LDS row: pad (x = -1) at 1, column x at 2 + x, pad (x = nx) at NX + 2
+    constexpr int PS = PY * PXL;     // LDS plane
+    constexpr unsigned ES = sizeof(T), WS = B16 ? 2u : (unsigned)sizeof(T);   // iterate / weight bytes
     const int b = blockIdx.x;
     const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
     if (idx >= G.total) return;
     const int chunk = idx / G.nty, tyi = idx - chunk * G.nty;
-    const int nx = L.nx, ny = L.ny, nz = L.nz, plane = nx * ny;
-    const int PXL = nx + 4;   // LDS row: 2 pad (x = -1 at 1), nx columns from 2, pad (x = nx)
-    const int PS = PY * PXL;                            // LDS plane
+    const int ny = L.ny, nz = L.nz, plane = NX * ny;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int w = wv % G.W, ty = wv / G.W;              // wave-uniform
+    const int w = wv % W, ty = wv / W;                  // wave-uniform
     const int x0 = 2 * (lane + 64 * w);
     const int Y = tyi * FY - 1 + ty;
     int Yw;
@@ -726,34 +777,59 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
     wrap_coord(Y, ny, L.pery, Yw, oky);
     const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
     const bool outrow = oky && ty >= 1 && ty <= FY && Y < ny;
-    const unsigned col = (unsigned)(x0 + nx * Yw);      // per-lane offset within a plane
     const bool ring = (ty == 0) || (ty == EY - 1);
     int y2w = 0;
     bool ok2 = false;
     if (ring) wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, L.pery, y2w, ok2);
-    const unsigned col2 = (unsigned)(x0 + nx * y2w);
-    const unsigned colyn = (Yw < ny - 1) ? col + nx : (unsigned)x0;   // y-high faces: by or byh
-    const T *const byn_base = (Yw < ny - 1) ? L.by : L.byh;
-    const unsigned byn_plane = (Yw < ny - 1) ? (unsigned)plane : (unsigned)nx;
-    constexpr unsigned ES = sizeof(T);
-    const unsigned bcol = col * ES, bcol2 = col2 * ES, bcolyn = colyn * ES;   // byte offsets
-    const int lpos = (ty + 1) * PXL + 2 + x0;           // LDS index of column x0 (even)
-    const int lpos2 = (ty == 0 ? 0 : EY + 1) * PXL + 2 + x0;
-    const bool xbeg = (x0 == 0), xend = (x0 + 2 == nx), mirror = L.perx != 0;
+    const bool ylast = Yw == ny - 1, xend = x0 + 2 == NX;
+    // per-lane byte offsets within a plane
+    const unsigned col = (unsigned)(x0 + NX * Yw);
+    const unsigned vw = col * WS, vp = col * ES, vp2 = (unsigned)(x0 + NX * y2w) * ES;
+    const unsigned vyn = (ylast ? (unsigned)x0 : col + NX) * WS;    // y-high faces: by or the byh slab
+    const unsigned vx2 = (xend ? col : col + 2u) * WS;             // third x face (bxh replaces it on xend)
+    const unsigned bynstride = ylast ? (unsigned)NX : (unsigned)plane;
+    const void *p_bx, *p_by, *p_bz, *p_rh, *p_byn, *p_bxh, *p_bzh;
+    if constexpr (B16) {
+        p_bx = K.bx; p_by = K.by; p_bz = K.bz; p_rh = K.rhs; p_byn = ylast ? K.byh : K.by;
+        p_bxh = K.bxh; p_bzh = K.bzh;
+    } else {
+        p_bx = L.bx; p_by = L.by; p_bz = L.bz; p_rh = L.rhs; p_byn = ylast ? L.byh : L.by;
+        p_bxh = L.bxh; p_bzh = L.bzh;
+    }
+    const __amdgpu_buffer_rsrc_t Rbx = buf_rsrc(p_bx), Rby = buf_rsrc(p_by), Rbz = buf_rsrc(p_bz),
+                                 Rrh = buf_rsrc(p_rh), Rbyn = buf_rsrc(p_byn), Rbxh = buf_rsrc(p_bxh),
+                                 Rbzh = buf_rsrc(p_bzh), Rin = buf_rsrc(pin), Rout = buf_rsrc(pout);
+    // a column pair / one value of a weight or rhs array
+    auto ldw = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) -> T2 {
+        if constexpr (B16) {
+            const unsigned u = ld_rs<unsigned>(r, voff, soff);
+            T2 v;
+            v.x = bf16_lo(u);
+            v.y = bf16_hi(u);
+            return v;
+        } else {
+            return ld_rs<T2>(r, voff, soff);
+        }
+    };
+    auto ldw1 = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) -> T {
+        if constexpr (B16) return bf16_lo((unsigned)ld_rs<unsigned short>(r, voff, soff));
+        else return ld_rs<T>(r, voff, soff);
+    };
+    // LDS element indices: own column pair, ring-2 row, x-neighbours x0-1 and x0+2
+    const int lp = (ty + 1) * PXL + 2 + x0;
+    const int lp2 = (ty == 0 ? 0 : EY + 1) * PXL + 2 + x0;
+    const int lxl = (x0 == 0 && L.perx) ? lp + NX - 1 : lp - 1;
+    const int lxr = (xend && L.perx) ? lp - (NX - 2) : lp + 2;
     extern __shared__ __align__(16) unsigned char rows_smem[];
     T *lds = (T *)rows_smem;
-    for (int q = threadIdx.x; q < 3 * PS; q += blockDim.x) lds[q] = T(0);
+    for (int q = threadIdx.x; q < 4 * PS; q += blockDim.x) lds[q] = T(0);
 
-    auto kwrap = [&](int k, int &kw) -> bool {
-        if (k >= 0 && k < nz) { kw = k; return true; }
-        if (L.perz) {
-            while (k < 0) k += nz;
-            while (k >= nz) k -= nz;
-            kw = k;
-            return true;
-        }
-        kw = 0;
-        return false;
+    // plane k (at most one period outside the domain) -> stored plane kw;
+    // false outside a non-periodic side (kw = 0 stands in)
+    auto zwrap = [&](int k, int &kw) -> bool {
+        const bool in = (unsigned)k < (unsigned)nz;
+        kw = in ? k : (L.perz ? (k < 0 ? k + nz : k - nz) : 0);
+        return in || L.perz;
     };
     // zero pairs are built from immediates (a zero object in memory would
     // turn "cond ? load : zero" into a flat load through a selected pointer)
@@ -765,45 +841,51 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
     };
     // old iterate of the column pair on plane k (zero outside / in MODE 1)
     auto ldphi = [&](int k) -> T2 {
-        int kw;
-        if (MODE == 0 && oky && kwrap(k, kw)) return ld_off<T2>(pin + (size_t)plane * kw, bcol);
-        return z2();
+        if constexpr (MODE == 1) {
+            return z2();
+        } else {
+            int kw;
+            const bool ok = zwrap(k, kw) && oky;
+            const T2 v = ld_rs<T2>(Rin, vp, (unsigned)kw * (unsigned)plane * ES);
+            return ok ? v : z2();
+        }
     };
     // per-plane inputs: [0..2] x faces at x0, x0+1, x0+2; [3,4] y-low faces;
     // [5,6] y-high faces; [7,8] z-low faces; [9,10] rhs; [11,12] ring-2 old
     // values (ring waves).  The z-high faces of plane k are the z-low faces of
     // plane k+1 (next slot of the ring), except on the domain's top plane,
-    // whose high faces come from the bzh slab.  Every entry is assigned and every load
-    // is made from an in-range address (plane 0 stands in for a plane outside
-    // a non-periodic side, row 0 for a row outside; those values are never
-    // used), so the queue stays in registers.  Addresses are a wave-uniform
-    // plane base plus the lane's 32-bit offset.
+    // whose high faces come from the bzh slab.  Every entry is assigned and every
+    // load is made from an in-range address (plane 0 stands in for a plane
+    // outside a non-periodic side, row 0 for a row outside; those values are
+    // never used), so the queue stays in registers.
     struct In {
         T v[NI];
     };
     auto fetch = [&](int k) -> In {
         In o;
         int kw;
-        const bool kok = kwrap(k, kw);
-        const size_t pk = (size_t)plane * kw;
-        const T2 vx = ld_off<T2>(L.bx + pk, bcol);
+        const bool kok = zwrap(k, kw);
+        const unsigned sw = (unsigned)kw * (unsigned)plane * WS;
+        const T2 vx = ldw(Rbx, vw, sw);
         o.v[0] = vx.x;
         o.v[1] = vx.y;
-        o.v[2] = xend ? L.bxh[Yw + ny * kw] : ld_off<T>(L.bx + pk + 2, bcol);
-        const T2 vy = ld_off<T2>(L.by + pk, bcol);
+        const T bx2 = ldw1(Rbx, vx2, sw);
+        const T bxh = ldw1(Rbxh, 0u, (unsigned)(Yw + ny * kw) * WS);
+        o.v[2] = xend ? bxh : bx2;
+        const T2 vy = ldw(Rby, vw, sw);
         o.v[3] = vy.x;
         o.v[4] = vy.y;
-        const T2 vyn = ld_off<T2>(byn_base + (size_t)byn_plane * kw, bcolyn);
-        o.v[5] = vyn.x;
-        o.v[6] = vyn.y;
-        const T2 vz = ld_off<T2>(L.bz + pk, bcol);
+        const T2 vn = ldw(Rbyn, vyn, (unsigned)kw * bynstride * WS);
+        o.v[5] = vn.x;
+        o.v[6] = vn.y;
+        const T2 vz = ldw(Rbz, vw, sw);
         o.v[7] = vz.x;
         o.v[8] = vz.y;
-        const T2 vr = ld_off<T2>(L.rhs + pk, bcol);
+        const T2 vr = ldw(Rrh, vw, sw);
         o.v[9] = vr.x;
         o.v[10] = vr.y;
         T2 va = z2();
-        if (MODE == 0 && ring) va = ld_off<T2>(pin + pk, bcol2);
+        if (MODE == 0 && ring) va = ld_rs<T2>(Rin, vp2, (unsigned)kw * (unsigned)plane * ES);
         const bool use2 = kok && ok2;
         o.v[11] = use2 ? va.x : T(0);
         o.v[12] = use2 ? va.y : T(0);
@@ -812,7 +894,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
     const int kb = z0 - 1, ke = z1;
     const int nsteps = ((ke - kb + 1) + 7) & ~7;   // a multiple of the unroll
     // rings indexed by the step m = k - kb: inputs of plane m in wq[m & 3],
-    // old/updated iterate of plane m in ph[m & 7], LDS plane m in buffer m % 3
+    // old/updated iterate of plane m in ph[m & 7], LDS plane m in buffer m & 3
     In wq[4];
     struct Ring {
         T v[16];
@@ -831,11 +913,11 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
     };
     put(std::integral_constant<int, 7>{}, ldphi(kb - 1));
     put(std::integral_constant<int, 0>{}, ldphi(kb));
-    put(std::integral_constant<int, 1>{}, ldphi(kb + 1));
-    put(std::integral_constant<int, 2>{}, ldphi(kb + 2));
-    put(std::integral_constant<int, 3>{}, ldphi(kb + 3));
-    put(std::integral_constant<int, 4>{}, ldphi(kb + 4));
-    put(std::integral_constant<int, 5>{}, ldphi(kb + 5));
+    put(std::integral_constant<int, 1>{}, ldphi(min(kb + 1, ke + 1)));
+    put(std::integral_constant<int, 2>{}, ldphi(min(kb + 2, ke + 1)));
+    put(std::integral_constant<int, 3>{}, ldphi(min(kb + 3, ke + 1)));
+    put(std::integral_constant<int, 4>{}, ldphi(min(kb + 4, ke + 1)));
+    put(std::integral_constant<int, 5>{}, ldphi(min(kb + 5, ke + 1)));
     PH(6, 0) = PH(6, 1) = T(0);
     wq[0] = fetch(kb);
     wq[1] = fetch(min(kb + 1, ke + 1));
@@ -845,29 +927,23 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
         constexpr int S = decltype(SC)::value;
         constexpr int C0 = S & 3, CP = (S + 3) & 3, C1 = (S + 1) & 3, CN = (S + 2) & 3;   // m, m-1, m+1, m+2
         constexpr int F0 = S, FM1 = (S + 7) & 7, FM2 = (S + 6) & 7, FP1 = (S + 1) & 7;
+        T *const buf = lds + (S & 3) * PS, *const pbuf = lds + ((S + 3) & 3) * PS;
         const int k = kb + m;
-        // LDS planes rotate through three buffers (plane m in buffer m mod 3)
-        const int b3 = m % 3;
-        T *buf = lds + b3 * PS, *pbuf = lds + (b3 == 0 ? 2 : b3 - 1) * PS;
         int kw;
-        const bool kok = kwrap(k, kw) && k <= ke;
+        const bool kok = zwrap(min(k, ke + 1), kw) && k <= ke;
         wq[CN] = fetch(min(k + 2, ke + 1));
         // (a) publish the old values of plane k (and the ring-2 row)
         if (oky) {
             T2 v;
             v.x = kok ? PH(F0, 0) : T(0);
             v.y = kok ? PH(F0, 1) : T(0);
-            *(T2 *)(buf + lpos) = v;
-            if (mirror && xbeg) buf[lpos + nx] = v.x;        // pad x = nx <- column 0
-            if (mirror && xend) buf[lpos - nx + 1] = v.y;    // pad x = -1 <- column nx-1
+            *(T2 *)(buf + lp) = v;
         }
         if (ring) {
             T2 v;
             v.x = kok ? wq[C0].v[11] : T(0);
             v.y = kok ? wq[C0].v[12] : T(0);
-            *(T2 *)(buf + lpos2) = v;
-            if (mirror && xbeg) buf[lpos2 + nx] = v.x;
-            if (mirror && xend) buf[lpos2 - nx + 1] = v.y;
+            *(T2 *)(buf + lp2) = v;
         }
         __syncthreads();
         // r is uniform over the wave (one row, one plane): each colour case
@@ -881,7 +957,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
         // z-low faces, or the bzh slab on the domain's top plane
         T zh0, zh1, zb0, zb1;
         if (kw == nz - 1) {
-            const T2 t = ld_off<T2>(L.bzh, bcol);
+            const T2 t = ldw(Rbzh, vw, 0u);
             zh0 = t.x;
             zh1 = t.y;
         } else {
@@ -889,7 +965,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
             zh1 = wq[C1].v[8];
         }
         if (km == nz - 1) {
-            const T2 t = ld_off<T2>(L.bzh, bcol);
+            const T2 t = ldw(Rbzh, vw, 0u);
             zb0 = t.x;
             zb1 = t.y;
         } else {
@@ -912,17 +988,15 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
                 T xm, xp;
                 if constexpr (R == 1) {
                     xm = f0a;
-                    xp = buf[lpos + 2];
+                    xp = buf[lxr];
                 } else {
-                    xm = buf[lpos - 1];
+                    xm = buf[lxl];
                     xp = f0b;
                 }
-                const T nb[6] = {xm, xp, buf[lpos - PXL + R], buf[lpos + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
+                const T nb[6] = {xm, xp, buf[lp - PXL + R], buf[lp + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
                 red_v = gs_value<3, T>(L, wt, nb, cw.v[9 + R]);
-                buf[lpos + R] = red_v;
+                buf[lp + R] = red_v;
                 f0R = red_v;
-                if (R == 0 && mirror && xbeg) buf[lpos + nx] = red_v;
-                if (R == 1 && mirror && xend) buf[lpos - nx + 1] = red_v;
             }
             if (do_black) {
                 // (c) the black cell of plane k-1 (same column R); the other
@@ -931,17 +1005,17 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
                 T xm, xp;
                 if constexpr (R == 1) {
                     xm = m1a;
-                    xp = pbuf[lpos + 2];
+                    xp = pbuf[lxr];
                 } else {
-                    xm = pbuf[lpos - 1];
+                    xm = pbuf[lxl];
                     xp = m1b;
                 }
-                const T nb[6] = {xm, xp, pbuf[lpos - PXL + R], pbuf[lpos + PXL + R], R ? m2b : m2a, f0R};
+                const T nb[6] = {xm, xp, pbuf[lp - PXL + R], pbuf[lp + PXL + R], R ? m2b : m2a, f0R};
                 blk_v = gs_value<3, T>(L, wt, nb, pw.v[9 + R]);
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
-                st_off<T2>(pout + (size_t)plane * km, bcol, o);
+                st_rs<T2>(Rout, vp, (unsigned)km * (unsigned)plane * ES, o);
             }
         };
         if (r) update(std::integral_constant<int, 1>{});
@@ -1333,10 +1407,12 @@ struct RowMarchGrid {
     int per_xcd;   // blocks per XCD
 };
 
-template <int MODE>
+// B16: the FP32 right side goes out as bfloat16 (r16) instead of r32
+template <int MODE, bool B16>
 __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *__restrict__ e,
                                                    double *__restrict__ phi_out, float *__restrict__ r32,
-                                                   unsigned long long *nrm, RowMarchGrid G)
+                                                   unsigned short *__restrict__ r16, unsigned long long *nrm,
+                                                   RowMarchGrid G)
 {
     using D2 = double2;
     using F2 = float2;
@@ -1486,7 +1562,9 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
                 a = fmax(a, fabs(res[c]));
             }
             const size_t pk = (size_t)plane * k;
-            if constexpr (MODE >= 1) {
+            if constexpr (MODE >= 1 && B16) {
+                st_off<unsigned>(r16 + pk, oc * 2u, bf16_pack((float)res[0], (float)res[1]));
+            } else if constexpr (MODE >= 1) {
                 F2 o;
                 o.x = (float)res[0];
                 o.y = (float)res[1];
@@ -1514,10 +1592,12 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
 // pair 2J, 2J+1, and marches fine planes with the planes below and above in
 // registers.  Per-cell arithmetic and the summation order of the eight fine
 // residuals are those of k_resid_restrict.  3D, fine nx = 128 W.
-template <typename T>
+// B16: the fine weights and right side come from compact storage K
+template <typename T, bool B16>
 __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__ phi, Lev<T> C,
-                                                RowMarchGrid G)
+                                                RowMarchGrid G, Compact16 K)
 {
+    static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
     using T2 = typename Vec2<T>::type;
     constexpr unsigned ES = sizeof(T);
     const int lane = threadIdx.x & 63;
@@ -1548,13 +1628,29 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
     struct Z {   // own two rows on one plane: old iterate and z-low faces
         T p00, p01, p10, p11, b00, b01, b10, b11;   // [row][column]
     };
+    // a column pair of a weight / rhs array at element offset o (base: plane start)
+    auto pair = [&](const T *b32, const unsigned short *b16, unsigned o) -> T2 {
+        if constexpr (B16) {
+            const unsigned u = ld_off<unsigned>(b16, o * 2u);
+            T2 v;
+            v.x = bf16_lo(u);
+            v.y = bf16_hi(u);
+            return v;
+        } else {
+            return ld_off<T2>(b32, o * ES);
+        }
+    };
+    auto one = [&](const T *b32, const unsigned short *b16, unsigned o) -> T {
+        if constexpr (B16) return bf16_lo((unsigned)ld_u16(b16, o * 2u));
+        else return ld_off<T>(b32, o * ES);
+    };
     auto loadz = [&](int k) -> Z {
         Z z;
         int kw;
         const bool ok = kwrap(k, kw);
         const size_t pk = (size_t)plane * kw;
         const T2 a = ld_off<T2>(phi + pk, o0 * ES), c = ld_off<T2>(phi + pk, o1 * ES);
-        const T2 ba = ld_off<T2>(F.bz + pk, o0 * ES), bc = ld_off<T2>(F.bz + pk, o1 * ES);
+        const T2 ba = pair(F.bz + pk, K.bz + pk, o0), bc = pair(F.bz + pk, K.bz + pk, o1);
         z.p00 = ok ? a.x : T(0);
         z.p01 = ok ? a.y : T(0);
         z.p10 = ok ? c.x : T(0);
@@ -1572,13 +1668,22 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
         const T2 ym = ld_off<T2>(ph, oym * ES), yp = ld_off<T2>(ph, oyp * ES);
         const T xm0 = ld_off<T>(ph, oxm0 * ES), xp0 = ld_off<T>(ph, oxp0 * ES);
         const T xm1 = ld_off<T>(ph, (oxm0 + nx) * ES), xp1 = ld_off<T>(ph, (oxp0 + nx) * ES);
-        const T2 bx0 = ld_off<T2>(F.bx + pk, o0 * ES), bx1 = ld_off<T2>(F.bx + pk, o1 * ES);
-        const T bxe0 = (x0 + 2 < nx) ? ld_off<T>(F.bx + pk, (o0 + 2u) * ES) : F.bxh[j0 + ny * k];
-        const T bxe1 = (x0 + 2 < nx) ? ld_off<T>(F.bx + pk, (o1 + 2u) * ES) : F.bxh[j1 + ny * k];
-        const T2 by0 = ld_off<T2>(F.by + pk, o0 * ES), by1 = ld_off<T2>(F.by + pk, o1 * ES);
-        const T2 byn = (j1 < ny - 1) ? ld_off<T2>(F.by + pk, (o1 + nx) * ES)
-                                     : ld_off<T2>(F.byh + (size_t)nx * k, (unsigned)x0 * ES);
-        const T2 rh0 = ld_off<T2>(F.rhs + pk, o0 * ES), rh1 = ld_off<T2>(F.rhs + pk, o1 * ES);
+        const T2 bx0 = pair(F.bx + pk, K.bx + pk, o0), bx1 = pair(F.bx + pk, K.bx + pk, o1);
+        T bxe0, bxe1;
+        if (x0 + 2 < nx) {
+            bxe0 = one(F.bx + pk, K.bx + pk, o0 + 2u);
+            bxe1 = one(F.bx + pk, K.bx + pk, o1 + 2u);
+        } else if constexpr (B16) {
+            bxe0 = bf16_lo((unsigned)K.bxh[j0 + ny * k]);
+            bxe1 = bf16_lo((unsigned)K.bxh[j1 + ny * k]);
+        } else {
+            bxe0 = F.bxh[j0 + ny * k];
+            bxe1 = F.bxh[j1 + ny * k];
+        }
+        const T2 by0 = pair(F.by + pk, K.by + pk, o0), by1 = pair(F.by + pk, K.by + pk, o1);
+        const T2 byn = (j1 < ny - 1) ? pair(F.by + pk, K.by + pk, o1 + nx)
+                                     : pair(F.byh + (size_t)nx * k, K.byh + (size_t)nx * k, (unsigned)x0);
+        const T2 rh0 = pair(F.rhs + pk, K.rhs + pk, o0), rh1 = pair(F.rhs + pk, K.rhs + pk, o1);
         T bzn00, bzn01, bzn10, bzn11;
         if (k < nz - 1) {
             bzn00 = zp.b00;
@@ -1586,7 +1691,7 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
             bzn10 = zp.b10;
             bzn11 = zp.b11;
         } else {
-            const T2 ha = ld_off<T2>(F.bzh, o0 * ES), hb = ld_off<T2>(F.bzh, o1 * ES);
+            const T2 ha = pair(F.bzh, K.bzh, o0), hb = pair(F.bzh, K.bzh, o1);
             bzn00 = ha.x;
             bzn01 = ha.y;
             bzn10 = hb.x;
@@ -2682,32 +2787,35 @@ static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
         hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD>), g, b, 0, s, L, pin, pout, G);
 }
 
-// Full-row sweep: usable when nx = 128 W and a block of EY rows (64 W EY
-// threads) fits in NTH.  FP32 runs 8 rows in blocks of up to 1024 threads
-// (at most 128 VGPRs per lane); FP64, whose register rings are twice as wide,
-// runs 6 rows in blocks of up to 768 (168 VGPRs).  The z-chunk count is
-// chosen so the number of blocks fills whole rounds of the resident block
-// slots (CUs x blocks per CU) with the least z-ring overhead.
 template <typename T, int EY, int NTH>
-static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
+static bool rows_fit(const Lev<T> &L)
 {
-    {
-    if (L.nx % 128 != 0) return false;
+    if (L.nx != 128 && L.nx != 256) return false;   // W = 1, 2 (compile-time row width)
+    if (L.nz < 8) return false;                      // the march wraps at most one period
     const int W = L.nx / 128;
-    if (64 * W * EY > NTH || L.nz < 2) return false;
-    const size_t lds = sizeof(T) * 3 * (size_t)(EY + 2) * (L.nx + 4);
-    if (lds > 64 * 1024) return false;
+    if (64 * W * EY > NTH) return false;
+    // plane offsets are 32-bit byte offsets
+    if ((double)L.nx * L.ny * L.nz * sizeof(T) >= 4294967296.0) return false;
+    return sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4) <= 96 * 1024;
+}
+
+template <typename T, int EY, int NTH, bool B16, int W>
+static bool launch_rows_w(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 &K)
+{
+    const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
     const int threads = 64 * W * EY;
-    static int slots_cache[2][17] = {{0}};   // [mode][W] resident blocks per device
-    int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0][std::min(W, 16)];
+    static int slots_cache[2] = {0, 0};   // [mode] resident blocks per device
+    int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0];
     if (slots == 0) {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return false;
         hipError_t e = (mode == SWEEP_ZERO)
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH>, threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH>, threads, lds);
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W>,
+                                                                          threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W>,
+                                                                          threads, lds);
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
     }
@@ -2737,49 +2845,106 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH>), g, b, lds, s, L, pin, pout, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W>), g, b, lds, s, L, pin, pout, G, K);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH>), g, b, lds, s, L, pin, pout, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W>), g, b, lds, s, L, pin, pout, G, K);
     return true;
-    }
 }
 
-template <typename T>
-void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
-                       const EllBC &bc, hipStream_t s)
+// Full-row sweep: usable when nx = 128 W (W = 1, 2) and a block of EY rows
+// (64 W EY threads) fits in NTH.  FP32 runs 8 rows in blocks of up to 1024
+// threads (at most 128 VGPRs per lane); FP64, whose register rings are twice
+// as wide, runs 6 rows in blocks of up to 768.  The z-chunk count is chosen so
+// the number of blocks fills whole rounds of the resident block slots (CUs x
+// blocks per CU) with the least z-ring overhead.
+template <typename T, int EY, int NTH, bool B16>
+static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 &K)
+{
+    if (!rows_fit<T, EY, NTH>(L)) return false;
+    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1>(L, pin, pout, mode, s, K);
+    return launch_rows_w<T, EY, NTH, B16, 2>(L, pin, pout, mode, s, K);
+}
+
+static int sweep_variant()
 {
     // 0: one column per thread; 8/16: x-pairs with 8/16 rows; 100*PD + EY:
     // one column per thread with PD planes of prefetch and EY rows;
-    // 1000 + 10*EY + PD: full rows (falls back to 208 where it does not fit)
+    // 1082: full rows (falls back to 208 where they do not fit)
     static int variant = -1;
     if (variant < 0) {
         const char *e = getenv("MAC_SWEEP");
         variant = e ? atoi(e) : 1082;
     }
+    return variant;
+}
+
+static bool rr_rows_on()
+{
+    static int rows = -1;   // MAC_RR_ROWS=0 keeps the block-per-coarse-row kernel
+    if (rows < 0) {
+        const char *v = getenv("MAC_RR_ROWS");
+        rows = v ? atoi(v) : 1;
+    }
+    return rows != 0;
+}
+
+static bool res_rows_on(int &tz_env)
+{
+    static int on = -1, tz = 0;
+    if (on < 0) {
+        const char *v = getenv("MAC_RES_ROWS");
+        on = v ? atoi(v) : 1;
+        const char *t = getenv("MAC_RES_TZ");
+        tz = t ? atoi(t) : 0;
+    }
+    tz_env = tz;
+    return on != 0;
+}
+
+bool compact_level_ok(int dim, const Lev<float> &L)
+{
+    int tz_env;
+    return dim == 3 && sweep_variant() == 1082 && rows_fit<float, 8, 1024>(L) && rr_rows_on() &&
+           res_rows_on(tz_env);
+}
+
+template <typename T>
+bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
+                       const EllBC &bc, hipStream_t s, const Compact16 *K)
+{
+    const int variant = sweep_variant();
+    if constexpr (sizeof(T) == 4) {
+        if (K) {
+            if (mode == SWEEP_PROLONG || variant != 1082) return false;
+            return launch_rows<T, 8, 1024, true>(L, pin, pout, mode, s, *K);
+        }
+    } else {
+        if (K) return false;
+    }
+    const Compact16 K0 = {};
     if (mode != SWEEP_PROLONG && variant >= 1000) {
         bool done = false;
         switch (variant) {
         case 1082:
-            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768>(L, pin, pout, mode, s);
-            else done = launch_rows<T, 8, 1024>(L, pin, pout, mode, s);
+            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, K0);
+            else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, K0);
             break;
-        case 1042: done = launch_rows<T, 4, 512>(L, pin, pout, mode, s); break;
         default: break;
         }
-        if (done) return;
+        if (done) return true;
         launch_deep<T, 8, 2>(L, pin, pout, mode, s);
-        return;
+        return true;
     }
     if (mode != SWEEP_PROLONG) {
         switch (variant) {
-        case 8: launch_pair<T, 8>(L, pin, pout, mode, s); return;
-        case 16: launch_pair<T, 16>(L, pin, pout, mode, s); return;
-        case 208: launch_deep<T, 8, 2>(L, pin, pout, mode, s); return;
-        case 308: launch_deep<T, 8, 3>(L, pin, pout, mode, s); return;
-        case 408: launch_deep<T, 8, 4>(L, pin, pout, mode, s); return;
-        case 304: launch_deep<T, 4, 3>(L, pin, pout, mode, s); return;
-        case 404: launch_deep<T, 4, 4>(L, pin, pout, mode, s); return;
-        case 604: launch_deep<T, 4, 6>(L, pin, pout, mode, s); return;
+        case 8: launch_pair<T, 8>(L, pin, pout, mode, s); return true;
+        case 16: launch_pair<T, 16>(L, pin, pout, mode, s); return true;
+        case 208: launch_deep<T, 8, 2>(L, pin, pout, mode, s); return true;
+        case 308: launch_deep<T, 8, 3>(L, pin, pout, mode, s); return true;
+        case 408: launch_deep<T, 8, 4>(L, pin, pout, mode, s); return true;
+        case 304: launch_deep<T, 4, 3>(L, pin, pout, mode, s); return true;
+        case 404: launch_deep<T, 4, 4>(L, pin, pout, mode, s); return true;
+        case 604: launch_deep<T, 4, 6>(L, pin, pout, mode, s); return true;
         default: break;
         }
     }
@@ -2806,17 +2971,14 @@ void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
         hipLaunchKernelGGL((k_gsrb_fused<T, 2>), g, b, 0, s, L, pin, pout, G, PS);
     else
         hipLaunchKernelGGL((k_gsrb_fused<T, 0>), g, b, 0, s, L, pin, pout, G, PS);
+    return true;
 }
 
 template <typename T>
-void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s)
+bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
+                           const Compact16 *K)
 {
-    static int rows = -1;   // MAC_RR_ROWS=0 keeps the block-per-coarse-row kernel
-    if (rows < 0) {
-        const char *v = getenv("MAC_RR_ROWS");
-        rows = v ? atoi(v) : 1;
-    }
-    if (rows && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
+    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
         RowMarchGrid G;
         G.W = F.nx / 128;
         // coarse planes per chunk: at most 16, fewer on small levels so that
@@ -2828,24 +2990,39 @@ void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
         G.total = C.ny * G.W * chunks;
         const int blocks = (G.total + 3) / 4;
         G.per_xcd = (blocks + 7) / 8;
-        hipLaunchKernelGGL((k_rr_rows<T>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G);
-        return;
+        if (K) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL((k_rr_rows<T, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, *K);
+            else
+                return false;
+        } else {
+            const Compact16 K0 = {};
+            hipLaunchKernelGGL((k_rr_rows<T, false>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, K0);
+        }
+        return true;
     }
+    if (K) return false;
     dim3 g((C.nx + 63) / 64, C.ny, C.nz);
     if (dim == 3)
         hipLaunchKernelGGL((k_resid_restrict<3, T>), g, dim3(256), 0, s, F, phi, C);
     else
         hipLaunchKernelGGL((k_resid_restrict<2, T>), g, dim3(128), 0, s, F, phi, C);
+    return true;
 }
 
-template void launch_gsrb_fused<double>(const Lev<double> &, const double *, double *, int,
-                                        const Lev<double> *, const EllBC &, hipStream_t);
-template void launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int,
-                                       const Lev<float> *, const EllBC &, hipStream_t);
-template void launch_resid_restrict<double>(int, const Lev<double> &, const double *,
-                                            const Lev<double> &, hipStream_t);
-template void launch_resid_restrict<float>(int, const Lev<float> &, const float *,
-                                           const Lev<float> &, hipStream_t);
+template bool launch_gsrb_fused<double>(const Lev<double> &, const double *, double *, int,
+                                        const Lev<double> *, const EllBC &, hipStream_t, const Compact16 *);
+template bool launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int,
+                                       const Lev<float> *, const EllBC &, hipStream_t, const Compact16 *);
+template bool launch_resid_restrict<double>(int, const Lev<double> &, const double *,
+                                            const Lev<double> &, hipStream_t, const Compact16 *);
+template bool launch_resid_restrict<float>(int, const Lev<float> &, const float *,
+                                           const Lev<float> &, hipStream_t, const Compact16 *);
+
+void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n)), dim3(TPB), 0, s, in, out, n);
+}
 
 static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
 
@@ -2854,16 +3031,10 @@ static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
 // kernels.  The z-chunk length is 32 planes (MAC_RES_TZ overrides).
 template <int MODE>
 static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_out, float *r32,
-                              unsigned long long *nrm, hipStream_t s)
+                              unsigned long long *nrm, hipStream_t s, unsigned short *r16 = nullptr)
 {
-    static int on = -1, tz_env = 0;
-    if (on < 0) {
-        const char *v = getenv("MAC_RES_ROWS");
-        on = v ? atoi(v) : 1;
-        const char *t = getenv("MAC_RES_TZ");
-        tz_env = t ? atoi(t) : 0;
-    }
-    if (!on || L.nx % 128 != 0 || L.nz < 2) return false;
+    int tz_env;
+    if (!res_rows_on(tz_env) || L.nx % 128 != 0 || L.nz < 2) return false;
     RowMarchGrid G;
     G.W = L.nx / 128;
     // planes per chunk: 32, fewer on small levels (some 4096 waves in all)
@@ -2874,7 +3045,12 @@ static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_
     G.total = L.ny * G.W * chunks;
     const int blocks = (G.total + 3) / 4;
     G.per_xcd = (blocks + 7) / 8;
-    hipLaunchKernelGGL(k_resid_rows<MODE>, dim3(8 * G.per_xcd), dim3(256), 0, s, L, e, phi_out, r32, nrm, G);
+    if (r16)
+        hipLaunchKernelGGL((k_resid_rows<MODE, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, L, e, phi_out,
+                           r32, r16, nrm, G);
+    else
+        hipLaunchKernelGGL((k_resid_rows<MODE, false>), dim3(8 * G.per_xcd), dim3(256), 0, s, L, e, phi_out,
+                           r32, r16, nrm, G);
     return true;
 }
 
@@ -2892,17 +3068,17 @@ void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, 
 }
 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
-                        hipStream_t s)
+                        hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s)) return;
+    if (dim == 3 && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
 
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
-                               unsigned long long *nrm, hipStream_t s)
+                               unsigned long long *nrm, hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s)) return;
+    if (dim == 3 && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_update_residual<D_>, dim3(g), dim3(TPB), 0, s, L, e, phi_out,
                                          r32, nrm));

@@ -86,21 +86,31 @@ template <typename T> void launch_coarsen(int dim, const Lev<T> &F, const Lev<T>
 void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, hipStream_t s);
 template <typename T> void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s);
 // one full red-black sweep of a 3D level, pin -> pout; mode SWEEP_*; for
-// SWEEP_PROLONG the coarse level C supplies the correction
+// SWEEP_PROLONG the coarse level C supplies the correction.  K (FP32 only,
+// SWEEP_PLAIN/ZERO) takes the weights and right side from compact storage;
+// the call then returns false if the level does not fit the full-row kernel
+// (compact_level_ok) and launches nothing.
 template <typename T>
-void launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
-                       const EllBC &bc, hipStream_t s);
-// fine residual (of phi) restricted to C.rhs; C.phi cleared
+bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
+                       const EllBC &bc, hipStream_t s, const Compact16 *K = nullptr);
+// fine residual (of phi) restricted to C.rhs; C.phi cleared.  K as above.
 template <typename T>
-void launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s);
+bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
+                           const Compact16 *K = nullptr);
+// whether every level-0 kernel of the mixed path has a compact-storage form
+// for this level (3D, nx a multiple of 128 and the full-row sweep fits)
+bool compact_level_ok(int dim, const Lev<float> &L);
+// FP32 -> bfloat16 copy of n values
+void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s);
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
+// r16 (compact level 0) replaces r32 as the output of the FP32 right side
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
-                               unsigned long long *nrm, hipStream_t s);
+                               unsigned long long *nrm, hipStream_t s, unsigned short *r16 = nullptr);
 template <typename A, typename B> void launch_convert(const A *in, B *out, long n, hipStream_t s);
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
-                        hipStream_t s);
+                        hipStream_t s, unsigned short *r16 = nullptr);
 template <typename T>
 void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hipStream_t s);
 template <typename T>

@@ -67,6 +67,10 @@ struct mac_plan {
     int mapg[3] = {1, 1, 1}, mapn[3] = {1, 1, 1};
     Hier<double> H64;
     Hier<float> H32;
+    // level 0 of H32 in compact storage (MAC_PREC_MIXED_BF16 where it applies)
+    Compact16 c16 = {};
+    std::vector<void *> c16_allocs;
+    bool launch_failed = false;   // a compact-storage launch found no kernel
     double *part = nullptr;
     long part_cap = 0;
     double *mean_dev = nullptr;
@@ -283,14 +287,21 @@ static bool use_fused(const mac_plan *P)
 
 // nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
 // (cur tracks which holds it).  first_mode applies to the first sweep.
+static bool fuse_prolong_on()
+{
+    static int fuse_prolong = -1;
+    if (fuse_prolong < 0) fuse_prolong = getenv("MAC_FUSED_PROLONG") ? 1 : 0;
+    return fuse_prolong != 0;
+}
+
+// K: level l's weights and right side in compact storage (level 0 only)
 template <typename T>
-static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur)
+static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur,
+                         const Compact16 *K = nullptr)
 {
     const bool f64 = sizeof(T) == 8;
     Lev<T> &L = H.lev[l];
-    static int fuse_prolong = -1;
-    if (fuse_prolong < 0) fuse_prolong = getenv("MAC_FUSED_PROLONG") ? 1 : 0;
-    if (first_mode == SWEEP_PROLONG && !fuse_prolong) {
+    if (first_mode == SWEEP_PROLONG && !fuse_prolong_on()) {
         // the interpolation inside the sweep stalls its plane pipeline; a
         // separate streaming pass (phi += P e, 17 B/cell) is cheaper
         Lev<T> Lc = L;
@@ -307,16 +318,18 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         const int cls = mode == SWEEP_ZERO ? (f64 ? MAC_PROF_GSRB64_ZERO : MAC_PROF_GSRB32_ZERO)
                                            : (f64 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32);
         int q = (l == 0) ? prof_begin(P, cls) : -1;
-        launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
-                             P->bc, P->stream);
+        if (!launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
+                                  P->bc, P->stream, K))
+            P->launch_failed = true;
         prof_end(P, q);
         cur ^= 1;
     }
 }
 
-// zero_first: the level-0 iterate is known to be zero (refinement step)
+// zero_first: the level-0 iterate is known to be zero (refinement step);
+// K: level 0's weights and right side in compact storage (fused path)
 template <typename T>
-static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
+static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compact16 *K = nullptr)
 {
     hipStream_t s = P->stream;
     const int dim = P->dim;
@@ -325,11 +338,13 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
         std::vector<int> cur(H.nlev, 0);
         for (int l = 0; l < H.lc; ++l) {
             const bool zero = (l > 0) || zero_first;
-            if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur[l]);
+            const Compact16 *Kl = l == 0 ? K : nullptr;
+            if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur[l], Kl);
             else if (zero) launch_fill<T>(H.lev[l].phi, (long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz, T(0), s);
             const Lev<T> &L = H.lev[l];
             int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_RESTRICT64 : MAC_PROF_RESTRICT32) : -1;
-            launch_resid_restrict<T>(dim, L, cur[l] ? L.phi2 : L.phi, H.lev[l + 1], s);
+            if (!launch_resid_restrict<T>(dim, L, cur[l] ? L.phi2 : L.phi, H.lev[l + 1], s, Kl))
+                P->launch_failed = true;
             prof_end(P, q);
         }
         {
@@ -348,7 +363,7 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
         for (int l = H.lc - 1; l >= 0; --l) {
             Lev<T> &L = H.lev[l];
             if (o.nu2 > 0) {
-                fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur[l]);
+                fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur[l], l == 0 ? K : nullptr);
             } else {
                 Lev<T> Lc = L;
                 Lc.phi = cur[l] ? L.phi2 : L.phi;
@@ -391,6 +406,45 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false)
     }
 }
 
+// Level 0 of the FP32 hierarchy in compact storage (MAC_PREC_MIXED_BF16):
+// *K is set where every level-0 kernel of the mixed path has its compact
+// form (compact_level_ok), else left null and the FP32 arrays are used.  The
+// weights are refreshed from H32 level 0; the right side is written by the
+// refinement residual.
+static int compact_level0(mac_plan *P, const Compact16 **K)
+{
+    *K = nullptr;
+    if (P->opts.precision != MAC_PREC_MIXED_BF16 || !P->H32.built) return MAC_OK;
+    const Lev<float> &F = P->H32.lev[0];
+    if (!use_fused(P) || fuse_prolong_on() || !compact_level_ok(P->dim, F)) return MAC_OK;
+    Compact16 &C = P->c16;
+    if (P->c16_allocs.empty()) {
+        const long nc = (long)F.nx * F.ny * F.nz;
+        unsigned short **arr[7] = {&C.bx, &C.by, &C.bz, &C.bxh, &C.byh, &C.bzh, &C.rhs};
+        const long len[7] = {nc, nc, nc, (long)F.ny * F.nz, (long)F.nx * F.nz, (long)F.nx * F.ny, nc};
+        for (int a = 0; a < 7; ++a) {
+            void *q = nullptr;
+            if (hipMalloc(&q, sizeof(unsigned short) * (size_t)len[a]) != hipSuccess) {
+                for (void *p : P->c16_allocs) (void)hipFree(p);
+                P->c16_allocs.clear();
+                C = Compact16{};
+                return MAC_ERR_NOMEM;
+            }
+            P->c16_allocs.push_back(q);
+            *arr[a] = (unsigned short *)q;
+        }
+    }
+    const long nc = (long)F.nx * F.ny * F.nz;
+    launch_to_bf16(F.bx, C.bx, nc, P->stream);
+    launch_to_bf16(F.by, C.by, nc, P->stream);
+    launch_to_bf16(F.bz, C.bz, nc, P->stream);
+    launch_to_bf16(F.bxh, C.bxh, (long)F.ny * F.nz, P->stream);
+    launch_to_bf16(F.byh, C.byh, (long)F.nx * F.nz, P->stream);
+    launch_to_bf16(F.bzh, C.bzh, (long)F.nx * F.ny, P->stream);
+    *K = &C;
+    return MAC_OK;
+}
+
 static int read_norm(mac_plan *P, double *out)
 {
     HIPCHK(hipMemcpyAsync(P->nrm_host, P->nrm_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -413,11 +467,11 @@ static int fp64_norm(mac_plan *P, double *out)
     return read_norm(P, out);
 }
 
-static int ir_norm(mac_plan *P, double *out)
+static int ir_norm(mac_plan *P, double *out, const Compact16 *K)
 {
     HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
     int q = prof_begin(P, MAC_PROF_RESID64);
-    launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, P->nrm_dev, P->stream);
+    launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, P->nrm_dev, P->stream, K ? K->rhs : nullptr);
     prof_end(P, q);
     return read_norm(P, out);
 }
@@ -435,18 +489,22 @@ static int solve(mac_plan *P, mac_stats *st)
 {
     int rc;
     const mac_opts &o = P->opts;
+    if (o.precision < MAC_PREC_FP64 || o.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
     double r0 = 0, r = 0;
     int it = 0, inner = 0;
-    if (o.precision == MAC_PREC_MIXED) {
+    if (o.precision == MAC_PREC_MIXED || o.precision == MAC_PREC_MIXED_BF16) {
         if ((rc = build_hier<float>(P, P->H32))) return rc;
         launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
         coarsen_all<float>(P, P->H32);
-        if ((rc = ir_norm(P, &r0))) return rc;
+        const Compact16 *K = nullptr;
+        if ((rc = compact_level0(P, &K))) return rc;
+        P->launch_failed = false;
+        if ((rc = ir_norm(P, &r0, K))) return rc;
         r = r0;
         const Lev<float> &E = P->H32.lev[0];
         while (!converged(o, r, r0) && it < o.max_iter) {
             for (int c = 0; c < std::max(1, o.inner_cycles); ++c) {
-                vcycle<float>(P, P->H32, c == 0);
+                vcycle<float>(P, P->H32, c == 0, K);
                 ++inner;
             }
             // phi64 + e32 lands in phi2 together with the next residual;
@@ -455,12 +513,14 @@ static int solve(mac_plan *P, mac_stats *st)
             Lev<double> &L0 = P->H64.lev[0];
             HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
             int q = prof_begin(P, MAC_PROF_IR_UPDATE);
-            launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, P->nrm_dev, P->stream);
+            launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, P->nrm_dev, P->stream,
+                                      K ? K->rhs : nullptr);
             prof_end(P, q);
             std::swap(L0.phi, L0.phi2);
             HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &L0, sizeof(Lev<double>), hipMemcpyHostToDevice,
                                   P->stream));
             if ((rc = read_norm(P, &r))) return rc;
+            if (P->launch_failed) return MAC_ERR_INVALID;
             ++it;
             if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
         }
@@ -681,6 +741,7 @@ extern "C" int mac_plan_destroy(mac_plan *P)
         if (e) (void)hipEventDestroy(e);
     free_hier(P->H64);
     free_hier(P->H32);
+    for (void *p : P->c16_allocs) (void)hipFree(p);
     free_stage(P);
     if (P->boxes_dev) (void)hipFree(P->boxes_dev);
     if (P->boxmap_dev) (void)hipFree(P->boxmap_dev);
@@ -825,7 +886,7 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     if (beta0_1d && beta0_cart_edge) return MAC_ERR_INVALID;      // macproject.f90:99-101
     if (beta0_1d && !beta0_1d_edge) return MAC_ERR_INVALID;
     if (!beta0_1d && beta0_1d_edge) return MAC_ERR_INVALID;
-    if (P->opts.precision != MAC_PREC_FP64 && P->opts.precision != MAC_PREC_MIXED) return MAC_ERR_INVALID;
+    if (P->opts.precision < MAC_PREC_FP64 || P->opts.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
     if (st) memset(st, 0, sizeof(*st));
     hipStream_t s = P->stream;
     const int nb = P->nboxes, dim = P->dim;
@@ -1148,11 +1209,15 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         launch_weights_to_f32(P->dim, L, F, s);
         launch_convert<double, float>(L.phi, F.phi, n, s);
         launch_convert<double, float>(L.rhs, F.rhs, n, s);
+        const Compact16 *K = nullptr;
+        if (level == 0 && (rc = compact_level0(P, &K))) return rc;
+        if (K) launch_to_bf16(F.rhs, K->rhs, n, s);
         int cur = 0;
         for (int c = 0; c < count; ++c) {
             if (use_fused(P)) {
-                launch_gsrb_fused<float>(F, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, nullptr,
-                                         P->bc, s);
+                if (!launch_gsrb_fused<float>(F, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, nullptr,
+                                              P->bc, s, K))
+                    return MAC_ERR_INVALID;
                 cur ^= 1;
             } else {
                 launch_gsrb<float>(P->dim, F, 0, s);

@@ -88,7 +88,8 @@ module macproject_module
 
   ! runtime knobs (Source/_parameters:552-554, Source/mg_eps.f90:22-28)
   logical, save :: use_alt_energy_fix = .true.
-  integer, save :: mac_precision = 0            ! 0 FP64 V-cycle, 1 FP32 V-cycle + FP64 refinement
+  integer, save :: mac_precision = 0            ! 0 FP64 V-cycle, 1 FP32 V-cycle + FP64 refinement,
+                                                ! 2 as 1 with bfloat16 finest-level weights and rhs
   integer, save :: mg_verbose = 0
   real(dp_t), save :: eps_mac = 1.d-10
   real(dp_t), save :: eps_mac_max = 1.d-8

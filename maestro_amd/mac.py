@@ -27,7 +27,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmaestro_mac.so")
 
 PERIODIC, INTERIOR, INLET, OUTLET, SYMMETRY, SLIP_WALL, NO_SLIP_WALL = -1, 0, 11, 12, 13, 14, 15
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
 ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
           3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory"}
 SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",

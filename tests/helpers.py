@@ -102,3 +102,11 @@ def fold(bc, n, b_raw, dim):
         out[names[d]] = lo.ravel()
         out[names[d] + "h"] = np.squeeze(hi, axis=ax).ravel()
     return out
+
+
+def bf16_round(x):
+    """Round to bfloat16 (nearest, ties to even) through FP32, returned as the
+    FP32 values the GPU kernels widen the stored bfloat16 patterns to."""
+    u = np.ascontiguousarray(np.asarray(x, dtype=np.float32)).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32)

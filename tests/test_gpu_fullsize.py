@@ -106,19 +106,22 @@ def test_plane_parallel_256_fp64_and_mixed(gpu):
     torch = gpu
     wl = W.make_workload(2, n=(256,) * 3, box=64)
     res = {}
-    for prec in (mac.PREC_FP64, mac.PREC_MIXED):
+    for prec in (mac.PREC_FP64, mac.PREC_MIXED, mac.PREC_MIXED_BF16):
         faces, ph, st = project_device(torch, wl, prec)
+        print(f"  prec {prec}: iterations {st['iters']}")
         check_projection(torch, wl, faces, st)
         # periodic duplicates equal, wall-normal faces zero
         assert torch.equal(faces[0][:, :, 0], faces[0][:, :, -1])
         assert torch.equal(faces[1][:, 0, :], faces[1][:, -1, :])
         assert float(faces[2][0].abs().max()) == 0.0 and float(faces[2][-1].abs().max()) == 0.0
         res[prec] = (faces, ph - ph.mean())
-    (f64, p64), (fmx, pmx) = res[mac.PREC_FP64], res[mac.PREC_MIXED]
-    e_phi = float((pmx - p64).norm() / p64.norm())
-    e_u = max(float((fmx[d] - f64[d]).norm() / f64[d].norm()) for d in range(3))
-    print(f"  mixed vs FP64: phi {e_phi:.2e}  U {e_u:.2e}")
-    assert e_phi <= 1e-8 and e_u <= 1e-8
+    f64, p64 = res[mac.PREC_FP64]
+    for prec in (mac.PREC_MIXED, mac.PREC_MIXED_BF16):
+        fmx, pmx = res[prec]
+        e_phi = float((pmx - p64).norm() / p64.norm())
+        e_u = max(float((fmx[d] - f64[d]).norm() / f64[d].norm()) for d in range(3))
+        print(f"  prec {prec} vs FP64: phi {e_phi:.2e}  U {e_u:.2e}")
+        assert e_phi <= 1e-8 and e_u <= 1e-8
 
 
 def test_star_512_mixed_and_fp64(gpu):

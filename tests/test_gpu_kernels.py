@@ -8,7 +8,7 @@ import pytest
 from maestro_amd import mac
 from maestro_amd import workloads as W
 
-from .helpers import ell, fold
+from .helpers import bf16_round, ell, fold
 
 pytestmark = pytest.mark.gpu
 
@@ -76,15 +76,19 @@ def test_gsrb_sweep_bitwise_2d(gpu, oracle_lib, bc):
     assert np.array_equal(got, ref)
 
 
-def fp32_reference(O, n, bc, b, phi, rhs, sweeps):
+def fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=False):
     """The oracle's FP32 sweep on the level rounded to FP32 as the library
-    rounds it (weights, rhs, phi and 1/h^2 each cast from FP64)."""
+    rounds it (weights, rhs, phi and 1/h^2 each cast from FP64); bf16: the
+    weights and rhs further rounded to bfloat16 (compact storage)."""
     dim = len(n)
     w = {k: np.asarray(v, dtype=np.float32) for k, v in fold(bc, n, b, dim).items()}
+    r32 = rhs.astype(np.float32)
+    if bf16:
+        w = {k: bf16_round(v) for k, v in w.items()}
+        r32 = bf16_round(r32).reshape(rhs.shape)
     ih2 = [np.float32(1.0 / (1.0 / n[0]) ** 2)] * dim
     per = [1 if bc[d][0] == P else 0 for d in range(dim)]
-    out = O.gsrb_folded_f32(dim, n, per, ih2, w, phi.astype(np.float32), rhs.astype(np.float32),
-                            sweeps=sweeps)
+    out = O.gsrb_folded_f32(dim, n, per, ih2, w, phi.astype(np.float32), r32, sweeps=sweeps)
     return out.astype(np.float64)
 
 
@@ -105,6 +109,31 @@ def test_gsrb32_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
     got = plan.debug_get(0, "phi").reshape(phi.shape)
     ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps)
     assert np.array_equal(got, ref), np.abs(got - ref).max()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (128, 30, 6), (128, 12, 9), (64, 48, 40)])
+@pytest.mark.parametrize("bc", BCS_3D)
+def test_gsrb32_bf16_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
+    """The compact-storage smoother (bfloat16 weights and rhs, FP32 iterate
+    and arithmetic) against the oracle's FP32 sweep on the same rounded
+    values, bit for bit.  Levels without the full-row kernel (nx other than
+    128 or 256, or fewer than 8 planes) have no compact form: the plan keeps
+    FP32 storage there, and the FP32 reference applies."""
+    O = oracle_lib
+    rng = np.random.default_rng(29)
+    g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
+    b, phi, rhs = rand_level(g, n, 3, rng, O)
+    plan = make_plan(n, bc, precision=mac.PREC_MIXED_BF16)
+    push_level(plan, 0, bc, n, b, phi, rhs, 3)
+    plan.debug_op(0, "gsrb32", sweeps)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    compact = n[0] in (128, 256) and n[2] >= 8
+    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=compact)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    if compact:
+        # the rounding is visible: the FP32-storage sweep differs
+        assert not np.array_equal(got, fp32_reference(O, n, bc, b, phi, rhs, sweeps))
 
 
 @pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL]], [[OUT, WALL], [P, P]]])

@@ -29,6 +29,14 @@ CASES = [
     (2, (128, 64, 64), 32, 1, 1),
     (2, (256, 64, 64), 64, 0, 1),
     (3, (128, 64, 64), 32, 0, 1),
+    # bfloat16 storage of the finest level's weights and rhs (compact form
+    # for nx = 128, 256; FP32 storage for nx = 64 and in 2D)
+    (1, (128, 64, 64), 32, 1, 2),
+    (2, (256, 64, 64), 64, 1, 2),
+    (3, (128, 64, 64), 32, 0, 2),
+    (3, (128, 128, 128), 64, 1, 2),
+    (2, (64, 64, 32), 32, 1, 2),
+    (0, (64, 64), 16, 1, 2),
 ]
 
 
@@ -49,8 +57,18 @@ def test_macproject_matches_oracle(gpu, cfg, n, box, alt, prec):
     e_u = max(rel_err(out["faces"][d], truth["faces"][d]) for d in range(dim))
     # error of the reference algorithm's own 1e-10 solve against the truth
     e_u_ref = max(rel_err(same["faces"][d], truth["faces"][d]) for d in range(dim))
-    print(f"  vs 1e-13 truth: phi {e_phi:.2e}  U {e_u:.2e} (oracle@1e-10: U {e_u_ref:.2e})")
-    assert e_phi <= 1e-8
+    e_phi_ref = rel_err(same["phi"], truth["phi"], remove_mean=singular)
+    print(f"  vs 1e-13 truth: phi {e_phi:.2e}  U {e_u:.2e} "
+          f"(oracle@1e-10: phi {e_phi_ref:.2e} U {e_u_ref:.2e})")
+    if prec == 2:
+        # bfloat16 weights perturb the inner operator by up to 2^-9, so the
+        # refinement stops at a different 1e-10 iterate than the reference;
+        # its error is held to that of the reference's own 1e-10 solve, and
+        # the extra refinement steps are bounded
+        assert e_phi <= max(1e-8, 2.0 * e_phi_ref)
+        assert st["iters"] <= 1.5 * same["stats"].iters + 1
+    else:
+        assert e_phi <= 1e-8
     assert e_u <= max(1e-8, 2.0 * e_u_ref)
     if prec == 0:
         # FP64 path: the same V-cycle as the oracle, so the 1e-10 iterates agree

@@ -329,3 +329,22 @@ def test_workload_compatibility(cfg, n):
             assert np.array_equal(lo, hi)
         elif wl["bc"][d][0] == WALL:
             assert not lo.any() and not hi.any()
+
+
+def test_bf16_rounding_matches_torch():
+    """The bfloat16 rounding that the compact-storage tests apply to the
+    oracle's inputs (round to nearest, ties to even): against torch's own
+    float32 -> bfloat16 conversion, on random magnitudes and on exact ties,
+    plus the textbook tie cases."""
+    import torch
+    from .helpers import bf16_round
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal(200000) * 10.0 ** rng.integers(-30, 30, 200000)).astype(np.float32)
+    ties = ((rng.integers(1, 0x7F00, 20000).astype(np.uint32) << 16) | 0x8000).view(np.float32)
+    x = np.concatenate([x, ties, -ties])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(bf16_round(x), ref)
+    one = np.float32(1.0)
+    assert bf16_round(one + np.float32(2.0 ** -8)) == one                              # tie, even below
+    assert bf16_round(one + np.float32(3 * 2.0 ** -8)) == one + np.float32(2.0 ** -6)  # tie, even above
+    assert bf16_round(one + np.float32(2.0 ** -8 + 2.0 ** -20)) == one + np.float32(2.0 ** -7)
