@@ -752,9 +752,16 @@ __device__ __forceinline__ void st_rs(__amdgpu_buffer_rsrc_t r, unsigned voff, u
 // 8-step unroll), one barrier per plane.
 // B16: the face weights and right side come from compact (bfloat16) storage
 // K instead of L; the arithmetic is the same FP32 code on the widened values.
+// Byte offsets of the arrays a row sweep touches: the iterates from bp, the
+// weights and rhs from bw (one allocation each, so one buffer resource each)
+struct RowBufs {
+    const char *bp, *bw;
+    unsigned in, out;
+    unsigned bx, by, bz, rhs, bxh, byh, bzh;
+};
+
 template <typename T, int MODE, int EY, int NTH, bool B16, int W>
-__global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict__ pin,
-                                                    T *__restrict__ pout, RowSweepGrid G, Compact16 K)
+__global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweepGrid G)
 {
     static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
     using T2 = typename Vec2<T>::type;
@@ -788,17 +795,8 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
     const unsigned vyn = (ylast ? (unsigned)x0 : col + NX) * WS;    // y-high faces: by or the byh slab
     const unsigned vx2 = (xend ? col : col + 2u) * WS;             // third x face (bxh replaces it on xend)
     const unsigned bynstride = ylast ? (unsigned)NX : (unsigned)plane;
-    const void *p_bx, *p_by, *p_bz, *p_rh, *p_byn, *p_bxh, *p_bzh;
-    if constexpr (B16) {
-        p_bx = K.bx; p_by = K.by; p_bz = K.bz; p_rh = K.rhs; p_byn = ylast ? K.byh : K.by;
-        p_bxh = K.bxh; p_bzh = K.bzh;
-    } else {
-        p_bx = L.bx; p_by = L.by; p_bz = L.bz; p_rh = L.rhs; p_byn = ylast ? L.byh : L.by;
-        p_bxh = L.bxh; p_bzh = L.bzh;
-    }
-    const __amdgpu_buffer_rsrc_t Rbx = buf_rsrc(p_bx), Rby = buf_rsrc(p_by), Rbz = buf_rsrc(p_bz),
-                                 Rrh = buf_rsrc(p_rh), Rbyn = buf_rsrc(p_byn), Rbxh = buf_rsrc(p_bxh),
-                                 Rbzh = buf_rsrc(p_bzh), Rin = buf_rsrc(pin), Rout = buf_rsrc(pout);
+    const unsigned obyn = ylast ? B.byh : B.by;
+    const __amdgpu_buffer_rsrc_t Rp = buf_rsrc(B.bp), Rw = buf_rsrc(B.bw);
     // a column pair / one value of a weight or rhs array
     auto ldw = [&](__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) -> T2 {
         if constexpr (B16) {
@@ -846,7 +844,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
         } else {
             int kw;
             const bool ok = zwrap(k, kw) && oky;
-            const T2 v = ld_rs<T2>(Rin, vp, (unsigned)kw * (unsigned)plane * ES);
+            const T2 v = ld_rs<T2>(Rp, vp, B.in + (unsigned)kw * (unsigned)plane * ES);
             return ok ? v : z2();
         }
     };
@@ -866,26 +864,26 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
         int kw;
         const bool kok = zwrap(k, kw);
         const unsigned sw = (unsigned)kw * (unsigned)plane * WS;
-        const T2 vx = ldw(Rbx, vw, sw);
+        const T2 vx = ldw(Rw, vw, B.bx + sw);
         o.v[0] = vx.x;
         o.v[1] = vx.y;
-        const T bx2 = ldw1(Rbx, vx2, sw);
-        const T bxh = ldw1(Rbxh, 0u, (unsigned)(Yw + ny * kw) * WS);
+        const T bx2 = ldw1(Rw, vx2, B.bx + sw);
+        const T bxh = ldw1(Rw, 0u, B.bxh + (unsigned)(Yw + ny * kw) * WS);
         o.v[2] = xend ? bxh : bx2;
-        const T2 vy = ldw(Rby, vw, sw);
+        const T2 vy = ldw(Rw, vw, B.by + sw);
         o.v[3] = vy.x;
         o.v[4] = vy.y;
-        const T2 vn = ldw(Rbyn, vyn, (unsigned)kw * bynstride * WS);
+        const T2 vn = ldw(Rw, vyn, obyn + (unsigned)kw * bynstride * WS);
         o.v[5] = vn.x;
         o.v[6] = vn.y;
-        const T2 vz = ldw(Rbz, vw, sw);
+        const T2 vz = ldw(Rw, vw, B.bz + sw);
         o.v[7] = vz.x;
         o.v[8] = vz.y;
-        const T2 vr = ldw(Rrh, vw, sw);
+        const T2 vr = ldw(Rw, vw, B.rhs + sw);
         o.v[9] = vr.x;
         o.v[10] = vr.y;
         T2 va = z2();
-        if (MODE == 0 && ring) va = ld_rs<T2>(Rin, vp2, (unsigned)kw * (unsigned)plane * ES);
+        if (MODE == 0 && ring) va = ld_rs<T2>(Rp, vp2, B.in + (unsigned)kw * (unsigned)plane * ES);
         const bool use2 = kok && ok2;
         o.v[11] = use2 ? va.x : T(0);
         o.v[12] = use2 ? va.y : T(0);
@@ -957,7 +955,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
         // z-low faces, or the bzh slab on the domain's top plane
         T zh0, zh1, zb0, zb1;
         if (kw == nz - 1) {
-            const T2 t = ldw(Rbzh, vw, 0u);
+            const T2 t = ldw(Rw, vw, B.bzh);
             zh0 = t.x;
             zh1 = t.y;
         } else {
@@ -965,7 +963,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
             zh1 = wq[C1].v[8];
         }
         if (km == nz - 1) {
-            const T2 t = ldw(Rbzh, vw, 0u);
+            const T2 t = ldw(Rw, vw, B.bzh);
             zb0 = t.x;
             zb1 = t.y;
         } else {
@@ -1015,7 +1013,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, const T *__restrict
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
-                st_rs<T2>(Rout, vp, (unsigned)km * (unsigned)plane * ES, o);
+                st_rs<T2>(Rp, vp, B.out + (unsigned)km * (unsigned)plane * ES, o);
             }
         };
         if (r) update(std::integral_constant<int, 1>{});
@@ -2794,13 +2792,55 @@ static bool rows_fit(const Lev<T> &L)
     if (L.nz < 8) return false;                      // the march wraps at most one period
     const int W = L.nx / 128;
     if (64 * W * EY > NTH) return false;
-    // plane offsets are 32-bit byte offsets
-    if ((double)L.nx * L.ny * L.nz * sizeof(T) >= 4294967296.0) return false;
+    // byte offsets within a level's allocation are 32-bit
+    if (10.0 * L.nx * L.ny * L.nz * sizeof(T) >= 4294967296.0) return false;
     return sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4) <= 96 * 1024;
 }
 
+// offsets of a row sweep's arrays from the lowest address of each group;
+// false if one does not fit 32 bits
+template <typename T>
+static bool row_bufs(const Lev<T> &L, const T *pin, T *pout, const Compact16 *K, RowBufs &B)
+{
+    const long nc = (long)L.nx * L.ny * L.nz;
+    auto group = [](const char *const *p, const long *bytes, int n, const char *&base, unsigned *off) {
+        base = p[0];
+        for (int a = 1; a < n; ++a) base = std::min(base, p[a]);
+        for (int a = 0; a < n; ++a) {
+            const long o = p[a] - base;
+            if (o < 0 || (double)o + (double)bytes[a] >= 4294967296.0) return false;
+            off[a] = (unsigned)o;
+        }
+        return true;
+    };
+    const char *pp[2] = {(const char *)pin, (const char *)pout};
+    const long pb[2] = {nc * (long)sizeof(T), nc * (long)sizeof(T)};
+    unsigned po[2];
+    if (!group(pp, pb, 2, B.bp, po)) return false;
+    B.in = po[0];
+    B.out = po[1];
+    const char *wp[7];
+    long es;
+    if (K) {
+        const unsigned short *k[7] = {K->bx, K->by, K->bz, K->rhs, K->bxh, K->byh, K->bzh};
+        for (int a = 0; a < 7; ++a) wp[a] = (const char *)k[a];
+        es = 2;
+    } else {
+        const T *w[7] = {L.bx, L.by, L.bz, L.rhs, L.bxh, L.byh, L.bzh};
+        for (int a = 0; a < 7; ++a) wp[a] = (const char *)w[a];
+        es = sizeof(T);
+    }
+    const long wb[7] = {nc * es, nc * es, nc * es, nc * es, (long)L.ny * L.nz * es, (long)L.nx * L.nz * es,
+                        (long)L.nx * L.ny * es};
+    unsigned wo[7];
+    if (!group(wp, wb, 7, B.bw, wo)) return false;
+    B.bx = wo[0]; B.by = wo[1]; B.bz = wo[2]; B.rhs = wo[3];
+    B.bxh = wo[4]; B.byh = wo[5]; B.bzh = wo[6];
+    return true;
+}
+
 template <typename T, int EY, int NTH, bool B16, int W>
-static bool launch_rows_w(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 &K)
+static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
     const int threads = 64 * W * EY;
@@ -2845,9 +2885,9 @@ static bool launch_rows_w(const Lev<T> &L, const T *pin, T *pout, int mode, hipS
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W>), g, b, lds, s, L, pin, pout, G, K);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W>), g, b, lds, s, L, RB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W>), g, b, lds, s, L, pin, pout, G, K);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W>), g, b, lds, s, L, RB, G);
     return true;
 }
 
@@ -2858,11 +2898,12 @@ static bool launch_rows_w(const Lev<T> &L, const T *pin, T *pout, int mode, hipS
 // the number of blocks fills whole rounds of the resident block slots (CUs x
 // blocks per CU) with the least z-ring overhead.
 template <typename T, int EY, int NTH, bool B16>
-static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 &K)
+static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 *K)
 {
-    if (!rows_fit<T, EY, NTH>(L)) return false;
-    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1>(L, pin, pout, mode, s, K);
-    return launch_rows_w<T, EY, NTH, B16, 2>(L, pin, pout, mode, s, K);
+    RowBufs RB;
+    if (!rows_fit<T, EY, NTH>(L) || !row_bufs<T>(L, pin, pout, B16 ? K : nullptr, RB)) return false;
+    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1>(L, RB, mode, s);
+    return launch_rows_w<T, EY, NTH, B16, 2>(L, RB, mode, s);
 }
 
 static int sweep_variant()
@@ -2916,18 +2957,17 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
     if constexpr (sizeof(T) == 4) {
         if (K) {
             if (mode == SWEEP_PROLONG || variant != 1082) return false;
-            return launch_rows<T, 8, 1024, true>(L, pin, pout, mode, s, *K);
+            return launch_rows<T, 8, 1024, true>(L, pin, pout, mode, s, K);
         }
     } else {
         if (K) return false;
     }
-    const Compact16 K0 = {};
     if (mode != SWEEP_PROLONG && variant >= 1000) {
         bool done = false;
         switch (variant) {
         case 1082:
-            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, K0);
-            else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, K0);
+            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
+            else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
             break;
         default: break;
         }

@@ -161,18 +161,14 @@ extern "C" int mac_default_opts(mac_opts *o)
 extern "C" const char *mac_version(void) { return "maestro_mac 0.1 (gfx950)"; }
 
 // ------------------------------------------------------------ hierarchy
-template <typename T>
-static int alloc_dev(Hier<T> &H, T **p, long n)
-{
-    void *q = nullptr;
-    HIPCHK(hipMalloc(&q, sizeof(T) * (size_t)std::max(n, 1L)));
-    H.allocs.push_back(q);
-    *p = (T *)q;
-    return MAC_OK;
-}
 
-static int coarse_threshold(int dim)
+// Levels of at most this many cells run inside the single-workgroup coarse
+// cycle (8^3 in 3D, 32^2 in 2D).  A 16^3 level fits its LDS in FP32 but is
+// slower there than as separate multi-block launches: one workgroup runs on
+// one CU.
+static int coarse_threshold(int dim, int elem_bytes)
 {
+    (void)elem_bytes;
     const char *e = getenv("MAC_COARSE_CELLS");
     if (e) return atoi(e);
     return dim == 3 ? 512 : 1024;
@@ -208,29 +204,29 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         L.ih2x = (T)(1.0 / (h[0] * h[0]));
         L.ih2y = (T)(1.0 / (h[1] * h[1]));
         L.ih2z = (P->dim == 3) ? (T)(1.0 / (h[2] * h[2])) : T(0);
-        long nc = ncell(n);
-        int rc;
-        if ((rc = alloc_dev(H, &L.phi, nc))) return rc;
-        L.phi2 = nullptr;
+        // the level's arrays share one allocation (256-byte aligned parts), so
+        // a kernel can reach all of them through one buffer resource
+        const long nc = ncell(n);
+        T **arr[10] = {&L.phi, &L.phi2, &L.rhs, &L.res, &L.bx, &L.by, &L.bz, &L.bxh, &L.byh, &L.bzh};
         // second iterate: ping-pong of the fused 3D sweep, and on level 0
         // the output of the fused refinement update
-        if (P->dim == 3 || l == 0)
-            if ((rc = alloc_dev(H, &L.phi2, nc))) return rc;
-        if ((rc = alloc_dev(H, &L.rhs, nc))) return rc;
-        if ((rc = alloc_dev(H, &L.res, nc))) return rc;
-        if ((rc = alloc_dev(H, &L.bx, nc))) return rc;
-        if ((rc = alloc_dev(H, &L.by, nc))) return rc;
-        if ((rc = alloc_dev(H, &L.bxh, (long)n[1] * n[2]))) return rc;
-        if ((rc = alloc_dev(H, &L.byh, (long)n[0] * n[2]))) return rc;
-        if (P->dim == 3) {
-            if ((rc = alloc_dev(H, &L.bz, nc))) return rc;
-            if ((rc = alloc_dev(H, &L.bzh, (long)n[0] * n[1]))) return rc;
-        } else {
-            L.bz = L.bzh = nullptr;
+        const bool has2 = P->dim == 3 || l == 0, d3 = P->dim == 3;
+        const long len[10] = {nc, has2 ? nc : 0, nc, nc, nc, nc, d3 ? nc : 0, (long)n[1] * n[2],
+                              (long)n[0] * n[2], d3 ? (long)n[0] * n[1] : 0};
+        auto part = [](long cnt) { return ((cnt * (long)sizeof(T) + 255) / 256) * 256; };
+        long total = 0;
+        for (int a = 0; a < 10; ++a) total += part(len[a]);
+        void *blk = nullptr;
+        HIPCHK(hipMalloc(&blk, (size_t)total));
+        H.allocs.push_back(blk);
+        long off = 0;
+        for (int a = 0; a < 10; ++a) {
+            *arr[a] = len[a] ? (T *)((char *)blk + off) : nullptr;
+            off += part(len[a]);
         }
         for (int d = 0; d < P->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
     }
-    int thr = coarse_threshold(P->dim);
+    int thr = coarse_threshold(P->dim, (int)sizeof(T));
     H.lc = nl - 1;
     for (int l = 0; l < nl; ++l)
         if ((long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz <= thr) { H.lc = l; break; }
@@ -418,20 +414,23 @@ static int compact_level0(mac_plan *P, const Compact16 **K)
     const Lev<float> &F = P->H32.lev[0];
     if (!use_fused(P) || fuse_prolong_on() || !compact_level_ok(P->dim, F)) return MAC_OK;
     Compact16 &C = P->c16;
-    if (P->c16_allocs.empty()) {
+    if (P->c16_allocs.empty()) {   // one allocation (one buffer resource for the kernels)
         const long nc = (long)F.nx * F.ny * F.nz;
         unsigned short **arr[7] = {&C.bx, &C.by, &C.bz, &C.bxh, &C.byh, &C.bzh, &C.rhs};
         const long len[7] = {nc, nc, nc, (long)F.ny * F.nz, (long)F.nx * F.nz, (long)F.nx * F.ny, nc};
+        auto part = [](long cnt) { return ((cnt * 2L + 255) / 256) * 256; };
+        long total = 0;
+        for (int a = 0; a < 7; ++a) total += part(len[a]);
+        void *q = nullptr;
+        if (hipMalloc(&q, (size_t)total) != hipSuccess) {
+            C = Compact16{};
+            return MAC_ERR_NOMEM;
+        }
+        P->c16_allocs.push_back(q);
+        long off = 0;
         for (int a = 0; a < 7; ++a) {
-            void *q = nullptr;
-            if (hipMalloc(&q, sizeof(unsigned short) * (size_t)len[a]) != hipSuccess) {
-                for (void *p : P->c16_allocs) (void)hipFree(p);
-                P->c16_allocs.clear();
-                C = Compact16{};
-                return MAC_ERR_NOMEM;
-            }
-            P->c16_allocs.push_back(q);
-            *arr[a] = (unsigned short *)q;
+            *arr[a] = (unsigned short *)((char *)q + off);
+            off += part(len[a]);
         }
     }
     const long nc = (long)F.nx * F.ny * F.nz;
@@ -525,6 +524,8 @@ static int solve(mac_plan *P, mac_stats *st)
             if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
         }
     } else {
+        // the FP64 hierarchy's coarse weights (the mixed paths coarsen the FP32 one)
+        coarsen_all<double>(P, P->H64);
         if ((rc = fp64_norm(P, &r0))) return rc;
         r = r0;
         while (!converged(o, r, r0) && it < o.max_iter) {
@@ -952,7 +953,6 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
         launch_mean(P->part, np, (double)n0, P->mean_dev, s);
         launch_sub_mean<double>(P->H64.lev[0].rhs, n0, P->mean_dev, s);
     }
-    coarsen_all<double>(P, P->H64);
     // max |rh| (the "Initial rhs" figure of the solver log)
     HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
     launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
@@ -1024,7 +1024,6 @@ static int level_weights_and_rhs(mac_plan *P, LevelIO &IO)
         launch_mean(P->part, np, (double)n0, P->mean_dev, P->stream);
         launch_sub_mean<double>(L0.rhs, n0, P->mean_dev, P->stream);
     }
-    coarsen_all<double>(P, P->H64);
     return MAC_OK;
 }
 
