@@ -2919,6 +2919,25 @@ static int sweep_variant()
     return variant;
 }
 
+// Rows per block of the FP32 full-row sweep.  which = 0: the compact
+// (bfloat16) form, 6 rows of 768 threads by default (MAC_ROWS_EY = 6 or 8):
+// a 256^3 level then gives 64 y-tiles x 4 z-chunks, one block per CU on all
+// 256 CUs, where 8 rows leave 43 x 5 = 215 blocks and idle CUs.  which = 1:
+// FP32 storage at nx = 128 (MAC_ROWS_EY1 = 8 or 16, default 16: 1024 threads,
+// 14 of 16 rows written).
+static int rows_ey(int which)
+{
+    static int ey[2] = {-1, -1};
+    if (ey[which] < 0) {
+        const char *v = getenv(which ? "MAC_ROWS_EY1" : "MAC_ROWS_EY");
+        const int d = which ? 16 : 6;
+        ey[which] = v ? atoi(v) : d;
+        if (which == 0 && ey[0] != 6 && ey[0] != 8) ey[0] = d;
+        if (which == 1 && ey[1] != 8 && ey[1] != 16) ey[1] = d;
+    }
+    return ey[which];
+}
+
 static bool rr_rows_on()
 {
     static int rows = -1;   // MAC_RR_ROWS=0 keeps the block-per-coarse-row kernel
@@ -2945,8 +2964,8 @@ static bool res_rows_on(int &tz_env)
 bool compact_level_ok(int dim, const Lev<float> &L)
 {
     int tz_env;
-    return dim == 3 && sweep_variant() == 1082 && rows_fit<float, 8, 1024>(L) && rr_rows_on() &&
-           res_rows_on(tz_env);
+    const bool fit = rows_ey(0) == 6 ? rows_fit<float, 6, 768>(L) : rows_fit<float, 8, 1024>(L);
+    return dim == 3 && sweep_variant() == 1082 && fit && rr_rows_on() && res_rows_on(tz_env);
 }
 
 template <typename T>
@@ -2957,6 +2976,7 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
     if constexpr (sizeof(T) == 4) {
         if (K) {
             if (mode == SWEEP_PROLONG || variant != 1082) return false;
+            if (rows_ey(0) == 6) return launch_rows<T, 6, 768, true>(L, pin, pout, mode, s, K);
             return launch_rows<T, 8, 1024, true>(L, pin, pout, mode, s, K);
         }
     } else {
@@ -2967,6 +2987,8 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
         switch (variant) {
         case 1082:
             if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
+            else if (L.nx == 128 && rows_ey(1) == 16)
+                done = launch_rows<T, 16, 1024, false>(L, pin, pout, mode, s, nullptr);
             else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
             break;
         default: break;
