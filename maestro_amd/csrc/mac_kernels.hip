@@ -3146,6 +3146,22 @@ void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, do
                                          r32, nrm));
 }
 
+__global__ void k_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(dst + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(src, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq);
+}
+
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s)
 {
     unsigned g = std::min(nblk(n), NORM_GRID);

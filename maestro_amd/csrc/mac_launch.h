@@ -109,6 +109,10 @@ void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, do
                                unsigned long long *nrm, hipStream_t s, unsigned short *r16 = nullptr);
 template <typename A, typename B> void launch_convert(const A *in, B *out, long n, hipStream_t s);
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
+// one thread: dst[0] = *src, then dst[1] = seq (system scope, dst mapped host
+// memory), then *src = 0
+void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
+                         hipStream_t s);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s, unsigned short *r16 = nullptr);
 template <typename T>

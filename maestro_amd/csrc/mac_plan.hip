@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -75,7 +76,14 @@ struct mac_plan {
     long part_cap = 0;
     double *mean_dev = nullptr;
     unsigned long long *nrm_dev = nullptr;
-    unsigned long long *nrm_host = nullptr;
+    unsigned long long *nrm_host = nullptr;   // pinned: [0] a copied norm
+    // residual norms reach the host through mapped pinned memory: a
+    // one-thread kernel writes the value and then a sequence number, which
+    // the host polls (no stream synchronisation per refinement step)
+    unsigned long long *nrm_map = nullptr;    // host pointer ([0] value, [1] sequence)
+    unsigned long long *nrm_map_dev = nullptr;
+    unsigned long long nrm_seq = 0;
+    bool nrm_map_ok = true;
     int *err_dev = nullptr;
     double *cg = nullptr;
     long cg_cap = 0;
@@ -444,12 +452,42 @@ static int compact_level0(mac_plan *P, const Compact16 **K)
     return MAC_OK;
 }
 
+// Residual norm slot: nrm_dev[1], accumulated by the residual kernels
+// (atomic max) and cleared again by the publishing kernel after each read.
+static unsigned long long *res_slot(mac_plan *P) { return P->nrm_dev + 1; }
+
 static int read_norm(mac_plan *P, double *out)
 {
-    HIPCHK(hipMemcpyAsync(P->nrm_host, P->nrm_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                          P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
-    unsigned long long u = *P->nrm_host;
+    unsigned long long u = 0;
+    bool have = false;
+    if (P->nrm_map_ok) {
+        const unsigned long long seq = ++P->nrm_seq;
+        launch_publish_norm(res_slot(P), P->nrm_map_dev, seq, P->stream);
+        HIPCHK(hipGetLastError());
+        volatile unsigned long long *h = P->nrm_map;
+        for (long it = 0;; ++it) {
+            if (h[1] == seq) { have = true; break; }
+            if ((it & 1023) == 1023) {
+                const hipError_t e = hipStreamQuery(P->stream);
+                if (e == hipSuccess) {   // drained: the value is visible unless the mapping failed
+                    std::atomic_thread_fence(std::memory_order_seq_cst);
+                    if (h[1] == seq) have = true;
+                    else P->nrm_map_ok = false;
+                    break;
+                }
+                if (e != hipErrorNotReady) return MAC_ERR_HIP;
+            }
+        }
+        std::atomic_thread_fence(std::memory_order_acquire);
+        if (have) u = h[0];
+    }
+    if (!have) {   // plain copy (the slot is then cleared here)
+        HIPCHK(hipMemcpyAsync(P->nrm_host, res_slot(P), sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        u = *P->nrm_host;
+    }
     double v;
     memcpy(&v, &u, sizeof v);
     *out = v;
@@ -459,18 +497,17 @@ static int read_norm(mac_plan *P, double *out)
 static int fp64_norm(mac_plan *P, double *out)
 {
     const Lev<double> &L = P->H64.lev[0];
-    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
     int q = prof_begin(P, MAC_PROF_RESID64);
-    launch_residual<double>(P->dim, L, (double *)nullptr, P->nrm_dev, P->stream);
+    launch_residual<double>(P->dim, L, (double *)nullptr, res_slot(P), P->stream);
     prof_end(P, q);
     return read_norm(P, out);
 }
 
 static int ir_norm(mac_plan *P, double *out, const Compact16 *K)
 {
-    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
     int q = prof_begin(P, MAC_PROF_RESID64);
-    launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, P->nrm_dev, P->stream, K ? K->rhs : nullptr);
+    launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, res_slot(P), P->stream,
+                       K ? K->rhs : nullptr);
     prof_end(P, q);
     return read_norm(P, out);
 }
@@ -491,6 +528,7 @@ static int solve(mac_plan *P, mac_stats *st)
     if (o.precision < MAC_PREC_FP64 || o.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
     double r0 = 0, r = 0;
     int it = 0, inner = 0;
+    HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
     if (o.precision == MAC_PREC_MIXED || o.precision == MAC_PREC_MIXED_BF16) {
         if ((rc = build_hier<float>(P, P->H32))) return rc;
         launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
@@ -507,22 +545,21 @@ static int solve(mac_plan *P, mac_stats *st)
                 ++inner;
             }
             // phi64 + e32 lands in phi2 together with the next residual;
-            // the two buffers then trade names (and the device copy of the
-            // level table follows)
+            // the two buffers then trade names
             Lev<double> &L0 = P->H64.lev[0];
-            HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), P->stream));
             int q = prof_begin(P, MAC_PROF_IR_UPDATE);
-            launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, P->nrm_dev, P->stream,
+            launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, res_slot(P), P->stream,
                                       K ? K->rhs : nullptr);
             prof_end(P, q);
             std::swap(L0.phi, L0.phi2);
-            HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &L0, sizeof(Lev<double>), hipMemcpyHostToDevice,
-                                  P->stream));
             if ((rc = read_norm(P, &r))) return rc;
             if (P->launch_failed) return MAC_ERR_INVALID;
             ++it;
             if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
         }
+        // the device copy of the level table follows the buffer swaps
+        HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &P->H64.lev[0], sizeof(Lev<double>), hipMemcpyHostToDevice,
+                              P->stream));
     } else {
         // the FP64 hierarchy's coarse weights (the mixed paths coarsen the FP32 one)
         coarsen_all<double>(P, P->H64);
@@ -690,6 +727,14 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
     if (hipMalloc(&P->mean_dev, sizeof(double)) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMalloc(&P->nrm_dev, sizeof(unsigned long long) * 4) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipHostMalloc(&P->nrm_host, sizeof(unsigned long long) * 4, 0) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipMemset(P->nrm_dev, 0, sizeof(unsigned long long) * 4) != hipSuccess) return fail(MAC_ERR_HIP);
+    if (hipHostMalloc(&P->nrm_map, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&P->nrm_map_dev, P->nrm_map, 0) != hipSuccess) {
+        P->nrm_map_ok = false;   // norms then come back through a plain copy
+        (void)hipGetLastError();
+    } else {
+        memset(P->nrm_map, 0, 64);
+    }
     if (hipMalloc(&P->err_dev, sizeof(int)) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMalloc(&P->tab_dev, sizeof(double *) * (size_t)P->nboxes * (2 * P->dim + 3)) != hipSuccess)
         return fail(MAC_ERR_HIP);
@@ -750,6 +795,7 @@ extern "C" int mac_plan_destroy(mac_plan *P)
     if (P->mean_dev) (void)hipFree(P->mean_dev);
     if (P->nrm_dev) (void)hipFree(P->nrm_dev);
     if (P->nrm_host) (void)hipHostFree(P->nrm_host);
+    if (P->nrm_map) (void)hipHostFree(P->nrm_map);
     if (P->err_dev) (void)hipFree(P->err_dev);
     if (P->cg) (void)hipFree(P->cg);
     if (P->tab_dev) (void)hipFree(P->tab_dev);
