@@ -180,6 +180,11 @@ int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
 #define MAC_ARR_BYH 6
 #define MAC_ARR_BZH 7
 #define MAC_ARR_RES 8
+#define MAC_ARR_F32 0x10     /* or-ed in (mac_debug_get only): the FP32 hierarchy's
+                                copy of the array, widened to FP64            */
+#define MAC_ARR_BF16 0x20    /* or-ed in (mac_debug_get only, level 0): the
+                                compact bfloat16 copy of a weight array or the
+                                right side, widened to FP64                    */
 #define MAC_OP_GSRB 0        /* one red-black sweep on `level`             */
 #define MAC_OP_RESIDUAL 1    /* res = rhs - A phi on `level`              */
 #define MAC_OP_RESTRICT 2    /* rhs(level+1) = average of res(level)      */

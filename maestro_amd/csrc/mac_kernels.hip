@@ -168,8 +168,22 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
 __global__ void __launch_bounds__(1024) k_mean_finalize(const double *part, long nparts, double ncell,
                                                         double *mean_out)
 {
+    // batches of 16 pairs per thread are loaded before they are added, so
+    // the loads overlap instead of waiting one by one
+    constexpr int BT = 16;
     double s = 0.0, c = 0.0;
-    for (long p = threadIdx.x; p < nparts; p += blockDim.x) comp_add(s, c, part[2 * p], part[2 * p + 1]);
+    const long step = (long)blockDim.x;
+    for (long p0 = threadIdx.x; p0 < nparts; p0 += BT * step) {
+        double vs[BT], vc[BT];
+#pragma unroll
+        for (int t = 0; t < BT; ++t) {
+            const long p = p0 + t * step;
+            vs[t] = p < nparts ? part[2 * p] : 0.0;
+            vc[t] = p < nparts ? part[2 * p + 1] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < BT; ++t) comp_add(s, c, vs[t], vc[t]);
+    }
     wave_comp_sum(s, c);
     __shared__ double ss[16], sc[16];
     const int wv = threadIdx.x >> 6;
@@ -262,6 +276,76 @@ __global__ void __launch_bounds__(TPB) k_weights_to_f32(Lev<double> S, Lev<float
     if (I == 0) D.bxh[J + (long)S.ny * K] = (float)S.bxh[J + (long)S.ny * K];
     if (J == 0) D.byh[I + (long)S.nx * K] = (float)S.byh[I + (long)S.nx * K];
     if (DIM == 3 && K == 0) D.bzh[I + (long)S.nx * J] = (float)S.bzh[I + (long)S.nx * J];
+}
+
+// Compact level 0 and FP32 level 1 from the FP64 level 0 in one pass (3D):
+// one thread per coarse cell reads the weights of its 2x2x2 fine cells,
+// writes them as bfloat16 of their FP32 rounding (fine column pairs as one
+// 32-bit store), and writes the coarse weights with k_coarsen's FP32
+// arithmetic on the same FP32 values.
+__global__ void __launch_bounds__(TPB) k_compact_level0(Lev<double> F, Compact16 K, Lev<float> C)
+{
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    const long nc = (long)C.nx * C.ny * C.nz;
+    if (q >= nc) return;
+    const int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), Kc = (int)(q / ((long)C.nx * C.ny));
+    const int i = 2 * I, j = 2 * J, k = 2 * Kc;
+    auto fc = [&](int jj, int kk) { return i + (long)F.nx * (jj + (long)F.ny * kk); };
+    float wx[2][2][2], wy[2][2][2], wz[2][2][2];   // [dk][dj][di]
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const long f = fc(j + dj, k + dk);
+            const double2 bx = *(const double2 *)(F.bx + f), by = *(const double2 *)(F.by + f),
+                          bz = *(const double2 *)(F.bz + f);
+            wx[dk][dj][0] = (float)bx.x; wx[dk][dj][1] = (float)bx.y;
+            wy[dk][dj][0] = (float)by.x; wy[dk][dj][1] = (float)by.y;
+            wz[dk][dj][0] = (float)bz.x; wz[dk][dj][1] = (float)bz.y;
+            *(unsigned *)(K.bx + f) = bf16_pack(wx[dk][dj][0], wx[dk][dj][1]);
+            *(unsigned *)(K.by + f) = bf16_pack(wy[dk][dj][0], wy[dk][dj][1]);
+            *(unsigned *)(K.bz + f) = bf16_pack(wz[dk][dj][0], wz[dk][dj][1]);
+        }
+    const float q4 = 0.25f;
+    C.bx[q] = q4 * ((wx[0][0][0] + wx[0][1][0]) + (wx[1][0][0] + wx[1][1][0]));
+    C.by[q] = q4 * ((wy[0][0][0] + wy[0][0][1]) + (wy[1][0][0] + wy[1][0][1]));
+    C.bz[q] = q4 * ((wz[0][0][0] + wz[0][0][1]) + (wz[0][1][0] + wz[0][1][1]));
+    if (I == C.nx - 1) {   // x-high slab entries of rows j, j+1 on planes k, k+1
+        float h[2][2];
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const long s = (j + dj) + (long)F.ny * (k + dk);
+                h[dk][dj] = (float)F.bxh[s];
+                K.bxh[s] = f32_to_bf16(h[dk][dj]);
+            }
+        C.bxh[J + (long)C.ny * Kc] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
+    if (J == C.ny - 1) {   // y-high slab entries of columns i, i+1 on planes k, k+1
+        float h[2][2];
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const long s = (i + di) + (long)F.nx * (k + dk);
+                h[dk][di] = (float)F.byh[s];
+                K.byh[s] = f32_to_bf16(h[dk][di]);
+            }
+        C.byh[I + (long)C.nx * Kc] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
+    if (Kc == C.nz - 1) {   // z-high slab entries of columns i, i+1 on rows j, j+1
+        float h[2][2];
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const long s = (i + di) + (long)F.nx * (j + dj);
+                h[dj][di] = (float)F.bzh[s];
+                K.bzh[s] = f32_to_bf16(h[dj][di]);
+            }
+        C.bzh[I + (long)C.nx * J] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
 }
 
 __global__ void __launch_bounds__(TPB) k_to_bf16(const float *__restrict__ in, unsigned short *__restrict__ out,
@@ -3080,6 +3164,12 @@ template bool launch_resid_restrict<double>(int, const Lev<double> &, const doub
                                             const Lev<double> &, hipStream_t, const Compact16 *);
 template bool launch_resid_restrict<float>(int, const Lev<float> &, const float *,
                                            const Lev<float> &, hipStream_t, const Compact16 *);
+
+void launch_compact_level0(const Lev<double> &F, const Compact16 &K, const Lev<float> &C, hipStream_t s)
+{
+    const long n = (long)C.nx * C.ny * C.nz;
+    hipLaunchKernelGGL(k_compact_level0, dim3(nblk(n)), dim3(TPB), 0, s, F, K, C);
+}
 
 void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s)
 {

@@ -102,6 +102,9 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
 bool compact_level_ok(int dim, const Lev<float> &L);
 // FP32 -> bfloat16 copy of n values
 void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s);
+// compact level 0 (bfloat16 of the FP32 rounding of F's weights) and the FP32
+// weights of level 1 (C), from the FP64 level 0 F in one pass (3D)
+void launch_compact_level0(const Lev<double> &F, const Compact16 &K, const Lev<float> &C, hipStream_t s);
 template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s);
 // r16 (compact level 0) replaces r32 as the output of the FP32 right side

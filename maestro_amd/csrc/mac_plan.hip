@@ -410,17 +410,25 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compa
     }
 }
 
-// Level 0 of the FP32 hierarchy in compact storage (MAC_PREC_MIXED_BF16):
-// *K is set where every level-0 kernel of the mixed path has its compact
-// form (compact_level_ok), else left null and the FP32 arrays are used.  The
-// weights are refreshed from H32 level 0; the right side is written by the
-// refinement residual.
-static int compact_level0(mac_plan *P, const Compact16 **K)
+// whether level 0 of the FP32 hierarchy is kept in compact storage
+// (MAC_PREC_MIXED_BF16 where every level-0 kernel of the mixed path has its
+// compact form)
+static bool compact_applies(mac_plan *P)
+{
+    return P->opts.precision == MAC_PREC_MIXED_BF16 && P->H32.built && use_fused(P) && !fuse_prolong_on() &&
+           compact_level_ok(P->dim, P->H32.lev[0]);
+}
+
+// Level 0 of the FP32 hierarchy in compact storage: *K is set where it
+// applies, else left null and the FP32 arrays are used.  from_f64: the
+// weights come from the FP64 level 0 in one pass that also writes the FP32
+// level-1 weights; otherwise from H32 level 0.  The right side is written by
+// the refinement residual.
+static int compact_level0(mac_plan *P, const Compact16 **K, bool from_f64)
 {
     *K = nullptr;
-    if (P->opts.precision != MAC_PREC_MIXED_BF16 || !P->H32.built) return MAC_OK;
+    if (!compact_applies(P)) return MAC_OK;
     const Lev<float> &F = P->H32.lev[0];
-    if (!use_fused(P) || fuse_prolong_on() || !compact_level_ok(P->dim, F)) return MAC_OK;
     Compact16 &C = P->c16;
     if (P->c16_allocs.empty()) {   // one allocation (one buffer resource for the kernels)
         const long nc = (long)F.nx * F.ny * F.nz;
@@ -442,6 +450,11 @@ static int compact_level0(mac_plan *P, const Compact16 **K)
         }
     }
     const long nc = (long)F.nx * F.ny * F.nz;
+    if (from_f64 && P->H32.nlev > 1) {
+        launch_compact_level0(P->H64.lev[0], C, P->H32.lev[1], P->stream);
+        *K = &C;
+        return MAC_OK;
+    }
     launch_to_bf16(F.bx, C.bx, nc, P->stream);
     launch_to_bf16(F.by, C.by, nc, P->stream);
     launch_to_bf16(F.bz, C.bz, nc, P->stream);
@@ -531,10 +544,17 @@ static int solve(mac_plan *P, mac_stats *st)
     HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
     if (o.precision == MAC_PREC_MIXED || o.precision == MAC_PREC_MIXED_BF16) {
         if ((rc = build_hier<float>(P, P->H32))) return rc;
-        launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
-        coarsen_all<float>(P, P->H32);
         const Compact16 *K = nullptr;
-        if ((rc = compact_level0(P, &K))) return rc;
+        if (compact_applies(P) && P->H32.nlev > 1) {
+            // compact level 0 and FP32 level 1 in one pass, then the rest
+            if ((rc = compact_level0(P, &K, true))) return rc;
+            for (int l = 2; l < P->H32.nlev; ++l)
+                launch_coarsen<float>(P->dim, P->H32.lev[l - 1], P->H32.lev[l], P->stream);
+        } else {
+            launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
+            coarsen_all<float>(P, P->H32);
+            if ((rc = compact_level0(P, &K, false))) return rc;
+        }
         P->launch_failed = false;
         if ((rc = ir_norm(P, &r0, K))) return rc;
         r = r0;
@@ -1217,8 +1237,54 @@ extern "C" int mac_debug_level_dims(mac_plan *P, int level, int dims[3])
     return MAC_OK;
 }
 
+// the FP32-hierarchy (MAC_ARR_F32) or compact level-0 (MAC_ARR_BF16) copy of
+// an array, widened to FP64 on the host
+static int debug_get_narrow(mac_plan *P, int level, int what, double *out)
+{
+    const int base = what & 0xf;
+    if (!P->H32.built || level < 0 || level >= P->H32.nlev) return MAC_ERR_INVALID;
+    const Lev<float> &F = P->H32.lev[level];
+    const long nc = (long)F.nx * F.ny * F.nz;
+    const void *src = nullptr;
+    long n = nc;
+    if (what & MAC_ARR_F32) {
+        const float *a[9] = {F.phi, F.rhs, F.bx, F.by, F.bz, F.bxh, F.byh, F.bzh, F.res};
+        if (base > 8) return MAC_ERR_INVALID;
+        src = a[base];
+    } else {
+        const Compact16 &K = P->c16;
+        if (level != 0 || P->c16_allocs.empty()) return MAC_ERR_INVALID;
+        const unsigned short *a[8] = {nullptr, K.rhs, K.bx, K.by, K.bz, K.bxh, K.byh, K.bzh};
+        if (base > 7) return MAC_ERR_INVALID;
+        src = a[base];
+    }
+    if (base == MAC_ARR_BXH) n = (long)F.ny * F.nz;
+    if (base == MAC_ARR_BYH) n = (long)F.nx * F.nz;
+    if (base == MAC_ARR_BZH) n = (long)F.nx * F.ny;
+    if (!src) return MAC_ERR_INVALID;
+    const size_t es = (what & MAC_ARR_F32) ? 4 : 2;
+    std::vector<unsigned char> h(es * (size_t)n);
+    HIPCHK(hipMemcpyAsync(h.data(), src, h.size(), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (long q = 0; q < n; ++q) {
+        float f;
+        if (es == 4) {
+            memcpy(&f, &h[4 * q], 4);
+        } else {
+            unsigned short u;
+            memcpy(&u, &h[2 * q], 2);
+            const unsigned w = (unsigned)u << 16;
+            memcpy(&f, &w, 4);
+        }
+        out[q] = (double)f;
+    }
+    return MAC_OK;
+}
+
 extern "C" int mac_debug_get(mac_plan *P, int level, int what, double *out)
 {
+    if (!P || !out) return MAC_ERR_INVALID;
+    if (what & (MAC_ARR_F32 | MAC_ARR_BF16)) return debug_get_narrow(P, level, what, out);
     double *p;
     long n;
     int rc = debug_array(P, level, what, &p, &n);
@@ -1255,7 +1321,7 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         launch_convert<double, float>(L.phi, F.phi, n, s);
         launch_convert<double, float>(L.rhs, F.rhs, n, s);
         const Compact16 *K = nullptr;
-        if (level == 0 && (rc = compact_level0(P, &K))) return rc;
+        if (level == 0 && (rc = compact_level0(P, &K, false))) return rc;
         if (K) launch_to_bf16(F.rhs, K->rhs, n, s);
         int cur = 0;
         for (int c = 0; c < count; ++c) {

@@ -38,6 +38,7 @@ PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coars
         "gsrb64_zero": 6, "gsrb32_zero": 7, "restrict64": 8, "restrict32": 9, "prolong64": 10,
         "prolong32": 11, "ir_update": 12}
 ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
+ARR_F32, ARR_BF16 = 0x10, 0x20   # debug_get(level, what, copy="f32" | "bf16")
 OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5,
       "resid_restrict": 6, "prolong_sweep": 7, "gsrb32": 8}
 
@@ -281,9 +282,12 @@ class MacPlan:
         nx, ny, nz = self.level_dims(level)
         return {"bxh": ny * nz, "byh": nx * nz, "bzh": nx * ny}.get(what, nx * ny * nz)
 
-    def debug_get(self, level, what):
+    def debug_get(self, level, what, copy=None):
+        """An internal level array (FP64); copy="f32" / "bf16": the FP32
+        hierarchy's / the compact level-0 copy, widened to FP64."""
         out = np.zeros(self._arr_size(level, what))
-        rc = lib().mac_debug_get(self._h, level, ARR[what], C.c_void_p(out.ctypes.data))
+        code = ARR[what] | {None: 0, "f32": ARR_F32, "bf16": ARR_BF16}[copy]
+        rc = lib().mac_debug_get(self._h, level, code, C.c_void_p(out.ctypes.data))
         if rc:
             raise MacError(rc, "mac_debug_get")
         return out

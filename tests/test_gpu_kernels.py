@@ -288,3 +288,40 @@ def test_setup_and_correction_match_oracle(gpu, oracle_lib, cfg, n, box, alt):
     for d in range(dim):
         assert np.array_equal(out["faces"][d], proj[d]), d
     plan.close()
+
+
+@pytest.mark.parametrize("cfg,n,box", [(2, (256, 64, 64), 64), (3, (128, 64, 64), 32), (1, (128, 32, 32), 32)])
+def test_compact_level0_and_level1_weights_bitwise(gpu, cfg, n, box):
+    """After a MAC_PREC_MIXED_BF16 solve: the compact finest-level weights are
+    the bfloat16 rounding of the FP32 rounding of the FP64 weights, and the
+    FP32 level-1 weights are the FP32 averages of the FP32-rounded fine faces
+    (the one-pass conversion against the same steps done here in numpy)."""
+    from .helpers import run_gpu
+    wl = W.make_workload(cfg, n=n, box=box)
+    from .helpers import layout_for
+    plan = mac.MacPlan(layout_for(wl), mac.default_opts(precision=mac.PREC_MIXED_BF16))
+    out = run_gpu(wl, plan=plan)
+    assert out["stats"]["converged"]
+    nx, ny, nz = n
+    shapes = {"bx": (nz, ny, nx), "by": (nz, ny, nx), "bz": (nz, ny, nx),
+              "bxh": (nz, ny), "byh": (nz, nx), "bzh": (ny, nx)}
+    f32 = {}
+    for w, shp in shapes.items():
+        a = plan.debug_get(0, w).reshape(shp).astype(np.float32)
+        f32[w] = a
+        got = plan.debug_get(0, w, copy="bf16").reshape(shp)
+        assert np.array_equal(got, bf16_round(a).reshape(shp).astype(np.float64)), w
+    q4 = np.float32(0.25)
+    bx, by, bz = f32["bx"], f32["by"], f32["bz"]
+    ref = {
+        "bx": q4 * ((bx[0::2, 0::2, 0::2] + bx[0::2, 1::2, 0::2]) + (bx[1::2, 0::2, 0::2] + bx[1::2, 1::2, 0::2])),
+        "by": q4 * ((by[0::2, 0::2, 0::2] + by[0::2, 0::2, 1::2]) + (by[1::2, 0::2, 0::2] + by[1::2, 0::2, 1::2])),
+        "bz": q4 * ((bz[0::2, 0::2, 0::2] + bz[0::2, 0::2, 1::2]) + (bz[0::2, 1::2, 0::2] + bz[0::2, 1::2, 1::2])),
+    }
+    for w in ("bxh", "byh", "bzh"):
+        h = f32[w]
+        ref[w] = q4 * ((h[0::2, 0::2] + h[0::2, 1::2]) + (h[1::2, 0::2] + h[1::2, 1::2]))
+    for w, r in ref.items():
+        got = plan.debug_get(1, w, copy="f32").reshape(r.shape)
+        assert np.array_equal(got, r.astype(np.float64)), w
+    plan.close()
