@@ -45,10 +45,11 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
     if v in (8, 16):
         return f"k_gsrb_pair<{t}, 0, {v}>"
     c = "true" if b16 else "false"
-    if v == 1082:   # full rows: 8 rows x 1024 threads (FP32), 6 rows x 768 threads (FP64); W = nx / 128
-        w = max(1, nx // 128)
-        return (f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}>" if s == 4
-                else f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}>")
+    if v == 1082:   # full rows, W = nx / 128; rows x threads per block: FP64 6 x 768, FP32 8 x 1024
+        w = max(1, nx // 128)   # (16 x 1024 at nx = 128), FP32 with bfloat16 storage 6 x 768
+        if s == 8 or b16:
+            return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}>"
+        return f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1>" if w == 1 else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}>"
     return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}>"
 
 
