@@ -112,7 +112,8 @@ def test_gsrb32_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
 
 
 @pytest.mark.parametrize("sweeps", [1, 2])
-@pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (128, 30, 6), (128, 12, 9), (64, 48, 40)])
+@pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (256, 14, 8), (256, 30, 9), (128, 30, 6),
+                               (128, 12, 9), (64, 48, 40)])
 @pytest.mark.parametrize("bc", BCS_3D)
 def test_gsrb32_bf16_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
     """The compact-storage smoother (bfloat16 weights and rhs, FP32 iterate
