@@ -165,6 +165,65 @@ int mac_adjoint_level(mac_plan *plan, const double *phi, const double *beta_x,
                       double *out_uy, double *out_uz, double *out_bx,
                       double *out_by, double *out_bz, mac_stats *stats);
 
+/* ---- producers of the projection's inputs --------------------------------
+ * Per-box arrays as for mac_project (the plan's layout; pointers follow
+ * opts.device_ptrs, 1-D arrays included).  Each is one pass on the plan's
+ * stream and returns when it completes. */
+
+/* cell_to_edge (Source/cell_to_edge.f90:14-68, one level, planar): the
+ * radial edge values s0_edge[0..nr] of the bin values s0_cell[0..nr-1] --
+ * edges 0 and nr copy the end bins, edges 1 and nr-1 average their two bins,
+ * the others take the fourth-order average 7/12 (s_r + s_r-1) - 1/12 (s_r+1
+ * + s_r-2) limited to the range of s_r-1, s_r.  nr >= 2. */
+int mac_cell_to_edge(mac_plan *plan, const double *s0_cell, int nr, double *s0_edge);
+
+/* Spherical base-state geometry (Source/geometry.f90:140-156,
+ * initialize.f90:276-291): nr radial bins of width dr with centres
+ * (i + 1/2) dr; the star's centre and the domain's low corner in physical
+ * units; interp = s0_interp_type of put_1d_array_on_cart_3d_sphr
+ * (fill_3d_data.f90:406-531): 1 piecewise constant, 2 piecewise linear,
+ * 3 quadratic (the reference default, _parameters:632). */
+typedef struct mac_sphr {
+    double dr;
+    int nr;
+    double center[3];
+    double prob_lo[3];
+    int interp;
+} mac_sphr;
+
+/* Inputs of make_macrhs (Source/make_macrhs.f90:24-342).  Per-box cell
+ * arrays with their ghost widths; 1-D arrays indexed by the radial
+ * coordinate (the last direction's cell index when planar, the radial bin
+ * when spherical).  The dp/dt term is on when dpdt_factor > 0 (planar) or
+ * >= 0 (spherical, as :306 of the reference); it then needs delta_p_term,
+ * delta_chi, gamma1bar, p0 and, spherical, rho0. */
+typedef struct mac_macrhs_in {
+    const double *const *S_cc;               int ng_S;
+    const double *const *delta_gamma1_term;  int ng_dg;
+    const double *const *delta_p_term;       int ng_dp;     /* may be NULL if the term is off */
+    double *const *delta_chi;                int ng_dchi;   /* in/out; NULL if the term is off */
+    const double *Sbar, *beta0, *gamma1bar, *p0, *rho0;
+    double dt, dpdt_factor;
+    int base_cutoff_coord;          /* planar: the term applies below this radial index */
+    double base_cutoff_density;     /* spherical: ... where rho0 exceeds this density   */
+    int is_predictor;               /* delta_chi starts from zero (whole arrays)       */
+} mac_macrhs_in;
+
+/* make_macrhs: macrhs = beta0 (S_cc - Sbar + delta_gamma1_term) per valid
+ * cell, plus beta0 delta_chi with delta_chi += dpdt_factor delta_p_term /
+ * (dt gamma1bar p0) where the dp/dt term applies.  sph NULL: planar (2D/3D);
+ * otherwise spherical (3D), the 1-D profiles put on the cells by sph->interp.
+ * macrhs uses the layout's ng_macrhs. */
+int mac_make_macrhs(mac_plan *plan, double *const *macrhs, const mac_macrhs_in *in,
+                    const mac_sphr *sph);
+
+/* make_s0mac, s0mac_interp_type 1 (Source/fill_3d_data.f90:942-1075): the
+ * radial profile s0[0..sph->nr-1] on the cell centres, cells outside the
+ * domain by first-order extrapolation, then every face of the per-box face
+ * arrays s0mac[box*3 + d] (one ghost layer, the layout's ng_beta0_cart
+ * must be 1) the mean of its two cells.  3D. */
+int mac_make_s0mac(mac_plan *plan, const double *s0, const mac_sphr *sph, double *const *s0mac);
+
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Read or overwrite an internal FP64 array of MG level `level` (0 = finest)
  * through host memory, and run single solver operators on it, so tests can

@@ -77,6 +77,52 @@ struct AdjIO {
     EllBC bc;
 };
 
+// spherical base-state geometry: radial bins of width dr (centres
+// (i + 1/2) dr, i < nr), the star's centre, the domain's low corner, the cell
+// size and the radial interpolation type (1 constant, 2 linear, 3 quadratic)
+struct SphrGeom {
+    double dr;
+    int nr;
+    double center[3], prob_lo[3], dx[3];
+    int interp;
+};
+
+// make_macrhs over the caller's boxes (see k_make_macrhs)
+struct MacrhsArgs {
+    int dim, nboxes;
+    const BoxDev *boxes;
+    int dlo[3];
+    double *const *rhs;
+    int ng_rhs;
+    const double *const *S;
+    int ng_S;
+    const double *const *dg;
+    int ng_dg;
+    const double *const *dp;
+    int ng_dp;
+    double *const *dchi;
+    int ng_dchi;
+    const double *Sbar, *beta0, *gamma1bar, *p0, *rho0;   // 1-D, device
+    double dt, dpdt;
+    int dpdt_on, predictor, cutoff_coord;
+    double cutoff_density;
+    SphrGeom geo;
+};
+
+// make_s0mac over the caller's boxes (face arrays with one ghost layer)
+struct S0macArgs {
+    int nboxes;
+    const BoxDev *boxes;
+    int dlo[3], n[3];
+    const double *s0;   // radial profile, device
+    double *const *out; // [box*3 + d]
+    SphrGeom geo;
+};
+
+void launch_cell_to_edge(const double *c, int nr, double *e, hipStream_t s);
+void launch_make_macrhs(const MacrhsArgs &A, int maxcells, bool sph, hipStream_t s);
+void launch_make_s0mac(const S0macArgs &A, int maxfaces, hipStream_t s);
+
 void launch_setup(const SetupArgs &A, int maxcells, hipStream_t s);
 long setup_nparts(int maxcells, int nboxes);
 void launch_mean(const double *part, long nparts, double ncell, double *mean, hipStream_t s);

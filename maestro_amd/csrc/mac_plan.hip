@@ -1402,3 +1402,209 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
     HIPCHK(hipStreamSynchronize(s));
     return MAC_OK;
 }
+
+// ------------------------------------------ producers of the path's inputs
+namespace {
+
+// device mirrors of a call's caller arrays (host pointers when
+// opts.device_ptrs == 0) and device pointer tables; freed with the object
+struct CallStage {
+    mac_plan *P;
+    std::vector<void *> mem;
+    explicit CallStage(mac_plan *p) : P(p) {}
+    ~CallStage()
+    {
+        if (P->stream) (void)hipStreamSynchronize(P->stream);
+        for (void *q : mem) (void)hipFree(q);
+    }
+    // device view of n doubles at h (copied in when in)
+    int view(const double *h, size_t n, bool in, double **d)
+    {
+        if (!h) { *d = nullptr; return MAC_OK; }
+        if (P->opts.device_ptrs) { *d = const_cast<double *>(h); return MAC_OK; }
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, sizeof(double) * std::max<size_t>(n, 1)));
+        mem.push_back(q);
+        if (in) HIPCHK(hipMemcpyAsync(q, h, sizeof(double) * n, hipMemcpyHostToDevice, P->stream));
+        *d = (double *)q;
+        return MAC_OK;
+    }
+    int back(double *h, const double *d, size_t n)
+    {
+        if (!h || P->opts.device_ptrs) return MAC_OK;
+        HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
+        return MAC_OK;
+    }
+    // a device table of device views of per-box arrays
+    int table(const double *const *h, const std::vector<size_t> &n, bool in, std::vector<double *> &views,
+              double ***tab)
+    {
+        views.assign(n.size(), nullptr);
+        if (!h) { *tab = nullptr; return MAC_OK; }
+        int rc;
+        for (size_t a = 0; a < n.size(); ++a)
+            if ((rc = view(h[a], n[a], in, &views[a]))) return rc;
+        void *q = nullptr;
+        HIPCHK(hipMalloc(&q, sizeof(double *) * std::max<size_t>(n.size(), 1)));
+        mem.push_back(q);
+        HIPCHK(hipMemcpyAsync(q, views.data(), sizeof(double *) * n.size(), hipMemcpyHostToDevice, P->stream));
+        *tab = (double **)q;
+        return MAC_OK;
+    }
+};
+
+SphrGeom sphr_geom(const mac_plan *P, const mac_sphr *s)
+{
+    SphrGeom G;
+    G.dr = s->dr;
+    G.nr = s->nr;
+    for (int d = 0; d < 3; ++d) {
+        G.center[d] = s->center[d];
+        G.prob_lo[d] = s->prob_lo[d];
+        G.dx[d] = P->dx[d];
+    }
+    G.interp = s->interp;
+    return G;
+}
+
+bool sphr_ok(const mac_sphr *s)
+{
+    return s && s->dr > 0.0 && s->nr >= 3 && s->interp >= 1 && s->interp <= 3;
+}
+
+// every cell centre's bin index int(radius / dr) is below nr: the kernels'
+// expressions evaluated at the eight corner cells, where the radius (monotone
+// in each coordinate's distance from the centre) is largest
+bool sphr_covers(const mac_plan *P, const mac_sphr *s)
+{
+    for (int c = 0; c < 8; ++c) {
+        double x[3];
+        for (int d = 0; d < 3; ++d) {
+            const int i = (c >> d) & 1 ? P->n[d] - 1 : 0;
+            x[d] = s->prob_lo[d] + ((double)i + 0.5) * P->dx[d] - s->center[d];
+        }
+        const double radius = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        if (!(radius / s->dr < (double)s->nr) || (int)(radius / s->dr) > s->nr - 1) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" int mac_cell_to_edge(mac_plan *P, const double *s0_cell, int nr, double *s0_edge)
+{
+    if (!P || !s0_cell || !s0_edge || nr < 2) return MAC_ERR_INVALID;
+    HIPCHK(hipSetDevice(P->device));
+    int rc;
+    CallStage S(P);
+    double *c, *e;
+    if ((rc = S.view(s0_cell, nr, true, &c)) || (rc = S.view(s0_edge, nr + 1, false, &e))) return rc;
+    launch_cell_to_edge(c, nr, e, P->stream);
+    HIPCHK(hipGetLastError());
+    if ((rc = S.back(s0_edge, e, nr + 1))) return rc;
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}
+
+extern "C" int mac_make_macrhs(mac_plan *P, double *const *macrhs, const mac_macrhs_in *in, const mac_sphr *sph)
+{
+    if (!P || !macrhs || !in || !in->S_cc || !in->delta_gamma1_term || !in->Sbar || !in->beta0)
+        return MAC_ERR_INVALID;
+    if (in->ng_S < 0 || in->ng_dg < 0 || in->ng_dp < 0 || in->ng_dchi < 0) return MAC_ERR_LAYOUT;
+    if (sph && (P->dim != 3 || !sphr_ok(sph) || !sphr_covers(P, sph))) return MAC_ERR_INVALID;
+    const bool dpdt_on = sph ? in->dpdt_factor >= 0.0 : in->dpdt_factor > 0.0;
+    if (dpdt_on && (!in->delta_p_term || !in->delta_chi || !in->gamma1bar || !in->p0 || (sph && !in->rho0)))
+        return MAC_ERR_INVALID;
+    if (dpdt_on && in->dt == 0.0) return MAC_ERR_INVALID;
+    HIPCHK(hipSetDevice(P->device));
+    const int nb = P->nboxes, dim = P->dim;
+    const int nr = sph ? sph->nr : P->n[dim - 1];
+    int rc;
+    CallStage S(P);
+    std::vector<size_t> n_rhs(nb), n_S(nb), n_dg(nb), n_dp(nb), n_dchi(nb);
+    for (int b = 0; b < nb; ++b) {
+        n_rhs[b] = box_elems(P, b, P->ng_rhs, -1);
+        n_S[b] = box_elems(P, b, in->ng_S, -1);
+        n_dg[b] = box_elems(P, b, in->ng_dg, -1);
+        n_dp[b] = box_elems(P, b, in->ng_dp, -1);
+        n_dchi[b] = box_elems(P, b, in->ng_dchi, -1);
+    }
+    std::vector<double *> v_rhs, v_S, v_dg, v_dp, v_dchi;
+    double **t_rhs, **t_S, **t_dg, **t_dp = nullptr, **t_dchi = nullptr;
+    // macrhs is staged in as well: its ghost cells are left as the caller had them
+    if ((rc = S.table(macrhs, n_rhs, true, v_rhs, &t_rhs)) || (rc = S.table(in->S_cc, n_S, true, v_S, &t_S)) ||
+        (rc = S.table(in->delta_gamma1_term, n_dg, true, v_dg, &t_dg)))
+        return rc;
+    if (dpdt_on && ((rc = S.table(in->delta_p_term, n_dp, true, v_dp, &t_dp)) ||
+                    (rc = S.table((const double *const *)in->delta_chi, n_dchi, true, v_dchi, &t_dchi))))
+        return rc;
+    MacrhsArgs A;
+    A.dim = dim;
+    A.nboxes = nb;
+    A.boxes = P->boxes_dev;
+    for (int d = 0; d < 3; ++d) A.dlo[d] = P->dlo[d];
+    A.rhs = t_rhs; A.ng_rhs = P->ng_rhs;
+    A.S = t_S; A.ng_S = in->ng_S;
+    A.dg = t_dg; A.ng_dg = in->ng_dg;
+    A.dp = t_dp; A.ng_dp = in->ng_dp;
+    A.dchi = t_dchi; A.ng_dchi = in->ng_dchi;
+    double *d1[5];
+    const double *h1[5] = {in->Sbar, in->beta0, dpdt_on ? in->gamma1bar : nullptr, dpdt_on ? in->p0 : nullptr,
+                           (dpdt_on && sph) ? in->rho0 : nullptr};
+    for (int a = 0; a < 5; ++a)
+        if ((rc = S.view(h1[a], nr, true, &d1[a]))) return rc;
+    A.Sbar = d1[0]; A.beta0 = d1[1]; A.gamma1bar = d1[2]; A.p0 = d1[3]; A.rho0 = d1[4];
+    A.dt = in->dt;
+    A.dpdt = in->dpdt_factor;
+    A.dpdt_on = dpdt_on;
+    A.predictor = in->is_predictor != 0;
+    A.cutoff_coord = in->base_cutoff_coord;
+    A.cutoff_density = in->base_cutoff_density;
+    if (sph) A.geo = sphr_geom(P, sph);
+    else memset(&A.geo, 0, sizeof A.geo);
+    launch_make_macrhs(A, P->maxcells, sph != nullptr, P->stream);
+    HIPCHK(hipGetLastError());
+    for (int b = 0; b < nb; ++b) {
+        if ((rc = S.back(macrhs[b], v_rhs[b], n_rhs[b]))) return rc;
+        if (dpdt_on && (rc = S.back(in->delta_chi[b], v_dchi[b], n_dchi[b]))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}
+
+extern "C" int mac_make_s0mac(mac_plan *P, const double *s0, const mac_sphr *sph, double *const *s0mac)
+{
+    if (!P || !s0 || !s0mac || !sphr_ok(sph) || P->dim != 3 || !sphr_covers(P, sph)) return MAC_ERR_INVALID;
+    if (P->ng_cart != 1) return MAC_ERR_LAYOUT;   // make_s0mac assumes one ghost layer (:985-987)
+    HIPCHK(hipSetDevice(P->device));
+    const int nb = P->nboxes;
+    int rc;
+    CallStage S(P);
+    std::vector<size_t> n_f((size_t)nb * 3);
+    int maxfaces = 0;
+    for (int b = 0; b < nb; ++b)
+        for (int d = 0; d < 3; ++d) {
+            n_f[b * 3 + d] = box_elems(P, b, 1, d);
+            maxfaces = std::max(maxfaces, (int)n_f[b * 3 + d]);
+        }
+    std::vector<double *> v_f;
+    double **t_f;
+    double *d_s0;
+    if ((rc = S.table(s0mac, n_f, false, v_f, &t_f)) || (rc = S.view(s0, sph->nr, true, &d_s0))) return rc;
+    S0macArgs A;
+    A.nboxes = nb;
+    A.boxes = P->boxes_dev;
+    for (int d = 0; d < 3; ++d) {
+        A.dlo[d] = P->dlo[d];
+        A.n[d] = P->n[d];
+    }
+    A.s0 = d_s0;
+    A.out = t_f;
+    A.geo = sphr_geom(P, sph);
+    launch_make_s0mac(A, maxfaces, P->stream);
+    HIPCHK(hipGetLastError());
+    for (size_t a = 0; a < n_f.size(); ++a)
+        if ((rc = S.back(s0mac[a], v_f[a], n_f[a]))) return rc;
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MAC_OK;
+}

@@ -32,6 +32,7 @@ ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
           3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory"}
 SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
            "mac_project", "mac_project_level", "mac_adjoint_level", "mac_version",
+           "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
            "mac_plan_profile", "mac_plan_profile_read")
 PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5,
@@ -66,6 +67,34 @@ class COpts(C.Structure):
                 ("verbose", C.c_int)]
 
 
+class CSphr(C.Structure):
+    """mac_sphr: spherical base-state geometry."""
+    _fields_ = [("dr", C.c_double), ("nr", C.c_int), ("center", C.c_double * 3),
+                ("prob_lo", C.c_double * 3), ("interp", C.c_int)]
+
+
+class CMacrhsIn(C.Structure):
+    """mac_macrhs_in: the inputs of make_macrhs."""
+    _fields_ = [("S_cc", C.c_void_p), ("ng_S", C.c_int),
+                ("delta_gamma1_term", C.c_void_p), ("ng_dg", C.c_int),
+                ("delta_p_term", C.c_void_p), ("ng_dp", C.c_int),
+                ("delta_chi", C.c_void_p), ("ng_dchi", C.c_int),
+                ("Sbar", C.c_void_p), ("beta0", C.c_void_p), ("gamma1bar", C.c_void_p),
+                ("p0", C.c_void_p), ("rho0", C.c_void_p),
+                ("dt", C.c_double), ("dpdt_factor", C.c_double),
+                ("base_cutoff_coord", C.c_int), ("base_cutoff_density", C.c_double),
+                ("is_predictor", C.c_int)]
+
+
+def sphr(dr, nr, center, prob_lo=(0.0, 0.0, 0.0), interp=3) -> CSphr:
+    g = CSphr()
+    g.dr, g.nr, g.interp = float(dr), int(nr), int(interp)
+    for d in range(3):
+        g.center[d] = float(center[d])
+        g.prob_lo[d] = float(prob_lo[d])
+    return g
+
+
 class CStats(C.Structure):
     _fields_ = [("iters", C.c_int), ("inner_cycles", C.c_int), ("converged", C.c_int),
                 ("nlevels", C.c_int), ("rhs_norm", C.c_double), ("res0", C.c_double),
@@ -96,6 +125,9 @@ def lib():
         L.mac_project_level.argtypes = [vp] * 9 + [C.POINTER(CStats)]
         L.mac_adjoint_level.argtypes = [vp] * 16 + [C.POINTER(CStats)]
         L.mac_version.restype = C.c_char_p
+        L.mac_cell_to_edge.argtypes = [vp, vp, ip, vp]
+        L.mac_make_macrhs.argtypes = [vp, vp, C.POINTER(CMacrhsIn), C.POINTER(CSphr)]
+        L.mac_make_s0mac.argtypes = [vp, vp, C.POINTER(CSphr), vp]
         L.mac_debug_level_dims.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.mac_debug_get.argtypes = [vp, ip, ip, vp]
         L.mac_debug_set.argtypes = [vp, ip, ip, vp]
@@ -106,7 +138,8 @@ def lib():
                   "mac_plan_profile", "mac_plan_profile_read"):
             getattr(L, f).restype = ip
         for f in ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
-                  "mac_project", "mac_project_level", "mac_adjoint_level"):
+                  "mac_project", "mac_project_level", "mac_adjoint_level", "mac_cell_to_edge",
+                  "mac_make_macrhs", "mac_make_s0mac"):
             getattr(L, f).restype = ip
         _lib = L
     return _lib
@@ -253,6 +286,54 @@ class MacPlan:
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+    # ---- producers of the projection's inputs
+    def _mode(self, xs):
+        dev = any(_is_torch(x) for x in xs if x is not None)
+        if bool(self.opts.device_ptrs) != dev:
+            self.set_opts(device_ptrs=int(dev))
+        self._sync_torch(list(xs))
+
+    def cell_to_edge(self, s0_cell, s0_edge):
+        """cell_to_edge into s0_edge (nr + 1 values)."""
+        self._mode([s0_cell, s0_edge])
+        rc = lib().mac_cell_to_edge(self._h, _ptr(s0_cell), int(s0_cell.shape[0]), _ptr(s0_edge))
+        if rc:
+            raise MacError(rc, "mac_cell_to_edge")
+
+    def make_macrhs(self, macrhs, S_cc, delta_gamma1_term, Sbar, beta0, ng_S=0, ng_dg=0,
+                    delta_p_term=None, ng_dp=0, delta_chi=None, ng_dchi=0, gamma1bar=None, p0=None,
+                    rho0=None, dt=1.0, dpdt_factor=0.0, base_cutoff_coord=0, base_cutoff_density=0.0,
+                    is_predictor=True, sph=None):
+        """make_macrhs into the per-box arrays macrhs (lists of per-box
+        arrays; 1-D profiles as arrays)."""
+        flat = list(macrhs) + list(S_cc) + list(delta_gamma1_term) + list(delta_p_term or []) + \
+            list(delta_chi or []) + [Sbar, beta0, gamma1bar, p0, rho0]
+        self._mode(flat)
+        keep = [_ptr_array(list(macrhs)), _ptr_array(list(S_cc)), _ptr_array(list(delta_gamma1_term)),
+                _ptr_array(delta_p_term), _ptr_array(delta_chi)]
+        a = CMacrhsIn()
+        a.S_cc, a.ng_S = C.cast(keep[1], C.c_void_p), ng_S
+        a.delta_gamma1_term, a.ng_dg = C.cast(keep[2], C.c_void_p), ng_dg
+        a.delta_p_term, a.ng_dp = (C.cast(keep[3], C.c_void_p) if keep[3] is not None else None), ng_dp
+        a.delta_chi, a.ng_dchi = (C.cast(keep[4], C.c_void_p) if keep[4] is not None else None), ng_dchi
+        for f, x in (("Sbar", Sbar), ("beta0", beta0), ("gamma1bar", gamma1bar), ("p0", p0), ("rho0", rho0)):
+            setattr(a, f, _ptr(x).value if x is not None else None)
+        a.dt, a.dpdt_factor = float(dt), float(dpdt_factor)
+        a.base_cutoff_coord, a.base_cutoff_density = int(base_cutoff_coord), float(base_cutoff_density)
+        a.is_predictor = int(bool(is_predictor))
+        rc = lib().mac_make_macrhs(self._h, keep[0], C.byref(a), C.byref(sph) if sph is not None else None)
+        if rc:
+            raise MacError(rc, "mac_make_macrhs")
+
+    def make_s0mac(self, s0, sph, s0mac):
+        """make_s0mac into s0mac: per box, three face arrays (one ghost layer)."""
+        flat = [f for box in s0mac for f in box]
+        self._mode(flat + [s0])
+        tab = _ptr_array(flat)
+        rc = lib().mac_make_s0mac(self._h, _ptr(s0), C.byref(sph), tab)
+        if rc:
+            raise MacError(rc, "mac_make_s0mac")
 
     # ---- kernel timing on the plan's stream
     def profile(self, enable=True):

@@ -111,6 +111,20 @@ int orc_macproject(const orc_grid *g, double *u, double *v, double *w,
                    double *bx_out, double *by_out, double *bz_out,
                    double *rh_out);
 
+/* producers of the projection's inputs (mac_inputs_oracle.c) */
+void orc_cell_to_edge(const double *c, int nr, double *e);
+double orc_sphr_cell_value(const double *s0, int nr, double dr, const double *center,
+                           const double *prob_lo, const double *dx, int type, int i, int j, int k);
+void orc_make_macrhs(int dim, const int *n, const double *S, const double *dg, const double *Sbar,
+                     const double *beta0, const double *gamma1bar, const double *p0,
+                     const double *rho0, const double *dp, double *dchi, double dt,
+                     double dpdt_factor, int base_cutoff_coord, double base_cutoff_density,
+                     int is_predictor, double sph_dr, int sph_nr, const double *center,
+                     const double *prob_lo, const double *dx, int interp, double *rhs);
+void orc_make_s0mac(const int *n, const double *s0, int nr, double dr, const double *center,
+                    const double *prob_lo, const double *dx, int interp, double *fx, double *fy,
+                    double *fz);
+
 #ifdef __cplusplus
 }
 #endif

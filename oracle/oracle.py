@@ -98,6 +98,14 @@ def lib():
         L.orc_macproject.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp, dp,
                                      dp, dp, dp, C.POINTER(Opts), C.POINTER(Stats),
                                      dp, dp, dp, dp]
+        L.orc_cell_to_edge.argtypes = [dp, C.c_int, dp]
+        L.orc_sphr_cell_value.restype = C.c_double
+        L.orc_sphr_cell_value.argtypes = [dp, C.c_int, C.c_double, dp, dp, dp, C.c_int, C.c_int, C.c_int,
+                                          C.c_int]
+        L.orc_make_macrhs.argtypes = [C.c_int, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double,
+                                      C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                      dp, dp, dp, C.c_int, dp]
+        L.orc_make_s0mac.argtypes = [ip, dp, C.c_int, C.c_double, dp, dp, dp, C.c_int, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -293,3 +301,57 @@ def macproject(g, faces, phi, rho, macrhs=None, beta0=None, beta0_edge=None,
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+# ---- producers of the projection's inputs (mac_inputs_oracle.c)
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def cell_to_edge(s0_cell):
+    """cell_to_edge (Source/cell_to_edge.f90:14-68), one level, planar."""
+    c = np.ascontiguousarray(s0_cell, dtype=np.float64)
+    e = np.zeros(c.size + 1)
+    lib().orc_cell_to_edge(_p(c), c.size, _p(e))
+    return e
+
+
+def sphr_cell_value(s0, dr, center, prob_lo, dx, interp, i, j, k):
+    s = np.ascontiguousarray(s0, dtype=np.float64)
+    return lib().orc_sphr_cell_value(_p(s), s.size, dr, _d3(center), _d3(prob_lo), _d3(dx), interp, i, j, k)
+
+
+def make_macrhs(n, S, dg, Sbar, beta0, gamma1bar=None, p0=None, rho0=None, dp=None, dchi=None, dt=1.0,
+                dpdt_factor=0.0, base_cutoff_coord=0, base_cutoff_density=0.0, is_predictor=True,
+                sph=None, dx=None):
+    """make_macrhs (Source/make_macrhs.f90) on whole-domain arrays; sph =
+    (dr, center, prob_lo, interp) selects the spherical form (3D).  Returns
+    rhs (and updates dchi in place)."""
+    dim = len(n)
+    ni = (C.c_int * 3)(*(list(n) + [1] * (3 - dim)))
+    shp = tuple(reversed(n))
+    rhs = np.zeros(shp)
+    a = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+    S, dg, Sbar, beta0 = a(S), a(dg), a(Sbar), a(beta0)
+    gamma1bar, p0, rho0, dp = a(gamma1bar), a(p0), a(rho0), a(dp)
+    if dchi is not None:
+        assert dchi.dtype == np.float64 and dchi.flags.c_contiguous
+    dr, center, prob_lo, interp = sph if sph is not None else (0.0, (0, 0, 0), (0, 0, 0), 1)
+    dxa = dx if dx is not None else (1.0, 1.0, 1.0)
+    lib().orc_make_macrhs(dim, ni, _p(S), _p(dg), _p(Sbar), _p(beta0), _p(gamma1bar), _p(p0), _p(rho0),
+                          _p(dp), _p(dchi), dt, dpdt_factor, base_cutoff_coord, base_cutoff_density,
+                          int(is_predictor), dr, Sbar.size, _d3(center), _d3(prob_lo), _d3(dxa), interp,
+                          _p(rhs))
+    return rhs
+
+
+def make_s0mac(n, s0, dr, center, prob_lo, dx, interp):
+    """make_s0mac (type 1) on the grown face ranges: returns (fx, fy, fz) of
+    extents (nz+2, ny+2, nx+3), (nz+2, ny+3, nx+2), (nz+3, ny+2, nx+2);
+    face index -1 is at position 0."""
+    s = np.ascontiguousarray(s0, dtype=np.float64)
+    nx, ny, nz = n
+    f = [np.zeros((nz + 2 + (d == 2), ny + 2 + (d == 1), nx + 2 + (d == 0))) for d in range(3)]
+    lib().orc_make_s0mac((C.c_int * 3)(*n), _p(s), s.size, dr, _d3(center), _d3(prob_lo), _d3(dx), interp,
+                         _p(f[0]), _p(f[1]), _p(f[2]))
+    return f

@@ -348,3 +348,109 @@ def test_bf16_rounding_matches_torch():
     assert bf16_round(one + np.float32(2.0 ** -8)) == one                              # tie, even below
     assert bf16_round(one + np.float32(3 * 2.0 ** -8)) == one + np.float32(2.0 ** -6)  # tie, even above
     assert bf16_round(one + np.float32(2.0 ** -8 + 2.0 ** -20)) == one + np.float32(2.0 ** -7)
+
+
+# ---- producers of the projection's inputs (cell_to_edge, make_macrhs, make_s0mac)
+def test_cell_to_edge_known_answers(oracle_lib):
+    """Cell averages of x^3 (r^3 + r/4 over [r-1/2, r+1/2]) give the exact
+    edge values (r - 1/2)^3 at every interior edge (the fourth-order weights
+    7/12, -1/12); end edges copy / average by the boundary rules; a step is
+    limited to the range of its two cells (the unlimited formula gives 13/12)."""
+    O = oracle_lib
+    nr = 12
+    r = np.arange(nr, dtype=np.float64)
+    c = r ** 3 + r / 4
+    e = O.cell_to_edge(c)
+    for k in range(2, nr - 1):
+        assert abs(e[k] - (k - 0.5) ** 3) <= 1e-12 * max(1.0, (k - 0.5) ** 3), k
+    assert e[0] == c[0] and e[nr] == c[nr - 1]
+    assert e[1] == 0.5 * (c[0] + c[1]) and e[nr - 1] == 0.5 * (c[nr - 1] + c[nr - 2])
+    step = np.array([0, 0, 0, 1, 1, 1, 1, 1], dtype=np.float64)
+    es = O.cell_to_edge(step)
+    assert es[4] == 1.0 and 0.0 <= es.min() and es.max() <= 1.0
+    assert abs(7 / 12 * 2 - 1 / 12 - 13 / 12) < 1e-15   # the value the limiter removed
+
+
+def _sph_setup(n=16, drdxfac=4):
+    dx = 1.0 / n
+    dr = dx / drdxfac
+    nr = int(np.sqrt(3) * 0.5 / dr) + 1
+    return dx, dr, nr, (0.5, 0.5, 0.5), (0.0, 0.0, 0.0)
+
+
+def test_sphr_profiles_known_answers(oracle_lib):
+    """Radial profile on cell centres: type 1 takes the bin int(r/dr); types
+    2 and 3 reproduce a linear profile a + b r, type 3 a quadratic a + c r^2
+    (outside the first bin), to rounding."""
+    O = oracle_lib
+    dx, dr, nr, ctr, plo = _sph_setup()
+    rc = (np.arange(nr) + 0.5) * dr
+    lin = 2.0 - 0.7 * rc
+    quad = 1.0 + 3.0 * rc ** 2
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        i, j, k = (int(v) for v in rng.integers(0, 16, 3))
+        x = [(q + 0.5) * dx - 0.5 for q in (i, j, k)]
+        rad = float(np.sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]))
+        v1 = O.sphr_cell_value(lin, dr, ctr, plo, (dx,) * 3, 1, i, j, k)
+        assert v1 == lin[int(rad / dr)]
+        if rad >= rc[0]:
+            for t in (2, 3):
+                v = O.sphr_cell_value(lin, dr, ctr, plo, (dx,) * 3, t, i, j, k)
+                assert abs(v - (2.0 - 0.7 * rad)) <= 1e-12
+            v = O.sphr_cell_value(quad, dr, ctr, plo, (dx,) * 3, 3, i, j, k)
+            assert abs(v - (1.0 + 3.0 * rad * rad)) <= 1e-12
+
+
+def test_make_macrhs_known_answers(oracle_lib):
+    """S = Sbar gives beta0 * delta_gamma1_term exactly; the dp/dt term with
+    dpdt 1, dt 2, gamma1bar 1, p0 4, delta_p 8 adds exactly beta0 below the
+    cutoff only; a corrector accumulates on the predictor's delta_chi; the
+    spherical form takes the term with dpdt_factor = 0 as well."""
+    O = oracle_lib
+    n = (8, 6, 10)
+    rng = np.random.default_rng(5)
+    Sbar = rng.standard_normal(n[2])
+    beta0 = rng.uniform(0.5, 2.0, n[2])
+    S = np.broadcast_to(Sbar[:, None, None], (n[2], n[1], n[0])).copy()
+    dg = rng.standard_normal((n[2], n[1], n[0]))
+    rhs = O.make_macrhs(n, S, dg, Sbar, beta0)
+    assert np.array_equal(rhs, beta0[:, None, None] * dg)
+    ones = np.ones(n[2])
+    dp = np.full((n[2], n[1], n[0]), 8.0)
+    dchi = np.zeros_like(dp)
+    rhs2 = O.make_macrhs(n, S, np.zeros_like(dg), Sbar, beta0, gamma1bar=ones, p0=4 * ones, dp=dp, dchi=dchi,
+                         dt=2.0, dpdt_factor=1.0, base_cutoff_coord=4, is_predictor=True)
+    assert np.array_equal(rhs2[:4], np.broadcast_to(beta0[:4, None, None], rhs2[:4].shape))
+    assert np.all(rhs2[4:] == 0.0) and np.all(dchi[:4] == 1.0) and np.all(dchi[4:] == 0.0)
+    O.make_macrhs(n, S, np.zeros_like(dg), Sbar, beta0, gamma1bar=ones, p0=4 * ones, dp=dp, dchi=dchi, dt=2.0,
+                  dpdt_factor=1.0, base_cutoff_coord=4, is_predictor=False)
+    assert np.all(dchi[:4] == 2.0)
+    # spherical: dpdt_factor 0 still applies the (zero) increment and the beta0 delta_chi term
+    dx, dr, nr, ctr, plo = _sph_setup(n=8)
+    m = (8, 8, 8)
+    prof = np.ones(nr)
+    S3 = np.zeros(m[::-1])
+    dchi3 = np.full(m[::-1], 0.25)
+    r3 = O.make_macrhs(m, S3, np.zeros(m[::-1]), 0 * prof, 2 * prof, gamma1bar=prof, p0=prof, rho0=prof,
+                       dp=np.ones(m[::-1]), dchi=dchi3, dt=1.0, dpdt_factor=0.0, base_cutoff_density=0.5,
+                       is_predictor=False, sph=(dr, ctr, plo, 1), dx=(dx,) * 3)
+    assert np.all(r3 == 0.5) and np.all(dchi3 == 0.25)
+
+
+def test_make_s0mac_known_answers(oracle_lib):
+    """Faces average their two cells (a linear profile: the mean of the two
+    cells' a + b r); faces beyond the domain see the boundary cells."""
+    O = oracle_lib
+    dx, dr, nr, ctr, plo = _sph_setup(n=8)
+    n = (8, 8, 8)
+    rc = (np.arange(nr) + 0.5) * dr
+    lin = 2.0 - 0.7 * rc
+    f = O.make_s0mac(n, lin, dr, ctr, plo, (dx,) * 3, 2)
+    cell = lambda i, j, k: 2.0 - 0.7 * np.sqrt(sum(((q + 0.5) * dx - 0.5) ** 2 for q in (i, j, k)))
+    clamp = lambda q: min(max(q, 0), 7)
+    for (i, j, k) in [(0, 3, 4), (4, 4, 4), (8, 2, 5), (9, 0, 0), (-1, 7, 7)]:
+        exp = 0.5 * (cell(clamp(i), clamp(j), clamp(k)) + cell(clamp(i - 1), clamp(j), clamp(k)))
+        assert abs(f[0][k + 1, j + 1, i + 1] - exp) <= 1e-12
+    # faces below the domain in z take the bottom cells' value
+    assert np.array_equal(f[2][0], f[2][1])
