@@ -117,6 +117,52 @@ def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1, 
     return (t1 - t0), stats[warmup:], prof
 
 
+def time_producers(plan, n, box, torch, reps=10):
+    """Producers of the projection's inputs on the bench layout (device
+    pointers): make_macrhs (planar, dp/dt term on, corrector: reads S,
+    delta_gamma1_term, delta_p_term, delta_chi, writes macrhs and delta_chi,
+    48 B/cell) and make_s0mac (spherical geometry on the same grid, quadratic
+    profile, three face arrays with one ghost layer written, 24 B/face).
+    Wall time per call (each call returns after its kernel), median of reps."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    boxes = W.box_list(n, box)
+    ncell = n[0] * n[1] * n[2]
+    cells = lambda ng: [torch.rand(BX.box_shape(lo, hi, ng, 3), dtype=torch.float64, device="cuda")
+                        for lo, hi in boxes]
+    rhs, S, dg, dp, dchi = cells(0), cells(0), cells(0), cells(0), cells(0)
+    one = lambda v: torch.full((n[2],), v, dtype=torch.float64, device="cuda")
+    out = {}
+    ts = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.make_macrhs(rhs, S, dg, one(0.1), one(1.0), delta_p_term=dp, delta_chi=dchi, gamma1bar=one(1.4),
+                         p0=one(2.0), dt=0.01, dpdt_factor=0.5, base_cutoff_coord=n[2] // 2, is_predictor=False)
+        ts.append(time.perf_counter() - t)
+    tm = float(np.median(ts[1:]))
+    out["make_macrhs"] = {"us": round(1e6 * tm, 1), "bytes": 48 * ncell,
+                          "GB_per_s": round(48 * ncell / tm / 1e9, 1)}
+    dx = 1.0 / n[0]
+    dr = dx / 5.0
+    nr = int(np.sqrt(3) * 0.5 / dr) + 1
+    sph = mac.sphr(dr, nr, (0.5, 0.5, 0.5), (0.0, 0.0, 0.0), 3)
+    s0 = torch.linspace(1.0, 1e-3, nr, dtype=torch.float64, device="cuda")
+    faces = [[torch.empty(BX.box_shape(lo, hi, 1, 3, face_dir=d), dtype=torch.float64, device="cuda")
+              for d in range(3)] for lo, hi in boxes]
+    nface = sum(f.numel() for bx in faces for f in bx)
+    ts = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.make_s0mac(s0, sph, faces)
+        ts.append(time.perf_counter() - t)
+    tm = float(np.median(ts[1:]))
+    out["make_s0mac"] = {"us": round(1e6 * tm, 1), "bytes": 8 * nface, "GB_per_s": round(8 * nface / tm / 1e9, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +174,8 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
     ap.add_argument("--no-profile", action="store_true",
                     help="time the steps without the per-kernel HIP events (no roofline figures)")
+    ap.add_argument("--no-producers", action="store_true",
+                    help="skip timing make_macrhs / make_s0mac on the bench layout")
     ap.add_argument("--headline", default="mixed_bf16", choices=("mixed", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=1,
@@ -205,6 +253,8 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                 "kernel": name + " (finest level, one full red-black sweep)",
                 "avg_launch_us": round(1e3 * avg, 2), "launches": cnt_, "bytes_per_launch": nbytes}
+
+    producers = None if args.no_producers else time_producers(plan, n, args.box, torch)
 
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
@@ -295,6 +345,7 @@ def main():
         "kernel_ms_per_solve": {
             p: {k: round(v[1], 3) for k, v in results[p]["prof"].items()} for p in results},
         "accuracy": acc,
+        "producers": producers,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

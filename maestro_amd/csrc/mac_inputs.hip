@@ -8,6 +8,8 @@
 // caller's ghost widths), with the reference's expression order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mac_launch.h"
 
 namespace mac {
@@ -172,10 +174,21 @@ void launch_make_macrhs(const MacrhsArgs &A, int maxcells, bool sph, hipStream_t
 }
 
 // ------------------------------------------------------------ make_s0mac
-// one thread per element of box blockIdx.y's face array of direction
-// blockIdx.z (one ghost layer: every face of the grown box); the two cells
-// of a face are clamped into the domain (first-order extrapolation of the
-// cell values outside it) and averaged
+// (1) the profile on every cell of the domain (whole-domain scratch, C order
+// with x fastest), one interpolation per cell
+__global__ void __launch_bounds__(ITPB) k_s0_cells(S0macArgs A)
+{
+    const long nc = (long)A.n[0] * A.n[1] * A.n[2];
+    for (long q = (long)blockIdx.x * ITPB + threadIdx.x; q < nc; q += (long)gridDim.x * ITPB) {
+        const int i = (int)(q % A.n[0]), j = (int)((q / A.n[0]) % A.n[1]), k = (int)(q / ((long)A.n[0] * A.n[1]));
+        A.cells[q] = sphr_cell_value(A.s0, A.geo, i, j, k);
+    }
+}
+
+// (2) one thread per element of box blockIdx.y's face array of direction
+// blockIdx.z (one ghost layer: every face of the grown box): the mean of the
+// face's two cells, clamped into the domain (first-order extrapolation of
+// the cell values outside it)
 __global__ void __launch_bounds__(ITPB) k_make_s0mac(S0macArgs A)
 {
     const int b = blockIdx.y, d = blockIdx.z;
@@ -193,13 +206,16 @@ __global__ void __launch_bounds__(ITPB) k_make_s0mac(S0macArgs A)
         c[d] -= s;   // s = 0: the face's high cell, 1: its low cell
 #pragma unroll
         for (int e = 0; e < 3; ++e) c[e] = c[e] < 0 ? 0 : (c[e] > A.n[e] - 1 ? A.n[e] - 1 : c[e]);
-        v[s] = sphr_cell_value(A.s0, A.geo, c[0], c[1], c[2]);
+        v[s] = A.cells[c[0] + (long)A.n[0] * (c[1] + (long)A.n[1] * c[2])];
     }
     A.out[b * 3 + d][q] = 0.5 * (v[0] + v[1]);
 }
 
 void launch_make_s0mac(const S0macArgs &A, int maxfaces, hipStream_t s)
 {
+    const long nc = (long)A.n[0] * A.n[1] * A.n[2];
+    hipLaunchKernelGGL(k_s0_cells, dim3((unsigned)std::min<long>((nc + ITPB - 1) / ITPB, 65536)), dim3(ITPB), 0, s,
+                       A);
     dim3 g((unsigned)((maxfaces + ITPB - 1) / ITPB), A.nboxes, 3);
     hipLaunchKernelGGL(k_make_s0mac, g, dim3(ITPB), 0, s, A);
 }

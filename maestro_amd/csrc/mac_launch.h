@@ -115,6 +115,7 @@ struct S0macArgs {
     const BoxDev *boxes;
     int dlo[3], n[3];
     const double *s0;   // radial profile, device
+    double *cells;      // whole-domain scratch: the profile on the cells
     double *const *out; // [box*3 + d]
     SphrGeom geo;
 };

@@ -93,7 +93,9 @@ struct mac_plan {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8];
     HostStage hs;
-    double *scratch_cells = nullptr;   // one FP64 cell array (adjoint)
+    double *scratch_cells = nullptr;   // one FP64 cell array (adjoint, make_s0mac)
+    void *call_tab = nullptr;          // device pointer tables of the producer calls
+    size_t call_tab_cap = 0;
     int device = 0;
     // kernel timing (mac_plan_profile)
     bool prof_on = false;
@@ -822,6 +824,7 @@ extern "C" int mac_plan_destroy(mac_plan *P)
     if (P->b0_dev) (void)hipFree(P->b0_dev);
     if (P->b0e_dev) (void)hipFree(P->b0e_dev);
     if (P->scratch_cells) (void)hipFree(P->scratch_cells);
+    if (P->call_tab) (void)hipFree(P->call_tab);
     for (int e = 0; e < 8; ++e)
         if (P->ev[e]) (void)hipEventDestroy(P->ev[e]);
     if (P->stream) (void)hipStreamDestroy(P->stream);
@@ -1411,7 +1414,21 @@ namespace {
 struct CallStage {
     mac_plan *P;
     std::vector<void *> mem;
+    size_t tab_used = 0;   // pointer slots of P->call_tab handed out
     explicit CallStage(mac_plan *p) : P(p) {}
+    // room for `slots` device pointers in the plan's table buffer (grown
+    // before any slot is handed out; calls on one plan run in stream order)
+    int reserve(size_t slots)
+    {
+        if (slots * sizeof(double *) <= P->call_tab_cap) return MAC_OK;
+        HIPCHK(hipStreamSynchronize(P->stream));
+        if (P->call_tab) (void)hipFree(P->call_tab);
+        P->call_tab = nullptr;
+        P->call_tab_cap = 0;
+        HIPCHK(hipMalloc(&P->call_tab, slots * sizeof(double *)));
+        P->call_tab_cap = slots * sizeof(double *);
+        return MAC_OK;
+    }
     ~CallStage()
     {
         if (P->stream) (void)hipStreamSynchronize(P->stream);
@@ -1444,11 +1461,11 @@ struct CallStage {
         int rc;
         for (size_t a = 0; a < n.size(); ++a)
             if ((rc = view(h[a], n[a], in, &views[a]))) return rc;
-        void *q = nullptr;
-        HIPCHK(hipMalloc(&q, sizeof(double *) * std::max<size_t>(n.size(), 1)));
-        mem.push_back(q);
+        if ((tab_used + n.size()) * sizeof(double *) > P->call_tab_cap) return MAC_ERR_INVALID;
+        double **q = (double **)P->call_tab + tab_used;
+        tab_used += n.size();
         HIPCHK(hipMemcpyAsync(q, views.data(), sizeof(double *) * n.size(), hipMemcpyHostToDevice, P->stream));
-        *tab = (double **)q;
+        *tab = q;
         return MAC_OK;
     }
 };
@@ -1531,6 +1548,7 @@ extern "C" int mac_make_macrhs(mac_plan *P, double *const *macrhs, const mac_mac
     }
     std::vector<double *> v_rhs, v_S, v_dg, v_dp, v_dchi;
     double **t_rhs, **t_S, **t_dg, **t_dp = nullptr, **t_dchi = nullptr;
+    if ((rc = S.reserve(5 * (size_t)nb))) return rc;
     // macrhs is staged in as well: its ghost cells are left as the caller had them
     if ((rc = S.table(macrhs, n_rhs, true, v_rhs, &t_rhs)) || (rc = S.table(in->S_cc, n_S, true, v_S, &t_S)) ||
         (rc = S.table(in->delta_gamma1_term, n_dg, true, v_dg, &t_dg)))
@@ -1590,7 +1608,9 @@ extern "C" int mac_make_s0mac(mac_plan *P, const double *s0, const mac_sphr *sph
     std::vector<double *> v_f;
     double **t_f;
     double *d_s0;
+    if ((rc = S.reserve(3 * (size_t)nb))) return rc;
     if ((rc = S.table(s0mac, n_f, false, v_f, &t_f)) || (rc = S.view(s0, sph->nr, true, &d_s0))) return rc;
+    if (!P->scratch_cells) HIPCHK(hipMalloc(&P->scratch_cells, sizeof(double) * (size_t)ncell(P->n)));
     S0macArgs A;
     A.nboxes = nb;
     A.boxes = P->boxes_dev;
@@ -1599,6 +1619,7 @@ extern "C" int mac_make_s0mac(mac_plan *P, const double *s0, const mac_sphr *sph
         A.n[d] = P->n[d];
     }
     A.s0 = d_s0;
+    A.cells = P->scratch_cells;
     A.out = t_f;
     A.geo = sphr_geom(P, sph);
     launch_make_s0mac(A, maxfaces, P->stream);

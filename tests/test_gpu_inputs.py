@@ -166,3 +166,71 @@ def test_radial_grid_too_short_is_rejected(gpu):
     with pytest.raises(mac.MacError):
         plan.make_s0mac(np.ones(nr // 2), mac.sphr(dr, nr // 2, ctr, plo, 1), out)
     plan.close()
+
+
+def _project_boxes(plan, wl, macrhs_boxes=None, cart_boxes=None):
+    boxes, umac, phi, rho, _, _ = BX.split_workload(wl)
+    st = plan.project(umac, phi, rho, macrhs_boxes,
+                      None if wl["beta0"] is None else np.ascontiguousarray(wl["beta0"]),
+                      None if wl["beta0_edge"] is None else np.ascontiguousarray(wl["beta0_edge"]), cart_boxes)
+    n = wl["n"]
+    faces = [BX.gather_faces([u[d] for u in umac], boxes, d, 1, n, 3) for d in range(3)]
+    return st, faces, BX.gather_cells(phi, boxes, 1, n, 3)
+
+
+def test_s0mac_feeds_macproject(gpu, oracle_lib):
+    """The spherical pipeline: make_s0mac's faces used as beta0_cart_edge by
+    macproject on the GPU, against the oracle's make_s0mac faces through the
+    oracle's macproject (phi and U_proj within 1e-8)."""
+    from .helpers import layout_for, rel_err, run_oracle
+    O = oracle_lib
+    n, box = (32, 32, 32), 16
+    wl = W.make_workload(3, n=n, box=box)
+    dx, dr, nr, ctr, plo = _sph(n)
+    rc = (np.arange(nr) + 0.5) * dr
+    s0 = np.maximum(1.0 - (rc / 0.45) ** 2, 1e-4) ** 1.5
+    f = O.make_s0mac(n, s0, dr, ctr, plo, (dx,) * 3, 3)
+    wl["cart"] = [np.ascontiguousarray(f[0][1:-1, 1:-1, 1:-1]), np.ascontiguousarray(f[1][1:-1, 1:-1, 1:-1]),
+                  np.ascontiguousarray(f[2][1:-1, 1:-1, 1:-1])]
+    truth = run_oracle(wl, eps=1e-13, alt=0)
+    plan = mac.MacPlan(layout_for(wl), mac.default_opts(use_alt_energy_fix=0))
+    boxes = W.box_list(n, box)
+    cart = [[np.zeros(BX.box_shape(lo, hi, 1, 3, face_dir=d)) for d in range(3)] for lo, hi in boxes]
+    plan.make_s0mac(s0, mac.sphr(dr, nr, ctr, plo, 3), cart)
+    st, faces, phi = _project_boxes(plan, wl, cart_boxes=cart)
+    assert st["converged"]
+    e_phi = rel_err(phi, truth["phi"])
+    e_u = max(rel_err(faces[d], truth["faces"][d]) for d in range(3))
+    print(f"  s0mac -> macproject: phi {e_phi:.2e} U {e_u:.2e}")
+    assert e_phi <= 1e-8 and e_u <= 1e-8
+    plan.close()
+
+
+def test_macrhs_feeds_macproject(gpu, oracle_lib):
+    """The planar pipeline: make_macrhs's output as the macrhs of macproject
+    on the GPU, against the oracle's make_macrhs through the oracle's
+    macproject."""
+    from .helpers import layout_for, rel_err, run_oracle
+    O = oracle_lib
+    n, box = (32, 32, 32), 16
+    wl = W.make_workload(1, n=n, box=box)
+    rng = np.random.default_rng(8)
+    S = rng.standard_normal((n[2], n[1], n[0]))
+    dg = 0.1 * rng.standard_normal((n[2], n[1], n[0]))
+    Sbar = S.mean(axis=(1, 2))
+    beta0 = np.ascontiguousarray(wl["beta0"])
+    macrhs = O.make_macrhs(n, S, dg, Sbar, beta0)
+    truth = run_oracle(wl, eps=1e-13, macrhs=macrhs)
+    plan = mac.MacPlan(layout_for(wl, ng_macrhs=0), mac.default_opts())
+    boxes = W.box_list(n, box)
+    bc = wl["bc"]
+    rhs_b = [np.zeros(BX.box_shape(lo, hi, 0, 3)) for lo, hi in boxes]
+    plan.make_macrhs(rhs_b, per_box(S, boxes, 0, 3, bc), per_box(dg, boxes, 0, 3, bc), np.ascontiguousarray(Sbar),
+                     beta0)
+    st, faces, phi = _project_boxes(plan, wl, macrhs_boxes=rhs_b)
+    assert st["converged"]
+    e_phi = rel_err(phi, truth["phi"], remove_mean=True)
+    e_u = max(rel_err(faces[d], truth["faces"][d]) for d in range(3))
+    print(f"  make_macrhs -> macproject: phi {e_phi:.2e} U {e_u:.2e}")
+    assert e_phi <= 1e-8 and e_u <= 1e-8
+    plan.close()
