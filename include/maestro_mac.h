@@ -224,6 +224,58 @@ int mac_make_macrhs(mac_plan *plan, double *const *macrhs, const mac_macrhs_in *
  * must be 1) the mean of its two cells.  3D. */
 int mac_make_s0mac(mac_plan *plan, const double *s0, const mac_sphr *sph, double *const *s0mac);
 
+/* ---- solver-level entry point ---------------------------------------------
+ * mac_multigrid(mla, rh, phi, fine_flx, alpha, beta, dx, the_bc_tower,
+ *               bc_comp, stencil_order, rel_solver_eps, abs_solver_eps)
+ *   Source/mac_multigrid.f90:18-71
+ * solves (alpha - D beta G) phi = rh with caller-built coefficients: alpha
+ * = 0 from macproject, alpha = rho and beta = (dt/2) k/c_p on faces from the
+ * thermal diffusion solve (Source/thermal_conduct.f90:316-377, bc_comp =
+ * dm + rhoh_comp).  Same V-cycle, bottom solver and precision options as
+ * mac_project; the solves with alpha run the generic (per-colour) kernels.
+ * Single level: fine_flx is not produced (the single-level callers discard
+ * it, thermal_conduct.f90:366-368). */
+
+/* elliptic boundary kinds (FBoxLib bc_module's BC_PER, BC_DIR, BC_NEU) */
+#define MAC_ELL_PER (-1)
+#define MAC_ELL_DIR 1
+#define MAC_ELL_NEU 2
+
+/* component classes of ell_bc_level_build (Source/define_bc_tower.f90:296-402) */
+#define MAC_COMP_PRESS 0    /* press_comp: OUTLET Dirichlet                    */
+#define MAC_COMP_SCALAR 1   /* rho, rho h, rho X, temp, pi, tracers: INLET
+                               Dirichlet                                       */
+#define MAC_COMP_EXTRAP 2   /* foextrap_comp, hoextrap_comp: Neumann           */
+
+/* The elliptic kind of a physical boundary code for a component class:
+ * MAC_ELL_PER for MAC_PERIODIC, else MAC_ELL_DIR or MAC_ELL_NEU as
+ * ell_bc_level_build assigns them; 0 for a code it does not list. */
+int mac_ell_bc(int phys_bc, int comp);
+
+typedef struct mac_solve_in {
+    const double *const *rh;     /* [nboxes] right side                          */
+    int ng_rh;
+    double *const *phi;          /* [nboxes] in: warm start (valid cells) and the
+                                    Dirichlet values (the ghost cells beyond each
+                                    MAC_ELL_DIR side hold the value on that face;
+                                    ng_phi >= 1 then); out: the solution on the
+                                    valid cells (ghost cells untouched)           */
+    int ng_phi;
+    const double *const *alpha;  /* [nboxes] cell coefficient; NULL: alpha = 0   */
+    int ng_alpha;
+    const double *const *beta;   /* [nboxes*dim] face coefficients, box-major    */
+    int ng_beta;
+    int ell_bc[3][2];            /* MAC_ELL_* per direction and side; periodic
+                                    exactly where the layout is periodic          */
+    double eps_rel;              /* stop at max|r| <= eps_rel * max|r0| ...       */
+    double eps_abs;              /* ... or max|r| <= eps_abs (<= 0: off)          */
+} mac_solve_in;
+
+/* Solve on the plan's layout.  The problem is singular (the mean of the
+ * right side is removed, phi defined up to a constant) when no side is
+ * Dirichlet and alpha is NULL or zero everywhere. */
+int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
+
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Read or overwrite an internal FP64 array of MG level `level` (0 = finest)
  * through host memory, and run single solver operators on it, so tests can

@@ -34,6 +34,10 @@ struct Lev {
     T *phi, *phi2, *rhs, *res;
     T *bx, *by, *bz;
     T *bxh, *byh, *bzh;
+    // cell coefficient of the Helmholtz operator (alpha - D beta G); null for
+    // the projection (alpha = 0).  Only the generic kernels (per-colour
+    // sweep, residual, coarsening, coarse cycle) read it.
+    T *alpha;
 };
 
 struct BoxDev {
@@ -115,9 +119,11 @@ __device__ __forceinline__ void nbr_values(const Lev<S> &L, const T *__restrict_
     }
 }
 
-// Gauss-Seidel point value (fixed evaluation order, shared with the oracle)
+// Gauss-Seidel point value (fixed evaluation order, shared with the oracle);
+// a = the cell's alpha, added to the diagonal last.  The default -0.0 adds nothing (x + -0.0 == x for every x), so the
+// compiler drops it from the kernels without alpha.
 template <int DIM, typename T>
-__device__ __forceinline__ T gs_value(const Lev<T> &L, const T w[6], const T p[6], T rhs)
+__device__ __forceinline__ T gs_value(const Lev<T> &L, const T w[6], const T p[6], T rhs, T a = T(-0.0))
 {
     T tx = dfma(w[0], p[0], w[1] * p[1]), sx = w[0] + w[1];
     T ty = dfma(w[2], p[2], w[3] * p[3]), sy = w[2] + w[3];
@@ -130,7 +136,19 @@ __device__ __forceinline__ T gs_value(const Lev<T> &L, const T w[6], const T p[6
         num = dfma(L.ih2x, tx, dfma(L.ih2y, ty, rhs));
         den = dfma(L.ih2x, sx, L.ih2y * sy);
     }
-    return num / den;
+    return num / (den + a);
+}
+// alpha of cell c, or -0.0 (no contribution) on a level without alpha
+template <typename T>
+__device__ __forceinline__ T alpha_at(const Lev<T> &L, long c)
+{
+    return L.alpha ? L.alpha[c] : T(-0.0);
+}
+// A phi at cell c from its stencil part ap: + alpha phi as one fma
+template <typename T>
+__device__ __forceinline__ T with_alpha(const Lev<T> &L, long c, T ap, T pc)
+{
+    return L.alpha ? dfma(L.alpha[c], pc, ap) : ap;
 }
 
 // A phi at a cell (fixed evaluation order, shared with the oracle)

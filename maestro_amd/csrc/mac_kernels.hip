@@ -233,6 +233,15 @@ __global__ void __launch_bounds__(TPB) k_coarsen(Lev<T> F, Lev<T> C)
     int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), K = (int)(q / ((long)C.nx * C.ny));
     int i = 2 * I, j = 2 * J, k = (DIM == 3) ? 2 * K : 0;
     auto fc = [&](int ii, int jj, int kk) { return ii + (long)F.nx * (jj + (long)F.ny * kk); };
+    if (C.alpha) {   // cell coefficient: mean of the 2^DIM fine cells (restriction's sum order)
+        const T *a = F.alpha;
+        T v = (a[fc(i, j, k)] + a[fc(i + 1, j, k)]) + (a[fc(i, j + 1, k)] + a[fc(i + 1, j + 1, k)]);
+        if (DIM == 3)
+            C.alpha[q] = T(0.125) * (v + ((a[fc(i, j, k + 1)] + a[fc(i + 1, j, k + 1)]) +
+                                          (a[fc(i, j + 1, k + 1)] + a[fc(i + 1, j + 1, k + 1)])));
+        else
+            C.alpha[q] = T(0.25) * v;
+    }
     if (DIM == 3) {
         const T q4 = T(0.25);
         C.bx[q] = q4 * ((F.bx[fc(i, j, k)] + F.bx[fc(i, j + 1, k)]) +
@@ -271,6 +280,7 @@ __global__ void __launch_bounds__(TPB) k_weights_to_f32(Lev<double> S, Lev<float
     if (q >= n) return;
     D.bx[q] = (float)S.bx[q];
     D.by[q] = (float)S.by[q];
+    if (D.alpha) D.alpha[q] = (float)S.alpha[q];
     if (DIM == 3) D.bz[q] = (float)S.bz[q];
     int I = (int)(q % S.nx), J = (int)((q / S.nx) % S.ny), K = (int)(q / ((long)S.nx * S.ny));
     if (I == 0) D.bxh[J + (long)S.ny * K] = (float)S.bxh[J + (long)S.ny * K];
@@ -379,7 +389,7 @@ __global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
     long c = i + (long)L.nx * (j + (long)L.ny * k);
     T w[6], p[6];
     stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
-    L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
+    L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c], alpha_at(L, c));
 }
 
 // ------------------------------------------- fused red-black sweep (3D)
@@ -1348,7 +1358,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Lev<T> F, const T *__res
             const long f = i + (long)F.nx * (j + (long)F.ny * k);
             T w[6], p[6];
             stencil<DIM, T>(F, phi, i, j, k, f, w, p);
-            r[a] = F.rhs[f] - apply_value<DIM, T>(F, w, p, phi[f]);
+            r[a] = F.rhs[f] - with_alpha(F, f, apply_value<DIM, T>(F, w, p, phi[f]), phi[f]);
         }
         pair = r[0] + r[1];
     }
@@ -1418,7 +1428,7 @@ __global__ void __launch_bounds__(TPB) k_residual(Lev<T> L, T *out, unsigned lon
     for_rows(L.nx, L.ny, L.nz, [&](int i, int j, int k, long q) {
         T w[6], p[6];
         stencil<DIM, T>(L, L.phi, i, j, k, q, w, p);
-        T r = L.rhs[q] - apply_value<DIM, T>(L, w, p, L.phi[q]);
+        T r = L.rhs[q] - with_alpha(L, q, apply_value<DIM, T>(L, w, p, L.phi[q]), L.phi[q]);
         if (out) out[q] = r;
         a = fmax(a, fabs((double)r));
     });
@@ -1435,7 +1445,7 @@ __global__ void __launch_bounds__(TPB) k_ir_residual(Lev<double> L, float *r32,
     for_rows(L.nx, L.ny, L.nz, [&](int i, int j, int k, long q) {
         double w[6], p[6];
         stencil<DIM, double>(L, L.phi, i, j, k, q, w, p);
-        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, L.phi[q]);
+        double r = L.rhs[q] - with_alpha(L, q, apply_value<DIM, double>(L, w, p, L.phi[q]), L.phi[q]);
         r32[q] = (float)r;
         a = fmax(a, fabs(r));
     });
@@ -1461,7 +1471,7 @@ __global__ void __launch_bounds__(TPB) k_ir_update_residual(Lev<double> L, const
 #pragma unroll
         for (int s = 0; s < 6; ++s) p[s] = p[s] + (double)pe[s];
         const double pc = L.phi[q] + (double)e[q];
-        double r = L.rhs[q] - apply_value<DIM, double>(L, w, p, pc);
+        double r = L.rhs[q] - with_alpha(L, q, apply_value<DIM, double>(L, w, p, pc), pc);
         phi_out[q] = pc;
         r32[q] = (float)r;
         a = fmax(a, fabs(r));
@@ -1974,7 +1984,7 @@ __device__ void blk_gsrb(const Lev<T> &L)
             const int c = i + L.nx * (j + L.ny * k);
             T w[6], p[6];
             stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
-            L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c]);
+            L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c], alpha_at(L, c));
         }
         __syncthreads();
     }
@@ -2017,7 +2027,8 @@ __device__ void blk_apply64(const Lev<T> &L, const double *x, double *out)
         } else {
             w[4] = w[5] = p[4] = p[5] = 0.0;
         }
-        out[q] = apply_value<DIM, double>(D, w, p, x[q]);
+        const double ap = apply_value<DIM, double>(D, w, p, x[q]);
+        out[q] = L.alpha ? fma((double)L.alpha[q], x[q], ap) : ap;
     }
     __syncthreads();
 }
@@ -2142,6 +2153,7 @@ __device__ void wave_bottom(const Lev<T> &L, const CoarseOpts &o)
     const int colour = (i + j + k) & 1;
     T phi = live ? L.phi[qc] : T(0);
     const T rhs = live ? L.rhs[qc] : T(0);
+    const T acell = alpha_at(L, qc);
     auto gather = [&](auto v, auto *p) {   // neighbour values of v (all lanes shuffle)
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
@@ -2154,7 +2166,7 @@ __device__ void wave_bottom(const Lev<T> &L, const CoarseOpts &o)
             for (int c = 0; c < 2; ++c) {
                 T p[6];
                 gather(phi, p);
-                const T v = gs_value<DIM, T>(L, w, p, rhs);
+                const T v = gs_value<DIM, T>(L, w, p, rhs, acell);
                 if (live && colour == c) phi = v;
             }
     };
@@ -2170,7 +2182,9 @@ __device__ void wave_bottom(const Lev<T> &L, const CoarseOpts &o)
     auto apply = [&](double x) -> double {
         double p[6];
         gather(x, p);
-        return live ? apply_value<DIM, double>(D, wd, p, x) : 0.0;
+        if (!live) return 0.0;
+        const double ap = apply_value<DIM, double>(D, wd, p, x);
+        return L.alpha ? fma((double)acell, x, ap) : ap;
     };
     const double nd = (double)n;
     const double m = o.singular ? wave_sum(live ? (double)rhs : 0.0) / nd : 0.0;
@@ -2223,7 +2237,7 @@ __device__ void blk_resid_restrict(const Lev<T> &F, const Lev<T> &C)
                     const int f = i + F.nx * (j + F.ny * k);
                     T w[6], p[6];
                     stencil<DIM, T>(F, F.phi, i, j, k, f, w, p);
-                    rv[c][b][a] = F.rhs[f] - apply_value<DIM, T>(F, w, p, F.phi[f]);
+                    rv[c][b][a] = F.rhs[f] - with_alpha(F, f, apply_value<DIM, T>(F, w, p, F.phi[f]), F.phi[f]);
                 }
         T v;
         if (DIM == 3) {
@@ -3056,6 +3070,7 @@ template <typename T>
 bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
                        const EllBC &bc, hipStream_t s, const Compact16 *K)
 {
+    if (L.alpha) return false;   // the fused sweeps have no alpha term
     const int variant = sweep_variant();
     if constexpr (sizeof(T) == 4) {
         if (K) {
@@ -3124,7 +3139,8 @@ template <typename T>
 bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
                            const Compact16 *K)
 {
-    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
+    if (F.alpha && K) return false;
+    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1 && !F.alpha) {
         RowMarchGrid G;
         G.W = F.nx / 128;
         // coarse planes per chunk: at most 16, fewer on small levels so that
@@ -3210,7 +3226,7 @@ template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
 {
     if constexpr (sizeof(T) == 8) {
-        if (dim == 3 && out == nullptr && nrm != nullptr &&
+        if (dim == 3 && out == nullptr && nrm != nullptr && !L.alpha &&
             launch_resid_rows<0>(L, nullptr, nullptr, nullptr, nrm, s))
             return;
     }
@@ -3222,7 +3238,7 @@ void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s, r16)) return;
+    if (dim == 3 && !L.alpha && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
@@ -3230,7 +3246,7 @@ void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
                                unsigned long long *nrm, hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s, r16)) return;
+    if (dim == 3 && !L.alpha && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_update_residual<D_>, dim3(g), dim3(TPB), 0, s, L, e, phi_out,
                                          r32, nrm));

@@ -120,6 +120,28 @@ struct S0macArgs {
     SphrGeom geo;
 };
 
+// mac_solve's setup over the caller's boxes (see k_solve_setup)
+struct SolveArgs {
+    int dim, nboxes;
+    const BoxDev *boxes;
+    int dlo[3], n[3];
+    EllBC bc;
+    const double *const *rh;
+    int ng_rh;
+    const double *const *phi;
+    int ng_phi;
+    const double *const *alpha;   // null: alpha = 0
+    int ng_alpha;
+    const double *const *beta;    // [box*dim + d]
+    int ng_beta;
+    Lev<double> L;                // level 0 (L.alpha set when alpha is given)
+    double *part;
+    int *err;
+    unsigned long long *amax;     // max |alpha| (bit pattern; atomic max)
+};
+
+void launch_solve_setup(const SolveArgs &A, int maxcells, hipStream_t s);
+
 void launch_cell_to_edge(const double *c, int nr, double *e, hipStream_t s);
 void launch_make_macrhs(const MacrhsArgs &A, int maxcells, bool sph, hipStream_t s);
 void launch_make_s0mac(const S0macArgs &A, int maxfaces, hipStream_t s);

@@ -40,6 +40,8 @@ struct Hier {
     Lev<T> *lev_dev = nullptr;
     std::vector<void *> allocs;
     bool built = false;
+    std::vector<T *> alpha;   // per-level alpha storage (mac_solve), made on first use
+    bool alpha_on = false;    // whether lev[].alpha (and lev_dev) point at it
 };
 
 struct HostStage {
@@ -58,6 +60,7 @@ struct mac_plan {
     EllBC bc;
     int phys[3][2];
     bool singular = true;
+    bool helm = false;            // inside mac_solve: alpha and the call's BCs active
     int ng_u = 0, ng_phi = 0, ng_rho = 0, ng_rhs = 0, ng_cart = 0;
     mac_opts opts;
     std::vector<BoxDev> boxes;
@@ -263,7 +266,31 @@ static void free_hier(Hier<T> &H)
     for (void *p : H.allocs) (void)hipFree(p);
     H.allocs.clear();
     H.lev.clear();
+    H.alpha.clear();
+    H.alpha_on = false;
     H.built = false;
+}
+
+// point the levels' alpha at the hierarchy's alpha storage (on) or at
+// nothing (off), allocating the storage on first use; the device copy of
+// the level table follows
+template <typename T>
+static int set_alpha(mac_plan *P, Hier<T> &H, bool on)
+{
+    if (!H.built || on == H.alpha_on) return MAC_OK;
+    if (on && H.alpha.empty()) {
+        for (int l = 0; l < H.nlev; ++l) {
+            void *q = nullptr;
+            const Lev<T> &L = H.lev[l];
+            HIPCHK(hipMalloc(&q, sizeof(T) * (size_t)L.nx * L.ny * L.nz));
+            H.allocs.push_back(q);
+            H.alpha.push_back((T *)q);
+        }
+    }
+    for (int l = 0; l < H.nlev; ++l) H.lev[l].alpha = on ? H.alpha[l] : nullptr;
+    HIPCHK(hipMemcpyAsync(H.lev_dev, H.lev.data(), sizeof(Lev<T>) * H.nlev, hipMemcpyHostToDevice, P->stream));
+    H.alpha_on = on;
+    return MAC_OK;
 }
 
 template <typename T>
@@ -288,7 +315,7 @@ static bool use_fused(const mac_plan *P)
 {
     static int off = -1;
     if (off < 0) off = getenv("MAC_NO_FUSED") ? 1 : 0;
-    return P->dim == 3 && !off;
+    return P->dim == 3 && !off && !P->helm;   // the fused sweeps have no alpha term
 }
 
 // nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
@@ -546,6 +573,8 @@ static int solve(mac_plan *P, mac_stats *st)
     HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
     if (o.precision == MAC_PREC_MIXED || o.precision == MAC_PREC_MIXED_BF16) {
         if ((rc = build_hier<float>(P, P->H32))) return rc;
+        // alpha in FP32 (from the FP64 level 0 and by coarsening) where the FP64 level has it
+        if ((rc = set_alpha<float>(P, P->H32, P->helm && P->H64.alpha_on))) return rc;
         const Compact16 *K = nullptr;
         if (compact_applies(P) && P->H32.nlev > 1) {
             // compact level 0 and FP32 level 1 in one pass, then the rest
@@ -1628,4 +1657,166 @@ extern "C" int mac_make_s0mac(mac_plan *P, const double *s0, const mac_sphr *sph
         if ((rc = S.back(s0mac[a], v_f[a], n_f[a]))) return rc;
     HIPCHK(hipStreamSynchronize(P->stream));
     return MAC_OK;
+}
+
+// ------------------------------------------- solver-level entry: mac_solve
+extern "C" int mac_ell_bc(int phys, int comp)
+{
+    // Source/define_bc_tower.f90:329-399
+    if (phys == MAC_PERIODIC) return MAC_ELL_PER;
+    switch (phys) {
+    case MAC_SLIP_WALL:
+    case MAC_NO_SLIP_WALL:
+    case MAC_SYMMETRY:
+        return MAC_ELL_NEU;
+    case MAC_INLET:
+        return comp == MAC_COMP_SCALAR ? MAC_ELL_DIR : MAC_ELL_NEU;
+    case MAC_OUTLET:
+        return comp == MAC_COMP_PRESS ? MAC_ELL_DIR : MAC_ELL_NEU;
+    default:
+        return 0;
+    }
+}
+
+namespace {
+
+// the call's BCs, tolerances and alpha for the duration of one mac_solve;
+// the plan's projection state comes back on every exit
+struct HelmScope {
+    mac_plan *P;
+    EllBC bc;
+    bool singular;
+    double eps_rel, eps_abs;
+    HelmScope(mac_plan *p, const EllBC &call_bc, double er, double ea)
+        : P(p), bc(p->bc), singular(p->singular), eps_rel(p->opts.eps_rel), eps_abs(p->opts.eps_abs)
+    {
+        P->bc = call_bc;
+        P->opts.eps_rel = er;
+        P->opts.eps_abs = ea;
+        P->helm = true;
+    }
+    ~HelmScope()
+    {
+        (void)set_alpha<double>(P, P->H64, false);
+        (void)set_alpha<float>(P, P->H32, false);
+        P->bc = bc;
+        P->singular = singular;
+        P->opts.eps_rel = eps_rel;
+        P->opts.eps_abs = eps_abs;
+        P->helm = false;
+    }
+};
+
+}  // namespace
+
+extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
+{
+    if (!P || !in || !in->rh || !in->phi || !in->beta) return MAC_ERR_INVALID;
+    if (in->ng_rh < 0 || in->ng_phi < 0 || in->ng_alpha < 0 || in->ng_beta < 0) return MAC_ERR_LAYOUT;
+    if (!(in->eps_rel > 0.0) && !(in->eps_abs > 0.0)) return MAC_ERR_INVALID;
+    if (P->opts.precision < MAC_PREC_FP64 || P->opts.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
+    const int nb = P->nboxes, dim = P->dim;
+    EllBC cb = P->bc;
+    bool dir = false;
+    for (int d = 0; d < dim; ++d)
+        for (int side = 0; side < 2; ++side) {
+            const int k = in->ell_bc[d][side];
+            if (k != MAC_ELL_PER && k != MAC_ELL_DIR && k != MAC_ELL_NEU) return MAC_ERR_INVALID;
+            // the hierarchy's periodic wrap comes from the layout
+            if ((k == MAC_ELL_PER) != (P->bc.k[d][side] == ELL_PER)) return MAC_ERR_INVALID;
+            cb.k[d][side] = k;
+            dir |= k == MAC_ELL_DIR;
+        }
+    if (dir && in->ng_phi < 1) return MAC_ERR_LAYOUT;   // Dirichlet values live in phi's ghosts
+    if (st) memset(st, 0, sizeof(*st));
+    HIPCHK(hipSetDevice(P->device));
+    hipStream_t s = P->stream;
+    int rc;
+    HIPCHK(hipEventRecord(P->ev[0], s));
+    HelmScope scope(P, cb, in->eps_rel, in->eps_abs);
+    CallStage S(P);
+    std::vector<size_t> n_rh(nb), n_phi(nb), n_a(nb), n_b((size_t)nb * dim);
+    for (int b = 0; b < nb; ++b) {
+        n_rh[b] = box_elems(P, b, in->ng_rh, -1);
+        n_phi[b] = box_elems(P, b, in->ng_phi, -1);
+        n_a[b] = box_elems(P, b, in->ng_alpha, -1);
+        for (int d = 0; d < dim; ++d) n_b[b * dim + d] = box_elems(P, b, in->ng_beta, d);
+    }
+    std::vector<double *> v_rh, v_phi, v_a, v_b;
+    double **t_rh, **t_phi, **t_a, **t_b;
+    if ((rc = S.reserve((size_t)nb * (3 + dim)))) return rc;
+    if ((rc = S.table(in->rh, n_rh, true, v_rh, &t_rh)) ||
+        (rc = S.table((const double *const *)in->phi, n_phi, true, v_phi, &t_phi)) ||
+        (rc = S.table(in->alpha, n_a, true, v_a, &t_a)) || (rc = S.table(in->beta, n_b, true, v_b, &t_b)))
+        return rc;
+    if ((rc = set_alpha<double>(P, P->H64, in->alpha != nullptr))) return rc;
+    HIPCHK(hipEventRecord(P->ev[1], s));
+    // ---- setup: weights, alpha, rh' and warm start from the boxes
+    SolveArgs A;
+    A.dim = dim;
+    A.nboxes = nb;
+    A.boxes = P->boxes_dev;
+    for (int d = 0; d < 3; ++d) { A.dlo[d] = P->dlo[d]; A.n[d] = P->n[d]; }
+    A.bc = P->bc;
+    A.rh = t_rh; A.ng_rh = in->ng_rh;
+    A.phi = t_phi; A.ng_phi = in->ng_phi;
+    A.alpha = t_a; A.ng_alpha = in->ng_alpha;
+    A.beta = t_b; A.ng_beta = in->ng_beta;
+    A.L = P->H64.lev[0];
+    const long np = setup_nparts(P->maxcells, nb);
+    if ((rc = ensure_part(P, np))) return rc;
+    A.part = P->part;
+    A.err = P->err_dev;
+    A.amax = P->nrm_dev + 2;
+    HIPCHK(hipMemsetAsync(P->err_dev, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(P->nrm_dev + 2, 0, sizeof(unsigned long long), s));
+    int pq = prof_begin(P, MAC_PROF_SETUP);
+    launch_solve_setup(A, P->maxcells, s);
+    prof_end(P, pq);
+    HIPCHK(hipGetLastError());
+    int err_host = 0;
+    unsigned long long amax_bits = 0;
+    HIPCHK(hipMemcpyAsync(&err_host, P->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&amax_bits, P->nrm_dev + 2, sizeof amax_bits, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err_host) return MAC_ERR_NONFINITE;
+    P->singular = !dir && amax_bits == 0;
+    const long n0 = ncell(P->n);
+    if (P->singular) {
+        launch_mean(P->part, np, (double)n0, P->mean_dev, s);
+        launch_sub_mean<double>(P->H64.lev[0].rhs, n0, P->mean_dev, s);
+    }
+    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
+    launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
+    unsigned long long rhs_bits = 0;
+    HIPCHK(hipMemcpyAsync(&rhs_bits, P->nrm_dev, sizeof rhs_bits, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(P->ev[2], s));
+    HIPCHK(hipStreamSynchronize(s));
+    // ---- solve
+    mac_stats local;
+    memset(&local, 0, sizeof local);
+    const int src = solve(P, &local);
+    memcpy(&local.rhs_norm, &rhs_bits, sizeof(double));
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    HIPCHK(hipEventRecord(P->ev[3], s));
+    // ---- phi out (valid cells)
+    hipLaunchKernelGGL(k_phi_out, dim3((P->maxcells + 255) / 256, nb), dim3(256), 0, s,
+                       (const BoxDev *)P->boxes_dev, (double *const *)t_phi, in->ng_phi, dim,
+                       (const double *)P->H64.lev[0].phi, P->dlo[0], P->dlo[1], P->dlo[2], P->n[0], P->n[1]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P->ev[4], s));
+    for (int b = 0; b < nb; ++b)
+        if ((rc = S.back(in->phi[b], v_phi[b], n_phi[b]))) return rc;
+    HIPCHK(hipEventRecord(P->ev[5], s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        *st = local;
+        st->t_h2d_ms = ev_ms(P, 0, 1);
+        st->t_setup_ms = ev_ms(P, 1, 2);
+        st->t_solve_ms = ev_ms(P, 2, 3);
+        st->t_correct_ms = ev_ms(P, 3, 4);
+        st->t_d2h_ms = ev_ms(P, 4, 5);
+        st->t_total_ms = ev_ms(P, 0, 5);
+    }
+    return src;
 }

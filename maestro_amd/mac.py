@@ -28,11 +28,13 @@ LIB_PATH = os.path.join(_HERE, "libmaestro_mac.so")
 
 PERIODIC, INTERIOR, INLET, OUTLET, SYMMETRY, SLIP_WALL, NO_SLIP_WALL = -1, 0, 11, 12, 13, 14, 15
 PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
+ELL_PER, ELL_DIR, ELL_NEU = -1, 1, 2               # elliptic boundary kinds (mac_solve)
+COMP_PRESS, COMP_SCALAR, COMP_EXTRAP = 0, 1, 2     # component classes of mac_ell_bc
 ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
           3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory"}
 SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
            "mac_project", "mac_project_level", "mac_adjoint_level", "mac_version",
-           "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac",
+           "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
            "mac_plan_profile", "mac_plan_profile_read")
 PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5,
@@ -86,6 +88,13 @@ class CMacrhsIn(C.Structure):
                 ("is_predictor", C.c_int)]
 
 
+class CSolveIn(C.Structure):
+    """mac_solve_in: the arguments of mac_multigrid (per-box tables)."""
+    _fields_ = [("rh", C.c_void_p), ("ng_rh", C.c_int), ("phi", C.c_void_p), ("ng_phi", C.c_int),
+                ("alpha", C.c_void_p), ("ng_alpha", C.c_int), ("beta", C.c_void_p), ("ng_beta", C.c_int),
+                ("ell_bc", (C.c_int * 2) * 3), ("eps_rel", C.c_double), ("eps_abs", C.c_double)]
+
+
 def sphr(dr, nr, center, prob_lo=(0.0, 0.0, 0.0), interp=3) -> CSphr:
     g = CSphr()
     g.dr, g.nr, g.interp = float(dr), int(nr), int(interp)
@@ -128,6 +137,8 @@ def lib():
         L.mac_cell_to_edge.argtypes = [vp, vp, ip, vp]
         L.mac_make_macrhs.argtypes = [vp, vp, C.POINTER(CMacrhsIn), C.POINTER(CSphr)]
         L.mac_make_s0mac.argtypes = [vp, vp, C.POINTER(CSphr), vp]
+        L.mac_solve.argtypes = [vp, C.POINTER(CSolveIn), C.POINTER(CStats)]
+        L.mac_ell_bc.argtypes = [ip, ip]
         L.mac_debug_level_dims.argtypes = [vp, ip, C.POINTER(C.c_int)]
         L.mac_debug_get.argtypes = [vp, ip, ip, vp]
         L.mac_debug_set.argtypes = [vp, ip, ip, vp]
@@ -139,7 +150,7 @@ def lib():
             getattr(L, f).restype = ip
         for f in ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
                   "mac_project", "mac_project_level", "mac_adjoint_level", "mac_cell_to_edge",
-                  "mac_make_macrhs", "mac_make_s0mac"):
+                  "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc"):
             getattr(L, f).restype = ip
         _lib = L
     return _lib
@@ -335,6 +346,43 @@ class MacPlan:
         if rc:
             raise MacError(rc, "mac_make_s0mac")
 
+    # ---- solver-level entry (mac_multigrid with alpha)
+    def solve(self, rh, phi, beta, alpha=None, ell_bc=None, comp=COMP_PRESS, eps_rel=1e-10, eps_abs=-1.0,
+              ng_rh=0, ng_phi=None, ng_alpha=0, ng_beta=0, check=True):
+        """mac_multigrid: solve (alpha - D beta G) phi = rh on per-box arrays
+        (``beta``: per box, ``dim`` face arrays; ``alpha`` None for 0).  The
+        elliptic BCs are ``ell_bc`` ([[lo, hi]] per direction, ELL_*) or those
+        of component class ``comp`` for the layout's physical BCs.  ``phi``
+        (warm start; Dirichlet values in its ghost cells) receives the
+        solution.  Returns the stats dict."""
+        lay = self.layout
+        dim = lay.dim
+        if ng_phi is None:
+            ng_phi = lay.ng_phi
+        if ell_bc is None:
+            ell_bc = [[ell_bc_of(lay.phys_bc[d][s], comp) for s in range(2)] for d in range(dim)]
+        flat_b = [f for box in beta for f in box]
+        assert all(len(box) == dim for box in beta)
+        self._mode(list(rh) + list(phi) + flat_b + list(alpha or []))
+        keep = [_ptr_array(list(rh)), _ptr_array(list(phi)), _ptr_array(list(alpha) if alpha else None),
+                _ptr_array(flat_b)]
+        a = CSolveIn()
+        a.rh, a.ng_rh = C.cast(keep[0], C.c_void_p), ng_rh
+        a.phi, a.ng_phi = C.cast(keep[1], C.c_void_p), ng_phi
+        a.alpha, a.ng_alpha = (C.cast(keep[2], C.c_void_p) if keep[2] is not None else None), ng_alpha
+        a.beta, a.ng_beta = C.cast(keep[3], C.c_void_p), ng_beta
+        for d in range(3):
+            for s in range(2):
+                a.ell_bc[d][s] = int(ell_bc[d][s]) if d < dim else ELL_PER
+        a.eps_rel, a.eps_abs = float(eps_rel), float(eps_abs)
+        st = CStats()
+        rc = lib().mac_solve(self._h, C.byref(a), C.byref(st))
+        if rc and (check or rc != 4):
+            raise MacError(rc, "mac_solve")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
     # ---- kernel timing on the plan's stream
     def profile(self, enable=True):
         rc = lib().mac_plan_profile(self._h, int(enable))
@@ -413,6 +461,12 @@ class MacPlan:
         d = st.as_dict()
         d["rc"] = rc
         return d
+
+
+def ell_bc_of(phys, comp=COMP_PRESS) -> int:
+    """Elliptic kind of a physical boundary code for a component class
+    (ell_bc_level_build, Source/define_bc_tower.f90:296-402)."""
+    return lib().mac_ell_bc(int(phys), int(comp))
 
 
 def macproject(layout: Layout, umac, phi, rho, macrhs=None, beta0_1d=None, beta0_1d_edge=None,

@@ -188,6 +188,7 @@ typedef struct {
     double h[3];
     double ih2[3];
     const double *b[3];
+    const double *alpha;   /* NULL: alpha = 0 */
 } orc_lv;
 
 static inline void face_pair(const orc_lv *L, const int *bc, const double *phi,
@@ -229,9 +230,11 @@ static void lv_from_level(const orc_level *lev, int dim, orc_lv *L)
         L->ih2[d] = (d < dim) ? 1.0 / (lev->h[d] * lev->h[d]) : 0.0;
         L->b[d] = lev->b[d];
     }
+    L->alpha = lev->alpha;
 }
 
-/* residual r = rhs - A phi at one cell, fixed evaluation order (fma) */
+/* residual r = rhs - A phi at one cell, fixed evaluation order (fma);
+ * A = alpha - D beta G (the alpha term last, as one fma)                  */
 static inline double cell_residual(const orc_lv *L, const int *bc, int dim,
                                    const double *phi, const double *rhs,
                                    int i, int j, int k)
@@ -253,10 +256,12 @@ static inline double cell_residual(const orc_lv *L, const int *bc, int dim,
     } else {
         Aphi = fma(L->ih2[0], ax, L->ih2[1] * ay);
     }
+    if (L->alpha) Aphi = fma(L->alpha[cidx(L->n, i, j, k)], pc, Aphi);
     return rhs[cidx(L->n, i, j, k)] - Aphi;
 }
 
-/* Gauss-Seidel point update, fixed evaluation order (fma) */
+/* Gauss-Seidel point update, fixed evaluation order (fma); alpha adds to
+ * the diagonal last                                                        */
 static inline double cell_gs(const orc_lv *L, const int *bc, int dim,
                              const double *phi, const double *rhs,
                              int i, int j, int k)
@@ -280,6 +285,7 @@ static inline double cell_gs(const orc_lv *L, const int *bc, int dim,
         num = fma(L->ih2[0], tx, fma(L->ih2[1], ty, rhs_c));
         den = fma(L->ih2[0], sx, L->ih2[1] * sy);
     }
+    if (L->alpha) den = den + L->alpha[cidx(L->n, i, j, k)];
     return num / den;
 }
 
@@ -362,6 +368,32 @@ static void restrict_defect(int dim, const orc_level *f, orc_level *c)
                         (r[cidx(f->n, i, j + 1, 0)] + r[cidx(f->n, i + 1, j + 1, 0)]);
                 }
                 c->rhs[cidx(c->n, I, J, K)] = s * v;
+            }
+}
+
+/* Coarse alpha = average of the 2^dm fine cells (same sum order as the
+ * defect restriction).  FBoxLib's coarsening of cell coefficients is not in
+ * the reference tree; averaging is this build's documented choice.        */
+static void average_alpha(int dim, const orc_level *f, orc_level *c)
+{
+    const double *a = f->alpha;
+    double s = (dim == 3) ? 0.125 : 0.25;
+    for (int K = 0; K < c->n[2]; ++K)
+        for (int J = 0; J < c->n[1]; ++J)
+            for (int I = 0; I < c->n[0]; ++I) {
+                int i = 2 * I, j = 2 * J;
+                double v;
+                if (dim == 3) {
+                    int k = 2 * K;
+                    v = ((a[cidx(f->n, i, j, k)] + a[cidx(f->n, i + 1, j, k)]) +
+                         (a[cidx(f->n, i, j + 1, k)] + a[cidx(f->n, i + 1, j + 1, k)])) +
+                        ((a[cidx(f->n, i, j, k + 1)] + a[cidx(f->n, i + 1, j, k + 1)]) +
+                         (a[cidx(f->n, i, j + 1, k + 1)] + a[cidx(f->n, i + 1, j + 1, k + 1)]));
+                } else {
+                    v = (a[cidx(f->n, i, j, 0)] + a[cidx(f->n, i + 1, j, 0)]) +
+                        (a[cidx(f->n, i, j + 1, 0)] + a[cidx(f->n, i + 1, j + 1, 0)]);
+                }
+                c->alpha[cidx(c->n, I, J, K)] = s * v;
             }
 }
 
@@ -563,9 +595,23 @@ int orc_mg_nlevels(const orc_grid *g, int max_levels)
 orc_mg *orc_mg_create(const orc_grid *g, const double *bx, const double *by,
                       const double *bz, const orc_opts *o)
 {
+    return orc_mg_create_alpha(g, NULL, bx, by, bz, o);
+}
+
+/* hierarchy of (alpha - D beta G); alpha NULL or all zero: alpha = 0 */
+orc_mg *orc_mg_create_alpha(const orc_grid *g, const double *alpha, const double *bx,
+                            const double *by, const double *bz, const orc_opts *o)
+{
     orc_mg *mg = calloc(1, sizeof(orc_mg));
     mg->g = *g;
     mg->opts = *o;
+    if (alpha) {
+        long nc0 = ncells(g->n), q;
+        for (q = 0; q < nc0; ++q)
+            if (alpha[q] != 0.0) break;
+        if (q == nc0) alpha = NULL;
+    }
+    mg->singular = orc_is_singular(g) && !alpha;
     mg->nlev = orc_mg_nlevels(g, o->max_levels);
     mg->lev = calloc((size_t)mg->nlev, sizeof(orc_level));
     int n[3] = {g->n[0], g->n[1], g->n[2]};
@@ -581,12 +627,15 @@ orc_mg *orc_mg_create(const orc_grid *g, const double *bx, const double *by,
             if (d < g->dim) L->b[d] = calloc((size_t)nfaces(n, d), sizeof(double));
             else L->b[d] = NULL;
         }
+        if (alpha) L->alpha = calloc((size_t)nc, sizeof(double));
         if (l == 0) {
             const double *src[3] = {bx, by, bz};
             for (int d = 0; d < g->dim; ++d)
                 memcpy(L->b[d], src[d], sizeof(double) * nfaces(n, d));
+            if (alpha) memcpy(L->alpha, alpha, sizeof(double) * nc);
         } else {
             average_coeffs(g->dim, &mg->lev[l - 1], L);
+            if (alpha) average_alpha(g->dim, &mg->lev[l - 1], L);
         }
         for (int d = 0; d < g->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
     }
@@ -597,7 +646,7 @@ void orc_mg_destroy(orc_mg *mg)
 {
     if (!mg) return;
     for (int l = 0; l < mg->nlev; ++l) {
-        free(mg->lev[l].phi); free(mg->lev[l].rhs); free(mg->lev[l].res);
+        free(mg->lev[l].phi); free(mg->lev[l].rhs); free(mg->lev[l].res); free(mg->lev[l].alpha);
         for (int d = 0; d < 3; ++d) free(mg->lev[l].b[d]);
     }
     free(mg->lev);
@@ -612,7 +661,7 @@ static void vcycle(orc_mg *mg, int l)
     orc_level *L = &mg->lev[l];
     if (l == mg->nlev - 1) {
         for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
-        bottom_cg(g, L, mg->opts.bottom_eps, mg->opts.bottom_maxit, orc_is_singular(g));
+        bottom_cg(g, L, mg->opts.bottom_eps, mg->opts.bottom_maxit, mg->singular);
         for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
         return;
     }
@@ -640,6 +689,7 @@ void orc_level_prolong(const orc_grid *g, const orc_level *c, orc_level *f)
 void orc_level_coarsen(const orc_grid *g, const orc_level *f, orc_level *c)
 {
     average_coeffs(g->dim, f, c);
+    if (f->alpha && c->alpha) average_alpha(g->dim, f, c);
 }
 
 /* one V-cycle on the hierarchy, level 0 holding phi/rhs already */
@@ -655,7 +705,7 @@ int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st)
     const orc_grid *g = &mg->g;
     orc_level *F = &mg->lev[0];
     long nc = ncells(F->n);
-    if (orc_is_singular(g)) {
+    if (mg->singular) {
         double m = orc_sum(rh, nc) / (double)nc;
         for (long q = 0; q < nc; ++q) rh[q] -= m;
     }
@@ -788,6 +838,75 @@ int orc_macproject(const orc_grid *g, double *u, double *v, double *w,
 
     free(rh);
     for (int d = 0; d < 3; ++d) free(b[d]);
+    return rc;
+}
+
+/* ------------------------------------- mac_multigrid with alpha (Helmholtz)
+ * mac_multigrid (Source/mac_multigrid.f90:18-71) solves
+ *   (alpha - D beta G) phi = rh
+ * for any cell coefficient alpha (0 in macproject, rho in the thermal
+ * diffusion solve, Source/thermal_conduct.f90:330-377) with the elliptic BCs
+ * of the component bc_comp.  The oracle grid carries physical codes; an
+ * elliptic Dirichlet side is given as OUTLET, Neumann as SLIP_WALL.
+ *
+ * phi_g: the solution with one ghost layer in each direction of the
+ * problem, C order with x fastest ((nz+2, ny+2, nx+2) in 3D, (ny+2, nx+2)
+ * in 2D).  Its ghost cells next to a Dirichlet side hold the boundary value
+ * on that face (FBoxLib takes Dirichlet values from phi's ghost cells; the
+ * thermal solve fills them from h, thermal_conduct.f90:343-348); the
+ * two-point boundary form then adds (2 beta / h^2) g to the right side of
+ * the boundary cell.  Its other ghost cells are not read.                  */
+static inline long gidx(const orc_grid *g, int i, int j, int k)
+{
+    const long ex = g->n[0] + 2, ey = g->n[1] + 2;
+    return (long)(i + 1) + ex * ((long)(j + 1) + ey * (long)(g->dim == 3 ? k + 1 : 0));
+}
+
+void orc_dirichlet_rhs(const orc_grid *g, const double *bx, const double *by, const double *bz,
+                       const double *phi_g, double *rh)
+{
+    const int *n = g->n;
+    const double *b[3] = {bx, by, bz};
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) {
+                const int c[3] = {i, j, k};
+                double r = rh[cidx(n, i, j, k)];
+                for (int d = 0; d < g->dim; ++d) {
+                    const double ih2 = 1.0 / (g->dx[d] * g->dx[d]);
+                    for (int side = 0; side < 2; ++side) {
+                        if (c[d] != (side ? n[d] - 1 : 0)) continue;
+                        if (orc_ell_bc(g->bc[d][side]) != ORC_ELL_DIR) continue;
+                        int f[3] = {i, j, k}, q[3] = {i, j, k};
+                        f[d] += side;
+                        q[d] += side ? 1 : -1;
+                        const double w = 2.0 * b[d][fidx(n, d, f[0], f[1], f[2])];
+                        r = fma(ih2, w * phi_g[gidx(g, q[0], q[1], q[2])], r);
+                    }
+                }
+                rh[cidx(n, i, j, k)] = r;
+            }
+}
+
+int orc_solve(const orc_grid *g, const double *alpha, const double *bx, const double *by,
+              const double *bz, const double *rh, double *phi_g, const orc_opts *o, orc_stats *st)
+{
+    const int *n = g->n;
+    const long nc = ncells(n);
+    double *r = malloc(sizeof(double) * nc), *phi = malloc(sizeof(double) * nc);
+    memcpy(r, rh, sizeof(double) * nc);
+    orc_dirichlet_rhs(g, bx, by, bz, phi_g, r);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) phi[cidx(n, i, j, k)] = phi_g[gidx(g, i, j, k)];
+    orc_mg *mg = orc_mg_create_alpha(g, alpha, bx, by, bz, o);
+    int rc = orc_mg_solve(mg, phi, r, st);
+    orc_mg_destroy(mg);
+    for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+            for (int i = 0; i < n[0]; ++i) phi_g[gidx(g, i, j, k)] = phi[cidx(n, i, j, k)];
+    free(r);
+    free(phi);
     return rc;
 }
 

@@ -53,6 +53,7 @@ typedef struct {
     double h[3];
     double *b[3];     /* raw face coefficients, natural (n+1)-in-d layout */
     double *phi, *rhs, *res;
+    double *alpha;    /* cell coefficient of (alpha - D beta G); NULL: 0 */
 } orc_level;
 
 typedef struct {
@@ -60,6 +61,7 @@ typedef struct {
     orc_opts opts;
     int nlev;
     orc_level *lev;
+    int singular;     /* no Dirichlet side and alpha == 0 */
 } orc_mg;
 
 void orc_default_opts(orc_opts *o);
@@ -96,7 +98,13 @@ void orc_mg_vcycle(orc_mg *mg);
 int orc_mg_nlevels(const orc_grid *g, int max_levels);
 orc_mg *orc_mg_create(const orc_grid *g, const double *bx, const double *by,
                       const double *bz, const orc_opts *o);
+orc_mg *orc_mg_create_alpha(const orc_grid *g, const double *alpha, const double *bx,
+                            const double *by, const double *bz, const orc_opts *o);
 void orc_mg_destroy(orc_mg *mg);
+void orc_dirichlet_rhs(const orc_grid *g, const double *bx, const double *by, const double *bz,
+                       const double *phi_g, double *rh);
+int orc_solve(const orc_grid *g, const double *alpha, const double *bx, const double *by,
+              const double *bz, const double *rh, double *phi_g, const orc_opts *o, orc_stats *st);
 int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st);
 
 void orc_mkumac(const orc_grid *g, double *u, double *v, double *w,

@@ -47,7 +47,7 @@ class Level(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("h", C.c_double * 3),
                 ("b", C.POINTER(C.c_double) * 3),
                 ("phi", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
-                ("res", C.POINTER(C.c_double))]
+                ("res", C.POINTER(C.c_double)), ("alpha", C.POINTER(C.c_double))]
 
 
 def build(force: bool = False) -> str:
@@ -91,6 +91,9 @@ def lib():
         L.orc_mg_solve.restype = C.c_int
         L.orc_mg_solve.argtypes = [C.c_void_p, dp, dp, C.POINTER(Stats)]
         L.orc_mkumac.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, dp]
+        L.orc_dirichlet_rhs.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp]
+        L.orc_solve.restype = C.c_int
+        L.orc_solve.argtypes = [C.POINTER(Grid), dp, dp, dp, dp, dp, dp, C.POINTER(Opts), C.POINTER(Stats)]
         for f in ("orc_level_restrict", "orc_level_prolong", "orc_level_coarsen"):
             getattr(L, f).argtypes = [C.POINTER(Grid), C.POINTER(Level), C.POINTER(Level)]
         L.orc_mg_vcycle.argtypes = [C.c_void_p]
@@ -180,8 +183,9 @@ class LevelView:
     """Keeps numpy buffers alive behind an oracle ``orc_level`` (extents
     ``n`` and spacing ``h`` default to the grid's)."""
 
-    def __init__(self, g, b, phi, rhs, h=None, n=None, res=None):
+    def __init__(self, g, b, phi, rhs, h=None, n=None, res=None, alpha=None):
         self.g = g
+        self.alpha = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
         self.b = [np.ascontiguousarray(x, dtype=np.float64) for x in b]
         self.phi = np.ascontiguousarray(phi, dtype=np.float64)
         self.rhs = np.ascontiguousarray(rhs, dtype=np.float64)
@@ -194,6 +198,7 @@ class LevelView:
         L.phi = _p(self.phi)
         L.rhs = _p(self.rhs)
         L.res = _p(self.res)
+        L.alpha = _p(self.alpha) if self.alpha is not None else C.POINTER(C.c_double)()
         self.L = L
 
 
@@ -212,22 +217,22 @@ def level_coarsen(g, fine: LevelView, coarse: LevelView):
     lib().orc_level_coarsen(C.byref(g), C.byref(fine.L), C.byref(coarse.L))
 
 
-def apply(g, b, phi):
-    lv = LevelView(g, b, phi, np.zeros(cell_shape(g)))
+def apply(g, b, phi, alpha=None):
+    lv = LevelView(g, b, phi, np.zeros(cell_shape(g)), alpha=alpha)
     out = np.zeros(cell_shape(g))
     lib().orc_apply(C.byref(g), C.byref(lv.L), _p(lv.phi), _p(out))
     return out
 
 
-def residual(g, b, phi, rhs):
-    lv = LevelView(g, b, phi, rhs)
+def residual(g, b, phi, rhs, alpha=None):
+    lv = LevelView(g, b, phi, rhs, alpha=alpha)
     res = np.zeros(cell_shape(g))
     m = lib().orc_residual(C.byref(g), C.byref(lv.L), _p(lv.phi), _p(lv.rhs), _p(res))
     return res, m
 
 
-def gsrb(g, b, phi, rhs, sweeps=1):
-    lv = LevelView(g, b, np.array(phi, dtype=np.float64, copy=True), rhs)
+def gsrb(g, b, phi, rhs, sweeps=1, alpha=None):
+    lv = LevelView(g, b, np.array(phi, dtype=np.float64, copy=True), rhs, alpha=alpha)
     for _ in range(sweeps):
         lib().orc_gsrb(C.byref(g), C.byref(lv.L))
     return lv.phi
@@ -267,6 +272,44 @@ def mg_solve(g, b, phi, rh, opts=None):
     finally:
         lib().orc_mg_destroy(mg)
     return phi, rh, st, rc
+
+
+def ghost_shape(g):
+    """phi with one ghost layer in each direction of the problem"""
+    return (g.n[2] + 2, g.n[1] + 2, g.n[0] + 2) if g.dim == 3 else (g.n[1] + 2, g.n[0] + 2)
+
+
+def ell_grid(n, dx, ell):
+    """Grid whose physical codes give the elliptic kinds ``ell`` (per
+    direction and side: ELL_PER, ELL_DIR or ELL_NEU) under the press_comp
+    mapping of the oracle (Dirichlet <- OUTLET, Neumann <- SLIP_WALL)."""
+    code = {ELL_PER: PERIODIC, ELL_DIR: OUTLET, ELL_NEU: SLIP_WALL}
+    return make_grid(n, dx, [[code[k] for k in side] for side in ell])
+
+
+def dirichlet_rhs(g, b, phi_g, rh):
+    """rh plus the Dirichlet boundary terms (2 beta / h^2) g of the boundary
+    values g held in phi_g's ghost cells"""
+    out = np.array(rh, dtype=np.float64, copy=True)
+    bb = [np.ascontiguousarray(x, dtype=np.float64) for x in b] + [None] * (3 - len(b))
+    lib().orc_dirichlet_rhs(C.byref(g), _p(bb[0]), _p(bb[1]), _p(bb[2]),
+                            _p(np.ascontiguousarray(phi_g, dtype=np.float64)), _p(out))
+    return out
+
+
+def solve(g, alpha, b, rh, phi_g, opts=None):
+    """mac_multigrid with alpha: solve (alpha - D beta G) phi = rh (plus the
+    Dirichlet boundary terms of phi_g's ghost cells).  Returns (phi_g with
+    the solution in its valid cells, stats, rc)."""
+    opts = opts or default_opts()
+    phi_g = np.array(phi_g, dtype=np.float64, copy=True)
+    bb = [np.ascontiguousarray(x, dtype=np.float64) for x in b] + [None] * (3 - len(b))
+    a = None if alpha is None else np.ascontiguousarray(alpha, dtype=np.float64)
+    st = Stats()
+    rc = lib().orc_solve(C.byref(g), _p(a), _p(bb[0]), _p(bb[1]), _p(bb[2]),
+                         _p(np.ascontiguousarray(rh, dtype=np.float64)), _p(phi_g), C.byref(opts),
+                         C.byref(st))
+    return phi_g, st, rc
 
 
 def mkumac(g, faces, phi, b):

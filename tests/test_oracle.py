@@ -454,3 +454,90 @@ def test_make_s0mac_known_answers(oracle_lib):
         assert abs(f[0][k + 1, j + 1, i + 1] - exp) <= 1e-12
     # faces below the domain in z take the bottom cells' value
     assert np.array_equal(f[2][0], f[2][1])
+
+
+# ---------------------------------------------- mac_multigrid with alpha
+
+
+def test_helmholtz_fourier_mode_and_constant(oracle_lib):
+    """(alpha - D beta G) with constant alpha = c and beta = 1, periodic: a
+    constant plus cos(2 pi k x / L) is mapped to c*const + (c + (4/h^2)
+    sin^2(pi k h / L)) cos.  The operator must reproduce that, and the solve
+    must return the constant as well (alpha > 0 makes it non-singular)."""
+    O = oracle_lib
+    n = (16, 8, 8)
+    h = 1.0 / 16
+    g = O.make_grid(n, [h] * 3, [[P, P]] * 3)
+    c, kx = 3.5, 3
+    x = (np.arange(n[0]) + 0.5) * h
+    mode = np.broadcast_to(np.cos(2 * np.pi * kx * x), (n[2], n[1], n[0])).copy()
+    phi = 0.75 + mode
+    alpha = np.full((n[2], n[1], n[0]), c)
+    b = [np.ones((n[2], n[1], n[0] + 1)), np.ones((n[2], n[1] + 1, n[0])), np.ones((n[2] + 1, n[1], n[0]))]
+    lam = c + 4.0 / h**2 * np.sin(np.pi * kx * h) ** 2
+    rh = c * 0.75 + lam * mode
+    assert np.abs(O.apply(g, b, phi, alpha=alpha) - rh).max() < 1e-12 * lam
+    pg = np.zeros(O.ghost_shape(g))
+    out, st, rc = O.solve(g, alpha, b, rh, pg, O.default_opts(eps_rel=1e-13))
+    assert rc == 0 and st.converged
+    assert np.abs(out[1:-1, 1:-1, 1:-1] - phi).max() < 1e-11
+
+
+def test_helmholtz_linear_profile_with_dirichlet_values(oracle_lib):
+    """Linear phi = a x + p0 between Dirichlet faces holding its face values
+    (in phi's ghost cells), beta = 1, constant alpha: the two-point boundary
+    form is exact for it, so the discrete solution of (alpha - L) phi =
+    alpha phi is the profile itself (in 2D and 3D)."""
+    O = oracle_lib
+    for n in [(16, 8), (16, 8, 4)]:
+        dim = len(n)
+        h = [1.0 / 16, 0.1, 0.2][:dim]
+        ell = [[O.ELL_DIR, O.ELL_DIR]] + [[O.ELL_PER, O.ELL_PER]] * (dim - 1)
+        g = O.ell_grid(n, h, ell)
+        a, p0, c = -2.5, 1.25, 0.8
+        shp = tuple(reversed(n))
+        x = (np.arange(n[0]) + 0.5) * h[0]
+        phi = np.broadcast_to(a * x + p0, shp).copy()
+        b = [np.ones(O.face_shape(g, d)) for d in range(dim)]
+        pg = np.zeros(O.ghost_shape(g))
+        pg[..., 0] = p0                      # value on the face x = 0
+        pg[..., -1] = a * n[0] * h[0] + p0   # value on the face x = L
+        alpha = np.full(shp, c)
+        out, st, rc = O.solve(g, alpha, b, c * phi, pg, O.default_opts(eps_rel=1e-13))
+        assert rc == 0
+        inner = out[(slice(1, -1),) * dim]
+        assert np.abs(inner - phi).max() < 1e-11, n
+        # with alpha = 0 the profile solves the homogeneous problem too
+        out0, st0, rc0 = O.solve(g, None, b, np.zeros(shp), pg, O.default_opts(eps_rel=1e-13))
+        assert rc0 == 0 and np.abs(out0[(slice(1, -1),) * dim] - phi).max() < 1e-11, n
+
+
+def test_helmholtz_operator_symmetric_positive(oracle_lib):
+    """Variable alpha > 0 and beta > 0 with Neumann, Dirichlet and periodic
+    sides: A is symmetric and positive definite (no null space), and the MG
+    solve agrees with a direct sparse solve of the operator's own matrix."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    O = oracle_lib
+    n = (8, 8, 4)
+    rng = np.random.default_rng(17)
+    g = O.ell_grid(n, [0.125, 0.125, 0.25], [[O.ELL_NEU, O.ELL_DIR], [O.ELL_PER, O.ELL_PER],
+                                             [O.ELL_NEU, O.ELL_NEU]])
+    shp = (n[2], n[1], n[0])
+    alpha = rng.uniform(0.5, 2.0, shp)
+    b = [rng.uniform(0.5, 2.0, O.face_shape(g, d)) for d in range(3)]
+    b[1][:, -1, :] = b[1][:, 0, :]     # the periodic seam is one face
+    N = alpha.size
+    cols = []
+    for q in range(N):
+        e = np.zeros(N)
+        e[q] = 1.0
+        cols.append(O.apply(g, b, e.reshape(shp), alpha=alpha).ravel())
+    A = np.array(cols).T
+    assert np.abs(A - A.T).max() < 1e-12 * np.abs(A).max()
+    assert np.linalg.eigvalsh(A).min() > 0.4
+    rh = rng.standard_normal(shp)
+    ref = spla.spsolve(sp.csr_matrix(A), rh.ravel()).reshape(shp)
+    out, st, rc = O.solve(g, alpha, b, rh, np.zeros(O.ghost_shape(g)), O.default_opts(eps_rel=1e-13))
+    assert rc == 0
+    assert np.abs(out[1:-1, 1:-1, 1:-1] - ref).max() < 1e-10 * np.abs(ref).max()
