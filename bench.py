@@ -163,6 +163,43 @@ def time_producers(plan, n, box, torch, reps=10):
     return out
 
 
+def time_helmholtz(plan, n, box, torch, reps=3):
+    """mac_solve (mac_multigrid with alpha) on the bench layout: a
+    thermal-diffusion-like Helmholtz problem (alpha - D beta G) phi = rh with
+    alpha = (0.05/h^2) U[0.5, 2], beta = U[0.5, 2] on faces, x/y periodic, z
+    Dirichlet below (values from phi's ghost cells) and Neumann above, rel
+    residual 1e-10.  Wall time per solve (device pointers), median of reps,
+    FP64 and mixed precision; the same inputs each time."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    boxes = W.box_list(n, box)
+    h = 1.0 / n[0]
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    uni = lambda shp, a, b: a + (b - a) * torch.rand(shp, dtype=torch.float64, device="cuda", generator=gen)
+    rh = [torch.randn(BX.box_shape(lo, hi, 0, 3), dtype=torch.float64, device="cuda", generator=gen)
+          for lo, hi in boxes]
+    alpha = [uni(BX.box_shape(lo, hi, 0, 3), 0.05 / h**2 * 0.5, 0.05 / h**2 * 2.0) for lo, hi in boxes]
+    beta = [[uni(BX.box_shape(lo, hi, 0, 3, face_dir=d), 0.5, 2.0) for d in range(3)] for lo, hi in boxes]
+    phi0 = [uni(BX.box_shape(lo, hi, 1, 3), -1.0, 1.0) for lo, hi in boxes]
+    ell = [[mac.ELL_PER, mac.ELL_PER], [mac.ELL_PER, mac.ELL_PER], [mac.ELL_DIR, mac.ELL_NEU]]
+    out = {"workload": f"3D {n[0]}^3 Helmholtz (alpha - D beta G) phi = rh, {len(boxes)} boxes, x/y periodic, "
+                       "z Dirichlet/Neumann, alpha/(beta/h^2) ~ 0.05, rel residual 1e-10"}
+    for name, prec in (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED)):
+        plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=1)
+        ts, st = [], None
+        for _ in range(reps + 1):
+            phi = [a.clone() for a in phi0]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = plan.solve(rh, phi, beta, alpha, ell_bc=ell, eps_rel=1e-10)
+            ts.append(time.perf_counter() - t)
+        tm = float(np.median(ts[1:]))
+        out[name] = {"ms": round(1e3 * tm, 2), "solves_per_s": round(1.0 / tm, 2), "iters": st["iters"],
+                     "res_over_res0": st["res"] / st["res0"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +212,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="time the steps without the per-kernel HIP events (no roofline figures)")
     ap.add_argument("--no-producers", action="store_true",
-                    help="skip timing make_macrhs / make_s0mac on the bench layout")
+                    help="skip timing make_macrhs / make_s0mac and the Helmholtz mac_solve on the bench layout")
     ap.add_argument("--headline", default="mixed_bf16", choices=("mixed", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=1,
@@ -255,6 +292,7 @@ def main():
                 "avg_launch_us": round(1e3 * avg, 2), "launches": cnt_, "bytes_per_launch": nbytes}
 
     producers = None if args.no_producers else time_producers(plan, n, args.box, torch)
+    helmholtz = None if args.no_producers else time_helmholtz(plan, n, args.box, torch)
 
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
@@ -346,6 +384,7 @@ def main():
             p: {k: round(v[1], 3) for k, v in results[p]["prof"].items()} for p in results},
         "accuracy": acc,
         "producers": producers,
+        "helmholtz": helmholtz,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

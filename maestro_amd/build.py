@@ -2,9 +2,9 @@
 
 ``python -m maestro_amd.build`` compiles:
   * ``maestro_amd/libmaestro_mac.so``   hipcc, --offload-arch=gfx950
-  * ``maestro_amd/fortran/libmacproject_hip.so`` and the Fortran driver
-    ``maestro_amd/fortran/macproject_driver`` (amdflang; skipped when no
-    Fortran compiler is present)
+  * ``maestro_amd/fortran/libmacproject_hip.so`` and the Fortran drivers
+    ``maestro_amd/fortran/macproject_driver`` and ``mac_multigrid_driver``
+    (amdflang; skipped when no Fortran compiler is present)
 """
 from __future__ import annotations
 
@@ -72,10 +72,12 @@ def build_fortran(verbose=False):
     exe = os.path.join(FDIR, "macproject_driver")
     _run([fc, "-O2", "-fPIC", "-shared", "-J", FDIR, mod, "-o", shim, f"-L{HERE}",
           "-lmaestro_mac", f"-Wl,-rpath,{HERE}"], cwd=FDIR)
-    if os.path.exists(drv):
-        _run([fc, "-O2", "-J", FDIR, drv, "-o", exe, f"-L{FDIR}", "-lmacproject_hip",
-              f"-L{HERE}", "-lmaestro_mac", f"-Wl,-rpath,{FDIR}", f"-Wl,-rpath,{HERE}",
-              "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."], cwd=FDIR)
+    for src, out in ((drv, exe), (os.path.join(FDIR, "mac_multigrid_driver.f90"),
+                                  os.path.join(FDIR, "mac_multigrid_driver"))):
+        if os.path.exists(src):
+            _run([fc, "-O2", "-J", FDIR, src, "-o", out, f"-L{FDIR}", "-lmacproject_hip",
+                  f"-L{HERE}", "-lmaestro_mac", f"-Wl,-rpath,{FDIR}", f"-Wl,-rpath,{HERE}",
+                  "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,$ORIGIN/.."], cwd=FDIR)
     return shim
 
 

@@ -48,6 +48,13 @@ module mac_types_module
      integer :: phys_bc(3,2) = PERIODIC       ! (direction, lo/hi)
   end type bc_tower
 
+  ! FBoxLib's flux register argument of mac_multigrid; a single-level solve
+  ! leaves it empty (the single-level callers discard it,
+  ! Source/thermal_conduct.f90:366-368)
+  type bndry_reg
+     integer :: unused = 0
+  end type bndry_reg
+
 contains
 
   subroutine multifab_build(mf, mla, ng, nodal_dir)
@@ -82,9 +89,9 @@ module macproject_module
   implicit none
 
   private
-  public :: macproject, macproject_release
+  public :: macproject, macproject_release, mac_plan_ensure, fail
   public :: use_alt_energy_fix, mac_precision, mg_verbose, eps_mac, eps_mac_max, &
-            mac_level_factor, last_stats, c_mac_stats
+            mac_level_factor, last_stats, c_mac_stats, c_mac_opts, mac_default_opts
 
   ! runtime knobs (Source/_parameters:552-554, Source/mg_eps.f90:22-28)
   logical, save :: use_alt_energy_fix = .true.
@@ -172,10 +179,8 @@ contains
     real(dp_t), intent(in), optional, target :: beta0_1d_edge(:,0:)
     type(multifab), intent(in), optional, target :: beta0_cart_edge(:,:)
 
-    type(c_mac_layout) :: lay
     type(c_mac_opts) :: o
     type(c_mac_stats) :: st
-    integer(c_int), allocatable, target :: blo(:), bhi(:)
     type(c_ptr), allocatable, target :: pu(:), pp(:), pr(:), pm(:), pc(:)
     real(dp_t), allocatable, target :: b0(:), b0e(:)
     type(c_ptr) :: cm, cc, cb0, cb0e
@@ -205,37 +210,7 @@ contains
     o%verbose = mg_verbose
     o%device_ptrs = 0
 
-    if (.not. c_associated(mla%plan) .or. any(mla%plan_ng /= ng)) then
-       if (c_associated(mla%plan)) rc = mac_plan_destroy(mla%plan)
-       allocate(blo(3*nb), bhi(3*nb))
-       do b = 1, nb
-          blo(3*(b-1)+1:3*b) = mla%lo(:,b)
-          bhi(3*(b-1)+1:3*b) = mla%hi(:,b)
-       end do
-       lay%dim = dm
-       lay%nboxes = nb
-       lay%box_lo = c_loc(blo)
-       lay%box_hi = c_loc(bhi)
-       lay%domain_lo = mla%pd_lo
-       lay%domain_hi = mla%pd_hi
-       lay%dx = 1.0_dp_t
-       lay%dx(1:dm) = dx(1,1:dm)
-       do d = 1, 3
-          lay%phys_bc(1,d) = the_bc_tower%phys_bc(d,1)
-          lay%phys_bc(2,d) = the_bc_tower%phys_bc(d,2)
-       end do
-       lay%ng_umac = ng(1)
-       lay%ng_phi = ng(2)
-       lay%ng_rho = ng(3)
-       lay%ng_macrhs = ng(4)
-       lay%ng_beta0_cart = ng(5)
-       rc = mac_plan_create(lay, o, mla%plan)
-       if (rc /= 0) call fail('mac_plan_create failed, code ', rc)
-       mla%plan_ng = ng
-    else
-       rc = mac_plan_set_opts(mla%plan, o)
-       if (rc /= 0) call fail('mac_plan_set_opts failed, code ', rc)
-    end if
+    call mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
 
     ! box-major pointer tables: umac(b*dm + d)
     allocate(pu(nb*dm), pp(nb), pr(nb))
@@ -287,6 +262,54 @@ contains
 
   end subroutine macproject
 
+  ! The device plan of the layout, built on first use (with the ghost widths
+  ! ng = [umac, phi, rho, macrhs, beta0_cart_edge] of the projection's
+  ! arrays) and cached in the ml_layout; otherwise the options o are set on
+  ! the cached plan.
+  subroutine mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
+    type(ml_layout), intent(inout) :: mla
+    real(dp_t), intent(in) :: dx(:,:)
+    type(bc_tower), intent(in) :: the_bc_tower
+    integer, intent(in) :: ng(5)
+    type(c_mac_opts), intent(in) :: o
+    type(c_mac_layout) :: lay
+    integer(c_int), allocatable, target :: blo(:), bhi(:)
+    integer :: dm, nb, b, d, rc
+    dm = mla%dim
+    nb = mla%nboxes
+    if (.not. c_associated(mla%plan) .or. any(mla%plan_ng /= ng)) then
+       if (c_associated(mla%plan)) rc = mac_plan_destroy(mla%plan)
+       allocate(blo(3*nb), bhi(3*nb))
+       do b = 1, nb
+          blo(3*(b-1)+1:3*b) = mla%lo(:,b)
+          bhi(3*(b-1)+1:3*b) = mla%hi(:,b)
+       end do
+       lay%dim = dm
+       lay%nboxes = nb
+       lay%box_lo = c_loc(blo)
+       lay%box_hi = c_loc(bhi)
+       lay%domain_lo = mla%pd_lo
+       lay%domain_hi = mla%pd_hi
+       lay%dx = 1.0_dp_t
+       lay%dx(1:dm) = dx(1,1:dm)
+       do d = 1, 3
+          lay%phys_bc(1,d) = the_bc_tower%phys_bc(d,1)
+          lay%phys_bc(2,d) = the_bc_tower%phys_bc(d,2)
+       end do
+       lay%ng_umac = ng(1)
+       lay%ng_phi = ng(2)
+       lay%ng_rho = ng(3)
+       lay%ng_macrhs = ng(4)
+       lay%ng_beta0_cart = ng(5)
+       rc = mac_plan_create(lay, o, mla%plan)
+       if (rc /= 0) call fail('mac_plan_create failed, code ', rc)
+       mla%plan_ng = ng
+    else
+       rc = mac_plan_set_opts(mla%plan, o)
+       if (rc /= 0) call fail('mac_plan_set_opts failed, code ', rc)
+    end if
+  end subroutine mac_plan_ensure
+
   subroutine macproject_release(mla)
     type(ml_layout), intent(inout) :: mla
     integer :: rc
@@ -296,3 +319,139 @@ contains
   end subroutine macproject_release
 
 end module macproject_module
+
+
+! The solver-level drop-in: FBoxLib's cell-centred solve as MAESTRO calls it,
+!   mac_multigrid(mla, rh, phi, fine_flx, solver_alpha, solver_beta, dx,
+!                 the_bc_tower, bc_comp, stencil_order, rel_solver_eps,
+!                 abs_solver_eps)
+! (Source/mac_multigrid.f90:18-71): (alpha - D beta G) phi = rh on the
+! library's mac_solve.  The elliptic BCs of component bc_comp follow
+! ell_bc_level_build (Source/define_bc_tower.f90:296-402); the caller names
+! its press_comp, foextrap_comp and hoextrap_comp indices (every other
+! component above dm is a scalar: density, (rho h), species, temperature, pi,
+! tracers).  Dirichlet values come from phi's ghost cells.
+module mac_multigrid_module
+
+  use iso_c_binding
+  use mac_types_module
+  use macproject_module, only: mac_plan_ensure, fail, c_mac_opts, c_mac_stats, mac_default_opts, &
+                               mac_precision, mg_verbose, last_stats
+  implicit none
+
+  private
+  public :: mac_multigrid, press_comp, foextrap_comp, hoextrap_comp
+
+  integer, save :: press_comp = 0, foextrap_comp = 0, hoextrap_comp = 0
+
+  type, bind(C) :: c_mac_solve_in
+     type(c_ptr) :: rh
+     integer(c_int) :: ng_rh
+     type(c_ptr) :: phi
+     integer(c_int) :: ng_phi
+     type(c_ptr) :: alpha
+     integer(c_int) :: ng_alpha
+     type(c_ptr) :: beta
+     integer(c_int) :: ng_beta
+     integer(c_int) :: ell_bc(2,3)
+     real(c_double) :: eps_rel, eps_abs
+  end type c_mac_solve_in
+
+  interface
+     integer(c_int) function mac_solve(plan, in, st) bind(C, name="mac_solve")
+       import :: c_int, c_ptr, c_mac_solve_in, c_mac_stats
+       type(c_ptr), value :: plan
+       type(c_mac_solve_in), intent(in) :: in
+       type(c_mac_stats), intent(out) :: st
+     end function mac_solve
+     integer(c_int) function mac_ell_bc(phys, comp) bind(C, name="mac_ell_bc")
+       import :: c_int
+       integer(c_int), value :: phys, comp
+     end function mac_ell_bc
+  end interface
+
+contains
+
+  subroutine mac_multigrid(mla, rh, phi, fine_flx, solver_alpha, solver_beta, dx, the_bc_tower, &
+                           bc_comp, stencil_order, rel_solver_eps, abs_solver_eps)
+
+    type(ml_layout), intent(inout) :: mla
+    type(multifab), intent(inout), target :: rh(:), phi(:)
+    type(bndry_reg), intent(inout) :: fine_flx(:)
+    type(multifab), intent(in), target :: solver_alpha(:), solver_beta(:,:)
+    real(dp_t), intent(in) :: dx(:,:)
+    type(bc_tower), intent(in) :: the_bc_tower
+    integer, intent(in) :: bc_comp, stencil_order
+    real(dp_t), intent(in) :: rel_solver_eps, abs_solver_eps
+
+    type(c_mac_opts) :: o
+    type(c_mac_stats) :: st
+    type(c_mac_solve_in) :: a
+    type(c_ptr), allocatable, target :: pr(:), pp(:), pa(:), pb(:)
+    integer :: dm, nb, b, d, s, rc, cls, ng(5)
+
+    dm = mla%dim
+    nb = mla%nboxes
+    if (size(phi) /= 1) call fail('only single-level solves are supported, nlevs = ', size(phi))
+    if (stencil_order /= 2) call fail('only stencil_order = 2 is supported, got ', stencil_order)
+    if (bc_comp == press_comp) then
+       cls = 0
+    else if (bc_comp == foextrap_comp .or. bc_comp == hoextrap_comp) then
+       cls = 2
+    else if (bc_comp > dm) then
+       cls = 1
+    else
+       call fail('velocity components have no solver BCs here, bc_comp = ', bc_comp)
+    end if
+
+    rc = mac_default_opts(o)
+    o%precision = mac_precision
+    o%verbose = mg_verbose
+    o%device_ptrs = 0
+    ! a cached projection plan serves as it is; a new one gets the projection's usual widths
+    if (c_associated(mla%plan)) then
+       ng = mla%plan_ng
+    else
+       ng = [1, phi(1)%ng, 1, 0, 1]
+    end if
+    call mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
+
+    allocate(pr(nb), pp(nb), pa(nb), pb(nb*dm))
+    do b = 1, nb
+       pr(b) = c_loc(rh(1)%fabs(b)%p)
+       pp(b) = c_loc(phi(1)%fabs(b)%p)
+       pa(b) = c_loc(solver_alpha(1)%fabs(b)%p)
+       do d = 1, dm
+          pb((b-1)*dm + d) = c_loc(solver_beta(1,d)%fabs(b)%p)
+       end do
+    end do
+    a%rh = c_loc(pr)
+    a%ng_rh = rh(1)%ng
+    a%phi = c_loc(pp)
+    a%ng_phi = phi(1)%ng
+    a%alpha = c_loc(pa)
+    a%ng_alpha = solver_alpha(1)%ng
+    a%beta = c_loc(pb)
+    a%ng_beta = solver_beta(1,1)%ng
+    a%ell_bc = -1
+    do d = 1, dm
+       do s = 1, 2
+          a%ell_bc(s,d) = mac_ell_bc(the_bc_tower%phys_bc(d,s), cls)
+       end do
+    end do
+    a%eps_rel = rel_solver_eps
+    a%eps_abs = abs_solver_eps
+
+    rc = mac_solve(mla%plan, a, st)
+    last_stats = st
+    if (mg_verbose > 0) then
+       write(*,'(a,es15.8)') 'F90mg: Initial rhs                  = ', st%rhs_norm
+       write(*,'(a,es15.8)') 'F90mg: Initial error (error0)       = ', st%res0
+       write(*,'(a,i4,a,es15.8)') 'F90mg: Final Iter. ', st%iters, ' error/error0 = ', &
+            st%res / max(st%res0, tiny(1.0_dp_t))
+    end if
+    if (rc /= 0) call fail('mac_solve failed, code ', rc)
+
+  end subroutine mac_multigrid
+
+end module mac_multigrid_module

@@ -106,3 +106,87 @@ def test_fortran_two_calls_reuse_plan(gpu, tmp_path):
     assert r.returncode == 0
     lines = [l for l in r.stdout.splitlines() if l.startswith("call")]
     assert len(lines) == 2
+
+
+SOLVE_DRIVER = os.path.join(ROOT, "maestro_amd", "fortran", "mac_multigrid_driver")
+
+
+@pytest.mark.parametrize("n,box,pbc,bc_comp,prec", [
+    # thermal diffusion: bc_comp = dm + rhoh_comp (a scalar; INLET Dirichlet)
+    ((32, 32, 32), 16, [[W.PERIODIC] * 2, [W.SLIP_WALL] * 2, [W.INLET, W.OUTLET]], 5, 0),
+    ((64, 32), 16, [[W.INLET, W.SLIP_WALL], [W.PERIODIC] * 2], 4, 1),
+    # press_comp (OUTLET Dirichlet) and foextrap_comp (Neumann everywhere)
+    ((32, 32, 32), 16, [[W.PERIODIC] * 2, [W.PERIODIC] * 2, [W.SLIP_WALL, W.OUTLET]], 100, 0),
+    ((32, 32, 32), 16, [[W.PERIODIC] * 2, [W.PERIODIC] * 2, [W.INLET, W.OUTLET]], 101, 1),
+])
+def test_fortran_mac_multigrid_matches_binding_and_oracle(gpu, oracle_lib, tmp_path, n, box, pbc, bc_comp,
+                                                          prec):
+    """The Fortran mac_multigrid (alpha = rho-like, beta on faces, Dirichlet
+    values in phi's ghost cells, reference ghost widths: alpha and beta 1)
+    equals the Python binding of mac_solve bit for bit and the oracle's
+    solve to its tolerance."""
+    from maestro_amd import mac
+    O = oracle_lib
+    assert os.path.exists(SOLVE_DRIVER), "build the Fortran shim first (maestro_amd.build)"
+    dim = len(n)
+    comp = {100: mac.COMP_PRESS, 101: mac.COMP_EXTRAP}.get(bc_comp, mac.COMP_SCALAR)
+    ell = [[mac.ell_bc_of(pbc[d][s], comp) for s in range(2)] for d in range(dim)]
+    rng = np.random.default_rng(bc_comp + dim)
+    h = 1.0 / n[0]
+    g = O.ell_grid(n, [h] * dim, ell)
+    b = [rng.uniform(0.5, 2.0, O.face_shape(g, d)) for d in range(dim)]
+    for d in range(dim):
+        if ell[d][0] == mac.ELL_PER:
+            ax = 2 - d
+            lo, hi = [slice(None)] * 3, [slice(None)] * 3
+            lo[ax], hi[ax] = 0, -1
+            b[d][tuple(hi)] = b[d][tuple(lo)]
+    alpha = 0.05 / h**2 * rng.uniform(0.5, 2.0, O.cell_shape(g))
+    rh = rng.standard_normal(O.cell_shape(g))
+    phi_g = rng.uniform(-1.0, 1.0, O.ghost_shape(g))
+    boxes = W.box_list(n, box)
+    ng_rh, ng_phi, ng_a, ng_b = 0, 1, 1, 1
+    rhb = [BX.split_cells(rh, lo_, hi_, ng_rh, dim, pbc) for lo_, hi_ in boxes]
+    phb = [BX.split_cells(phi_g, lo_, hi_, ng_phi, dim, pbc, glob_ghost=1) for lo_, hi_ in boxes]
+    ab = [BX.split_cells(alpha, lo_, hi_, ng_a, dim, pbc) for lo_, hi_ in boxes]
+    bb = [[BX.split_faces(b[d], d, lo_, hi_, ng_b, dim, pbc) for d in range(dim)] for lo_, hi_ in boxes]
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    n3 = list(n) + [1] * (3 - dim)
+    bc3 = pbc + [[W.PERIODIC] * 2] * (3 - dim)
+    with open(fin, "wb") as f:
+        np.array([dim, len(boxes), ng_rh, ng_phi, ng_a, ng_b, prec, bc_comp] + [x - 1 for x in n3[:dim]] +
+                 [0] * (3 - dim) + [bc3[d][0] for d in range(3)] + [bc3[d][1] for d in range(3)],
+                 dtype=np.int32).tofile(f)
+        for lo_, hi_ in boxes:
+            np.array(list(lo_) + [0] * (3 - dim) + list(hi_) + [0] * (3 - dim), dtype=np.int32).tofile(f)
+        np.array([h] * dim + [1.0] * (3 - dim) + [1e-10, -1.0], dtype=np.float64).tofile(f)
+        for q in range(len(boxes)):
+            rhb[q].tofile(f)
+            phb[q].tofile(f)
+            ab[q].tofile(f)
+            for d in range(dim):
+                bb[q][d].tofile(f)
+    r = subprocess.run([SOLVE_DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    raw = np.fromfile(fout, dtype=np.uint8)
+    off, fp = 0, []
+    for a in phb:
+        fp.append(raw[off:off + a.nbytes].view(np.float64).reshape(a.shape))
+        off += a.nbytes
+    iters, conv = raw[off:off + 8].view(np.int32)
+    res0, res = raw[off + 8:off + 24].view(np.float64)
+    assert conv == 1 and res <= 1e-10 * res0
+    # the Python binding of the same entry point on the same inputs
+    lay = mac.Layout.uniform(n, box, [h] * dim, pbc, ng_phi=ng_phi)
+    plan = mac.MacPlan(lay, mac.default_opts(precision=prec))
+    php = [a.copy() for a in phb]
+    st = plan.solve(rhb, php, bb, ab, comp=comp, ng_phi=ng_phi, ng_alpha=ng_a, ng_beta=ng_b)
+    plan.close()
+    assert st["iters"] == iters
+    for a, c in zip(fp, php):
+        assert np.array_equal(a, c)
+    ref, st_o, rc = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-10))
+    assert rc == 0
+    got = BX.gather_cells(fp, boxes, ng_phi, n3, dim)
+    assert rel_err(got, ref[(slice(1, -1),) * dim].reshape(got.shape)) <= 1e-8
