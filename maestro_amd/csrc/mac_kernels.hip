@@ -585,10 +585,12 @@ __global__ void __launch_bounds__(fz::NT) k_gsrb_fused(Lev<T> L, const T *__rest
 // issued PD steps earlier, so a block keeps PD planes of traffic in flight.
 // One column per thread (64 x EY tile, inner 62 x (EY-2) written), 32-bit
 // indexing (levels below 2^31 cells).  MODE 0: plain; 1: zero old iterate.
-template <typename T, int MODE, int EY, int PD>
+// AL: the level has alpha (its value travels with the face weights)
+template <typename T, int MODE, int EY, int PD, bool AL = false>
 __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__restrict__ pin,
                                                       T *__restrict__ pout, FusedGrid G)
 {
+    constexpr int NW = AL ? 7 : 6;   // face weights (+ alpha) per cell
     constexpr int EX = 64, FX = EX - 2, FY = EY - 2, PX = EX + 2, PY = EY + 2, NT = EX * EY;
     constexpr int QF = PD;   // old-iterate queue: planes k+2 .. k+1+QF
     const int b = blockIdx.x;
@@ -655,16 +657,20 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
 #pragma unroll
     for (int i = 0; i < QF; ++i) fq[i] = active ? ld(col, kb + 2 + i) : T(0);
     // per-plane inputs queue: sw/srhs/sa/sb[i] = plane k+1+i at step k
-    T cw[6], crhs = T(0), pw[6], prhs = T(0);
-    T sw[PD][6], srhs[PD], sa[PD], sb[PD];
+    T cw[NW], crhs = T(0), pw[NW], prhs = T(0);
+    T sw[PD][NW], srhs[PD], sa[PD], sb[PD];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) { cw[s] = T(0); pw[s] = T(0); }
+    for (int s = 0; s < NW; ++s) { cw[s] = T(0); pw[s] = T(0); }
 #pragma unroll
     for (int i = 0; i < PD; ++i) {
         srhs[i] = sa[i] = sb[i] = T(0);
 #pragma unroll
-        for (int s = 0; s < 6; ++s) sw[i][s] = T(0);
+        for (int s = 0; s < NW; ++s) sw[i][s] = T(0);
     }
+    auto alpha_of = [](const T *w) -> T {
+        if constexpr (AL) return w[6];
+        else return T(-0.0);
+    };
     auto fetch = [&](int k, T *nw, T &nrhs, T &na, T &nb) {
         int kw;
         const bool kok = kwrap(k, kw);
@@ -676,6 +682,7 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
             nw[3] = (Yw < ny - 1) ? L.by[c + nx] : L.byh[Xw + nx * kw];
             nw[4] = L.bz[c];
             nw[5] = (kw < nz - 1) ? L.bz[c + plane] : L.bzh[col];
+            if constexpr (AL) nw[6] = L.alpha[c];
             nrhs = L.rhs[c];
         }
         na = (r2a >= 0) ? ld(c2a, k) : T(0);
@@ -698,14 +705,14 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
         const bool kok = kwrap(k, kw);
         // advance the input queue: head -> current plane, fetch plane k+PD
 #pragma unroll
-        for (int s = 0; s < 6; ++s) { pw[s] = cw[s]; cw[s] = sw[0][s]; }
+        for (int s = 0; s < NW; ++s) { pw[s] = cw[s]; cw[s] = sw[0][s]; }
         prhs = crhs;
         crhs = srhs[0];
         const T r2va = sa[0], r2vb = sb[0];
 #pragma unroll
         for (int i = 0; i + 1 < PD; ++i) {
 #pragma unroll
-            for (int s = 0; s < 6; ++s) sw[i][s] = sw[i + 1][s];
+            for (int s = 0; s < NW; ++s) sw[i][s] = sw[i + 1][s];
             srhs[i] = srhs[i + 1];
             sa[i] = sa[i + 1];
             sb[i] = sb[i + 1];
@@ -720,7 +727,7 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
         // (b) red cells of plane k (tile and ring)
         if (active && red) {
             T p[6] = {buf[pos - 1], buf[pos + 1], buf[pos - PX], buf[pos + PX], pm1, pp1};
-            T v = gs_value<3, T>(L, cw, p, crhs);
+            T v = gs_value<3, T>(L, cw, p, crhs, alpha_of(cw));
             buf[pos] = v;
             p0 = v;
         }
@@ -729,7 +736,7 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
         if (outcol && km >= z0 && km < z1) {
             if ((pxy + km) & 1) {
                 T p[6] = {pbuf[pos - 1], pbuf[pos + 1], pbuf[pos - PX], pbuf[pos + PX], pm2, p0};
-                pm1 = gs_value<3, T>(L, pw, p, prhs);
+                pm1 = gs_value<3, T>(L, pw, p, prhs, alpha_of(pw));
             }
             pout[col + plane * km] = pm1;
         }
@@ -852,14 +859,17 @@ struct RowBufs {
     const char *bp, *bw;
     unsigned in, out;
     unsigned bx, by, bz, rhs, bxh, byh, bzh;
+    unsigned alpha;   // AL only (alpha is part of the weights' allocation)
 };
 
-template <typename T, int MODE, int EY, int NTH, bool B16, int W>
+// AL: the level has alpha (a column pair per plane beside the weights)
+template <typename T, int MODE, int EY, int NTH, bool B16, int W, bool AL = false>
 __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweepGrid G)
 {
     static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
+    static_assert(!(AL && B16), "compact storage carries no alpha");
     using T2 = typename Vec2<T>::type;
-    constexpr int NX = 128 * W, FY = EY - 2, PY = EY + 2, NI = 13;
+    constexpr int NX = 128 * W, FY = EY - 2, PY = EY + 2, NI = AL ? 15 : 13;
     constexpr int PXL = NX + 4;      // LDS row: pad (x = -1) at 1, column x at 2 + x, pad (x = nx) at NX + 2
     constexpr int PS = PY * PXL;     // LDS plane
     constexpr unsigned ES = sizeof(T), WS = B16 ? 2u : (unsigned)sizeof(T);   // iterate / weight bytes
@@ -981,6 +991,11 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
         const bool use2 = kok && ok2;
         o.v[11] = use2 ? va.x : T(0);
         o.v[12] = use2 ? va.y : T(0);
+        if constexpr (AL) {   // [13, 14]: alpha of the column pair
+            const T2 al = ldw(Rw, vw, B.alpha + sw);
+            o.v[13] = al.x;
+            o.v[14] = al.y;
+        }
         return o;
     };
     const int kb = z0 - 1, ke = z1;
@@ -1073,6 +1088,10 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
         auto update = [&](auto RC) __attribute__((always_inline)) {
             constexpr int R = decltype(RC)::value;
             const In &cw = wq[C0], &pw = wq[CP];
+            auto alpha_of = [](const In &q) -> T {
+                if constexpr (AL) return q.v[13 + R];
+                else return T(-0.0);
+            };
             T f0R = R ? f0b : f0a;   // column R of plane k, after the red update
             if (do_red) {
                 // (b) the red cell of plane k: column R
@@ -1086,7 +1105,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
                     xp = f0b;
                 }
                 const T nb[6] = {xm, xp, buf[lp - PXL + R], buf[lp + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
-                red_v = gs_value<3, T>(L, wt, nb, cw.v[9 + R]);
+                red_v = gs_value<3, T>(L, wt, nb, cw.v[9 + R], alpha_of(cw));
                 buf[lp + R] = red_v;
                 f0R = red_v;
             }
@@ -1103,7 +1122,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
                     xp = m1b;
                 }
                 const T nb[6] = {xm, xp, pbuf[lp - PXL + R], pbuf[lp + PXL + R], R ? m2b : m2a, f0R};
-                blk_v = gs_value<3, T>(L, wt, nb, pw.v[9 + R]);
+                blk_v = gs_value<3, T>(L, wt, nb, pw.v[9 + R], alpha_of(pw));
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
@@ -1499,8 +1518,9 @@ struct RowMarchGrid {
     int per_xcd;   // blocks per XCD
 };
 
-// B16: the FP32 right side goes out as bfloat16 (r16) instead of r32
-template <int MODE, bool B16>
+// B16: the FP32 right side goes out as bfloat16 (r16) instead of r32.
+// AL: the level has alpha (+ alpha phi in each residual, as one fma).
+template <int MODE, bool B16, bool AL = false>
 __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *__restrict__ e,
                                                    double *__restrict__ phi_out, float *__restrict__ r32,
                                                    unsigned short *__restrict__ r16, unsigned long long *nrm,
@@ -1561,7 +1581,7 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
         struct P {   // in-plane inputs of one plane
             double ym0, ym1, yp0, yp1, xm, xp;
             float eym0, eym1, eyp0, eyp1, exm, exp;
-            double bx0, bx1, bx2, by0, by1, byn0, byn1, r0, r1;
+            double bx0, bx1, bx2, by0, by1, byn0, byn1, r0, r1, a0, a1;
         };
         auto loadp = [&](int k) -> P {
             P q;
@@ -1602,6 +1622,13 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
             const D2 r = ld_off<D2>(L.rhs + pk, oc * 8u);
             q.r0 = r.x;
             q.r1 = r.y;
+            if constexpr (AL) {
+                const D2 al = ld_off<D2>(L.alpha + pk, oc * 8u);
+                q.a0 = al.x;
+                q.a1 = al.y;
+            } else {
+                q.a0 = q.a1 = 0.0;
+            }
             return q;
         };
         Z zm = loadz(z0 - 1), zc = loadz(z0), zp = loadz(z0 + 1);
@@ -1649,7 +1676,9 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
                     for (int s = 0; s < 6; ++s) p[s] = p[s] + (double)pe[s];
                     pcc = pcc + (double)(c ? zc.e1 : zc.e0);
                 }
-                res[c] = (c ? pc.r1 : pc.r0) - apply_value<3, double>(L, w, p, pcc);
+                double ap = apply_value<3, double>(L, w, p, pcc);
+                if constexpr (AL) ap = fma(c ? pc.a1 : pc.a0, pcc, ap);
+                res[c] = (c ? pc.r1 : pc.r0) - ap;
                 pcen[c] = pcc;
                 a = fmax(a, fabs(res[c]));
             }
@@ -1684,12 +1713,14 @@ __global__ void __launch_bounds__(256) k_resid_rows(Lev<double> L, const float *
 // pair 2J, 2J+1, and marches fine planes with the planes below and above in
 // registers.  Per-cell arithmetic and the summation order of the eight fine
 // residuals are those of k_resid_restrict.  3D, fine nx = 128 W.
-// B16: the fine weights and right side come from compact storage K
-template <typename T, bool B16>
+// B16: the fine weights and right side come from compact storage K.
+// AL: the fine level has alpha (+ alpha phi in each residual, as one fma).
+template <typename T, bool B16, bool AL = false>
 __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__ phi, Lev<T> C,
                                                 RowMarchGrid G, Compact16 K)
 {
     static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
+    static_assert(!(AL && B16), "compact storage carries no alpha");
     using T2 = typename Vec2<T>::type;
     constexpr unsigned ES = sizeof(T);
     const int lane = threadIdx.x & 63;
@@ -1776,6 +1807,11 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
         const T2 byn = (j1 < ny - 1) ? pair(F.by + pk, K.by + pk, o1 + nx)
                                      : pair(F.byh + (size_t)nx * k, K.byh + (size_t)nx * k, (unsigned)x0);
         const T2 rh0 = pair(F.rhs + pk, K.rhs + pk, o0), rh1 = pair(F.rhs + pk, K.rhs + pk, o1);
+        T2 al0, al1;
+        if constexpr (AL) {
+            al0 = ld_off<T2>(F.alpha + pk, o0 * ES);
+            al1 = ld_off<T2>(F.alpha + pk, o1 * ES);
+        }
         T bzn00, bzn01, bzn10, bzn11;
         if (k < nz - 1) {
             bzn00 = zp.b00;
@@ -1811,7 +1847,9 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
                 p[5] = b ? (a ? zp.p11 : zp.p10) : (a ? zp.p01 : zp.p00);
                 const T pcc = a ? pr1 : pr0;
                 const T rh = b ? (a ? rh1.y : rh1.x) : (a ? rh0.y : rh0.x);
-                r[b][a] = rh - apply_value<3, T>(F, w, p, pcc);
+                T ap = apply_value<3, T>(F, w, p, pcc);
+                if constexpr (AL) ap = dfma(b ? (a ? al1.y : al1.x) : (a ? al0.y : al0.x), pcc, ap);
+                r[b][a] = rh - ap;
             }
     };
     Z zm = loadz(2 * K0 - 1), zc = loadz(2 * K0), zp = loadz(2 * K0 + 1);
@@ -2877,6 +2915,13 @@ static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     G.total = tiles * zc;
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(64 * EY);
+    if (L.alpha) {
+        if (mode == SWEEP_ZERO)
+            hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD, true>), g, b, 0, s, L, pin, pout, G);
+        else
+            hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD, true>), g, b, 0, s, L, pin, pout, G);
+        return;
+    }
     if (mode == SWEEP_ZERO)
         hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD>), g, b, 0, s, L, pin, pout, G);
     else
@@ -2890,8 +2935,8 @@ static bool rows_fit(const Lev<T> &L)
     if (L.nz < 8) return false;                      // the march wraps at most one period
     const int W = L.nx / 128;
     if (64 * W * EY > NTH) return false;
-    // byte offsets within a level's allocation are 32-bit
-    if (10.0 * L.nx * L.ny * L.nz * sizeof(T) >= 4294967296.0) return false;
+    // byte offsets within a level's allocation (11 cell-sized parts at most) are 32-bit
+    if (11.0 * L.nx * L.ny * L.nz * sizeof(T) >= 4294967296.0) return false;
     return sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4) <= 96 * 1024;
 }
 
@@ -2917,7 +2962,7 @@ static bool row_bufs(const Lev<T> &L, const T *pin, T *pout, const Compact16 *K,
     if (!group(pp, pb, 2, B.bp, po)) return false;
     B.in = po[0];
     B.out = po[1];
-    const char *wp[7];
+    const char *wp[8];
     long es;
     if (K) {
         const unsigned short *k[7] = {K->bx, K->by, K->bz, K->rhs, K->bxh, K->byh, K->bzh};
@@ -2928,16 +2973,20 @@ static bool row_bufs(const Lev<T> &L, const T *pin, T *pout, const Compact16 *K,
         for (int a = 0; a < 7; ++a) wp[a] = (const char *)w[a];
         es = sizeof(T);
     }
-    const long wb[7] = {nc * es, nc * es, nc * es, nc * es, (long)L.ny * L.nz * es, (long)L.nx * L.nz * es,
-                        (long)L.nx * L.ny * es};
-    unsigned wo[7];
-    if (!group(wp, wb, 7, B.bw, wo)) return false;
+    const long wb[8] = {nc * es, nc * es, nc * es, nc * es, (long)L.ny * L.nz * es, (long)L.nx * L.nz * es,
+                        (long)L.nx * L.ny * es, nc * es};
+    // alpha joins the weights' group (it lies in the same allocation)
+    const int nw = (L.alpha && !K) ? 8 : 7;
+    if (nw == 8) wp[7] = (const char *)L.alpha;
+    unsigned wo[8];
+    if (!group(wp, wb, nw, B.bw, wo)) return false;
     B.bx = wo[0]; B.by = wo[1]; B.bz = wo[2]; B.rhs = wo[3];
     B.bxh = wo[4]; B.byh = wo[5]; B.bzh = wo[6];
+    B.alpha = nw == 8 ? wo[7] : 0u;
     return true;
 }
 
-template <typename T, int EY, int NTH, bool B16, int W>
+template <typename T, int EY, int NTH, bool B16, int W, bool AL = false>
 static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
@@ -2950,9 +2999,9 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return false;
         hipError_t e = (mode == SWEEP_ZERO)
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W>,
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W, AL>,
                                                                           threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W>,
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W, AL>,
                                                                           threads, lds);
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
@@ -2983,9 +3032,9 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL>), g, b, lds, s, L, RB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W, AL>), g, b, lds, s, L, RB, G);
     return true;
 }
 
@@ -3000,6 +3049,14 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
 {
     RowBufs RB;
     if (!rows_fit<T, EY, NTH>(L) || !row_bufs<T>(L, pin, pout, B16 ? K : nullptr, RB)) return false;
+    if constexpr (!B16) {
+        if (L.alpha) {
+            if (L.nx == 128) return launch_rows_w<T, EY, NTH, false, 1, true>(L, RB, mode, s);
+            return launch_rows_w<T, EY, NTH, false, 2, true>(L, RB, mode, s);
+        }
+    } else {
+        if (L.alpha) return false;
+    }
     if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1>(L, RB, mode, s);
     return launch_rows_w<T, EY, NTH, B16, 2>(L, RB, mode, s);
 }
@@ -3070,8 +3127,10 @@ template <typename T>
 bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
                        const EllBC &bc, hipStream_t s, const Compact16 *K)
 {
-    if (L.alpha) return false;   // the fused sweeps have no alpha term
     const int variant = sweep_variant();
+    // with alpha: the full-row sweep and its fallback, which carry the alpha
+    // term; the caller sweeps per colour otherwise
+    if (L.alpha && (K || mode == SWEEP_PROLONG || variant != 1082)) return false;
     if constexpr (sizeof(T) == 4) {
         if (K) {
             if (mode == SWEEP_PROLONG || variant != 1082) return false;
@@ -3085,7 +3144,11 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const L
         bool done = false;
         switch (variant) {
         case 1082:
-            if constexpr (sizeof(T) == 8) done = launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
+            // FP64 with alpha: 4 rows of 512 threads (the alpha ring does not
+            // fit the 168 VGPRs a 768-thread block leaves a lane)
+            if constexpr (sizeof(T) == 8)
+                done = L.alpha ? launch_rows<T, 4, 512, false>(L, pin, pout, mode, s, nullptr)
+                               : launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
             else if (L.nx == 128 && rows_ey(1) == 16)
                 done = launch_rows<T, 16, 1024, false>(L, pin, pout, mode, s, nullptr);
             else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
@@ -3140,7 +3203,7 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
                            const Compact16 *K)
 {
     if (F.alpha && K) return false;
-    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1 && !F.alpha) {
+    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
         RowMarchGrid G;
         G.W = F.nx / 128;
         // coarse planes per chunk: at most 16, fewer on small levels so that
@@ -3157,6 +3220,10 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
                 hipLaunchKernelGGL((k_rr_rows<T, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, *K);
             else
                 return false;
+        } else if (F.alpha) {
+            const Compact16 K0 = {};
+            hipLaunchKernelGGL((k_rr_rows<T, false, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G,
+                               K0);
         } else {
             const Compact16 K0 = {};
             hipLaunchKernelGGL((k_rr_rows<T, false>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, K0);
@@ -3213,7 +3280,11 @@ static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_
     G.total = L.ny * G.W * chunks;
     const int blocks = (G.total + 3) / 4;
     G.per_xcd = (blocks + 7) / 8;
-    if (r16)
+    if (L.alpha) {
+        if (r16) return false;   // compact storage carries no alpha
+        hipLaunchKernelGGL((k_resid_rows<MODE, false, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, L, e,
+                           phi_out, r32, r16, nrm, G);
+    } else if (r16)
         hipLaunchKernelGGL((k_resid_rows<MODE, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, L, e, phi_out,
                            r32, r16, nrm, G);
     else
@@ -3226,7 +3297,7 @@ template <typename T>
 void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, hipStream_t s)
 {
     if constexpr (sizeof(T) == 8) {
-        if (dim == 3 && out == nullptr && nrm != nullptr && !L.alpha &&
+        if (dim == 3 && out == nullptr && nrm != nullptr &&
             launch_resid_rows<0>(L, nullptr, nullptr, nullptr, nrm, s))
             return;
     }
@@ -3238,7 +3309,7 @@ void launch_residual(int dim, const Lev<T> &L, T *out, unsigned long long *nrm, 
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && !L.alpha && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s, r16)) return;
+    if (dim == 3 && launch_resid_rows<1>(L, nullptr, nullptr, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_residual<D_>, dim3(g), dim3(TPB), 0, s, L, r32, nrm));
 }
@@ -3246,7 +3317,7 @@ void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long
 void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, double *phi_out, float *r32,
                                unsigned long long *nrm, hipStream_t s, unsigned short *r16)
 {
-    if (dim == 3 && !L.alpha && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s, r16)) return;
+    if (dim == 3 && launch_resid_rows<2>(L, e, phi_out, r32, nrm, s, r16)) return;
     unsigned g = std::min(row_blocks(L.ny, L.nz), NORM_GRID);
     DISPATCH_DIM(dim, hipLaunchKernelGGL(k_ir_update_residual<D_>, dim3(g), dim3(TPB), 0, s, L, e, phi_out,
                                          r32, nrm));

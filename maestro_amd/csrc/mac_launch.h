@@ -137,7 +137,6 @@ struct SolveArgs {
     Lev<double> L;                // level 0 (L.alpha set when alpha is given)
     double *part;
     int *err;
-    unsigned long long *amax;     // max |alpha| (bit pattern; atomic max)
 };
 
 void launch_solve_setup(const SolveArgs &A, int maxcells, hipStream_t s);

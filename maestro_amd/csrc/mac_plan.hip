@@ -40,7 +40,7 @@ struct Hier {
     Lev<T> *lev_dev = nullptr;
     std::vector<void *> allocs;
     bool built = false;
-    std::vector<T *> alpha;   // per-level alpha storage (mac_solve), made on first use
+    std::vector<T *> alpha;   // per-level alpha storage (mac_solve), in the level's allocation
     bool alpha_on = false;    // whether lev[].alpha (and lev_dev) point at it
 };
 
@@ -220,23 +220,28 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         // the level's arrays share one allocation (256-byte aligned parts), so
         // a kernel can reach all of them through one buffer resource
         const long nc = ncell(n);
-        T **arr[10] = {&L.phi, &L.phi2, &L.rhs, &L.res, &L.bx, &L.by, &L.bz, &L.bxh, &L.byh, &L.bzh};
+        // the last part holds alpha (mac_solve); L.alpha points at it only
+        // while a solve with alpha runs
+        T *alpha_part = nullptr;
+        T **arr[11] = {&L.phi, &L.phi2, &L.rhs, &L.res, &L.bx, &L.by, &L.bz, &L.bxh, &L.byh, &L.bzh, &alpha_part};
         // second iterate: ping-pong of the fused 3D sweep, and on level 0
         // the output of the fused refinement update
         const bool has2 = P->dim == 3 || l == 0, d3 = P->dim == 3;
-        const long len[10] = {nc, has2 ? nc : 0, nc, nc, nc, nc, d3 ? nc : 0, (long)n[1] * n[2],
-                              (long)n[0] * n[2], d3 ? (long)n[0] * n[1] : 0};
+        const long len[11] = {nc, has2 ? nc : 0, nc, nc, nc, nc, d3 ? nc : 0, (long)n[1] * n[2],
+                              (long)n[0] * n[2], d3 ? (long)n[0] * n[1] : 0, nc};
         auto part = [](long cnt) { return ((cnt * (long)sizeof(T) + 255) / 256) * 256; };
         long total = 0;
-        for (int a = 0; a < 10; ++a) total += part(len[a]);
+        for (int a = 0; a < 11; ++a) total += part(len[a]);
         void *blk = nullptr;
         HIPCHK(hipMalloc(&blk, (size_t)total));
         H.allocs.push_back(blk);
         long off = 0;
-        for (int a = 0; a < 10; ++a) {
+        for (int a = 0; a < 11; ++a) {
             *arr[a] = len[a] ? (T *)((char *)blk + off) : nullptr;
             off += part(len[a]);
         }
+        L.alpha = nullptr;
+        H.alpha.push_back(alpha_part);
         for (int d = 0; d < P->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
     }
     int thr = coarse_threshold(P->dim, (int)sizeof(T));
@@ -272,21 +277,11 @@ static void free_hier(Hier<T> &H)
 }
 
 // point the levels' alpha at the hierarchy's alpha storage (on) or at
-// nothing (off), allocating the storage on first use; the device copy of
-// the level table follows
+// nothing (off); the device copy of the level table follows
 template <typename T>
 static int set_alpha(mac_plan *P, Hier<T> &H, bool on)
 {
     if (!H.built || on == H.alpha_on) return MAC_OK;
-    if (on && H.alpha.empty()) {
-        for (int l = 0; l < H.nlev; ++l) {
-            void *q = nullptr;
-            const Lev<T> &L = H.lev[l];
-            HIPCHK(hipMalloc(&q, sizeof(T) * (size_t)L.nx * L.ny * L.nz));
-            H.allocs.push_back(q);
-            H.alpha.push_back((T *)q);
-        }
-    }
     for (int l = 0; l < H.nlev; ++l) H.lev[l].alpha = on ? H.alpha[l] : nullptr;
     HIPCHK(hipMemcpyAsync(H.lev_dev, H.lev.data(), sizeof(Lev<T>) * H.nlev, hipMemcpyHostToDevice, P->stream));
     H.alpha_on = on;
@@ -315,7 +310,7 @@ static bool use_fused(const mac_plan *P)
 {
     static int off = -1;
     if (off < 0) off = getenv("MAC_NO_FUSED") ? 1 : 0;
-    return P->dim == 3 && !off && !P->helm;   // the fused sweeps have no alpha term
+    return P->dim == 3 && !off;
 }
 
 // nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
@@ -351,11 +346,22 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         const int cls = mode == SWEEP_ZERO ? (f64 ? MAC_PROF_GSRB64_ZERO : MAC_PROF_GSRB32_ZERO)
                                            : (f64 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32);
         int q = (l == 0) ? prof_begin(P, cls) : -1;
-        if (!launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
-                                  P->bc, P->stream, K))
+        if (launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
+                                 P->bc, P->stream, K)) {
+            cur ^= 1;
+        } else if (L.alpha && !K) {
+            // a level with alpha that no fused sweep takes: the two colours in
+            // place on the current iterate (same result as a fused sweep)
+            Lev<T> Lc = L;
+            Lc.phi = pin;
+            if (mode == SWEEP_ZERO) launch_fill<T>(pin, (long)L.nx * L.ny * L.nz, T(0), P->stream);
+            if (mode == SWEEP_PROLONG) launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
+            launch_gsrb<T>(3, Lc, 0, P->stream);
+            launch_gsrb<T>(3, Lc, 1, P->stream);
+        } else {
             P->launch_failed = true;
+        }
         prof_end(P, q);
-        cur ^= 1;
     }
 }
 
@@ -445,6 +451,7 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compa
 static bool compact_applies(mac_plan *P)
 {
     return P->opts.precision == MAC_PREC_MIXED_BF16 && P->H32.built && use_fused(P) && !fuse_prolong_on() &&
+           !P->H64.alpha_on &&
            compact_level_ok(P->dim, P->H32.lev[0]);
 }
 
@@ -1767,11 +1774,12 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
     if ((rc = ensure_part(P, np))) return rc;
     A.part = P->part;
     A.err = P->err_dev;
-    A.amax = P->nrm_dev + 2;
     HIPCHK(hipMemsetAsync(P->err_dev, 0, sizeof(int), s));
     HIPCHK(hipMemsetAsync(P->nrm_dev + 2, 0, sizeof(unsigned long long), s));
     int pq = prof_begin(P, MAC_PROF_SETUP);
     launch_solve_setup(A, P->maxcells, s);
+    // max |alpha| (zero everywhere: the problem may be singular)
+    if (in->alpha) launch_absmax(P->H64.lev[0].alpha, ncell(P->n), P->nrm_dev + 2, s);
     prof_end(P, pq);
     HIPCHK(hipGetLastError());
     int err_host = 0;

@@ -32,8 +32,8 @@ __device__ __forceinline__ long sbox_index(const BoxDev &B, int ng, int fd, int 
 //   rh' = rh + sum over Dirichlet sides of the cell (ih2_d * w) g,
 // g the boundary value held in phi's ghost cell beyond that side, in the
 // order x lo, x hi, y lo, ... (one fma each).  Also: phi's valid cells as
-// the warm start, a non-finite flag, max |alpha| and block compensated
-// partial sums of rh' (the solvability mean of singular problems).
+// the warm start, a non-finite flag and block compensated partial sums of
+// rh' (the solvability mean of singular problems).
 template <int DIM>
 __global__ void __launch_bounds__(STPB) k_solve_setup(SolveArgs A)
 {
@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(STPB) k_solve_setup(SolveArgs A)
     const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
     const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
     const long q = (long)blockIdx.x * STPB + threadIdx.x;
-    double rsum = 0.0, amax = 0.0;
+    double rsum = 0.0;
     bool bad = false;
     if (q < (long)ex * ey * ez) {
         const int i = B.lo[0] + (int)(q % ex);
@@ -94,15 +94,12 @@ __global__ void __launch_bounds__(STPB) k_solve_setup(SolveArgs A)
             const double a = A.alpha[b][sbox_index(B, A.ng_alpha, -1, DIM, i, j, k)];
             A.L.alpha[c] = a;
             bad |= !isfinite(a);
-            amax = fabs(a);
         }
         const double p0 = ph[sbox_index(B, A.ng_phi, -1, DIM, i, j, k)];
         bad |= !isfinite(p0);
         A.L.phi[c] = p0;
     }
     if (bad) atomicOr(A.err, 1);
-    amax = wave_max(amax);
-    if (A.alpha && (threadIdx.x & 63) == 0 && amax > 0.0) atomic_max_nonneg(A.amax, amax);
     // block compensated partial sum of rh'
     double s = rsum, cmp = 0.0;
     wave_comp_sum(s, cmp);

@@ -84,6 +84,11 @@ CASES = [
     ((32, 32, 32), 16, [[NEU, NEU], [PER, PER], [NEU, NEU]], None, 0, True),
     ((32, 32, 32), 16, [[PER, PER]] * 3, "zero", 1, True),
     ((128, 64, 64), 32, [[PER, PER], [PER, PER], [DIR, DIR]], "rand", 1, True),
+    # the full-row sweep and row-march residuals with alpha (nx = 128, 256)
+    ((128, 32, 32), 32, [[PER, PER], [PER, PER], [DIR, NEU]], "rand", 0, False),
+    ((256, 32, 16), 16, [[PER, PER], [NEU, DIR], [NEU, NEU]], "rand", 0, True),
+    ((256, 32, 16), 16, [[PER, PER], [NEU, DIR], [NEU, NEU]], "rand", 1, True),
+    ((128, 64, 32), 32, [[PER, PER], [PER, PER], [DIR, DIR]], "rand", 2, True),
 ]
 
 
