@@ -47,10 +47,11 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
     c = "true" if b16 else "false"
     if v == 1082:   # full rows, W = nx / 128; rows x threads per block: FP64 6 x 768, FP32 8 x 1024
         w = max(1, nx // 128)   # (16 x 1024 at nx = 128), FP32 with bfloat16 storage 6 x 768
-        if s == 8 or b16:
-            return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}>"
-        return f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1>" if w == 1 else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}>"
-    return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}>"
+        if s == 8 or b16:   # (the last argument: no alpha term)
+            return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false>"
+        return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false>" if w == 1
+                else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}, false>")
+    return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}, false>"
 
 
 def measured_traffic(name: str):

@@ -3014,18 +3014,35 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
         const char *e = getenv("MAC_ROWS_ZC");
         zc_env = e ? atoi(e) : -1;
     }
+    // time ~ rounds of resident blocks x (march steps + a block's start-up,
+    // about 8 steps: the ring prologue and the LDS clear); ties go to more
+    // blocks (better balance across CUs).  Measured at 256^3 FP64: 4 chunks
+    // (one round, 72 steps) 173 us against 12 chunks (three rounds of 24
+    // steps) 188 us, which a model without the start-up term preferred.
     int best_zc = 1;
-    double best = -1.0;
+    long best = -1, best_blocks = 0;
     for (int zc = 1; zc <= std::max(1, L.nz / 4); ++zc) {
         const int tz = (L.nz + zc - 1) / zc;
         const int nzc = (L.nz + tz - 1) / tz;
         const long blocks = (long)G.nty * nzc;
         const long rounds = (blocks + slots - 1) / slots;
         const int steps = ((tz + 2) + 7) & ~7;   // the march runs in groups of 8 planes
-        const double eff = (double)blocks / (double)(rounds * slots) * (double)tz / (double)steps;
-        if (eff > best + 1e-9) { best = eff; best_zc = zc; }
+        const long cost = rounds * (steps + 8);
+        if (best < 0 || cost < best || (cost == best && blocks > best_blocks)) {
+            best = cost;
+            best_zc = zc;
+            best_blocks = blocks;
+        }
     }
     if (zc_env > 0) best_zc = std::min(zc_env, L.nz);
+    static int verbose = -1;   // MAC_ROWS_VERBOSE: print the launch shape once per kernel form and size
+    if (verbose < 0) verbose = getenv("MAC_ROWS_VERBOSE") ? 1 : 0;
+    static int shown_nx = 0;
+    if (verbose && shown_nx != L.nx * 1000 + L.nz) {
+        shown_nx = L.nx * 1000 + L.nz;
+        fprintf(stderr, "maestro_mac: rows T%zu EY%d NTH%d B16=%d W%d AL=%d mode %d: nx %d nz %d slots %d zc %d\n",
+                sizeof(T), EY, NTH, (int)B16, W, (int)AL, mode, L.nx, L.nz, slots, best_zc);
+    }
     G.tz = (L.nz + best_zc - 1) / best_zc;
     const int nzc = (L.nz + G.tz - 1) / G.tz;
     G.total = G.nty * nzc;
