@@ -1884,6 +1884,21 @@ __global__ void __launch_bounds__(TPB) k_absmax(const double *x, long n, unsigne
     block_max_atomic(a, nrm);
 }
 
+// x -= mean and max |x| of the result in one pass (the solvability shift of
+// a singular problem and the norm of its right side)
+__global__ void __launch_bounds__(TPB) k_sub_mean_absmax(double *x, long n, const double *mean,
+                                                         unsigned long long *nrm)
+{
+    const double m = *mean;
+    double a = 0.0;
+    for (long q = (long)blockIdx.x * TPB + threadIdx.x; q < n; q += (long)gridDim.x * TPB) {
+        const double v = x[q] - m;
+        x[q] = v;
+        a = fmax(a, fabs(v));
+    }
+    block_max_atomic(a, nrm);
+}
+
 // ------------------------------------------------------- restriction
 template <int DIM, typename T>
 __global__ void __launch_bounds__(TPB) k_restrict(const T *r, int fnx, int fny, Lev<T> C)
@@ -2473,6 +2488,9 @@ __global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
     t = t - w * gphi;
     if (has) t = t / b0;
     u[o] = t;
+    // the x faces' pass also copies phi of each valid cell (the face's high
+    // side) into the caller's box array
+    if (d == 0 && A.phi_out && i <= B.hi[0]) A.phi_out[b][boxoff(B, A.ng_phi, -1, DIM, i, j, k)] = phi_hi;
 }
 
 // ghost faces that lie inside the (periodic) domain take the value of the
@@ -3354,6 +3372,12 @@ void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsig
                          hipStream_t s)
 {
     hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq);
+}
+
+void launch_sub_mean_absmax(double *x, long n, const double *mean, unsigned long long *nrm, hipStream_t s)
+{
+    unsigned g = std::min(nblk(n), NORM_GRID);
+    hipLaunchKernelGGL(k_sub_mean_absmax, dim3(g), dim3(TPB), 0, s, x, n, mean, nrm);
 }
 
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s)

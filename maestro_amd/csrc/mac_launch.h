@@ -42,6 +42,8 @@ struct CorrectArgs {
     const int *boxmap;
     int mapg[3], mapn[3];
     long maxplane;
+    double *const *phi_out;   // optional: the caller's per-box phi (valid cells written)
+    int ng_phi;
 };
 
 // tiling of the fused 3D sweep (k_gsrb_fused)
@@ -180,6 +182,8 @@ void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, do
                                unsigned long long *nrm, hipStream_t s, unsigned short *r16 = nullptr);
 template <typename A, typename B> void launch_convert(const A *in, B *out, long n, hipStream_t s);
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
+// x -= *mean with max |x - mean| into nrm, one pass
+void launch_sub_mean_absmax(double *x, long n, const double *mean, unsigned long long *nrm, hipStream_t s);
 // one thread: dst[0] = *src, then dst[1] = seq (system scope, dst mapped host
 // memory), then *src = 0
 void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,

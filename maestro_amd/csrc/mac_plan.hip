@@ -1054,13 +1054,15 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     launch_setup(A, P->maxcells, s);
     prof_end(P, pq);
     const long n0 = ncell(P->n);
+    // max |rh| (the "Initial rhs" figure of the solver log), after the mean
+    // removal of a singular problem (fused with it)
+    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
     if (P->singular) {
         launch_mean(P->part, np, (double)n0, P->mean_dev, s);
-        launch_sub_mean<double>(P->H64.lev[0].rhs, n0, P->mean_dev, s);
+        launch_sub_mean_absmax(P->H64.lev[0].rhs, n0, P->mean_dev, P->nrm_dev, s);
+    } else {
+        launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
     }
-    // max |rh| (the "Initial rhs" figure of the solver log)
-    HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
-    launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
     int err_host = 0;
     unsigned long long rhs_bits = 0;
     HIPCHK(hipMemcpyAsync(&err_host, P->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1091,12 +1093,11 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     C.bx = L0.bx; C.by = L0.by; C.bz = L0.bz; C.bxh = L0.bxh; C.byh = L0.byh; C.bzh = L0.bzh;
     C.boxmap = P->boxmap_dev;
     C.maxplane = P->maxplane;
+    C.phi_out = (double *const *)(td + nb * dim);   // phi out rides on the x-face correction
+    C.ng_phi = P->ng_phi;
     pq = prof_begin(P, MAC_PROF_CORRECT);
     launch_correct(C, P->maxfaces, P->maxghost, s);
     prof_end(P, pq);
-    hipLaunchKernelGGL(k_phi_out, dim3((P->maxcells + 255) / 256, nb), dim3(256), 0, s,
-                       (const BoxDev *)P->boxes_dev, (double *const *)(td + nb * dim), P->ng_phi, dim,
-                       (const double *)L0.phi, P->dlo[0], P->dlo[1], P->dlo[2], P->n[0], P->n[1]);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[4], s));
     if (!P->opts.device_ptrs) {
