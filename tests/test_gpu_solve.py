@@ -22,13 +22,13 @@ PER, DIR, NEU = mac.ELL_PER, mac.ELL_DIR, mac.ELL_NEU
 PHYS = {PER: W.PERIODIC, DIR: W.OUTLET, NEU: W.SLIP_WALL}
 
 
-def make_problem(O, n, ell, seed, alpha_kind):
+def make_problem(O, n, ell, seed, alpha_kind, dx=None):
     """Random positive beta (one value per physical face), alpha, rh and phi
     with Dirichlet values in its ghost cells (oracle layout)."""
     dim = len(n)
     rng = np.random.default_rng(seed)
     h = 1.0 / n[0]
-    g = O.ell_grid(n, [h] * dim, ell)
+    g = O.ell_grid(n, dx or [h] * dim, ell)
     b = [rng.uniform(0.5, 2.0, O.face_shape(g, d)) for d in range(dim)]
     for d in range(dim):
         if ell[d][0] == PER:          # the periodic seam is one face
@@ -48,7 +48,7 @@ def make_problem(O, n, ell, seed, alpha_kind):
     return g, b, alpha, rh, phi_g
 
 
-def per_box(n, box, ell, b, alpha, rh, phi_g, ng_phi=1, ng_alpha=0, ng_beta=0, ng_rh=0):
+def per_box(n, box, ell, b, alpha, rh, phi_g, ng_phi=1, ng_alpha=0, ng_beta=0, ng_rh=0, dx=None):
     dim = len(n)
     pbc = [[PHYS[k] for k in side] for side in ell]
     boxes = W.box_list(n, box)
@@ -56,7 +56,7 @@ def per_box(n, box, ell, b, alpha, rh, phi_g, ng_phi=1, ng_alpha=0, ng_beta=0, n
     phb = [BX.split_cells(phi_g, lo, hi, ng_phi, dim, pbc, glob_ghost=1) for lo, hi in boxes]
     ab = None if alpha is None else [BX.split_cells(alpha, lo, hi, ng_alpha, dim, pbc) for lo, hi in boxes]
     bb = [[BX.split_faces(b[d], d, lo, hi, ng_beta, dim, pbc) for d in range(dim)] for lo, hi in boxes]
-    lay = mac.Layout.uniform(n, box, [1.0 / n[0]] * dim, pbc, ng_phi=ng_phi)
+    lay = mac.Layout.uniform(n, box, dx or [1.0 / n[0]] * dim, pbc, ng_phi=ng_phi)
     return lay, boxes, rhb, phb, ab, bb
 
 
@@ -219,3 +219,28 @@ def test_solve_rejects_bad_arguments(gpu, oracle_lib):
     st = plan.solve(rhb, [a.copy() for a in phb], bb, ab, ell_bc=ell)
     assert st["converged"]
     plan.close()
+
+
+@pytest.mark.parametrize("n,box,ell,prec", [((64, 32, 16), 16, [[DIR, NEU], [PER, PER], [NEU, DIR]], 0),
+                                           ((128, 32, 16), 16, [[PER, PER], [DIR, DIR], [NEU, NEU]], 1),
+                                           ((64, 48), 16, [[DIR, DIR], [NEU, NEU]], 0)])
+def test_solve_anisotropic_cells(gpu, oracle_lib, n, box, ell, prec):
+    """Cells of different widths per direction (each direction's 1/h^2 in the
+    stencil and in the Dirichlet boundary terms)."""
+    O = oracle_lib
+    dim = len(n)
+    dx = [1.0 / 64, 1.0 / 40, 1.0 / 24][:dim]
+    g, b, alpha, rh, phi_g = make_problem(O, n, ell, 21, "rand", dx=dx)
+    same, st_o, rc = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-10))
+    assert rc == 0
+    lay, boxes, rhb, phb, ab, bb = per_box(n, box, ell, b, alpha, rh, phi_g, dx=dx)
+    plan = mac.MacPlan(lay, mac.default_opts(precision=prec))
+    st, out = gpu_solve(gpu, plan, rhb, phb, ab, bb, ell, True, eps_rel=1e-10)
+    plan.close()
+    assert st["converged"] and abs(st["iters"] - st_o.iters) <= (0 if prec == 0 else 1)
+    nn = list(n) + [1] * (3 - dim)
+    got = BX.gather_cells(out, boxes, 1, nn, dim)
+    ref = same[(slice(1, -1),) * dim].reshape(got.shape)
+    e = rel_err(got, ref)
+    print(f"n={n} dx={dx} prec={prec} iters={st['iters']} (oracle {st_o.iters}) vs oracle: {e:.2e}")
+    assert e <= (1e-13 if prec == 0 else 1e-9)
