@@ -37,6 +37,11 @@ CASES = [
     (3, (128, 128, 128), 64, 1, 2),
     (2, (64, 64, 32), 32, 1, 2),
     (0, (64, 64), 16, 1, 2),
+    # extents that are not powers of two (coarsening stops at an odd extent;
+    # the bottom level is larger than two cells per direction)
+    (1, (96, 80, 48), 16, 1, 0),
+    (3, (96, 80, 48), 16, 0, 1),
+    (0, (96, 80), 16, 1, 0),
 ]
 
 
