@@ -3358,10 +3358,12 @@ void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, do
                                          r32, nrm));
 }
 
-__global__ void k_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq)
+__global__ void k_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
+                               unsigned long long *keep)
 {
     if (threadIdx.x == 0) {
         const unsigned long long v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(keep, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(dst + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(src, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -3369,9 +3371,9 @@ __global__ void k_publish_norm(unsigned long long *src, unsigned long long *dst,
 }
 
 void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
-                         hipStream_t s)
+                         hipStream_t s, unsigned long long *keep)
 {
-    hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq);
+    hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq, keep);
 }
 
 void launch_sub_mean_absmax(double *x, long n, const double *mean, unsigned long long *nrm, hipStream_t s)

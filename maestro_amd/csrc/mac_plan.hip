@@ -505,13 +505,17 @@ static int compact_level0(mac_plan *P, const Compact16 **K, bool from_f64)
 // (atomic max) and cleared again by the publishing kernel after each read.
 static unsigned long long *res_slot(mac_plan *P) { return P->nrm_dev + 1; }
 
+// Publish the residual norm accumulated in the slot and wait for it: a
+// one-thread kernel copies it (also into nrm_dev[3]) with a sequence number
+// into mapped host memory and clears the slot; the host polls the number.
+// Without working mapped memory the value comes through a plain copy.
 static int read_norm(mac_plan *P, double *out)
 {
-    unsigned long long u = 0;
+    unsigned long long u = 0, seq = 0;
     bool have = false;
     if (P->nrm_map_ok) {
-        const unsigned long long seq = ++P->nrm_seq;
-        launch_publish_norm(res_slot(P), P->nrm_map_dev, seq, P->stream);
+        seq = ++P->nrm_seq;
+        launch_publish_norm(res_slot(P), P->nrm_map_dev, seq, P->stream, P->nrm_dev + 3);
         HIPCHK(hipGetLastError());
         volatile unsigned long long *h = P->nrm_map;
         for (long it = 0;; ++it) {
@@ -530,10 +534,10 @@ static int read_norm(mac_plan *P, double *out)
         std::atomic_thread_fence(std::memory_order_acquire);
         if (have) u = h[0];
     }
-    if (!have) {   // plain copy (the slot is then cleared here)
-        HIPCHK(hipMemcpyAsync(P->nrm_host, res_slot(P), sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
+    if (!have) {   // plain copy: the published value's device copy, or the slot itself (then cleared)
+        const unsigned long long *src = seq != 0 ? P->nrm_dev + 3 : res_slot(P);
+        HIPCHK(hipMemcpyAsync(P->nrm_host, src, sizeof(unsigned long long), hipMemcpyDeviceToHost, P->stream));
+        if (seq == 0) HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
         u = *P->nrm_host;
     }
