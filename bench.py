@@ -201,6 +201,33 @@ def time_helmholtz(plan, n, box, torch, reps=3):
     return out
 
 
+def time_host_pointers(plan, pb, wl, prec):
+    """One projection with host (numpy) per-box arrays: the library copies
+    them in and out around the solve (pageable host memory over PCIe).
+    Returns the split of the second of two calls (the first allocates the
+    device mirrors)."""
+    from maestro_amd import mac
+    boxes, umac, phi, rho, _, _ = pb
+    plan.set_opts(precision=prec, device_ptrs=0)
+    b0 = np.ascontiguousarray(wl["beta0"])
+    b0e = np.ascontiguousarray(wl["beta0_edge"])
+    st = None
+    for _ in range(2):
+        u = [[a.copy() for a in box] for box in umac]
+        p = [a.copy() for a in phi]
+        t = time.perf_counter()
+        st = plan.project(u, p, rho, None, b0, b0e, None)
+        wall = time.perf_counter() - t
+    nbytes_in = sum(a.nbytes for box in umac for a in box) + sum(a.nbytes for a in phi) + \
+        sum(a.nbytes for a in rho)
+    nbytes_out = sum(a.nbytes for box in umac for a in box) + sum(a.nbytes for a in phi)
+    plan.set_opts(device_ptrs=1)
+    return {"ms_total": round(1e3 * wall, 2), "solves_per_s": round(1.0 / wall, 2),
+            "h2d_ms": round(st["t_h2d_ms"], 2), "solve_ms": round(st["t_setup_ms"] + st["t_solve_ms"] +
+                                                              st["t_correct_ms"], 2),
+            "d2h_ms": round(st["t_d2h_ms"], 2), "h2d_bytes": nbytes_in, "d2h_bytes": nbytes_out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -294,6 +321,8 @@ def main():
 
     producers = None if args.no_producers else time_producers(plan, n, args.box, torch)
     helmholtz = None if args.no_producers else time_helmholtz(plan, n, args.box, torch)
+    host = None if args.no_producers else time_host_pointers(
+        plan, pb, wl, {"mixed": mac.PREC_MIXED, "mixed_bf16": mac.PREC_MIXED_BF16}[args.headline])
 
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
@@ -386,6 +415,7 @@ def main():
         "accuracy": acc,
         "producers": producers,
         "helmholtz": helmholtz,
+        "host_pointers": host,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
