@@ -251,7 +251,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(the sharding across GPUs is a design, not run)", file=sys.stderr)
         sys.exit(2)
 
     import torch
