@@ -49,6 +49,13 @@ extern "C" {
 #define MAC_ERR_HIP 5            /* a HIP runtime call failed */
 #define MAC_ERR_NOMEM 6
 
+/* cycle shapes, numbered as mg_cycle_type (Source/_parameters:376-377): a V
+ * visits the next coarser level once, a W twice (each visit a W), an F first
+ * as an F and then as a V; a second visit starts from the first one's iterate */
+#define MAC_CYCLE_F 1
+#define MAC_CYCLE_W 2
+#define MAC_CYCLE_V 3
+
 /* precision of the V-cycle */
 #define MAC_PREC_FP64 0          /* plain FP64 V-cycle (the reference) */
 #define MAC_PREC_MIXED 1         /* FP32 V-cycle inside FP64 refinement */
@@ -95,6 +102,9 @@ typedef struct mac_opts {
     int device_ptrs;         /* 1: every array pointer is a device pointer;
                                 0: host pointers (copied in and out)         */
     int verbose;
+    int cycle_type;          /* MAC_CYCLE_F, _W or _V; 0 = V (mg_cycle_type,
+                                Source/_parameters:376-377, passed as
+                                cycle_type in Source/mac_multigrid.f90:60)   */
 } mac_opts;
 
 typedef struct mac_stats {
@@ -301,7 +311,7 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
 #define MAC_OP_RESTRICT 2    /* rhs(level+1) = average of res(level)      */
 #define MAC_OP_PROLONG 3     /* phi(level) += P phi(level+1)              */
 #define MAC_OP_COARSEN 4     /* weights(level+1) from weights(level)      */
-#define MAC_OP_VCYCLE 5      /* one V-cycle from `level` (0 only)         */
+#define MAC_OP_VCYCLE 5      /* one cycle (of opts.cycle_type) from `level` (0 only) */
 #define MAC_OP_RESID_RESTRICT 6  /* rhs(level+1) = average of (rhs - A phi)(level),
                                     phi(level+1) = 0, one fused pass       */
 #define MAC_OP_PROLONG_SWEEP 7   /* phi(level) += P phi(level+1), then one

@@ -12,8 +12,8 @@
 //   k_residual         compute_defect (mg.tex:351)
 //   k_restrict         conservative-average restriction (mg.tex:353-354)
 //   k_prolong          linear prolongation + correction (mg.tex:363-365)
-//   k_coarse_cycle     the V-cycle on the coarse levels plus the FP64 CG
-//                      bottom solve (mg.tex:221-262) in one workgroup
+//   k_coarse_cycle     the cycle (V, W or F) on the coarse levels plus the
+//                      FP64 CG bottom solve (mg.tex:221-262) in one workgroup
 //   k_correct_valid    mkumac_3d (:908-975) fused with the division by beta0
 //                      (:232-236)
 //   k_fill_ghost       multifab_fill_boundary of umac (:245-251)
@@ -1988,11 +1988,12 @@ __global__ void __launch_bounds__(TPB) k_fill(T *x, long n, T v)
 }
 
 // -------------------------------------------- single-workgroup coarse cycle
-// V-cycle over levels lc..nlev-1 and the FP64 CG bottom solve, all inside
-// one 512-thread workgroup (these levels are latency bound; one launch
-// replaces dozens).  When they fit, the levels are staged in LDS for the
-// whole cycle; otherwise they stay in global memory (L2 resident).  512
-// threads leave 256 VGPRs per lane, so the inlined cycle does not spill.
+// The cycle (V, W or F) over levels lc..nlev-1 and the FP64 CG bottom
+// solve, all inside one 512-thread workgroup (these levels are latency
+// bound; one launch replaces dozens).  When they fit, the levels are
+// staged in LDS for the whole cycle; otherwise they stay in global memory
+// (L2 resident).  512 threads leave 256 VGPRs per lane, so the inlined
+// cycle does not spill.
 static constexpr int CTPB = 512;
 
 __device__ __forceinline__ double blk_sum(double v, double *sh)
@@ -2366,7 +2367,23 @@ __device__ Lev<T> coarse_level(const Lev<T> *levs, int lc, int l, char *smem)
     return lds_view<DIM, T>(levs[l], p);
 }
 
-template <int DIM, typename T, bool LDS>
+// the bottom level: one wave for at most 64 cells (one cell per lane in
+// registers, no block barriers), else nub sweeps, the block CG, nub sweeps
+template <int DIM, typename T>
+__device__ __forceinline__ void coarse_bottom(const Lev<T> &L, const CoarseOpts &o, double *cg, double *sh)
+{
+    if (L.nx * L.ny * L.nz <= 64) {
+        if (threadIdx.x < 64) wave_bottom<DIM, T>(L, o);
+        __syncthreads();
+    } else {
+        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+        blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
+        for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+    }
+}
+
+// VC: a V-cycle (straight down and up); else a W or F cycle
+template <int DIM, typename T, bool LDS, bool VC>
 __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict__ levs, int lc,
                                                        int nlev, CoarseOpts o, double *cg)
 {
@@ -2399,28 +2416,58 @@ __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict_
         cg = (double *)p;
         __syncthreads();
     }
-    for (int l = lc; l < bot; ++l) {
-        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
-        for (int s = 0; s < o.nu1; ++s) blk_gsrb<DIM, T>(L);
-        blk_resid_restrict<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem));
-    }
-    {
-        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, bot, smem);
-        if (L.nx * L.ny * L.nz <= 64) {
-            // a bottom level of at most 64 cells is solved by the first wave
-            // alone, one cell per lane in registers (no block barriers)
-            if (threadIdx.x < 64) wave_bottom<DIM, T>(L, o);
-            __syncthreads();
-        } else {
-            for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
-            blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
-            for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
+    if constexpr (VC) {
+        for (int l = lc; l < bot; ++l) {
+            const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
+            for (int s = 0; s < o.nu1; ++s) blk_gsrb<DIM, T>(L);
+            blk_resid_restrict<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem));
         }
-    }
-    for (int l = bot - 1; l >= lc; --l) {
-        const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
-        blk_prolong<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem), o.bc);
-        for (int s = 0; s < o.nu2; ++s) blk_gsrb<DIM, T>(L);
+        coarse_bottom<DIM, T>(coarse_level<DIM, T, LDS>(levs, lc, bot, smem), o, cg, sh);
+        for (int l = bot - 1; l >= lc; --l) {
+            const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
+            blk_prolong<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem), o.bc);
+            for (int s = 0; s < o.nu2; ++s) blk_gsrb<DIM, T>(L);
+        }
+    } else {
+        // W and F cycles, without recursion.  Going down, each level owes a
+        // second visit of the next coarser level (bit l of `again`; bit l of
+        // `again_w`: that visit is a W, else a V); going up, a level that
+        // owes one turns back down before its prolongation.  The first visit
+        // has the shape of the visiting level.  The masks are the same in
+        // every thread, so the block stays in step.
+        unsigned again = 0u, again_w = 0u;
+        bool w = o.cycle == MAC_CYCLE_W;   // shape of the current visit (W, or F/V below)
+        bool v = false;                     // the current visit is a V
+        int l = lc;
+        for (;;) {
+            for (; l < bot; ++l) {
+                const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
+                for (int s = 0; s < o.nu1; ++s) blk_gsrb<DIM, T>(L);
+                blk_resid_restrict<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem));
+                if (!v) {
+                    again |= 1u << l;
+                    if (w) again_w |= 1u << l;
+                    else again_w &= ~(1u << l);
+                }
+            }
+            coarse_bottom<DIM, T>(coarse_level<DIM, T, LDS>(levs, lc, bot, smem), o, cg, sh);
+            bool down = false;
+            while (l > lc) {
+                --l;
+                if (again & (1u << l)) {   // second visit of level l + 1
+                    again &= ~(1u << l);
+                    w = (again_w >> l) & 1u;
+                    v = !w;
+                    ++l;
+                    down = true;
+                    break;
+                }
+                const Lev<T> L = coarse_level<DIM, T, LDS>(levs, lc, l, smem);
+                blk_prolong<DIM, T>(L, coarse_level<DIM, T, LDS>(levs, lc, l + 1, smem), o.bc);
+                for (int s = 0; s < o.nu2; ++s) blk_gsrb<DIM, T>(L);
+            }
+            if (!down) break;
+        }
     }
     if (LDS) {
         const Lev<T> G = levs[lc];
@@ -3438,12 +3485,22 @@ void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_hos
     static int global_only = -1;   // MAC_COARSE_GLOBAL: keep the levels in global memory
     if (global_only < 0) global_only = getenv("MAC_COARSE_GLOBAL") ? 1 : 0;
     const long lds = coarse_lds_bytes<T>(dim, levs_host, lc, nlev);
-    if (!global_only && lds <= COARSE_LDS_MAX) {
-        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, true>), dim3(1), dim3(CTPB),
-                                             (unsigned)lds, s, levs_dev, lc, nlev, o, cg));
+    const bool in_lds = !global_only && lds <= COARSE_LDS_MAX;
+    const unsigned sh = in_lds ? (unsigned)lds : 0u;
+    if (o.cycle == MAC_CYCLE_V) {
+        if (in_lds)
+            DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, true, true>), dim3(1), dim3(CTPB), sh, s,
+                                                 levs_dev, lc, nlev, o, cg));
+        else
+            DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, false, true>), dim3(1), dim3(CTPB), sh,
+                                                 s, levs_dev, lc, nlev, o, cg));
     } else {
-        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, false>), dim3(1), dim3(CTPB), 0, s,
-                                             levs_dev, lc, nlev, o, cg));
+        if (in_lds)
+            DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, true, false>), dim3(1), dim3(CTPB), sh,
+                                                 s, levs_dev, lc, nlev, o, cg));
+        else
+            DISPATCH_DIM(dim, hipLaunchKernelGGL((k_coarse_cycle<D_, T, false, false>), dim3(1), dim3(CTPB), sh,
+                                                 s, levs_dev, lc, nlev, o, cg));
     }
 }
 

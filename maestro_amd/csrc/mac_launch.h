@@ -54,6 +54,7 @@ struct FusedGrid {
 enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2 };
 
 struct CoarseOpts {
+    int cycle;               // MAC_CYCLE_F, _W or _V
     int nu1, nu2, nub;
     double bottom_eps;
     int bottom_maxit;

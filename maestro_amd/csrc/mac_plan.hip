@@ -365,84 +365,91 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
     }
 }
 
-// zero_first: the level-0 iterate is known to be zero (refinement step);
-// K: level 0's weights and right side in compact storage (fused path)
+static CoarseOpts coarse_opts(const mac_plan *P, int type)
+{
+    const mac_opts &o = P->opts;
+    CoarseOpts co;
+    co.cycle = type;
+    co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
+    co.bottom_eps = o.bottom_eps;
+    co.bottom_maxit = o.bottom_max_iter;
+    co.singular = P->singular;
+    co.bc = P->bc;
+    return co;
+}
+
+// The shape of the cycle's next visit of level l + 1 after its first one
+// (0: none): a W visits twice (each visit a W), an F first as an F and then
+// as a V, a V once.
+static int second_visit(int type)
+{
+    return type == MAC_CYCLE_W ? MAC_CYCLE_W : type == MAC_CYCLE_F ? MAC_CYCLE_V : 0;
+}
+
+static int cycle_shape(const mac_opts &o) { return o.cycle_type == 0 ? MAC_CYCLE_V : o.cycle_type; }
+
+// One cycle of shape `type` from level l (levels lc.. run inside the
+// single-workgroup coarse cycle).  zero: level l's iterate is known to be
+// zero (a first visit below level 0, or a refinement step at level 0); a
+// second visit starts from the first one's iterate.  K: level 0's weights
+// and right side in compact storage (fused path).
 template <typename T>
-static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compact16 *K = nullptr)
+static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, const Compact16 *K)
 {
     hipStream_t s = P->stream;
     const int dim = P->dim;
     const mac_opts &o = P->opts;
-    if (use_fused(P)) {
-        std::vector<int> cur(H.nlev, 0);
-        for (int l = 0; l < H.lc; ++l) {
-            const bool zero = (l > 0) || zero_first;
-            const Compact16 *Kl = l == 0 ? K : nullptr;
-            if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur[l], Kl);
-            else if (zero) launch_fill<T>(H.lev[l].phi, (long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz, T(0), s);
-            const Lev<T> &L = H.lev[l];
-            int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_RESTRICT64 : MAC_PROF_RESTRICT32) : -1;
-            if (!launch_resid_restrict<T>(dim, L, cur[l] ? L.phi2 : L.phi, H.lev[l + 1], s, Kl))
-                P->launch_failed = true;
-            prof_end(P, q);
-        }
-        {
-            CoarseOpts co;
-            co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
-            co.bottom_eps = o.bottom_eps;
-            co.bottom_maxit = o.bottom_max_iter;
-            co.singular = P->singular;
-            co.bc = P->bc;
-            if (H.lc == 0 && zero_first)
-                launch_fill<T>(H.lev[0].phi, (long)H.lev[0].nx * H.lev[0].ny * H.lev[0].nz, T(0), s);
-            int q = prof_begin(P, MAC_PROF_COARSE);
-            launch_coarse_cycle<T>(dim, H.lev_dev, H.lev.data(), H.lc, H.nlev, co, P->cg, s);
-            prof_end(P, q);
-        }
-        for (int l = H.lc - 1; l >= 0; --l) {
-            Lev<T> &L = H.lev[l];
-            if (o.nu2 > 0) {
-                fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur[l], l == 0 ? K : nullptr);
-            } else {
-                Lev<T> Lc = L;
-                Lc.phi = cur[l] ? L.phi2 : L.phi;
-                launch_prolong<T>(dim, Lc, H.lev[l + 1], P->bc, s);
-            }
-            if (cur[l]) {   // bring the iterate back to phi
-                HIPCHK_V(hipMemcpyAsync(L.phi, L.phi2, sizeof(T) * (size_t)L.nx * L.ny * L.nz,
-                                        hipMemcpyDeviceToDevice, s));
-                cur[l] = 0;
-            }
-        }
+    Lev<T> &L = H.lev[l];
+    const long n = (long)L.nx * L.ny * L.nz;
+    if (l == H.lc) {
+        // below level 0 the fused restriction has zeroed the iterate already
+        if (zero && (l == 0 || !use_fused(P))) launch_fill<T>(L.phi, n, T(0), s);
+        int q = prof_begin(P, MAC_PROF_COARSE);
+        launch_coarse_cycle<T>(dim, H.lev_dev, H.lev.data(), H.lc, H.nlev, coarse_opts(P, type), P->cg, s);
+        prof_end(P, q);
         return;
     }
-    for (int l = 0; l < H.lc; ++l) {
-        const Lev<T> &L = H.lev[l];
-        if (l > 0 || zero_first) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
-        smooth<T>(P, L, l, o.nu1);
-        launch_residual<T>(dim, L, L.res, nullptr, s);
-        launch_restrict<T>(dim, L.res, L.nx, L.ny, H.lev[l + 1], s);
-    }
-    if (H.lc == 0 && zero_first)
-        launch_fill<T>(H.lev[0].phi, (long)H.lev[0].nx * H.lev[0].ny * H.lev[0].nz, T(0), s);
-    {
-        const Lev<T> &L = H.lev[H.lc];
-        if (H.lc > 0) launch_fill<T>(L.phi, (long)L.nx * L.ny * L.nz, T(0), s);
-        CoarseOpts co;
-        co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
-        co.bottom_eps = o.bottom_eps;
-        co.bottom_maxit = o.bottom_max_iter;
-        co.singular = P->singular;
-        co.bc = P->bc;
-        int q = prof_begin(P, MAC_PROF_COARSE);
-        launch_coarse_cycle<T>(dim, H.lev_dev, H.lev.data(), H.lc, H.nlev, co, P->cg, s);
+    const int again = second_visit(type);
+    if (use_fused(P)) {
+        int cur = 0;   // 1: the iterate is in phi2
+        const Compact16 *Kl = l == 0 ? K : nullptr;
+        if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur, Kl);
+        else if (zero) launch_fill<T>(L.phi, n, T(0), s);
+        int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_RESTRICT64 : MAC_PROF_RESTRICT32) : -1;
+        // also zeroes level l + 1's iterate
+        if (!launch_resid_restrict<T>(dim, L, cur ? L.phi2 : L.phi, H.lev[l + 1], s, Kl))
+            P->launch_failed = true;
         prof_end(P, q);
+        cycle_from<T>(P, H, l + 1, type, true, K);
+        if (again) cycle_from<T>(P, H, l + 1, again, false, K);
+        if (o.nu2 > 0) {
+            fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur, Kl);
+        } else {
+            Lev<T> Lc = L;
+            Lc.phi = cur ? L.phi2 : L.phi;
+            launch_prolong<T>(dim, Lc, H.lev[l + 1], P->bc, s);
+        }
+        if (cur)   // bring the iterate back to phi
+            HIPCHK_V(hipMemcpyAsync(L.phi, L.phi2, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        return;
     }
-    for (int l = H.lc - 1; l >= 0; --l) {
-        const Lev<T> &L = H.lev[l];
-        launch_prolong<T>(dim, L, H.lev[l + 1], P->bc, s);
-        smooth<T>(P, L, l, o.nu2);
-    }
+    if (zero) launch_fill<T>(L.phi, n, T(0), s);
+    smooth<T>(P, L, l, o.nu1);
+    launch_residual<T>(dim, L, L.res, nullptr, s);
+    launch_restrict<T>(dim, L.res, L.nx, L.ny, H.lev[l + 1], s);
+    cycle_from<T>(P, H, l + 1, type, true, K);
+    if (again) cycle_from<T>(P, H, l + 1, again, false, K);
+    launch_prolong<T>(dim, L, H.lev[l + 1], P->bc, s);
+    smooth<T>(P, L, l, o.nu2);
+}
+
+// One cycle of the plan's shape on level 0.  zero_first: the level-0
+// iterate is known to be zero (refinement step); K: level 0's weights and
+// right side in compact storage (fused path).
+template <typename T>
+static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compact16 *K = nullptr)
+{
+    cycle_from<T>(P, H, 0, cycle_shape(P->opts), zero_first, K);
 }
 
 // whether level 0 of the FP32 hierarchy is kept in compact storage
@@ -579,6 +586,7 @@ static int solve(mac_plan *P, mac_stats *st)
     int rc;
     const mac_opts &o = P->opts;
     if (o.precision < MAC_PREC_FP64 || o.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
+    if (o.cycle_type < 0 || o.cycle_type > MAC_CYCLE_V) return MAC_ERR_INVALID;
     double r0 = 0, r = 0;
     int it = 0, inner = 0;
     HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
@@ -631,7 +639,7 @@ static int solve(mac_plan *P, mac_stats *st)
             vcycle<double>(P, P->H64);
             if ((rc = fp64_norm(P, &r))) return rc;
             ++it;
-            if (o.verbose) fprintf(stderr, "maestro_mac: V-cycle %d  res/res0 = %.3e\n", it, r / r0);
+            if (o.verbose) fprintf(stderr, "maestro_mac: cycle %d  res/res0 = %.3e\n", it, r / r0);
         }
     }
     if (st) {

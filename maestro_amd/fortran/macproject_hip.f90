@@ -91,13 +91,16 @@ module macproject_module
   private
   public :: macproject, macproject_release, mac_plan_ensure, fail
   public :: use_alt_energy_fix, mac_precision, mg_verbose, eps_mac, eps_mac_max, &
-            mac_level_factor, last_stats, c_mac_stats, c_mac_opts, mac_default_opts
+            mac_level_factor, mg_cycle_type, mg_bottom_nu, last_stats, c_mac_stats, &
+            c_mac_opts, mac_default_opts
 
-  ! runtime knobs (Source/_parameters:552-554, Source/mg_eps.f90:22-28)
+  ! runtime knobs (Source/_parameters:370-392,552-554, Source/mg_eps.f90:22-28)
   logical, save :: use_alt_energy_fix = .true.
   integer, save :: mac_precision = 0            ! 0 FP64 V-cycle, 1 FP32 V-cycle + FP64 refinement,
                                                 ! 2 as 1 with bfloat16 finest-level weights and rhs
   integer, save :: mg_verbose = 0
+  integer, save :: mg_cycle_type = 3            ! 1 F-cycle, 2 W-cycle, 3 V-cycle
+  integer, save :: mg_bottom_nu = 10            ! sweeps around the bottom solve
   real(dp_t), save :: eps_mac = 1.d-10
   real(dp_t), save :: eps_mac_max = 1.d-8
   real(dp_t), save :: mac_level_factor = 10.d0
@@ -116,7 +119,7 @@ module macproject_module
      integer(c_int) :: max_iter, nu1, nu2, nub
      real(c_double) :: bottom_eps
      integer(c_int) :: bottom_max_iter, max_levels, use_alt_energy_fix, precision, &
-                       inner_cycles, device_ptrs, verbose
+                       inner_cycles, device_ptrs, verbose, cycle_type
   end type c_mac_opts
 
   type, bind(C) :: c_mac_stats
@@ -208,6 +211,8 @@ contains
     o%use_alt_energy_fix = merge(1, 0, use_alt_energy_fix)
     o%precision = mac_precision
     o%verbose = mg_verbose
+    o%cycle_type = mg_cycle_type
+    o%nub = mg_bottom_nu
     o%device_ptrs = 0
 
     call mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
@@ -336,7 +341,7 @@ module mac_multigrid_module
   use iso_c_binding
   use mac_types_module
   use macproject_module, only: mac_plan_ensure, fail, c_mac_opts, c_mac_stats, mac_default_opts, &
-                               mac_precision, mg_verbose, last_stats
+                               mac_precision, mg_verbose, mg_cycle_type, mg_bottom_nu, last_stats
   implicit none
 
   private
@@ -407,6 +412,8 @@ contains
     rc = mac_default_opts(o)
     o%precision = mac_precision
     o%verbose = mg_verbose
+    o%cycle_type = mg_cycle_type
+    o%nub = mg_bottom_nu
     o%device_ptrs = 0
     ! a cached projection plan serves as it is; a new one gets the projection's usual widths
     if (c_associated(mla%plan)) then

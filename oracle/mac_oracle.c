@@ -653,9 +653,14 @@ void orc_mg_destroy(orc_mg *mg)
     free(mg);
 }
 
-/* recursive V-cycle (mg.tex:342-370): nu1 GSRB sweeps, defect, restrict,
- * coarse solve, prolongate and correct, nu2 sweeps                        */
-static void vcycle(orc_mg *mg, int l)
+/* recursive cycle (mg.tex:342-370): nu1 GSRB sweeps, defect, restrict,
+ * coarse solve, prolongate and correct, nu2 sweeps.  The cycle shapes of
+ * mg_cycle_type (Source/_parameters:376-377; FBoxLib's cycle routine is not
+ * in the reference tree, so these are the textbook shapes of Trottenberg,
+ * Oosterlee & Schueller, "Multigrid" (2001), sec. 2.4): a V visits the next
+ * coarser level once, a W twice (each visit a W), an F first as an F and
+ * then as a V.  A second visit starts from the first one's iterate.       */
+static void cycle(orc_mg *mg, int l, int type)
 {
     const orc_grid *g = &mg->g;
     orc_level *L = &mg->lev[l];
@@ -670,9 +675,16 @@ static void vcycle(orc_mg *mg, int l)
     orc_level *C = &mg->lev[l + 1];
     restrict_defect(g->dim, L, C);
     memset(C->phi, 0, sizeof(double) * ncells(C->n));
-    vcycle(mg, l + 1);
+    cycle(mg, l + 1, type);
+    if (type == ORC_CYCLE_W) cycle(mg, l + 1, ORC_CYCLE_W);
+    else if (type == ORC_CYCLE_F) cycle(mg, l + 1, ORC_CYCLE_V);
     prolong_add(g, C, L);
     for (int s = 0; s < mg->opts.nu2; ++s) orc_gsrb(g, L);
+}
+
+static int cycle_shape(const orc_opts *o)
+{
+    return o->cycle_type == 0 ? ORC_CYCLE_V : o->cycle_type;
 }
 
 /* single operators on caller-built levels (used by the kernel tests) */
@@ -692,8 +704,9 @@ void orc_level_coarsen(const orc_grid *g, const orc_level *f, orc_level *c)
     if (f->alpha && c->alpha) average_alpha(g->dim, f, c);
 }
 
-/* one V-cycle on the hierarchy, level 0 holding phi/rhs already */
-void orc_mg_vcycle(orc_mg *mg) { vcycle(mg, 0); }
+/* one cycle (of the shape in the options) on the hierarchy, level 0
+ * holding phi/rhs already */
+void orc_mg_vcycle(orc_mg *mg) { cycle(mg, 0, cycle_shape(&mg->opts)); }
 
 /* MG solve of A phi = rh on the finest level, phi in/out (warm start).
  * Stops when max|rh - A phi| <= eps_rel * max|rh - A phi0| (the
@@ -705,6 +718,7 @@ int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st)
     const orc_grid *g = &mg->g;
     orc_level *F = &mg->lev[0];
     long nc = ncells(F->n);
+    if (mg->opts.cycle_type < 0 || mg->opts.cycle_type > ORC_CYCLE_V) return ORC_ERR_INVALID;
     if (mg->singular) {
         double m = orc_sum(rh, nc) / (double)nc;
         for (long q = 0; q < nc; ++q) rh[q] -= m;
@@ -715,7 +729,7 @@ int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st)
     double r = r0;
     int it = 0, conv = (r0 == 0.0) || (mg->opts.eps_abs > 0 && r0 <= mg->opts.eps_abs);
     while (!conv && it < mg->opts.max_iter) {
-        vcycle(mg, 0);
+        cycle(mg, 0, cycle_shape(&mg->opts));
         ++it;
         r = orc_residual(g, F, F->phi, F->rhs, NULL);
         if (st && it <= ORC_MAX_HIST) st->hist[it - 1] = r;

@@ -26,6 +26,9 @@ enum { ORC_OK = 0, ORC_ERR_INVALID = 1, ORC_ERR_NOT_CONVERGED = 4 };
 
 #define ORC_MAX_HIST 256
 
+/* cycle shapes, numbered as mg_cycle_type (Source/_parameters:376-377) */
+enum { ORC_CYCLE_F = 1, ORC_CYCLE_W = 2, ORC_CYCLE_V = 3 };
+
 typedef struct {
     int dim;          /* 2 or 3 */
     int n[3];         /* cells per direction; n[2] = 1 in 2D */
@@ -40,6 +43,7 @@ typedef struct {
     int bottom_maxit;
     int max_levels;   /* 0 = coarsen as far as possible */
     int use_alt_energy_fix;
+    int cycle_type;   /* ORC_CYCLE_F/W/V (mg_cycle_type); 0 = V */
 } orc_opts;
 
 typedef struct {

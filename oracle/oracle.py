@@ -24,6 +24,9 @@ ELL_PER, ELL_DIR, ELL_NEU = -1, 1, 2
 MAX_HIST = 256
 
 
+CYCLE_F, CYCLE_W, CYCLE_V = 1, 2, 3   # mg_cycle_type numbering
+
+
 class Grid(C.Structure):
     _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("dx", C.c_double * 3),
                 ("bc", (C.c_int * 2) * 3)]
@@ -34,7 +37,7 @@ class Opts(C.Structure):
                 ("max_iter", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int),
                 ("nub", C.c_int), ("bottom_eps", C.c_double),
                 ("bottom_maxit", C.c_int), ("max_levels", C.c_int),
-                ("use_alt_energy_fix", C.c_int)]
+                ("use_alt_energy_fix", C.c_int), ("cycle_type", C.c_int)]
 
 
 class Stats(C.Structure):

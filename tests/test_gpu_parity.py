@@ -219,3 +219,43 @@ def test_physical_edge_velocity_rules(gpu):
                             assert got == want, (b, comp, dr, side, g)
                             checked += 1
     assert checked > 0
+
+
+@pytest.mark.parametrize("cycle", [1, 2])
+@pytest.mark.parametrize("cfg,n,box,prec", [
+    (1, (64, 64, 64), 32, 0),      # three levels of separate launches above the coarse cycle
+    (3, (32, 32, 32), 16, 0),
+    (0, (64, 64), 16, 0),          # 2D: the two-colour kernels
+    (2, (128, 64, 64), 32, 1),
+    (3, (128, 64, 64), 32, 2),
+])
+def test_w_and_f_cycles_match_oracle(gpu, cycle, cfg, n, box, prec):
+    """mg_cycle_type 1 (F) and 2 (W): the GPU cycle against the oracle's
+    cycle of the same shape (FP64: the same iterates to the bottom solver's
+    summation order), and the answer against the 1e-13 truth."""
+    wl = W.make_workload(cfg, n=n, box=box)
+    truth = run_oracle(wl, eps=1e-13, alt=1)
+    same = run_oracle(wl, eps=1e-10, alt=1, cycle_type=cycle)
+    out = run_gpu(wl, use_alt_energy_fix=1, precision=prec, cycle_type=cycle)
+    st = out["stats"]
+    print(f"cycle {cycle} cfg{cfg} n={n} prec={prec} iters={st['iters']} "
+          f"(oracle {same['stats'].iters}) res/res0={st['res'] / st['res0']:.2e}")
+    assert st["converged"]
+    singular = all(b[0] != W.OUTLET for b in wl["bc"])
+    dim = wl["dim"]
+    e_phi = rel_err(out["phi"], truth["phi"], remove_mean=singular)
+    e_phi_ref = rel_err(same["phi"], truth["phi"], remove_mean=singular)
+    e_u = max(rel_err(out["faces"][d], truth["faces"][d]) for d in range(dim))
+    e_u_ref = max(rel_err(same["faces"][d], truth["faces"][d]) for d in range(dim))
+    assert e_phi <= (max(1e-8, 2.0 * e_phi_ref) if prec == 2 else 1e-8)
+    assert e_u <= max(1e-8, 2.0 * e_u_ref)
+    if prec == 0:
+        assert abs(st["iters"] - same["stats"].iters) <= 1
+        assert rel_err(out["phi"], same["phi"], remove_mean=singular) <= 1e-9
+
+
+def test_unknown_cycle_type_is_refused(gpu):
+    from maestro_amd import mac
+    wl = W.make_workload(1, n=(32, 32, 32), box=16)
+    with pytest.raises(mac.MacError):
+        run_gpu(wl, cycle_type=4)

@@ -14,11 +14,11 @@ P, WALL, OUT = W.PERIODIC, W.SLIP_WALL, W.OUTLET
 GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
 
 
-def oracle_project(O, wl, alt, macrhs=None, eps=1e-13):
+def oracle_project(O, wl, alt, macrhs=None, eps=1e-13, **kw):
     g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
     return O.macproject(g, wl["faces"], wl["phi"], wl["rho"], macrhs=macrhs, beta0=wl["beta0"],
                         beta0_edge=wl["beta0_edge"], cart=wl["cart"],
-                        opts=O.default_opts(eps_rel=eps, use_alt_energy_fix=alt))
+                        opts=O.default_opts(eps_rel=eps, use_alt_energy_fix=alt, **kw))
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
@@ -201,6 +201,134 @@ def test_vcycle_convergence_rate(oracle_lib, cfg, n):
     st = r["stats"]
     rate = (st.res / st.res0) ** (1.0 / st.iters)
     assert st.converged and rate < 0.15, rate
+
+
+def _dense_levels(O, n0, bc, rng):
+    """The 2D hierarchy 16^2 .. 2^2 as dense matrices probed from the
+    oracle's single operators: A (operator), M (one red-black sweep on the
+    error), R (restriction of a residual), P (prolongation)."""
+    grids, bs = [], []
+    n = list(n0)
+    while True:
+        g = O.make_grid(tuple(n), (1.0 / n[0], 1.0 / n[1]), bc)
+        grids.append(g)
+        if min(n) <= 2:
+            break
+        n = [m // 2 for m in n]
+    bs.append([rng.uniform(0.5, 2.0, O.face_shape(grids[0], d)) for d in range(2)])
+    for l in range(1, len(grids)):
+        fine = O.LevelView(grids[l - 1], bs[l - 1], np.zeros(O.cell_shape(grids[l - 1])),
+                           np.zeros(O.cell_shape(grids[l - 1])))
+        coarse = O.LevelView(grids[l], [np.zeros(O.face_shape(grids[l], d)) for d in range(2)],
+                             np.zeros(O.cell_shape(grids[l])), np.zeros(O.cell_shape(grids[l])))
+        O.level_coarsen(grids[l - 1], fine, coarse)
+        bs.append(coarse.b[:2])
+
+    def probe(shape, f):
+        m = int(np.prod(shape))
+        cols = []
+        for i in range(m):
+            e = np.zeros(m)
+            e[i] = 1.0
+            cols.append(np.asarray(f(e.reshape(shape))).ravel())
+        return np.array(cols).T
+
+    A, M, R, Pm = [], [], [], []
+    for l, g in enumerate(grids):
+        sh = O.cell_shape(g)
+        A.append(probe(sh, lambda e: O.apply(g, bs[l], e)))
+        M.append(probe(sh, lambda e: O.gsrb(g, bs[l], e, np.zeros(sh))))
+        if l + 1 < len(grids):
+            gc, shc = grids[l + 1], O.cell_shape(grids[l + 1])
+
+            def restrict(e):
+                fine = O.LevelView(g, bs[l], np.zeros(sh), np.zeros(sh), res=e)
+                coarse = O.LevelView(gc, bs[l + 1], np.zeros(shc), np.zeros(shc))
+                O.level_restrict(g, fine, coarse)
+                return coarse.rhs
+
+            def prolong(e):
+                fine = O.LevelView(g, bs[l], np.zeros(sh), np.zeros(sh))
+                coarse = O.LevelView(gc, bs[l + 1], e, np.zeros(shc))
+                O.level_prolong(g, coarse, fine)
+                return fine.phi
+
+            R.append(probe(sh, restrict))
+            Pm.append(probe(shc, prolong))
+    return grids, bs, A, M, R, Pm
+
+
+def test_cycle_shapes_error_propagation(oracle_lib):
+    """One V, W and F cycle of the oracle against the textbook error
+    propagation operators (Trottenberg, Oosterlee & Schueller, Multigrid,
+    sec. 2.4) assembled from dense matrices of the single operators:
+    E_l = M^nu2 (I - P (I - E_c) A_c^-1 R A_l) M^nu1, with E_c = E_{l+1}
+    (V), E_{l+1}^W E_{l+1}^W (W) or E_{l+1}^V E_{l+1}^F (F), and E = 0 on
+    the bottom level (solved exactly here).  Four levels, so the three
+    shapes differ at level 0."""
+    O = oracle_lib
+    rng = np.random.default_rng(11)
+    bc = [[P, P], [OUT, WALL]]
+    grids, bs, A, M, R, Pm = _dense_levels(O, (16, 16), bc, rng)
+    nl = len(grids)
+    assert nl == 4
+
+    def E(l, shape):
+        m = A[l].shape[0]
+        if l == nl - 1:
+            return np.zeros((m, m))
+        ev = E(l + 1, O.CYCLE_V)
+        if shape == O.CYCLE_V:
+            ec = ev
+        elif shape == O.CYCLE_W:
+            ew = E(l + 1, O.CYCLE_W)
+            ec = ew @ ew
+        else:
+            ec = ev @ E(l + 1, O.CYCLE_F)
+        mc = A[l + 1].shape[0]
+        cgc = np.eye(m) - Pm[l] @ (np.eye(mc) - ec) @ np.linalg.solve(A[l + 1], R[l] @ A[l])
+        s2 = M[l] @ M[l]
+        return s2 @ cgc @ s2
+
+    g0 = grids[0]
+    u = rng.standard_normal(O.cell_shape(g0))
+    f = O.apply(g0, bs[0], u)
+    e0 = rng.standard_normal(O.cell_shape(g0))
+    got = {}
+    for shape in (O.CYCLE_V, O.CYCLE_W, O.CYCLE_F):
+        opts = O.default_opts(eps_rel=1e-300, max_iter=1, bottom_eps=1e-13, cycle_type=shape)
+        phi, _, st, _ = O.mg_solve(g0, bs[0], u + e0, f, opts)
+        assert st.iters == 1 and st.nlev == nl
+        e1 = (phi - u).ravel()
+        want = E(0, shape) @ e0.ravel()
+        assert np.linalg.norm(e1 - want) <= 1e-9 * np.linalg.norm(want), shape
+        got[shape] = e1
+    # the three shapes are told apart by the check (their errors differ by
+    # far more than its tolerance; W and F only in the level-1 visits)
+    for a, b in ((O.CYCLE_V, O.CYCLE_W), (O.CYCLE_V, O.CYCLE_F), (O.CYCLE_W, O.CYCLE_F)):
+        assert np.linalg.norm(got[a] - got[b]) > 1e-5 * np.linalg.norm(got[a]), (a, b)
+    # an unknown shape is refused
+    opts = O.default_opts(max_iter=1, cycle_type=4)
+    assert O.mg_solve(g0, bs[0], u, f, opts)[3] != 0
+
+
+@pytest.mark.parametrize("cfg,n", [(1, (32, 32, 32)), (3, (32, 32, 32)), (0, (64, 64))])
+def test_w_and_f_cycles_converge_to_the_v_cycle_answer(oracle_lib, cfg, n):
+    """W and F cycles reach the V-cycle's answer to the tolerance, in no more
+    cycles than the V-cycle."""
+    O = oracle_lib
+    wl = W.make_workload(cfg, n=n)
+    out = {c: oracle_project(O, wl, alt=1, eps=1e-10, cycle_type=c)
+           for c in (O.CYCLE_V, O.CYCLE_W, O.CYCLE_F)}
+    singular = all(b[0] != W.OUTLET for b in wl["bc"])
+    v = out[O.CYCLE_V]
+    for c in (O.CYCLE_W, O.CYCLE_F):
+        st = out[c]["stats"]
+        assert st.converged and st.iters <= v["stats"].iters, (c, st.iters, v["stats"].iters)
+        a, b = out[c]["phi"], v["phi"]
+        if singular:
+            a, b = a - a.mean(), b - b.mean()
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), c
 
 
 def test_test_projection_2d_known_answers(oracle_lib):
