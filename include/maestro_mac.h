@@ -56,6 +56,15 @@ extern "C" {
 #define MAC_CYCLE_W 2
 #define MAC_CYCLE_V 3
 
+/* solvers of the bottom level, numbered as mg_bottom_solver (Docs/mg/mg.tex:
+ * 239-252), each between mac_opts.nub red-black sweeps: none (smoothing
+ * only), BiCGStab (FBoxLib's default) or CG, both FP64 to bottom_eps.  The
+ * hierarchy always coarsens the whole domain down to two cells per direction,
+ * as FBoxLib's agglomerating bottom solver (mg_bottom_solver = 4) does.     */
+#define MAC_BOTTOM_SMOOTH 0
+#define MAC_BOTTOM_BICGSTAB 1
+#define MAC_BOTTOM_CG 2
+
 /* precision of the V-cycle */
 #define MAC_PREC_FP64 0          /* plain FP64 V-cycle (the reference) */
 #define MAC_PREC_MIXED 1         /* FP32 V-cycle inside FP64 refinement */
@@ -90,7 +99,7 @@ typedef struct mac_opts {
     int nu1, nu2;            /* GSRB sweeps down / up; default 2, 2         */
     int nub;                 /* sweeps around the bottom solve; default 10
                                 (mg_bottom_nu, Source/_parameters:392)       */
-    double bottom_eps;       /* bottom CG relative tolerance; default 1e-3
+    double bottom_eps;       /* bottom solver's relative tolerance; default 1e-3
                                 (Source/mac_multigrid.f90:61)                */
     int bottom_max_iter;     /* default 1000 */
     int max_levels;          /* 0: coarsen down to 2 cells per direction     */
@@ -105,6 +114,9 @@ typedef struct mac_opts {
     int cycle_type;          /* MAC_CYCLE_F, _W or _V; 0 = V (mg_cycle_type,
                                 Source/_parameters:376-377, passed as
                                 cycle_type in Source/mac_multigrid.f90:60)   */
+    int bottom_solver;       /* MAC_BOTTOM_SMOOTH, _BICGSTAB or _CG; default
+                                CG (mg_bottom_solver, Source/_parameters:386,
+                                Source/mac_multigrid.f90:62)                 */
 } mac_opts;
 
 typedef struct mac_stats {

@@ -2158,6 +2158,99 @@ __device__ void blk_bottom_cg(const Lev<T> &L, double *cg, double eps, int maxit
     __syncthreads();
 }
 
+// BiCGStab (van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992) 631-644),
+// FBoxLib's default bottom solver (mg_bottom_solver = 1, mg.tex:242-243), in
+// FP64 on A x = rhs (- mean): the start, mean projection and stopping rule
+// of blk_bottom_cg; s = r - alpha v overwrites r.  cg holds 6 n doubles.
+template <int DIM, typename T>
+__device__ void blk_bottom_bicgstab(const Lev<T> &L, double *cg, double eps, int maxit, int singular,
+                                    double *sh)
+{
+    const int n = L.nx * L.ny * L.nz;
+    double *x = cg, *r = cg + n, *rh = cg + 2 * n, *p = cg + 3 * n, *v = cg + 4 * n, *t = cg + 5 * n;
+    double m = 0.0;
+    if (singular) {
+        double s = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) s += (double)L.rhs[q];
+        m = blk_sum(s, sh) / (double)n;
+    }
+    for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] = (double)L.phi[q];
+    __syncthreads();
+    blk_apply64<DIM, T>(L, x, v);
+    double rs = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        r[q] = ((double)L.rhs[q] - m) - v[q];
+        rs += r[q];
+    }
+    if (singular) {
+        double rm = blk_sum(rs, sh) / (double)n;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+    }
+    __syncthreads();
+    double a = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        a = fmax(a, fabs(r[q]));
+        rh[q] = r[q];
+    }
+    const double r0 = blk_max(a, sh);
+    // r -= c u, then the mean projection on singular levels; returns max |r|
+    auto axpy_r = [&](double c, const double *u) {
+        double rsum = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            r[q] -= c * u[q];
+            rsum += r[q];
+        }
+        if (singular) {
+            double rm = blk_sum(rsum, sh) / (double)n;
+            for (int q = threadIdx.x; q < n; q += blockDim.x) r[q] -= rm;
+        }
+        __syncthreads();
+        double am = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) am = fmax(am, fabs(r[q]));
+        return blk_max(am, sh);
+    };
+    auto dot = [&](const double *u, const double *w) {
+        double d = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) d += u[q] * w[q];
+        return blk_sum(d, sh);
+    };
+    if (r0 > 0.0) {
+        double rho_old = 1.0, al = 1.0, om = 1.0;
+        for (int it = 0; it < maxit; ++it) {
+            const double rho = dot(rh, r);
+            if (rho == 0.0) break;
+            if (it == 0) {
+                for (int q = threadIdx.x; q < n; q += blockDim.x) p[q] = r[q];
+            } else {
+                const double beta = (rho / rho_old) * (al / om);
+                for (int q = threadIdx.x; q < n; q += blockDim.x) p[q] = r[q] + beta * (p[q] - om * v[q]);
+            }
+            __syncthreads();
+            blk_apply64<DIM, T>(L, p, v);
+            const double rv = dot(rh, v);
+            if (rv == 0.0) break;
+            al = rho / rv;
+            if (axpy_r(al, v) <= eps * r0) {
+                for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] += al * p[q];
+                break;
+            }
+            blk_apply64<DIM, T>(L, r, t);
+            const double tt = dot(t, t);
+            if (!(tt > 0.0)) {
+                for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] += al * p[q];
+                break;
+            }
+            om = dot(t, r) / tt;
+            for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] += al * p[q] + om * r[q];
+            rho_old = rho;
+            if (axpy_r(om, t) <= eps * r0 || om == 0.0) break;
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += blockDim.x) L.phi[q] = (T)x[q];
+    __syncthreads();
+}
+
 // sums over the 64 lanes of a wave in a fixed order (butterfly), so every
 // lane holds the same, deterministic result
 __device__ __forceinline__ double wave_sum(double v)
@@ -2168,8 +2261,9 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // The bottom solve of a level of at most 64 cells inside one wave: nub
-// red-black sweeps, the FP64 CG of blk_bottom_cg (mean projected out on
-// singular problems), nub sweeps.  Lane q owns cell q; neighbour values come
+// red-black sweeps, the FP64 CG of blk_bottom_cg or BiCGStab of
+// blk_bottom_bicgstab (mean projected out on singular problems; neither for
+// smoothing only), nub sweeps.  Lane q owns cell q; neighbour values come
 // by lane shuffles (zero outside a non-periodic side), so the wave needs no
 // barrier.  Same updates and stopping rule as the block form; only the order
 // of the CG's sums differs.
@@ -2248,10 +2342,48 @@ __device__ void wave_bottom(const Lev<T> &L, const CoarseOpts &o)
         const double rm = wave_sum(r) / nd;
         if (live) r -= rm;
     }
-    double p = r;
     const double r0 = wave_max(fabs(r));
-    double rr = wave_sum(r * r);
-    if (r0 > 0.0) {
+    if (r0 > 0.0 && o.bottom == MAC_BOTTOM_BICGSTAB) {
+        // the BiCGStab of blk_bottom_bicgstab, one cell per lane
+        const double rh = r;
+        double p = 0.0, v = 0.0, rho_old = 1.0, al = 1.0, om = 1.0;
+        auto project = [&]() {
+            if (o.singular) {
+                const double rm = wave_sum(r) / nd;
+                if (live) r -= rm;
+            }
+        };
+        for (int it = 0; it < o.bottom_maxit; ++it) {
+            const double rho = wave_sum(rh * r);
+            if (rho == 0.0) break;
+            if (it == 0) p = r;
+            else p = r + ((rho / rho_old) * (al / om)) * (p - om * v);
+            v = apply(p);
+            const double rv = wave_sum(rh * v);
+            if (rv == 0.0) break;
+            al = rho / rv;
+            r -= al * v;
+            project();
+            if (wave_max(fabs(r)) <= o.bottom_eps * r0) {
+                x += al * p;
+                break;
+            }
+            const double t = apply(r);
+            const double tt = wave_sum(t * t);
+            if (!(tt > 0.0)) {
+                x += al * p;
+                break;
+            }
+            om = wave_sum(t * r) / tt;
+            x += al * p + om * r;
+            r -= om * t;
+            project();
+            rho_old = rho;
+            if (wave_max(fabs(r)) <= o.bottom_eps * r0 || om == 0.0) break;
+        }
+    } else if (r0 > 0.0 && o.bottom == MAC_BOTTOM_CG) {
+        double p = r;
+        double rr = wave_sum(r * r);
         for (int it = 0; it < o.bottom_maxit; ++it) {
             const double Ap = apply(p);
             const double pap = wave_sum(p * Ap);
@@ -2368,7 +2500,8 @@ __device__ Lev<T> coarse_level(const Lev<T> *levs, int lc, int l, char *smem)
 }
 
 // the bottom level: one wave for at most 64 cells (one cell per lane in
-// registers, no block barriers), else nub sweeps, the block CG, nub sweeps
+// registers, no block barriers), else nub sweeps, the block CG or BiCGStab
+// (none for smoothing only), nub sweeps
 template <int DIM, typename T>
 __device__ __forceinline__ void coarse_bottom(const Lev<T> &L, const CoarseOpts &o, double *cg, double *sh)
 {
@@ -2377,7 +2510,10 @@ __device__ __forceinline__ void coarse_bottom(const Lev<T> &L, const CoarseOpts 
         __syncthreads();
     } else {
         for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
-        blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
+        if (o.bottom == MAC_BOTTOM_CG)
+            blk_bottom_cg<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
+        else if (o.bottom == MAC_BOTTOM_BICGSTAB)
+            blk_bottom_bicgstab<DIM, T>(L, cg, o.bottom_eps, o.bottom_maxit, o.singular, sh);
         for (int s = 0; s < o.nub; ++s) blk_gsrb<DIM, T>(L);
     }
 }
@@ -3473,7 +3609,7 @@ static long coarse_lds_bytes(int dim, const Lev<T> *levs_host, int lc, int nlev)
         b += dim == 3 ? lds_level_bytes<3, T>(L.nx, L.ny, L.nz) : lds_level_bytes<2, T>(L.nx, L.ny, 1);
     }
     const Lev<T> &B = levs_host[nlev - 1];
-    return b + 4L * B.nx * B.ny * B.nz * (long)sizeof(double);
+    return b + (long)BOTTOM_VECS * B.nx * B.ny * B.nz * (long)sizeof(double);
 }
 
 static constexpr long COARSE_LDS_MAX = 64 * 1024;

@@ -53,8 +53,12 @@ struct FusedGrid {
 
 enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2 };
 
+// FP64 vectors of the bottom solver's scratch (BiCGStab: x r r^ p v t)
+static constexpr int BOTTOM_VECS = 6;
+
 struct CoarseOpts {
     int cycle;               // MAC_CYCLE_F, _W or _V
+    int bottom;              // MAC_BOTTOM_SMOOTH, _BICGSTAB or _CG
     int nu1, nu2, nub;
     double bottom_eps;
     int bottom_maxit;

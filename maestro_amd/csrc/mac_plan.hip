@@ -162,6 +162,7 @@ extern "C" int mac_default_opts(mac_opts *o)
     o->nub = 10;
     o->bottom_eps = 1e-3;
     o->bottom_max_iter = 1000;
+    o->bottom_solver = MAC_BOTTOM_CG;
     o->max_levels = 0;
     o->use_alt_energy_fix = 1;
     o->precision = MAC_PREC_FP64;
@@ -253,13 +254,13 @@ static int build_hier(mac_plan *P, Hier<T> &H)
     H.allocs.push_back(q);
     H.lev_dev = (Lev<T> *)q;
     HIPCHK(hipMemcpy(H.lev_dev, H.lev.data(), sizeof(Lev<T>) * nl, hipMemcpyHostToDevice));
-    // bottom CG scratch
+    // bottom solver scratch (doubles)
     const Lev<T> &B = H.lev[nl - 1];
     long nb = (long)B.nx * B.ny * B.nz;
-    if (4 * nb > P->cg_cap) {
+    if (BOTTOM_VECS * nb > P->cg_cap) {
         if (P->cg) (void)hipFree(P->cg);
-        HIPCHK(hipMalloc(&P->cg, sizeof(double) * 4 * nb));
-        P->cg_cap = 4 * nb;
+        HIPCHK(hipMalloc(&P->cg, sizeof(double) * BOTTOM_VECS * nb));
+        P->cg_cap = BOTTOM_VECS * nb;
     }
     H.built = true;
     return MAC_OK;
@@ -370,6 +371,7 @@ static CoarseOpts coarse_opts(const mac_plan *P, int type)
     const mac_opts &o = P->opts;
     CoarseOpts co;
     co.cycle = type;
+    co.bottom = o.bottom_solver;
     co.nu1 = o.nu1; co.nu2 = o.nu2; co.nub = o.nub;
     co.bottom_eps = o.bottom_eps;
     co.bottom_maxit = o.bottom_max_iter;
@@ -587,6 +589,7 @@ static int solve(mac_plan *P, mac_stats *st)
     const mac_opts &o = P->opts;
     if (o.precision < MAC_PREC_FP64 || o.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
     if (o.cycle_type < 0 || o.cycle_type > MAC_CYCLE_V) return MAC_ERR_INVALID;
+    if (o.bottom_solver < MAC_BOTTOM_SMOOTH || o.bottom_solver > MAC_BOTTOM_CG) return MAC_ERR_INVALID;
     double r0 = 0, r = 0;
     int it = 0, inner = 0;
     HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));

@@ -91,7 +91,7 @@ module macproject_module
   private
   public :: macproject, macproject_release, mac_plan_ensure, fail
   public :: use_alt_energy_fix, mac_precision, mg_verbose, eps_mac, eps_mac_max, &
-            mac_level_factor, mg_cycle_type, mg_bottom_nu, last_stats, c_mac_stats, &
+            mac_level_factor, mg_cycle_type, mg_bottom_nu, mg_bottom_solver, last_stats, c_mac_stats, &
             c_mac_opts, mac_default_opts
 
   ! runtime knobs (Source/_parameters:370-392,552-554, Source/mg_eps.f90:22-28)
@@ -101,6 +101,8 @@ module macproject_module
   integer, save :: mg_verbose = 0
   integer, save :: mg_cycle_type = 3            ! 1 F-cycle, 2 W-cycle, 3 V-cycle
   integer, save :: mg_bottom_nu = 10            ! sweeps around the bottom solve
+  integer, save :: mg_bottom_solver = -1        ! 0 smoothing only, 1 BiCGStab, 2 CG; -1 and 4
+                                                ! (agglomerating, always on here): the library's CG
   real(dp_t), save :: eps_mac = 1.d-10
   real(dp_t), save :: eps_mac_max = 1.d-8
   real(dp_t), save :: mac_level_factor = 10.d0
@@ -119,7 +121,7 @@ module macproject_module
      integer(c_int) :: max_iter, nu1, nu2, nub
      real(c_double) :: bottom_eps
      integer(c_int) :: bottom_max_iter, max_levels, use_alt_energy_fix, precision, &
-                       inner_cycles, device_ptrs, verbose, cycle_type
+                       inner_cycles, device_ptrs, verbose, cycle_type, bottom_solver
   end type c_mac_opts
 
   type, bind(C) :: c_mac_stats
@@ -213,6 +215,7 @@ contains
     o%verbose = mg_verbose
     o%cycle_type = mg_cycle_type
     o%nub = mg_bottom_nu
+    if (mg_bottom_solver /= -1 .and. mg_bottom_solver /= 4) o%bottom_solver = mg_bottom_solver
     o%device_ptrs = 0
 
     call mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
@@ -341,7 +344,8 @@ module mac_multigrid_module
   use iso_c_binding
   use mac_types_module
   use macproject_module, only: mac_plan_ensure, fail, c_mac_opts, c_mac_stats, mac_default_opts, &
-                               mac_precision, mg_verbose, mg_cycle_type, mg_bottom_nu, last_stats
+                               mac_precision, mg_verbose, mg_cycle_type, mg_bottom_nu, &
+                               mg_bottom_solver, last_stats
   implicit none
 
   private
@@ -414,6 +418,7 @@ contains
     o%verbose = mg_verbose
     o%cycle_type = mg_cycle_type
     o%nub = mg_bottom_nu
+    if (mg_bottom_solver /= -1 .and. mg_bottom_solver /= 4) o%bottom_solver = mg_bottom_solver
     o%device_ptrs = 0
     ! a cached projection plan serves as it is; a new one gets the projection's usual widths
     if (c_associated(mla%plan)) then

@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libmaestro_mac.so")
 PERIODIC, INTERIOR, INLET, OUTLET, SYMMETRY, SLIP_WALL, NO_SLIP_WALL = -1, 0, 11, 12, 13, 14, 15
 PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
 CYCLE_F, CYCLE_W, CYCLE_V = 1, 2, 3                # mg_cycle_type (Source/_parameters:376-377)
+BOTTOM_SMOOTH, BOTTOM_BICGSTAB, BOTTOM_CG = 0, 1, 2  # mg_bottom_solver (Docs/mg/mg.tex:239-252)
 ELL_PER, ELL_DIR, ELL_NEU = -1, 1, 2               # elliptic boundary kinds (mac_solve)
 COMP_PRESS, COMP_SCALAR, COMP_EXTRAP = 0, 1, 2     # component classes of mac_ell_bc
 ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
@@ -67,7 +68,7 @@ class COpts(C.Structure):
                 ("bottom_eps", C.c_double), ("bottom_max_iter", C.c_int),
                 ("max_levels", C.c_int), ("use_alt_energy_fix", C.c_int),
                 ("precision", C.c_int), ("inner_cycles", C.c_int), ("device_ptrs", C.c_int),
-                ("verbose", C.c_int), ("cycle_type", C.c_int)]
+                ("verbose", C.c_int), ("cycle_type", C.c_int), ("bottom_solver", C.c_int)]
 
 
 class CSphr(C.Structure):

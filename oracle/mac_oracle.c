@@ -574,6 +574,83 @@ static void bottom_cg(const orc_grid *g, orc_level *lev, double eps, int maxit,
     free(r); free(p); free(Ap);
 }
 
+/* BiCGStab (van der Vorst, SIAM J. Sci. Stat. Comput. 13 (1992) 631-644),
+ * FBoxLib's default bottom solver (mg_bottom_solver = 1, mg.tex:242-243).
+ * Same start, mean projection and stopping rule as bottom_cg; s = r -
+ * alpha v overwrites r.                                                   */
+static void bottom_bicgstab(const orc_grid *g, orc_level *lev, double eps, int maxit,
+                            int singular)
+{
+    long nc = ncells(lev->n);
+    if (singular) {
+        double m = orc_sum(lev->rhs, nc) / (double)nc;
+        for (long q = 0; q < nc; ++q) lev->rhs[q] -= m;
+    }
+    double *r = malloc(sizeof(double) * nc), *rh = malloc(sizeof(double) * nc),
+           *p = calloc(nc, sizeof(double)), *v = calloc(nc, sizeof(double)),
+           *t = malloc(sizeof(double) * nc);
+    double r0 = orc_residual(g, lev, lev->phi, lev->rhs, r);
+    if (singular) project_mean(r, nc);
+    if (r0 > 0.0) {
+        memcpy(rh, r, sizeof(double) * nc);
+        double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+        for (int it = 0; it < maxit; ++it) {
+            double rho = dot(rh, r, nc);
+            if (rho == 0.0) break;
+            if (it == 0) {
+                memcpy(p, r, sizeof(double) * nc);
+            } else {
+                double beta = (rho / rho_old) * (alpha / omega);
+                for (long q = 0; q < nc; ++q) p[q] = r[q] + beta * (p[q] - omega * v[q]);
+            }
+            orc_apply(g, lev, p, v);
+            double rv = dot(rh, v, nc);
+            if (rv == 0.0) break;
+            alpha = rho / rv;
+            double smax = 0.0;
+            for (long q = 0; q < nc; ++q) r[q] -= alpha * v[q];
+            if (singular) project_mean(r, nc);
+            for (long q = 0; q < nc; ++q)
+                if (fabs(r[q]) > smax) smax = fabs(r[q]);
+            if (smax <= eps * r0) {
+                for (long q = 0; q < nc; ++q) lev->phi[q] += alpha * p[q];
+                break;
+            }
+            orc_apply(g, lev, r, t);
+            double tt = dot(t, t, nc);
+            if (!(tt > 0.0)) {
+                for (long q = 0; q < nc; ++q) lev->phi[q] += alpha * p[q];
+                break;
+            }
+            omega = dot(t, r, nc) / tt;
+            double rmax = 0.0;
+            for (long q = 0; q < nc; ++q) {
+                lev->phi[q] += alpha * p[q] + omega * r[q];
+                r[q] -= omega * t[q];
+            }
+            if (singular) project_mean(r, nc);
+            for (long q = 0; q < nc; ++q)
+                if (fabs(r[q]) > rmax) rmax = fabs(r[q]);
+            rho_old = rho;
+            if (rmax <= eps * r0 || omega == 0.0) break;
+        }
+    }
+    free(r); free(rh); free(p); free(v); free(t);
+}
+
+/* the bottom level: nub sweeps, the chosen solver (none for smoothing
+ * only, mg.tex:239-240), nub sweeps                                        */
+static void bottom_solve(orc_mg *mg, orc_level *L)
+{
+    const orc_grid *g = &mg->g;
+    for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
+    if (mg->opts.bottom_solver == ORC_BOTTOM_CG)
+        bottom_cg(g, L, mg->opts.bottom_eps, mg->opts.bottom_maxit, mg->singular);
+    else if (mg->opts.bottom_solver == ORC_BOTTOM_BICGSTAB)
+        bottom_bicgstab(g, L, mg->opts.bottom_eps, mg->opts.bottom_maxit, mg->singular);
+    for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
+}
+
 /* -------------------------------------------------------- MG hierarchy */
 
 int orc_mg_nlevels(const orc_grid *g, int max_levels)
@@ -665,9 +742,7 @@ static void cycle(orc_mg *mg, int l, int type)
     const orc_grid *g = &mg->g;
     orc_level *L = &mg->lev[l];
     if (l == mg->nlev - 1) {
-        for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
-        bottom_cg(g, L, mg->opts.bottom_eps, mg->opts.bottom_maxit, mg->singular);
-        for (int s = 0; s < mg->opts.nub; ++s) orc_gsrb(g, L);
+        bottom_solve(mg, L);
         return;
     }
     for (int s = 0; s < mg->opts.nu1; ++s) orc_gsrb(g, L);
@@ -719,6 +794,8 @@ int orc_mg_solve(orc_mg *mg, double *phi, double *rh, orc_stats *st)
     orc_level *F = &mg->lev[0];
     long nc = ncells(F->n);
     if (mg->opts.cycle_type < 0 || mg->opts.cycle_type > ORC_CYCLE_V) return ORC_ERR_INVALID;
+    if (mg->opts.bottom_solver < ORC_BOTTOM_SMOOTH || mg->opts.bottom_solver > ORC_BOTTOM_CG)
+        return ORC_ERR_INVALID;
     if (mg->singular) {
         double m = orc_sum(rh, nc) / (double)nc;
         for (long q = 0; q < nc; ++q) rh[q] -= m;
@@ -935,6 +1012,7 @@ void orc_default_opts(orc_opts *o)
     o->nub = 10;            /* mg_bottom_nu, _parameters:392 */
     o->bottom_eps = 1e-3;   /* mac_multigrid.f90:61 */
     o->bottom_maxit = 1000;
+    o->bottom_solver = ORC_BOTTOM_CG;
     o->max_levels = 0;
     o->use_alt_energy_fix = 1;  /* _parameters:554 */
 }

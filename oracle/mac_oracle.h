@@ -28,6 +28,8 @@ enum { ORC_OK = 0, ORC_ERR_INVALID = 1, ORC_ERR_NOT_CONVERGED = 4 };
 
 /* cycle shapes, numbered as mg_cycle_type (Source/_parameters:376-377) */
 enum { ORC_CYCLE_F = 1, ORC_CYCLE_W = 2, ORC_CYCLE_V = 3 };
+/* bottom solvers, numbered as mg_bottom_solver (Docs/mg/mg.tex:239-252) */
+enum { ORC_BOTTOM_SMOOTH = 0, ORC_BOTTOM_BICGSTAB = 1, ORC_BOTTOM_CG = 2 };
 
 typedef struct {
     int dim;          /* 2 or 3 */
@@ -44,6 +46,7 @@ typedef struct {
     int max_levels;   /* 0 = coarsen as far as possible */
     int use_alt_energy_fix;
     int cycle_type;   /* ORC_CYCLE_F/W/V (mg_cycle_type); 0 = V */
+    int bottom_solver; /* ORC_BOTTOM_SMOOTH/BICGSTAB/CG (mg_bottom_solver) */
 } orc_opts;
 
 typedef struct {

@@ -25,6 +25,7 @@ MAX_HIST = 256
 
 
 CYCLE_F, CYCLE_W, CYCLE_V = 1, 2, 3   # mg_cycle_type numbering
+BOTTOM_SMOOTH, BOTTOM_BICGSTAB, BOTTOM_CG = 0, 1, 2   # mg_bottom_solver numbering
 
 
 class Grid(C.Structure):
@@ -37,7 +38,8 @@ class Opts(C.Structure):
                 ("max_iter", C.c_int), ("nu1", C.c_int), ("nu2", C.c_int),
                 ("nub", C.c_int), ("bottom_eps", C.c_double),
                 ("bottom_maxit", C.c_int), ("max_levels", C.c_int),
-                ("use_alt_energy_fix", C.c_int), ("cycle_type", C.c_int)]
+                ("use_alt_energy_fix", C.c_int), ("cycle_type", C.c_int),
+                ("bottom_solver", C.c_int)]
 
 
 class Stats(C.Structure):

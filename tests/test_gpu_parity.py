@@ -259,3 +259,41 @@ def test_unknown_cycle_type_is_refused(gpu):
     wl = W.make_workload(1, n=(32, 32, 32), box=16)
     with pytest.raises(mac.MacError):
         run_gpu(wl, cycle_type=4)
+
+
+BOTTOM_CASES = [(b, cfg, n, box, prec)
+                for cfg, n, box, prec in [(1, (64, 64, 64), 32, 0), (0, (64, 64), 16, 0),
+                                          (1, (96, 80, 48), 16, 0),   # 90-cell bottom: block form
+                                          (3, (96, 80, 48), 16, 1), (2, (128, 64, 64), 32, 1)]
+                for b in (0, 1)
+                if not (b == 0 and cfg == 3)]   # smoothing only stalls on the star's contrast
+
+
+@pytest.mark.parametrize("bottom,cfg,n,box,prec", BOTTOM_CASES)
+def test_bottom_solvers_match_oracle(gpu, bottom, cfg, n, box, prec):
+    """mg_bottom_solver 0 (smoothing only) and 1 (BiCGStab) against the
+    oracle with the same bottom solver (FP64: the same iterates up to the
+    bottom solve's summation order), and the answer against the truth."""
+    wl = W.make_workload(cfg, n=n, box=box)
+    truth = run_oracle(wl, eps=1e-13, alt=1)
+    same = run_oracle(wl, eps=1e-10, alt=1, bottom_solver=bottom)
+    out = run_gpu(wl, use_alt_energy_fix=1, precision=prec, bottom_solver=bottom)
+    st = out["stats"]
+    print(f"bottom {bottom} cfg{cfg} n={n} prec={prec} iters={st['iters']} (oracle {same['stats'].iters})")
+    assert st["converged"]
+    singular = all(b[0] != W.OUTLET for b in wl["bc"])
+    dim = wl["dim"]
+    assert rel_err(out["phi"], truth["phi"], remove_mean=singular) <= 1e-8
+    e_u = max(rel_err(out["faces"][d], truth["faces"][d]) for d in range(dim))
+    e_u_ref = max(rel_err(same["faces"][d], truth["faces"][d]) for d in range(dim))
+    assert e_u <= max(1e-8, 2.0 * e_u_ref)
+    if prec == 0:
+        assert abs(st["iters"] - same["stats"].iters) <= 1
+        assert rel_err(out["phi"], same["phi"], remove_mean=singular) <= 1e-9
+
+
+def test_unknown_bottom_solver_is_refused(gpu):
+    from maestro_amd import mac
+    wl = W.make_workload(1, n=(32, 32, 32), box=16)
+    with pytest.raises(mac.MacError):
+        run_gpu(wl, bottom_solver=3)

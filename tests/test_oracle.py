@@ -331,6 +331,87 @@ def test_w_and_f_cycles_converge_to_the_v_cycle_answer(oracle_lib, cfg, n):
         assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), c
 
 
+def _one_level(O, bc, rng, n=(8, 8)):
+    g = O.make_grid(n, tuple(1.0 / m for m in n), bc)
+    b = [rng.uniform(0.5, 2.0, O.face_shape(g, d)) for d in range(len(n))]
+    for d in range(len(n)):   # one weight on a periodic seam face (a symmetric operator)
+        if bc[d][0] == P:
+            sl = [slice(None)] * 3
+            sl[2 - d] = -1
+            first = [slice(None)] * 3
+            first[2 - d] = 0
+            b[d][tuple(sl)] = b[d][tuple(first)]
+    sh = O.cell_shape(g)
+    m = int(np.prod(sh))
+    A = np.array([O.apply(g, b, np.eye(m)[i].reshape(sh)).ravel() for i in range(m)]).T
+    return g, b, sh, A
+
+
+@pytest.mark.parametrize("solver", ["cg", "bicgstab"])
+def test_bottom_solvers_first_step_and_exact_solve(oracle_lib, solver):
+    """The bottom solvers on a one-level hierarchy (the bottom solve alone,
+    no sweeps): one step is the textbook first step -- CG: x + a r with
+    a = r.r / r.Ar; BiCGStab (van der Vorst 1992): x + a r + w s with
+    s = r - a Ar, w = As.s / As.As -- and to a tight tolerance they reach
+    the dense direct solution (mean-free where the problem is singular)."""
+    O = oracle_lib
+    code = O.BOTTOM_CG if solver == "cg" else O.BOTTOM_BICGSTAB
+    rng = np.random.default_rng(5)
+    for bc in ([[P, P], [OUT, WALL]], [[P, P], [WALL, WALL]]):
+        g, b, sh, A = _one_level(O, bc, rng)
+        singular = bc[1][0] != OUT
+        f = rng.standard_normal(sh)
+        if singular:
+            f -= f.mean()
+        x0 = rng.standard_normal(sh)
+        base = dict(eps_rel=1e-300, max_iter=1, max_levels=1, nub=0, bottom_solver=code)
+        x1, _, st, _ = O.mg_solve(g, b, x0, f, O.default_opts(bottom_maxit=1, bottom_eps=1e-300, **base))
+        assert st.nlev == 1
+        r = f.ravel() - A @ x0.ravel()
+        if singular:
+            r -= r.mean()
+        Ar = A @ r
+        a = (r @ r) / (r @ Ar)
+        want = x0.ravel() + a * r
+        if solver == "bicgstab":
+            s_ = r - a * Ar
+            if singular:
+                s_ -= s_.mean()
+            As = A @ s_
+            want = want + ((As @ s_) / (As @ As)) * s_
+        assert np.linalg.norm(x1.ravel() - want) <= 1e-12 * np.linalg.norm(want), bc
+        xs, _, st, _ = O.mg_solve(g, b, x0, f, O.default_opts(bottom_maxit=200, bottom_eps=1e-13, **base))
+        if singular:
+            exact = np.linalg.lstsq(A, f.ravel(), rcond=None)[0]
+            got = xs.ravel() - xs.mean()
+            exact -= exact.mean()
+        else:
+            exact = np.linalg.solve(A, f.ravel())
+            got = xs.ravel()
+        # residual 1e-13 relative, condition number ~250
+        assert np.linalg.norm(got - exact) <= 1e-9 * np.linalg.norm(exact), bc
+
+
+@pytest.mark.parametrize("cfg,n", [(1, (32, 32, 32)), (3, (32, 32, 32)), (0, (64, 64))])
+def test_bottom_solvers_reach_the_same_answer(oracle_lib, cfg, n):
+    """Smoothing only, BiCGStab and CG at the bottom of the V-cycle reach the
+    same answer to the tolerance.  (Smoothing only stalls near 2e-4 on the
+    star, whose bottom level carries the 1e6 coefficient contrast: 2 x 10
+    sweeps do not solve it.)"""
+    O = oracle_lib
+    wl = W.make_workload(cfg, n=n)
+    solvers = (O.BOTTOM_BICGSTAB,) if cfg == 3 else (O.BOTTOM_SMOOTH, O.BOTTOM_BICGSTAB)
+    out = {c: oracle_project(O, wl, alt=1, eps=1e-10, bottom_solver=c) for c in solvers + (O.BOTTOM_CG,)}
+    singular = all(bb[0] != W.OUTLET for bb in wl["bc"])
+    ref = out[O.BOTTOM_CG]["phi"]
+    for c in solvers:
+        assert out[c]["stats"].converged, c
+        a, b = out[c]["phi"], ref
+        if singular:
+            a, b = a - a.mean(), b - b.mean()
+        assert np.linalg.norm(a - b) <= 1e-8 * np.linalg.norm(b), c
+
+
 def test_test_projection_2d_known_answers(oracle_lib):
     """Exec/UNIT_TESTS/test_projection: the divergence-free field plus the
     gradient of a scalar projects back onto the field (test_projection.f90:
