@@ -228,13 +228,19 @@ def test_physical_edge_velocity_rules(gpu):
     (0, (64, 64), 16, 0),          # 2D: the two-colour kernels
     (2, (128, 64, 64), 32, 1),
     (3, (128, 64, 64), 32, 2),
+    # the star at sizes where the V-cycle diverges (8^3) or crawls (16^3):
+    # every level inside the single-workgroup coarse cycle
+    (3, (16, 16, 16), 8, 0),
+    (3, (8, 8, 8), 8, 0),
 ])
 def test_w_and_f_cycles_match_oracle(gpu, cycle, cfg, n, box, prec):
     """mg_cycle_type 1 (F) and 2 (W): the GPU cycle against the oracle's
     cycle of the same shape (FP64: the same iterates to the bottom solver's
     summation order), and the answer against the 1e-13 truth."""
     wl = W.make_workload(cfg, n=n, box=box)
-    truth = run_oracle(wl, eps=1e-13, alt=1)
+    # the truth by the same shape: the V-cycle diverges on the 8^3 star
+    truth = run_oracle(wl, eps=1e-13, alt=1, cycle_type=cycle)
+    assert truth["stats"].converged
     same = run_oracle(wl, eps=1e-10, alt=1, cycle_type=cycle)
     out = run_gpu(wl, use_alt_energy_fix=1, precision=prec, cycle_type=cycle)
     st = out["stats"]
