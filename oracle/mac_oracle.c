@@ -35,8 +35,14 @@
  *   orc_gsrb            fixed point of the exact solution; zero black residual
  *   restriction/prolong conservation; exactness on linear/constant fields
  *   orc_mg_solve        scipy direct-solve fixtures; residual invariant
+ *   cycle shapes V/W/F  the textbook error-propagation operators assembled
+ *                       from dense matrices of the single operators
+ *   bottom CG/BiCGStab  the textbook first step; the dense direct solution
  *   orc_mkumac          D(U_proj)=macrhs; test_projection exact recovery;
  *                       idempotence
+ * Parity unpinned (only FBoxLib could fix them): iteration counts and
+ * iterates, FBoxLib's own F/W cycle shapes and BiCGStab details, the
+ * stencil_order = 2 Dirichlet stencil, the coarse-coefficient rule.
  */
 #include <math.h>
 #include <stdint.h>
