@@ -95,7 +95,7 @@ def stage(per_box, copies, torch):
     return sets, dr
 
 
-def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=1, profile=True):
+def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=0, profile=True):
     plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner)
     stats = []
     for w in range(warmup):
@@ -190,7 +190,7 @@ def time_helmholtz(plan, n, box, torch, reps=3):
     out = {"workload": f"3D {n[0]}^3 Helmholtz (alpha - D beta G) phi = rh, {len(boxes)} boxes, x/y periodic, "
                        "z Dirichlet/Neumann, alpha/(beta/h^2) ~ 0.05, rel residual 1e-10"}
     for name, prec in (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED)):
-        plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=1)
+        plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=0)
         ts, st = [], None
         for _ in range(reps + 1):
             phi = [a.clone() for a in phi0]
@@ -246,8 +246,9 @@ def main():
                     help="skip timing make_macrhs / make_s0mac and the Helmholtz mac_solve on the bench layout")
     ap.add_argument("--headline", default="mixed", choices=("mixed", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
-    ap.add_argument("--inner-cycles", type=int, default=1,
-                    help="FP32 V-cycles per refinement step of the mixed path")
+    ap.add_argument("--inner-cycles", type=int, default=0,
+                    help="FP32 V-cycles per refinement step of the mixed paths "
+                         "(0: the library's default, 3 with FP32 storage, 2 with bfloat16 storage)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (the PMC byte-counter "
                          "calibration: 1 GiB read + 1 GiB written per launch)")
@@ -411,7 +412,9 @@ def main():
         "mixed_over_fp64": round(fp64["seconds"] / mixed["seconds"], 3),
         "compact_bf16_over_fp64": round(fp64["seconds"] / results["mixed_bf16"]["seconds"], 3),
         "iters": {"fp64_vcycles": fp64["stats"][-1]["iters"],
-                  **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"] for p in ("mixed", "mixed_bf16")}},
+                  **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"] for p in ("mixed", "mixed_bf16")},
+                  **{f"{p}_fp32_vcycles": results[p]["stats"][-1]["inner_cycles"]
+                     for p in ("mixed", "mixed_bf16")}},
         "roofline": sweep_roofline(head),
         "roofline_other": {p: sweep_roofline(p) for p in results if p != head},
         "kernel_ms_per_solve": {

@@ -107,7 +107,9 @@ typedef struct mac_opts {
                                 Source/_parameters:552-554); 0: beta0/rho    */
     int precision;           /* MAC_PREC_FP64, MAC_PREC_MIXED or
                                 MAC_PREC_MIXED_BF16                          */
-    int inner_cycles;        /* FP32 V-cycles per refinement step (mixed)    */
+    int inner_cycles;        /* FP32 V-cycles per refinement step (mixed);
+                                default 0: 3 with FP32 storage, 2 with
+                                bfloat16 storage                             */
     int device_ptrs;         /* 1: every array pointer is a device pointer;
                                 0: host pointers (copied in and out)         */
     int verbose;

@@ -166,7 +166,7 @@ extern "C" int mac_default_opts(mac_opts *o)
     o->max_levels = 0;
     o->use_alt_energy_fix = 1;
     o->precision = MAC_PREC_FP64;
-    o->inner_cycles = 1;
+    o->inner_cycles = 0;
     o->device_ptrs = 0;
     o->verbose = 0;
     return MAC_OK;
@@ -582,6 +582,19 @@ static bool converged(const mac_opts &o, double r, double r0)
     return false;
 }
 
+// FP32 V-cycles per refinement step; 0 picks by storage.  Three cycles
+// (error reduced by ~1e-3 per step at the V-cycle's rate) replace two FP64
+// update+residual passes of 56 B/cell each by FP32 passes inside the cycle;
+// with bfloat16 storage the rounded operator limits a step's reduction to
+// ~2^-9, so a third cycle buys nothing and two are used.  Measured at 256^3
+// (9 V-cycles in all): FP32 storage 1/2/3 cycles per step 92.5/93.0/105.0
+// solves/s; bfloat16 storage 115.2/117.6/87.6 solves/s.
+static int inner_per_step(const mac_opts &o)
+{
+    if (o.inner_cycles > 0) return o.inner_cycles;
+    return o.precision == MAC_PREC_MIXED_BF16 ? 2 : 3;
+}
+
 // Solve A phi = rhs on level 0 of H64 (phi holds the warm start).
 static int solve(mac_plan *P, mac_stats *st)
 {
@@ -613,7 +626,7 @@ static int solve(mac_plan *P, mac_stats *st)
         r = r0;
         const Lev<float> &E = P->H32.lev[0];
         while (!converged(o, r, r0) && it < o.max_iter) {
-            for (int c = 0; c < std::max(1, o.inner_cycles); ++c) {
+            for (int c = 0; c < inner_per_step(o); ++c) {
                 vcycle<float>(P, P->H32, c == 0, K);
                 ++inner;
             }

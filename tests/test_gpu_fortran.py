@@ -193,5 +193,15 @@ def test_fortran_mac_multigrid_matches_binding_and_oracle(gpu, oracle_lib, tmp_p
         assert np.array_equal(a, c)
     ref, st_o, rc = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-10))
     assert rc == 0
+    truth, _, rc = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-14))
+    assert rc == 0
     got = BX.gather_cells(fp, boxes, ng_phi, n3, dim)
-    assert rel_err(got, ref[(slice(1, -1),) * dim].reshape(got.shape)) <= 1e-8
+    inner = (slice(1, -1),) * dim
+    same, truth = ref[inner].reshape(got.shape), truth[inner].reshape(got.shape)
+    # a 1e-10 residual from a rough start leaves ~6e-8 in phi here (the oracle's
+    # own 1e-10 solve included): the mixed path is held to twice that error,
+    # the FP64 path (the oracle's iterates) to the oracle's answer
+    e_ref = rel_err(same, truth)
+    assert rel_err(got, truth) <= max(1e-8, 2.0 * e_ref)
+    if prec == 0:
+        assert rel_err(got, same) <= 1e-8

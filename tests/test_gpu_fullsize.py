@@ -137,10 +137,17 @@ def test_star_512_mixed_and_fp64(gpu):
         res[prec] = (faces, ph)
         del faces, ph
     (fmx, pmx), (f64, p64) = res[mac.PREC_MIXED], res[mac.PREC_FP64]
-    e_phi = float((pmx - p64).norm() / p64.norm())
-    e_u = max(float((fmx[d] - f64[d]).norm() / f64[d].norm()) for d in range(3))
-    print(f"  mixed vs FP64: phi {e_phi:.2e}  U {e_u:.2e}")
-    assert e_phi <= 1e-8 and e_u <= 1e-8
+    # the truth: the FP64 solve to 1e-12.  With the 1e6 contrast a 1e-10
+    # residual leaves ~1e-7 in phi, so each 1e-10 answer is held to a small
+    # multiple of the FP64 path's own error rather than to the other answer
+    ft, pt, st = project_device(torch, wl, mac.PREC_FP64, eps_rel=1e-12)
+    assert st["converged"]
+    err = lambda f, p: (float((p - pt).norm() / pt.norm()),
+                        max(float((f[d] - ft[d]).norm() / ft[d].norm()) for d in range(3)))
+    e64, emx = err(f64, p64), err(fmx, pmx)
+    print(f"  vs the 1e-12 solve: FP64 phi {e64[0]:.2e} U {e64[1]:.2e}; mixed phi {emx[0]:.2e} U {emx[1]:.2e}")
+    for k in range(2):
+        assert e64[k] <= 1e-6 and emx[k] <= max(1e-8, 3.0 * e64[k])
 
 
 def test_adjoint_256_matches_finite_differences(gpu, oracle_lib):

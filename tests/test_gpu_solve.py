@@ -237,10 +237,23 @@ def test_solve_anisotropic_cells(gpu, oracle_lib, n, box, ell, prec):
     plan = mac.MacPlan(lay, mac.default_opts(precision=prec))
     st, out = gpu_solve(gpu, plan, rhb, phb, ab, bb, ell, True, eps_rel=1e-10)
     plan.close()
-    assert st["converged"] and abs(st["iters"] - st_o.iters) <= (0 if prec == 0 else 1)
+    assert st["converged"]
+    if prec == 0:
+        assert st["iters"] == st_o.iters
+    else:
+        # refinement steps of several FP32 cycles: the cycle total matches the
+        # oracle's V-cycle count up to the last step's overshoot
+        assert st_o.iters - 1 <= st["inner_cycles"] <= st_o.iters + 6
     nn = list(n) + [1] * (3 - dim)
     got = BX.gather_cells(out, boxes, 1, nn, dim)
-    ref = same[(slice(1, -1),) * dim].reshape(got.shape)
-    e = rel_err(got, ref)
-    print(f"n={n} dx={dx} prec={prec} iters={st['iters']} (oracle {st_o.iters}) vs oracle: {e:.2e}")
-    assert e <= (1e-13 if prec == 0 else 1e-9)
+    inner = lambda a: a[(slice(1, -1),) * dim].reshape(got.shape)
+    e = rel_err(got, inner(same))
+    print(f"n={n} dx={dx} prec={prec} iters={st['iters']} cycles={st['inner_cycles']} "
+          f"(oracle {st_o.iters}) vs oracle: {e:.2e}")
+    if prec == 0:
+        assert e <= 1e-13
+    else:
+        truth, _, rc = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-14))
+        assert rc == 0
+        e_ref = rel_err(inner(same), inner(truth))
+        assert rel_err(got, inner(truth)) <= max(1e-9, 3.0 * e_ref)
