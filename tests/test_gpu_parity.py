@@ -303,3 +303,19 @@ def test_unknown_bottom_solver_is_refused(gpu):
     wl = W.make_workload(1, n=(32, 32, 32), box=16)
     with pytest.raises(mac.MacError):
         run_gpu(wl, bottom_solver=3)
+
+
+@pytest.mark.parametrize("prec,inner", [(1, 1), (1, 4), (2, 1), (2, 3)])
+def test_explicit_inner_cycles(gpu, prec, inner):
+    """A fixed count of FP32 V-cycles per refinement step (the default picks
+    3 or 2): the same answer to the tolerance, in the expected step count."""
+    wl = W.make_workload(2, n=(128, 64, 64), box=32)
+    truth = run_oracle(wl, eps=1e-13, alt=1)
+    out = run_gpu(wl, use_alt_energy_fix=1, precision=prec, inner_cycles=inner)
+    dflt = run_gpu(wl, use_alt_energy_fix=1, precision=prec)
+    st, sd = out["stats"], dflt["stats"]
+    print(f"prec {prec} inner {inner}: {st['iters']} steps, {st['inner_cycles']} cycles; "
+          f"default {sd['iters']} steps, {sd['inner_cycles']} cycles")
+    assert st["converged"] and st["inner_cycles"] == inner * st["iters"]
+    assert sd["inner_cycles"] == (3 if prec == 1 else 2) * sd["iters"]
+    assert rel_err(out["phi"], truth["phi"], remove_mean=True) <= 1e-8
