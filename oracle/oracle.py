@@ -162,6 +162,13 @@ def rho_shape(g):
     return (g.n[2] + 2 * gz, g.n[1] + 2, g.n[0] + 2)
 
 
+def comp_sum(x):
+    """The oracle's compensated (Neumaier) sum of a float64 vector, in its
+    order: the sum behind every mean the oracle removes."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    return float(lib().orc_sum(_p(x), x.size))
+
+
 def mk_mac_coeffs(g, rho):
     b = [np.zeros(face_shape(g, d)) for d in range(3)]
     lib().orc_mk_mac_coeffs(C.byref(g), _p(np.ascontiguousarray(rho)), _p(b[0]), _p(b[1]),

@@ -2,6 +2,7 @@
 sparse-matrix construction, the reference unit test's known answers
 (Exec/UNIT_TESTS/test_projection), closed forms and invariants."""
 import glob
+import math
 import os
 
 import numpy as np
@@ -201,6 +202,106 @@ def test_vcycle_convergence_rate(oracle_lib, cfg, n):
     st = r["stats"]
     rate = (st.res / st.res0) ** (1.0 / st.iters)
     assert st.converged and rate < 0.15, rate
+
+
+def cancelling_vector(rng, n, nspike=64):
+    """O(1) values plus pairs of spikes of 1e11..1e12 that cancel exactly:
+    the sum is O(sqrt(n)) while the partial sums of a plain left-to-right
+    (or pairwise) summation reach ~1e12, whose rounding (~1e-4 per add)
+    swamps it."""
+    x = rng.standard_normal(n)
+    big = rng.uniform(1e11, 1e12, nspike)
+    pos = rng.choice(n, 2 * nspike, replace=False)
+    x[pos[:nspike]] = big
+    x[pos[nspike:]] = -big
+    return x
+
+
+def test_compensated_sum_against_fsum(oracle_lib):
+    """orc_sum (the sum behind the solvability mean) against math.fsum, the
+    correctly rounded sum, on a vector with heavy cancellation; a plain
+    sequential sum and numpy's pairwise sum of the same vector miss by many
+    orders of magnitude more, so an uncompensated orc_sum fails here."""
+    O = oracle_lib
+    rng = np.random.default_rng(5)
+    for n in (1000, 1 << 16, 100003):
+        x = cancelling_vector(rng, n)
+        exact = math.fsum(x)
+        ulp = np.spacing(abs(exact))
+        got = O.comp_sum(x)
+        assert abs(got - exact) <= 2 * ulp, (n, got, exact)
+        naive = 0.0
+        for v in x:
+            naive += v
+        assert abs(naive - exact) > 1e6 * ulp, (n, naive, exact)
+        assert abs(float(np.sum(x)) - exact) > 1e6 * ulp, n
+    # no cancellation: still correctly rounded within an ulp
+    y = rng.uniform(0.0, 1.0, 1 << 15)
+    assert abs(O.comp_sum(y) - math.fsum(y)) <= np.spacing(math.fsum(y))
+    # a plain mistake the test must catch: dropping the correction term
+    x = cancelling_vector(rng, 4096)
+    s = 0.0
+    for v in x:
+        s += v
+    assert abs(s - math.fsum(x)) > 1e6 * np.spacing(abs(math.fsum(x)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_coarse_averaging_closed_form(oracle_lib, dim):
+    """Coarse face weight = mean of the 2^(dm-1) fine faces that cover the
+    coarse face; coarse alpha = mean of the 2^dm fine cells.  Each fine value
+    is 1000 x (a code of the coarse face or cell it belongs to) + a tag that
+    numbers the sub-faces 1..2^(dm-1) (cells: 1..2^dm), and fine faces that
+    lie between coarse faces carry +100.  The coarse value is then known
+    exactly: 1000 code + 1.5 (2D faces), 2.5 (3D faces, 2D cells), 4.5 (3D
+    cells).  A sum instead of a mean, a wrong transverse neighbour, a shifted
+    normal index or swapped directions each change it."""
+    O = oracle_lib
+    nf = (8, 12, 4)[:dim]
+    nc = tuple(m // 2 for m in nf)
+    bc = [[P, P]] * dim
+    gf = O.make_grid(nf, [1.0 / nf[0]] * dim, bc)
+    gc = O.make_grid(nc, [2.0 / nf[0]] * dim, bc)
+    mult = (1, 7, 31)
+
+    def code(c):   # c: coarse index arrays per direction
+        return sum(mult[d] * c[d] for d in range(dim))
+
+    def coords(shape):   # index arrays per direction (x first) of a C-ordered (z, y, x) array
+        idx = np.indices(shape)
+        return [idx[-1 - d] for d in range(dim)]
+
+    fine_b = []
+    want_b = []
+    for d in range(dim):
+        f = coords(O.face_shape(gf, d))
+        tr = [t for t in range(dim) if t != d]
+        tag = 1.0 + sum((1 << a) * (f[t] % 2) for a, t in enumerate(tr))
+        fine_b.append(1000.0 * code([f[t] // 2 for t in range(dim)]) + tag + 100.0 * (f[d] % 2))
+        c = coords(O.face_shape(gc, d))
+        want_b.append(1000.0 * code(c) + (1.5 if dim == 2 else 2.5))
+    fc = coords(O.cell_shape(gf))
+    fine_a = 1000.0 * code([x // 2 for x in fc]) + 1.0 + sum((1 << d) * (fc[d] % 2) for d in range(dim))
+    want_a = 1000.0 * code(coords(O.cell_shape(gc))) + (2.5 if dim == 2 else 4.5)
+    fine = O.LevelView(gf, fine_b, np.zeros(O.cell_shape(gf)), np.zeros(O.cell_shape(gf)), alpha=fine_a)
+    coarse = O.LevelView(gc, [np.zeros(O.face_shape(gc, d)) for d in range(dim)],
+                         np.zeros(O.cell_shape(gc)), np.zeros(O.cell_shape(gc)),
+                         alpha=np.zeros(O.cell_shape(gc)))
+    O.level_coarsen(gf, fine, coarse)
+    for d in range(dim):
+        assert np.array_equal(coarse.b[d], want_b[d]), d
+    assert np.array_equal(coarse.alpha, want_a)
+    # a constant stays that constant
+    fine = O.LevelView(gf, [np.full(O.face_shape(gf, d), 0.3) for d in range(dim)],
+                       np.zeros(O.cell_shape(gf)), np.zeros(O.cell_shape(gf)),
+                       alpha=np.full(O.cell_shape(gf), 0.7))
+    coarse = O.LevelView(gc, [np.zeros(O.face_shape(gc, d)) for d in range(dim)],
+                         np.zeros(O.cell_shape(gc)), np.zeros(O.cell_shape(gc)),
+                         alpha=np.zeros(O.cell_shape(gc)))
+    O.level_coarsen(gf, fine, coarse)
+    for d in range(dim):
+        assert np.allclose(coarse.b[d], 0.3, rtol=2e-16, atol=0), d
+    assert np.allclose(coarse.alpha, 0.7, rtol=2e-16, atol=0)
 
 
 def _dense_levels(O, n0, bc, rng):
