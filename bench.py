@@ -38,23 +38,18 @@ HBM_PEAK_GBS = 8000.0
 
 
 def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
-    """Name of the finest-level sweep kernel the library launches (the
-    MAC_SWEEP environment variable selects the variant; b16: the compact
-    bfloat16-storage form of the FP32 full-row kernel)."""
+    """Name of the finest-level sweep kernel the library launches (b16: the
+    compact bfloat16-storage form of the FP32 full-row kernel).  Full rows,
+    W = nx / 128 waves per row; rows x threads per block: FP64 6 x 768, FP32
+    8 x 1024 (16 x 1024 at nx = 128), FP32 with bfloat16 storage 6 x 768; the
+    last template argument: no alpha term."""
     t = "double" if s == 8 else "float"
-    v = int(os.environ.get("MAC_SWEEP", "1082"))
-    if v == 0:
-        return f"k_gsrb_fused<{t}, 0>"
-    if v in (8, 16):
-        return f"k_gsrb_pair<{t}, 0, {v}>"
     c = "true" if b16 else "false"
-    if v == 1082:   # full rows, W = nx / 128; rows x threads per block: FP64 6 x 768, FP32 8 x 1024
-        w = max(1, nx // 128)   # (16 x 1024 at nx = 128), FP32 with bfloat16 storage 6 x 768
-        if s == 8 or b16:   # (the last argument: no alpha term)
-            return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false>"
-        return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false>" if w == 1
-                else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}, false>")
-    return f"k_gsrb_deep<{t}, 0, {v % 100}, {v // 100}, false>"
+    w = max(1, nx // 128)
+    if s == 8 or b16:
+        return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false>"
+    return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false>" if w == 1
+            else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}, false>")
 
 
 def measured_traffic(name: str):

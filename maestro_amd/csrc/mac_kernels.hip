@@ -392,29 +392,6 @@ __global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
     L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c], alpha_at(L, c));
 }
 
-// ------------------------------------------- fused red-black sweep (3D)
-// One launch performs a whole GSRB sweep (red then black).  A 512-thread
-// block owns an FX x FY output tile of columns in x-y plus a one-column ring
-// and marches through a z-chunk.  At step k it updates red cells of plane k
-// (ring included: their red values are what the tile's black cells need) and
-// black cells of plane k-1, whose red neighbours are then all final.  The
-// sweep reads pin and writes pout, so overlapping tiles never observe each
-// other's writes.  Plane values rotate through three LDS buffers, one
-// barrier per plane.  Results equal the two-pass red/black update exactly.
-//   MODE 0: plain sweep; 1: previous iterate is zero (pin not read);
-//   MODE 2: pin + P e (first sweep after the coarse correction, e coarse).
-namespace fz {
-constexpr int EX = 64, EY = 8, FX = EX - 2, FY = EY - 2;
-constexpr int PX = EX + 2, PY = EY + 2;
-constexpr int NT = EX * EY;
-}
-
-struct ProlongSrc {
-    const void *e;
-    int cnx, cny, cnz;
-    EllBC bc;
-};
-
 __device__ __forceinline__ void wrap_coord(int x, int n, int per, int &xw, bool &ok)
 {
     if (x >= 0 && x < n) { xw = x; ok = true; return; }
@@ -423,169 +400,18 @@ __device__ __forceinline__ void wrap_coord(int x, int n, int per, int &xw, bool 
     ok = false;
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(fz::NT) k_gsrb_fused(Lev<T> L, const T *__restrict__ pin,
-                                                       T *__restrict__ pout, FusedGrid G, ProlongSrc PS)
-{
-    using namespace fz;
-    const int b = blockIdx.x;
-    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
-    if (idx >= G.total) return;
-    const int ntile = G.ntx * G.nty;
-    const int chunk = idx / ntile, tile = idx - chunk * ntile;
-    const int tyi = tile / G.ntx, txi = tile - tyi * G.ntx;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const long plane = (long)nx * ny;
-    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
-    const int tx = threadIdx.x & (EX - 1), ty = threadIdx.x >> 6;
-    const int X = txi * FX - 1 + tx, Y = tyi * FY - 1 + ty;
-    int Xw, Yw;
-    bool okx, oky;
-    wrap_coord(X, nx, L.perx, Xw, okx);
-    wrap_coord(Y, ny, L.pery, Yw, oky);
-    const bool active = okx && oky;
-    const bool outcol = tx >= 1 && tx <= FX && ty >= 1 && ty <= FY && X < nx && Y < ny;
-    const long col = Xw + (long)nx * Yw;
-    const int pos = (tx + 1) + PX * (ty + 1);
-    // ring-2 loads: a second ring of old values, neighbours of ring red cells
-    int r2a = -1, r2b = -1;
-    long c2a = 0, c2b = 0;
-    int x2a = 0, y2a = 0, x2b = 0, y2b = 0;   // ring-2 cell coordinates
-    if (ty == 0 || ty == EY - 1) {
-        int yy = (ty == 0) ? Y - 1 : Y + 1, yw;
-        bool ok;
-        wrap_coord(yy, ny, L.pery, yw, ok);
-        if (ok && okx) { r2a = (tx + 1) + PX * (ty == 0 ? 0 : EY + 1); c2a = Xw + (long)nx * yw; x2a = Xw; y2a = yw; }
-    }
-    if (tx == 0 || tx == EX - 1) {
-        int xx = (tx == 0) ? X - 1 : X + 1, xw;
-        bool ok;
-        wrap_coord(xx, nx, L.perx, xw, ok);
-        if (ok && oky) { r2b = (tx == 0 ? 0 : EX + 1) + PX * (ty + 1); c2b = xw + (long)nx * Yw; x2b = xw; y2b = Yw; }
-    }
-    __shared__ T tile_s[3][PX * PY];
-    // the ghost positions (outside non-periodic sides) stay zero
-    for (int q = threadIdx.x; q < 3 * PX * PY; q += NT) (&tile_s[0][0])[q] = T(0);
-
-    // plane index within the domain (periodic image) or false outside a
-    // non-periodic side; k stays within a few planes of [0, nz)
-    auto kwrap = [&](int k, int &kw) -> bool {
-        if (k >= 0 && k < nz) { kw = k; return true; }
-        if (L.perz) {
-            while (k < 0) k += nz;
-            while (k >= nz) k -= nz;
-            kw = k;
-            return true;
-        }
-        kw = 0;
-        return false;
-    };
-    // old value of cell (column c, xy parity pxy, plane k); black cells get
-    // the coarse correction in MODE 2
-    auto load_old = [&](long c, int pxy, int xw, int yw, int k) -> T {
-        int kw;
-        if (!kwrap(k, kw)) return T(0);
-        if (MODE == 1) return T(0);
-        T v = pin[c + plane * kw];
-        if (MODE == 2 && ((pxy + kw) & 1))
-            v = v + prolong_value<3, T>((const T *)PS.e, PS.cnx, PS.cny, PS.cnz, PS.bc, xw, yw, kw);
-        return v;
-    };
-    const int pxy = Xw + Yw;
-    const int kb = z0 - 1, ke = z1;
-    T pm2 = T(0), pm1 = T(0), p0 = T(0), pp1 = T(0), pp2 = T(0);
-    if (active) {
-        pm1 = load_old(col, pxy, Xw, Yw, kb - 1);
-        p0 = load_old(col, pxy, Xw, Yw, kb);
-        pp1 = load_old(col, pxy, Xw, Yw, kb + 1);
-        pp2 = load_old(col, pxy, Xw, Yw, kb + 2);
-    }
-    // per-plane inputs of this thread: its six face weights, rhs and the
-    // ring-2 values it publishes.  They are fetched one plane ahead (nw..)
-    // so their latency overlaps the previous plane's work.
-    // two planes of look-ahead: n1 (plane k+1) and n2 (plane k+2)
-    T cw[6], crhs = T(0), pw[6], prhs = T(0);
-    T n1w[6], n1rhs = T(0), n1a = T(0), n1b = T(0);
-    T n2w[6], n2rhs = T(0), n2a = T(0), n2b = T(0);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) { cw[s] = T(0); pw[s] = T(0); n1w[s] = T(0); n2w[s] = T(0); }
-    auto fetch = [&](int k, T *nw, T &nrhs, T &nr2a, T &nr2b) {
-        int kw;
-        const bool kok = kwrap(k, kw);
-        if (active && kok) {
-            const long c = col + plane * kw;
-            nw[0] = L.bx[c];
-            nw[1] = (Xw < nx - 1) ? L.bx[c + 1] : L.bxh[Yw + (long)ny * kw];
-            nw[2] = L.by[c];
-            nw[3] = (Yw < ny - 1) ? L.by[c + nx] : L.byh[Xw + (long)nx * kw];
-            nw[4] = L.bz[c];
-            nw[5] = (kw < nz - 1) ? L.bz[c + plane] : L.bzh[col];
-            nrhs = L.rhs[c];
-        }
-        nr2a = (r2a >= 0) ? load_old(c2a, x2a + y2a, x2a, y2a, k) : T(0);
-        nr2b = (r2b >= 0) ? load_old(c2b, x2b + y2b, x2b, y2b, k) : T(0);
-    };
-    fetch(kb, n1w, n1rhs, n1a, n1b);
-    if (kb + 1 <= ke) fetch(kb + 1, n2w, n2rhs, n2a, n2b);
-    __syncthreads();
-    T *buf = tile_s[0], *pbuf = tile_s[2], *qbuf = tile_s[1];   // planes k, k-1, k-2
-    for (int k = kb; k <= ke; ++k) {
-        if (k > kb) {   // rotate: (k-2, k-1, k) <- (k-1, k, k-2)
-            T *t = qbuf;
-            qbuf = pbuf;
-            pbuf = buf;
-            buf = t;
-        }
-        int kw;
-        const bool kok = kwrap(k, kw);
-#pragma unroll
-        for (int s = 0; s < 6; ++s) { pw[s] = cw[s]; cw[s] = n1w[s]; n1w[s] = n2w[s]; }
-        prhs = crhs;
-        crhs = n1rhs;
-        n1rhs = n2rhs;
-        const T r2va = n1a, r2vb = n1b;
-        n1a = n2a;
-        n1b = n2b;
-        if (k + 2 <= ke) fetch(k + 2, n2w, n2rhs, n2a, n2b);
-        // (a) publish the old values of plane k
-        if (active) buf[pos] = kok ? p0 : T(0);
-        if (r2a >= 0) buf[r2a] = r2va;
-        if (r2b >= 0) buf[r2b] = r2vb;
-        __syncthreads();
-        const bool red = kok && ((pxy + kw) & 1) == 0;
-        // (c) red cells of plane k (tile and ring)
-        if (active && red) {
-            T p[6] = {buf[pos - 1], buf[pos + 1], buf[pos - PX], buf[pos + PX], pm1, pp1};
-            T v = gs_value<3, T>(L, cw, p, crhs);
-            buf[pos] = v;
-            p0 = v;
-        }
-        // (d) black cells of plane k-1 (tile only), (e) write plane k-1
-        const int km = k - 1;
-        if (outcol && km >= z0 && km < z1) {
-            if ((pxy + km) & 1) {   // cell (col, k-1) is black
-                T p[6] = {pbuf[pos - 1], pbuf[pos + 1], pbuf[pos - PX], pbuf[pos + PX], pm2, p0};
-                pm1 = gs_value<3, T>(L, pw, p, prhs);
-            }
-            pout[col + plane * km] = pm1;
-        }
-        // rotate the column registers and fetch plane k+3
-        pm2 = pm1;
-        pm1 = p0;
-        p0 = pp1;
-        pp1 = pp2;
-        pp2 = (active && k + 3 <= ke + 1) ? load_old(col, pxy, Xw, Yw, k + 3) : T(0);
-    }
-}
-
 // ------------------------------- fused red-black sweep, deep prefetch (3D)
-// The z-march of k_gsrb_fused with its global loads issued PD planes ahead
-// (face weights, rhs, ring-2 values) and the old iterate PD+1 planes ahead,
-// held in register queues.  Each step of the march waits only for loads
-// issued PD steps earlier, so a block keeps PD planes of traffic in flight.
-// One column per thread (64 x EY tile, inner 62 x (EY-2) written), 32-bit
-// indexing (levels below 2^31 cells).  MODE 0: plain; 1: zero old iterate.
-// AL: the level has alpha (its value travels with the face weights)
+// One launch performs a whole GSRB sweep (red then black).  A block owns a
+// 64 x EY tile of x-y columns (the inner 62 x (EY-2) are written; the ring's
+// red values are what the tile's black cells need) and marches through a
+// z-chunk: at step k it updates the red cells of plane k and the black cells
+// of plane k-1, whose red neighbours are then all final.  The sweep reads pin
+// and writes pout, so overlapping tiles never observe each other's writes;
+// the result equals the two-pass red/black update exactly.  Global loads are
+// issued PD planes ahead (face weights, rhs, ring-2 values) and the old
+// iterate PD+1 planes ahead, held in register queues.  One column per thread,
+// 32-bit indexing (levels below 2^31 cells).  MODE 0: plain; 1: zero old
+// iterate.  AL: the level has alpha (its value travels with the face weights)
 template <typename T, int MODE, int EY, int PD, bool AL = false>
 __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__restrict__ pin,
                                                       T *__restrict__ pout, FusedGrid G)
@@ -1159,204 +985,6 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
     }
 }
 #undef PH
-
-// ------------------------------------- fused red-black sweep, x-pairs (3D)
-// Same algorithm as k_gsrb_fused, but each thread owns two neighbouring
-// columns (an x-pair: one red and one black cell on every plane), so every
-// lane does one red update of plane k and one black update of plane k-1 per
-// step; indexing is 32-bit and incremental.  Block: 32 pair-lanes x EY rows;
-// the tile covers 64 x EY columns and writes the inner 62 x (EY-2).
-namespace fp {
-constexpr int NPX = 32, EX = 64, FX = EX - 2;
-constexpr int PX = EX + 4;   // LDS pitch: ring-2 at 1 and EX+2, columns at 2..EX+1
-}
-
-template <typename T, int MODE, int EY>
-__global__ void __launch_bounds__(fp::NPX *EY) k_gsrb_pair(Lev<T> L, const T *__restrict__ pin,
-                                                          T *__restrict__ pout, FusedGrid G)
-{
-    using namespace fp;
-    constexpr int FY = EY - 2, PY = EY + 2, NT = NPX * EY;
-    const int b = blockIdx.x;
-    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
-    if (idx >= G.total) return;
-    const int ntile = G.ntx * G.nty;
-    const int chunk = idx / ntile, tile = idx - chunk * ntile;
-    const int tyi = tile / G.ntx, txi = tile - tyi * G.ntx;
-    const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const int plane = nx * ny;
-    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
-    const int p = threadIdx.x & (NPX - 1), ty = threadIdx.x / NPX;
-    const int Y = tyi * FY - 1 + ty;
-    const int X0 = txi * FX - 1 + 2 * p;    // global x of the pair's columns X0, X0+1
-    int Yw, Xw[2];
-    bool oky, okx[2];
-    wrap_coord(Y, ny, L.pery, Yw, oky);
-    wrap_coord(X0, nx, L.perx, Xw[0], okx[0]);
-    wrap_coord(X0 + 1, nx, L.perx, Xw[1], okx[1]);
-    bool act[2] = {okx[0] && oky, okx[1] && oky};
-    const bool outrow = ty >= 1 && ty <= FY && Y < ny;
-    bool out[2] = {outrow && (2 * p) >= 1 && X0 < nx, outrow && (2 * p + 1) <= FX && X0 + 1 < nx};
-    const int col[2] = {Xw[0] + nx * Yw, Xw[1] + nx * Yw};
-    const int pos0 = (2 * p + 2) + PX * (ty + 1);   // LDS position of column 0 (column 1 at +1)
-    // colour parity from unwrapped coordinates (periodic extents are even, so
-    // wrapping never changes it; ghost columns keep their natural parity)
-    const int par0 = (X0 + Y) & 1;
-    // ring-2 roles: rows -1/EY for the tile's 64 columns (two per thread), and
-    // columns -1/EX of every row (first/last pair-lane)
-    int r2pos[2] = {-1, -1}, r2col[2] = {0, 0};
-    if (ty == 0 || ty == EY - 1) {
-        int yw;
-        bool ok;
-        wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, L.pery, yw, ok);
-        for (int c = 0; c < 2; ++c)
-            if (ok && okx[c]) { r2pos[c] = pos0 + c + (ty == 0 ? -PX : PX); r2col[c] = Xw[c] + nx * yw; }
-    }
-    int r2xpos = -1, r2xcol = 0;
-    if (p == 0 || p == NPX - 1) {
-        int xw;
-        bool ok;
-        wrap_coord(p == 0 ? X0 - 1 : X0 + 2, nx, L.perx, xw, ok);
-        if (ok && oky) { r2xpos = (p == 0 ? pos0 - 1 : pos0 + 2); r2xcol = xw + nx * Yw; }
-    }
-    __shared__ T tile_s[3][PX * PY];
-    for (int q = threadIdx.x; q < 3 * PX * PY; q += NT) (&tile_s[0][0])[q] = T(0);
-
-    auto kwrap = [&](int k, int &kw) -> bool {
-        if (k >= 0 && k < nz) { kw = k; return true; }
-        if (L.perz) {
-            while (k < 0) k += nz;
-            while (k >= nz) k -= nz;
-            kw = k;
-            return true;
-        }
-        kw = 0;
-        return false;
-    };
-    auto ld = [&](int c, int k) -> T {   // old iterate, zero outside / in MODE 1
-        int kw;
-        if (MODE == 1 || !kwrap(k, kw)) return T(0);
-        return pin[c + plane * kw];
-    };
-    const int kb = z0 - 1, ke = z1;
-    // per column: final/old values at planes k-2 .. k+2
-    T pm2[2], pm1[2], p0[2], pp1[2], pp2[2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        pm2[c] = T(0);
-        pm1[c] = act[c] ? ld(col[c], kb - 1) : T(0);
-        p0[c] = act[c] ? ld(col[c], kb) : T(0);
-        pp1[c] = act[c] ? ld(col[c], kb + 1) : T(0);
-        pp2[c] = act[c] ? ld(col[c], kb + 2) : T(0);
-    }
-    // face weights and rhs of both columns: current plane (red), previous
-    // plane (black) and the next plane (fetched one step ahead)
-    T cw[2][6], crhs[2], pw[2][6], prhs[2], nw[2][6], nrhs[2], n2[3];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        crhs[c] = prhs[c] = nrhs[c] = T(0);
-#pragma unroll
-        for (int s = 0; s < 6; ++s) cw[c][s] = pw[c][s] = nw[c][s] = T(0);
-    }
-    n2[0] = n2[1] = n2[2] = T(0);
-    auto fetch = [&](int k) {
-        int kw;
-        const bool kok = kwrap(k, kw);
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-            if (act[c] && kok) {
-                const int q = col[c] + plane * kw;
-                nw[c][0] = L.bx[q];
-                nw[c][1] = (Xw[c] < nx - 1) ? L.bx[q + 1] : L.bxh[Yw + ny * kw];
-                nw[c][2] = L.by[q];
-                nw[c][3] = (Yw < ny - 1) ? L.by[q + nx] : L.byh[Xw[c] + nx * kw];
-                nw[c][4] = L.bz[q];
-                nw[c][5] = (kw < nz - 1) ? L.bz[q + plane] : L.bzh[col[c]];
-                nrhs[c] = L.rhs[q];
-            }
-        n2[0] = (r2pos[0] >= 0) ? ld(r2col[0], k) : T(0);
-        n2[1] = (r2pos[1] >= 0) ? ld(r2col[1], k) : T(0);
-        n2[2] = (r2xpos >= 0) ? ld(r2xcol, k) : T(0);
-    };
-    fetch(kb);
-    __syncthreads();
-    T *buf = tile_s[0], *pbuf = tile_s[2], *qbuf = tile_s[1];
-    for (int k = kb; k <= ke; ++k) {
-        if (k > kb) {
-            T *t = qbuf;
-            qbuf = pbuf;
-            pbuf = buf;
-            buf = t;
-        }
-        int kw;
-        const bool kok = kwrap(k, kw);
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-#pragma unroll
-            for (int s = 0; s < 6; ++s) { pw[c][s] = cw[c][s]; cw[c][s] = nw[c][s]; }
-            prhs[c] = crhs[c];
-            crhs[c] = nrhs[c];
-        }
-        const T r2v0 = n2[0], r2v1 = n2[1], r2vx = n2[2];
-        if (k < ke) fetch(k + 1);
-        // (a) publish the old values of plane k
-        if (act[0]) buf[pos0] = kok ? p0[0] : T(0);
-        if (act[1]) buf[pos0 + 1] = kok ? p0[1] : T(0);
-        if (r2pos[0] >= 0) buf[r2pos[0]] = r2v0;
-        if (r2pos[1] >= 0) buf[r2pos[1]] = r2v1;
-        if (r2xpos >= 0) buf[r2xpos] = r2vx;
-        __syncthreads();
-        // (c) the red cell of the pair on plane k: column r (0 when X0+Y+k even)
-        const int r = (par0 + k) & 1;
-        const bool ractive = kok && (r == 0 ? act[0] : act[1]);
-        if (ractive) {
-            const int pr = pos0 + r;
-            T w[6], nb[6];
-#pragma unroll
-            for (int s = 0; s < 6; ++s) w[s] = r ? cw[1][s] : cw[0][s];
-            nb[0] = r ? p0[0] : buf[pr - 1];          // x- (own black column or LDS)
-            nb[1] = r ? buf[pr + 1] : p0[1];          // x+
-            nb[2] = buf[pr - PX];
-            nb[3] = buf[pr + PX];
-            nb[4] = r ? pm1[1] : pm1[0];
-            nb[5] = r ? pp1[1] : pp1[0];
-            T v = gs_value<3, T>(L, w, nb, r ? crhs[1] : crhs[0]);
-            buf[pr] = v;
-            if (r) p0[1] = v;
-            else p0[0] = v;
-        }
-        // (d) the black cell of plane k-1 is the same column r; (e) write k-1
-        const int km = k - 1;
-        if (km >= z0 && km < z1) {
-            const bool bout = r ? out[1] : out[0];
-            if (bout) {
-                const int pr = pos0 + r;
-                T w[6], nb[6];
-#pragma unroll
-                for (int s = 0; s < 6; ++s) w[s] = r ? pw[1][s] : pw[0][s];
-                nb[0] = r ? pm1[0] : pbuf[pr - 1];    // the other column is red at k-1
-                nb[1] = r ? pbuf[pr + 1] : pm1[1];
-                nb[2] = pbuf[pr - PX];
-                nb[3] = pbuf[pr + PX];
-                nb[4] = r ? pm2[1] : pm2[0];
-                nb[5] = r ? p0[1] : p0[0];
-                T v = gs_value<3, T>(L, w, nb, r ? prhs[1] : prhs[0]);
-                if (r) pm1[1] = v;
-                else pm1[0] = v;
-            }
-            if (out[0]) pout[col[0] + plane * km] = pm1[0];
-            if (out[1]) pout[col[1] + plane * km] = pm1[1];
-        }
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            pm2[c] = pm1[c];
-            pm1[c] = p0[c];
-            p0[c] = pp1[c];
-            pp1[c] = pp2[c];
-            pp2[c] = (act[c] && k + 3 <= ke + 1) ? ld(col[c], k + 3) : T(0);
-        }
-    }
-}
 
 // residual of the fine level restricted onto the coarse level in one pass
 // (the fine residual is never stored); also clears the coarse iterate.
@@ -3078,26 +2706,6 @@ void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
 }
 
-template <typename T, int EY>
-static void launch_pair(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
-{
-    FusedGrid G;
-    G.ntx = (L.nx + fp::FX - 1) / fp::FX;
-    G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
-    const int tiles = G.ntx * G.nty;
-    int zc = (2048 + tiles - 1) / tiles;
-    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
-    G.tz = (L.nz + zc - 1) / zc;
-    zc = (L.nz + G.tz - 1) / G.tz;
-    G.total = tiles * zc;
-    G.per_xcd = (G.total + 7) / 8;
-    dim3 g(8 * G.per_xcd), b(fp::NPX * EY);
-    if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_pair<T, 1, EY>), g, b, 0, s, L, pin, pout, G);
-    else
-        hipLaunchKernelGGL((k_gsrb_pair<T, 0, EY>), g, b, 0, s, L, pin, pout, G);
-}
-
 template <typename T, int EY, int PD>
 static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s)
 {
@@ -3210,11 +2818,6 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
     RowSweepGrid G;
     G.W = W;
     G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
-    static int zc_env = -2;
-    if (zc_env == -2) {
-        const char *e = getenv("MAC_ROWS_ZC");
-        zc_env = e ? atoi(e) : -1;
-    }
     // time ~ rounds of resident blocks x (march steps + a block's start-up,
     // about 8 steps: the ring prologue and the LDS clear); ties go to more
     // blocks (better balance across CUs).  Measured at 256^3 FP64: 4 chunks
@@ -3234,15 +2837,6 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
             best_zc = zc;
             best_blocks = blocks;
         }
-    }
-    if (zc_env > 0) best_zc = std::min(zc_env, L.nz);
-    static int verbose = -1;   // MAC_ROWS_VERBOSE: print the launch shape once per kernel form and size
-    if (verbose < 0) verbose = getenv("MAC_ROWS_VERBOSE") ? 1 : 0;
-    static int shown_nx = 0;
-    if (verbose && shown_nx != L.nx * 1000 + L.nz) {
-        shown_nx = L.nx * 1000 + L.nz;
-        fprintf(stderr, "maestro_mac: rows T%zu EY%d NTH%d B16=%d W%d AL=%d mode %d: nx %d nz %d slots %d zc %d\n",
-                sizeof(T), EY, NTH, (int)B16, W, (int)AL, mode, L.nx, L.nz, slots, best_zc);
     }
     G.tz = (L.nz + best_zc - 1) / best_zc;
     const int nzc = (L.nz + G.tz - 1) / G.tz;
@@ -3279,140 +2873,41 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     return launch_rows_w<T, EY, NTH, B16, 2>(L, RB, mode, s);
 }
 
-static int sweep_variant()
-{
-    // 0: one column per thread; 8/16: x-pairs with 8/16 rows; 100*PD + EY:
-    // one column per thread with PD planes of prefetch and EY rows;
-    // 1082: full rows (falls back to 208 where they do not fit)
-    static int variant = -1;
-    if (variant < 0) {
-        const char *e = getenv("MAC_SWEEP");
-        variant = e ? atoi(e) : 1082;
-    }
-    return variant;
-}
-
-// Rows per block of the FP32 full-row sweep.  which = 0: the compact
-// (bfloat16) form, 6 rows of 768 threads by default (MAC_ROWS_EY = 6 or 8):
-// a 256^3 level then gives 64 y-tiles x 4 z-chunks, one block per CU on all
-// 256 CUs, where 8 rows leave 43 x 5 = 215 blocks and idle CUs.  which = 1:
-// FP32 storage at nx = 128 (MAC_ROWS_EY1 = 8 or 16, default 16: 1024 threads,
-// 14 of 16 rows written).
-static int rows_ey(int which)
-{
-    static int ey[2] = {-1, -1};
-    if (ey[which] < 0) {
-        const char *v = getenv(which ? "MAC_ROWS_EY1" : "MAC_ROWS_EY");
-        const int d = which ? 16 : 6;
-        ey[which] = v ? atoi(v) : d;
-        if (which == 0 && ey[0] != 6 && ey[0] != 8) ey[0] = d;
-        if (which == 1 && ey[1] != 8 && ey[1] != 16) ey[1] = d;
-    }
-    return ey[which];
-}
-
-static bool rr_rows_on()
-{
-    static int rows = -1;   // MAC_RR_ROWS=0 keeps the block-per-coarse-row kernel
-    if (rows < 0) {
-        const char *v = getenv("MAC_RR_ROWS");
-        rows = v ? atoi(v) : 1;
-    }
-    return rows != 0;
-}
-
-static bool res_rows_on(int &tz_env)
-{
-    static int on = -1, tz = 0;
-    if (on < 0) {
-        const char *v = getenv("MAC_RES_ROWS");
-        on = v ? atoi(v) : 1;
-        const char *t = getenv("MAC_RES_TZ");
-        tz = t ? atoi(t) : 0;
-    }
-    tz_env = tz;
-    return on != 0;
-}
-
 bool compact_level_ok(int dim, const Lev<float> &L)
 {
-    int tz_env;
-    const bool fit = rows_ey(0) == 6 ? rows_fit<float, 6, 768>(L) : rows_fit<float, 8, 1024>(L);
-    return dim == 3 && sweep_variant() == 1082 && fit && rr_rows_on() && res_rows_on(tz_env);
+    return dim == 3 && rows_fit<float, 6, 768>(L);
 }
 
+// One fused red-black sweep of a 3D level (pin -> pout); false where no fused
+// form applies (the caller then sweeps per colour).
+//   - nx = 128 or 256: the full-row sweep.  FP32 runs 8 rows in blocks of
+//     1024 threads (16 rows at nx = 128), the compact (bfloat16) form 6 rows
+//     of 768 threads: a 256^3 level then gives 64 y-tiles x 4 z-chunks, one
+//     block per CU on all 256 CUs, where 8 rows leave 43 x 5 = 215 blocks and
+//     idle CUs.  FP64 runs 6 rows of 768 threads, and 4 rows of 512 with alpha
+//     (the alpha ring does not fit the 168 VGPRs a 768-thread block leaves a
+//     lane).
+//   - other sizes: one column per thread (k_gsrb_deep, two planes of prefetch).
 template <typename T>
-bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
-                       const EllBC &bc, hipStream_t s, const Compact16 *K)
+bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 *K)
 {
-    const int variant = sweep_variant();
-    // with alpha: the full-row sweep and its fallback, which carry the alpha
-    // term; the caller sweeps per colour otherwise
-    if (L.alpha && (K || mode == SWEEP_PROLONG || variant != 1082)) return false;
     if constexpr (sizeof(T) == 4) {
         if (K) {
-            if (mode == SWEEP_PROLONG || variant != 1082) return false;
-            if (rows_ey(0) == 6) return launch_rows<T, 6, 768, true>(L, pin, pout, mode, s, K);
-            return launch_rows<T, 8, 1024, true>(L, pin, pout, mode, s, K);
+            if (L.alpha) return false;   // compact storage carries no alpha
+            return launch_rows<T, 6, 768, true>(L, pin, pout, mode, s, K);
         }
     } else {
         if (K) return false;
     }
-    if (mode != SWEEP_PROLONG && variant >= 1000) {
-        bool done = false;
-        switch (variant) {
-        case 1082:
-            // FP64 with alpha: 4 rows of 512 threads (the alpha ring does not
-            // fit the 168 VGPRs a 768-thread block leaves a lane)
-            if constexpr (sizeof(T) == 8)
-                done = L.alpha ? launch_rows<T, 4, 512, false>(L, pin, pout, mode, s, nullptr)
-                               : launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
-            else if (L.nx == 128 && rows_ey(1) == 16)
-                done = launch_rows<T, 16, 1024, false>(L, pin, pout, mode, s, nullptr);
-            else done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
-            break;
-        default: break;
-        }
-        if (done) return true;
-        launch_deep<T, 8, 2>(L, pin, pout, mode, s);
-        return true;
-    }
-    if (mode != SWEEP_PROLONG) {
-        switch (variant) {
-        case 8: launch_pair<T, 8>(L, pin, pout, mode, s); return true;
-        case 16: launch_pair<T, 16>(L, pin, pout, mode, s); return true;
-        case 208: launch_deep<T, 8, 2>(L, pin, pout, mode, s); return true;
-        case 308: launch_deep<T, 8, 3>(L, pin, pout, mode, s); return true;
-        case 408: launch_deep<T, 8, 4>(L, pin, pout, mode, s); return true;
-        case 304: launch_deep<T, 4, 3>(L, pin, pout, mode, s); return true;
-        case 404: launch_deep<T, 4, 4>(L, pin, pout, mode, s); return true;
-        case 604: launch_deep<T, 4, 6>(L, pin, pout, mode, s); return true;
-        default: break;
-        }
-    }
-    FusedGrid G;
-    G.ntx = (L.nx + fz::FX - 1) / fz::FX;
-    G.nty = (L.ny + fz::FY - 1) / fz::FY;
-    const int tiles = G.ntx * G.nty;
-    int zc = (2048 + tiles - 1) / tiles;            // aim at >= 2048 blocks
-    zc = std::max(1, std::min(zc, std::max(1, L.nz / 8)));
-    G.tz = (L.nz + zc - 1) / zc;
-    zc = (L.nz + G.tz - 1) / G.tz;
-    G.total = tiles * zc;
-    G.per_xcd = (G.total + 7) / 8;
-    ProlongSrc PS;
-    PS.e = C ? (const void *)C->phi : nullptr;
-    PS.cnx = C ? C->nx : 1;
-    PS.cny = C ? C->ny : 1;
-    PS.cnz = C ? C->nz : 1;
-    PS.bc = bc;
-    dim3 g(8 * G.per_xcd), b(fz::NT);
-    if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_fused<T, 1>), g, b, 0, s, L, pin, pout, G, PS);
-    else if (mode == SWEEP_PROLONG)
-        hipLaunchKernelGGL((k_gsrb_fused<T, 2>), g, b, 0, s, L, pin, pout, G, PS);
+    bool done;
+    if constexpr (sizeof(T) == 8)
+        done = L.alpha ? launch_rows<T, 4, 512, false>(L, pin, pout, mode, s, nullptr)
+                       : launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
+    else if (L.nx == 128)
+        done = launch_rows<T, 16, 1024, false>(L, pin, pout, mode, s, nullptr);
     else
-        hipLaunchKernelGGL((k_gsrb_fused<T, 0>), g, b, 0, s, L, pin, pout, G, PS);
+        done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
+    if (!done) launch_deep<T, 8, 2>(L, pin, pout, mode, s);
     return true;
 }
 
@@ -3421,7 +2916,7 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
                            const Compact16 *K)
 {
     if (F.alpha && K) return false;
-    if (rr_rows_on() && dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
+    if (dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
         RowMarchGrid G;
         G.W = F.nx / 128;
         // coarse planes per chunk: at most 16, fewer on small levels so that
@@ -3457,10 +2952,10 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
     return true;
 }
 
-template bool launch_gsrb_fused<double>(const Lev<double> &, const double *, double *, int,
-                                        const Lev<double> *, const EllBC &, hipStream_t, const Compact16 *);
-template bool launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int,
-                                       const Lev<float> *, const EllBC &, hipStream_t, const Compact16 *);
+template bool launch_gsrb_fused<double>(const Lev<double> &, const double *, double *, int, hipStream_t,
+                                        const Compact16 *);
+template bool launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int, hipStream_t,
+                                       const Compact16 *);
 template bool launch_resid_restrict<double>(int, const Lev<double> &, const double *,
                                             const Lev<double> &, hipStream_t, const Compact16 *);
 template bool launch_resid_restrict<float>(int, const Lev<float> &, const float *,
@@ -3480,19 +2975,18 @@ void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s)
 static constexpr unsigned NORM_GRID = 2048;   // 256 CUs x 8 blocks
 
 // Row-march form of the FP64 residual passes where it applies (3D, nx a
-// multiple of 128); false otherwise.  MAC_RES_ROWS=0 keeps the row-per-wave
-// kernels.  The z-chunk length is 32 planes (MAC_RES_TZ overrides).
+// multiple of 128); false otherwise (the row-per-wave kernels then run).
+// The z-chunk length is 32 planes, fewer on small levels.
 template <int MODE>
 static bool launch_resid_rows(const Lev<double> &L, const float *e, double *phi_out, float *r32,
                               unsigned long long *nrm, hipStream_t s, unsigned short *r16 = nullptr)
 {
-    int tz_env;
-    if (!res_rows_on(tz_env) || L.nx % 128 != 0 || L.nz < 2) return false;
+    if (L.nx % 128 != 0 || L.nz < 2) return false;
     RowMarchGrid G;
     G.W = L.nx / 128;
     // planes per chunk: 32, fewer on small levels (some 4096 waves in all)
     const long segs = (long)L.ny * G.W;
-    G.tz = tz_env > 0 ? tz_env : (int)std::max(2L, std::min(32L, segs * L.nz / 4096));
+    G.tz = (int)std::max(2L, std::min(32L, segs * L.nz / 4096));
     G.tz = std::min(G.tz, L.nz);
     const int chunks = (L.nz + G.tz - 1) / G.tz;
     G.total = L.ny * G.W * chunks;
@@ -3618,10 +3112,8 @@ template <typename T>
 void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_host, int lc, int nlev,
                          const CoarseOpts &o, double *cg, hipStream_t s)
 {
-    static int global_only = -1;   // MAC_COARSE_GLOBAL: keep the levels in global memory
-    if (global_only < 0) global_only = getenv("MAC_COARSE_GLOBAL") ? 1 : 0;
     const long lds = coarse_lds_bytes<T>(dim, levs_host, lc, nlev);
-    const bool in_lds = !global_only && lds <= COARSE_LDS_MAX;
+    const bool in_lds = lds <= COARSE_LDS_MAX;
     const unsigned sh = in_lds ? (unsigned)lds : 0u;
     if (o.cycle == MAC_CYCLE_V) {
         if (in_lds)

@@ -46,12 +46,12 @@ struct CorrectArgs {
     int ng_phi;
 };
 
-// tiling of the fused 3D sweep (k_gsrb_fused)
+// tiling of the one-column-per-thread fused 3D sweep (k_gsrb_deep)
 struct FusedGrid {
     int ntx, nty, tz, total, per_xcd;
 };
 
-enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1, SWEEP_PROLONG = 2 };
+enum { SWEEP_PLAIN = 0, SWEEP_ZERO = 1 };
 
 // FP64 vectors of the bottom solver's scratch (BiCGStab: x r r^ p v t)
 static constexpr int BOTTOM_VECS = 6;
@@ -160,20 +160,20 @@ template <typename T> long launch_partial_sum(const T *x, long n, double *part, 
 template <typename T> void launch_coarsen(int dim, const Lev<T> &F, const Lev<T> &C, hipStream_t s);
 void launch_weights_to_f32(int dim, const Lev<double> &S, const Lev<float> &D, hipStream_t s);
 template <typename T> void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s);
-// one full red-black sweep of a 3D level, pin -> pout; mode SWEEP_*; for
-// SWEEP_PROLONG the coarse level C supplies the correction.  K (FP32 only,
-// SWEEP_PLAIN/ZERO) takes the weights and right side from compact storage;
-// the call then returns false if the level does not fit the full-row kernel
-// (compact_level_ok) and launches nothing.
+// one full red-black sweep of a 3D level, pin -> pout; mode SWEEP_*.  K
+// (FP32 only) takes the weights and right side from compact storage; the
+// call then returns false if the level does not fit the full-row kernel
+// (compact_level_ok) and launches nothing.  A level with alpha in compact
+// storage also returns false.
 template <typename T>
-bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, const Lev<T> *C,
-                       const EllBC &bc, hipStream_t s, const Compact16 *K = nullptr);
+bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s,
+                       const Compact16 *K = nullptr);
 // fine residual (of phi) restricted to C.rhs; C.phi cleared.  K as above.
 template <typename T>
 bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
                            const Compact16 *K = nullptr);
 // whether every level-0 kernel of the mixed path has a compact-storage form
-// for this level (3D, nx a multiple of 128 and the full-row sweep fits)
+// for this level (3D, nx = 128 or 256 and the full-row sweep fits)
 bool compact_level_ok(int dim, const Lev<float> &L);
 // FP32 -> bfloat16 copy of n values
 void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s);

@@ -180,13 +180,7 @@ extern "C" const char *mac_version(void) { return "maestro_mac 0.1 (gfx950)"; }
 // cycle (8^3 in 3D, 32^2 in 2D).  A 16^3 level fits its LDS in FP32 but is
 // slower there than as separate multi-block launches: one workgroup runs on
 // one CU.
-static int coarse_threshold(int dim, int elem_bytes)
-{
-    (void)elem_bytes;
-    const char *e = getenv("MAC_COARSE_CELLS");
-    if (e) return atoi(e);
-    return dim == 3 ? 512 : 1024;
-}
+static int coarse_threshold(int dim) { return dim == 3 ? 512 : 1024; }
 
 template <typename T>
 static int build_hier(mac_plan *P, Hier<T> &H)
@@ -245,7 +239,7 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         H.alpha.push_back(alpha_part);
         for (int d = 0; d < P->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
     }
-    int thr = coarse_threshold(P->dim, (int)sizeof(T));
+    int thr = coarse_threshold(P->dim);
     H.lc = nl - 1;
     for (int l = 0; l < nl; ++l)
         if ((long)H.lev[l].nx * H.lev[l].ny * H.lev[l].nz <= thr) { H.lc = l; break; }
@@ -307,39 +301,19 @@ static void smooth(mac_plan *P, const Lev<T> &L, int level, int sweeps)
         }
 }
 
-static bool use_fused(const mac_plan *P)
-{
-    static int off = -1;
-    if (off < 0) off = getenv("MAC_NO_FUSED") ? 1 : 0;
-    return P->dim == 3 && !off;
-}
-
-// nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
-// (cur tracks which holds it).  first_mode applies to the first sweep.
-static bool fuse_prolong_on()
-{
-    static int fuse_prolong = -1;
-    if (fuse_prolong < 0) fuse_prolong = getenv("MAC_FUSED_PROLONG") ? 1 : 0;
-    return fuse_prolong != 0;
-}
+// 3D levels run the fused red-black sweeps (pin -> pout, one launch per
+// sweep); 2D levels sweep per colour in place
+static bool use_fused(const mac_plan *P) { return P->dim == 3; }
 
 // K: level l's weights and right side in compact storage (level 0 only)
+// nsweep fused sweeps of a 3D level; the iterate moves between phi and phi2
+// (cur tracks which holds it).  first_mode applies to the first sweep.
 template <typename T>
 static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur,
                          const Compact16 *K = nullptr)
 {
     const bool f64 = sizeof(T) == 8;
     Lev<T> &L = H.lev[l];
-    if (first_mode == SWEEP_PROLONG && !fuse_prolong_on()) {
-        // the interpolation inside the sweep stalls its plane pipeline; a
-        // separate streaming pass (phi += P e, 17 B/cell) is cheaper
-        Lev<T> Lc = L;
-        Lc.phi = cur ? L.phi2 : L.phi;
-        int q = (l == 0) ? prof_begin(P, f64 ? MAC_PROF_PROLONG64 : MAC_PROF_PROLONG32) : -1;
-        launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
-        prof_end(P, q);
-        first_mode = SWEEP_PLAIN;
-    }
     for (int it = 0; it < nsweep; ++it) {
         T *pin = cur ? L.phi2 : L.phi;
         T *pout = cur ? L.phi : L.phi2;
@@ -347,8 +321,7 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         const int cls = mode == SWEEP_ZERO ? (f64 ? MAC_PROF_GSRB64_ZERO : MAC_PROF_GSRB32_ZERO)
                                            : (f64 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32);
         int q = (l == 0) ? prof_begin(P, cls) : -1;
-        if (launch_gsrb_fused<T>(L, pin, pout, mode, mode == SWEEP_PROLONG ? &H.lev[l + 1] : nullptr,
-                                 P->bc, P->stream, K)) {
+        if (launch_gsrb_fused<T>(L, pin, pout, mode, P->stream, K)) {
             cur ^= 1;
         } else if (L.alpha && !K) {
             // a level with alpha that no fused sweep takes: the two colours in
@@ -356,7 +329,6 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
             Lev<T> Lc = L;
             Lc.phi = pin;
             if (mode == SWEEP_ZERO) launch_fill<T>(pin, (long)L.nx * L.ny * L.nz, T(0), P->stream);
-            if (mode == SWEEP_PROLONG) launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
             launch_gsrb<T>(3, Lc, 0, P->stream);
             launch_gsrb<T>(3, Lc, 1, P->stream);
         } else {
@@ -364,6 +336,20 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         }
         prof_end(P, q);
     }
+}
+
+// phi += P e on a 3D level's current iterate (the coarse correction as a
+// separate streaming pass: interpolating inside the sweep stalls its plane
+// pipeline on the coarse loads)
+template <typename T>
+static void prolong_cur(mac_plan *P, Hier<T> &H, int l, int cur)
+{
+    Lev<T> &L = H.lev[l];
+    Lev<T> Lc = L;
+    Lc.phi = cur ? L.phi2 : L.phi;
+    int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_PROLONG64 : MAC_PROF_PROLONG32) : -1;
+    launch_prolong<T>(3, Lc, H.lev[l + 1], P->bc, P->stream);
+    prof_end(P, q);
 }
 
 static CoarseOpts coarse_opts(const mac_plan *P, int type)
@@ -424,13 +410,8 @@ static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, cons
         prof_end(P, q);
         cycle_from<T>(P, H, l + 1, type, true, K);
         if (again) cycle_from<T>(P, H, l + 1, again, false, K);
-        if (o.nu2 > 0) {
-            fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PROLONG, cur, Kl);
-        } else {
-            Lev<T> Lc = L;
-            Lc.phi = cur ? L.phi2 : L.phi;
-            launch_prolong<T>(dim, Lc, H.lev[l + 1], P->bc, s);
-        }
+        prolong_cur<T>(P, H, l, cur);
+        if (o.nu2 > 0) fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PLAIN, cur, Kl);
         if (cur)   // bring the iterate back to phi
             HIPCHK_V(hipMemcpyAsync(L.phi, L.phi2, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, s));
         return;
@@ -459,8 +440,7 @@ static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compa
 // compact form)
 static bool compact_applies(mac_plan *P)
 {
-    return P->opts.precision == MAC_PREC_MIXED_BF16 && P->H32.built && use_fused(P) && !fuse_prolong_on() &&
-           !P->H64.alpha_on &&
+    return P->opts.precision == MAC_PREC_MIXED_BF16 && P->H32.built && use_fused(P) && !P->H64.alpha_on &&
            compact_level_ok(P->dim, P->H32.lev[0]);
 }
 
@@ -582,17 +562,18 @@ static bool converged(const mac_opts &o, double r, double r0)
     return false;
 }
 
-// FP32 V-cycles per refinement step; 0 picks by storage.  Three cycles
-// (error reduced by ~1e-3 per step at the V-cycle's rate) replace two FP64
-// update+residual passes of 56 B/cell each by FP32 passes inside the cycle;
-// with bfloat16 storage the rounded operator limits a step's reduction to
-// ~2^-9, so a third cycle buys nothing and two are used.  Measured at 256^3
-// (9 V-cycles in all): FP32 storage 1/2/3 cycles per step 92.5/93.0/105.0
-// solves/s; bfloat16 storage 115.2/117.6/87.6 solves/s.
-static int inner_per_step(const mac_opts &o)
+// FP32 V-cycles per refinement step; 0 picks by the storage level 0 actually
+// uses (compact: whether the finest level is held in bfloat16).  Three
+// cycles (error reduced by ~1e-3 per step at the V-cycle's rate) replace two
+// FP64 update+residual passes of 56 B/cell each by FP32 passes inside the
+// cycle; with bfloat16 storage the rounded operator limits a step's
+// reduction to ~2^-9, so a third cycle buys nothing and two are used.
+// Measured at 256^3 (9 V-cycles in all): FP32 storage 1/2/3 cycles per step
+// 92.5/93.0/105.0 solves/s; bfloat16 storage 115.2/117.6/87.6 solves/s.
+static int inner_per_step(const mac_opts &o, bool compact)
 {
     if (o.inner_cycles > 0) return o.inner_cycles;
-    return o.precision == MAC_PREC_MIXED_BF16 ? 2 : 3;
+    return compact ? 2 : 3;
 }
 
 // Solve A phi = rhs on level 0 of H64 (phi holds the warm start).
@@ -625,27 +606,32 @@ static int solve(mac_plan *P, mac_stats *st)
         if ((rc = ir_norm(P, &r0, K))) return rc;
         r = r0;
         const Lev<float> &E = P->H32.lev[0];
-        while (!converged(o, r, r0) && it < o.max_iter) {
-            for (int c = 0; c < inner_per_step(o); ++c) {
+        const int per_step = inner_per_step(o, K != nullptr);
+        Lev<double> &L0 = P->H64.lev[0];
+        bool swapped = false;
+        while (rc == MAC_OK && !converged(o, r, r0) && it < o.max_iter) {
+            for (int c = 0; c < per_step; ++c) {
                 vcycle<float>(P, P->H32, c == 0, K);
                 ++inner;
             }
             // phi64 + e32 lands in phi2 together with the next residual;
             // the two buffers then trade names
-            Lev<double> &L0 = P->H64.lev[0];
             int q = prof_begin(P, MAC_PROF_IR_UPDATE);
             launch_ir_update_residual(P->dim, L0, E.phi, L0.phi2, E.rhs, res_slot(P), P->stream,
                                       K ? K->rhs : nullptr);
             prof_end(P, q);
             std::swap(L0.phi, L0.phi2);
-            if ((rc = read_norm(P, &r))) return rc;
-            if (P->launch_failed) return MAC_ERR_INVALID;
+            swapped = !swapped;
+            if ((rc = read_norm(P, &r))) break;
+            if (P->launch_failed) { rc = MAC_ERR_INVALID; break; }
             ++it;
             if (o.verbose) fprintf(stderr, "maestro_mac: IR step %d  res/res0 = %.3e\n", it, r / r0);
         }
-        // the device copy of the level table follows the buffer swaps
-        HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &P->H64.lev[0], sizeof(Lev<double>), hipMemcpyHostToDevice,
-                              P->stream));
+        // the device copy of the level table follows the buffer swaps, on
+        // every exit of the loop (a later coarse cycle may read level 0)
+        if (swapped)
+            HIPCHK(hipMemcpyAsync(P->H64.lev_dev, &L0, sizeof(Lev<double>), hipMemcpyHostToDevice, P->stream));
+        if (rc) return rc;
     } else {
         // the FP64 hierarchy's coarse weights (the mixed paths coarsen the FP32 one)
         coarsen_all<double>(P, P->H64);
@@ -1394,8 +1380,7 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         int cur = 0;
         for (int c = 0; c < count; ++c) {
             if (use_fused(P)) {
-                if (!launch_gsrb_fused<float>(F, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, nullptr,
-                                              P->bc, s, K))
+                if (!launch_gsrb_fused<float>(F, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, s, K))
                     return MAC_ERR_INVALID;
                 cur ^= 1;
             } else {
@@ -1410,8 +1395,7 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
     }
     if (op == MAC_OP_GSRB && use_fused(P)) {   // sweeps alternate phi <-> phi2
         for (int c = 0; c < count; ++c)
-            launch_gsrb_fused<double>(L, (c & 1) ? L.phi2 : L.phi, (c & 1) ? L.phi : L.phi2,
-                                      SWEEP_PLAIN, nullptr, P->bc, s);
+            launch_gsrb_fused<double>(L, (c & 1) ? L.phi2 : L.phi, (c & 1) ? L.phi : L.phi2, SWEEP_PLAIN, s);
         if (count & 1)
             HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
                                   hipMemcpyDeviceToDevice, s));
@@ -1421,7 +1405,7 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         switch (op) {
         case MAC_OP_GSRB:
             if (use_fused(P)) {
-                launch_gsrb_fused<double>(L, L.phi, L.phi2, SWEEP_PLAIN, nullptr, P->bc, s);
+                launch_gsrb_fused<double>(L, L.phi, L.phi2, SWEEP_PLAIN, s);
                 HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
                                       hipMemcpyDeviceToDevice, s));
             } else {
@@ -1454,7 +1438,8 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
             if (!has_next) return MAC_ERR_INVALID;
             if (use_fused(P)) {
                 int cur = 0;
-                fused_sweeps<double>(P, H, level, 1, SWEEP_PROLONG, cur);
+                prolong_cur<double>(P, H, level, cur);
+                fused_sweeps<double>(P, H, level, 1, SWEEP_PLAIN, cur);
                 HIPCHK(hipMemcpyAsync(L.phi, L.phi2, sizeof(double) * (size_t)L.nx * L.ny * L.nz,
                                       hipMemcpyDeviceToDevice, s));
             } else {
