@@ -41,15 +41,16 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
     """Name of the finest-level sweep kernel the library launches (b16: the
     compact bfloat16-storage form of the FP32 full-row kernel).  Full rows,
     W = nx / 128 waves per row; rows x threads per block: FP64 6 x 768, FP32
-    8 x 1024 (16 x 1024 at nx = 128), FP32 with bfloat16 storage 6 x 768; the
-    last template argument: no alpha term."""
+    6 x 768 with the weights fetched one plane ahead (16 x 1024 at nx = 128),
+    FP32 with bfloat16 storage 6 x 768; the template's last two arguments: no
+    alpha term, the weights' prefetch distance."""
     t = "double" if s == 8 else "float"
     c = "true" if b16 else "false"
     w = max(1, nx // 128)
     if s == 8 or b16:
-        return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false>"
-    return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false>" if w == 1
-            else f"k_gsrb_rows<{t}, 0, 8, 1024, {c}, {w}, false>")
+        return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 2>"
+    return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false, 2>" if w == 1
+            else f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 1>")
 
 
 def measured_traffic(name: str):
@@ -199,6 +200,104 @@ def time_helmholtz(plan, n, box, torch, reps=3):
     return out
 
 
+def time_star_512(torch, steps=3, warmup=1):
+    """The 512^3 full-star workload (512 boxes of 64^3, n = 1.5 polytrope with
+    a density floor, coefficient contrast 1e6, outflow on every side, rel
+    residual 1e-10) in mixed precision, V- and F-cycles, plus the FP64
+    V-cycle.  Wall time per solve (device pointers, fresh input copies
+    staged in HBM before each timed solve), median of ``steps``."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    n, box = (512,) * 3, 64
+    wl = W.make_workload(3, n=n, box=box)
+    boxes, umac, phi, rho, _, cart = BX.split_workload(wl)
+    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    du0 = [[cv(a) for a in b] for b in umac]
+    dp0 = [cv(a) for a in phi]
+    dr = [cv(a) for a in rho]
+    dc = [[cv(a) for a in b] for b in cart]
+    del umac, phi, rho, cart
+    lay = mac.Layout.uniform(n, box, wl["dx"], wl["bc"])
+    plan = mac.MacPlan(lay, mac.default_opts(device_ptrs=1))
+    out = {"workload": "3D 512^3 full star in a box, 512 boxes of 64^3, n = 1.5 polytrope with a density floor "
+                       "(contrast 1e6), outflow on every side, rel residual 1e-10"}
+    du = [[a.clone() for a in b] for b in du0]
+    dp = [a.clone() for a in dp0]
+    for name, prec, cyc in (("mixed_v", mac.PREC_MIXED, mac.CYCLE_V), ("mixed_f", mac.PREC_MIXED, mac.CYCLE_F),
+                            ("fp64_v", mac.PREC_FP64, mac.CYCLE_V)):
+        plan.set_opts(precision=prec, cycle_type=cyc, device_ptrs=1, max_iter=200)
+        ts, st = [], None
+        for _ in range(warmup + steps):
+            for b, b0 in zip(du, du0):
+                for a, a0 in zip(b, b0):
+                    a.copy_(a0)
+            for a, a0 in zip(dp, dp0):
+                a.copy_(a0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = plan.project(du, dp, dr, None, None, None, dc)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        tm = float(np.median(ts[warmup:]))
+        out[name] = {"ms": round(1e3 * tm, 2), "solves_per_s": round(1.0 / tm, 3), "iters": st["iters"],
+                     "fp32_cycles": st["inner_cycles"], "res_over_res0": st["res"] / st["res0"]}
+        print(f"# c3 512^3 {name}: {1e3 * tm:.1f} ms, {st['iters']} iterations", file=sys.stderr, flush=True)
+    plan.close()
+    return out
+
+
+def time_adjoint_256(torch, steps=3, warmup=1):
+    """The 256^3 differentiable workload: C1's inputs, L = sum(w phi) +
+    1/2 ||U_p||^2 with w ~ N(0, 1), mixed-precision forward (the projection
+    through the torch.autograd.Function) and backward (the adjoint solve and
+    the per-face gradient kernel), rel residual 1e-10.  The face coefficients
+    beta0^2/rho are formed in torch from the workload's rho.  Wall time of
+    each, median of ``steps``."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import autograd as AG
+    n = (256,) * 3
+    wl = W.make_workload(4, n=n, box=64)
+    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    rho = cv(wl["rho"])   # one ghost layer
+    b0, b0e = cv(wl["beta0"]), cv(wl["beta0_edge"])
+    b0f = [b0[:, None, None], b0[:, None, None], b0e[:, None, None]]
+    # face beta = 2 / (rho_L + rho_R) beta0^2 on every face (the walls' normal
+    # faces carry no flux: their velocity is zero)
+    bx = 2.0 / (rho[1:-1, 1:-1, :-1] + rho[1:-1, 1:-1, 1:]) * b0f[0] ** 2
+    by = 2.0 / (rho[1:-1, :-1, 1:-1] + rho[1:-1, 1:, 1:-1]) * b0f[1] ** 2
+    bz = 2.0 / (rho[:-1, 1:-1, 1:-1] + rho[1:, 1:-1, 1:-1]) * b0f[2] ** 2
+    B = [bx.contiguous(), by.contiguous(), bz.contiguous()]
+    U = [cv(f) for f in wl["faces"]]
+    w = torch.from_numpy(np.random.default_rng(4321).standard_normal((n[2], n[1], n[0]))).cuda()
+    plan = AG.level_plan(n, wl["dx"], wl["bc"], precision=mac.PREC_MIXED, eps_rel=1e-10)
+    tf, tb = [], []
+    for _ in range(warmup + steps):
+        Ul = [u.clone().requires_grad_(True) for u in U]
+        Bl = [b.clone().requires_grad_(True) for b in B]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Us = [Ul[d] * b0f[d] for d in range(3)]
+        phi, ux, uy, uz = AG.mac_project(plan, Us[0], Us[1], Us[2], Bl[0], Bl[1], Bl[2])
+        Up = [ux / b0f[0], uy / b0f[1], uz / b0f[2]]
+        L = (w * phi).sum() + 0.5 * sum((u * u).sum() for u in Up)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    plan.close()
+    f, b = float(np.median(tf[warmup:])), float(np.median(tb[warmup:]))
+    print(f"# c4 256^3 adjoint: forward {1e3 * f:.1f} ms, backward {1e3 * b:.1f} ms", file=sys.stderr, flush=True)
+    return {"workload": "3D 256^3 differentiable MAC projection (C1 inputs), L = sum(w phi) + 1/2 ||U_p||^2, "
+                        "mixed precision, rel residual 1e-10",
+            "forward_ms": round(1e3 * f, 2), "backward_ms": round(1e3 * b, 2),
+            "forward_plus_backward_per_s": round(1.0 / (f + b), 3)}
+
+
 def time_host_pointers(plan, pb, wl, prec):
     """One projection with host (numpy) per-box arrays: the library copies
     them in and out around the solve (pageable host memory over PCIe).
@@ -234,11 +333,15 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--box", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-solves", type=int, default=5,
+                    help="timed oracle solves of the CPU baseline (median; after one warm-up)")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy check")
     ap.add_argument("--no-profile", action="store_true",
                     help="time the steps without the per-kernel HIP events (no roofline figures)")
     ap.add_argument("--no-producers", action="store_true",
                     help="skip timing make_macrhs / make_s0mac and the Helmholtz mac_solve on the bench layout")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip timing the 512^3 star (mixed V/F, FP64) and the 256^3 adjoint")
     ap.add_argument("--headline", default="mixed", choices=("mixed", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=0,
@@ -323,6 +426,9 @@ def main():
     host = None if args.no_producers else time_host_pointers(
         plan, pb, wl, {"mixed": mac.PREC_MIXED, "mixed_bf16": mac.PREC_MIXED_BF16}[args.headline])
 
+    c3 = None if args.no_configs else time_star_512(torch)
+    c4 = None if args.no_configs else time_adjoint_256(torch)
+
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
     value = args.steps / mixed["seconds"]
@@ -359,13 +465,18 @@ def main():
         from oracle import oracle as O
         O.build()
         g = O.make_grid(wl["n"], wl["dx"], wl["bc"])
-        t = time.perf_counter()
-        ref = O.macproject(g, wl["faces"], wl["phi"], wl["rho"], beta0=wl["beta0"],
-                           beta0_edge=wl["beta0_edge"], opts=O.default_opts())
-        tc = time.perf_counter() - t
+        tcs = []
+        for _ in range(1 + args.cpu_solves):   # one warm-up, then the timed solves
+            t = time.perf_counter()
+            ref = O.macproject(g, wl["faces"], wl["phi"], wl["rho"], beta0=wl["beta0"],
+                               beta0_edge=wl["beta0_edge"], opts=O.default_opts())
+            tcs.append(time.perf_counter() - t)
+        tc = float(np.median(tcs[1:]))
         cpu = {"value": 1.0 / tc, "unit": "solves/s", "cores": O.num_threads(), "kind": "port",
-               "sample": f"one full 256^3 FP64 solve of the same inputs by the C/OpenMP oracle "
-                         f"({ref['stats'].iters} V-cycles, {tc:.1f} s)"}
+               "sample": f"full 256^3 FP64 solves of the same inputs by the C/OpenMP oracle (the build's "
+                         f"CPU restatement of macproject and the FBoxLib-documented V-cycle), "
+                         f"{ref['stats'].iters} V-cycles each: median of {args.cpu_solves} after one warm-up, "
+                         f"{tc:.2f} s (all: {', '.join('%.2f' % x for x in tcs[1:])} s)"}
         if not args.no_check:
             po = ref["phi"] - ref["phi"].mean()
             for path in ("fp64", "mixed", "mixed_bf16"):
@@ -419,6 +530,8 @@ def main():
         "helmholtz": helmholtz,
         "host_pointers": host,
         "cpu_baseline": cpu,
+        "c3_star_512": c3,
+        "c4_adjoint_256": c4,
     }
     print(json.dumps(line))
 

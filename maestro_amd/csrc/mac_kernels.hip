@@ -66,7 +66,46 @@ __device__ __forceinline__ double beta0_face(const SetupArgs &A, int b, int d, i
     return 1.0;
 }
 
+// ------------------------------------------------- column addressing
+// A box array (ghost width ng, nodal in direction fd, or -1 for cells) seen
+// from the column (li, lj) of the box's valid region: the element offset of
+// the column's first valid plane and the stride between planes.  32-bit: a
+// box array holds fewer than 2^31 elements.
+struct Col {
+    int off, sy, pz;   // first valid element of the column, row and plane strides
+    // element offset of the neighbour (di, dj, dk) of the column's plane kk
+    __device__ __forceinline__ int at(int kk, int di = 0, int dj = 0, int dk = 0) const
+    {
+        return off + (kk + dk) * pz + dj * sy + di;
+    }
+};
+template <int DIM>
+__device__ __forceinline__ Col col_of(const BoxDev &B, int ng, int fd, int li, int lj)
+{
+    const int ex = B.hi[0] - B.lo[0] + 1 + 2 * ng + (fd == 0);
+    const int ey = B.hi[1] - B.lo[1] + 1 + 2 * ng + (fd == 1);
+    Col c;
+    c.sy = ex;
+    c.pz = ex * ey;
+    c.off = (li + ng) + ex * (lj + ng) + (DIM == 3 ? c.pz * ng : 0);
+    return c;
+}
+
+// Planes of a box that one thread of the column kernels marches through
+// (3D; the grid's z dimension counts the chunks).  Eight keep some 32K waves
+// in flight at 256^3 while the carried values save all but 1/8 of the
+// re-reads of the plane below.
+static constexpr int ZCH = 8;
+
 // ------------------------------------------------------------- k_setup
+// One thread per (i, j) column of a box marching k through a chunk of ZCH
+// planes (3D): the scaled z faces and rho of the plane below are carried in
+// registers, so each caller array element is read once from HBM (the x and y
+// neighbours come from the same cache lines the neighbouring lanes read),
+// and the indexing is 32-bit.  Per cell: rh = macrhs - D(beta0 U), the folded face
+// coefficients 2/(rho_L + rho_R) beta0^p (with the domain's high faces into
+// the slabs) and the warm start.  The column's rh values are summed
+// compensated, then per block (the solvability mean).
 template <int DIM>
 __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
 {
@@ -74,82 +113,152 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
     const BoxDev B = A.boxes[b];
     const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
     const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
-    const long q = (long)blockIdx.x * TPB + threadIdx.x;
-    double rh = 0.0;
+    const int q = blockIdx.x * TPB + threadIdx.x;
+    const int kz0 = (DIM == 3) ? (int)blockIdx.z * ZCH : 0, kz1 = (DIM == 3) ? min(kz0 + ZCH, ez) : 1;
+    double s = 0.0, cmp = 0.0;
     bool bad = false;
-    if (q < (long)ex * ey * ez) {
-        const int i = B.lo[0] + (int)(q % ex);
-        const int j = B.lo[1] + (int)((q / ex) % ey);
-        const int k = (DIM == 3) ? B.lo[2] + (int)(q / ((long)ex * ey)) : 0;
-        const int I = i - A.dlo[0], J = j - A.dlo[1], K = (DIM == 3) ? k - A.dlo[2] : 0;
+    if (q < ex * ey && kz0 < kz1) {
+        const int li = q % ex, lj = q / ex;
+        const int i = B.lo[0] + li, j = B.lo[1] + lj;
+        const int I = i - A.dlo[0], J = j - A.dlo[1];
         const int nx = A.n[0], ny = A.n[1];
-        const long c = I + (long)nx * (J + (long)ny * K);
-        const double *rho = A.rho[b];
-        auto R = [&](int ii, int jj, int kk) { return rho[boxoff(B, A.ng_rho, -1, DIM, ii, jj, kk)]; };
-        // ---- beta0-scaled face velocities and divergence (divumac)
-        double uf[DIM][2];
+        const int rd = DIM - 1;   // radial direction of a planar beta0
+        const Col cr = col_of<DIM>(B, A.ng_rho, -1, li, lj), cp = col_of<DIM>(B, A.ng_phi, -1, li, lj);
+        const Col cm = col_of<DIM>(B, A.ng_rhs, -1, li, lj);
+        Col cu[3];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) cu[d] = col_of<DIM>(B, A.ng_u, d, li, lj);
+        const double *__restrict__ rho = A.rho[b];
+        const double *__restrict__ phi = A.phi[b];
+        const double *__restrict__ mrh = A.macrhs ? A.macrhs[b] : nullptr;
+        double *__restrict__ lrhs = A.L.rhs;
+        double *__restrict__ lphi = A.L.phi;
+        const double *u[3];
+        const double *cart[3];
+        Col cc[3];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const double *u = A.umac[b * DIM + d];
-#pragma unroll
-            for (int s = 0; s < 2; ++s) {
-                int fi = i + (d == 0 ? s : 0), fj = j + (d == 1 ? s : 0), fk = k + (d == 2 ? s : 0);
-                double v = u[boxoff(B, A.ng_u, d, DIM, fi, fj, fk)];
-                bool has;
-                double b0 = beta0_face(A, b, d, fi, fj, fk, has);
-                if (has) v = v * b0;
-                uf[d][s] = v;
+            u[d] = A.umac[b * DIM + d];
+            cart[d] = A.cart ? A.cart[b * DIM + d] : nullptr;
+            cc[d] = col_of<DIM>(B, A.ng_cart, d, li, lj);
+        }
+        // beta0 of face s (0 low, 1 high) of direction d at plane kk; has: any
+        auto b0f = [&](int d, int sd, int kk, bool &has) -> double {
+            if (A.beta0) {
+                has = true;
+                const int r = (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd] + (d == rd ? sd : 0);
+                return (d == rd) ? A.beta0e[r] : A.beta0[r];
             }
-        }
-        double D = (uf[0][1] - uf[0][0]) / A.dx[0] + (uf[1][1] - uf[1][0]) / A.dx[1];
-        if (DIM == 3) D = D + (uf[2][1] - uf[2][0]) / A.dx[2];
-        if (A.macrhs) {
-            double m = A.macrhs[b][boxoff(B, A.ng_rhs, -1, DIM, i, j, k)];
-            rh = -(D - m);
-        } else {
-            rh = -D;
-        }
-        A.L.rhs[c] = rh;
-        bad |= !isfinite(rh);
-        // ---- face coefficients (mk_mac_coeffs, beta0 scaling, BC folding)
-        const double rc = R(i, j, k);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            int di = (d == 0), dj = (d == 1), dk = (d == 2);
-            int Id = (d == 0) ? I : (d == 1 ? J : K);
-            double beta = 2.0 / (rc + R(i - di, j - dj, k - dk));
+            if (A.cart) {
+                has = true;
+                return cart[d][cc[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0)];
+            }
+            has = false;
+            return 1.0;
+        };
+        // the caller values one plane needs (raw face velocities: [0, 1] x low
+        // and high, [2, 3] y, [4] the z high face; rho of the cell and of its
+        // x and y low neighbours; macrhs; the warm start).  The next plane's
+        // are loaded before this plane's results are stored, so the march
+        // never waits on its own stores.
+        struct PIn {
+            double u[5], r, rx, ry, m, p;
+        };
+        auto load = [&](int kk) -> PIn {
+            PIn o;
+            o.u[0] = u[0][cu[0].at(kk)];
+            o.u[1] = u[0][cu[0].at(kk, 1)];
+            o.u[2] = u[1][cu[1].at(kk)];
+            o.u[3] = u[1][cu[1].at(kk, 0, 1)];
+            o.u[4] = (DIM == 3) ? u[2][cu[2].at(kk, 0, 0, 1)] : 0.0;
+            o.r = rho[cr.at(kk)];
+            o.rx = rho[cr.at(kk, -1)];
+            o.ry = rho[cr.at(kk, 0, -1)];
+            o.m = mrh ? mrh[cm.at(kk)] : 0.0;
+            o.p = phi[cp.at(kk)];
+            return o;
+        };
+        // scaled z faces and rho of the previous plane (3D carries)
+        double uz_lo = 0.0, r_below = 0.0;
+        if (DIM == 3) {
             bool has;
-            double b0 = beta0_face(A, b, d, i, j, k, has);
-            if (has) {
-                beta = beta * b0;
-                if (A.alt) beta = beta * b0;
-            }
-            if (Id == 0) beta = fold_bc(beta, A.bc.k[d][0]);
-            bad |= !isfinite(beta);
-            double *bl = (d == 0) ? A.L.bx : (d == 1 ? A.L.by : A.L.bz);
-            bl[c] = beta;
-            if (Id == A.n[d] - 1) {
-                double bh = 2.0 / (R(i + di, j + dj, k + dk) + rc);
-                double b0h = beta0_face(A, b, d, i + di, j + dj, k + dk, has);
-                if (has) {
-                    bh = bh * b0h;
-                    if (A.alt) bh = bh * b0h;
-                }
-                bh = fold_bc(bh, A.bc.k[d][1]);
-                bad |= !isfinite(bh);
-                if (d == 0) A.L.bxh[J + (long)ny * K] = bh;
-                else if (d == 1) A.L.byh[I + (long)nx * K] = bh;
-                else A.L.bzh[I + (long)nx * J] = bh;
-            }
+            const double b0 = b0f(2, 0, kz0, has);
+            uz_lo = u[2][cu[2].at(kz0)];
+            if (has) uz_lo = uz_lo * b0;
+            r_below = rho[cr.at(kz0, 0, 0, -1)];
         }
-        // ---- warm start
-        double p0 = A.phi[b][boxoff(B, A.ng_phi, -1, DIM, i, j, k)];
-        bad |= !isfinite(p0);
-        A.L.phi[c] = p0;
+        PIn cur = load(kz0);
+        for (int kk = kz0; kk < kz1; ++kk) {
+            const PIn nxt = load(min(kk + 1, kz1 - 1));
+            const int k = (DIM == 3) ? B.lo[2] + kk : 0, K = (DIM == 3) ? k - A.dlo[2] : 0;
+            const int c = I + nx * (J + ny * K);
+            double uf[DIM][2];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+                for (int sd = 0; sd < 2; ++sd) {
+                    if (DIM == 3 && d == 2 && sd == 0) {
+                        uf[2][0] = uz_lo;
+                        continue;
+                    }
+                    double v = cur.u[d == 2 ? 4 : 2 * d + sd];
+                    bool has;
+                    const double b0 = b0f(d, sd, kk, has);
+                    if (has) v = v * b0;
+                    uf[d][sd] = v;
+                }
+            }
+            double D = (uf[0][1] - uf[0][0]) / A.dx[0] + (uf[1][1] - uf[1][0]) / A.dx[1];
+            if (DIM == 3) D = D + (uf[2][1] - uf[2][0]) / A.dx[2];
+            double rh;
+            if (mrh) rh = -(D - cur.m);
+            else rh = -D;
+            lrhs[c] = rh;
+            bad |= !isfinite(rh);
+            comp_add(s, cmp, rh, 0.0);
+            // ---- face coefficients (mk_mac_coeffs, beta0 scaling, BC folding)
+            const double rc = cur.r;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                const int Id = (d == 0) ? I : (d == 1 ? J : K);
+                const double rl = (DIM == 3 && d == 2) ? r_below : (d == 0 ? cur.rx : cur.ry);
+                double beta = 2.0 / (rc + rl);
+                bool has;
+                const double b0 = b0f(d, 0, kk, has);
+                if (has) {
+                    beta = beta * b0;
+                    if (A.alt) beta = beta * b0;
+                }
+                if (Id == 0) beta = fold_bc(beta, A.bc.k[d][0]);
+                bad |= !isfinite(beta);
+                double *bl = (d == 0) ? A.L.bx : (d == 1 ? A.L.by : A.L.bz);
+                bl[c] = beta;
+                if (Id == A.n[d] - 1) {
+                    double bh = 2.0 / (rho[cr.at(kk, d == 0, d == 1, d == 2)] + rc);
+                    const double b0h = b0f(d, 1, kk, has);
+                    if (has) {
+                        bh = bh * b0h;
+                        if (A.alt) bh = bh * b0h;
+                    }
+                    bh = fold_bc(bh, A.bc.k[d][1]);
+                    bad |= !isfinite(bh);
+                    if (d == 0) A.L.bxh[J + ny * K] = bh;
+                    else if (d == 1) A.L.byh[I + nx * K] = bh;
+                    else A.L.bzh[I + nx * J] = bh;
+                }
+            }
+            // ---- warm start
+            bad |= !isfinite(cur.p);
+            lphi[c] = cur.p;
+            if (DIM == 3) {
+                uz_lo = uf[2][1];
+                r_below = rc;
+            }
+            cur = nxt;
+        }
     }
     if (bad) atomicOr(A.err, 1);
     // ---- block compensated partial sum of rh (solvability mean)
-    double s = rh, cmp = 0.0;
     wave_comp_sum(s, cmp);
     __shared__ double ss[TPB / 64], sc[TPB / 64];
     const int wv = threadIdx.x >> 6;
@@ -158,7 +267,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
     if (threadIdx.x == 0) {
         double S = ss[0], Cc = sc[0];
         for (int w = 1; w < TPB / 64; ++w) comp_add(S, Cc, ss[w], sc[w]);
-        long pid = (long)b * gridDim.x + blockIdx.x;
+        long pid = ((long)b * gridDim.z + blockIdx.z) * gridDim.x + blockIdx.x;
         A.part[2 * pid] = S;
         A.part[2 * pid + 1] = Cc;
     }
@@ -655,16 +764,24 @@ __device__ __forceinline__ V ld_rs(__amdgpu_buffer_rsrc_t r, unsigned voff, unsi
     else
         return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
-template <typename V>
+// Cache policy of a buffer access: 0 default; CPOL_NT the non-temporal
+// (streaming) hint.  The sweeps write their output with it: the new iterate
+// is written once and read by the next launch, and the streaming stores
+// measured 2-3% faster per sweep at 256^3 (FP32 111 -> 108.6 us, FP64 173 ->
+// 167.8 us).  Their loads keep the default: the y-ring rows a neighbouring
+// tile re-reads come from L2 only if the first read allocated them there.
+static constexpr int CPOL_NT = 2;
+
+template <typename V, int CPOL = 0>
 __device__ __forceinline__ void st_rs(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, const V &v)
 {
     static_assert(sizeof(V) == 8 || sizeof(V) == 16, "buffer store width");
     if constexpr (sizeof(V) == 8)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
-                                              r, voff, soff, 0);
+                                              r, voff, soff, CPOL);
     else
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
-                                               r, voff, soff, 0);
+                                               r, voff, soff, CPOL);
 }
 
 // Full-row sweep, nx = 128 W (W a template parameter, so every LDS offset is
@@ -688,8 +805,12 @@ struct RowBufs {
     unsigned alpha;   // AL only (alpha is part of the weights' allocation)
 };
 
-// AL: the level has alpha (a column pair per plane beside the weights)
-template <typename T, int MODE, int EY, int NTH, bool B16, int W, bool AL = false>
+// AL: the level has alpha (a column pair per plane beside the weights).
+// PDW: how many planes ahead the weights, rhs and ring-2 values are fetched
+// (the z faces always two): 2, or 1, which keeps one slot of the input ring
+// fewer in registers (the FP32 form of 6 rows then fits 80 VGPRs, two blocks
+// per CU).
+template <typename T, int MODE, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2>
 __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweepGrid G)
 {
     static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
@@ -824,6 +945,21 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
         }
         return o;
     };
+    // PDW = 1: everything but the z faces one plane ahead, into the slot of
+    // plane k+1 (whose z faces were fetched a step earlier); z faces two ahead
+    auto fetch_w = [&](int k, In &o) {
+        const In f = fetch(k);
+#pragma unroll
+        for (int t = 0; t < NI; ++t)
+            if (t != 7 && t != 8) o.v[t] = f.v[t];
+    };
+    auto fetch_z = [&](int k, In &o) {
+        int kw;
+        zwrap(k, kw);
+        const T2 vz = ldw(Rw, vw, B.bz + (unsigned)kw * (unsigned)plane * WS);
+        o.v[7] = vz.x;
+        o.v[8] = vz.y;
+    };
     const int kb = z0 - 1, ke = z1;
     const int nsteps = ((ke - kb + 1) + 7) & ~7;   // a multiple of the unroll
     // rings indexed by the step m = k - kb: inputs of plane m in wq[m & 3],
@@ -853,7 +989,8 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
     put(std::integral_constant<int, 5>{}, ldphi(min(kb + 5, ke + 1)));
     PH(6, 0) = PH(6, 1) = T(0);
     wq[0] = fetch(kb);
-    wq[1] = fetch(min(kb + 1, ke + 1));
+    if constexpr (PDW == 2) wq[1] = fetch(min(kb + 1, ke + 1));
+    else fetch_z(min(kb + 1, ke + 1), wq[1]);
     __syncthreads();
 
     auto step = [&](auto SC, int m) __attribute__((always_inline)) {
@@ -864,7 +1001,12 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
         const int k = kb + m;
         int kw;
         const bool kok = zwrap(min(k, ke + 1), kw) && k <= ke;
-        wq[CN] = fetch(min(k + 2, ke + 1));
+        if constexpr (PDW == 2) {
+            wq[CN] = fetch(min(k + 2, ke + 1));
+        } else {
+            fetch_w(min(k + 1, ke + 1), wq[C1]);
+            fetch_z(min(k + 2, ke + 1), wq[CN]);
+        }
         // (a) publish the old values of plane k (and the ring-2 row)
         if (oky) {
             T2 v;
@@ -952,7 +1094,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
-                st_rs<T2>(Rp, vp, B.out + (unsigned)km * (unsigned)plane * ES, o);
+                st_rs<T2, CPOL_NT>(Rp, vp, B.out + (unsigned)km * (unsigned)plane * ES, o);
             }
         };
         if (r) update(std::integral_constant<int, 1>{});
@@ -2242,66 +2384,127 @@ __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict_
 }
 
 // ------------------------------------------------------ face correction
-// valid faces of every box: U = (beta0 U - w (phi_hi - phi_lo)/dx) / beta0
+// Valid faces of every box: U = (beta0 U - w (phi_hi - phi_lo)/dx) / beta0.
+// One thread per (i, j) column of a box marching k through a chunk of ZCH
+// planes: each cell corrects its three low faces, and the cells of the box's
+// last column, row and plane its high faces too.  phi of the plane below is carried, so each
+// phi value is read once from HBM (the x and y neighbours come from the cache
+// lines the neighbouring lanes read), and the thread also copies phi into the
+// caller's phi box.  32-bit indexing.
 template <int DIM>
 __global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
 {
-    const int b = blockIdx.y, d = blockIdx.z;
+    const int b = blockIdx.y;
     const BoxDev B = A.boxes[b];
-    const int ex = B.hi[0] - B.lo[0] + 1 + (d == 0), ey = B.hi[1] - B.lo[1] + 1 + (d == 1);
-    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 + (d == 2) : 1;
-    const long q = (long)blockIdx.x * TPB + threadIdx.x;
-    if (q >= (long)ex * ey * ez) return;
-    const int i = B.lo[0] + (int)(q % ex), j = B.lo[1] + (int)((q / ex) % ey);
-    const int k = (DIM == 3) ? B.lo[2] + (int)(q / ((long)ex * ey)) : 0;
-    int G[3] = {i - A.dlo[0], j - A.dlo[1], (DIM == 3) ? k - A.dlo[2] : 0};
-    const int nd = A.n[d];
-    // effective weight of the face and the phi values on both sides
-    int f = G[d];
-    int per = (A.bc.k[d][0] == ELL_PER);
-    if (per && f == nd) f = 0;
-    int Cu[3] = {G[0], G[1], G[2]};
-    Cu[d] = f;
-    const long sx = 1, sy = A.n[0], sz = (long)A.n[0] * A.n[1];
-    const long s[3] = {sx, sy, sz};
-    double w;
-    double phi_hi, phi_lo;
-    if (f < nd) {
-        long cu = Cu[0] + sy * Cu[1] + sz * Cu[2];
-        w = (d == 0 ? A.bx : (d == 1 ? A.by : A.bz))[cu];
-        phi_hi = A.phi[cu];
-        if (f > 0) phi_lo = A.phi[cu - s[d]];
-        else phi_lo = per ? A.phi[cu + (long)(nd - 1) * s[d]] : 0.0;
-    } else {  // domain high face, non-periodic
-        long cl = Cu[0] + sy * Cu[1] + sz * Cu[2] - s[d];
-        if (d == 0) w = A.bxh[G[1] + (long)A.n[1] * G[2]];
-        else if (d == 1) w = A.byh[G[0] + (long)A.n[0] * G[2]];
-        else w = A.bzh[G[0] + (long)A.n[0] * G[1]];
-        phi_hi = 0.0;
-        phi_lo = A.phi[cl];
+    const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
+    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
+    const int q = blockIdx.x * TPB + threadIdx.x;
+    const int kz0 = (DIM == 3) ? (int)blockIdx.z * ZCH : 0, kz1 = (DIM == 3) ? min(kz0 + ZCH, ez) : 1;
+    if (q >= ex * ey || kz0 >= kz1) return;
+    const int li = q % ex, lj = q / ex;
+    const int i = B.lo[0] + li, j = B.lo[1] + lj;
+    const int I = i - A.dlo[0], J = j - A.dlo[1];
+    const int nx = A.n[0], ny = A.n[1];
+    const int st[3] = {1, nx, nx * ny};   // level strides
+    const int rd = DIM - 1;
+    Col cu[3], cc[3];
+    double *u[3];
+    const double *cart[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        cu[d] = col_of<DIM>(B, A.ng_u, d, li, lj);
+        cc[d] = col_of<DIM>(B, A.ng_cart, d, li, lj);
+        u[d] = A.umac[b * DIM + d];
+        cart[d] = A.cart ? A.cart[b * DIM + d] : nullptr;
     }
-    double *u = A.umac[b * DIM + d];
-    long o = boxoff(B, A.ng_u, d, DIM, i, j, k);
-    double b0 = 1.0;
-    bool has = false;
-    if (A.beta0) {
-        int rd = DIM - 1;
-        int r = (rd == 2 ? k : j) - A.dlo[rd];
-        b0 = (d == rd) ? A.beta0e[r] : A.beta0[r];
-        has = true;
-    } else if (A.cart) {
-        b0 = A.cart[b * DIM + d][boxoff(B, A.ng_cart, d, DIM, i, j, k)];
-        has = true;
+    const Col cp = col_of<DIM>(B, A.ng_phi, -1, li, lj);
+    double *pout = A.phi_out ? A.phi_out[b] : nullptr;
+    const double *wl[3] = {A.bx, A.by, A.bz};
+    // one face: its position in the column (sd: 0 low, 1 high face of the
+    // cell), its velocity as read, effective weight and the phi values on both
+    // sides; the corrected velocity is stored
+    auto correct = [&](int d, int sd, int kk, double t, double w, double phi_hi, double phi_lo) {
+        const int o = cu[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0);
+        double b0 = 1.0;
+        bool has = false;
+        if (A.beta0) {
+            const int r = (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd] + (d == rd ? sd : 0);
+            b0 = (d == rd) ? A.beta0e[r] : A.beta0[r];
+            has = true;
+        } else if (A.cart) {
+            b0 = cart[d][cc[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0)];
+            has = true;
+        }
+        if (has) t = t * b0;
+        const double gphi = (phi_hi - phi_lo) / A.dx[d];
+        t = t - w * gphi;
+        if (has) t = t / b0;
+        u[d][o] = t;
+    };
+    auto face_u = [&](int d, int sd, int kk) {
+        return u[d][cu[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0)];
+    };
+    // phi below the cell's low face in direction d (periodic image, or 0
+    // outside a non-periodic side, whose folded weight carries the BC)
+    auto below = [&](int d, int Id, int c) -> double {
+        if (Id > 0) return A.phi[c - st[d]];
+        return (A.bc.k[d][0] == ELL_PER) ? A.phi[c + (A.n[d] - 1) * st[d]] : 0.0;
+    };
+    const int K0 = (DIM == 3) ? B.lo[2] - A.dlo[2] : 0;
+    // what one plane's low faces need: their velocities and weights, phi of
+    // the cell and of its x and y low neighbours.  The next plane's are
+    // loaded before this plane's faces are stored, so the march never waits
+    // on its own stores.
+    struct CIn {
+        double u[3], w[3], pc, px, py;
+    };
+    auto load = [&](int kk) -> CIn {
+        CIn o;
+        const int c = I + nx * (J + ny * (K0 + kk));
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            o.u[d] = d < DIM ? face_u(d, 0, kk) : 0.0;
+            o.w[d] = d < DIM ? wl[d][c] : 0.0;
+        }
+        o.pc = A.phi[c];
+        o.px = below(0, I, c);
+        o.py = below(1, J, c);
+        return o;
+    };
+    double pz_lo = 0.0;
+    if (DIM == 3) pz_lo = below(2, K0 + kz0, I + nx * (J + ny * (K0 + kz0)));
+    CIn cur = load(kz0);
+    for (int kk = kz0; kk < kz1; ++kk) {
+        const CIn nxt = load(min(kk + 1, kz1 - 1));
+        const int K = K0 + kk;
+        const int c = I + nx * (J + ny * K);
+        const int Ix[3] = {I, J, K};
+        const double pc = cur.pc;
+        // low faces
+        correct(0, 0, kk, cur.u[0], cur.w[0], pc, cur.px);
+        correct(1, 0, kk, cur.u[1], cur.w[1], pc, cur.py);
+        if (DIM == 3) correct(2, 0, kk, cur.u[2], cur.w[2], pc, pz_lo);
+        // the box's high faces
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const bool last = d == 0 ? li == ex - 1 : (d == 1 ? lj == ey - 1 : kk == ez - 1);
+            if (!last) continue;
+            const int nd = A.n[d];
+            const double t = face_u(d, 1, kk);
+            if (Ix[d] + 1 < nd) {
+                correct(d, 1, kk, t, wl[d][c + st[d]], A.phi[c + st[d]], pc);
+            } else if (A.bc.k[d][0] == ELL_PER) {   // the periodic image of face 0
+                const int c0 = c - (nd - 1) * st[d];
+                correct(d, 1, kk, t, wl[d][c0], A.phi[c0], pc);
+            } else {                                 // the domain's high face
+                const double w = d == 0 ? A.bxh[J + ny * K] : (d == 1 ? A.byh[I + nx * K] : A.bzh[I + nx * J]);
+                correct(d, 1, kk, t, w, 0.0, pc);
+            }
+        }
+        if (pout) pout[cp.at(kk)] = pc;
+        pz_lo = pc;
+        cur = nxt;
     }
-    double t = u[o];
-    if (has) t = t * b0;
-    double gphi = (phi_hi - phi_lo) / A.dx[d];
-    t = t - w * gphi;
-    if (has) t = t / b0;
-    u[o] = t;
-    // the x faces' pass also copies phi of each valid cell (the face's high
-    // side) into the caller's box array
-    if (d == 0 && A.phi_out && i <= B.hi[0]) A.phi_out[b][boxoff(B, A.ng_phi, -1, DIM, i, j, k)] = phi_hi;
 }
 
 // ghost faces that lie inside the (periodic) domain take the value of the
@@ -2652,13 +2855,18 @@ static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
         }                                 \
     } while (0)
 
-void launch_setup(const SetupArgs &A, int maxcells, hipStream_t s)
+static unsigned zchunks(int dim, int maxez) { return dim == 3 ? (unsigned)((maxez + ZCH - 1) / ZCH) : 1u; }
+
+void launch_setup(const SetupArgs &A, int maxcols, int maxez, hipStream_t s)
 {
-    dim3 g(nblk(maxcells), A.nboxes, 1);
+    dim3 g(nblk(maxcols), A.nboxes, zchunks(A.dim, maxez));
     DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_setup<D_>, g, dim3(TPB), 0, s, A));
 }
 
-long setup_nparts(int maxcells, int nboxes) { return (long)nblk(maxcells) * nboxes; }
+long setup_nparts(int dim, int maxcols, int maxez, int nboxes)
+{
+    return (long)nblk(maxcols) * nboxes * zchunks(dim, maxez);
+}
 
 void launch_mean(const double *part, long nparts, double ncell, double *mean, hipStream_t s)
 {
@@ -2795,7 +3003,7 @@ static bool row_bufs(const Lev<T> &L, const T *pin, T *pout, const Compact16 *K,
     return true;
 }
 
-template <typename T, int EY, int NTH, bool B16, int W, bool AL = false>
+template <typename T, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2>
 static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
@@ -2808,9 +3016,9 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return false;
         hipError_t e = (mode == SWEEP_ZERO)
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W, AL>,
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW>,
                                                                           threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W, AL>,
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW>,
                                                                           threads, lds);
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
@@ -2844,19 +3052,18 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW>), g, b, lds, s, L, RB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W, AL>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW>), g, b, lds, s, L, RB, G);
     return true;
 }
 
 // Full-row sweep: usable when nx = 128 W (W = 1, 2) and a block of EY rows
-// (64 W EY threads) fits in NTH.  FP32 runs 8 rows in blocks of up to 1024
-// threads (at most 128 VGPRs per lane); FP64, whose register rings are twice
-// as wide, runs 6 rows in blocks of up to 768.  The z-chunk count is chosen so
-// the number of blocks fills whole rounds of the resident block slots (CUs x
-// blocks per CU) with the least z-ring overhead.
-template <typename T, int EY, int NTH, bool B16>
+// (64 W EY threads) fits in NTH.  The z-chunk count is chosen so the number
+// of blocks fills whole rounds of the resident block slots (CUs x blocks per
+// CU) with the least z-ring overhead.  PDW: the kernel's prefetch distance of
+// the weights (with alpha it stays 2).
+template <typename T, int EY, int NTH, bool B16, int PDW = 2>
 static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 *K)
 {
     RowBufs RB;
@@ -2869,8 +3076,8 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     } else {
         if (L.alpha) return false;
     }
-    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1>(L, RB, mode, s);
-    return launch_rows_w<T, EY, NTH, B16, 2>(L, RB, mode, s);
+    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1, false, PDW>(L, RB, mode, s);
+    return launch_rows_w<T, EY, NTH, B16, 2, false, PDW>(L, RB, mode, s);
 }
 
 bool compact_level_ok(int dim, const Lev<float> &L)
@@ -2880,13 +3087,15 @@ bool compact_level_ok(int dim, const Lev<float> &L)
 
 // One fused red-black sweep of a 3D level (pin -> pout); false where no fused
 // form applies (the caller then sweeps per colour).
-//   - nx = 128 or 256: the full-row sweep.  FP32 runs 8 rows in blocks of
-//     1024 threads (16 rows at nx = 128), the compact (bfloat16) form 6 rows
-//     of 768 threads: a 256^3 level then gives 64 y-tiles x 4 z-chunks, one
-//     block per CU on all 256 CUs, where 8 rows leave 43 x 5 = 215 blocks and
-//     idle CUs.  FP64 runs 6 rows of 768 threads, and 4 rows of 512 with alpha
-//     (the alpha ring does not fit the 168 VGPRs a 768-thread block leaves a
-//     lane).
+//   - nx = 128 or 256: the full-row sweep.  FP32 at nx = 256 and the compact
+//     (bfloat16) form run 6 rows in blocks of 768 threads with 80 VGPRs (the
+//     FP32 form with the weights fetched one plane ahead), two blocks per CU:
+//     a 256^3 level gives 64 y-tiles x 8 z-chunks, 512 blocks on 256 CUs.
+//     8 rows of 1024 threads (88 VGPRs, one block per CU) leave 43 x 5 = 215
+//     blocks and idle CUs: 111 us against 106.5 us per FP32 sweep.  FP32 at
+//     nx = 128 runs 16 rows of 1024 threads, and 8 rows of 1024 with alpha.
+//     FP64 runs 6 rows of 768 threads, and 4 rows of 512 with alpha (the
+//     alpha ring does not fit the 168 VGPRs a 768-thread block leaves a lane).
 //   - other sizes: one column per thread (k_gsrb_deep, two planes of prefetch).
 template <typename T>
 bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s, const Compact16 *K)
@@ -2905,8 +3114,10 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStre
                        : launch_rows<T, 6, 768, false>(L, pin, pout, mode, s, nullptr);
     else if (L.nx == 128)
         done = launch_rows<T, 16, 1024, false>(L, pin, pout, mode, s, nullptr);
-    else
+    else if (L.alpha)
         done = launch_rows<T, 8, 1024, false>(L, pin, pout, mode, s, nullptr);
+    else
+        done = launch_rows<T, 6, 768, false, 1>(L, pin, pout, mode, s, nullptr);
     if (!done) launch_deep<T, 8, 2>(L, pin, pout, mode, s);
     return true;
 }
@@ -3132,10 +3343,11 @@ void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_hos
     }
 }
 
-void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s)
+void launch_correct(const CorrectArgs &A, int maxcols, int maxez, int maxghost, hipStream_t s)
 {
-    DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_correct_valid<D_>, dim3(nblk(maxfaces), A.nboxes, A.dim),
-                                           dim3(TPB), 0, s, A));
+    DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_correct_valid<D_>,
+                                           dim3(nblk(maxcols), A.nboxes, zchunks(A.dim, maxez)), dim3(TPB), 0,
+                                           s, A));
     if (A.ng_u > 0) {
         DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_fill_ghost<D_>, dim3(nblk(maxghost), A.nboxes, A.dim),
                                                dim3(TPB), 0, s, A));

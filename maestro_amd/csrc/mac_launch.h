@@ -152,8 +152,12 @@ void launch_cell_to_edge(const double *c, int nr, double *e, hipStream_t s);
 void launch_make_macrhs(const MacrhsArgs &A, int maxcells, bool sph, hipStream_t s);
 void launch_make_s0mac(const S0macArgs &A, int maxfaces, hipStream_t s);
 
-void launch_setup(const SetupArgs &A, int maxcells, hipStream_t s);
-long setup_nparts(int maxcells, int nboxes);
+// maxcols: the largest x-y cross-section (ex * ey) of a box, maxez the
+// largest z extent; one thread per column and chunk of planes of each box
+void launch_setup(const SetupArgs &A, int maxcols, int maxez, hipStream_t s);
+// partial sums written by launch_setup (and by any other kernel on the same
+// grid); per-cell kernels pass dim 2 and their cell count as maxcols
+long setup_nparts(int dim, int maxcols, int maxez, int nboxes);
 void launch_mean(const double *part, long nparts, double ncell, double *mean, hipStream_t s);
 template <typename T> void launch_sub_mean(T *x, long n, const double *mean, hipStream_t s);
 template <typename T> long launch_partial_sum(const T *x, long n, double *part, hipStream_t s);
@@ -203,7 +207,7 @@ template <typename T> void launch_fill(T *x, long n, T v, hipStream_t s);
 template <typename T>
 void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_host, int lc, int nlev,
                          const CoarseOpts &o, double *cg, hipStream_t s);
-void launch_correct(const CorrectArgs &A, int maxfaces, int maxghost, hipStream_t s);
+void launch_correct(const CorrectArgs &A, int maxcols, int maxez, int maxghost, hipStream_t s);
 void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s);
 void launch_level_correct(int dim, const LevelIO &A, const Lev<double> &L, hipStream_t s);
 void launch_adj_rhs(int dim, const AdjIO &A, const Lev<double> &L, double *part, hipStream_t s);

@@ -66,6 +66,7 @@ struct mac_plan {
     std::vector<BoxDev> boxes;
     BoxDev *boxes_dev = nullptr;
     int maxcells = 0, maxfaces = 0, maxghost = 0;
+    int maxcols = 0, maxez = 1;   // largest x-y cross-section and z extent of a box (column kernels)
     long maxplane = 0;
     int *boxmap_dev = nullptr;
     int mapg[3] = {1, 1, 1}, mapn[3] = {1, 1, 1};
@@ -766,6 +767,8 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
         int e[3];
         for (int d = 0; d < 3; ++d) e[d] = (d < P->dim) ? B.hi[d] - B.lo[d] + 1 : 1;
         P->maxcells = std::max(P->maxcells, e[0] * e[1] * e[2]);
+        P->maxcols = std::max(P->maxcols, e[0] * e[1]);
+        P->maxez = std::max(P->maxez, e[2]);
         for (int d = 0; d < P->dim; ++d) {
             int f[3] = {e[0], e[1], e[2]};
             f[d] += 1;
@@ -1059,13 +1062,13 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     A.beta0e = beta0_1d ? P->b0e_dev : nullptr;
     A.alt = P->opts.use_alt_energy_fix;
     A.L = P->H64.lev[0];
-    long np = setup_nparts(P->maxcells, nb);
+    long np = setup_nparts(dim, P->maxcols, P->maxez, nb);
     if ((rc = ensure_part(P, np))) return rc;
     A.part = P->part;
     A.err = P->err_dev;
     HIPCHK(hipMemsetAsync(P->err_dev, 0, sizeof(int), s));
     int pq = prof_begin(P, MAC_PROF_SETUP);
-    launch_setup(A, P->maxcells, s);
+    launch_setup(A, P->maxcols, P->maxez, s);
     prof_end(P, pq);
     const long n0 = ncell(P->n);
     // max |rh| (the "Initial rhs" figure of the solver log), after the mean
@@ -1110,7 +1113,7 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     C.phi_out = (double *const *)(td + nb * dim);   // phi out rides on the x-face correction
     C.ng_phi = P->ng_phi;
     pq = prof_begin(P, MAC_PROF_CORRECT);
-    launch_correct(C, P->maxfaces, P->maxghost, s);
+    launch_correct(C, P->maxcols, P->maxez, P->maxghost, s);
     prof_end(P, pq);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[4], s));
@@ -1784,7 +1787,7 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
     A.alpha = t_a; A.ng_alpha = in->ng_alpha;
     A.beta = t_b; A.ng_beta = in->ng_beta;
     A.L = P->H64.lev[0];
-    const long np = setup_nparts(P->maxcells, nb);
+    const long np = setup_nparts(2, P->maxcells, 1, nb);
     if ((rc = ensure_part(P, np))) return rc;
     A.part = P->part;
     A.err = P->err_dev;
