@@ -48,6 +48,7 @@ extern "C" {
 #define MAC_ERR_NOT_CONVERGED 4  /* max_iter reached; stats are filled */
 #define MAC_ERR_HIP 5            /* a HIP runtime call failed */
 #define MAC_ERR_NOMEM 6
+#define MAC_ERR_TOO_LARGE 7      /* mac_project_noalloc: the call needs more than its fixed scratch */
 
 /* cycle shapes, numbered as mg_cycle_type (Source/_parameters:376-377): a V
  * visits the next coarser level once, a W twice (each visit a W), an F first
@@ -164,6 +165,27 @@ int mac_project(mac_plan *plan, double *const *umac, double *const *phi,
                 const double *const *rho, const double *const *macrhs,
                 const double *beta0_1d, const double *beta0_1d_edge,
                 const double *const *beta0_cart_edge, mac_stats *stats);
+
+/* mac_project that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results (bit for bit) and return codes as mac_project, plus:
+ *   - opts.device_ptrs must be 1, and the plan's solver storage for its
+ *     precision must exist: the FP64 levels do from mac_plan_create on, the
+ *     FP32 levels (and the compact finest level of MAC_PREC_MIXED_BF16) after
+ *     one mac_project call in that precision; otherwise MAC_ERR_INVALID;
+ *   - the call's own device buffers (the pointer table, the partial sums of
+ *     the solvability mean, the mean, an error flag) live in a fixed 2 MiB
+ *     per-GPU array of the library: nboxes * (2*dim + 3) pointers plus 16
+ *     bytes per partial-sum block (about nboxes * ceil(box cross-section /
+ *     256) * ceil(box z extent / 8) blocks) must fit, else MAC_ERR_TOO_LARGE
+ *     and nothing runs (65536 boxes' worth of blocks at 64^3 per box, so
+ *     512^3 in 64^3 boxes fits);
+ *   - calls that overlap on one GPU share that array and race. */
+int mac_project_noalloc(mac_plan *plan, double *const *umac, double *const *phi,
+                        const double *const *rho, const double *const *macrhs,
+                        const double *beta0_1d, const double *beta0_1d_edge,
+                        const double *const *beta0_cart_edge, mac_stats *stats);
 
 /* ---- whole-domain device entry points (used by the autograd wrapper) ----
  * Arrays are single device buffers over the whole domain, C order with x

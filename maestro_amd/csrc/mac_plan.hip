@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -92,6 +93,7 @@ struct mac_plan {
     double *cg = nullptr;
     long cg_cap = 0;
     double **tab_dev = nullptr;   // pointer tables
+    std::vector<double *> tab_host;   // host side of a call's pointer table (sized at creation)
     double *b0_dev = nullptr, *b0e_dev = nullptr;
     int nr = 0;
     hipStream_t stream = nullptr;
@@ -813,6 +815,7 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
     if (hipMalloc(&P->err_dev, sizeof(int)) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMalloc(&P->tab_dev, sizeof(double *) * (size_t)P->nboxes * (2 * P->dim + 3)) != hipSuccess)
         return fail(MAC_ERR_HIP);
+    P->tab_host.assign((size_t)P->nboxes * (2 * P->dim + 3), nullptr);
     if (hipMalloc(&P->b0_dev, sizeof(double) * (P->nr + 1)) != hipSuccess) return fail(MAC_ERR_HIP);
     if (hipMalloc(&P->b0e_dev, sizeof(double) * (P->nr + 1)) != hipSuccess) return fail(MAC_ERR_HIP);
     if ((rc = build_hier<double>(P, P->H64))) return fail(rc);
@@ -1000,24 +1003,95 @@ static float ev_ms(mac_plan *P, int a, int b)
 }
 
 // --------------------------------------------------------- mac_project
-extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
-                           const double *const *rho, const double *const *macrhs,
-                           const double *beta0_1d, const double *beta0_1d_edge,
-                           const double *const *beta0_cart_edge, mac_stats *st)
+// ------------------------------------------ no-allocation projection scratch
+// mac_project_noalloc keeps the per-call device buffers of a projection (the
+// caller pointer table, the partial sums of the solvability mean, the mean
+// and the error flag) in this file-scope array instead of plan storage that
+// grows on demand.  Each GPU holds its own copy for the life of the process;
+// in host code the array's name is not a device address, so the address is
+// taken with hipGetSymbolAddress once per device and used everywhere else.
+// Calls that overlap on one GPU share the array and race.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)2 << 20;   // 2 MiB
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static unsigned char *scratch_dev[SCRATCH_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+// device dev's copy of the array (set up on the first call on that device,
+// with dev current)
+static int scratch_base(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return MAC_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return MAC_ERR_HIP;
+        scratch_dev[dev] = (unsigned char *)ptr;
+    }
+    *out = scratch_dev[dev];
+    return MAC_OK;
+}
+
+// the device buffers one projection call uses
+struct ProjBufs {
+    double **tab;   // caller pointer table
+    double *part;   // partial sums of the solvability mean (pairs)
+    double *mean;
+    int *err;
+};
+
+// whether the solver storage of the plan's precision is allocated (the FP64
+// hierarchy at plan creation; the FP32 one, and the compact level where it
+// applies, on the first solve in that precision)
+static bool storage_ready(mac_plan *P)
+{
+    if (!P->H64.built) return false;
+    if (P->opts.precision == MAC_PREC_FP64) return true;
+    if (!P->H32.built) return false;
+    return !(compact_applies(P) && P->c16_allocs.empty());
+}
+
+// mac_project; noalloc: the call's buffers come from the scratch array and
+// nothing is allocated (device pointers only, solver storage already there)
+static int project_impl(mac_plan *P, double *const *umac, double *const *phi, const double *const *rho,
+                        const double *const *macrhs, const double *beta0_1d, const double *beta0_1d_edge,
+                        const double *const *beta0_cart_edge, mac_stats *st, bool noalloc)
 {
     if (!P || !umac || !phi || !rho) return MAC_ERR_INVALID;
     if (beta0_1d && beta0_cart_edge) return MAC_ERR_INVALID;      // macproject.f90:99-101
     if (beta0_1d && !beta0_1d_edge) return MAC_ERR_INVALID;
     if (!beta0_1d && beta0_1d_edge) return MAC_ERR_INVALID;
     if (P->opts.precision < MAC_PREC_FP64 || P->opts.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
+    if (noalloc && (!P->opts.device_ptrs || !storage_ready(P))) return MAC_ERR_INVALID;
     if (st) memset(st, 0, sizeof(*st));
     hipStream_t s = P->stream;
     const int nb = P->nboxes, dim = P->dim;
     int rc;
     HIPCHK(hipSetDevice(P->device));
+    // the call's device buffers
+    const long np = setup_nparts(dim, P->maxcols, P->maxez, nb);
+    ProjBufs PB;
+    if (noalloc) {
+        unsigned char *base = nullptr;
+        if ((rc = scratch_base(P->device, &base))) return rc;
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t tab_b = up(sizeof(double *) * P->tab_host.size()), part_b = up(sizeof(double) * 2 * np);
+        if (tab_b + part_b + 512 > NOALLOC_SCRATCH_BYTES) return MAC_ERR_TOO_LARGE;
+        PB.tab = (double **)base;
+        PB.part = (double *)(base + tab_b);
+        PB.mean = (double *)(base + tab_b + part_b);
+        PB.err = (int *)(base + tab_b + part_b + 256);
+    } else {
+        if ((rc = ensure_part(P, np))) return rc;
+        PB.tab = P->tab_dev;
+        PB.part = P->part;
+        PB.mean = P->mean_dev;
+        PB.err = P->err_dev;
+    }
     HIPCHK(hipEventRecord(P->ev[0], s));
     // resolve device pointers
-    std::vector<double *> tab((size_t)nb * (2 * dim + 3), nullptr);
+    std::vector<double *> &tab = P->tab_host;
     double **t_umac = tab.data(), **t_phi = t_umac + nb * dim, **t_rho = t_phi + nb,
            **t_rhs = t_rho + nb, **t_cart = t_rhs + nb;
     if (P->opts.device_ptrs) {
@@ -1040,7 +1114,7 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
         if (beta0_cart_edge)
             for (int q = 0; q < nb * dim; ++q) t_cart[q] = P->hs.cart[q];
     }
-    HIPCHK(hipMemcpyAsync(P->tab_dev, tab.data(), sizeof(double *) * tab.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(PB.tab, tab.data(), sizeof(double *) * tab.size(), hipMemcpyHostToDevice, s));
     if (beta0_1d) {
         HIPCHK(hipMemcpyAsync(P->b0_dev, beta0_1d, sizeof(double) * P->nr,
                               P->opts.device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
@@ -1054,7 +1128,7 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     for (int d = 0; d < 3; ++d) { A.dlo[d] = P->dlo[d]; A.n[d] = P->n[d]; A.dx[d] = P->dx[d]; }
     A.bc = P->bc;
     A.ng_u = P->ng_u; A.ng_phi = P->ng_phi; A.ng_rho = P->ng_rho; A.ng_rhs = P->ng_rhs; A.ng_cart = P->ng_cart;
-    double **td = P->tab_dev;
+    double **td = PB.tab;
     A.umac = td; A.phi = td + nb * dim; A.rho = td + nb * dim + nb;
     A.macrhs = macrhs ? td + nb * dim + 2 * nb : nullptr;
     A.cart = beta0_cart_edge ? td + nb * dim + 3 * nb : nullptr;
@@ -1062,11 +1136,9 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     A.beta0e = beta0_1d ? P->b0e_dev : nullptr;
     A.alt = P->opts.use_alt_energy_fix;
     A.L = P->H64.lev[0];
-    long np = setup_nparts(dim, P->maxcols, P->maxez, nb);
-    if ((rc = ensure_part(P, np))) return rc;
-    A.part = P->part;
-    A.err = P->err_dev;
-    HIPCHK(hipMemsetAsync(P->err_dev, 0, sizeof(int), s));
+    A.part = PB.part;
+    A.err = PB.err;
+    HIPCHK(hipMemsetAsync(PB.err, 0, sizeof(int), s));
     int pq = prof_begin(P, MAC_PROF_SETUP);
     launch_setup(A, P->maxcols, P->maxez, s);
     prof_end(P, pq);
@@ -1075,14 +1147,14 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
     // removal of a singular problem (fused with it)
     HIPCHK(hipMemsetAsync(P->nrm_dev, 0, sizeof(unsigned long long), s));
     if (P->singular) {
-        launch_mean(P->part, np, (double)n0, P->mean_dev, s);
-        launch_sub_mean_absmax(P->H64.lev[0].rhs, n0, P->mean_dev, P->nrm_dev, s);
+        launch_mean(PB.part, np, (double)n0, PB.mean, s);
+        launch_sub_mean_absmax(P->H64.lev[0].rhs, n0, PB.mean, P->nrm_dev, s);
     } else {
         launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
     }
     int err_host = 0;
     unsigned long long rhs_bits = 0;
-    HIPCHK(hipMemcpyAsync(&err_host, P->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&err_host, PB.err, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&rhs_bits, P->nrm_dev, sizeof rhs_bits, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(P->ev[2], s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1132,6 +1204,22 @@ extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
         st->t_total_ms = ev_ms(P, 0, 5);
     }
     return src;
+}
+
+extern "C" int mac_project(mac_plan *P, double *const *umac, double *const *phi,
+                           const double *const *rho, const double *const *macrhs,
+                           const double *beta0_1d, const double *beta0_1d_edge,
+                           const double *const *beta0_cart_edge, mac_stats *st)
+{
+    return project_impl(P, umac, phi, rho, macrhs, beta0_1d, beta0_1d_edge, beta0_cart_edge, st, false);
+}
+
+extern "C" int mac_project_noalloc(mac_plan *P, double *const *umac, double *const *phi,
+                                   const double *const *rho, const double *const *macrhs,
+                                   const double *beta0_1d, const double *beta0_1d_edge,
+                                   const double *const *beta0_cart_edge, mac_stats *st)
+{
+    return project_impl(P, umac, phi, rho, macrhs, beta0_1d, beta0_1d_edge, beta0_cart_edge, st, true);
 }
 
 // -------------------------------------------------- whole-domain entries

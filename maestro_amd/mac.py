@@ -33,9 +33,10 @@ BOTTOM_SMOOTH, BOTTOM_BICGSTAB, BOTTOM_CG = 0, 1, 2  # mg_bottom_solver (Docs/mg
 ELL_PER, ELL_DIR, ELL_NEU = -1, 1, 2               # elliptic boundary kinds (mac_solve)
 COMP_PRESS, COMP_SCALAR, COMP_EXTRAP = 0, 1, 2     # component classes of mac_ell_bc
 ERRORS = {1: "invalid argument or option combination", 2: "bad box layout",
-          3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory"}
+          3: "non-finite input", 4: "not converged", 5: "HIP runtime error", 6: "out of memory",
+          7: "the call needs more than the fixed scratch of mac_project_noalloc"}
 SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
-           "mac_project", "mac_project_level", "mac_adjoint_level", "mac_version",
+           "mac_project", "mac_project_noalloc", "mac_project_level", "mac_adjoint_level", "mac_version",
            "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
            "mac_plan_profile", "mac_plan_profile_read")
@@ -133,6 +134,7 @@ def lib():
         L.mac_plan_destroy.argtypes = [vp]
         L.mac_plan_set_opts.argtypes = [vp, C.POINTER(COpts)]
         L.mac_project.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
+        L.mac_project_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
         L.mac_project_level.argtypes = [vp] * 9 + [C.POINTER(CStats)]
         L.mac_adjoint_level.argtypes = [vp] * 16 + [C.POINTER(CStats)]
         L.mac_version.restype = C.c_char_p
@@ -276,9 +278,11 @@ class MacPlan:
             torch.cuda.current_stream().synchronize()
 
     def project(self, umac, phi, rho, macrhs=None, beta0_1d=None, beta0_1d_edge=None,
-                beta0_cart_edge=None, check=True):
+                beta0_cart_edge=None, check=True, noalloc=False):
         """macproject on per-box arrays; ``umac`` is a list (per box) of ``dim``
-        face arrays.  Returns the stats dict."""
+        face arrays.  Returns the stats dict.  noalloc: through
+        mac_project_noalloc (device arrays; the plan's solver storage for its
+        precision already allocated by an earlier call)."""
         dim = self.layout.dim
         flat_u = [u for box in umac for u in box]
         flat_c = None if beta0_cart_edge is None else [c for box in beta0_cart_edge for c in box]
@@ -292,8 +296,8 @@ class MacPlan:
         b0 = _ptr(beta0_1d) if beta0_1d is not None else None
         b0e = _ptr(beta0_1d_edge) if beta0_1d_edge is not None else None
         st = CStats()
-        rc = lib().mac_project(self._h, keep[0], keep[1], keep[2], keep[3], b0, b0e, keep[4],
-                               C.byref(st))
+        entry = lib().mac_project_noalloc if noalloc else lib().mac_project
+        rc = entry(self._h, keep[0], keep[1], keep[2], keep[3], b0, b0e, keep[4], C.byref(st))
         if rc and (check or rc != 4):
             raise MacError(rc, "mac_project")
         d = st.as_dict()
