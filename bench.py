@@ -10,7 +10,9 @@ step on its own fresh copy of the inputs (staged in HBM before timing).
 
 ``value`` is the mixed-precision path of the configuration (FP32 V-cycle
 with FP32 iterate, coefficients and residual inside FP64 iterative
-refinement) in solves/s; the plain FP64 V-cycle and the variant that stores
+refinement; the finest level's FP32 coefficients are formed in the sweep
+from the FP32 density, 2 beta0^2 / (rho_L + rho_R), the coarser levels'
+stored) in solves/s; the plain FP64 V-cycle and the variant that stores
 the finest level's FP32-rounded weights and right side as bfloat16 (same
 FP32 arithmetic, same FP64 tolerance) are measured the same way and reported
 beside it.  ``roofline`` covers the dominant kernel of the
@@ -37,9 +39,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
+def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False) -> str:
     """Name of the finest-level sweep kernel the library launches (b16: the
-    compact bfloat16-storage form of the FP32 full-row kernel).  Full rows,
+    compact bfloat16-storage form of the FP32 full-row kernel; rho: the FP32
+    form that forms its weights from the density, which the plane-parallel
+    mixed path runs).  Full rows,
     W = nx / 128 waves per row; rows x threads per block: FP64 6 x 768, FP32
     6 x 768 with the weights fetched one plane ahead (16 x 1024 at nx = 128),
     FP32 with bfloat16 storage 6 x 768; the template's last two arguments: no
@@ -47,6 +51,8 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256) -> str:
     t = "double" if s == 8 else "float"
     c = "true" if b16 else "false"
     w = max(1, nx // 128)
+    if rho:   # FP32, the weights formed from rho: 6 rows x 768 (12 x 768 at nx = 128)
+        return f"k_gsrb_rows_rho<0, 6, 768, {w}>" if w == 2 else "k_gsrb_rows_rho<0, 12, 768, 1>"
     if s == 8 or b16:
         return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 2>"
     return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false, 2>" if w == 1
@@ -69,13 +75,14 @@ def measured_traffic(name: str):
     return None
 
 
-def gsrb_launch_bytes(ncell: int, s: int, ws: int = None) -> float:
+def gsrb_launch_bytes(ncell: int, s: int, ws: int = None, rho: bool = False) -> float:
     """Algorithmic bytes of one launch of the fused 3D red-black sweep (one
     full sweep): three face weights and rhs (element size ws), the old
     iterate read and the new one written (element size s), each once per
-    cell -> 2 s + 4 ws per cell."""
+    cell -> 2 s + 4 ws per cell; in the rho form (the weights formed from
+    the FP32 density) rho and rhs -> 2 s + 2 ws."""
     ws = s if ws is None else ws
-    return (2.0 * s + 4.0 * ws) * ncell
+    return (2.0 * s + (2.0 if rho else 4.0) * ws) * ncell
 
 
 def stage(per_box, copies, torch):
@@ -409,13 +416,13 @@ def main():
     def sweep_roofline(path):
         """Roofline of the finest-level sweep of one path (the launches that
         read the old iterate)."""
-        s_, ws, b16, cls = {"fp64": (8, 8, False, "gsrb64"), "mixed": (4, 4, False, "gsrb32"),
-                            "mixed_bf16": (4, 2, True, "gsrb32")}[path]
+        s_, ws, b16, rho, cls = {"fp64": (8, 8, False, False, "gsrb64"), "mixed": (4, 4, False, True, "gsrb32"),
+                                 "mixed_bf16": (4, 2, True, False, "gsrb32")}[path]
         cnt_, ms_ = results[path]["prof"][cls]
         avg = ms_ / cnt_ if cnt_ else float("inf")   # --no-profile: no per-kernel timing
-        nbytes = gsrb_launch_bytes(ncell, s_, ws)
+        nbytes = gsrb_launch_bytes(ncell, s_, ws, rho)
         ach = nbytes / (avg * 1e-3) / 1e9
-        name = sweep_kernel_name(s_, b16, n[0])
+        name = sweep_kernel_name(s_, b16, n[0], rho)
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                 "kernel": name + " (finest level, one full red-black sweep)",
@@ -493,7 +500,7 @@ def main():
                 }
         print(f"# cpu oracle: {tc:.2f} s/solve on {O.num_threads()} threads", file=sys.stderr)
 
-    storage = {"mixed": "FP32 V-cycle (FP32 storage) inside FP64 IR",
+    storage = {"mixed": "FP32 V-cycle inside FP64 IR, finest-level FP32 coefficients formed from the FP32 density",
                "mixed_bf16": "FP32 V-cycle inside FP64 IR, finest-level weights and rhs stored as bfloat16"}
     line = {
         "metric": "MAC solves/s at 256^3 to 1e-10 resid: FP32-MG+FP64 IR vs FP64; HBM GB/s",

@@ -218,6 +218,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
             comp_add(s, cmp, rh, 0.0);
             // ---- face coefficients (mk_mac_coeffs, beta0 scaling, BC folding)
             const double rc = cur.r;
+            if (A.rho32) A.rho32[c] = (float)rc;   // the FP32 solver's rho form
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
                 const int Id = (d == 0) ? I : (d == 1 ? J : K);
@@ -462,6 +463,80 @@ __global__ void __launch_bounds__(TPB) k_compact_level0(Lev<double> F, Compact16
                 const long s = (i + di) + (long)F.nx * (j + dj);
                 h[dj][di] = (float)F.bzh[s];
                 K.bzh[s] = f32_to_bf16(h[dj][di]);
+            }
+        C.bzh[I + (long)C.nx * J] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
+}
+
+// The rho form's face factors: cxy[k] = 2 beta0(k)^p on the planes, cz[f] =
+// 2 beta0_edge(f)^p on the z faces (p = 2 with use_alt_energy_fix), the
+// FP64 coefficient formula's factors (2 / (rho_L + rho_R) beta0 beta0)
+// rounded once to FP32.
+__global__ void __launch_bounds__(TPB) k_rho_coef(const double *b0, const double *b0e, int alt, int nz, RhoLev R)
+{
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t < nz) R.cxy[t] = (float)(alt ? 2.0 * b0[t] * b0[t] : 2.0 * b0[t]);
+    if (t <= nz) R.cz[t] = (float)(alt ? 2.0 * b0e[t] * b0e[t] : 2.0 * b0e[t]);
+}
+
+// The FP32 level 1 from the FP64 level-0 weights (one thread per coarse
+// cell, the averaging of k_coarsen on the FP32-rounded fine weights) and the
+// rho form's z slabs: the FP32 folded weights of the domain's low z faces
+// (R.bzl) and high ones (bzh0, the FP32 level 0's slab).
+__global__ void __launch_bounds__(TPB) k_rho_level0(Lev<double> F, RhoLev R, float *bzh0, Lev<float> C)
+{
+    const long q = (long)blockIdx.x * TPB + threadIdx.x;
+    const long nc = (long)C.nx * C.ny * C.nz;
+    if (q >= nc) return;
+    const int I = (int)(q % C.nx), J = (int)((q / C.nx) % C.ny), Kc = (int)(q / ((long)C.nx * C.ny));
+    const int i = 2 * I, j = 2 * J, k = 2 * Kc;
+    auto fc = [&](int jj, int kk) { return i + (long)F.nx * (jj + (long)F.ny * kk); };
+    float wx[2][2][2], wy[2][2][2], wz[2][2][2];   // [dk][dj][di]
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            const long f = fc(j + dj, k + dk);
+            const double2 bx = *(const double2 *)(F.bx + f), by = *(const double2 *)(F.by + f),
+                          bz = *(const double2 *)(F.bz + f);
+            wx[dk][dj][0] = (float)bx.x; wx[dk][dj][1] = (float)bx.y;
+            wy[dk][dj][0] = (float)by.x; wy[dk][dj][1] = (float)by.y;
+            wz[dk][dj][0] = (float)bz.x; wz[dk][dj][1] = (float)bz.y;
+        }
+    const float q4 = 0.25f;
+    C.bx[q] = q4 * ((wx[0][0][0] + wx[0][1][0]) + (wx[1][0][0] + wx[1][1][0]));
+    C.by[q] = q4 * ((wy[0][0][0] + wy[0][0][1]) + (wy[1][0][0] + wy[1][0][1]));
+    C.bz[q] = q4 * ((wz[0][0][0] + wz[0][0][1]) + (wz[0][1][0] + wz[0][1][1]));
+    if (Kc == 0)   // the fine level's low z faces: plane 0's z weights
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) R.bzl[(i + di) + (long)F.nx * (j + dj)] = wz[0][dj][di];
+    if (I == C.nx - 1) {
+        float h[2][2];
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) h[dk][dj] = (float)F.bxh[(j + dj) + (long)F.ny * (k + dk)];
+        C.bxh[J + (long)C.ny * Kc] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
+    if (J == C.ny - 1) {
+        float h[2][2];
+#pragma unroll
+        for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) h[dk][di] = (float)F.byh[(i + di) + (long)F.nx * (k + dk)];
+        C.byh[I + (long)C.nx * Kc] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
+    }
+    if (Kc == C.nz - 1) {
+        float h[2][2];
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const long sidx = (i + di) + (long)F.nx * (j + dj);
+                h[dj][di] = (float)F.bzh[sidx];
+                bzh0[sidx] = h[dj][di];
             }
         C.bzh[I + (long)C.nx * J] = q4 * ((h[0][0] + h[0][1]) + (h[1][0] + h[1][1]));
     }
@@ -1128,6 +1203,293 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
 }
 #undef PH
 
+// -------------------- fused red-black sweep, full rows, weights from rho
+// The FP32 full-row sweep of a plane-parallel projection's finest level with
+// the face weights formed from the cell density instead of read: a face
+// between cells of densities rho_L and rho_R has w = c / (rho_L + rho_R),
+// c = 2 beta0^p of the face (x and y faces: the plane's cell-centred beta0;
+// z faces: the edge value; p = 2 with use_alt_energy_fix).  A cell update
+// then reads rho and rhs (8 B) where the stored form reads three weights and
+// rhs (16 B).  x and y are periodic; the z faces on the domain's low and high
+// sides come from FP32 slabs of the folded FP64 weights (walls, outlets or
+// the periodic image).  The reciprocal is the hardware's (1 ulp): these
+// weights drive the FP32 preconditioner inside FP64 refinement, whose FP64
+// residual uses the FP64 weights.  Structure as k_gsrb_rows (one column pair
+// per lane, W waves per row, EY rows, z-march, one barrier per plane); rho
+// rotates through LDS beside phi.  The z faces of both columns are formed
+// once per plane, into a four-slot register ring.
+struct RhoBufs {      // byte offsets in the weights' allocation (with B.rhs)
+    unsigned rho;     // FP32 cell density
+    unsigned cxy;     // per plane: 2 beta0^p (x and y faces)
+    unsigned cz;      // per z face (nz + 1): 2 beta0_edge^p
+    unsigned bzl;     // FP32 folded weights of the domain's low z faces (nx ny)
+    unsigned bzh;     // and of its high z faces
+};
+
+template <int MODE, int EY, int NTH, int W>
+__global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, RhoBufs RB, RowSweepGrid G)
+{
+    using T = float;
+    using T2 = float2;
+    constexpr int NX = 128 * W, FY = EY - 2, PY = EY + 2, NI = 8;
+    constexpr int PXL = NX + 4, PS = PY * PXL;
+    constexpr unsigned ES = 4u;
+    const int b = blockIdx.x;
+    const int idx = (b & 7) * G.per_xcd + (b >> 3);    // XCD-contiguous tiles
+    if (idx >= G.total) return;
+    const int chunk = idx / G.nty, tyi = idx - chunk * G.nty;
+    const int ny = L.ny, nz = L.nz, plane = NX * ny;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int w = wv % W, ty = wv / W;                  // wave-uniform
+    const int x0 = 2 * (lane + 64 * w);
+    const int Y = tyi * FY - 1 + ty;
+    int Yw;
+    bool oky;
+    wrap_coord(Y, ny, 1, Yw, oky);                      // y periodic
+    const int z0 = chunk * G.tz, z1 = min(z0 + G.tz, nz);
+    const bool outrow = ty >= 1 && ty <= FY && Y < ny;
+    const bool ring = (ty == 0) || (ty == EY - 1);
+    int y2w = 0;
+    bool ok2 = false;
+    if (ring) wrap_coord(ty == 0 ? Y - 1 : Y + 1, ny, 1, y2w, ok2);
+    const bool xend = x0 + 2 == NX;
+    const unsigned col = (unsigned)(x0 + NX * Yw);
+    const unsigned vw = col * ES, vp2 = (unsigned)(x0 + NX * y2w) * ES;
+    const __amdgpu_buffer_rsrc_t Rp = buf_rsrc(B.bp), Rw = buf_rsrc(B.bw);
+    const float *cxy = (const float *)(B.bw + RB.cxy), *cz = (const float *)(B.bw + RB.cz);
+    // LDS element indices: own column pair, ring-2 row, x-neighbours x0-1 and
+    // x0+2 (x periodic: across the row end from the other end of the row)
+    const int lp = (ty + 1) * PXL + 2 + x0;
+    const int lp2 = (ty == 0 ? 0 : EY + 1) * PXL + 2 + x0;
+    const int lxl = x0 == 0 ? lp + NX - 1 : lp - 1;
+    const int lxr = xend ? lp - (NX - 2) : lp + 2;
+    extern __shared__ __align__(16) unsigned char rows_smem[];
+    T *lds = (T *)rows_smem;          // phi: four planes
+    T *ldr = lds + 4 * PS;            // rho: four planes
+    for (int q = threadIdx.x; q < 8 * PS; q += blockDim.x) lds[q] = T(0);
+
+    auto zwrap = [&](int k, int &kw) -> bool {
+        const bool in = (unsigned)k < (unsigned)nz;
+        kw = in ? k : (L.perz ? (k < 0 ? k + nz : k - nz) : 0);
+        return in || L.perz;
+    };
+    auto z2 = []() {
+        T2 v;
+        v.x = T(0);
+        v.y = T(0);
+        return v;
+    };
+    auto ldphi = [&](int k) -> T2 {
+        if constexpr (MODE == 1) {
+            return z2();
+        } else {
+            int kw;
+            const bool ok = zwrap(k, kw);
+            const T2 v = ld_rs<T2>(Rp, vw, B.in + (unsigned)kw * (unsigned)plane * ES);
+            return ok ? v : z2();
+        }
+    };
+    // per-plane inputs: [0, 1] rho of the column pair, [2, 3] rhs, [4, 5]
+    // ring-2 old phi and [6, 7] ring-2 rho (ring waves)
+    struct In {
+        T v[NI];
+    };
+    auto fetch = [&](int k) -> In {
+        In o;
+        int kw;
+        const bool kok = zwrap(k, kw);
+        const unsigned sw = (unsigned)kw * (unsigned)plane * ES;
+        const T2 r = ld_rs<T2>(Rw, vw, RB.rho + sw);
+        o.v[0] = r.x;
+        o.v[1] = r.y;
+        const T2 h = ld_rs<T2>(Rw, vw, B.rhs + sw);
+        o.v[2] = h.x;
+        o.v[3] = h.y;
+        T2 va = z2(), vr = z2();
+        if (ring) {
+            if (MODE == 0) va = ld_rs<T2>(Rp, vp2, B.in + sw);
+            vr = ld_rs<T2>(Rw, vp2, RB.rho + sw);
+        }
+        const bool use2 = kok && ok2;
+        o.v[4] = use2 ? va.x : T(0);
+        o.v[5] = use2 ? va.y : T(0);
+        o.v[6] = use2 ? vr.x : T(0);
+        o.v[7] = use2 ? vr.y : T(0);
+        return o;
+    };
+    // the z faces f of the column pair (face f between planes f-1 and f),
+    // from the pair's rho on planes f-1 and f; the domain's sides from the
+    // slabs; faces beyond a non-periodic side (never used) are zero
+    auto facez = [&](int f, const In &lo, const In &hi) -> T2 {
+        int fw = f;
+        if (L.perz) fw = f < 0 ? f + nz : (f > nz ? f - nz : f);
+        if (fw == 0 || fw == nz) return ld_rs<T2>(Rw, vw, fw == 0 ? RB.bzl : RB.bzh);   // slab: one plane
+        if (fw < 0 || fw > nz) return z2();
+        const T c = cz[fw];
+        T2 v;
+        v.x = c * __builtin_amdgcn_rcpf(lo.v[0] + hi.v[0]);
+        v.y = c * __builtin_amdgcn_rcpf(lo.v[1] + hi.v[1]);
+        return v;
+    };
+    const int kb = z0 - 1, ke = z1;
+    const int nsteps = ((ke - kb + 1) + 7) & ~7;
+    In wq[4];
+    struct Ring {
+        T v[16];
+    } phr;
+#define PH(slot, c) phr.v[2 * (slot) + (c)]
+    T2 wz[4];   // z faces of the pair: face kb + m in wz[m & 3]
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        wz[q] = z2();
+#pragma unroll
+        for (int t = 0; t < NI; ++t) wq[q].v[t] = T(0);
+    }
+    auto put = [&](auto SC, const T2 &e) __attribute__((always_inline)) {
+        constexpr int S = decltype(SC)::value;
+        PH(S, 0) = e.x;
+        PH(S, 1) = e.y;
+    };
+    put(std::integral_constant<int, 7>{}, ldphi(kb - 1));
+    put(std::integral_constant<int, 0>{}, ldphi(kb));
+    put(std::integral_constant<int, 1>{}, ldphi(min(kb + 1, ke + 1)));
+    put(std::integral_constant<int, 2>{}, ldphi(min(kb + 2, ke + 1)));
+    put(std::integral_constant<int, 3>{}, ldphi(min(kb + 3, ke + 1)));
+    put(std::integral_constant<int, 4>{}, ldphi(min(kb + 4, ke + 1)));
+    put(std::integral_constant<int, 5>{}, ldphi(min(kb + 5, ke + 1)));
+    PH(6, 0) = PH(6, 1) = T(0);
+    wq[3] = fetch(kb - 1);   // plane kb-1: its rho forms the z face kb
+    wq[0] = fetch(kb);
+    wq[1] = fetch(min(kb + 1, ke + 1));
+    wz[0] = facez(kb, wq[3], wq[0]);
+    __syncthreads();
+
+    auto step = [&](auto SC, int m) __attribute__((always_inline)) {
+        constexpr int S = decltype(SC)::value;
+        constexpr int C0 = S & 3, CP = (S + 3) & 3, C1 = (S + 1) & 3, CN = (S + 2) & 3;   // m, m-1, m+1, m+2
+        constexpr int F0 = S, FM1 = (S + 7) & 7, FM2 = (S + 6) & 7, FP1 = (S + 1) & 7;
+        T *const buf = lds + (S & 3) * PS, *const pbuf = lds + ((S + 3) & 3) * PS;
+        T *const rbuf = ldr + (S & 3) * PS, *const prbuf = ldr + ((S + 3) & 3) * PS;
+        const int k = kb + m;
+        int kw;
+        const bool kok = zwrap(min(k, ke + 1), kw) && k <= ke;
+        int kwm;
+        zwrap(k - 1, kwm);
+        // the z faces of plane k's top (face k+1), from planes k and k+1
+        wz[C1] = facez(k + 1, wq[C0], wq[C1]);
+        wq[CN] = fetch(min(k + 2, ke + 1));
+        // (a) publish the old phi and rho of plane k (and the ring-2 rows)
+        {
+            T2 v, r;
+            v.x = kok ? PH(F0, 0) : T(0);
+            v.y = kok ? PH(F0, 1) : T(0);
+            r.x = kok ? wq[C0].v[0] : T(0);
+            r.y = kok ? wq[C0].v[1] : T(0);
+            *(T2 *)(buf + lp) = v;
+            *(T2 *)(rbuf + lp) = r;
+        }
+        if (ring) {
+            T2 v, r;
+            v.x = kok ? wq[C0].v[4] : T(0);
+            v.y = kok ? wq[C0].v[5] : T(0);
+            r.x = kok ? wq[C0].v[6] : T(0);
+            r.y = kok ? wq[C0].v[7] : T(0);
+            *(T2 *)(buf + lp2) = v;
+            *(T2 *)(rbuf + lp2) = r;
+        }
+        __syncthreads();
+        const int r = (Yw + k) & 1;
+        const int km = k - 1;
+        const bool do_red = kok, do_black = outrow && km >= z0 && km < z1;
+        const T cxy0 = cxy[kw], cxym = cxy[kwm];
+        const T f0a = PH(F0, 0), f0b = PH(F0, 1), m1a = PH(FM1, 0), m1b = PH(FM1, 1);
+        const T p1a = PH(FP1, 0), p1b = PH(FP1, 1), m2a = PH(FM2, 0), m2b = PH(FM2, 1);
+        T red_v = T(0), blk_v = T(0);
+        // x and y faces of column R of a plane: own pair (ra, rb), the LDS plane rb_
+        auto faces_xy = [&](auto RC, T c, T ra, T rb, const T *rl, T wt[4]) __attribute__((always_inline)) {
+            constexpr int R = decltype(RC)::value;
+            const T rR = R ? rb : ra;
+            if constexpr (R == 0) {
+                wt[0] = c * __builtin_amdgcn_rcpf(rl[lxl] + ra);
+                wt[1] = c * __builtin_amdgcn_rcpf(ra + rb);
+            } else {
+                wt[0] = c * __builtin_amdgcn_rcpf(ra + rb);
+                wt[1] = c * __builtin_amdgcn_rcpf(rb + rl[lxr]);
+            }
+            wt[2] = c * __builtin_amdgcn_rcpf(rl[lp - PXL + R] + rR);
+            wt[3] = c * __builtin_amdgcn_rcpf(rR + rl[lp + PXL + R]);
+        };
+        auto update = [&](auto RC) __attribute__((always_inline)) {
+            constexpr int R = decltype(RC)::value;
+            const In &cw = wq[C0], &pw = wq[CP];
+            T f0R = R ? f0b : f0a;
+            if (do_red) {
+                T wxy[4];
+                faces_xy(RC, cxy0, cw.v[0], cw.v[1], rbuf, wxy);
+                const T wt[6] = {wxy[0], wxy[1], wxy[2], wxy[3], R ? wz[C0].y : wz[C0].x, R ? wz[C1].y : wz[C1].x};
+                T xm, xp;
+                if constexpr (R == 1) {
+                    xm = f0a;
+                    xp = buf[lxr];
+                } else {
+                    xm = buf[lxl];
+                    xp = f0b;
+                }
+                const T nb[6] = {xm, xp, buf[lp - PXL + R], buf[lp + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
+                red_v = gs_value<3, T>(L, wt, nb, cw.v[2 + R]);
+                buf[lp + R] = red_v;
+                f0R = red_v;
+            }
+            if (do_black) {
+                T wxy[4];
+                faces_xy(RC, cxym, pw.v[0], pw.v[1], prbuf, wxy);
+                const T wt[6] = {wxy[0], wxy[1], wxy[2], wxy[3], R ? wz[CP].y : wz[CP].x, R ? wz[C0].y : wz[C0].x};
+                T xm, xp;
+                if constexpr (R == 1) {
+                    xm = m1a;
+                    xp = pbuf[lxr];
+                } else {
+                    xm = pbuf[lxl];
+                    xp = m1b;
+                }
+                const T nb[6] = {xm, xp, pbuf[lp - PXL + R], pbuf[lp + PXL + R], R ? m2b : m2a, f0R};
+                blk_v = gs_value<3, T>(L, wt, nb, pw.v[2 + R]);
+                T2 o;
+                o.x = R ? m1a : blk_v;
+                o.y = R ? blk_v : m1b;
+                st_rs<T2, CPOL_NT>(Rp, vw, B.out + (unsigned)km * (unsigned)plane * ES, o);
+            }
+        };
+        if (r) update(std::integral_constant<int, 1>{});
+        else update(std::integral_constant<int, 0>{});
+        if (do_red) {
+            PH(F0, 0) = r ? f0a : red_v;
+            PH(F0, 1) = r ? red_v : f0b;
+        }
+        if (do_black) {
+            PH(FM1, 0) = r ? m1a : blk_v;
+            PH(FM1, 1) = r ? blk_v : m1b;
+        }
+        {
+            const T2 e = (k + 6 <= ke + 1) ? ldphi(k + 6) : z2();
+            PH(FM2, 0) = e.x;
+            PH(FM2, 1) = e.y;
+        }
+    };
+    for (int m = 0; m < nsteps; m += 8) {
+        step(std::integral_constant<int, 0>{}, m);
+        step(std::integral_constant<int, 1>{}, m + 1);
+        step(std::integral_constant<int, 2>{}, m + 2);
+        step(std::integral_constant<int, 3>{}, m + 3);
+        step(std::integral_constant<int, 4>{}, m + 4);
+        step(std::integral_constant<int, 5>{}, m + 5);
+        step(std::integral_constant<int, 6>{}, m + 6);
+        step(std::integral_constant<int, 7>{}, m + 7);
+    }
+}
+#undef PH
+
 // residual of the fine level restricted onto the coarse level in one pass
 // (the fine residual is never stored); also clears the coarse iterate.
 // Block: 64 coarse cells in x times the 2 (y) x 2 (z) fine offsets.
@@ -1639,6 +2001,137 @@ __global__ void __launch_bounds__(256) k_rr_rows(Lev<T> F, const T *__restrict__
             zp = zn;
         }
         const T v = T(0.125) * ((s[0][0] + s[0][1]) + (s[1][0] + s[1][1]));
+        const size_t cp = (size_t)C.nx * C.ny * K;
+        st_off<T>(C.rhs + cp, oc * ES, v);
+        st_off<T>(C.phi + cp, oc * ES, T(0));
+    }
+}
+
+// k_rr_rows for the rho form of an FP32 finest level (x and y periodic): the
+// face weights of the residual formed from rho as in k_gsrb_rows_rho (the
+// domain's z sides from the slabs).  Same march and restriction.
+__global__ void __launch_bounds__(256) k_rr_rows_rho(Lev<float> F, const float *__restrict__ phi, Lev<float> C,
+                                                    RowMarchGrid G, RhoLev R)
+{
+    using T = float;
+    using T2 = float2;
+    constexpr unsigned ES = 4u;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int bidx = (blockIdx.x & 7) * G.per_xcd + (blockIdx.x >> 3);
+    const int gw = bidx * 4 + wv;
+    if (gw >= G.total) return;
+    const int nx = F.nx, ny = F.ny, nz = F.nz, plane = nx * ny;
+    const int seg = gw % G.W, t = gw / G.W;
+    const int J = t % C.ny, chunk = t / C.ny;
+    const int K0 = chunk * G.tz, K1 = min(K0 + G.tz, C.nz);
+    const int I = lane + 64 * seg, x0 = 2 * I;
+    const int j0 = 2 * J, j1 = j0 + 1;
+    const int jm = j0 > 0 ? j0 - 1 : ny - 1, jp = j1 < ny - 1 ? j1 + 1 : 0;   // y periodic
+    const int xm = x0 > 0 ? x0 - 1 : nx - 1, xp = x0 + 2 < nx ? x0 + 2 : 0;   // x periodic
+    const unsigned o0 = (unsigned)(x0 + nx * j0), o1 = o0 + nx;
+    const unsigned oym = (unsigned)(x0 + nx * jm), oyp = (unsigned)(x0 + nx * jp);
+    const unsigned oxm0 = (unsigned)(xm + nx * j0), oxp0 = (unsigned)(xp + nx * j0);
+    auto kwrap = [&](int k, int &kw) -> bool {
+        if (k >= 0 && k < nz) { kw = k; return true; }
+        if (F.perz) { kw = k < 0 ? k + nz : k - nz; return true; }
+        kw = 0;
+        return false;
+    };
+    auto rcp = [](T x) { return __builtin_amdgcn_rcpf(x); };
+    struct Z {   // own two rows on one plane: old iterate and rho
+        T p00, p01, p10, p11, r00, r01, r10, r11;   // [row][column]
+    };
+    auto loadz = [&](int k) -> Z {
+        Z z;
+        int kw;
+        const bool ok = kwrap(k, kw);
+        const size_t pk = (size_t)plane * kw;
+        const T2 a = ld_off<T2>(phi + pk, o0 * ES), c = ld_off<T2>(phi + pk, o1 * ES);
+        const T2 ra = ld_off<T2>(R.rho + pk, o0 * ES), rc = ld_off<T2>(R.rho + pk, o1 * ES);
+        z.p00 = ok ? a.x : T(0);
+        z.p01 = ok ? a.y : T(0);
+        z.p10 = ok ? c.x : T(0);
+        z.p11 = ok ? c.y : T(0);
+        z.r00 = ra.x;
+        z.r01 = ra.y;
+        z.r10 = rc.x;
+        z.r11 = rc.y;
+        return z;
+    };
+    // z faces f of the four cells (between planes f-1 and f)
+    auto facez = [&](int f, const Z &lo, const Z &hi, T w[2][2]) {
+        if (f == 0 || f == nz) {
+            const float *sl = f == 0 ? R.bzl : F.bzh;
+            const T2 a = ld_off<T2>(sl, o0 * ES), c = ld_off<T2>(sl, o1 * ES);
+            w[0][0] = a.x; w[0][1] = a.y; w[1][0] = c.x; w[1][1] = c.y;
+            return;
+        }
+        const T cf = R.cz[f];
+        w[0][0] = cf * rcp(lo.r00 + hi.r00);
+        w[0][1] = cf * rcp(lo.r01 + hi.r01);
+        w[1][0] = cf * rcp(lo.r10 + hi.r10);
+        w[1][1] = cf * rcp(lo.r11 + hi.r11);
+    };
+    auto plane_res = [&](int k, const Z &zm, const Z &zc, const Z &zp, T r[2][2]) {
+        const size_t pk = (size_t)plane * k;
+        const T *ph = phi + pk, *rh = R.rho + pk;
+        const T2 ym = ld_off<T2>(ph, oym * ES), yp = ld_off<T2>(ph, oyp * ES);
+        const T xm0 = ld_off<T>(ph, oxm0 * ES), xp0 = ld_off<T>(ph, oxp0 * ES);
+        const T xm1 = ld_off<T>(ph, (oxm0 + nx) * ES), xp1 = ld_off<T>(ph, (oxp0 + nx) * ES);
+        const T2 rym = ld_off<T2>(rh, oym * ES), ryp = ld_off<T2>(rh, oyp * ES);
+        const T rxm0 = ld_off<T>(rh, oxm0 * ES), rxp0 = ld_off<T>(rh, oxp0 * ES);
+        const T rxm1 = ld_off<T>(rh, (oxm0 + nx) * ES), rxp1 = ld_off<T>(rh, (oxp0 + nx) * ES);
+        const T2 rh0 = ld_off<T2>(F.rhs + pk, o0 * ES), rh1 = ld_off<T2>(F.rhs + pk, o1 * ES);
+        const T c = R.cxy[k];
+        T wzl[2][2], wzh[2][2];
+        facez(k, zm, zc, wzl);
+        facez(k + 1, zc, zp, wzh);
+        // rho of the 4 x 4 neighbourhood rows: [row][col] for cols x0-1 .. x0+2
+        const T rr[2][4] = {{rxm0, zc.r00, zc.r01, rxp0}, {rxm1, zc.r10, zc.r11, rxp1}};
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                T w[6], p[6];
+                const T rc = rr[b][1 + a];
+                w[0] = c * rcp(rr[b][a] + rc);
+                w[1] = c * rcp(rc + rr[b][2 + a]);
+                const T rlo = b ? (a ? zc.r01 : zc.r00) : (a ? rym.y : rym.x);
+                const T rhi = b ? (a ? ryp.y : ryp.x) : (a ? zc.r11 : zc.r10);
+                w[2] = c * rcp(rlo + rc);
+                w[3] = c * rcp(rc + rhi);
+                w[4] = wzl[b][a];
+                w[5] = wzh[b][a];
+                const T pr0 = b ? zc.p10 : zc.p00, pr1 = b ? zc.p11 : zc.p01;   // this row
+                p[0] = a ? pr0 : (b ? xm1 : xm0);
+                p[1] = a ? (b ? xp1 : xp0) : pr1;
+                p[2] = b ? (a ? zc.p01 : zc.p00) : (a ? ym.y : ym.x);
+                p[3] = b ? (a ? yp.y : yp.x) : (a ? zc.p11 : zc.p10);
+                p[4] = b ? (a ? zm.p11 : zm.p10) : (a ? zm.p01 : zm.p00);
+                p[5] = b ? (a ? zp.p11 : zp.p10) : (a ? zp.p01 : zp.p00);
+                const T pcc = a ? pr1 : pr0;
+                const T rhv = b ? (a ? rh1.y : rh1.x) : (a ? rh0.y : rh0.x);
+                r[b][a] = rhv - apply_value<3, T>(F, w, p, pcc);
+            }
+    };
+    Z zm = loadz(2 * K0 - 1), zc = loadz(2 * K0), zp = loadz(2 * K0 + 1);
+    const unsigned oc = (unsigned)(I + C.nx * J);
+    for (int K = K0; K < K1; ++K) {
+        T s2[2][2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int k = 2 * K + c;
+            const Z zn = loadz(k + 2);
+            T r[2][2];
+            plane_res(k, zm, zc, zp, r);
+            s2[c][0] = r[0][0] + r[0][1];
+            s2[c][1] = r[1][0] + r[1][1];
+            zm = zc;
+            zc = zp;
+            zp = zn;
+        }
+        const T v = T(0.125) * ((s2[0][0] + s2[0][1]) + (s2[1][0] + s2[1][1]));
         const size_t cp = (size_t)C.nx * C.ny * K;
         st_off<T>(C.rhs + cp, oc * ES, v);
         st_off<T>(C.phi + cp, oc * ES, T(0));
@@ -3003,6 +3496,50 @@ static bool row_bufs(const Lev<T> &L, const T *pin, T *pout, const Compact16 *K,
     return true;
 }
 
+// Tiling of a full-row sweep: y-tiles of EY - 2 written rows, and the
+// z-chunk count.  Time ~ rounds of resident blocks x (march steps + a block's
+// start-up, about 8 steps: the ring prologue and the LDS clear); ties go to
+// more blocks (better balance across CUs).  Measured at 256^3 FP64: 4 chunks
+// (one round, 72 steps) 173 us against 12 chunks (three rounds of 24 steps)
+// 188 us, which a model without the start-up term preferred.
+static RowSweepGrid row_grid(int ny, int nz, int EY, int W, int slots)
+{
+    RowSweepGrid G;
+    G.W = W;
+    G.nty = (ny + (EY - 2) - 1) / (EY - 2);
+    int best_zc = 1;
+    long best = -1, best_blocks = 0;
+    for (int zc = 1; zc <= std::max(1, nz / 4); ++zc) {
+        const int tz = (nz + zc - 1) / zc;
+        const int nzc = (nz + tz - 1) / tz;
+        const long blocks = (long)G.nty * nzc;
+        const long rounds = (blocks + slots - 1) / slots;
+        const int steps = ((tz + 2) + 7) & ~7;   // the march runs in groups of 8 planes
+        const long cost = rounds * (steps + 8);
+        if (best < 0 || cost < best || (cost == best && blocks > best_blocks)) {
+            best = cost;
+            best_zc = zc;
+            best_blocks = blocks;
+        }
+    }
+    G.tz = (nz + best_zc - 1) / best_zc;
+    const int nzc = (nz + G.tz - 1) / G.tz;
+    G.total = G.nty * nzc;
+    G.per_xcd = (G.total + 7) / 8;
+    return G;
+}
+
+// resident blocks of a kernel on the device (CUs x blocks per CU), 0 on error
+static int resident_slots(const void *kern, int threads, size_t lds)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, threads, lds) != hipSuccess || per < 1) return 0;
+    return cus * per;
+}
+
 template <typename T, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2>
 static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStream_t s)
 {
@@ -3023,33 +3560,7 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
     }
-    RowSweepGrid G;
-    G.W = W;
-    G.nty = (L.ny + (EY - 2) - 1) / (EY - 2);
-    // time ~ rounds of resident blocks x (march steps + a block's start-up,
-    // about 8 steps: the ring prologue and the LDS clear); ties go to more
-    // blocks (better balance across CUs).  Measured at 256^3 FP64: 4 chunks
-    // (one round, 72 steps) 173 us against 12 chunks (three rounds of 24
-    // steps) 188 us, which a model without the start-up term preferred.
-    int best_zc = 1;
-    long best = -1, best_blocks = 0;
-    for (int zc = 1; zc <= std::max(1, L.nz / 4); ++zc) {
-        const int tz = (L.nz + zc - 1) / zc;
-        const int nzc = (L.nz + tz - 1) / tz;
-        const long blocks = (long)G.nty * nzc;
-        const long rounds = (blocks + slots - 1) / slots;
-        const int steps = ((tz + 2) + 7) & ~7;   // the march runs in groups of 8 planes
-        const long cost = rounds * (steps + 8);
-        if (best < 0 || cost < best || (cost == best && blocks > best_blocks)) {
-            best = cost;
-            best_zc = zc;
-            best_blocks = blocks;
-        }
-    }
-    G.tz = (L.nz + best_zc - 1) / best_zc;
-    const int nzc = (L.nz + G.tz - 1) / G.tz;
-    G.total = G.nty * nzc;
-    G.per_xcd = (G.total + 7) / 8;
+    const RowSweepGrid G = row_grid(L.ny, L.nz, EY, W, slots);
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
         hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW>), g, b, lds, s, L, RB, G);
@@ -3078,6 +3589,62 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     }
     if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1, false, PDW>(L, RB, mode, s);
     return launch_rows_w<T, EY, NTH, B16, 2, false, PDW>(L, RB, mode, s);
+}
+
+// The rho form of the FP32 finest-level sweep (k_gsrb_rows_rho): 6 rows of
+// 768 threads at nx = 256, 12 rows of 768 at nx = 128; phi and rho planes
+// in LDS (8 planes), two blocks per CU.
+template <int EY, int NTH, int W>
+static bool launch_rows_rho_w(const Lev<float> &L, const RowBufs &RB, const RhoBufs &HB, int mode, hipStream_t s)
+{
+    const size_t lds = sizeof(float) * 8 * (size_t)(EY + 2) * (L.nx + 4);
+    const int threads = 64 * W * EY;
+    static int slots_cache[2] = {0, 0};   // [mode] resident blocks per device
+    int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0];
+    if (slots == 0) {
+        slots = (mode == SWEEP_ZERO) ? resident_slots((const void *)k_gsrb_rows_rho<1, EY, NTH, W>, threads, lds)
+                                     : resident_slots((const void *)k_gsrb_rows_rho<0, EY, NTH, W>, threads, lds);
+        if (slots == 0) return false;
+    }
+    const RowSweepGrid G = row_grid(L.ny, L.nz, EY, W, slots);
+    dim3 g(8 * G.per_xcd), b(threads);
+    if (mode == SWEEP_ZERO)
+        hipLaunchKernelGGL((k_gsrb_rows_rho<1, EY, NTH, W>), g, b, lds, s, L, RB, HB, G);
+    else
+        hipLaunchKernelGGL((k_gsrb_rows_rho<0, EY, NTH, W>), g, b, lds, s, L, RB, HB, G);
+    return true;
+}
+
+bool rho_level_ok(int dim, const Lev<float> &L)
+{
+    if (dim != 3 || !L.perx || !L.pery || L.alpha) return false;
+    if (L.nx == 256) return rows_fit<float, 6, 768>(L);
+    if (L.nx == 128) return rows_fit<float, 12, 768>(L);
+    return false;
+}
+
+// byte offsets of the rho form's arrays in the level's allocation (the
+// weights' group of the row sweep); false if one does not fit 32 bits
+static bool rho_bufs(const Lev<float> &L, const RhoLev &R, const RowBufs &RB, RhoBufs &HB)
+{
+    const float *p[5] = {R.rho, R.cxy, R.cz, R.bzl, L.bzh};
+    unsigned o[5];
+    for (int a = 0; a < 5; ++a) {
+        const long d = (const char *)p[a] - RB.bw;
+        if (!p[a] || d < 0 || (double)d + 4.0 * L.nx * L.ny * L.nz >= 4294967296.0) return false;
+        o[a] = (unsigned)d;
+    }
+    HB.rho = o[0]; HB.cxy = o[1]; HB.cz = o[2]; HB.bzl = o[3]; HB.bzh = o[4];
+    return true;
+}
+
+bool launch_gsrb_rho(const Lev<float> &L, const RhoLev &R, const float *pin, float *pout, int mode, hipStream_t s)
+{
+    RowBufs RB;
+    RhoBufs HB;
+    if (!rho_level_ok(3, L) || !row_bufs<float>(L, pin, pout, nullptr, RB) || !rho_bufs(L, R, RB, HB)) return false;
+    if (L.nx == 256) return launch_rows_rho_w<6, 768, 2>(L, RB, HB, mode, s);
+    return launch_rows_rho_w<12, 768, 1>(L, RB, HB, mode, s);
 }
 
 bool compact_level_ok(int dim, const Lev<float> &L)
@@ -3124,9 +3691,10 @@ bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStre
 
 template <typename T>
 bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
-                           const Compact16 *K)
+                           const Compact16 *K, const RhoLev *Rh)
 {
     if (F.alpha && K) return false;
+    if (Rh && (K || sizeof(T) != 4 || !rho_level_ok(dim, *(const Lev<float> *)&F))) return false;
     if (dim == 3 && F.nx % 128 == 0 && C.nz >= 1) {
         RowMarchGrid G;
         G.W = F.nx / 128;
@@ -3139,7 +3707,12 @@ bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> 
         G.total = C.ny * G.W * chunks;
         const int blocks = (G.total + 3) / 4;
         G.per_xcd = (blocks + 7) / 8;
-        if (K) {
+        if (Rh) {
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_rr_rows_rho, dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, *Rh);
+            else
+                return false;
+        } else if (K) {
             if constexpr (sizeof(T) == 4)
                 hipLaunchKernelGGL((k_rr_rows<T, true>), dim3(8 * G.per_xcd), dim3(256), 0, s, F, phi, C, G, *K);
             else
@@ -3168,14 +3741,22 @@ template bool launch_gsrb_fused<double>(const Lev<double> &, const double *, dou
 template bool launch_gsrb_fused<float>(const Lev<float> &, const float *, float *, int, hipStream_t,
                                        const Compact16 *);
 template bool launch_resid_restrict<double>(int, const Lev<double> &, const double *,
-                                            const Lev<double> &, hipStream_t, const Compact16 *);
+                                            const Lev<double> &, hipStream_t, const Compact16 *, const RhoLev *);
 template bool launch_resid_restrict<float>(int, const Lev<float> &, const float *,
-                                           const Lev<float> &, hipStream_t, const Compact16 *);
+                                           const Lev<float> &, hipStream_t, const Compact16 *, const RhoLev *);
 
 void launch_compact_level0(const Lev<double> &F, const Compact16 &K, const Lev<float> &C, hipStream_t s)
 {
     const long n = (long)C.nx * C.ny * C.nz;
     hipLaunchKernelGGL(k_compact_level0, dim3(nblk(n)), dim3(TPB), 0, s, F, K, C);
+}
+
+void launch_rho_prep(const Lev<double> &F, const RhoLev &R, const Lev<float> &F32, const Lev<float> &C,
+                     const double *b0, const double *b0e, int alt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rho_coef, dim3(nblk(F.nz + 1)), dim3(TPB), 0, s, b0, b0e, alt, F.nz, R);
+    const long n = (long)C.nx * C.ny * C.nz;
+    hipLaunchKernelGGL(k_rho_level0, dim3(nblk(n)), dim3(TPB), 0, s, F, R, F32.bzh, C);
 }
 
 void launch_to_bf16(const float *in, unsigned short *out, long n, hipStream_t s)

@@ -24,6 +24,7 @@ struct SetupArgs {
     Lev<double> L;
     double *part;
     int *err;
+    float *rho32;   // optional: rho of each cell in FP32 (the FP32 solver's rho form)
 };
 
 struct CorrectArgs {
@@ -172,10 +173,31 @@ template <typename T> void launch_gsrb(int dim, const Lev<T> &L, int color, hipS
 template <typename T>
 bool launch_gsrb_fused(const Lev<T> &L, const T *pin, T *pout, int mode, hipStream_t s,
                        const Compact16 *K = nullptr);
-// fine residual (of phi) restricted to C.rhs; C.phi cleared.  K as above.
+// fine residual (of phi) restricted to C.rhs; C.phi cleared.  K as above;
+// Rh: FP32 level in the rho form (the weights formed from rho)
+struct RhoLev;
 template <typename T>
 bool launch_resid_restrict(int dim, const Lev<T> &F, const T *phi, const Lev<T> &C, hipStream_t s,
-                           const Compact16 *K = nullptr);
+                           const Compact16 *K = nullptr, const RhoLev *Rh = nullptr);
+// FP32 finest level of a plane-parallel projection in the rho form: the
+// cell density and the factors the rho sweep forms its face weights from
+// (parts of the level's allocation): cxy[k] = 2 beta0(k)^p per plane, cz[f]
+// = 2 beta0_edge(f)^p per z face, bzl the folded low z faces (the level's
+// bzh the high ones)
+struct RhoLev {
+    float *rho, *cxy, *cz, *bzl;
+};
+// whether a level takes the rho-form sweep (3D, x and y periodic, nx = 128
+// or 256, no alpha)
+bool rho_level_ok(int dim, const Lev<float> &L);
+// the rho form's face factors (from the planar beta0 and beta0_edge), its
+// z slabs and the FP32 level 1 C, from the FP64 level 0 F (F32: the FP32
+// level 0, whose bzh slab is written)
+void launch_rho_prep(const Lev<double> &F, const RhoLev &R, const Lev<float> &F32, const Lev<float> &C,
+                     const double *b0, const double *b0e, int alt, hipStream_t s);
+// one fused red-black sweep (pin -> pout) with the weights formed from rho;
+// false (nothing launched) where it does not apply
+bool launch_gsrb_rho(const Lev<float> &L, const RhoLev &R, const float *pin, float *pout, int mode, hipStream_t s);
 // whether every level-0 kernel of the mixed path has a compact-storage form
 // for this level (3D, nx = 128 or 256 and the full-row sweep fits)
 bool compact_level_ok(int dim, const Lev<float> &L);

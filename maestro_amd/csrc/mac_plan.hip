@@ -43,6 +43,9 @@ struct Hier {
     bool built = false;
     std::vector<T *> alpha;   // per-level alpha storage (mac_solve), in the level's allocation
     bool alpha_on = false;    // whether lev[].alpha (and lev_dev) point at it
+    // FP32 3D level 0 only: the rho form of the finest level (parts of its
+    // allocation, so the row kernels reach them through the weights' buffer)
+    RhoLev rho0 = {};
 };
 
 struct HostStage {
@@ -62,6 +65,7 @@ struct mac_plan {
     int phys[3][2];
     bool singular = true;
     bool helm = false;            // inside mac_solve: alpha and the call's BCs active
+    bool planar = false;          // the current projection has a planar beta0 (the rho form may apply)
     int ng_u = 0, ng_phi = 0, ng_rho = 0, ng_rhs = 0, ng_cart = 0;
     mac_opts opts;
     std::vector<BoxDev> boxes;
@@ -221,23 +225,30 @@ static int build_hier(mac_plan *P, Hier<T> &H)
         // the last part holds alpha (mac_solve); L.alpha points at it only
         // while a solve with alpha runs
         T *alpha_part = nullptr;
-        T **arr[11] = {&L.phi, &L.phi2, &L.rhs, &L.res, &L.bx, &L.by, &L.bz, &L.bxh, &L.byh, &L.bzh, &alpha_part};
+        // the FP32 3D finest level also holds its rho form: rho, the plane and
+        // z-face factors and the low z slab
+        T *rho_p = nullptr, *cxy_p = nullptr, *cz_p = nullptr, *bzl_p = nullptr;
+        T **arr[15] = {&L.phi, &L.phi2, &L.rhs, &L.res, &L.bx, &L.by, &L.bz, &L.bxh, &L.byh, &L.bzh, &alpha_part,
+                       &rho_p, &cxy_p, &cz_p, &bzl_p};
         // second iterate: ping-pong of the fused 3D sweep, and on level 0
         // the output of the fused refinement update
         const bool has2 = P->dim == 3 || l == 0, d3 = P->dim == 3;
-        const long len[11] = {nc, has2 ? nc : 0, nc, nc, nc, nc, d3 ? nc : 0, (long)n[1] * n[2],
-                              (long)n[0] * n[2], d3 ? (long)n[0] * n[1] : 0, nc};
+        const bool rh = sizeof(T) == 4 && l == 0 && d3;
+        const long len[15] = {nc, has2 ? nc : 0, nc, nc, nc, nc, d3 ? nc : 0, (long)n[1] * n[2],
+                              (long)n[0] * n[2], d3 ? (long)n[0] * n[1] : 0, nc,
+                              rh ? nc : 0, rh ? (long)n[2] : 0, rh ? (long)n[2] + 1 : 0, rh ? (long)n[0] * n[1] : 0};
         auto part = [](long cnt) { return ((cnt * (long)sizeof(T) + 255) / 256) * 256; };
         long total = 0;
-        for (int a = 0; a < 11; ++a) total += part(len[a]);
+        for (int a = 0; a < 15; ++a) total += part(len[a]);
         void *blk = nullptr;
         HIPCHK(hipMalloc(&blk, (size_t)total));
         H.allocs.push_back(blk);
         long off = 0;
-        for (int a = 0; a < 11; ++a) {
+        for (int a = 0; a < 15; ++a) {
             *arr[a] = len[a] ? (T *)((char *)blk + off) : nullptr;
             off += part(len[a]);
         }
+        if (rh) H.rho0 = RhoLev{(float *)rho_p, (float *)cxy_p, (float *)cz_p, (float *)bzl_p};
         L.alpha = nullptr;
         H.alpha.push_back(alpha_part);
         for (int d = 0; d < P->dim; ++d) { n[d] /= 2; h[d] *= 2.0; }
@@ -313,7 +324,7 @@ static bool use_fused(const mac_plan *P) { return P->dim == 3; }
 // (cur tracks which holds it).  first_mode applies to the first sweep.
 template <typename T>
 static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_mode, int &cur,
-                         const Compact16 *K = nullptr)
+                         const Compact16 *K = nullptr, const RhoLev *Rh = nullptr)
 {
     const bool f64 = sizeof(T) == 8;
     Lev<T> &L = H.lev[l];
@@ -324,7 +335,13 @@ static void fused_sweeps(mac_plan *P, Hier<T> &H, int l, int nsweep, int first_m
         const int cls = mode == SWEEP_ZERO ? (f64 ? MAC_PROF_GSRB64_ZERO : MAC_PROF_GSRB32_ZERO)
                                            : (f64 ? MAC_PROF_GSRB64 : MAC_PROF_GSRB32);
         int q = (l == 0) ? prof_begin(P, cls) : -1;
-        if (launch_gsrb_fused<T>(L, pin, pout, mode, P->stream, K)) {
+        bool ok;
+        if constexpr (sizeof(T) == 4)
+            ok = Rh ? launch_gsrb_rho(L, *Rh, pin, pout, mode, P->stream)
+                    : launch_gsrb_fused<T>(L, pin, pout, mode, P->stream, K);
+        else
+            ok = launch_gsrb_fused<T>(L, pin, pout, mode, P->stream, K);
+        if (ok) {
             cur ^= 1;
         } else if (L.alpha && !K) {
             // a level with alpha that no fused sweep takes: the two colours in
@@ -383,9 +400,10 @@ static int cycle_shape(const mac_opts &o) { return o.cycle_type == 0 ? MAC_CYCLE
 // single-workgroup coarse cycle).  zero: level l's iterate is known to be
 // zero (a first visit below level 0, or a refinement step at level 0); a
 // second visit starts from the first one's iterate.  K: level 0's weights
-// and right side in compact storage (fused path).
+// and right side in compact storage; Rh: level 0 in the rho form (fused path).
 template <typename T>
-static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, const Compact16 *K)
+static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, const Compact16 *K,
+                       const RhoLev *Rh = nullptr)
 {
     hipStream_t s = P->stream;
     const int dim = P->dim;
@@ -404,17 +422,18 @@ static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, cons
     if (use_fused(P)) {
         int cur = 0;   // 1: the iterate is in phi2
         const Compact16 *Kl = l == 0 ? K : nullptr;
-        if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur, Kl);
+        const RhoLev *Rl = l == 0 ? Rh : nullptr;
+        if (o.nu1 > 0) fused_sweeps<T>(P, H, l, o.nu1, zero ? SWEEP_ZERO : SWEEP_PLAIN, cur, Kl, Rl);
         else if (zero) launch_fill<T>(L.phi, n, T(0), s);
         int q = (l == 0) ? prof_begin(P, sizeof(T) == 8 ? MAC_PROF_RESTRICT64 : MAC_PROF_RESTRICT32) : -1;
         // also zeroes level l + 1's iterate
-        if (!launch_resid_restrict<T>(dim, L, cur ? L.phi2 : L.phi, H.lev[l + 1], s, Kl))
+        if (!launch_resid_restrict<T>(dim, L, cur ? L.phi2 : L.phi, H.lev[l + 1], s, Kl, Rl))
             P->launch_failed = true;
         prof_end(P, q);
         cycle_from<T>(P, H, l + 1, type, true, K);
         if (again) cycle_from<T>(P, H, l + 1, again, false, K);
         prolong_cur<T>(P, H, l, cur);
-        if (o.nu2 > 0) fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PLAIN, cur, Kl);
+        if (o.nu2 > 0) fused_sweeps<T>(P, H, l, o.nu2, SWEEP_PLAIN, cur, Kl, Rl);
         if (cur)   // bring the iterate back to phi
             HIPCHK_V(hipMemcpyAsync(L.phi, L.phi2, sizeof(T) * (size_t)n, hipMemcpyDeviceToDevice, s));
         return;
@@ -433,9 +452,10 @@ static void cycle_from(mac_plan *P, Hier<T> &H, int l, int type, bool zero, cons
 // iterate is known to be zero (refinement step); K: level 0's weights and
 // right side in compact storage (fused path).
 template <typename T>
-static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compact16 *K = nullptr)
+static void vcycle(mac_plan *P, Hier<T> &H, bool zero_first = false, const Compact16 *K = nullptr,
+                   const RhoLev *Rh = nullptr)
 {
-    cycle_from<T>(P, H, 0, cycle_shape(P->opts), zero_first, K);
+    cycle_from<T>(P, H, 0, cycle_shape(P->opts), zero_first, K, Rh);
 }
 
 // whether level 0 of the FP32 hierarchy is kept in compact storage
@@ -445,6 +465,15 @@ static bool compact_applies(mac_plan *P)
 {
     return P->opts.precision == MAC_PREC_MIXED_BF16 && P->H32.built && use_fused(P) && !P->H64.alpha_on &&
            compact_level_ok(P->dim, P->H32.lev[0]);
+}
+
+// whether the FP32 finest level runs in the rho form: MAC_PREC_MIXED, a 3D
+// projection with a planar beta0 (the face coefficients are 2 beta0^p /
+// (rho_L + rho_R)), x and y periodic, nx = 128 or 256, at least two levels
+static bool rho_mode(mac_plan *P)
+{
+    return P->opts.precision == MAC_PREC_MIXED && P->dim == 3 && P->planar && !P->helm && !P->H64.alpha_on &&
+           P->H32.built && P->H32.nlev > 1 && P->H32.rho0.rho && rho_level_ok(3, P->H32.lev[0]);
 }
 
 // Level 0 of the FP32 hierarchy in compact storage: *K is set where it
@@ -595,7 +624,17 @@ static int solve(mac_plan *P, mac_stats *st)
         // alpha in FP32 (from the FP64 level 0 and by coarsening) where the FP64 level has it
         if ((rc = set_alpha<float>(P, P->H32, P->helm && P->H64.alpha_on))) return rc;
         const Compact16 *K = nullptr;
-        if (compact_applies(P) && P->H32.nlev > 1) {
+        const RhoLev *Rh = nullptr;
+        if (rho_mode(P)) {
+            // the finest level in the rho form (rho written by the setup):
+            // its face factors and z slabs, and FP32 level 1 from the FP64
+            // weights, in one pass; then the rest
+            launch_rho_prep(P->H64.lev[0], P->H32.rho0, P->H32.lev[0], P->H32.lev[1], P->b0_dev, P->b0e_dev,
+                            P->opts.use_alt_energy_fix, P->stream);
+            for (int l = 2; l < P->H32.nlev; ++l)
+                launch_coarsen<float>(P->dim, P->H32.lev[l - 1], P->H32.lev[l], P->stream);
+            Rh = &P->H32.rho0;
+        } else if (compact_applies(P) && P->H32.nlev > 1) {
             // compact level 0 and FP32 level 1 in one pass, then the rest
             if ((rc = compact_level0(P, &K, true))) return rc;
             for (int l = 2; l < P->H32.nlev; ++l)
@@ -614,7 +653,7 @@ static int solve(mac_plan *P, mac_stats *st)
         bool swapped = false;
         while (rc == MAC_OK && !converged(o, r, r0) && it < o.max_iter) {
             for (int c = 0; c < per_step; ++c) {
-                vcycle<float>(P, P->H32, c == 0, K);
+                vcycle<float>(P, P->H32, c == 0, K, Rh);
                 ++inner;
             }
             // phi64 + e32 lands in phi2 together with the next residual;
@@ -1069,6 +1108,11 @@ static int project_impl(mac_plan *P, double *const *umac, double *const *phi, co
     const int nb = P->nboxes, dim = P->dim;
     int rc;
     HIPCHK(hipSetDevice(P->device));
+    // a planar beta0 lets the FP32 finest level run in the rho form, whose
+    // rho the setup writes: the FP32 storage is allocated before it
+    P->planar = beta0_1d != nullptr;
+    if (!noalloc && (P->opts.precision == MAC_PREC_MIXED || P->opts.precision == MAC_PREC_MIXED_BF16))
+        if ((rc = build_hier<float>(P, P->H32))) return rc;
     // the call's device buffers
     const long np = setup_nparts(dim, P->maxcols, P->maxez, nb);
     ProjBufs PB;
@@ -1136,6 +1180,7 @@ static int project_impl(mac_plan *P, double *const *umac, double *const *phi, co
     A.beta0e = beta0_1d ? P->b0e_dev : nullptr;
     A.alt = P->opts.use_alt_energy_fix;
     A.L = P->H64.lev[0];
+    A.rho32 = rho_mode(P) ? P->H32.rho0.rho : nullptr;
     A.part = PB.part;
     A.err = PB.err;
     HIPCHK(hipMemsetAsync(PB.err, 0, sizeof(int), s));
@@ -1260,6 +1305,7 @@ extern "C" int mac_project_level(mac_plan *P, double *ux, double *uy, double *uz
     HIPCHK(hipEventRecord(P->ev[1], s));
     mac_stats local;
     memset(&local, 0, sizeof local);
+    P->planar = false;   // the rho form is for mac_project, whose setup writes rho
     int src = solve(P, &local);
     if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
     HIPCHK(hipEventRecord(P->ev[2], s));
@@ -1332,6 +1378,7 @@ extern "C" int mac_adjoint_level(mac_plan *P, const double *phi, const double *b
     HIPCHK(hipEventRecord(P->ev[1], s));
     mac_stats local;
     memset(&local, 0, sizeof local);
+    P->planar = false;   // the rho form is for mac_project, whose setup writes rho
     int src = solve(P, &local);
     if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
     if (P->singular)
@@ -1908,6 +1955,7 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
     // ---- solve
     mac_stats local;
     memset(&local, 0, sizeof local);
+    P->planar = false;   // the rho form is for mac_project, whose setup writes rho
     const int src = solve(P, &local);
     memcpy(&local.rhs_norm, &rhs_bits, sizeof(double));
     if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
