@@ -358,6 +358,12 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
                                     bfloat16 on level 0 of a MAC_PREC_MIXED_BF16
                                     plan where it stores them so), the result
                                     widened into phi */
+#define MAC_OP_GSRB32_RHO 9      /* `count` FP32 sweeps of level 0 in the rho form
+                                    (face weights formed from the FP32 density):
+                                    rho and its factors as the last MAC_PREC_MIXED
+                                    plane-parallel mac_project on this plan left
+                                    them, rhs and phi of level 0 rounded to FP32;
+                                    MAC_ERR_INVALID where the form does not apply */
 int mac_debug_level_dims(mac_plan *plan, int level, int dims[3]);
 int mac_debug_get(mac_plan *plan, int level, int what, double *out);
 int mac_debug_set(mac_plan *plan, int level, int what, const double *in);

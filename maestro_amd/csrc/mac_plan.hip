@@ -1531,6 +1531,24 @@ extern "C" int mac_debug_op(mac_plan *P, int level, int op, int count)
         HIPCHK(hipStreamSynchronize(s));
         return MAC_OK;
     }
+    if (op == MAC_OP_GSRB32_RHO) {
+        if (level != 0 || !P->H32.built || !P->H32.rho0.rho || !rho_level_ok(P->dim, P->H32.lev[0]))
+            return MAC_ERR_INVALID;
+        const Lev<float> &F = P->H32.lev[0];
+        const long n = (long)L.nx * L.ny * L.nz;
+        launch_convert<double, float>(L.phi, F.phi, n, s);
+        launch_convert<double, float>(L.rhs, F.rhs, n, s);
+        int cur = 0;
+        for (int c = 0; c < count; ++c) {
+            if (!launch_gsrb_rho(F, P->H32.rho0, cur ? F.phi2 : F.phi, cur ? F.phi : F.phi2, SWEEP_PLAIN, s))
+                return MAC_ERR_INVALID;
+            cur ^= 1;
+        }
+        launch_convert<float, double>(cur ? F.phi2 : F.phi, L.phi, n, s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        return MAC_OK;
+    }
     if (op == MAC_OP_GSRB && use_fused(P)) {   // sweeps alternate phi <-> phi2
         for (int c = 0; c < count; ++c)
             launch_gsrb_fused<double>(L, (c & 1) ? L.phi2 : L.phi, (c & 1) ? L.phi : L.phi2, SWEEP_PLAIN, s);

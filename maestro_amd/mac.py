@@ -46,7 +46,7 @@ PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coars
 ARR = {"phi": 0, "rhs": 1, "bx": 2, "by": 3, "bz": 4, "bxh": 5, "byh": 6, "bzh": 7, "res": 8}
 ARR_F32, ARR_BF16 = 0x10, 0x20   # debug_get(level, what, copy="f32" | "bf16")
 OP = {"gsrb": 0, "residual": 1, "restrict": 2, "prolong": 3, "coarsen": 4, "vcycle": 5,
-      "resid_restrict": 6, "prolong_sweep": 7, "gsrb32": 8}
+      "resid_restrict": 6, "prolong_sweep": 7, "gsrb32": 8, "gsrb32_rho": 9}
 
 
 class MacError(RuntimeError):

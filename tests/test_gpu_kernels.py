@@ -326,3 +326,66 @@ def test_compact_level0_and_level1_weights_bitwise(gpu, cfg, n, box):
         got = plan.debug_get(1, w, copy="f32").reshape(r.shape)
         assert np.array_equal(got, r.astype(np.float64)), w
     plan.close()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("n,box", [((128, 64, 32), 32), ((256, 16, 24), 8)])
+@pytest.mark.parametrize("zbc", [[WALL, WALL], [OUT, WALL], [P, P]])
+def test_gsrb32_rho_sweep(gpu, oracle_lib, n, box, zbc, sweeps):
+    """The rho form of the FP32 finest-level sweep (face weights formed in the
+    sweep from the FP32 density: 2 beta0^2 / (rho_L + rho_R) with the
+    hardware reciprocal, 1 ulp) against the oracle's FP32 sweep on the same
+    weights formed with IEEE division, and the domain's z faces from the
+    folded FP64 weights: within 1e-5 relative (a 1-ulp weight moves a cell
+    by ~1e-7 of its value).  The plan's rho and factors come from a
+    mixed-precision plane-parallel mac_project on the same inputs."""
+    O = oracle_lib
+    bc = [[P, P], [P, P], zbc]
+    rng = np.random.default_rng(31)
+    wl = W.make_workload(1, n=n, box=box)
+    wl["bc"] = bc
+    wl["rho"] = W._fill_rho_ghosts(wl["rho"][1:-1, 1:-1, 1:-1], bc, 3)
+    wl["faces"] = W._random_faces(rng, n, 3, bc)
+    if zbc[0] == P:   # one face at z = 0 and z = L: one beta0 there
+        wl["beta0_edge"] = wl["beta0_edge"].copy()
+        wl["beta0_edge"][-1] = wl["beta0_edge"][0]
+    from maestro_amd import boxes as BX
+    boxes, umac, phi0, rho, _, _ = BX.split_workload(wl)
+    lay = mac.Layout.uniform(n, box, wl["dx"], bc)
+    plan = mac.MacPlan(lay, mac.default_opts(precision=mac.PREC_MIXED, max_iter=0))
+    plan.project(umac, phi0, rho, None, np.ascontiguousarray(wl["beta0"]),
+                 np.ascontiguousarray(wl["beta0_edge"]), None, check=False)
+    phi = rng.standard_normal((n[2], n[1], n[0]))
+    rhs = rng.standard_normal((n[2], n[1], n[0])) * 1e4
+    plan.debug_set(0, "phi", phi.ravel())
+    plan.debug_set(0, "rhs", rhs.ravel())
+    plan.debug_op(0, "gsrb32_rho", sweeps)
+    got = plan.debug_get(0, "phi").reshape(phi.shape)
+    plan.close()
+    # the weights the rho form stands for, in FP32 with IEEE division
+    f32 = np.float32
+    rg = wl["rho"]                                  # one ghost layer
+    r32 = rg[1:-1, 1:-1, 1:-1].astype(f32)
+    b0, b0e = wl["beta0"], wl["beta0_edge"]
+    cxy = (2.0 * b0 * b0).astype(f32)[:, None, None]
+    cz = (2.0 * b0e * b0e).astype(f32)[:, None, None]
+    inv = lambda a, b_: f32(1.0) / (a + b_)
+    wx = cxy * inv(np.roll(r32, 1, axis=2), r32)    # low x faces (periodic)
+    wy = cxy * inv(np.roll(r32, 1, axis=1), r32)
+    wz = np.empty_like(r32)
+    wz[1:] = cz[1:-1] * inv(r32[:-1], r32[1:])
+    # the domain's z faces: the FP64 coefficient folded by the BC, in FP32
+    def zface(k_lo, k_hi, e, side):
+        beta = 2.0 / (rg[k_lo, 1:-1, 1:-1] + rg[k_hi, 1:-1, 1:-1]) * b0e[e] * b0e[e]
+        kind = ell(bc[2][side])
+        return (0.0 * beta if kind == 2 else (2.0 * beta if kind == 1 else beta)).astype(f32)
+    wz[0] = zface(0, 1, 0, 0)
+    bzh = zface(-2, -1, n[2], 1)
+    w = {"bx": wx.ravel(), "by": wy.ravel(), "bz": wz.ravel(), "bxh": wx[:, :, 0].ravel(),
+         "byh": wy[:, 0, :].ravel(), "bzh": bzh.ravel()}
+    ih2 = [f32(1.0 / wl["dx"][d] ** 2) for d in range(3)]
+    per = [1, 1, 1 if zbc[0] == P else 0]
+    ref = O.gsrb_folded_f32(3, n, per, ih2, w, phi.astype(f32), rhs.astype(f32), sweeps=sweeps).astype(np.float64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"n={n} zbc={zbc} sweeps={sweeps}: rel max diff {err:.2e}")
+    assert err <= 1e-5
