@@ -40,9 +40,16 @@
  *   bottom CG/BiCGStab  the textbook first step; the dense direct solution
  *   orc_mkumac          D(U_proj)=macrhs; test_projection exact recovery;
  *                       idempotence
+ *   orc_sum             math.fsum (the correctly rounded sum) on vectors with
+ *                       heavy cancellation, where plain and pairwise sums miss
+ *                       by more than 1e6 ulp
+ *   average_coeffs,     a tagged closed form: each fine face or cell carries a
+ *   average_alpha       code of the coarse face or cell it belongs to plus a
+ *                       sub-face tag, so the coarse mean is known exactly
  * Parity unpinned (only FBoxLib could fix them): iteration counts and
  * iterates, FBoxLib's own F/W cycle shapes and BiCGStab details, the
- * stencil_order = 2 Dirichlet stencil, the coarse-coefficient rule.
+ * stencil_order = 2 Dirichlet stencil, and whether FBoxLib's coarse
+ * coefficient is this face mean (the mean itself is pinned above).
  */
 #include <math.h>
 #include <stdint.h>

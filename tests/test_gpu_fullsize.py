@@ -208,7 +208,8 @@ def test_star_512_mixed_and_fp64(gpu):
     e64, emx = err(f64, p64), err(fmx, pmx)
     print(f"  vs the 1e-12 solve: FP64 phi {e64[0]:.2e} U {e64[1]:.2e}; mixed phi {emx[0]:.2e} U {emx[1]:.2e}")
     for k in range(2):
-        assert e64[k] <= 1e-6 and emx[k] <= max(1e-8, 3.0 * e64[k])
+        # measured: FP64 phi 1.2e-7, U 1.8e-9; mixed phi 2.8e-8, U 4.1e-10
+        assert e64[k] <= (3e-7, 5e-9)[k] and emx[k] <= max(1e-8, 3.0 * e64[k])
 
 
 def test_adjoint_256_matches_finite_differences(gpu, oracle_lib):

@@ -319,3 +319,17 @@ def test_explicit_inner_cycles(gpu, prec, inner):
     assert st["converged"] and st["inner_cycles"] == inner * st["iters"]
     assert sd["inner_cycles"] == (3 if prec == 1 else 2) * sd["iters"]
     assert rel_err(out["phi"], truth["phi"], remove_mean=True) <= 1e-8
+
+
+@pytest.mark.parametrize("cfg,n,box", [(0, (128, 128), 32), (2, (96, 64, 64), 32)])
+def test_default_inner_cycles_follow_storage(gpu, cfg, n, box):
+    """MAC_PREC_MIXED_BF16 where the compact storage does not apply (2D; nx
+    neither 128 nor 256) keeps FP32 storage, and so runs the FP32-storage
+    default of 3 FP32 cycles per refinement step."""
+    wl = W.make_workload(cfg, n=n, box=box)
+    truth = run_oracle(wl, eps=1e-13, alt=1)
+    out = run_gpu(wl, use_alt_energy_fix=1, precision=2)
+    st = out["stats"]
+    print(f"n={n}: {st['iters']} steps, {st['inner_cycles']} cycles")
+    assert st["converged"] and st["inner_cycles"] == 3 * st["iters"]
+    assert rel_err(out["phi"], truth["phi"], remove_mean=True) <= 1e-8
