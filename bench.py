@@ -12,10 +12,14 @@ step on its own fresh copy of the inputs (staged in HBM before timing).
 with FP32 iterate, coefficients and residual inside FP64 iterative
 refinement; the finest level's FP32 coefficients are formed in the sweep
 from the FP32 density, 2 beta0^2 / (rho_L + rho_R), the coarser levels'
-stored) in solves/s; the plain FP64 V-cycle and the variant that stores
-the finest level's FP32-rounded weights and right side as bfloat16 (same
-FP32 arithmetic, same FP64 tolerance) are measured the same way and reported
-beside it.  ``roofline`` covers the dominant kernel of the
+stored; Gauss-Seidel red-black sweeps, the library's default) in solves/s;
+the plain FP64 V-cycle (the reference algorithm) and the variant that
+stores the finest level's FP32-rounded weights and right side as bfloat16
+(same FP32 arithmetic, same FP64 tolerance) are measured the same way and
+reported beside it, and so are the mixed and FP64 paths with the sweeps
+over-relaxed by 1.25 (mac_opts.omega): fewer cycles to the same 1e-10
+relative residual, with phi further from the exact discrete solution
+(``accuracy`` reports each path against the FP64 path and the oracle).  ``roofline`` covers the dominant kernel of the
 headline path: the finest-level fused red-black sweep (one full sweep per
 launch, the launches that read the old iterate), timed with HIP events on
 the library's own stream during the timed steps.  Its ``traffic`` figure is
@@ -39,7 +43,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False) -> str:
+def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False, sor: bool = False) -> str:
     """Name of the finest-level sweep kernel the library launches (b16: the
     compact bfloat16-storage form of the FP32 full-row kernel; rho: the FP32
     form that forms its weights from the density, which the plane-parallel
@@ -47,16 +51,17 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = Fals
     W = nx / 128 waves per row; rows x threads per block: FP64 6 x 768, FP32
     6 x 768 with the weights fetched one plane ahead (16 x 1024 at nx = 128),
     FP32 with bfloat16 storage 6 x 768; the template's last two arguments: no
-    alpha term, the weights' prefetch distance."""
+    alpha term, the weights' prefetch distance; the last: over-relaxed (sor)."""
     t = "double" if s == 8 else "float"
     c = "true" if b16 else "false"
+    o = "true" if sor else "false"
     w = max(1, nx // 128)
     if rho:   # FP32, the weights formed from rho: 6 rows x 768 (12 x 768 at nx = 128)
-        return f"k_gsrb_rows_rho<0, 6, 768, {w}>" if w == 2 else "k_gsrb_rows_rho<0, 12, 768, 1>"
+        return f"k_gsrb_rows_rho<0, 6, 768, {w}, {o}>" if w == 2 else f"k_gsrb_rows_rho<0, 12, 768, 1, {o}>"
     if s == 8 or b16:
-        return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 2>"
-    return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false, 2>" if w == 1
-            else f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 1>")
+        return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 2, {o}>"
+    return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false, 2, {o}>" if w == 1
+            else f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 1, {o}>")
 
 
 def measured_traffic(name: str):
@@ -98,8 +103,8 @@ def stage(per_box, copies, torch):
     return sets, dr
 
 
-def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=0, profile=True):
-    plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner)
+def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=0, profile=True, omega=0.0):
+    plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner, omega=omega)
     stats = []
     for w in range(warmup):
         du, dp = sets[w]
@@ -392,8 +397,10 @@ def main():
     ncell = n[0] * n[1] * n[2]
 
     results = {}
-    paths = (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED), ("mixed_bf16", mac.PREC_MIXED_BF16))
-    for name, prec in paths:
+    # (name, precision, omega): 0 = the default, Gauss-Seidel red-black
+    paths = (("fp64", mac.PREC_FP64, 0.0), ("mixed", mac.PREC_MIXED, 0.0), ("mixed_bf16", mac.PREC_MIXED_BF16, 0.0),
+             ("mixed_over_relaxed", mac.PREC_MIXED, 1.25), ("fp64_over_relaxed", mac.PREC_FP64, 1.25))
+    for name, prec, omega in paths:
         # refresh the staged inputs from the host copies (outside timing)
         boxes, umac, phi, _, _, _ = pb
         for du, dp in sets:
@@ -404,7 +411,7 @@ def main():
                 a.copy_(torch.from_numpy(h))
         torch.cuda.synchronize()
         dt, st, prof = run_precision(plan, sets, dr, b0, b0e, args.warmup, args.steps, prec, torch,
-                                     args.inner_cycles, profile=not args.no_profile)
+                                     args.inner_cycles, profile=not args.no_profile, omega=omega)
         # host snapshot of the last step's outputs (the staged buffers are reused)
         results[name] = {"seconds": dt, "stats": st, "prof": prof,
                          "umac": [[a.cpu().numpy() for a in box] for box in sets[-1][0]],
@@ -412,17 +419,21 @@ def main():
         print(f"# {name}: {args.steps / dt:.2f} solves/s, {1e3 * dt / args.steps:.2f} ms/solve, "
               f"iters {[s['iters'] for s in st]}, res/res0 {st[-1]['res'] / st[-1]['res0']:.2e}",
               file=sys.stderr, flush=True)
+    plan.set_opts(omega=0.0)
 
     def sweep_roofline(path):
         """Roofline of the finest-level sweep of one path (the launches that
         read the old iterate)."""
-        s_, ws, b16, rho, cls = {"fp64": (8, 8, False, False, "gsrb64"), "mixed": (4, 4, False, True, "gsrb32"),
-                                 "mixed_bf16": (4, 2, True, False, "gsrb32")}[path]
+        s_, ws, b16, rho, cls, sor = {
+            "fp64": (8, 8, False, False, "gsrb64", False), "mixed": (4, 4, False, True, "gsrb32", False),
+            "mixed_bf16": (4, 2, True, False, "gsrb32", False),
+            "mixed_over_relaxed": (4, 4, False, True, "gsrb32", True),
+            "fp64_over_relaxed": (8, 8, False, False, "gsrb64", True)}[path]
         cnt_, ms_ = results[path]["prof"][cls]
         avg = ms_ / cnt_ if cnt_ else float("inf")   # --no-profile: no per-kernel timing
         nbytes = gsrb_launch_bytes(ncell, s_, ws, rho)
         ach = nbytes / (avg * 1e-3) / 1e9
-        name = sweep_kernel_name(s_, b16, n[0], rho)
+        name = sweep_kernel_name(s_, b16, n[0], rho, sor)
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name),
                 "kernel": name + " (finest level, one full red-black sweep)",
@@ -463,7 +474,7 @@ def main():
                 "u_max": float(max(np.abs(fa[d] - fb[d]).max() / np.abs(fb[d]).max() for d in range(dim))),
                 "u_rel": float(max(np.linalg.norm(fa[d] - fb[d]) / np.linalg.norm(fb[d]) for d in range(dim))),
             }
-        for path in ("mixed", "mixed_bf16"):
+        for path in ("mixed", "mixed_bf16", "mixed_over_relaxed", "fp64_over_relaxed"):
             fm, pm = gather(results[path])
             acc[f"{path}_vs_fp64_gpu"] = errs(fm, pm, f6, p6)
 
@@ -485,19 +496,21 @@ def main():
                          f"{ref['stats'].iters} V-cycles each: median of {args.cpu_solves} after one warm-up, "
                          f"{tc:.2f} s (all: {', '.join('%.2f' % x for x in tcs[1:])} s)"}
         if not args.no_check:
-            po = ref["phi"] - ref["phi"].mean()
-            for path in ("fp64", "mixed", "mixed_bf16"):
-                tag = f"{path}_vs_oracle"
-                fa, pa = gather(results[path])
-                pa0 = pa - pa.mean()
-                acc[tag] = {
-                    "phi_max": float(np.abs(pa0 - po).max() / np.abs(po).max()),
-                    "phi_rel": float(np.linalg.norm(pa0 - po) / np.linalg.norm(po)),
-                    "u_max": float(max(np.abs(fa[d] - ref["faces"][d]).max() /
-                                       np.abs(ref["faces"][d]).max() for d in range(dim))),
-                    "u_rel": float(max(np.linalg.norm(fa[d] - ref["faces"][d]) /
-                                       np.linalg.norm(ref["faces"][d]) for d in range(dim))),
-                }
+            # against the oracle at the same tolerance, and against its 1e-13
+            # solve (the exact discrete solution to rounding)
+            exact = O.macproject(g, wl["faces"], wl["phi"], wl["rho"], beta0=wl["beta0"],
+                                 beta0_edge=wl["beta0_edge"], opts=O.default_opts(eps_rel=1e-13))
+            for which, r in (("oracle", ref), ("oracle_1e-13", exact)):
+                fo, po = r["faces"], r["phi"] - r["phi"].mean()
+                for path in ("fp64", "mixed", "mixed_bf16", "mixed_over_relaxed", "fp64_over_relaxed"):
+                    fa, pa = gather(results[path])
+                    pa0 = pa - pa.mean()
+                    acc[f"{path}_vs_{which}"] = {
+                        "phi_max": float(np.abs(pa0 - po).max() / np.abs(po).max()),
+                        "phi_rel": float(np.linalg.norm(pa0 - po) / np.linalg.norm(po)),
+                        "u_max": float(max(np.abs(fa[d] - fo[d]).max() / np.abs(fo[d]).max() for d in range(dim))),
+                        "u_rel": float(max(np.linalg.norm(fa[d] - fo[d]) / np.linalg.norm(fo[d]) for d in range(dim))),
+                    }
         print(f"# cpu oracle: {tc:.2f} s/solve on {O.num_threads()} threads", file=sys.stderr)
 
     storage = {"mixed": "FP32 V-cycle inside FP64 IR, finest-level FP32 coefficients formed from the FP32 density",
@@ -524,10 +537,18 @@ def main():
         "solves_per_s": {p: round(args.steps / results[p]["seconds"], 3) for p in results},
         "mixed_over_fp64": round(fp64["seconds"] / mixed["seconds"], 3),
         "compact_bf16_over_fp64": round(fp64["seconds"] / results["mixed_bf16"]["seconds"], 3),
+        # the sweeps over-relaxed by 1.25 (mac_opts.omega): against the plain
+        # FP64 V-cycle, and against the FP64 V-cycle with the same sweeps
+        "over_relaxed": {
+            "mixed_over_fp64": round(fp64["seconds"] / results["mixed_over_relaxed"]["seconds"], 3),
+            "mixed_over_fp64_over_relaxed": round(results["fp64_over_relaxed"]["seconds"] /
+                                                  results["mixed_over_relaxed"]["seconds"], 3)},
         "iters": {"fp64_vcycles": fp64["stats"][-1]["iters"],
-                  **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"] for p in ("mixed", "mixed_bf16")},
+                  "fp64_over_relaxed_vcycles": results["fp64_over_relaxed"]["stats"][-1]["iters"],
+                  **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"]
+                     for p in ("mixed", "mixed_bf16", "mixed_over_relaxed")},
                   **{f"{p}_fp32_vcycles": results[p]["stats"][-1]["inner_cycles"]
-                     for p in ("mixed", "mixed_bf16")}},
+                     for p in ("mixed", "mixed_bf16", "mixed_over_relaxed")}},
         "roofline": sweep_roofline(head),
         "roofline_other": {p: sweep_roofline(p) for p in results if p != head},
         "kernel_ms_per_solve": {

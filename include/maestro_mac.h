@@ -120,6 +120,16 @@ typedef struct mac_opts {
     int bottom_solver;       /* MAC_BOTTOM_SMOOTH, _BICGSTAB or _CG; default
                                 CG (mg_bottom_solver, Source/_parameters:386,
                                 Source/mac_multigrid.f90:62)                 */
+    double omega;            /* over-relaxation of the red-black sweeps on the
+                                levels above the single-workgroup coarse
+                                cycle (a cell moves to old + omega (gs - old)),
+                                0 < omega < 2; default 0: 1, FBoxLib's GSRB.
+                                1.25 takes the 256^3 plane-parallel solve from
+                                9 to 6 V-cycles to the same relative residual,
+                                with phi 6.5e-9 from the exact discrete
+                                solution against 5.1e-10.  No reference
+                                counterpart (the reference's smoother is
+                                GS_RB).                                      */
 } mac_opts;
 
 typedef struct mac_stats {

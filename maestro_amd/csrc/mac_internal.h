@@ -38,6 +38,11 @@ struct Lev {
     // the projection (alpha = 0).  Only the generic kernels (per-colour
     // sweep, residual, coarsening, coarse cycle) read it.
     T *alpha;
+    // over-relaxation of this level's red-black sweeps: a cell moves to
+    // old + omega (gs - old); 1 (set when the level is built) keeps
+    // Gauss-Seidel, and only then do the sweeps store the Gauss-Seidel value
+    // itself (the single-workgroup coarse cycle never reads it)
+    T omega;
 };
 
 struct BoxDev {
@@ -72,6 +77,15 @@ __device__ __forceinline__ unsigned bf16_pack(float lo, float hi)
 // ------------------------------------------------------------ math helpers
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __fma_rn(a, b, c); }
 __device__ __forceinline__ float dfma(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+// the new value of a cell from its Gauss-Seidel value gs and its old value:
+// gs itself, or over-relaxed old + omega (gs - old) as one fma (red-black SOR
+// as a multigrid smoother: I. Yavneh, SIAM J. Sci. Comput. 17 (1996) 180-192)
+template <bool SOR, typename T>
+__device__ __forceinline__ T relax(T gs, T old, T omega)
+{
+    if constexpr (SOR) return dfma(omega, gs - old, old);
+    else return gs;
+}
 
 // Neighbour values and face weights of cell (i,j,k), index c.
 // Order: x-, x+, y-, y+, z-, z+.

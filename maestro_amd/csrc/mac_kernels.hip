@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(TPB) k_convert(const A *__restrict__ in, B *__
 
 // --------------------------------------------------------------- GSRB
 // one colour of a red-black sweep; thread per (i-pair, j, k)
-template <int DIM, typename T>
+template <int DIM, typename T, bool SOR = false>
 __global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
 {
     const int hx = (L.nx + 1) >> 1;
@@ -573,7 +573,8 @@ __global__ void __launch_bounds__(TPB) k_gsrb(Lev<T> L, int color)
     long c = i + (long)L.nx * (j + (long)L.ny * k);
     T w[6], p[6];
     stencil<DIM, T>(L, L.phi, i, j, k, c, w, p);
-    L.phi[c] = gs_value<DIM, T>(L, w, p, L.rhs[c], alpha_at(L, c));
+    const T v = gs_value<DIM, T>(L, w, p, L.rhs[c], alpha_at(L, c));
+    L.phi[c] = relax<SOR>(v, L.phi[c], L.omega);
 }
 
 __device__ __forceinline__ void wrap_coord(int x, int n, int per, int &xw, bool &ok)
@@ -596,7 +597,7 @@ __device__ __forceinline__ void wrap_coord(int x, int n, int per, int &xw, bool 
 // iterate PD+1 planes ahead, held in register queues.  One column per thread,
 // 32-bit indexing (levels below 2^31 cells).  MODE 0: plain; 1: zero old
 // iterate.  AL: the level has alpha (its value travels with the face weights)
-template <typename T, int MODE, int EY, int PD, bool AL = false>
+template <typename T, int MODE, int EY, int PD, bool AL = false, bool SOR = false>
 __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__restrict__ pin,
                                                       T *__restrict__ pout, FusedGrid G)
 {
@@ -737,7 +738,7 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
         // (b) red cells of plane k (tile and ring)
         if (active && red) {
             T p[6] = {buf[pos - 1], buf[pos + 1], buf[pos - PX], buf[pos + PX], pm1, pp1};
-            T v = gs_value<3, T>(L, cw, p, crhs, alpha_of(cw));
+            T v = relax<SOR>(gs_value<3, T>(L, cw, p, crhs, alpha_of(cw)), p0, L.omega);
             buf[pos] = v;
             p0 = v;
         }
@@ -746,7 +747,7 @@ __global__ void __launch_bounds__(64 * EY) k_gsrb_deep(Lev<T> L, const T *__rest
         if (outcol && km >= z0 && km < z1) {
             if ((pxy + km) & 1) {
                 T p[6] = {pbuf[pos - 1], pbuf[pos + 1], pbuf[pos - PX], pbuf[pos + PX], pm2, p0};
-                pm1 = gs_value<3, T>(L, pw, p, prhs, alpha_of(pw));
+                pm1 = relax<SOR>(gs_value<3, T>(L, pw, p, prhs, alpha_of(pw)), pm1, L.omega);
             }
             pout[col + plane * km] = pm1;
         }
@@ -885,7 +886,7 @@ struct RowBufs {
 // (the z faces always two): 2, or 1, which keeps one slot of the input ring
 // fewer in registers (the FP32 form of 6 rows then fits 80 VGPRs, two blocks
 // per CU).
-template <typename T, int MODE, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2>
+template <typename T, int MODE, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2, bool SOR = false>
 __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweepGrid G)
 {
     static_assert(!B16 || sizeof(T) == 4, "compact storage widens to FP32");
@@ -1148,7 +1149,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
                     xp = f0b;
                 }
                 const T nb[6] = {xm, xp, buf[lp - PXL + R], buf[lp + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
-                red_v = gs_value<3, T>(L, wt, nb, cw.v[9 + R], alpha_of(cw));
+                red_v = relax<SOR>(gs_value<3, T>(L, wt, nb, cw.v[9 + R], alpha_of(cw)), f0R, L.omega);
                 buf[lp + R] = red_v;
                 f0R = red_v;
             }
@@ -1165,7 +1166,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
                     xp = m1b;
                 }
                 const T nb[6] = {xm, xp, pbuf[lp - PXL + R], pbuf[lp + PXL + R], R ? m2b : m2a, f0R};
-                blk_v = gs_value<3, T>(L, wt, nb, pw.v[9 + R], alpha_of(pw));
+                blk_v = relax<SOR>(gs_value<3, T>(L, wt, nb, pw.v[9 + R], alpha_of(pw)), R ? m1b : m1a, L.omega);
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
@@ -1226,7 +1227,7 @@ struct RhoBufs {      // byte offsets in the weights' allocation (with B.rhs)
     unsigned bzh;     // and of its high z faces
 };
 
-template <int MODE, int EY, int NTH, int W>
+template <int MODE, int EY, int NTH, int W, bool SOR = false>
 __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, RhoBufs RB, RowSweepGrid G)
 {
     using T = float;
@@ -1437,7 +1438,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, 
                     xp = f0b;
                 }
                 const T nb[6] = {xm, xp, buf[lp - PXL + R], buf[lp + PXL + R], R ? m1b : m1a, R ? p1b : p1a};
-                red_v = gs_value<3, T>(L, wt, nb, cw.v[2 + R]);
+                red_v = relax<SOR>(gs_value<3, T>(L, wt, nb, cw.v[2 + R]), f0R, L.omega);
                 buf[lp + R] = red_v;
                 f0R = red_v;
             }
@@ -1454,7 +1455,7 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, 
                     xp = m1b;
                 }
                 const T nb[6] = {xm, xp, pbuf[lp - PXL + R], pbuf[lp + PXL + R], R ? m2b : m2a, f0R};
-                blk_v = gs_value<3, T>(L, wt, nb, pw.v[2 + R]);
+                blk_v = relax<SOR>(gs_value<3, T>(L, wt, nb, pw.v[2 + R]), R ? m1b : m1a, L.omega);
                 T2 o;
                 o.x = R ? m1a : blk_v;
                 o.y = R ? blk_v : m1b;
@@ -3404,7 +3405,10 @@ template <typename T>
 void launch_gsrb(int dim, const Lev<T> &L, int color, hipStream_t s)
 {
     long nq = (long)((L.nx + 1) >> 1) * L.ny * L.nz;
-    DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
+    if (L.omega != T(1))
+        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T, true>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
+    else
+        DISPATCH_DIM(dim, hipLaunchKernelGGL((k_gsrb<D_, T>), dim3(nblk(nq)), dim3(TPB), 0, s, L, color));
 }
 
 template <typename T, int EY, int PD>
@@ -3425,17 +3429,18 @@ static void launch_deep(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
     G.total = tiles * zc;
     G.per_xcd = (G.total + 7) / 8;
     dim3 g(8 * G.per_xcd), b(64 * EY);
-    if (L.alpha) {
+    auto go = [&](auto AC, auto SC) {
+        constexpr bool AL = decltype(AC)::value, SOR = decltype(SC)::value;
         if (mode == SWEEP_ZERO)
-            hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD, true>), g, b, 0, s, L, pin, pout, G);
+            hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD, AL, SOR>), g, b, 0, s, L, pin, pout, G);
         else
-            hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD, true>), g, b, 0, s, L, pin, pout, G);
-        return;
-    }
-    if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_deep<T, 1, EY, PD>), g, b, 0, s, L, pin, pout, G);
-    else
-        hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD>), g, b, 0, s, L, pin, pout, G);
+            hipLaunchKernelGGL((k_gsrb_deep<T, 0, EY, PD, AL, SOR>), g, b, 0, s, L, pin, pout, G);
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+    const bool sor = L.omega != T(1);
+    if (L.alpha) sor ? go(Y{}, Y{}) : go(Y{}, N{});
+    else sor ? go(N{}, Y{}) : go(N{}, N{});
 }
 
 template <typename T, int EY, int NTH>
@@ -3540,7 +3545,7 @@ static int resident_slots(const void *kern, int threads, size_t lds)
     return cus * per;
 }
 
-template <typename T, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2>
+template <typename T, int EY, int NTH, bool B16, int W, bool AL = false, int PDW = 2, bool SOR = false>
 static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(T) * 4 * (size_t)(EY + 2) * (L.nx + 4);
@@ -3553,19 +3558,19 @@ static bool launch_rows_w(const Lev<T> &L, const RowBufs &RB, int mode, hipStrea
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             return false;
         hipError_t e = (mode == SWEEP_ZERO)
-                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW>,
-                                                                          threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW>,
-                                                                          threads, lds);
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                 &per, k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW, SOR>, threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                 &per, k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW, SOR>, threads, lds);
         if (e != hipSuccess || per < 1) return false;
         slots = cus * per;
     }
     const RowSweepGrid G = row_grid(L.ny, L.nz, EY, W, slots);
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 1, EY, NTH, B16, W, AL, PDW, SOR>), g, b, lds, s, L, RB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW>), g, b, lds, s, L, RB, G);
+        hipLaunchKernelGGL((k_gsrb_rows<T, 0, EY, NTH, B16, W, AL, PDW, SOR>), g, b, lds, s, L, RB, G);
     return true;
 }
 
@@ -3579,22 +3584,29 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
 {
     RowBufs RB;
     if (!rows_fit<T, EY, NTH>(L) || !row_bufs<T>(L, pin, pout, B16 ? K : nullptr, RB)) return false;
+    const bool sor = L.omega != T(1);
     if constexpr (!B16) {
         if (L.alpha) {
-            if (L.nx == 128) return launch_rows_w<T, EY, NTH, false, 1, true>(L, RB, mode, s);
-            return launch_rows_w<T, EY, NTH, false, 2, true>(L, RB, mode, s);
+            if (L.nx == 128)
+                return sor ? launch_rows_w<T, EY, NTH, false, 1, true, 2, true>(L, RB, mode, s)
+                           : launch_rows_w<T, EY, NTH, false, 1, true>(L, RB, mode, s);
+            return sor ? launch_rows_w<T, EY, NTH, false, 2, true, 2, true>(L, RB, mode, s)
+                       : launch_rows_w<T, EY, NTH, false, 2, true>(L, RB, mode, s);
         }
     } else {
         if (L.alpha) return false;
     }
-    if (L.nx == 128) return launch_rows_w<T, EY, NTH, B16, 1, false, PDW>(L, RB, mode, s);
-    return launch_rows_w<T, EY, NTH, B16, 2, false, PDW>(L, RB, mode, s);
+    if (L.nx == 128)
+        return sor ? launch_rows_w<T, EY, NTH, B16, 1, false, PDW, true>(L, RB, mode, s)
+                   : launch_rows_w<T, EY, NTH, B16, 1, false, PDW>(L, RB, mode, s);
+    return sor ? launch_rows_w<T, EY, NTH, B16, 2, false, PDW, true>(L, RB, mode, s)
+               : launch_rows_w<T, EY, NTH, B16, 2, false, PDW>(L, RB, mode, s);
 }
 
 // The rho form of the FP32 finest-level sweep (k_gsrb_rows_rho): 6 rows of
 // 768 threads at nx = 256, 12 rows of 768 at nx = 128; phi and rho planes
 // in LDS (8 planes), two blocks per CU.
-template <int EY, int NTH, int W>
+template <int EY, int NTH, int W, bool SOR>
 static bool launch_rows_rho_w(const Lev<float> &L, const RowBufs &RB, const RhoBufs &HB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(float) * 8 * (size_t)(EY + 2) * (L.nx + 4);
@@ -3602,16 +3614,16 @@ static bool launch_rows_rho_w(const Lev<float> &L, const RowBufs &RB, const RhoB
     static int slots_cache[2] = {0, 0};   // [mode] resident blocks per device
     int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0];
     if (slots == 0) {
-        slots = (mode == SWEEP_ZERO) ? resident_slots((const void *)k_gsrb_rows_rho<1, EY, NTH, W>, threads, lds)
-                                     : resident_slots((const void *)k_gsrb_rows_rho<0, EY, NTH, W>, threads, lds);
+        slots = (mode == SWEEP_ZERO) ? resident_slots((const void *)k_gsrb_rows_rho<1, EY, NTH, W, SOR>, threads, lds)
+                                     : resident_slots((const void *)k_gsrb_rows_rho<0, EY, NTH, W, SOR>, threads, lds);
         if (slots == 0) return false;
     }
     const RowSweepGrid G = row_grid(L.ny, L.nz, EY, W, slots);
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows_rho<1, EY, NTH, W>), g, b, lds, s, L, RB, HB, G);
+        hipLaunchKernelGGL((k_gsrb_rows_rho<1, EY, NTH, W, SOR>), g, b, lds, s, L, RB, HB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows_rho<0, EY, NTH, W>), g, b, lds, s, L, RB, HB, G);
+        hipLaunchKernelGGL((k_gsrb_rows_rho<0, EY, NTH, W, SOR>), g, b, lds, s, L, RB, HB, G);
     return true;
 }
 
@@ -3643,8 +3655,12 @@ bool launch_gsrb_rho(const Lev<float> &L, const RhoLev &R, const float *pin, flo
     RowBufs RB;
     RhoBufs HB;
     if (!rho_level_ok(3, L) || !row_bufs<float>(L, pin, pout, nullptr, RB) || !rho_bufs(L, R, RB, HB)) return false;
-    if (L.nx == 256) return launch_rows_rho_w<6, 768, 2>(L, RB, HB, mode, s);
-    return launch_rows_rho_w<12, 768, 1>(L, RB, HB, mode, s);
+    const bool sor = L.omega != 1.0f;
+    if (L.nx == 256)
+        return sor ? launch_rows_rho_w<6, 768, 2, true>(L, RB, HB, mode, s)
+                   : launch_rows_rho_w<6, 768, 2, false>(L, RB, HB, mode, s);
+    return sor ? launch_rows_rho_w<12, 768, 1, true>(L, RB, HB, mode, s)
+               : launch_rows_rho_w<12, 768, 1, false>(L, RB, HB, mode, s);
 }
 
 bool compact_level_ok(int dim, const Lev<float> &L)

@@ -91,8 +91,8 @@ module macproject_module
   private
   public :: macproject, macproject_release, mac_plan_ensure, fail
   public :: use_alt_energy_fix, mac_precision, mg_verbose, eps_mac, eps_mac_max, &
-            mac_level_factor, mg_cycle_type, mg_bottom_nu, mg_bottom_solver, last_stats, c_mac_stats, &
-            c_mac_opts, mac_default_opts
+            mac_level_factor, mg_cycle_type, mg_bottom_nu, mg_bottom_solver, mac_omega, last_stats, &
+            c_mac_stats, c_mac_opts, mac_default_opts
 
   ! runtime knobs (Source/_parameters:370-392,552-554, Source/mg_eps.f90:22-28)
   logical, save :: use_alt_energy_fix = .true.
@@ -103,6 +103,8 @@ module macproject_module
   integer, save :: mg_bottom_nu = 10            ! sweeps around the bottom solve
   integer, save :: mg_bottom_solver = -1        ! 0 smoothing only, 1 BiCGStab, 2 CG; -1 and 4
                                                 ! (agglomerating, always on here): the library's CG
+  real(dp_t), save :: mac_omega = 0.d0          ! over-relaxation of the red-black sweeps (no
+                                                ! reference counterpart); 0: 1, Gauss-Seidel
   real(dp_t), save :: eps_mac = 1.d-10
   real(dp_t), save :: eps_mac_max = 1.d-8
   real(dp_t), save :: mac_level_factor = 10.d0
@@ -122,6 +124,7 @@ module macproject_module
      real(c_double) :: bottom_eps
      integer(c_int) :: bottom_max_iter, max_levels, use_alt_energy_fix, precision, &
                        inner_cycles, device_ptrs, verbose, cycle_type, bottom_solver
+     real(c_double) :: omega
   end type c_mac_opts
 
   type, bind(C) :: c_mac_stats
@@ -216,6 +219,7 @@ contains
     o%cycle_type = mg_cycle_type
     o%nub = mg_bottom_nu
     if (mg_bottom_solver /= -1 .and. mg_bottom_solver /= 4) o%bottom_solver = mg_bottom_solver
+    o%omega = mac_omega
     o%device_ptrs = 0
 
     call mac_plan_ensure(mla, dx, the_bc_tower, ng, o)
@@ -345,7 +349,7 @@ module mac_multigrid_module
   use mac_types_module
   use macproject_module, only: mac_plan_ensure, fail, c_mac_opts, c_mac_stats, mac_default_opts, &
                                mac_precision, mg_verbose, mg_cycle_type, mg_bottom_nu, &
-                               mg_bottom_solver, last_stats
+                               mg_bottom_solver, mac_omega, last_stats
   implicit none
 
   private
@@ -419,6 +423,7 @@ contains
     o%cycle_type = mg_cycle_type
     o%nub = mg_bottom_nu
     if (mg_bottom_solver /= -1 .and. mg_bottom_solver /= 4) o%bottom_solver = mg_bottom_solver
+    o%omega = mac_omega
     o%device_ptrs = 0
     ! a cached projection plan serves as it is; a new one gets the projection's usual widths
     if (c_associated(mla%plan)) then

@@ -69,7 +69,8 @@ class COpts(C.Structure):
                 ("bottom_eps", C.c_double), ("bottom_max_iter", C.c_int),
                 ("max_levels", C.c_int), ("use_alt_energy_fix", C.c_int),
                 ("precision", C.c_int), ("inner_cycles", C.c_int), ("device_ptrs", C.c_int),
-                ("verbose", C.c_int), ("cycle_type", C.c_int), ("bottom_solver", C.c_int)]
+                ("verbose", C.c_int), ("cycle_type", C.c_int), ("bottom_solver", C.c_int),
+                ("omega", C.c_double)]
 
 
 class CSphr(C.Structure):

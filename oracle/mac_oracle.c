@@ -342,7 +342,15 @@ double orc_residual(const orc_grid *g, const orc_level *lev, const double *phi,
 
 /* One red-black Gauss-Seidel sweep (MG_SMOOTHER_GS_RB, mg.tex:41-44,201):
  * even-parity cells (i+j+k even) first, then odd.                         */
-void orc_gsrb(const orc_grid *g, orc_level *lev)
+void orc_gsrb(const orc_grid *g, orc_level *lev) { orc_gsrb_omega(g, lev, 1.0); }
+
+/* One red-black sweep over-relaxed by omega: each cell moves to
+ * old + omega (gs - old), as fma(omega, gs - old, old); omega = 1 stores the
+ * Gauss-Seidel value itself.  Over-relaxed red-black smoothing in multigrid:
+ * I. Yavneh, "On red-black SOR smoothing in multigrid", SIAM J. Sci. Comput.
+ * 17 (1996) 180-192.  No reference counterpart (FBoxLib's GSRB is omega = 1):
+ * the smoother of the library's FP32 cycles.                               */
+void orc_gsrb_omega(const orc_grid *g, orc_level *lev, double omega)
 {
     orc_lv L;
     lv_from_level(lev, g->dim, &L);
@@ -352,9 +360,20 @@ void orc_gsrb(const orc_grid *g, orc_level *lev)
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 0; k < L.n[2]; ++k)
             for (int j = 0; j < L.n[1]; ++j)
-                for (int i = (j + k + color) & 1; i < L.n[0]; i += 2)
-                    phi[cidx(L.n, i, j, k)] = cell_gs(&L, bc, g->dim, phi, lev->rhs, i, j, k);
+                for (int i = (j + k + color) & 1; i < L.n[0]; i += 2) {
+                    const long c = cidx(L.n, i, j, k);
+                    const double v = cell_gs(&L, bc, g->dim, phi, lev->rhs, i, j, k);
+                    phi[c] = (omega == 1.0) ? v : fma(omega, v - phi[c], phi[c]);
+                }
     }
+}
+
+/* the over-relaxation of level L's sweeps in a cycle (ORC_SOR_MAX_CELLS) */
+static double level_omega(const orc_mg *mg, const orc_level *L)
+{
+    const long cells = (long)L->n[0] * L->n[1] * L->n[2];
+    const long thr = mg->g.dim == 3 ? ORC_SOR_MAX_CELLS : ORC_SOR_MAX_CELLS_2D;
+    return cells > thr ? mg->opts.omega : 1.0;
 }
 
 /* ------------------------------------------------ MG transfer operators */
@@ -758,7 +777,8 @@ static void cycle(orc_mg *mg, int l, int type)
         bottom_solve(mg, L);
         return;
     }
-    for (int s = 0; s < mg->opts.nu1; ++s) orc_gsrb(g, L);
+    const double w = level_omega(mg, L);
+    for (int s = 0; s < mg->opts.nu1; ++s) orc_gsrb_omega(g, L, w);
     orc_residual(g, L, L->phi, L->rhs, L->res);
     orc_level *C = &mg->lev[l + 1];
     restrict_defect(g->dim, L, C);
@@ -767,7 +787,7 @@ static void cycle(orc_mg *mg, int l, int type)
     if (type == ORC_CYCLE_W) cycle(mg, l + 1, ORC_CYCLE_W);
     else if (type == ORC_CYCLE_F) cycle(mg, l + 1, ORC_CYCLE_V);
     prolong_add(g, C, L);
-    for (int s = 0; s < mg->opts.nu2; ++s) orc_gsrb(g, L);
+    for (int s = 0; s < mg->opts.nu2; ++s) orc_gsrb_omega(g, L, w);
 }
 
 static int cycle_shape(const orc_opts *o)
@@ -1028,6 +1048,7 @@ void orc_default_opts(orc_opts *o)
     o->bottom_solver = ORC_BOTTOM_CG;
     o->max_levels = 0;
     o->use_alt_energy_fix = 1;  /* _parameters:554 */
+    o->omega = 1.0;             /* FBoxLib's GSRB */
 }
 
 int orc_num_threads(void)
@@ -1054,6 +1075,16 @@ void orc_gsrb_folded_f32(int dim, const int *n, const int *per, const float *ih2
                          const float *bx, const float *by, const float *bz,
                          const float *bxh, const float *byh, const float *bzh,
                          float *phi, const float *rhs)
+{
+    orc_gsrb_folded_f32_omega(dim, n, per, ih2, bx, by, bz, bxh, byh, bzh, phi, rhs, 1.0f);
+}
+
+/* the same sweep over-relaxed by omega, as orc_gsrb_omega: a cell moves to
+ * fmaf(omega, gs - old, old); omega = 1 stores gs itself                    */
+void orc_gsrb_folded_f32_omega(int dim, const int *n, const int *per, const float *ih2,
+                               const float *bx, const float *by, const float *bz,
+                               const float *bxh, const float *byh, const float *bzh,
+                               float *phi, const float *rhs, float omega)
 {
     const int nx = n[0], ny = n[1], nz = (dim == 3) ? n[2] : 1;
     const long sy = nx, sz = (long)nx * ny;
@@ -1087,7 +1118,8 @@ void orc_gsrb_folded_f32(int dim, const int *n, const int *per, const float *ih2
                         num = fmaf(ih2[0], tx, fmaf(ih2[1], ty, rhs[c]));
                         den = fmaf(ih2[0], sx, ih2[1] * sy2);
                     }
-                    phi[c] = num / den;
+                    const float v = num / den;
+                    phi[c] = (omega == 1.0f) ? v : fmaf(omega, v - phi[c], phi[c]);
                 }
     }
 }

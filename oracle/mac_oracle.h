@@ -47,7 +47,17 @@ typedef struct {
     int use_alt_energy_fix;
     int cycle_type;   /* ORC_CYCLE_F/W/V (mg_cycle_type); 0 = V */
     int bottom_solver; /* ORC_BOTTOM_SMOOTH/BICGSTAB/CG (mg_bottom_solver) */
+    double omega;     /* over-relaxation of the red-black sweeps on the levels
+                         above ORC_SOR_MAX_CELLS cells; 1 = Gauss-Seidel (the
+                         reference's smoother).  No reference counterpart:
+                         the smoother of the library's FP32 cycles.          */
 } orc_opts;
+
+/* levels of at most this many cells (3D; 2D: ORC_SOR_MAX_CELLS_2D) are swept
+ * without over-relaxation: the library runs them in its single-workgroup
+ * coarse cycle, which keeps plain Gauss-Seidel                               */
+#define ORC_SOR_MAX_CELLS 512
+#define ORC_SOR_MAX_CELLS_2D 1024
 
 typedef struct {
     int iters, converged, nlev;
@@ -92,10 +102,15 @@ void orc_apply(const orc_grid *g, const orc_level *lev, const double *phi,
 double orc_residual(const orc_grid *g, const orc_level *lev, const double *phi,
                     const double *rhs, double *res);
 void orc_gsrb(const orc_grid *g, orc_level *lev);
+void orc_gsrb_omega(const orc_grid *g, orc_level *lev, double omega);
 void orc_gsrb_folded_f32(int dim, const int *n, const int *per, const float *ih2,
                          const float *bx, const float *by, const float *bz,
                          const float *bxh, const float *byh, const float *bzh,
                          float *phi, const float *rhs);
+void orc_gsrb_folded_f32_omega(int dim, const int *n, const int *per, const float *ih2,
+                               const float *bx, const float *by, const float *bz,
+                               const float *bxh, const float *byh, const float *bzh,
+                               float *phi, const float *rhs, float omega);
 
 void orc_level_restrict(const orc_grid *g, const orc_level *f, orc_level *c);
 void orc_level_prolong(const orc_grid *g, const orc_level *c, orc_level *f);

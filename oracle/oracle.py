@@ -39,7 +39,7 @@ class Opts(C.Structure):
                 ("nub", C.c_int), ("bottom_eps", C.c_double),
                 ("bottom_maxit", C.c_int), ("max_levels", C.c_int),
                 ("use_alt_energy_fix", C.c_int), ("cycle_type", C.c_int),
-                ("bottom_solver", C.c_int)]
+                ("bottom_solver", C.c_int), ("omega", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -85,9 +85,11 @@ def lib():
         L.orc_residual.restype = C.c_double
         L.orc_residual.argtypes = [C.POINTER(Grid), C.POINTER(Level), dp, dp, dp]
         L.orc_gsrb.argtypes = [C.POINTER(Grid), C.POINTER(Level)]
+        L.orc_gsrb_omega.argtypes = [C.POINTER(Grid), C.POINTER(Level), C.c_double]
         fp = C.POINTER(C.c_float)
         ip = C.POINTER(C.c_int)
         L.orc_gsrb_folded_f32.argtypes = [C.c_int, ip, ip, fp, fp, fp, fp, fp, fp, fp, fp, fp]
+        L.orc_gsrb_folded_f32_omega.argtypes = [C.c_int, ip, ip, fp, fp, fp, fp, fp, fp, fp, fp, fp, C.c_float]
         L.orc_mg_nlevels.restype = C.c_int
         L.orc_mg_nlevels.argtypes = [C.POINTER(Grid), C.c_int]
         L.orc_mg_create.restype = C.c_void_p
@@ -243,14 +245,18 @@ def residual(g, b, phi, rhs, alpha=None):
     return res, m
 
 
-def gsrb(g, b, phi, rhs, sweeps=1, alpha=None):
+def gsrb(g, b, phi, rhs, sweeps=1, alpha=None, omega=None):
+    """Red-black sweeps (over-relaxed by ``omega`` if given)."""
     lv = LevelView(g, b, np.array(phi, dtype=np.float64, copy=True), rhs, alpha=alpha)
     for _ in range(sweeps):
-        lib().orc_gsrb(C.byref(g), C.byref(lv.L))
+        if omega is None:
+            lib().orc_gsrb(C.byref(g), C.byref(lv.L))
+        else:
+            lib().orc_gsrb_omega(C.byref(g), C.byref(lv.L), float(omega))
     return lv.phi
 
 
-def gsrb_folded_f32(dim, n, per, ih2, w, phi, rhs, sweeps=1):
+def gsrb_folded_f32(dim, n, per, ih2, w, phi, rhs, sweeps=1, omega=None):
     """Red-black sweeps in FP32 on folded weights ``w`` (dict of bx, by, bz,
     bxh, byh, bzh float32 arrays as the GPU library stores a level); returns
     the new phi (float32, shape of ``phi``)."""
@@ -265,9 +271,12 @@ def gsrb_folded_f32(dim, n, per, ih2, w, phi, rhs, sweeps=1):
     out = np.array(phi, dtype=np.float32, copy=True, order="C")
     r = f32(rhs)
     for _ in range(sweeps):
-        lib().orc_gsrb_folded_f32(dim, nn, pp, hh, fptr(arrs["bx"]), fptr(arrs["by"]), fptr(arrs["bz"]),
-                                  fptr(arrs["bxh"]), fptr(arrs["byh"]), fptr(arrs["bzh"]), fptr(out),
-                                  fptr(r))
+        args = (dim, nn, pp, hh, fptr(arrs["bx"]), fptr(arrs["by"]), fptr(arrs["bz"]),
+                fptr(arrs["bxh"]), fptr(arrs["byh"]), fptr(arrs["bzh"]), fptr(out), fptr(r))
+        if omega is None:
+            lib().orc_gsrb_folded_f32(*args)
+        else:
+            lib().orc_gsrb_folded_f32_omega(*args, float(omega))
     return out
 
 

@@ -46,37 +46,56 @@ def push_level(plan, level, bc, n, b, phi, rhs, dim):
     plan.debug_set(level, "rhs", rhs.ravel())
 
 
+# over-relaxation of the sweeps (mac_opts.omega): 0 = the default,
+# Gauss-Seidel; 1.25 = over-relaxed red-black (the oracle's orc_gsrb_omega)
+OMEGAS = [0.0, 1.25]
+
+
+def ref_omega(n, w):
+    """The over-relaxation a plan applies to level 0's sweeps: w where level
+    0 lies above the single-workgroup coarse cycle (more than 512 cells in
+    3D, 1024 in 2D, and a coarser level exists: every extent even and at
+    least 4), else none (Gauss-Seidel, also for w = 0, the default)."""
+    coarser = all(m % 2 == 0 and m // 2 >= 2 for m in n)
+    above = int(np.prod(n)) > (512 if len(n) == 3 else 1024) and coarser
+    return w if (w != 0.0 and above) else None
+
+
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("n", SIZES_3D)
 @pytest.mark.parametrize("bc", BCS_3D)
-def test_gsrb_sweep_bitwise(gpu, oracle_lib, n, bc):
+def test_gsrb_sweep_bitwise(gpu, oracle_lib, n, bc, omega):
     O = oracle_lib
     rng = np.random.default_rng(7)
     g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
     b, phi, rhs = rand_level(g, n, 3, rng, O)
-    plan = make_plan(n, bc)
+    plan = make_plan(n, bc, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 3)
     plan.debug_op(0, "gsrb", 2)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
-    ref = O.gsrb(g, b, phi, rhs, sweeps=2)
+    ref = O.gsrb(g, b, phi, rhs, sweeps=2, omega=ref_omega(n, omega))
     assert np.array_equal(got, ref), np.abs(got - ref).max()
+    if ref_omega(n, omega) is not None:
+        assert not np.array_equal(got, O.gsrb(g, b, phi, rhs, sweeps=2))
 
 
+@pytest.mark.parametrize("omega", [0.0, 1.2])
 @pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL]], [[OUT, WALL], [P, P]]])
-def test_gsrb_sweep_bitwise_2d(gpu, oracle_lib, bc):
+def test_gsrb_sweep_bitwise_2d(gpu, oracle_lib, bc, omega):
     O = oracle_lib
     n = (48, 32)
     rng = np.random.default_rng(3)
     g = O.make_grid(n, [1.0 / n[0]] * 2, bc)
     b, phi, rhs = rand_level(g, n, 2, rng, O)
-    plan = make_plan(n, bc)
+    plan = make_plan(n, bc, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 2)
     plan.debug_op(0, "gsrb", 1)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
-    ref = O.gsrb(g, b, phi, rhs, sweeps=1)
+    ref = O.gsrb(g, b, phi, rhs, sweeps=1, omega=ref_omega(n, omega))
     assert np.array_equal(got, ref)
 
 
-def fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=False):
+def fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=False, omega=None):
     """The oracle's FP32 sweep on the level rounded to FP32 as the library
     rounds it (weights, rhs, phi and 1/h^2 each cast from FP64); bf16: the
     weights and rhs further rounded to bfloat16 (compact storage)."""
@@ -88,34 +107,36 @@ def fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=False):
         r32 = bf16_round(r32).reshape(rhs.shape)
     ih2 = [np.float32(1.0 / (1.0 / n[0]) ** 2)] * dim
     per = [1 if bc[d][0] == P else 0 for d in range(dim)]
-    out = O.gsrb_folded_f32(dim, n, per, ih2, w, phi.astype(np.float32), r32, sweeps=sweeps)
+    out = O.gsrb_folded_f32(dim, n, per, ih2, w, phi.astype(np.float32), r32, sweeps=sweeps, omega=omega)
     return out.astype(np.float64)
 
 
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("sweeps", [1, 2])
 @pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (64, 48, 40), (128, 30, 6), (128, 12, 9)])
 @pytest.mark.parametrize("bc", BCS_3D)
-def test_gsrb32_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
+def test_gsrb32_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps, omega):
     """The FP32 smoother of the mixed path (full-row kernel where nx is a
     multiple of 128, the one-column kernel elsewhere) against the oracle's
-    FP32 sweep, bit for bit."""
+    FP32 sweep, bit for bit, plain and over-relaxed."""
     O = oracle_lib
     rng = np.random.default_rng(21)
     g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
     b, phi, rhs = rand_level(g, n, 3, rng, O)
-    plan = make_plan(n, bc)
+    plan = make_plan(n, bc, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 3)
     plan.debug_op(0, "gsrb32", sweeps)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
-    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps)
+    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps, omega=ref_omega(n, omega))
     assert np.array_equal(got, ref), np.abs(got - ref).max()
 
 
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("sweeps", [1, 2])
 @pytest.mark.parametrize("n", [(128, 12, 8), (256, 20, 10), (256, 14, 8), (256, 30, 9), (128, 30, 6),
                                (128, 12, 9), (64, 48, 40)])
 @pytest.mark.parametrize("bc", BCS_3D)
-def test_gsrb32_bf16_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
+def test_gsrb32_bf16_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps, omega):
     """The compact-storage smoother (bfloat16 weights and rhs, FP32 iterate
     and arithmetic) against the oracle's FP32 sweep on the same rounded
     values, bit for bit.  Levels without the full-row kernel (nx other than
@@ -125,30 +146,32 @@ def test_gsrb32_bf16_sweep_bitwise(gpu, oracle_lib, n, bc, sweeps):
     rng = np.random.default_rng(29)
     g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
     b, phi, rhs = rand_level(g, n, 3, rng, O)
-    plan = make_plan(n, bc, precision=mac.PREC_MIXED_BF16)
+    plan = make_plan(n, bc, precision=mac.PREC_MIXED_BF16, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 3)
     plan.debug_op(0, "gsrb32", sweeps)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
     compact = n[0] in (128, 256) and n[2] >= 8
-    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=compact)
+    w = ref_omega(n, omega)
+    ref = fp32_reference(O, n, bc, b, phi, rhs, sweeps, bf16=compact, omega=w)
     assert np.array_equal(got, ref), np.abs(got - ref).max()
     if compact:
         # the rounding is visible: the FP32-storage sweep differs
-        assert not np.array_equal(got, fp32_reference(O, n, bc, b, phi, rhs, sweeps))
+        assert not np.array_equal(got, fp32_reference(O, n, bc, b, phi, rhs, sweeps, omega=w))
 
 
+@pytest.mark.parametrize("omega", [0.0, 1.2])
 @pytest.mark.parametrize("bc", [[[P, P], [WALL, WALL]], [[OUT, WALL], [P, P]]])
-def test_gsrb32_sweep_bitwise_2d(gpu, oracle_lib, bc):
+def test_gsrb32_sweep_bitwise_2d(gpu, oracle_lib, bc, omega):
     O = oracle_lib
     n = (48, 32)
     rng = np.random.default_rng(23)
     g = O.make_grid(n, [1.0 / n[0]] * 2, bc)
     b, phi, rhs = rand_level(g, n, 2, rng, O)
-    plan = make_plan(n, bc)
+    plan = make_plan(n, bc, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 2)
     plan.debug_op(0, "gsrb32", 1)
     got = plan.debug_get(0, "phi").reshape(phi.shape)
-    ref = fp32_reference(O, n, bc, b, phi, rhs, 1)
+    ref = fp32_reference(O, n, bc, b, phi, rhs, 1, omega=ref_omega(n, omega))
     assert np.array_equal(got, ref)
 
 
@@ -215,16 +238,18 @@ def test_coarsen_bitwise(gpu, oracle_lib, bc):
         assert np.array_equal(plan.debug_get(1, k), v), k
 
 
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("bc", [BCS_3D[1], BCS_3D[2]])
-def test_vcycle_matches_oracle(gpu, oracle_lib, bc):
+def test_vcycle_matches_oracle(gpu, oracle_lib, bc, omega):
     """One full V-cycle (non-singular BCs): equal up to the bottom solver's
-    reduction order."""
+    reduction order; with over-relaxed sweeps on the levels above the coarse
+    cycle (64x32x32 .. 16x8x8), as the oracle's cycle applies them."""
     O = oracle_lib
     n = (64, 32, 32)
     rng = np.random.default_rng(9)
     g = O.make_grid(n, [1.0 / n[0]] * 3, bc)
     b, phi, rhs = rand_level(g, n, 3, rng, O)
-    plan = make_plan(n, bc, bottom_eps=1e-15, bottom_max_iter=200)
+    plan = make_plan(n, bc, bottom_eps=1e-15, bottom_max_iter=200, omega=omega)
     push_level(plan, 0, bc, n, b, phi, rhs, 3)
     nlev = 0
     while True:
@@ -237,7 +262,8 @@ def test_vcycle_matches_oracle(gpu, oracle_lib, bc):
     plan.debug_op(0, "vcycle")
     got = plan.debug_get(0, "phi").reshape(phi.shape)
     ref, _, st, _ = O.mg_solve(g, b, phi, rhs, O.default_opts(max_iter=1, eps_rel=1e-300,
-                                                                   bottom_eps=1e-15, bottom_maxit=200))
+                                                                   bottom_eps=1e-15, bottom_maxit=200,
+                                                                   omega=omega or 1.0))
     assert st.iters == 1
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err < 1e-12, err
@@ -328,10 +354,11 @@ def test_compact_level0_and_level1_weights_bitwise(gpu, cfg, n, box):
     plan.close()
 
 
+@pytest.mark.parametrize("omega", OMEGAS)
 @pytest.mark.parametrize("sweeps", [1, 2])
 @pytest.mark.parametrize("n,box", [((128, 64, 32), 32), ((256, 16, 24), 8)])
 @pytest.mark.parametrize("zbc", [[WALL, WALL], [OUT, WALL], [P, P]])
-def test_gsrb32_rho_sweep(gpu, oracle_lib, n, box, zbc, sweeps):
+def test_gsrb32_rho_sweep(gpu, oracle_lib, n, box, zbc, sweeps, omega):
     """The rho form of the FP32 finest-level sweep (face weights formed in the
     sweep from the FP32 density: 2 beta0^2 / (rho_L + rho_R) with the
     hardware reciprocal, 1 ulp) against the oracle's FP32 sweep on the same
@@ -352,7 +379,7 @@ def test_gsrb32_rho_sweep(gpu, oracle_lib, n, box, zbc, sweeps):
     from maestro_amd import boxes as BX
     boxes, umac, phi0, rho, _, _ = BX.split_workload(wl)
     lay = mac.Layout.uniform(n, box, wl["dx"], bc)
-    plan = mac.MacPlan(lay, mac.default_opts(precision=mac.PREC_MIXED, max_iter=0))
+    plan = mac.MacPlan(lay, mac.default_opts(precision=mac.PREC_MIXED, max_iter=0, omega=omega))
     plan.project(umac, phi0, rho, None, np.ascontiguousarray(wl["beta0"]),
                  np.ascontiguousarray(wl["beta0_edge"]), None, check=False)
     phi = rng.standard_normal((n[2], n[1], n[0]))
@@ -385,7 +412,8 @@ def test_gsrb32_rho_sweep(gpu, oracle_lib, n, box, zbc, sweeps):
          "byh": wy[:, 0, :].ravel(), "bzh": bzh.ravel()}
     ih2 = [f32(1.0 / wl["dx"][d] ** 2) for d in range(3)]
     per = [1, 1, 1 if zbc[0] == P else 0]
-    ref = O.gsrb_folded_f32(3, n, per, ih2, w, phi.astype(f32), rhs.astype(f32), sweeps=sweeps).astype(np.float64)
+    ref = O.gsrb_folded_f32(3, n, per, ih2, w, phi.astype(f32), rhs.astype(f32), sweeps=sweeps,
+                            omega=ref_omega(n, omega)).astype(np.float64)
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print(f"n={n} zbc={zbc} sweeps={sweeps}: rel max diff {err:.2e}")
     assert err <= 1e-5

@@ -133,6 +133,68 @@ def test_gsrb_fixed_point_and_black_residual(oracle_lib):
     assert np.abs(r[black]).max() <= 1e-12 * np.abs(r).max()
 
 
+@pytest.mark.parametrize("n,bc", [((8, 6, 4), [[OUT, WALL], [P, P], [WALL, OUT]]),
+                                  ((10, 8), [[P, P], [OUT, WALL]])])
+def test_over_relaxed_sweep_matrix_form(oracle_lib, n, bc):
+    """orc_gsrb_omega against the matrix form of red-black SOR built from the
+    oracle's own operator matrix A (probed column by column): with D the
+    diagonal and the cells split by colour (no red-red or black-black
+    coupling in the cross stencil), x_r' = (1 - w) x_r + w D_r^-1 (b_r -
+    A_rb x_b), then x_b' = (1 - w) x_b + w D_b^-1 (b_b - A_br x_r').  A
+    wrong colour order, a dropped (1 - w) term or w applied to one colour
+    only fails it; omega = 1 equals orc_gsrb bit for bit."""
+    O = oracle_lib
+    dim = len(n)
+    g = O.make_grid(n, tuple(1.0 / m for m in n), bc)
+    rng = np.random.default_rng(9)
+    b = [rng.uniform(0.5, 2, O.face_shape(g, d)) for d in range(dim)]
+    sh = O.cell_shape(g)
+    m = int(np.prod(sh))
+    A = np.empty((m, m))
+    for c in range(m):
+        e = np.zeros(m)
+        e[c] = 1.0
+        A[:, c] = np.asarray(O.apply(g, b, e.reshape(sh))).ravel()
+    rhs = rng.standard_normal(sh)
+    x0 = rng.standard_normal(sh)
+    idx = np.indices(sh)
+    red = (idx.sum(axis=0) & 1).ravel() == 0
+    for w in (1.0, 1.3, 0.8):
+        x = x0.ravel().copy()
+        bb = rhs.ravel()
+        d = np.diag(A)
+        for col in (red, ~red):
+            off = A[np.ix_(col, ~col)] @ x[~col]
+            x[col] = (1.0 - w) * x[col] + w * (bb[col] - off) / d[col]
+        got = O.gsrb(g, b, x0, rhs, omega=w)
+        assert np.abs(got.ravel() - x).max() <= 1e-12 * np.abs(x).max(), w
+    assert np.array_equal(O.gsrb(g, b, x0, rhs, omega=1.0), O.gsrb(g, b, x0, rhs))
+
+
+@pytest.mark.parametrize("cfg,n", [(2, (64, 64, 64)), (0, (128, 128))])
+def test_over_relaxed_cycles_converge_faster(oracle_lib, cfg, n):
+    """The V-cycle with over-relaxed sweeps above the coarse levels (omega
+    1.25 in 3D, 1.2 in 2D) reaches 1e-10 in fewer cycles than Gauss-Seidel on
+    the plane-parallel and 2D workloads, to the same answer; omega = 1 in the
+    options is the default cycle bit for bit."""
+    O = oracle_lib
+    wl = W.make_workload(cfg, n=n)
+    w = 1.25 if len(n) == 3 else 1.2
+    gs = oracle_project(O, wl, alt=1, eps=1e-10)
+    same = oracle_project(O, wl, alt=1, eps=1e-10, omega=1.0)
+    sor = oracle_project(O, wl, alt=1, eps=1e-10, omega=w)
+    assert np.array_equal(gs["phi"], same["phi"])
+    assert sor["stats"].converged
+    # measured: 8 -> 6 cycles at 64^3; 8 -> 7 at 128^2 (in 2D only the 64^2
+    # and 128^2 levels lie above the coarse cycle's 32^2)
+    gain = 2 if len(n) == 3 else 1
+    assert sor["stats"].iters <= gs["stats"].iters - gain, (sor["stats"].iters, gs["stats"].iters)
+    truth = oracle_project(O, wl, alt=1, eps=1e-13)
+    t = truth["phi"] - truth["phi"].mean()
+    err = lambda r: np.linalg.norm(r["phi"] - r["phi"].mean() - t) / np.linalg.norm(t)
+    assert err(sor) <= max(1e-8, 3.0 * err(gs)), (err(sor), err(gs))
+
+
 @pytest.mark.parametrize("n,bc", [
     ((16, 12, 10), [[P, P], [P, P], [WALL, WALL]]),
     ((12, 10, 8), [[OUT, OUT], [WALL, OUT], [P, P]]),
