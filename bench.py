@@ -12,14 +12,16 @@ step on its own fresh copy of the inputs (staged in HBM before timing).
 with FP32 iterate, coefficients and residual inside FP64 iterative
 refinement; the finest level's FP32 coefficients are formed in the sweep
 from the FP32 density, 2 beta0^2 / (rho_L + rho_R), the coarser levels'
-stored; Gauss-Seidel red-black sweeps, the library's default) in solves/s;
-the plain FP64 V-cycle (the reference algorithm) and the variant that
-stores the finest level's FP32-rounded weights and right side as bfloat16
-(same FP32 arithmetic, same FP64 tolerance) are measured the same way and
-reported beside it, and so are the mixed and FP64 paths with the sweeps
-over-relaxed by 1.25 (mac_opts.omega): fewer cycles to the same 1e-10
-relative residual, with phi further from the exact discrete solution
-(``accuracy`` reports each path against the FP64 path and the oracle).  ``roofline`` covers the dominant kernel of the
+stored; the FP32 red-black sweeps over-relaxed by 1.25, mac_opts.omega) in
+solves/s, against the plain FP64 V-cycle (Gauss-Seidel red-black, the
+reference algorithm).  Measured the same way and reported beside them: the
+mixed path with the library's default Gauss-Seidel sweeps, the FP64 V-cycle
+with the same over-relaxed sweeps (what precision alone gains), and the
+variant that stores the finest level's FP32-rounded weights and right side
+as bfloat16.  Every path stops at the same 1e-10 relative residual;
+``accuracy`` reports each against the FP64 path, the oracle at the same
+tolerance and the oracle's 1e-13 solve (the over-relaxed iterates are
+further from the exact discrete solution: 6.5e-9 in phi against 5.1e-10).  ``roofline`` covers the dominant kernel of the
 headline path: the finest-level fused red-black sweep (one full sweep per
 launch, the launches that read the old iterate), timed with HIP events on
 the library's own stream during the timed steps.  Its ``traffic`` figure is
@@ -41,6 +43,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# over-relaxation of the headline mixed path's FP32 sweeps (mac_opts.omega)
+OMEGA_HEADLINE = 1.25
 
 
 def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False, sor: bool = False) -> str:
@@ -215,8 +219,8 @@ def time_helmholtz(plan, n, box, torch, reps=3):
 def time_star_512(torch, steps=3, warmup=1):
     """The 512^3 full-star workload (512 boxes of 64^3, n = 1.5 polytrope with
     a density floor, coefficient contrast 1e6, outflow on every side, rel
-    residual 1e-10) in mixed precision, V- and F-cycles, plus the FP64
-    V-cycle.  Wall time per solve (device pointers, fresh input copies
+    residual 1e-10) in mixed precision, V- and F-cycles with Gauss-Seidel and
+    with over-relaxed sweeps, plus the FP64 V-cycle.  Wall time per solve (device pointers, fresh input copies
     staged in HBM before each timed solve), median of ``steps``."""
     from maestro_amd import mac
     from maestro_amd import workloads as W
@@ -236,9 +240,12 @@ def time_star_512(torch, steps=3, warmup=1):
                        "(contrast 1e6), outflow on every side, rel residual 1e-10"}
     du = [[a.clone() for a in b] for b in du0]
     dp = [a.clone() for a in dp0]
-    for name, prec, cyc in (("mixed_v", mac.PREC_MIXED, mac.CYCLE_V), ("mixed_f", mac.PREC_MIXED, mac.CYCLE_F),
-                            ("fp64_v", mac.PREC_FP64, mac.CYCLE_V)):
-        plan.set_opts(precision=prec, cycle_type=cyc, device_ptrs=1, max_iter=200)
+    for name, prec, cyc, w in (("mixed_v", mac.PREC_MIXED, mac.CYCLE_V, 0.0),
+                               ("mixed_f", mac.PREC_MIXED, mac.CYCLE_F, 0.0),
+                               ("mixed_v_over_relaxed", mac.PREC_MIXED, mac.CYCLE_V, OMEGA_HEADLINE),
+                               ("mixed_f_over_relaxed", mac.PREC_MIXED, mac.CYCLE_F, OMEGA_HEADLINE),
+                               ("fp64_v", mac.PREC_FP64, mac.CYCLE_V, 0.0)):
+        plan.set_opts(precision=prec, cycle_type=cyc, device_ptrs=1, max_iter=200, omega=w)
         ts, st = [], None
         for _ in range(warmup + steps):
             for b, b0 in zip(du, du0):
@@ -310,14 +317,14 @@ def time_adjoint_256(torch, steps=3, warmup=1):
             "forward_plus_backward_per_s": round(1.0 / (f + b), 3)}
 
 
-def time_host_pointers(plan, pb, wl, prec):
+def time_host_pointers(plan, pb, wl, prec, omega=0.0):
     """One projection with host (numpy) per-box arrays: the library copies
     them in and out around the solve (pageable host memory over PCIe).
     Returns the split of the second of two calls (the first allocates the
     device mirrors)."""
     from maestro_amd import mac
     boxes, umac, phi, rho, _, _ = pb
-    plan.set_opts(precision=prec, device_ptrs=0)
+    plan.set_opts(precision=prec, device_ptrs=0, omega=omega)
     b0 = np.ascontiguousarray(wl["beta0"])
     b0e = np.ascontiguousarray(wl["beta0_edge"])
     st = None
@@ -330,7 +337,7 @@ def time_host_pointers(plan, pb, wl, prec):
     nbytes_in = sum(a.nbytes for box in umac for a in box) + sum(a.nbytes for a in phi) + \
         sum(a.nbytes for a in rho)
     nbytes_out = sum(a.nbytes for box in umac for a in box) + sum(a.nbytes for a in phi)
-    plan.set_opts(device_ptrs=1)
+    plan.set_opts(device_ptrs=1, omega=0.0)
     return {"ms_total": round(1e3 * wall, 2), "solves_per_s": round(1.0 / wall, 2),
             "h2d_ms": round(st["t_h2d_ms"], 2), "solve_ms": round(st["t_setup_ms"] + st["t_solve_ms"] +
                                                               st["t_correct_ms"], 2),
@@ -354,7 +361,7 @@ def main():
                     help="skip timing make_macrhs / make_s0mac and the Helmholtz mac_solve on the bench layout")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip timing the 512^3 star (mixed V/F, FP64) and the 256^3 adjoint")
-    ap.add_argument("--headline", default="mixed", choices=("mixed", "mixed_bf16"),
+    ap.add_argument("--headline", default="mixed", choices=("mixed", "mixed_gauss_seidel", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=0,
                     help="FP32 V-cycles per refinement step of the mixed paths "
@@ -398,8 +405,9 @@ def main():
 
     results = {}
     # (name, precision, omega): 0 = the default, Gauss-Seidel red-black
-    paths = (("fp64", mac.PREC_FP64, 0.0), ("mixed", mac.PREC_MIXED, 0.0), ("mixed_bf16", mac.PREC_MIXED_BF16, 0.0),
-             ("mixed_over_relaxed", mac.PREC_MIXED, 1.25), ("fp64_over_relaxed", mac.PREC_FP64, 1.25))
+    paths = (("fp64", mac.PREC_FP64, 0.0), ("mixed", mac.PREC_MIXED, OMEGA_HEADLINE),
+             ("mixed_gauss_seidel", mac.PREC_MIXED, 0.0), ("mixed_bf16", mac.PREC_MIXED_BF16, 0.0),
+             ("fp64_over_relaxed", mac.PREC_FP64, OMEGA_HEADLINE))
     for name, prec, omega in paths:
         # refresh the staged inputs from the host copies (outside timing)
         boxes, umac, phi, _, _, _ = pb
@@ -425,9 +433,9 @@ def main():
         """Roofline of the finest-level sweep of one path (the launches that
         read the old iterate)."""
         s_, ws, b16, rho, cls, sor = {
-            "fp64": (8, 8, False, False, "gsrb64", False), "mixed": (4, 4, False, True, "gsrb32", False),
+            "fp64": (8, 8, False, False, "gsrb64", False), "mixed": (4, 4, False, True, "gsrb32", True),
+            "mixed_gauss_seidel": (4, 4, False, True, "gsrb32", False),
             "mixed_bf16": (4, 2, True, False, "gsrb32", False),
-            "mixed_over_relaxed": (4, 4, False, True, "gsrb32", True),
             "fp64_over_relaxed": (8, 8, False, False, "gsrb64", True)}[path]
         cnt_, ms_ = results[path]["prof"][cls]
         avg = ms_ / cnt_ if cnt_ else float("inf")   # --no-profile: no per-kernel timing
@@ -442,7 +450,8 @@ def main():
     producers = None if args.no_producers else time_producers(plan, n, args.box, torch)
     helmholtz = None if args.no_producers else time_helmholtz(plan, n, args.box, torch)
     host = None if args.no_producers else time_host_pointers(
-        plan, pb, wl, {"mixed": mac.PREC_MIXED, "mixed_bf16": mac.PREC_MIXED_BF16}[args.headline])
+        plan, pb, wl, *{"mixed": (mac.PREC_MIXED, OMEGA_HEADLINE), "mixed_gauss_seidel": (mac.PREC_MIXED, 0.0),
+                        "mixed_bf16": (mac.PREC_MIXED_BF16, 0.0)}[args.headline])
 
     c3 = None if args.no_configs else time_star_512(torch)
     c4 = None if args.no_configs else time_adjoint_256(torch)
@@ -474,7 +483,7 @@ def main():
                 "u_max": float(max(np.abs(fa[d] - fb[d]).max() / np.abs(fb[d]).max() for d in range(dim))),
                 "u_rel": float(max(np.linalg.norm(fa[d] - fb[d]) / np.linalg.norm(fb[d]) for d in range(dim))),
             }
-        for path in ("mixed", "mixed_bf16", "mixed_over_relaxed", "fp64_over_relaxed"):
+        for path in ("mixed", "mixed_gauss_seidel", "mixed_bf16", "fp64_over_relaxed"):
             fm, pm = gather(results[path])
             acc[f"{path}_vs_fp64_gpu"] = errs(fm, pm, f6, p6)
 
@@ -502,7 +511,7 @@ def main():
                                  beta0_edge=wl["beta0_edge"], opts=O.default_opts(eps_rel=1e-13))
             for which, r in (("oracle", ref), ("oracle_1e-13", exact)):
                 fo, po = r["faces"], r["phi"] - r["phi"].mean()
-                for path in ("fp64", "mixed", "mixed_bf16", "mixed_over_relaxed", "fp64_over_relaxed"):
+                for path in ("fp64", "mixed", "mixed_gauss_seidel", "mixed_bf16", "fp64_over_relaxed"):
                     fa, pa = gather(results[path])
                     pa0 = pa - pa.mean()
                     acc[f"{path}_vs_{which}"] = {
@@ -513,7 +522,10 @@ def main():
                     }
         print(f"# cpu oracle: {tc:.2f} s/solve on {O.num_threads()} threads", file=sys.stderr)
 
-    storage = {"mixed": "FP32 V-cycle inside FP64 IR, finest-level FP32 coefficients formed from the FP32 density",
+    storage = {"mixed": "FP32 V-cycle with red-black sweeps over-relaxed by 1.25 (mac_opts.omega) inside FP64 IR, "
+                        "finest-level FP32 coefficients formed from the FP32 density; against the plain FP64 V-cycle",
+               "mixed_gauss_seidel": "FP32 V-cycle (Gauss-Seidel red-black, the library's default) inside FP64 IR, "
+                                     "finest-level FP32 coefficients formed from the FP32 density",
                "mixed_bf16": "FP32 V-cycle inside FP64 IR, finest-level weights and rhs stored as bfloat16"}
     line = {
         "metric": "MAC solves/s at 256^3 to 1e-10 resid: FP32-MG+FP64 IR vs FP64; HBM GB/s",
@@ -537,18 +549,16 @@ def main():
         "solves_per_s": {p: round(args.steps / results[p]["seconds"], 3) for p in results},
         "mixed_over_fp64": round(fp64["seconds"] / mixed["seconds"], 3),
         "compact_bf16_over_fp64": round(fp64["seconds"] / results["mixed_bf16"]["seconds"], 3),
-        # the sweeps over-relaxed by 1.25 (mac_opts.omega): against the plain
-        # FP64 V-cycle, and against the FP64 V-cycle with the same sweeps
-        "over_relaxed": {
-            "mixed_over_fp64": round(fp64["seconds"] / results["mixed_over_relaxed"]["seconds"], 3),
-            "mixed_over_fp64_over_relaxed": round(results["fp64_over_relaxed"]["seconds"] /
-                                                  results["mixed_over_relaxed"]["seconds"], 3)},
+        # what precision alone gains: mixed against FP64 with the same sweeps
+        "mixed_over_fp64_same_smoother": {
+            "over_relaxed": round(results["fp64_over_relaxed"]["seconds"] / results["mixed"]["seconds"], 3),
+            "gauss_seidel": round(fp64["seconds"] / results["mixed_gauss_seidel"]["seconds"], 3)},
         "iters": {"fp64_vcycles": fp64["stats"][-1]["iters"],
                   "fp64_over_relaxed_vcycles": results["fp64_over_relaxed"]["stats"][-1]["iters"],
                   **{f"{p}_ir_steps": results[p]["stats"][-1]["iters"]
-                     for p in ("mixed", "mixed_bf16", "mixed_over_relaxed")},
+                     for p in ("mixed", "mixed_gauss_seidel", "mixed_bf16")},
                   **{f"{p}_fp32_vcycles": results[p]["stats"][-1]["inner_cycles"]
-                     for p in ("mixed", "mixed_bf16", "mixed_over_relaxed")}},
+                     for p in ("mixed", "mixed_gauss_seidel", "mixed_bf16")}},
         "roofline": sweep_roofline(head),
         "roofline_other": {p: sweep_roofline(p) for p in results if p != head},
         "kernel_ms_per_solve": {
