@@ -122,13 +122,18 @@ def test_plane_parallel_256_fp64_and_mixed(gpu):
     """256^3 in 64 boxes of 64^3, isothermal beta0 (contrast 1e4), x/y
     periodic, z walls: every precision converges, satisfies the projection
     identity, agrees with the others and is within 1e-8 of the oracle's
-    1e-13 solve of the same inputs (phi mean-removed, and U_p)."""
+    1e-13 solve of the same inputs (phi mean-removed, and U_p); so does the
+    benchmark's headline, the mixed path with its FP32 sweeps over-relaxed by
+    1.25 (measured: phi 6.5e-9, U 8.8e-11 from the 1e-13 solve)."""
     torch = gpu
     wl = W.make_workload(2, n=(256,) * 3, box=64)
     truth = oracle_truth(wl)
     res = {}
-    for prec in (mac.PREC_FP64, mac.PREC_MIXED, mac.PREC_MIXED_BF16):
-        faces, ph, st = project_device(torch, wl, prec)
+    for prec in (mac.PREC_FP64, mac.PREC_MIXED, mac.PREC_MIXED_BF16, "mixed_over_relaxed"):
+        if prec == "mixed_over_relaxed":
+            faces, ph, st = project_device(torch, wl, mac.PREC_MIXED, omega=1.25)
+        else:
+            faces, ph, st = project_device(torch, wl, prec)
         print(f"  prec {prec}: iterations {st['iters']}")
         check_projection(torch, wl, faces, st)
         # periodic duplicates equal, wall-normal faces zero
@@ -140,7 +145,7 @@ def test_plane_parallel_256_fp64_and_mixed(gpu):
         assert e_phi <= 1e-8 and e_u <= 1e-8
         res[prec] = (faces, ph - ph.mean())
     f64, p64 = res[mac.PREC_FP64]
-    for prec in (mac.PREC_MIXED, mac.PREC_MIXED_BF16):
+    for prec in (mac.PREC_MIXED, mac.PREC_MIXED_BF16, "mixed_over_relaxed"):
         fmx, pmx = res[prec]
         e_phi = float((pmx - p64).norm() / p64.norm())
         e_u = max(float((fmx[d] - f64[d]).norm() / f64[d].norm()) for d in range(3))
