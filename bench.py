@@ -266,6 +266,73 @@ def time_star_512(torch, steps=3, warmup=1):
     return out
 
 
+def time_slab_share(torch, steps=5, warmup=2):
+    """One GPU's share of the 256^3 plane-parallel workload split in z over 8
+    GPUs: the bottom 256 x 256 x 32 cells of the same inputs (64 boxes of
+    32^3), solved on its own (x/y periodic, z walls) to 1e-10.  It
+    times the kernels at the slab's real shape, FP64 and the headline mixed
+    path, per solve and per finest-level sweep; the halo exchange and the
+    all-reduces a sharded solve adds are not run (one GPU)."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    n, box, nzs = (256, 256, 32), 32, 32
+    # the bottom slab of the 256^3 inputs: cubic cells of 1/256, z walls in
+    # place of the slab's halo interfaces
+    full = W.make_workload(2, n=(256,) * 3, box=64)
+    wl = dict(full)
+    wl["n"], wl["box"] = n, box
+    wl["faces"] = [np.ascontiguousarray(full["faces"][0][:nzs]), np.ascontiguousarray(full["faces"][1][:nzs]),
+                   np.ascontiguousarray(full["faces"][2][:nzs + 1])]
+    wl["faces"][2][0] = 0.0
+    wl["faces"][2][-1] = 0.0
+    wl["rho"] = W._fill_rho_ghosts(full["rho"][1:nzs + 1, 1:-1, 1:-1], full["bc"], 3)
+    wl["phi"] = np.zeros((nzs, n[1], n[0]))
+    wl["beta0"] = np.ascontiguousarray(full["beta0"][:nzs])
+    wl["beta0_edge"] = np.ascontiguousarray(full["beta0_edge"][:nzs + 1])
+    del full
+    boxes_, umac, phi, rho, _, _ = BX.split_workload(wl)
+    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    du0 = [[cv(a) for a in b] for b in umac]
+    dp0 = [cv(a) for a in phi]
+    dr = [cv(a) for a in rho]
+    b0, b0e = cv(wl["beta0"]), cv(wl["beta0_edge"])
+    lay = mac.Layout(dim=3, boxes=boxes_, domain_lo=(0, 0, 0), domain_hi=tuple(m - 1 for m in n),
+                     dx=tuple(wl["dx"]), phys_bc=wl["bc"])
+    plan = mac.MacPlan(lay, mac.default_opts(device_ptrs=1))
+    out = {"workload": "one GPU's z-slab of the 256^3 plane-parallel workload over 8 GPUs: 256 x 256 x 32 "
+                       "cells of 1/256 (64 boxes of 32^3, the bottom 32 planes of the 256^3 inputs), x/y "
+                       "periodic, z walls in place of the halo interfaces, rel residual 1e-10"}
+    du = [[a.clone() for a in b] for b in du0]
+    dp = [a.clone() for a in dp0]
+    for name, prec, w in (("fp64", mac.PREC_FP64, 0.0), ("mixed", mac.PREC_MIXED, OMEGA_HEADLINE)):
+        plan.set_opts(precision=prec, device_ptrs=1, omega=w)
+        ts, st = [], None
+        for k in range(warmup + steps):
+            for b, b0_ in zip(du, du0):
+                for a, a0 in zip(b, b0_):
+                    a.copy_(a0)
+            for a, a0 in zip(dp, dp0):
+                a.copy_(a0)
+            if k == warmup + steps - 1:
+                plan.profile(True)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = plan.project(du, dp, dr, None, b0, b0e, None)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        prof = plan.profile_read()
+        plan.profile(False)
+        cls = "gsrb64" if prec == mac.PREC_FP64 else "gsrb32"
+        cnt, ms = prof[cls]
+        tm = float(np.median(ts[warmup:]))
+        out[name] = {"ms": round(1e3 * tm, 3), "iters": st["iters"], "fp32_cycles": st["inner_cycles"],
+                     "finest_sweep_us": round(1e3 * ms / cnt, 2) if cnt else None}
+        print(f"# slab 256x256x32 {name}: {1e3 * tm:.2f} ms, {st['iters']} iterations", file=sys.stderr, flush=True)
+    plan.close()
+    return out
+
+
 def time_adjoint_256(torch, steps=3, warmup=1):
     """The 256^3 differentiable workload: C1's inputs, L = sum(w phi) +
     1/2 ||U_p||^2 with w ~ N(0, 1), mixed-precision forward (the projection
@@ -455,6 +522,7 @@ def main():
 
     c3 = None if args.no_configs else time_star_512(torch)
     c4 = None if args.no_configs else time_adjoint_256(torch)
+    slab = None if args.no_configs else time_slab_share(torch)
 
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
@@ -570,6 +638,7 @@ def main():
         "cpu_baseline": cpu,
         "c3_star_512": c3,
         "c4_adjoint_256": c4,
+        "slab_share_256x256x32": slab,
     }
     print(json.dumps(line))
 

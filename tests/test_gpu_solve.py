@@ -92,6 +92,44 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("n,box,ell,prec", [
+    ((128, 32, 32), 32, [[PER, PER], [PER, PER], [DIR, NEU]], 0),    # full-row FP64 sweep with alpha
+    ((128, 32, 32), 32, [[PER, PER], [PER, PER], [DIR, NEU]], 1),    # FP32 full-row sweep with alpha
+    ((64, 64), 16, [[DIR, NEU], [NEU, DIR]], 0),                     # 2D per-colour sweeps
+    ((96, 32, 32), 16, [[PER, PER], [NEU, DIR], [NEU, NEU]], 1),     # one-column sweep with alpha
+])
+def test_solve_over_relaxed_matches_oracle(gpu, oracle_lib, n, box, ell, prec):
+    """mac_solve with alpha and mac_opts.omega = 1.25: FP64 against the
+    oracle's over-relaxed cycle (the same cycle count, the same 1e-10
+    iterate to the bottom solver's summation order), mixed to the oracle's
+    accuracy at the same tolerance; both in fewer cycles than Gauss-Seidel."""
+    O = oracle_lib
+    g, b, alpha, rh, phi_g = make_problem(O, n, ell, 41, "rand")
+    truth, _, rc_t = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-13))
+    same, st_o, rc_o = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-10, omega=1.25))
+    _, st_gs, _ = O.solve(g, alpha, b, rh, phi_g, O.default_opts(eps_rel=1e-10))
+    assert rc_t == 0 and rc_o == 0
+    assert st_o.iters < st_gs.iters
+    lay, boxes, rhb, phb, ab, bb = per_box(n, box, ell, b, alpha, rh, phi_g)
+    plan = mac.MacPlan(lay, mac.default_opts(precision=prec, omega=1.25))
+    st, out = gpu_solve(gpu, plan, rhb, phb, ab, bb, ell, True, eps_rel=1e-10)
+    plan.close()
+    dim = len(n)
+    nn = list(n) + [1] * (3 - dim)
+    got = BX.gather_cells(out, boxes, 1, nn, dim)
+    inner = lambda a: a[(slice(1, -1),) * dim].reshape(got.shape)
+    print(f"n={n} prec={prec}: {st['iters']} iterations ({st['inner_cycles']} FP32 cycles), oracle "
+          f"{st_o.iters} over-relaxed, {st_gs.iters} Gauss-Seidel")
+    assert st["converged"]
+    if prec == 0:
+        assert abs(st["iters"] - st_o.iters) <= 1
+        assert rel_err(got, inner(same)) <= 1e-9
+    else:
+        assert st["inner_cycles"] <= st_o.iters + 1 + 2
+    e_ref = rel_err(inner(same), inner(truth))
+    assert rel_err(got, inner(truth)) <= max(1e-8, 3.0 * e_ref)
+
+
 @pytest.mark.parametrize("n,box,ell,alpha_kind,prec,device", CASES)
 def test_solve_matches_oracle(gpu, oracle_lib, n, box, ell, alpha_kind, prec, device):
     O = oracle_lib
