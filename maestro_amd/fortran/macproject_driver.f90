@@ -8,7 +8,7 @@
 ! Input (stream, native endianness; arrays per box in Fortran order with
 ! their ghost cells):
 !   int32  dim, nboxes, ng_u, ng_phi, ng_rho, alt, has_beta0, precision, ncalls,
-!          mg_cycle_type, mg_bottom_solver
+!          mg_cycle_type, mg_bottom_solver, mac_omega x 1000
 !   int32  domain_hi(3), phys_bc(3,2) (direction fastest)
 !   int32  box lo(3), hi(3) for every box
 !   real64 dx(3)
@@ -27,7 +27,7 @@ program macproject_driver
   type(bc_tower) :: bct
   type(multifab), allocatable :: umac(:,:), phi(:), rho(:)
   real(dp_t), allocatable :: beta0(:,:), beta0_edge(:,:), dx(:,:)
-  integer :: hdr(11), dhi(3), pbc(3,2), dim, nb, b, d, nr, call_no
+  integer :: hdr(12), dhi(3), pbc(3,2), dim, nb, b, d, nr, call_no
   real(dp_t) :: dx3(3)
   character(len=512) :: fin, fout
   integer :: u
@@ -68,6 +68,7 @@ program macproject_driver
   mac_precision = hdr(8)
   mg_cycle_type = hdr(10)
   mg_bottom_solver = hdr(11)
+  mac_omega = real(hdr(12), dp_t) / 1000.0_dp_t
   if (hdr(7) /= 0) then
      nr = dhi(dim) + 1
      allocate(beta0(1,0:nr-1), beta0_edge(1,0:nr))

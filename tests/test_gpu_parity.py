@@ -335,21 +335,22 @@ def test_default_inner_cycles_follow_storage(gpu, cfg, n, box):
     assert rel_err(out["phi"], truth["phi"], remove_mean=True) <= 1e-8
 
 
-@pytest.mark.parametrize("cfg,n,box", [(1, (32, 32, 32), 16), (2, (128, 64, 64), 32), (0, (64, 64), 16),
-                                       (3, (32, 32, 32), 16)])
-def test_over_relaxed_fp64_matches_oracle(gpu, cfg, n, box):
+@pytest.mark.parametrize("cfg,n,box,cycle", [(1, (32, 32, 32), 16, 3), (2, (128, 64, 64), 32, 3),
+                                             (0, (64, 64), 16, 3), (3, (32, 32, 32), 16, 3),
+                                             (3, (64, 64, 64), 32, 1), (1, (64, 64, 32), 32, 2)])
+def test_over_relaxed_fp64_matches_oracle(gpu, cfg, n, box, cycle):
     """FP64 with mac_opts.omega = 1.25 (1.2 in 2D) against the oracle's
-    over-relaxed cycle (the same levels over-relaxed: those above the
-    single-workgroup coarse cycle): the same cycle count (+-1) and the same
-    1e-10 iterate up to the bottom solver's summation order; fewer cycles
-    than the Gauss-Seidel cycle."""
+    over-relaxed cycle of the same shape (V, F, W; the same levels
+    over-relaxed: those above the single-workgroup coarse cycle): the same
+    cycle count (+-1) and the same 1e-10 iterate up to the bottom solver's
+    summation order; fewer cycles than the Gauss-Seidel cycle."""
     wl = W.make_workload(cfg, n=n, box=box)
     w = 1.25 if len(n) == 3 else 1.2
-    same = run_oracle(wl, eps=1e-10, alt=1, omega=w)
-    gs = run_oracle(wl, eps=1e-10, alt=1)
-    out = run_gpu(wl, use_alt_energy_fix=1, precision=0, omega=w)
+    same = run_oracle(wl, eps=1e-10, alt=1, omega=w, cycle_type=cycle)
+    gs = run_oracle(wl, eps=1e-10, alt=1, cycle_type=cycle)
+    out = run_gpu(wl, use_alt_energy_fix=1, precision=0, omega=w, cycle_type=cycle)
     st = out["stats"]
-    print(f"cfg{cfg} n={n} omega={w}: {st['iters']} cycles (oracle {same['stats'].iters}, "
+    print(f"cfg{cfg} n={n} cycle {cycle} omega={w}: {st['iters']} cycles (oracle {same['stats'].iters}, "
           f"Gauss-Seidel {gs['stats'].iters})")
     assert st["converged"]
     assert abs(st["iters"] - same["stats"].iters) <= 1
