@@ -47,23 +47,23 @@ __device__ __forceinline__ double fold_bc(double beta, int kind)
     return beta;
 }
 
-// beta0 of the face (dir d, global face index g[]) -- mult_edge_by_1d_coeff
-// uses beta0(r) on tangential faces and beta0_edge(r) on radial faces
-__device__ __forceinline__ double beta0_face(const SetupArgs &A, int b, int d, int i, int j,
-                                             int k, bool &has)
+// A planar beta0 seen from one plane of a column: beta0(r) on the tangential
+// faces, beta0_edge(r) and beta0_edge(r + 1) on the radial low and high faces
+// (r follows k in 3D, j in 2D); ones without a beta0.  The column kernels load
+// it a plane ahead like the caller arrays: a load issued at its use stalls the
+// march.
+struct B0 {
+    double t, lo, hi;
+};
+__device__ __forceinline__ B0 planar_b0(const double *b0, const double *b0e, int r)
 {
-    if (A.beta0) {
-        has = true;
-        int rd = A.dim - 1;
-        int r = (rd == 2 ? k : j) - A.dlo[rd];
-        return (d == rd) ? A.beta0e[r] : A.beta0[r];
+    B0 o = {1.0, 1.0, 1.0};
+    if (b0) {
+        o.t = b0[r];
+        o.lo = b0e[r];
+        o.hi = b0e[r + 1];
     }
-    if (A.cart) {
-        has = true;
-        return A.cart[b * A.dim + d][boxoff(A.boxes[b], A.ng_cart, d, A.dim, i, j, k)];
-    }
-    has = false;
-    return 1.0;
+    return o;
 }
 
 // ------------------------------------------------- column addressing
@@ -142,12 +142,15 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
             cart[d] = A.cart ? A.cart[b * DIM + d] : nullptr;
             cc[d] = col_of<DIM>(B, A.ng_cart, d, li, lj);
         }
-        // beta0 of face s (0 low, 1 high) of direction d at plane kk; has: any
-        auto b0f = [&](int d, int sd, int kk, bool &has) -> double {
+        auto b0load = [&](int kk) -> B0 {
+            return planar_b0(A.beta0, A.beta0e, (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd]);
+        };
+        // beta0 of face s (0 low, 1 high) of direction d at plane kk, whose
+        // planar values are pb; has: any
+        auto b0f = [&](int d, int sd, int kk, const B0 &pb, bool &has) -> double {
             if (A.beta0) {
                 has = true;
-                const int r = (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd] + (d == rd ? sd : 0);
-                return (d == rd) ? A.beta0e[r] : A.beta0[r];
+                return (d == rd) ? (sd ? pb.hi : pb.lo) : pb.t;
             }
             if (A.cart) {
                 has = true;
@@ -180,9 +183,10 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
         };
         // scaled z faces and rho of the previous plane (3D carries)
         double uz_lo = 0.0, r_below = 0.0;
+        B0 bcur = b0load(kz0);
         if (DIM == 3) {
             bool has;
-            const double b0 = b0f(2, 0, kz0, has);
+            const double b0 = b0f(2, 0, kz0, bcur, has);
             uz_lo = u[2][cu[2].at(kz0)];
             if (has) uz_lo = uz_lo * b0;
             r_below = rho[cr.at(kz0, 0, 0, -1)];
@@ -190,6 +194,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
         PIn cur = load(kz0);
         for (int kk = kz0; kk < kz1; ++kk) {
             const PIn nxt = load(min(kk + 1, kz1 - 1));
+            const B0 bnxt = b0load(min(kk + 1, kz1 - 1));
             const int k = (DIM == 3) ? B.lo[2] + kk : 0, K = (DIM == 3) ? k - A.dlo[2] : 0;
             const int c = I + nx * (J + ny * K);
             double uf[DIM][2];
@@ -203,7 +208,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
                     }
                     double v = cur.u[d == 2 ? 4 : 2 * d + sd];
                     bool has;
-                    const double b0 = b0f(d, sd, kk, has);
+                    const double b0 = b0f(d, sd, kk, bcur, has);
                     if (has) v = v * b0;
                     uf[d][sd] = v;
                 }
@@ -225,7 +230,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
                 const double rl = (DIM == 3 && d == 2) ? r_below : (d == 0 ? cur.rx : cur.ry);
                 double beta = 2.0 / (rc + rl);
                 bool has;
-                const double b0 = b0f(d, 0, kk, has);
+                const double b0 = b0f(d, 0, kk, bcur, has);
                 if (has) {
                     beta = beta * b0;
                     if (A.alt) beta = beta * b0;
@@ -236,7 +241,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
                 bl[c] = beta;
                 if (Id == A.n[d] - 1) {
                     double bh = 2.0 / (rho[cr.at(kk, d == 0, d == 1, d == 2)] + rc);
-                    const double b0h = b0f(d, 1, kk, has);
+                    const double b0h = b0f(d, 1, kk, bcur, has);
                     if (has) {
                         bh = bh * b0h;
                         if (A.alt) bh = bh * b0h;
@@ -256,6 +261,7 @@ __global__ void __launch_bounds__(TPB) k_setup(SetupArgs A)
                 r_below = rc;
             }
             cur = nxt;
+            bcur = bnxt;
         }
     }
     if (bad) atomicOr(A.err, 1);
@@ -2978,14 +2984,13 @@ __global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
     const double *wl[3] = {A.bx, A.by, A.bz};
     // one face: its position in the column (sd: 0 low, 1 high face of the
     // cell), its velocity as read, effective weight and the phi values on both
-    // sides; the corrected velocity is stored
-    auto correct = [&](int d, int sd, int kk, double t, double w, double phi_hi, double phi_lo) {
+    // sides; pb: the planar beta0 of plane kk.  The corrected velocity is stored
+    auto correct = [&](int d, int sd, int kk, const B0 &pb, double t, double w, double phi_hi, double phi_lo) {
         const int o = cu[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0);
         double b0 = 1.0;
         bool has = false;
         if (A.beta0) {
-            const int r = (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd] + (d == rd ? sd : 0);
-            b0 = (d == rd) ? A.beta0e[r] : A.beta0[r];
+            b0 = (d == rd) ? (sd ? pb.hi : pb.lo) : pb.t;
             has = true;
         } else if (A.cart) {
             b0 = cart[d][cc[d].at(kk, d == 0 ? sd : 0, d == 1 ? sd : 0, d == 2 ? sd : 0)];
@@ -3029,17 +3034,22 @@ __global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
     };
     double pz_lo = 0.0;
     if (DIM == 3) pz_lo = below(2, K0 + kz0, I + nx * (J + ny * (K0 + kz0)));
+    auto b0load = [&](int kk) -> B0 {
+        return planar_b0(A.beta0, A.beta0e, (rd == 2 ? B.lo[2] + kk : j) - A.dlo[rd]);
+    };
     CIn cur = load(kz0);
+    B0 bcur = b0load(kz0);
     for (int kk = kz0; kk < kz1; ++kk) {
         const CIn nxt = load(min(kk + 1, kz1 - 1));
+        const B0 bnxt = b0load(min(kk + 1, kz1 - 1));
         const int K = K0 + kk;
         const int c = I + nx * (J + ny * K);
         const int Ix[3] = {I, J, K};
         const double pc = cur.pc;
         // low faces
-        correct(0, 0, kk, cur.u[0], cur.w[0], pc, cur.px);
-        correct(1, 0, kk, cur.u[1], cur.w[1], pc, cur.py);
-        if (DIM == 3) correct(2, 0, kk, cur.u[2], cur.w[2], pc, pz_lo);
+        correct(0, 0, kk, bcur, cur.u[0], cur.w[0], pc, cur.px);
+        correct(1, 0, kk, bcur, cur.u[1], cur.w[1], pc, cur.py);
+        if (DIM == 3) correct(2, 0, kk, bcur, cur.u[2], cur.w[2], pc, pz_lo);
         // the box's high faces
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
@@ -3048,18 +3058,19 @@ __global__ void __launch_bounds__(TPB) k_correct_valid(CorrectArgs A)
             const int nd = A.n[d];
             const double t = face_u(d, 1, kk);
             if (Ix[d] + 1 < nd) {
-                correct(d, 1, kk, t, wl[d][c + st[d]], A.phi[c + st[d]], pc);
+                correct(d, 1, kk, bcur, t, wl[d][c + st[d]], A.phi[c + st[d]], pc);
             } else if (A.bc.k[d][0] == ELL_PER) {   // the periodic image of face 0
                 const int c0 = c - (nd - 1) * st[d];
-                correct(d, 1, kk, t, wl[d][c0], A.phi[c0], pc);
+                correct(d, 1, kk, bcur, t, wl[d][c0], A.phi[c0], pc);
             } else {                                 // the domain's high face
                 const double w = d == 0 ? A.bxh[J + ny * K] : (d == 1 ? A.byh[I + nx * K] : A.bzh[I + nx * J]);
-                correct(d, 1, kk, t, w, 0.0, pc);
+                correct(d, 1, kk, bcur, t, w, 0.0, pc);
             }
         }
         if (pout) pout[cp.at(kk)] = pc;
         pz_lo = pc;
         cur = nxt;
+        bcur = bnxt;
     }
 }
 
