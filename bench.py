@@ -47,7 +47,8 @@ HBM_PEAK_GBS = 8000.0
 OMEGA_HEADLINE = 1.25
 
 
-def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False, sor: bool = False) -> str:
+def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = False, sor: bool = False,
+                      pz: bool = False) -> str:
     """Name of the finest-level sweep kernel the library launches (b16: the
     compact bfloat16-storage form of the FP32 full-row kernel; rho: the FP32
     form that forms its weights from the density, which the plane-parallel
@@ -55,13 +56,16 @@ def sweep_kernel_name(s: int, b16: bool = False, nx: int = 256, rho: bool = Fals
     W = nx / 128 waves per row; rows x threads per block: FP64 6 x 768, FP32
     6 x 768 with the weights fetched one plane ahead (16 x 1024 at nx = 128),
     FP32 with bfloat16 storage 6 x 768; the template's last two arguments: no
-    alpha term, the weights' prefetch distance; the last: over-relaxed (sor)."""
+    alpha term, the weights' prefetch distance; the last: over-relaxed (sor).
+    The rho form's last argument: z periodic (pz; the workloads here have z
+    walls)."""
     t = "double" if s == 8 else "float"
     c = "true" if b16 else "false"
     o = "true" if sor else "false"
     w = max(1, nx // 128)
     if rho:   # FP32, the weights formed from rho: 6 rows x 768 (12 x 768 at nx = 128)
-        return f"k_gsrb_rows_rho<0, 6, 768, {w}, {o}>" if w == 2 else f"k_gsrb_rows_rho<0, 12, 768, 1, {o}>"
+        z = "true" if pz else "false"
+        return f"k_gsrb_rows_rho<0, 6, 768, {w}, {o}, {z}>" if w == 2 else f"k_gsrb_rows_rho<0, 12, 768, 1, {o}, {z}>"
     if s == 8 or b16:
         return f"k_gsrb_rows<{t}, 0, 6, 768, {c}, {w}, false, 2, {o}>"
     return (f"k_gsrb_rows<{t}, 0, 16, 1024, {c}, 1, false, 2, {o}>" if w == 1

@@ -1224,7 +1224,8 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows(Lev<T> L, RowBufs B, RowSweep
 // residual uses the FP64 weights.  Structure as k_gsrb_rows (one column pair
 // per lane, W waves per row, EY rows, z-march, one barrier per plane); rho
 // rotates through LDS beside phi.  The z faces of both columns are formed
-// once per plane, into a four-slot register ring.
+// once per plane, into a four-slot register ring.  PZ: z periodic, a template
+// parameter because the wrap logic per step costs 5% of the sweep at run time.
 struct RhoBufs {      // byte offsets in the weights' allocation (with B.rhs)
     unsigned rho;     // FP32 cell density
     unsigned cxy;     // per plane: 2 beta0^p (x and y faces)
@@ -1233,7 +1234,7 @@ struct RhoBufs {      // byte offsets in the weights' allocation (with B.rhs)
     unsigned bzh;     // and of its high z faces
 };
 
-template <int MODE, int EY, int NTH, int W, bool SOR = false>
+template <int MODE, int EY, int NTH, int W, bool SOR = false, bool PZ = false>
 __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, RhoBufs RB, RowSweepGrid G)
 {
     using T = float;
@@ -1278,8 +1279,8 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, 
 
     auto zwrap = [&](int k, int &kw) -> bool {
         const bool in = (unsigned)k < (unsigned)nz;
-        kw = in ? k : (L.perz ? (k < 0 ? k + nz : k - nz) : 0);
-        return in || L.perz;
+        kw = in ? k : (PZ ? (k < 0 ? k + nz : k - nz) : 0);
+        return in || PZ;
     };
     auto z2 = []() {
         T2 v;
@@ -1333,12 +1334,12 @@ __global__ void __launch_bounds__(NTH) k_gsrb_rows_rho(Lev<float> L, RowBufs B, 
     // sweep); cz of face f, clamped into the table
     auto czload = [&](int f) -> T {
         int fw = f;
-        if (L.perz) fw = f < 0 ? f + nz : (f > nz ? f - nz : f);
+        if (PZ) fw = f < 0 ? f + nz : (f > nz ? f - nz : f);
         return cz[min(max(fw, 0), nz)];
     };
     auto facez = [&](int f, const In &lo, const In &hi, T c) -> T2 {
         int fw = f;
-        if (L.perz) fw = f < 0 ? f + nz : (f > nz ? f - nz : f);
+        if (PZ) fw = f < 0 ? f + nz : (f > nz ? f - nz : f);
         if (fw == 0 || fw == nz) return ld_rs<T2>(Rw, vw, fw == 0 ? RB.bzl : RB.bzh);   // slab: one plane
         if (fw < 0 || fw > nz) return z2();
         T2 v;
@@ -3679,7 +3680,7 @@ static bool launch_rows(const Lev<T> &L, const T *pin, T *pout, int mode, hipStr
 // The rho form of the FP32 finest-level sweep (k_gsrb_rows_rho): 6 rows of
 // 768 threads at nx = 256, 12 rows of 768 at nx = 128; phi and rho planes
 // in LDS (8 planes), two blocks per CU.
-template <int EY, int NTH, int W, bool SOR>
+template <int EY, int NTH, int W, bool SOR, bool PZ>
 static bool launch_rows_rho_w(const Lev<float> &L, const RowBufs &RB, const RhoBufs &HB, int mode, hipStream_t s)
 {
     const size_t lds = sizeof(float) * 8 * (size_t)(EY + 2) * (L.nx + 4);
@@ -3687,16 +3688,16 @@ static bool launch_rows_rho_w(const Lev<float> &L, const RowBufs &RB, const RhoB
     static int slots_cache[2] = {0, 0};   // [mode] resident blocks per device
     int &slots = slots_cache[mode == SWEEP_ZERO ? 1 : 0];
     if (slots == 0) {
-        slots = (mode == SWEEP_ZERO) ? resident_slots((const void *)k_gsrb_rows_rho<1, EY, NTH, W, SOR>, threads, lds)
-                                     : resident_slots((const void *)k_gsrb_rows_rho<0, EY, NTH, W, SOR>, threads, lds);
+        slots = (mode == SWEEP_ZERO) ? resident_slots((const void *)k_gsrb_rows_rho<1, EY, NTH, W, SOR, PZ>, threads, lds)
+                                     : resident_slots((const void *)k_gsrb_rows_rho<0, EY, NTH, W, SOR, PZ>, threads, lds);
         if (slots == 0) return false;
     }
     const RowSweepGrid G = row_grid(L.ny, L.nz, EY, W, slots);
     dim3 g(8 * G.per_xcd), b(threads);
     if (mode == SWEEP_ZERO)
-        hipLaunchKernelGGL((k_gsrb_rows_rho<1, EY, NTH, W, SOR>), g, b, lds, s, L, RB, HB, G);
+        hipLaunchKernelGGL((k_gsrb_rows_rho<1, EY, NTH, W, SOR, PZ>), g, b, lds, s, L, RB, HB, G);
     else
-        hipLaunchKernelGGL((k_gsrb_rows_rho<0, EY, NTH, W, SOR>), g, b, lds, s, L, RB, HB, G);
+        hipLaunchKernelGGL((k_gsrb_rows_rho<0, EY, NTH, W, SOR, PZ>), g, b, lds, s, L, RB, HB, G);
     return true;
 }
 
@@ -3729,11 +3730,15 @@ bool launch_gsrb_rho(const Lev<float> &L, const RhoLev &R, const float *pin, flo
     RhoBufs HB;
     if (!rho_level_ok(3, L) || !row_bufs<float>(L, pin, pout, nullptr, RB) || !rho_bufs(L, R, RB, HB)) return false;
     const bool sor = L.omega != 1.0f;
-    if (L.nx == 256)
-        return sor ? launch_rows_rho_w<6, 768, 2, true>(L, RB, HB, mode, s)
-                   : launch_rows_rho_w<6, 768, 2, false>(L, RB, HB, mode, s);
-    return sor ? launch_rows_rho_w<12, 768, 1, true>(L, RB, HB, mode, s)
-               : launch_rows_rho_w<12, 768, 1, false>(L, RB, HB, mode, s);
+    auto go = [&](auto SC, auto PC) {
+        constexpr bool SR = decltype(SC)::value, PZ = decltype(PC)::value;
+        if (L.nx == 256) return launch_rows_rho_w<6, 768, 2, SR, PZ>(L, RB, HB, mode, s);
+        return launch_rows_rho_w<12, 768, 1, SR, PZ>(L, RB, HB, mode, s);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (sor) return L.perz ? go(T_{}, T_{}) : go(T_{}, F_{});
+    return L.perz ? go(F_{}, T_{}) : go(F_{}, F_{});
 }
 
 bool compact_level_ok(int dim, const Lev<float> &L)
@@ -3917,21 +3922,24 @@ void launch_ir_update_residual(int dim, const Lev<double> &L, const float *e, do
 }
 
 __global__ void k_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
-                               unsigned long long *keep)
+                               unsigned long long *keep, const int *flag)
 {
     if (threadIdx.x == 0) {
         const unsigned long long v = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(keep, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(dst, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (flag)
+            __hip_atomic_store(dst + 2, (unsigned long long)(unsigned)*flag, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(dst + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(src, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
-                         hipStream_t s, unsigned long long *keep)
+                         hipStream_t s, unsigned long long *keep, const int *flag)
 {
-    hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq, keep);
+    hipLaunchKernelGGL(k_publish_norm, dim3(1), dim3(64), 0, s, src, dst, seq, keep, flag);
 }
 
 void launch_sub_mean_absmax(double *x, long n, const double *mean, unsigned long long *nrm, hipStream_t s)

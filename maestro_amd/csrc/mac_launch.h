@@ -215,10 +215,11 @@ template <typename A, typename B> void launch_convert(const A *in, B *out, long 
 void launch_absmax(const double *x, long n, unsigned long long *nrm, hipStream_t s);
 // x -= *mean with max |x - mean| into nrm, one pass
 void launch_sub_mean_absmax(double *x, long n, const double *mean, unsigned long long *nrm, hipStream_t s);
-// one thread: *keep = dst[0] = *src, then dst[1] = seq (system scope, dst
-// mapped host memory), then *src = 0
+// one thread: *keep = dst[0] = *src (and dst[2] = *flag where flag is
+// given), then dst[1] = seq (system scope, dst mapped host memory), then
+// *src = 0
 void launch_publish_norm(unsigned long long *src, unsigned long long *dst, unsigned long long seq,
-                         hipStream_t s, unsigned long long *keep);
+                         hipStream_t s, unsigned long long *keep, const int *flag = nullptr);
 void launch_ir_residual(int dim, const Lev<double> &L, float *r32, unsigned long long *nrm,
                         hipStream_t s, unsigned short *r16 = nullptr);
 template <typename T>

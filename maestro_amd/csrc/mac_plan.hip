@@ -85,7 +85,7 @@ struct mac_plan {
     long part_cap = 0;
     double *mean_dev = nullptr;
     unsigned long long *nrm_dev = nullptr;
-    unsigned long long *nrm_host = nullptr;   // pinned: [0] a copied norm
+    unsigned long long *nrm_host = nullptr;   // pinned: [0] a copied norm, [1] a copied flag, [2] max |rh|
     // residual norms reach the host through mapped pinned memory: a
     // one-thread kernel writes the value and then a sequence number, which
     // the host polls (no stream synchronisation per refinement step)
@@ -93,6 +93,10 @@ struct mac_plan {
     unsigned long long *nrm_map_dev = nullptr;
     unsigned long long nrm_seq = 0;
     bool nrm_map_ok = true;
+    // the projection's device flag of non-finite inputs: read with the
+    // initial residual norm rather than by a stream synchronisation of its
+    // own between the setup and the solve
+    const int *setup_err = nullptr;
     int *err_dev = nullptr;
     double *cg = nullptr;
     long cg_cap = 0;
@@ -530,14 +534,17 @@ static unsigned long long *res_slot(mac_plan *P) { return P->nrm_dev + 1; }
 // Publish the residual norm accumulated in the slot and wait for it: a
 // one-thread kernel copies it (also into nrm_dev[3]) with a sequence number
 // into mapped host memory and clears the slot; the host polls the number.
-// Without working mapped memory the value comes through a plain copy.
-static int read_norm(mac_plan *P, double *out)
+// Without working mapped memory the value comes through a plain copy.  bad
+// (where given) receives the projection's non-finite-input flag.
+static int read_norm(mac_plan *P, double *out, int *bad = nullptr)
 {
     unsigned long long u = 0, seq = 0;
     bool have = false;
+    const int *flag = bad ? P->setup_err : nullptr;
+    if (bad) *bad = 0;
     if (P->nrm_map_ok) {
         seq = ++P->nrm_seq;
-        launch_publish_norm(res_slot(P), P->nrm_map_dev, seq, P->stream, P->nrm_dev + 3);
+        launch_publish_norm(res_slot(P), P->nrm_map_dev, seq, P->stream, P->nrm_dev + 3, flag);
         HIPCHK(hipGetLastError());
         volatile unsigned long long *h = P->nrm_map;
         for (long it = 0;; ++it) {
@@ -554,14 +561,20 @@ static int read_norm(mac_plan *P, double *out)
             }
         }
         std::atomic_thread_fence(std::memory_order_acquire);
-        if (have) u = h[0];
+        if (have) {
+            u = h[0];
+            if (flag) *bad = (int)h[2];
+        }
     }
     if (!have) {   // plain copy: the published value's device copy, or the slot itself (then cleared)
         const unsigned long long *src = seq != 0 ? P->nrm_dev + 3 : res_slot(P);
         HIPCHK(hipMemcpyAsync(P->nrm_host, src, sizeof(unsigned long long), hipMemcpyDeviceToHost, P->stream));
+        if (flag)
+            HIPCHK(hipMemcpyAsync(P->nrm_host + 1, flag, sizeof(int), hipMemcpyDeviceToHost, P->stream));
         if (seq == 0) HIPCHK(hipMemsetAsync(res_slot(P), 0, sizeof(unsigned long long), P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
         u = *P->nrm_host;
+        if (flag) memcpy(bad, P->nrm_host + 1, sizeof(int));
     }
     double v;
     memcpy(&v, &u, sizeof v);
@@ -569,22 +582,22 @@ static int read_norm(mac_plan *P, double *out)
     return MAC_OK;
 }
 
-static int fp64_norm(mac_plan *P, double *out)
+static int fp64_norm(mac_plan *P, double *out, int *bad = nullptr)
 {
     const Lev<double> &L = P->H64.lev[0];
     int q = prof_begin(P, MAC_PROF_RESID64);
     launch_residual<double>(P->dim, L, (double *)nullptr, res_slot(P), P->stream);
     prof_end(P, q);
-    return read_norm(P, out);
+    return read_norm(P, out, bad);
 }
 
-static int ir_norm(mac_plan *P, double *out, const Compact16 *K)
+static int ir_norm(mac_plan *P, double *out, const Compact16 *K, int *bad)
 {
     int q = prof_begin(P, MAC_PROF_RESID64);
     launch_ir_residual(P->dim, P->H64.lev[0], P->H32.lev[0].rhs, res_slot(P), P->stream,
                        K ? K->rhs : nullptr);
     prof_end(P, q);
-    return read_norm(P, out);
+    return read_norm(P, out, bad);
 }
 
 static bool converged(const mac_opts &o, double r, double r0)
@@ -667,7 +680,9 @@ static int solve(mac_plan *P, mac_stats *st)
             if ((rc = compact_level0(P, &K, false))) return rc;
         }
         P->launch_failed = false;
-        if ((rc = ir_norm(P, &r0, K))) return rc;
+        int bad = 0;
+        if ((rc = ir_norm(P, &r0, K, &bad))) return rc;
+        if (bad) return MAC_ERR_NONFINITE;
         r = r0;
         const Lev<float> &E = P->H32.lev[0];
         const int per_step = inner_per_step(o, K != nullptr);
@@ -700,7 +715,9 @@ static int solve(mac_plan *P, mac_stats *st)
         set_omega(P->H64, sweep_omega(P));
         // the FP64 hierarchy's coarse weights (the mixed paths coarsen the FP32 one)
         coarsen_all<double>(P, P->H64);
-        if ((rc = fp64_norm(P, &r0))) return rc;
+        int bad = 0;
+        if ((rc = fp64_norm(P, &r0, &bad))) return rc;
+        if (bad) return MAC_ERR_NONFINITE;
         r = r0;
         while (!converged(o, r, r0) && it < o.max_iter) {
             vcycle<double>(P, P->H64);
@@ -1220,19 +1237,19 @@ static int project_impl(mac_plan *P, double *const *umac, double *const *phi, co
     } else {
         launch_absmax(P->H64.lev[0].rhs, n0, P->nrm_dev, s);
     }
-    int err_host = 0;
-    unsigned long long rhs_bits = 0;
-    HIPCHK(hipMemcpyAsync(&err_host, PB.err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&rhs_bits, P->nrm_dev, sizeof rhs_bits, hipMemcpyDeviceToHost, s));
     HIPCHK(hipEventRecord(P->ev[2], s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (err_host) return MAC_ERR_NONFINITE;
-    // ---- solve
+    // ---- solve (the non-finite-input flag comes back with the initial
+    // residual norm, so the host does not wait for the setup on its own; the
+    // solver leaves nrm_dev[0], max |rh|, alone)
     mac_stats local;
     memset(&local, 0, sizeof local);
+    P->setup_err = PB.err;
     int src = solve(P, &local);
-    memcpy(&local.rhs_norm, &rhs_bits, sizeof(double));
-    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    P->setup_err = nullptr;
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) {
+        (void)hipStreamSynchronize(s);
+        return src;
+    }
     HIPCHK(hipEventRecord(P->ev[3], s));
     // ---- correction, ghost fill, phi out
     CorrectArgs C;
@@ -1261,7 +1278,9 @@ static int project_impl(mac_plan *P, double *const *umac, double *const *phi, co
         if ((rc = stage_out(P, umac, phi))) return rc;
     }
     HIPCHK(hipEventRecord(P->ev[5], s));
+    HIPCHK(hipMemcpyAsync(P->nrm_host + 2, P->nrm_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    memcpy(&local.rhs_norm, P->nrm_host + 2, sizeof(double));
     if (st) {
         *st = local;
         st->t_h2d_ms = ev_ms(P, 0, 1);
