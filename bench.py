@@ -112,11 +112,17 @@ def stage(per_box, copies, torch):
 
 
 def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=0, profile=True, omega=0.0):
+    """Warm-up and timed projections, one input set per call.  The per-box
+    pointer tables are built once per set, outside the timed region, as a
+    Fortran or C caller holds its MultiFab's data pointers from step to step."""
+    from maestro_amd import mac
     plan.set_opts(precision=prec, device_ptrs=1, inner_cycles=inner, omega=omega)
+    tabs = [(mac.PtrTable([a for box in du for a in box]), mac.PtrTable(dp)) for du, dp in sets]
+    drt = mac.PtrTable(dr)
     stats = []
     for w in range(warmup):
-        du, dp = sets[w]
-        stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
+        du, dp = tabs[w]
+        stats.append(plan.project(du, dp, drt, None, b0, b0e, None))
     # per-kernel HIP events bracket the launches of the last timed step only:
     # each bracket stalls the stream for a few microseconds, so instrumenting
     # every step would slow the measured throughput itself
@@ -128,8 +134,8 @@ def run_precision(plan, sets, dr, b0, b0e, warmup, steps, prec, torch, inner=0, 
     for k in range(steps):
         if profile and k == steps - 1:
             plan.profile(True)
-        du, dp = sets[warmup + k]
-        stats.append(plan.project(du, dp, dr, None, b0, b0e, None))
+        du, dp = tabs[warmup + k]
+        stats.append(plan.project(du, dp, drt, None, b0, b0e, None))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = plan.profile_read()
@@ -250,6 +256,8 @@ def time_star_512(torch, steps=3, warmup=1):
                                ("mixed_f_over_relaxed", mac.PREC_MIXED, mac.CYCLE_F, OMEGA_HEADLINE),
                                ("fp64_v", mac.PREC_FP64, mac.CYCLE_V, 0.0)):
         plan.set_opts(precision=prec, cycle_type=cyc, device_ptrs=1, max_iter=200, omega=w)
+        tu, tp, tr = mac.PtrTable([a for b in du for a in b]), mac.PtrTable(dp), mac.PtrTable(dr)
+        tc = mac.PtrTable([c for b in dc for c in b])
         ts, st = [], None
         for _ in range(warmup + steps):
             for b, b0 in zip(du, du0):
@@ -259,7 +267,7 @@ def time_star_512(torch, steps=3, warmup=1):
                 a.copy_(a0)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            st = plan.project(du, dp, dr, None, None, None, dc)
+            st = plan.project(tu, tp, tr, None, None, None, tc)
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t)
         tm = float(np.median(ts[warmup:]))
@@ -311,6 +319,7 @@ def time_slab_share(torch, steps=5, warmup=2):
     dp = [a.clone() for a in dp0]
     for name, prec, w in (("fp64", mac.PREC_FP64, 0.0), ("mixed", mac.PREC_MIXED, OMEGA_HEADLINE)):
         plan.set_opts(precision=prec, device_ptrs=1, omega=w)
+        tu, tp, tr = mac.PtrTable([a for b in du for a in b]), mac.PtrTable(dp), mac.PtrTable(dr)
         ts, st = [], None
         for k in range(warmup + steps):
             for b, b0_ in zip(du, du0):
@@ -322,7 +331,7 @@ def time_slab_share(torch, steps=5, warmup=2):
                 plan.profile(True)
             torch.cuda.synchronize()
             t = time.perf_counter()
-            st = plan.project(du, dp, dr, None, b0, b0e, None)
+            st = plan.project(tu, tp, tr, None, b0, b0e, None)
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t)
         prof = plan.profile_read()

@@ -238,8 +238,31 @@ def _ptr(x):
 def _ptr_array(xs):
     if xs is None:
         return None
+    if isinstance(xs, PtrTable):
+        return xs.arr
     arr = (C.c_void_p * len(xs))(*[_ptr(x).value for x in xs])
     return arr
+
+
+class PtrTable:
+    """The data pointers of a list of arrays (all torch device tensors or all
+    numpy host arrays), checked once and kept with the arrays they point
+    into.  A caller that passes the same per-box arrays on every call (a
+    MultiFab's boxes, step after step) builds the table once: building it
+    from some thousands of Python array objects costs about as much as a
+    millisecond per call.  Accepted wherever ``project`` takes a list of
+    arrays; for ``umac`` it holds the flattened per-box list (box-major,
+    ``dim`` faces per box)."""
+
+    def __init__(self, xs):
+        self.items = list(xs)
+        self.device = bool(self.items) and _is_torch(self.items[0])
+        if any(_is_torch(x) != self.device for x in self.items):
+            raise ValueError("a pointer table holds device arrays or host arrays, not both")
+        self.arr = _ptr_array(self.items)
+
+    def __len__(self):
+        return len(self.items)
 
 
 class MacPlan:
@@ -285,15 +308,33 @@ class MacPlan:
         mac_project_noalloc (device arrays; the plan's solver storage for its
         precision already allocated by an earlier call)."""
         dim = self.layout.dim
-        flat_u = [u for box in umac for u in box]
-        flat_c = None if beta0_cart_edge is None else [c for box in beta0_cart_edge for c in box]
-        dev = _is_torch(flat_u[0])
+        nb = len(self.layout.boxes)
+
+        def table(xs, per_box, name):
+            if xs is None:
+                return None
+            if isinstance(xs, PtrTable):
+                t = xs
+            elif per_box == 1:
+                t = list(xs)
+            else:
+                if any(len(box) != per_box for box in xs):
+                    raise ValueError(f"{name}: {per_box} arrays per box")
+                t = [a for box in xs for a in box]
+            if len(t) != nb * per_box:
+                raise ValueError(f"{name}: {nb * per_box} arrays for {nb} boxes, got {len(t)}")
+            return t
+
+        flat_u = table(umac, dim, "umac")
+        phi_t, rho_t = table(phi, 1, "phi"), table(rho, 1, "rho")
+        rhs_t, flat_c = table(macrhs, 1, "macrhs"), table(beta0_cart_edge, dim, "beta0_cart_edge")
+        dev = flat_u.device if isinstance(flat_u, PtrTable) else _is_torch(flat_u[0])
         if bool(self.opts.device_ptrs) != dev:
             self.set_opts(device_ptrs=int(dev))
-        self._sync_torch(flat_u + list(phi) + list(rho))
-        assert all(len(box) == dim for box in umac)
-        keep = [_ptr_array(flat_u), _ptr_array(list(phi)), _ptr_array(list(rho)),
-                _ptr_array(macrhs), _ptr_array(flat_c)]
+        if dev:
+            self._sync_torch([flat_u.items[0] if isinstance(flat_u, PtrTable) else flat_u[0]])
+        keep = [_ptr_array(flat_u), _ptr_array(phi_t), _ptr_array(rho_t),
+                _ptr_array(rhs_t), _ptr_array(flat_c)]
         b0 = _ptr(beta0_1d) if beta0_1d is not None else None
         b0e = _ptr(beta0_1d_edge) if beta0_1d_edge is not None else None
         st = CStats()

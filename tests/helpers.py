@@ -18,9 +18,10 @@ def layout_for(wl, **ng):
     return Layout.uniform(wl["n"], wl["box"], wl["dx"], wl["bc"], **ng)
 
 
-def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, check=True, **kw):
+def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, check=True, tables=False, **kw):
     """Run macproject through the C ABI on per-box arrays; returns the
-    whole-domain projected faces, phi and the stats."""
+    whole-domain projected faces, phi and the stats.  tables: the device
+    arrays go in as prebuilt pointer tables (mac.PtrTable)."""
     from maestro_amd import mac
     boxes, umac, phi, rho, rhs, cart = per_box_inputs(wl, ng_u=ng_u, macrhs=macrhs)
     lay = layout_for(wl, ng_umac=ng_u, ng_macrhs=0)
@@ -35,7 +36,12 @@ def run_gpu(wl, opts=None, device=False, macrhs=None, ng_u=1, plan=None, check=T
         dc = None if cart is None else [[cv(a) for a in box] for box in cart]
         b0 = None if wl["beta0"] is None else cv(np.ascontiguousarray(wl["beta0"]))
         b0e = None if wl["beta0_edge"] is None else cv(np.ascontiguousarray(wl["beta0_edge"]))
-        st = p.project(du, dp, dr, drh, b0, b0e, dc, check=check)
+        if tables:
+            T = mac.PtrTable
+            st = p.project(T([a for box in du for a in box]), T(dp), T(dr), None if drh is None else T(drh),
+                           b0, b0e, None if dc is None else T([a for box in dc for a in box]), check=check)
+        else:
+            st = p.project(du, dp, dr, drh, b0, b0e, dc, check=check)
         umac = [[a.cpu().numpy() for a in box] for box in du]
         phi = [a.cpu().numpy() for a in dp]
     else:

@@ -89,3 +89,25 @@ def test_python_mirror_raises_on_invalid_combination():
     with pytest.raises(mac.MacError) as e:
         mac.macproject(lay, [], [], [], beta0_1d=object(), beta0_cart_edge=object())
     assert e.value.code == 1
+
+
+def test_pointer_table_holds_checked_pointers():
+    """mac.PtrTable: the arrays' data pointers in order, one kind of array per
+    table, float64 and contiguous only."""
+    import numpy as np
+    import torch
+    xs = [np.zeros((3, 4)), np.ones(5)]
+    t = mac.PtrTable(xs)
+    assert len(t) == 2 and not t.device
+    assert [t.arr[i] for i in range(2)] == [a.ctypes.data for a in xs]
+    ts = [torch.zeros(4, dtype=torch.float64), torch.ones(2, 3, dtype=torch.float64)]
+    tt = mac.PtrTable(ts)
+    assert tt.device and [tt.arr[i] for i in range(2)] == [a.data_ptr() for a in ts]
+    with pytest.raises(ValueError):
+        mac.PtrTable([np.zeros(3), torch.zeros(3, dtype=torch.float64)])
+    with pytest.raises(ValueError):
+        mac.PtrTable([np.zeros(3, dtype=np.float32)])
+    with pytest.raises(ValueError):
+        mac.PtrTable([np.zeros((4, 4))[:, ::2]])
+    with pytest.raises(ValueError):
+        mac.PtrTable([torch.zeros(4, 4, dtype=torch.float64).t()])

@@ -94,6 +94,39 @@ def test_host_and_device_pointers_agree(gpu):
     assert np.array_equal(a["phi"], b["phi"])
 
 
+@pytest.mark.parametrize("cfg,macrhs", [(1, False), (3, False), (2, True)])
+def test_pointer_tables_agree_with_lists(gpu, cfg, macrhs):
+    """Per-box device arrays passed as prebuilt pointer tables give the
+    projection of the same arrays passed as lists, bit for bit."""
+    wl = W.make_workload(cfg, n=(32, 32, 32), box=16)
+    rhs = np.random.default_rng(5).standard_normal(wl["n"][::-1]) if macrhs else None
+    a = run_gpu(wl, device=True, macrhs=rhs)
+    b = run_gpu(wl, device=True, macrhs=rhs, tables=True)
+    for d in range(3):
+        assert np.array_equal(a["faces"][d], b["faces"][d])
+    assert np.array_equal(a["phi"], b["phi"])
+
+
+def test_array_counts_are_checked(gpu):
+    """A list or pointer table with the wrong number of arrays for the
+    layout's boxes is refused before the library reads it."""
+    from maestro_amd import mac
+    from .helpers import layout_for, per_box_inputs
+    wl = W.make_workload(1, n=(32, 32, 32), box=16)
+    boxes, umac, phi, rho, _, _ = per_box_inputs(wl)
+    plan = mac.MacPlan(layout_for(wl, ng_umac=1, ng_macrhs=0), mac.default_opts())
+    b0, b0e = np.ascontiguousarray(wl["beta0"]), np.ascontiguousarray(wl["beta0_edge"])
+    with pytest.raises(ValueError):
+        plan.project(umac[:-1], phi, rho, None, b0, b0e, None)
+    with pytest.raises(ValueError):
+        plan.project(umac, phi, rho[:-1], None, b0, b0e, None)
+    with pytest.raises(ValueError):
+        plan.project([box[:2] for box in umac], phi, rho, None, b0, b0e, None)
+    with pytest.raises(ValueError):
+        plan.project(mac.PtrTable([a for box in umac for a in box][:-1]), phi, rho, None, b0, b0e, None)
+    plan.close()
+
+
 def test_deterministic(gpu):
     wl = W.make_workload(1, n=(32, 32, 32), box=16)
     a = run_gpu(wl)
