@@ -2264,14 +2264,19 @@ __global__ void __launch_bounds__(TPB) k_prolong(Lev<T> F, const T *e, int cnx, 
     });
 }
 
-// The same prolongation with one wave per fine (x-row segment, plane): the
-// row's coarse rows/planes and weights are wave-uniform, each lane owns the
-// fine column pair 2I, 2I+1 (coarse column I) and reads the coarse values at
-// I-1, I, I+1; the fine pair is loaded and stored as a vector.  Evaluation
-// order is that of prolong_value.  3D, fine nx = 128 W.
+// The same prolongation with one wave per fine (x-row segment, chunk of pz
+// fine planes): the row's coarse rows and the x/y weights are wave-uniform,
+// each lane owns the fine column pair 2I, 2I+1 (coarse column I) and reads
+// the coarse values at I-1, I, I+1.  The wave marches the chunk's coarse
+// planes: the x/y-interpolated correction of a coarse plane is formed once
+// and serves the two fine planes it parents and the fine planes on either
+// side that take it as their second z point (the row form above formed two
+// per fine plane).  The fine pair is loaded and stored as a vector.
+// Evaluation order is that of prolong_value.  3D, fine nx = 128 W, fine
+// ny, nz twice the coarse extents, pz even.
 template <typename T>
 __global__ void __launch_bounds__(256) k_prolong_rows(Lev<T> F, const T *__restrict__ e, int cnx, int cny,
-                                                     int cnz, EllBC bc, int W, int total)
+                                                     int cnz, EllBC bc, int W, int pz, int total)
 {
     using T2 = typename Vec2<T>::type;
     constexpr unsigned ES = sizeof(T);
@@ -2279,38 +2284,61 @@ __global__ void __launch_bounds__(256) k_prolong_rows(Lev<T> F, const T *__restr
     const int gw = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (gw >= total) return;
     const int seg = gw % W, t = gw / W;
-    const int j = t % F.ny, k = t / F.ny;
+    const int j = t % F.ny, ch = t / F.ny;
+    const int Kb = ch * (pz >> 1), Ke = min(Kb + (pz >> 1), cnz);
     const int I = lane + 64 * seg, x0 = 2 * I;
     // coarse x neighbours of the pair: I-1 for the even fine cell, I+1 for the odd
     int xa0, xa1, xb0, xb1;
     T wa0, wa1, wb0, wb1;
     lin_w<T>(x0, cnx, bc.k[0][0], bc.k[0][1], xa0, xa1, wa0, wa1);
     lin_w<T>(x0 + 1, cnx, bc.k[0][0], bc.k[0][1], xb0, xb1, wb0, wb1);
-    int yi0, yi1, zi0, zi1;
-    T wy0, wy1, wz0, wz1;
+    int yi0, yi1;
+    T wy0, wy1;
     lin_w<T>(j, cny, bc.k[1][0], bc.k[1][1], yi0, yi1, wy0, wy1);
-    lin_w<T>(k, cnz, bc.k[2][0], bc.k[2][1], zi0, zi1, wz0, wz1);
-    const int yy[2] = {yi0, yi1}, zz[2] = {zi0, zi1};
-    T va[2], vb[2];
-#pragma unroll
-    for (int cz = 0; cz < 2; ++cz) {
+    const int yy[2] = {yi0, yi1};
+    // the pair's x/y-interpolated correction on coarse plane K
+    auto plane = [&](int K) -> T2 {
         T ya[2], yb[2];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy) {
-            const T *row = e + (size_t)cnx * (yy[cy] + (size_t)cny * zz[cz]);
+            const T *row = e + (size_t)cnx * (yy[cy] + (size_t)cny * K);
             const T c0 = ld_off<T>(row, (unsigned)xa0 * ES);   // = coarse I
             ya[cy] = dfma(wa1, ld_off<T>(row, (unsigned)xa1 * ES), wa0 * c0);
             yb[cy] = dfma(wb1, ld_off<T>(row, (unsigned)xb1 * ES), wb0 * c0);
         }
-        va[cz] = dfma(wy1, ya[1], wy0 * ya[0]);
-        vb[cz] = dfma(wy1, yb[1], wy0 * yb[0]);
-    }
+        T2 v;
+        v.x = dfma(wy1, ya[1], wy0 * ya[0]);
+        v.y = dfma(wy1, yb[1], wy0 * yb[0]);
+        return v;
+    };
     const unsigned off = (unsigned)(x0 + F.nx * j) * ES;
-    T *fp = F.phi + (size_t)F.nx * F.ny * k;
-    T2 p = ld_off<T2>(fp, off);
-    p.x = p.x + dfma(wz1, va[1], wz0 * va[0]);
-    p.y = p.y + dfma(wz1, vb[1], wz0 * vb[0]);
-    st_off<T2>(fp, off, p);
+    const size_t fplane = (size_t)F.nx * F.ny;
+    // fine plane k: its parent's correction vp and its second z point's vq
+    auto fine = [&](int k, T w0, T w1, const T2 &vp, const T2 &vq) {
+        T *fp = F.phi + fplane * k;
+        T2 p = ld_off<T2>(fp, off);
+        p.x = p.x + dfma(w1, vq.x, w0 * vp.x);
+        p.y = p.y + dfma(w1, vq.y, w0 * vp.y);
+        st_off<T2>(fp, off, p);
+    };
+    // the second z point of fine plane 2K is coarse plane K-1 (its periodic
+    // image, or K itself at a non-periodic side), that of 2K+1 is K+1
+    int zp, zq;
+    T w0, w1;
+    lin_w<T>(2 * Kb, cnz, bc.k[2][0], bc.k[2][1], zp, zq, w0, w1);
+    T2 vc = plane(Kb);
+    T2 vm = (zq == Kb) ? vc : plane(zq);
+    for (int K = Kb; K < Ke; ++K) {
+        int zp0, zq0, zp1, zq1;
+        T w00, w01, w10, w11;
+        lin_w<T>(2 * K, cnz, bc.k[2][0], bc.k[2][1], zp0, zq0, w00, w01);
+        lin_w<T>(2 * K + 1, cnz, bc.k[2][0], bc.k[2][1], zp1, zq1, w10, w11);
+        const T2 vn = (zq1 == K) ? vc : plane(zq1);
+        fine(2 * K, w00, w01, vc, (zq0 == K) ? vc : vm);
+        fine(2 * K + 1, w10, w11, vc, vn);
+        vm = vc;
+        vc = vn;
+    }
 }
 
 template <typename T>
@@ -3964,11 +3992,15 @@ void launch_restrict(int dim, const T *r, int fnx, int fny, const Lev<T> &C, hip
 template <typename T>
 void launch_prolong(int dim, const Lev<T> &F, const Lev<T> &C, const EllBC &bc, hipStream_t s)
 {
-    if (dim == 3 && F.nx % 128 == 0) {
+    if (dim == 3 && F.nx % 128 == 0 && F.ny == 2 * C.ny && F.nz == 2 * C.nz) {
+        // fine planes per wave: up to 16 while some 8192 waves remain
         const int W = F.nx / 128;
-        const int total = F.ny * F.nz * W;
+        const long rows = (long)W * F.ny;
+        const int pz = (int)std::max(2L, std::min(16L, (rows * F.nz / 8192) & ~1L));
+        const int nch = (F.nz + pz - 1) / pz;
+        const int total = (int)(rows * nch);
         hipLaunchKernelGGL((k_prolong_rows<T>), dim3((total + 3) / 4), dim3(256), 0, s, F, (const T *)C.phi,
-                           C.nx, C.ny, C.nz, bc, W, total);
+                           C.nx, C.ny, C.nz, bc, W, pz, total);
         return;
     }
     DISPATCH_DIM(dim, hipLaunchKernelGGL((k_prolong<D_, T>), dim3(row_blocks(F.ny, F.nz)), dim3(TPB), 0, s,

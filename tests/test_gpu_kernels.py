@@ -196,7 +196,10 @@ def test_residual_restrict_bitwise(gpu, oracle_lib, n, bc):
     assert not plan.debug_get(1, "phi").any()
 
 
-@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40), (128, 12, 8), (256, 20, 12), (128, 36, 10)])
+# nx = 128, 256: the z-marching row prolongation, one coarse plane per wave
+# up to (256, 256, 128), where a wave marches four coarse planes
+@pytest.mark.parametrize("n", [(32, 16, 16), (64, 48, 40), (128, 12, 8), (256, 20, 12), (128, 36, 10),
+                               (256, 128, 128), (256, 256, 128)])
 @pytest.mark.parametrize("bc", BCS_3D)
 def test_prolong_then_sweep_bitwise(gpu, oracle_lib, n, bc):
     O = oracle_lib
