@@ -109,7 +109,8 @@ def test_pointer_tables_agree_with_lists(gpu, cfg, macrhs):
 
 def test_array_counts_are_checked(gpu):
     """A list or pointer table with the wrong number of arrays for the
-    layout's boxes is refused before the library reads it."""
+    layout's boxes, or a call mixing device and host arrays, is refused
+    before the library reads it."""
     from maestro_amd import mac
     from .helpers import layout_for, per_box_inputs
     wl = W.make_workload(1, n=(32, 32, 32), box=16)
@@ -124,6 +125,12 @@ def test_array_counts_are_checked(gpu):
         plan.project([box[:2] for box in umac], phi, rho, None, b0, b0e, None)
     with pytest.raises(ValueError):
         plan.project(mac.PtrTable([a for box in umac for a in box][:-1]), phi, rho, None, b0, b0e, None)
+    # device faces with host phi, rho and beta0: every pointer of a call is of one kind
+    du = [[gpu.from_numpy(a).cuda() for a in box] for box in umac]
+    with pytest.raises(ValueError):
+        plan.project(du, phi, rho, None, b0, b0e, None)
+    with pytest.raises(ValueError):
+        plan.project(mac.PtrTable([a for box in du for a in box]), phi, rho, None, b0, b0e, None)
     plan.close()
 
 
