@@ -9,11 +9,14 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "mac_launch.h"
@@ -48,11 +51,79 @@ struct Hier {
     RhoLev rho0 = {};
 };
 
+// A few host threads that run one parallel copy at a time: run(fn) calls
+// fn(t) for t = 0 .. size()-1 (t = 0 on the calling thread) and returns when
+// every call has returned.  One run at a time, so every worker sees each run.
+class CopyPool {
+  public:
+    explicit CopyPool(int n) : n_(n)
+    {
+        for (int t = 1; t < n_; ++t) th_.emplace_back([this, t] { loop(t); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &x : th_) x.join();
+    }
+    int size() const { return n_; }
+    void run(const std::function<void(int)> &fn)
+    {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            fn_ = &fn;
+            left_ = n_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        fn(0);
+        std::unique_lock<std::mutex> g(m_);
+        done_.wait(g, [this] { return left_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    void loop(int t)
+    {
+        long seen = 0;
+        for (;;) {
+            const std::function<void(int)> *fn;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                fn = fn_;
+            }
+            (*fn)(t);
+            std::lock_guard<std::mutex> g(m_);
+            if (--left_ == 0) done_.notify_one();
+        }
+    }
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    long gen_ = 0;
+    int left_ = 0;
+    bool stop_ = false;
+};
+
 struct HostStage {
     // device mirrors of the caller's per-box host arrays
     std::vector<double *> umac, phi, rho, rhs, cart;
     std::vector<size_t> umac_n, cell_n_phi, cell_n_rho, cell_n_rhs, cart_n;
     bool rhs_ready = false, cart_ready = false, base_ready = false;
+    // pinned staging of the copies: two slots; the host threads fill (or
+    // drain) one while the copy engine moves the other
+    char *slot[2] = {nullptr, nullptr};
+    hipEvent_t moved[2] = {nullptr, nullptr};
+    CopyPool *pool = nullptr;
 };
 
 }  // namespace
@@ -911,6 +982,14 @@ static void free_stage(mac_plan *P)
     };
     fr(P->hs.umac); fr(P->hs.phi); fr(P->hs.rho); fr(P->hs.rhs); fr(P->hs.cart);
     P->hs.base_ready = P->hs.rhs_ready = P->hs.cart_ready = false;
+    for (int i = 0; i < 2; ++i) {
+        if (P->hs.slot[i]) (void)hipHostFree(P->hs.slot[i]);
+        if (P->hs.moved[i]) (void)hipEventDestroy(P->hs.moved[i]);
+        P->hs.slot[i] = nullptr;
+        P->hs.moved[i] = nullptr;
+    }
+    delete P->hs.pool;
+    P->hs.pool = nullptr;
 }
 
 extern "C" int mac_plan_profile(mac_plan *P, int enable)
@@ -990,6 +1069,125 @@ static size_t box_elems(const mac_plan *P, int b, int ng, int fd)
     return v;
 }
 
+// ------------------------------------------- host arrays through pinned slots
+// A host-pointer call's arrays (pageable memory) move through two pinned
+// slots: host threads copy one slot's share of the arrays while the copy
+// engine moves the other slot, so the link runs from pinned memory and the
+// host copies overlap it.  Small calls (below STAGE_MIN bytes) copy each array
+// directly.
+static constexpr size_t STAGE_SLOT = 64ull << 20;
+static constexpr size_t STAGE_MIN = 16ull << 20;
+
+struct Piece {
+    const char *src;
+    char *dst;
+    size_t n;
+};
+
+// copy the pieces with the pool's threads, each an equal share of the bytes
+static void pool_copy(CopyPool &pool, const std::vector<Piece> &pc, size_t total)
+{
+    const int T = pool.size();
+    pool.run([&](int t) {
+        const size_t lo = total * (size_t)t / (size_t)T, hi = total * (size_t)(t + 1) / (size_t)T;
+        size_t at = 0;
+        for (const Piece &p : pc) {
+            const size_t a = std::max(lo, at), b = std::min(hi, at + p.n);
+            if (a < b) memcpy(p.dst + (a - at), p.src + (a - at), b - a);
+            at += p.n;
+            if (at >= hi) break;
+        }
+    });
+}
+
+static int stage_pinned(mac_plan *P)
+{
+    HostStage &S = P->hs;
+    if (S.pool) return MAC_OK;
+    for (int i = 0; i < 2; ++i) {
+        HIPCHK(hipHostMalloc((void **)&S.slot[i], STAGE_SLOT, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&S.moved[i], hipEventDisableTiming));
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    S.pool = new CopyPool((int)std::max(1u, std::min(8u, hw ? hw : 1u)));
+    return MAC_OK;
+}
+
+// in: host -> device (src host, dst device); else device -> host
+static int move_pieces(mac_plan *P, const std::vector<Piece> &jobs, bool in)
+{
+    HostStage &S = P->hs;
+    const hipMemcpyKind kind = in ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+    size_t total = 0;
+    for (const Piece &j : jobs) total += j.n;
+    if (total < STAGE_MIN) {
+        for (const Piece &j : jobs) HIPCHK(hipMemcpyAsync(j.dst, j.src, j.n, kind, P->stream));
+        return MAC_OK;
+    }
+    int rc = stage_pinned(P);
+    if (rc) return rc;
+    // a transfer of an earlier call that returned early may still use a slot
+    for (int i = 0; i < 2; ++i) HIPCHK(hipEventSynchronize(S.moved[i]));
+    // groups of pieces that fill one slot each (an array larger than a slot is split)
+    std::vector<std::vector<Piece>> groups(1);
+    size_t used = 0;
+    for (const Piece &j : jobs) {
+        size_t done = 0;
+        while (done < j.n) {
+            if (used == STAGE_SLOT) {
+                groups.emplace_back();
+                used = 0;
+            }
+            const size_t take = std::min(j.n - done, STAGE_SLOT - used);
+            groups.back().push_back({j.src + done, j.dst + done, take});
+            used += take;
+            done += take;
+        }
+    }
+    // the slot side of a group: pieces between the slot and the host arrays
+    auto host_side = [&](const std::vector<Piece> &g, int s) {
+        std::vector<Piece> hp;
+        size_t off = 0;
+        for (const Piece &p : g) {
+            if (in) hp.push_back({p.src, S.slot[s] + off, p.n});
+            else hp.push_back({S.slot[s] + off, p.dst, p.n});
+            off += p.n;
+        }
+        pool_copy(*S.pool, hp, off);
+    };
+    auto engine = [&](const std::vector<Piece> &g, int s) -> int {
+        size_t off = 0;
+        for (const Piece &p : g) {
+            if (in) HIPCHK(hipMemcpyAsync(p.dst, S.slot[s] + off, p.n, kind, P->stream));
+            else HIPCHK(hipMemcpyAsync(S.slot[s] + off, p.src, p.n, kind, P->stream));
+            off += p.n;
+        }
+        HIPCHK(hipEventRecord(S.moved[s], P->stream));
+        return MAC_OK;
+    };
+    const size_t ng = groups.size();
+    if (in) {
+        // fill slot g & 1 once its previous transfer (group g - 2) has left it
+        for (size_t g = 0; g < ng; ++g) {
+            const int s = (int)(g & 1);
+            if (g >= 2) HIPCHK(hipEventSynchronize(S.moved[s]));
+            host_side(groups[g], s);
+            if ((rc = engine(groups[g], s))) return rc;
+        }
+    } else {
+        // group g lands in slot g & 1 while the host drains group g - 1
+        for (size_t g = 0; g <= ng; ++g) {
+            if (g < ng && (rc = engine(groups[g], (int)(g & 1)))) return rc;
+            if (g >= 1) {
+                const int s = (int)((g - 1) & 1);
+                HIPCHK(hipEventSynchronize(S.moved[s]));
+                host_side(groups[g - 1], s);
+            }
+        }
+    }
+    return MAC_OK;
+}
+
 static int stage_in(mac_plan *P, double *const *umac, double *const *phi, const double *const *rho,
                     const double *const *macrhs, const double *const *cart)
 {
@@ -1020,38 +1218,34 @@ static int stage_in(mac_plan *P, double *const *umac, double *const *phi, const 
                 HIPCHK(hipMalloc(&S.cart[b * dim + d], sizeof(double) * box_elems(P, b, P->ng_cart, d)));
         S.cart_ready = true;
     }
+    std::vector<Piece> jobs;
+    auto add = [&](const double *src, double *dst, long n) {
+        jobs.push_back({(const char *)src, (char *)dst, sizeof(double) * (size_t)n});
+    };
     for (int b = 0; b < nb; ++b) {
-        for (int d = 0; d < dim; ++d)
-            HIPCHK(hipMemcpyAsync(S.umac[b * dim + d], umac[b * dim + d],
-                                  sizeof(double) * box_elems(P, b, P->ng_u, d), hipMemcpyHostToDevice,
-                                  P->stream));
-        HIPCHK(hipMemcpyAsync(S.phi[b], phi[b], sizeof(double) * box_elems(P, b, P->ng_phi, -1),
-                              hipMemcpyHostToDevice, P->stream));
-        HIPCHK(hipMemcpyAsync(S.rho[b], rho[b], sizeof(double) * box_elems(P, b, P->ng_rho, -1),
-                              hipMemcpyHostToDevice, P->stream));
-        if (macrhs)
-            HIPCHK(hipMemcpyAsync(S.rhs[b], macrhs[b], sizeof(double) * box_elems(P, b, P->ng_rhs, -1),
-                                  hipMemcpyHostToDevice, P->stream));
+        for (int d = 0; d < dim; ++d) add(umac[b * dim + d], S.umac[b * dim + d], box_elems(P, b, P->ng_u, d));
+        add(phi[b], S.phi[b], box_elems(P, b, P->ng_phi, -1));
+        add(rho[b], S.rho[b], box_elems(P, b, P->ng_rho, -1));
+        if (macrhs) add(macrhs[b], S.rhs[b], box_elems(P, b, P->ng_rhs, -1));
         if (cart)
-            for (int d = 0; d < dim; ++d)
-                HIPCHK(hipMemcpyAsync(S.cart[b * dim + d], cart[b * dim + d],
-                                      sizeof(double) * box_elems(P, b, P->ng_cart, d),
-                                      hipMemcpyHostToDevice, P->stream));
+            for (int d = 0; d < dim; ++d) add(cart[b * dim + d], S.cart[b * dim + d], box_elems(P, b, P->ng_cart, d));
     }
-    return MAC_OK;
+    return move_pieces(P, jobs, true);
 }
 
 static int stage_out(mac_plan *P, double *const *umac, double *const *phi)
 {
     HostStage &S = P->hs;
+    std::vector<Piece> jobs;
+    auto add = [&](const double *src, double *dst, long n) {
+        jobs.push_back({(const char *)src, (char *)dst, sizeof(double) * (size_t)n});
+    };
     for (int b = 0; b < P->nboxes; ++b) {
-        for (int d = 0; d < P->dim; ++d)
-            HIPCHK(hipMemcpyAsync(umac[b * P->dim + d], S.umac[b * P->dim + d],
-                                  sizeof(double) * box_elems(P, b, P->ng_u, d), hipMemcpyDeviceToHost,
-                                  P->stream));
-        HIPCHK(hipMemcpyAsync(phi[b], S.phi[b], sizeof(double) * box_elems(P, b, P->ng_phi, -1),
-                              hipMemcpyDeviceToHost, P->stream));
+        for (int d = 0; d < P->dim; ++d) add(S.umac[b * P->dim + d], umac[b * P->dim + d], box_elems(P, b, P->ng_u, d));
+        add(S.phi[b], phi[b], box_elems(P, b, P->ng_phi, -1));
     }
+    int rc = move_pieces(P, jobs, false);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(P->stream));
     return MAC_OK;
 }

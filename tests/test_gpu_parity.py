@@ -85,8 +85,12 @@ def test_macproject_matches_oracle(gpu, cfg, n, box, alt, prec):
         assert e_same_phi <= 1e-9 and e_same_u <= 1e-9
 
 
-def test_host_and_device_pointers_agree(gpu):
-    wl = W.make_workload(1, n=(32, 32, 32), box=16)
+# the second size moves about 195 MiB in and 157 MiB out through the two
+# 64 MiB pinned slots of the host-pointer path (four and three slot loads, so
+# each slot is reused); the first copies each array directly
+@pytest.mark.parametrize("n,box", [((32, 32, 32), 16), ((256, 256, 64), 32)])
+def test_host_and_device_pointers_agree(gpu, n, box):
+    wl = W.make_workload(1, n=n, box=box)
     a = run_gpu(wl, device=False)
     b = run_gpu(wl, device=True)
     for d in range(3):
