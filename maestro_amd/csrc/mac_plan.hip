@@ -1898,15 +1898,45 @@ struct CallStage {
         HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, P->stream));
         return MAC_OK;
     }
-    // a device table of device views of per-box arrays
+    // the per-box views back into the caller's host arrays (through the pinned slots)
+    int back_all(double *const *h, const std::vector<double *> &views, const std::vector<size_t> &n)
+    {
+        if (!h || P->opts.device_ptrs) return MAC_OK;
+        std::vector<Piece> pc;
+        for (size_t a = 0; a < n.size(); ++a)
+            if (h[a] && views[a]) pc.push_back({(const char *)views[a], (char *)h[a], sizeof(double) * n[a]});
+        return move_pieces(P, pc, false);
+    }
+    // a device table of device views of per-box arrays (host arrays copied in
+    // through the pinned slots when in)
     int table(const double *const *h, const std::vector<size_t> &n, bool in, std::vector<double *> &views,
               double ***tab)
     {
         views.assign(n.size(), nullptr);
         if (!h) { *tab = nullptr; return MAC_OK; }
         int rc;
-        for (size_t a = 0; a < n.size(); ++a)
-            if ((rc = view(h[a], n[a], in, &views[a]))) return rc;
+        if (P->opts.device_ptrs) {
+            for (size_t a = 0; a < n.size(); ++a) views[a] = const_cast<double *>(h[a]);
+        } else {
+            // one allocation for the table's arrays, each at a 256-byte boundary
+            auto part = [](size_t cnt) { return (sizeof(double) * std::max<size_t>(cnt, 1) + 255) & ~(size_t)255; };
+            size_t total = 0;
+            for (size_t a = 0; a < n.size(); ++a) total += part(n[a]);
+            void *q = nullptr;
+            HIPCHK(hipMalloc(&q, std::max<size_t>(total, 256)));
+            mem.push_back(q);
+            size_t off = 0;
+            for (size_t a = 0; a < n.size(); ++a) {
+                if (h[a]) views[a] = (double *)((char *)q + off);
+                off += part(n[a]);
+            }
+        }
+        if (in && !P->opts.device_ptrs) {
+            std::vector<Piece> pc;
+            for (size_t a = 0; a < n.size(); ++a)
+                if (h[a]) pc.push_back({(const char *)h[a], (char *)views[a], sizeof(double) * n[a]});
+            if ((rc = move_pieces(P, pc, true))) return rc;
+        }
         if ((tab_used + n.size()) * sizeof(double *) > P->call_tab_cap) return MAC_ERR_INVALID;
         double **q = (double **)P->call_tab + tab_used;
         tab_used += n.size();
@@ -2028,10 +2058,8 @@ extern "C" int mac_make_macrhs(mac_plan *P, double *const *macrhs, const mac_mac
     else memset(&A.geo, 0, sizeof A.geo);
     launch_make_macrhs(A, P->maxcells, sph != nullptr, P->stream);
     HIPCHK(hipGetLastError());
-    for (int b = 0; b < nb; ++b) {
-        if ((rc = S.back(macrhs[b], v_rhs[b], n_rhs[b]))) return rc;
-        if (dpdt_on && (rc = S.back(in->delta_chi[b], v_dchi[b], n_dchi[b]))) return rc;
-    }
+    if ((rc = S.back_all(macrhs, v_rhs, n_rhs))) return rc;
+    if (dpdt_on && (rc = S.back_all(in->delta_chi, v_dchi, n_dchi))) return rc;
     HIPCHK(hipStreamSynchronize(P->stream));
     return MAC_OK;
 }
@@ -2070,8 +2098,7 @@ extern "C" int mac_make_s0mac(mac_plan *P, const double *s0, const mac_sphr *sph
     A.geo = sphr_geom(P, sph);
     launch_make_s0mac(A, maxfaces, P->stream);
     HIPCHK(hipGetLastError());
-    for (size_t a = 0; a < n_f.size(); ++a)
-        if ((rc = S.back(s0mac[a], v_f[a], n_f[a]))) return rc;
+    if ((rc = S.back_all(s0mac, v_f, n_f))) return rc;
     HIPCHK(hipStreamSynchronize(P->stream));
     return MAC_OK;
 }
@@ -2224,8 +2251,7 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
                        (const double *)P->H64.lev[0].phi, P->dlo[0], P->dlo[1], P->dlo[2], P->n[0], P->n[1]);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P->ev[4], s));
-    for (int b = 0; b < nb; ++b)
-        if ((rc = S.back(in->phi[b], v_phi[b], n_phi[b]))) return rc;
+    if ((rc = S.back_all(in->phi, v_phi, n_phi))) return rc;
     HIPCHK(hipEventRecord(P->ev[5], s));
     HIPCHK(hipStreamSynchronize(s));
     if (st) {

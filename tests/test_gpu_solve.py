@@ -89,6 +89,8 @@ CASES = [
     ((256, 32, 16), 16, [[PER, PER], [NEU, DIR], [NEU, NEU]], "rand", 0, True),
     ((256, 32, 16), 16, [[PER, PER], [NEU, DIR], [NEU, NEU]], "rand", 1, True),
     ((128, 64, 32), 32, [[PER, PER], [PER, PER], [DIR, DIR]], "rand", 2, True),
+    # host arrays of some 60 MiB: they move through the pinned slots of the host-pointer path
+    ((128, 128, 64), 32, [[PER, PER], [PER, PER], [DIR, NEU]], "rand", 1, False),
 ]
 
 
