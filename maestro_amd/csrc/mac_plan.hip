@@ -1100,17 +1100,30 @@ static void pool_copy(CopyPool &pool, const std::vector<Piece> &pc, size_t total
     });
 }
 
-static int stage_pinned(mac_plan *P)
+// the slots, their events and the copy threads; false where any of them
+// cannot be had (the arrays are then copied directly)
+static bool stage_pinned(mac_plan *P)
 {
     HostStage &S = P->hs;
-    if (S.pool) return MAC_OK;
+    if (S.pool) return true;
     for (int i = 0; i < 2; ++i) {
-        HIPCHK(hipHostMalloc((void **)&S.slot[i], STAGE_SLOT, hipHostMallocDefault));
-        HIPCHK(hipEventCreateWithFlags(&S.moved[i], hipEventDisableTiming));
+        if (!S.slot[i] && hipHostMalloc((void **)&S.slot[i], STAGE_SLOT, hipHostMallocDefault) != hipSuccess) {
+            S.slot[i] = nullptr;
+            return false;
+        }
+        if (!S.moved[i] && hipEventCreateWithFlags(&S.moved[i], hipEventDisableTiming) != hipSuccess) {
+            S.moved[i] = nullptr;
+            return false;
+        }
     }
     const unsigned hw = std::thread::hardware_concurrency();
-    S.pool = new CopyPool((int)std::max(1u, std::min(8u, hw ? hw : 1u)));
-    return MAC_OK;
+    try {
+        S.pool = new CopyPool((int)std::max(1u, std::min(8u, hw ? hw : 1u)));
+    } catch (...) {   // no threads to be had: stay on direct copies
+        S.pool = nullptr;
+        return false;
+    }
+    return true;
 }
 
 // in: host -> device (src host, dst device); else device -> host
@@ -1120,12 +1133,11 @@ static int move_pieces(mac_plan *P, const std::vector<Piece> &jobs, bool in)
     const hipMemcpyKind kind = in ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
     size_t total = 0;
     for (const Piece &j : jobs) total += j.n;
-    if (total < STAGE_MIN) {
+    if (total < STAGE_MIN || !stage_pinned(P)) {
         for (const Piece &j : jobs) HIPCHK(hipMemcpyAsync(j.dst, j.src, j.n, kind, P->stream));
         return MAC_OK;
     }
-    int rc = stage_pinned(P);
-    if (rc) return rc;
+    int rc;
     // a transfer of an earlier call that returned early may still use a slot
     for (int i = 0; i < 2; ++i) HIPCHK(hipEventSynchronize(S.moved[i]));
     // groups of pieces that fill one slot each (an array larger than a slot is split)

@@ -328,14 +328,25 @@ class MacPlan:
         flat_u = table(umac, dim, "umac")
         phi_t, rho_t = table(phi, 1, "phi"), table(rho, 1, "rho")
         rhs_t, flat_c = table(macrhs, 1, "macrhs"), table(beta0_cart_edge, dim, "beta0_cart_edge")
-        kind = lambda t: t.device if isinstance(t, PtrTable) else _is_torch(t[0])
-        dev = kind(flat_u)
         # one kind of array for the whole call: the library reads every pointer
         # as a device pointer or every one as a host pointer
+        mixed = ValueError("mac_project: all arrays must be device tensors or all host arrays")
+
+        def kind(t):
+            if isinstance(t, PtrTable):
+                return t.device
+            if hasattr(t, "shape"):
+                return _is_torch(t)
+            ks = {_is_torch(x) for x in t}
+            if len(ks) != 1:
+                raise mixed
+            return ks.pop()
+
+        dev = kind(flat_u)
         others = [t for t in (phi_t, rho_t, rhs_t, flat_c) if t is not None and len(t)]
         others += [x for x in (beta0_1d, beta0_1d_edge) if x is not None]
-        if any((kind(t) if not hasattr(t, "shape") else _is_torch(t)) != dev for t in others):
-            raise ValueError("mac_project: all arrays must be device tensors or all host arrays")
+        if any(kind(t) != dev for t in others):
+            raise mixed
         if bool(self.opts.device_ptrs) != dev:
             self.set_opts(device_ptrs=int(dev))
         if dev:
