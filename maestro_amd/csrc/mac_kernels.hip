@@ -2889,24 +2889,44 @@ __global__ void __launch_bounds__(CTPB) k_coarse_cycle(const Lev<T> *__restrict_
     const int bot = nlev - 1;
     if (LDS) {
         // stage the weights of every coarse level, and phi/rhs of the first
-        // (deeper phi/rhs are written by the restriction before use)
+        // (deeper phi/rhs are written by the restriction before use).  One
+        // pass per level: a thread issues every load of its cell and slab
+        // entries before it stores any, so each level costs one round of
+        // global latency rather than one per array
         for (int l = lc; l <= bot; ++l) {
             const Lev<T> G = levs[l];
             const Lev<T> S = coarse_level<DIM, T, true>(levs, lc, l, smem);
-            const int n = G.nx * G.ny * G.nz;
-            for (int q = threadIdx.x; q < n; q += blockDim.x) {
-                S.bx[q] = G.bx[q];
-                S.by[q] = G.by[q];
-                if (DIM == 3) S.bz[q] = G.bz[q];
-                if (l == lc) {
-                    S.phi[q] = G.phi[q];
-                    S.rhs[q] = G.rhs[q];
+            const int n = G.nx * G.ny * G.nz, nsx = G.ny * G.nz, nsy = G.nx * G.nz;
+            const int nsz = (DIM == 3) ? G.nx * G.ny : 0;
+            const int m = max(max(n, nsx), max(nsy, nsz));
+            for (int q = threadIdx.x; q < m; q += blockDim.x) {
+                const bool c = q < n, hx = q < nsx, hy = q < nsy, hz = q < nsz;
+                T vx = T(0), vy = T(0), vz = T(0), vp = T(0), vr = T(0), wx = T(0), wy = T(0), wz = T(0);
+                if (c) {
+                    vx = G.bx[q];
+                    vy = G.by[q];
+                    if (DIM == 3) vz = G.bz[q];
+                    if (l == lc) {
+                        vp = G.phi[q];
+                        vr = G.rhs[q];
+                    }
                 }
+                if (hx) wx = G.bxh[q];
+                if (hy) wy = G.byh[q];
+                if (hz) wz = G.bzh[q];
+                if (c) {
+                    S.bx[q] = vx;
+                    S.by[q] = vy;
+                    if (DIM == 3) S.bz[q] = vz;
+                    if (l == lc) {
+                        S.phi[q] = vp;
+                        S.rhs[q] = vr;
+                    }
+                }
+                if (hx) S.bxh[q] = wx;
+                if (hy) S.byh[q] = wy;
+                if (hz) S.bzh[q] = wz;
             }
-            for (int q = threadIdx.x; q < G.ny * G.nz; q += blockDim.x) S.bxh[q] = G.bxh[q];
-            for (int q = threadIdx.x; q < G.nx * G.nz; q += blockDim.x) S.byh[q] = G.byh[q];
-            if (DIM == 3)
-                for (int q = threadIdx.x; q < G.nx * G.ny; q += blockDim.x) S.bzh[q] = G.bzh[q];
         }
         char *p = smem;
         for (int m = lc; m <= bot; ++m) p += lds_level_bytes<DIM, T>(levs[m].nx, levs[m].ny, levs[m].nz);
