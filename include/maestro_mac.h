@@ -21,7 +21,8 @@
  * The pointer handed in for a box addresses element (lo-ng, ...).  Arrays
  * of box pointers are ordered box-major: umac[b*dim + d].
  *
- * Single level only.  The boxes must tile the domain exactly.
+ * mac_project is single level (the boxes must tile the domain exactly);
+ * mac_ml_project takes two levels.
  * Return codes: MAC_OK, or one of MAC_ERR_* below.
  */
 #ifndef MAESTRO_MAC_H
@@ -331,6 +332,36 @@ typedef struct mac_solve_in {
  * right side is removed, phi defined up to a constant) when no side is
  * Dirichlet and alpha is NULL or zero everywhere. */
 int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
+
+/* ---- two-level (AMR) projection ------------------------------------------
+ * macproject(mla, umac, phi, rho, dx, the_bc_tower[, macrhs]
+ *            [, beta0_1d, beta0_1d_edge]) with nlevs = 2, refinement ratio 2
+ *   Source/macproject.f90:51-328 (the nlevs > 1 branches: :141-143 via
+ *   divumac :357-361 and :401-406, :217, :256-275, mkumac :829-833)
+ * layouts[0]: the coarse level, boxes tiling its domain (which starts at
+ * 0); layouts[1]: the fine level, domain the coarse one refined by 2, same
+ * physical BCs and ghost conventions, boxes tiling one rectangle that
+ * covers whole coarse cells.  The composite problem (FBoxLib ml_cc_solve,
+ * absent from the reference tree; Docs/mg/mg.tex:9-19,124-156): fine cells
+ * next to a coarse-fine face see the ghost value 8/15 C + 2/3 phi_0 - 1/5
+ * phi_1 (C: the coarse value across the face moved to the ghost's
+ * transverse position with linear slopes), coarse cells there take the mean
+ * of the fine fluxes.  Solved by fast adaptive composite iteration with the
+ * two levels' V-cycles to min(eps_rel * 10, 1e-8) relative (:217).  Device
+ * pointers only (opts.device_ptrs = 1); beta0_cart_edge (spherical) is not
+ * taken.  Per level l: umac[l][box*dim + d], phi[l][box], rho[l][box]
+ * (one filled ghost layer), macrhs[l][box] or macrhs NULL, beta0_1d[l]
+ * and beta0_1d_edge[l] over level l's whole radial extent or both NULL.
+ * Returned: both levels' umac (ghost faces filled: the coarse level as by
+ * mac_project, the fine level from its boxes, the coarse faces beyond the
+ * refined region and the physical rules) and phi (covered coarse cells the
+ * mean of their fine cells); stats.iters = composite steps. */
+typedef struct mac_ml_plan mac_ml_plan;
+int mac_ml_plan_create(int nlevs, const mac_layout *layouts, const mac_opts *opts, mac_ml_plan **plan);
+int mac_ml_plan_destroy(mac_ml_plan *plan);
+int mac_ml_project(mac_ml_plan *plan, double *const *const *umac, double *const *const *phi,
+                   const double *const *const *rho, const double *const *const *macrhs,
+                   const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *stats);
 
 /* ---- diagnostics (no counterpart in the reference) ----------------------
  * Read or overwrite an internal FP64 array of MG level `level` (0 = finest)

@@ -26,7 +26,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-SOURCES = ["mac_kernels.hip", "mac_inputs.hip", "mac_solve.hip", "mac_plan.hip"]
+SOURCES = ["mac_kernels.hip", "mac_inputs.hip", "mac_solve.hip", "mac_ml.hip", "mac_plan.hip"]
 
 
 def hipcc() -> str:

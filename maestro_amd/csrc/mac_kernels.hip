@@ -4078,17 +4078,23 @@ void launch_correct(const CorrectArgs &A, int maxcols, int maxez, int maxghost, 
     DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_correct_valid<D_>,
                                            dim3(nblk(maxcols), A.nboxes, zchunks(A.dim, maxez)), dim3(TPB), 0,
                                            s, A));
-    if (A.ng_u > 0) {
+    launch_fill_ghosts(A, maxghost, true, true, s);
+}
+
+void launch_fill_ghosts(const CorrectArgs &A, int maxghost, bool fill, bool phys, hipStream_t s)
+{
+    if (A.ng_u <= 0) return;
+    if (fill)
         DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_fill_ghost<D_>, dim3(nblk(maxghost), A.nboxes, A.dim),
                                                dim3(TPB), 0, s, A));
-        for (int d = 0; d < A.dim; ++d)
-            for (int side = 0; side < 2; ++side) {
-                int phys = A.phys[d][side];
-                if (phys == MAC_PERIODIC || phys == MAC_INTERIOR) continue;
-                DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_physbc<D_>, dim3(nblk(A.maxplane), A.nboxes, A.dim),
-                                                       dim3(TPB), 0, s, A, d, side, phys));
-            }
-    }
+    if (!phys) return;
+    for (int d = 0; d < A.dim; ++d)
+        for (int side = 0; side < 2; ++side) {
+            int pc = A.phys[d][side];
+            if (pc == MAC_PERIODIC || pc == MAC_INTERIOR) continue;
+            DISPATCH_DIM(A.dim, hipLaunchKernelGGL(k_physbc<D_>, dim3(nblk(A.maxplane), A.nboxes, A.dim),
+                                                   dim3(TPB), 0, s, A, d, side, pc));
+        }
 }
 
 void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s)

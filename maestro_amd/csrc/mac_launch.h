@@ -148,6 +148,71 @@ struct SolveArgs {
 };
 
 void launch_solve_setup(const SolveArgs &A, int maxcells, hipStream_t s);
+// ---- two-level projection (mac_ml.hip) ------------------------------------
+// region side classes: periodic (the region spans a periodic direction), on a
+// physical side of the domain, or coarse-fine
+enum { ML_SIDE_PER = 0, ML_SIDE_PHYS = 1, ML_SIDE_CF = 2 };
+struct MlGeom {
+    int dim;
+    int nc[3];               // coarse cells (1 beyond dim); the domain starts at 0
+    int rlo[3], nr[3];       // refined rectangle: lowest fine cell, fine extents
+    double ihc[3], ihf[3];   // 1/h of the coarse and the fine level
+    int ckind[3][2];         // elliptic kind of each physical side (ELL_*)
+    int side[3][2];          // ML_SIDE_* of each side of the region
+};
+// whole-level arrays the composite operator reads
+struct MlState {
+    MlGeom G;
+    const double *bc[3], *bf[3];   // face coefficients (beta0^p / rho form)
+    const double *rhc, *rhf;       // right sides
+    const double *phc, *phf;       // iterate
+};
+// caller's boxes of one level -> whole-level arrays (see k_ml_gather)
+struct MlGather {
+    const BoxDev *boxes;
+    int org[3], n[3];        // the level array's lowest cell and extents
+    int ng_u, ng_rho, ng_rhs, ng_phi;
+    double *const *umac;     // [box*dim + d]
+    const double *const *rho;
+    const double *const *macrhs;   // null: none
+    double *const *phi;
+    const double *b0, *b0e;  // the level's planar beta0 over its whole domain; null: none
+    int alt;
+    double *u[3], *beta[3], *rhs, *phi_out;
+    int *err;
+};
+struct MlScatter {
+    const BoxDev *boxes;
+    int org[3], n[3];
+    int ng_u, ng_phi;
+    double *const *umac;
+    double *const *phi;
+    const double *u[3];
+    const double *phi_in;
+};
+struct MlGrown {
+    MlGeom G;
+    const BoxDev *boxes;
+    int ng_u;
+    double *const *umac;     // the fine level's boxes
+    const double *uc[3];     // final coarse faces (whole level)
+};
+void launch_ml_gather(int dim, const MlGather &A, int nboxes, int maxcells, hipStream_t s);
+void launch_ml_edge_restrict(const MlGeom &G, double *const uc[3], const double *const uf[3], hipStream_t s);
+void launch_ml_div(int dim, const int n[3], const double dx[3], const double *const u[3], const double *macrhs,
+                   double *rh, hipStream_t s);
+void launch_ml_cell_restrict(const MlGeom &G, double *xc, const double *xf, hipStream_t s);
+void launch_ml_comp_vec(const MlGeom &G, const double *rhc, const double *rhf, double *out, hipStream_t s);
+void launch_ml_resid(const MlState &S, double *rc, double *rf, unsigned long long *nrm, bool fine_only,
+                     hipStream_t s);
+void launch_ml_prolong(const MlGeom &G, const double *e, double *phc, double *phf, hipStream_t s);
+void launch_ml_axpy(long n, const double *x, double *y, hipStream_t s);
+void launch_ml_mkumac(const MlState &S, double *const uc[3], double *const uf[3], hipStream_t s);
+void launch_ml_unscale(int dim, const int n[3], int org_r, double *const u[3], const double *b0, const double *b0e,
+                       hipStream_t s);
+void launch_ml_scatter(int dim, const MlScatter &A, int nboxes, int maxfaces, hipStream_t s);
+void launch_ml_grown(int dim, const MlGrown &A, int nboxes, int maxgrown, hipStream_t s);
+
 
 void launch_cell_to_edge(const double *c, int nr, double *e, hipStream_t s);
 void launch_make_macrhs(const MacrhsArgs &A, int maxcells, bool sph, hipStream_t s);
@@ -231,6 +296,10 @@ template <typename T>
 void launch_coarse_cycle(int dim, const Lev<T> *levs_dev, const Lev<T> *levs_host, int lc, int nlev,
                          const CoarseOpts &o, double *cg, hipStream_t s);
 void launch_correct(const CorrectArgs &A, int maxcols, int maxez, int maxghost, hipStream_t s);
+// the ghost faces of the caller's boxes: box and periodic images (fill),
+// then the physical edge-velocity rules of every non-periodic side whose
+// phys code is not MAC_INTERIOR (phys)
+void launch_fill_ghosts(const CorrectArgs &A, int maxghost, bool fill, bool phys, hipStream_t s);
 void launch_level_setup(int dim, const LevelIO &A, const Lev<double> &L, double *part, hipStream_t s);
 void launch_level_correct(int dim, const LevelIO &A, const Lev<double> &L, hipStream_t s);
 void launch_adj_rhs(int dim, const AdjIO &A, const Lev<double> &L, double *part, hipStream_t s);

@@ -1,0 +1,644 @@
+// mac_ml.hip -- kernels of the two-level (AMR) MAC projection, MAESTRO's
+// macproject with nlevs = 2 (Source/macproject.f90:51-328):
+//   k_ml_gather     the caller's boxes of one level -> whole-level arrays:
+//                   beta0 U (:154-158), beta = beta0^p / rho (mk_mac_coeffs
+//                   :698-738, then :178-193), macrhs and the warm start
+//   k_ml_edge_restrict  ml_edge_restriction (:357-361, :829-833): coarse
+//                   faces under the fine region <- mean of the fine faces
+//   k_ml_div        divumac (:457-480): rh = macrhs - D per level
+//   k_ml_cell_restrict  ml_cc_restriction (:401-404): covered coarse cells
+//                   <- mean of their fine cells
+//   k_ml_resid      residual of the composite operator (FBoxLib ml_cc_solve,
+//                   Docs/mg/mg.tex:9-19,124-156) and its max norm
+//   k_ml_prolong    the coarse correction of one composite iteration, added
+//                   on both levels
+//   k_ml_mkumac_*   mkumac (:908-975) with the composite fluxes at the
+//                   coarse-fine (C-F) faces
+//   k_ml_unscale, k_ml_scatter, k_ml_grown  U / beta0 (:229-240), the
+//                   caller's boxes, and the fine ghost faces outside the
+//                   fine region from the coarse faces (create_umac_grown)
+//
+// Whole-level arrays are C order, x fastest, no ghosts: coarse cells nc,
+// coarse faces nc + e_d; fine arrays cover the refined rectangle only
+// (region cells nr, faces nr + e_d), whose lowest fine cell is rlo.
+//
+// The composite operator at a C-F face: the fine cell f0 next to it sees
+// the ghost value
+//     g = 8/15 Cint + 2/3 phi(f0) - 1/5 phi(f1)
+// (the quadratic along the face normal through the coarse value, one fine
+// cell beyond the face, and the first two fine cells inside), Cint the
+// coarse cell across the face moved to the ghost's transverse position with
+// its linear slopes (central differences; one-sided next to a wall).  The
+// fine flux is beta (phi(f0) - g) / h_f (sign by side); the coarse cell
+// across takes the mean of the fine fluxes through its face.  FBoxLib's own
+// interpolation is not in the reference tree; this one is second order.
+#include <hip/hip_runtime.h>
+
+#include "mac_launch.h"
+
+namespace mac {
+
+static constexpr int MTPB = 256;
+static constexpr double W_COARSE = 8.0 / 15.0, W_FINE0 = 2.0 / 3.0, W_FINE1 = -1.0 / 5.0;
+
+static inline unsigned mblk(long n) { return (unsigned)((n + MTPB - 1) / MTPB); }
+
+__device__ __forceinline__ long nat_cell(const int *n, int i, int j, int k)
+{
+    return i + (long)n[0] * (j + (long)n[1] * k);
+}
+__device__ __forceinline__ long nat_face(const int *n, int d, int i, int j, int k)
+{
+    const long ex = n[0] + (d == 0), ey = n[1] + (d == 1);
+    return i + ex * (j + ey * (long)k);
+}
+// offset of global index (i,j,k) in a caller's box array (ghost width ng,
+// face direction fd or -1 for cells)
+__device__ __forceinline__ long ml_boxoff(const BoxDev &B, int ng, int fd, int dim, int i, int j, int k)
+{
+    const long ex = B.hi[0] - B.lo[0] + 1 + 2 * ng + (fd == 0);
+    const long ey = B.hi[1] - B.lo[1] + 1 + 2 * ng + (fd == 1);
+    const long li = i - B.lo[0] + ng, lj = j - B.lo[1] + ng;
+    const long lk = (dim == 3) ? (k - B.lo[2] + ng) : 0;
+    return li + ex * (lj + ey * lk);
+}
+
+// coarse neighbour of c along d: false beyond a non-periodic side
+__device__ __forceinline__ bool ml_cnb(const MlGeom &G, int c[3], int d, int step)
+{
+    c[d] += step;
+    if (c[d] >= 0 && c[d] < G.nc[d]) return true;
+    if (G.ckind[d][0] != ELL_PER) return false;
+    c[d] = (c[d] + G.nc[d]) % G.nc[d];
+    return true;
+}
+__device__ __forceinline__ bool ml_covered(const MlGeom &G, const int c[3])
+{
+    for (int d = 0; d < G.dim; ++d)
+        if (c[d] < (G.rlo[d] >> 1) || c[d] > ((G.rlo[d] + G.nr[d] - 1) >> 1)) return false;
+    return true;
+}
+__device__ __forceinline__ bool ml_in_region(const MlGeom &G, const int f[3])
+{
+    for (int d = 0; d < G.dim; ++d)
+        if (f[d] < G.rlo[d] || f[d] >= G.rlo[d] + G.nr[d]) return false;
+    return true;
+}
+__device__ __forceinline__ double ml_phic(const MlState &S, const int c[3])
+{
+    return S.phc[nat_cell(S.G.nc, c[0], c[1], c[2])];
+}
+__device__ __forceinline__ double ml_phif(const MlState &S, const int f[3])
+{
+    return S.phf[nat_cell(S.G.nr, f[0] - S.G.rlo[0], f[1] - S.G.rlo[1], f[2] - S.G.rlo[2])];
+}
+
+// the coarse value of c moved by sig[t] coarse cells along each transverse
+// direction t != d (d = -1: every direction), linear slopes of x
+__device__ double ml_lin(const MlGeom &G, const double *x, const int c[3], int d, const double sig[3])
+{
+    const double xc = x[nat_cell(G.nc, c[0], c[1], c[2])];
+    double v = xc;
+    for (int t = 0; t < G.dim; ++t) {
+        if (t == d) continue;
+        int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
+        const bool hp = ml_cnb(G, cp, t, 1), hm = ml_cnb(G, cm, t, -1);
+        double sl = 0.0;
+        if (hp && hm) sl = 0.5 * (x[nat_cell(G.nc, cp[0], cp[1], cp[2])] - x[nat_cell(G.nc, cm[0], cm[1], cm[2])]);
+        else if (hp) sl = x[nat_cell(G.nc, cp[0], cp[1], cp[2])] - xc;
+        else if (hm) sl = xc - x[nat_cell(G.nc, cm[0], cm[1], cm[2])];
+        v = dfma(sig[t], sl, v);
+    }
+    return v;
+}
+
+// flux beta d(phi)/dx_d (positive along +d) through the face on side s of
+// fine cell f (global fine indices)
+__device__ double ml_fine_flux(const MlState &S, const int f[3], int d, int s)
+{
+    const MlGeom &G = S.G;
+    int fr[3] = {f[0] - G.rlo[0], f[1] - G.rlo[1], f[2] - G.rlo[2]};
+    fr[d] += s;
+    const double beta = S.bf[d][nat_face(G.nr, d, fr[0], fr[1], fr[2])];
+    const double ih = G.ihf[d];
+    int nb[3] = {f[0], f[1], f[2]};
+    nb[d] += s ? 1 : -1;
+    const double p0 = ml_phif(S, f);
+    if (!ml_in_region(G, nb)) {
+        const int cls = G.side[d][s];
+        if (cls == ML_SIDE_PER) {
+            nb[d] = G.rlo[d] + (nb[d] - G.rlo[d] + G.nr[d]) % G.nr[d];
+        } else if (cls == ML_SIDE_PHYS) {
+            if (G.ckind[d][s] == ELL_DIR) return (s ? -2.0 : 2.0) * beta * ih * p0;
+            return 0.0;
+        } else {
+            // coarse-fine face
+            const int inward = s ? -1 : 1;
+            int f1[3] = {f[0], f[1], f[2]};
+            f1[d] += inward;
+            int C[3] = {f[0] >> 1, f[1] >> 1, f[2] >> 1};
+            ml_cnb(G, C, d, -inward);
+            double sig[3];
+            for (int t = 0; t < 3; ++t) sig[t] = (f[t] & 1) ? 0.25 : -0.25;
+            const double ci = ml_lin(G, S.phc, C, d, sig);
+            const double g = dfma(W_COARSE, ci, dfma(W_FINE0, p0, W_FINE1 * ml_phif(S, f1)));
+            return s ? beta * ih * (g - p0) : beta * ih * (p0 - g);
+        }
+    }
+    const double pn = ml_phif(S, nb);
+    return s ? beta * ih * (pn - p0) : beta * ih * (p0 - pn);
+}
+
+// flux through the face on side s of uncovered coarse cell c
+__device__ double ml_coarse_flux(const MlState &S, const int c[3], int d, int s)
+{
+    const MlGeom &G = S.G;
+    int fc[3] = {c[0], c[1], c[2]};
+    fc[d] += s;
+    const double beta = S.bc[d][nat_face(G.nc, d, fc[0], fc[1], fc[2])];
+    const double ih = G.ihc[d];
+    int nb[3] = {c[0], c[1], c[2]};
+    const double p0 = ml_phic(S, c);
+    if (!ml_cnb(G, nb, d, s ? 1 : -1)) {
+        if (G.ckind[d][s] == ELL_DIR) return (s ? -2.0 : 2.0) * beta * ih * p0;
+        return 0.0;
+    }
+    if (!ml_covered(G, nb)) {
+        const double pn = ml_phic(S, nb);
+        return s ? beta * ih * (pn - p0) : beta * ih * (p0 - pn);
+    }
+    // C-F face: the mean of the fine fluxes through it, from the fine cells
+    // just inside the region (their face on the far side from c)
+    const int fs = s ? 0 : 1;
+    const int fd = s ? 2 * nb[d] : 2 * nb[d] + 1;
+    const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
+    const int cnt = (G.dim == 3) ? 4 : 2;
+    double F = 0.0;
+    for (int m = 0; m < cnt; ++m) {
+        int f[3];
+        f[d] = fd;
+        f[t0] = 2 * c[t0] + (m & 1);
+        f[t1] = (G.dim == 3) ? 2 * c[t1] + (m >> 1) : 0;
+        F += ml_fine_flux(S, f, d, fs);
+    }
+    return F / cnt;
+}
+
+// ---------------------------------------------------------------- gather
+// One thread per valid cell of each caller box of a level: the low face of
+// the cell in every direction (and the high face where the cell ends the
+// level's rectangle): beta0 U and the face coefficient 2 / (rho + rho_nb)
+// times beta0^p; macrhs and phi's valid cells.  Each face of the rectangle
+// is written by exactly one box.
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_gather(MlGather A)
+{
+    const int b = blockIdx.y;
+    const BoxDev B = A.boxes[b];
+    const int ex = B.hi[0] - B.lo[0] + 1, ey = B.hi[1] - B.lo[1] + 1;
+    const int ez = (DIM == 3) ? B.hi[2] - B.lo[2] + 1 : 1;
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)ex * ey * ez) return;
+    const int g[3] = {B.lo[0] + (int)(q % ex), B.lo[1] + (int)((q / ex) % ey),
+                      (DIM == 3) ? B.lo[2] + (int)(q / ((long)ex * ey)) : 0};
+    const int r[3] = {g[0] - A.org[0], g[1] - A.org[1], g[2] - A.org[2]};
+    const int rd = DIM - 1;
+    bool bad = false;
+    const double *rho = A.rho[b];
+    const double rc = rho[ml_boxoff(B, A.ng_rho, -1, DIM, g[0], g[1], g[2])];
+    bad |= !isfinite(rc);
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double *u = A.umac[b * DIM + d];
+        for (int s = 0; s < 2; ++s) {
+            if (s == 1 && r[d] != A.n[d] - 1) break;
+            int f[3] = {g[0], g[1], g[2]};
+            f[d] += s;
+            int nb[3] = {g[0], g[1], g[2]};
+            nb[d] += s ? 1 : -1;
+            double fac = 1.0;
+            if (A.b0) fac = (d == rd) ? A.b0e[f[rd]] : A.b0[f[rd]];
+            const double uv = u[ml_boxoff(B, A.ng_u, d, DIM, f[0], f[1], f[2])];
+            const double rn = rho[ml_boxoff(B, A.ng_rho, -1, DIM, nb[0], nb[1], nb[2])];
+            double beta = 2.0 / (rc + rn);
+            if (A.b0) {
+                beta = beta * fac;
+                if (A.alt) beta = beta * fac;
+            }
+            bad |= !isfinite(uv) || !isfinite(beta);
+            const long o = nat_face(A.n, d, r[0] + (d == 0) * s, r[1] + (d == 1) * s, r[2] + (d == 2) * s);
+            A.u[d][o] = A.b0 ? uv * fac : uv;
+            A.beta[d][o] = beta;
+        }
+    }
+    const long c = nat_cell(A.n, r[0], r[1], r[2]);
+    if (A.macrhs) {
+        const double m = A.macrhs[b][ml_boxoff(B, A.ng_rhs, -1, DIM, g[0], g[1], g[2])];
+        bad |= !isfinite(m);
+        A.rhs[c] = m;
+    }
+    const double p = A.phi[b][ml_boxoff(B, A.ng_phi, -1, DIM, g[0], g[1], g[2])];
+    bad |= !isfinite(p);
+    A.phi_out[c] = p;
+    if (bad) atomicOr(A.err, 1);
+}
+
+// --------------------------------------------------------- level transfers
+// coarse faces under the region (its boundary faces included) <- mean of the
+// 2^(dim-1) fine faces; where the region meets a periodic seam without
+// spanning the direction, the face's periodic image as well
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_edge_restrict(MlGeom G, double *uc0, double *uc1, double *uc2,
+                                                          const double *uf0, const double *uf1, const double *uf2)
+{
+    const int d = blockIdx.y;
+    double *uc = d == 0 ? uc0 : (d == 1 ? uc1 : uc2);
+    const double *uf = d == 0 ? uf0 : (d == 1 ? uf1 : uf2);
+    int m[3];
+    for (int e = 0; e < 3; ++e) m[e] = (e < DIM) ? (G.nr[e] >> 1) + (e == d) : 1;
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)m[0] * m[1] * m[2]) return;
+    const int l[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    const int t0 = (d == 0) ? 1 : 0, t1 = (d == 2) ? 1 : 2;
+    const int cnt = (DIM == 3) ? 4 : 2;
+    double s = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        int f[3];
+        f[d] = 2 * l[d];
+        f[t0] = 2 * l[t0] + (k & 1);
+        f[t1] = (DIM == 3) ? 2 * l[t1] + (k >> 1) : 0;
+        s += uf[nat_face(G.nr, d, f[0], f[1], f[2])];
+    }
+    const double v = s / cnt;
+    int c[3] = {(G.rlo[0] >> 1) + l[0], (G.rlo[1] >> 1) + l[1], (DIM == 3) ? (G.rlo[2] >> 1) + l[2] : 0};
+    uc[nat_face(G.nc, d, c[0], c[1], c[2])] = v;
+    if (G.ckind[d][0] == ELL_PER && G.side[d][0] == ML_SIDE_CF) {
+        if (c[d] == 0) c[d] = G.nc[d];
+        else if (c[d] == G.nc[d]) c[d] = 0;
+        else return;
+        uc[nat_face(G.nc, d, c[0], c[1], c[2])] = v;
+    }
+}
+
+// rh = macrhs - D (rh = -D without macrhs), whole level
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_div(int n0, int n1, int n2, double dx0, double dx1, double dx2,
+                                                 const double *u, const double *v, const double *w,
+                                                 const double *macrhs, double *rh)
+{
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    const long nc = (long)n0 * n1 * n2;
+    if (q >= nc) return;
+    const int i = (int)(q % n0), j = (int)((q / n0) % n1), k = (int)(q / ((long)n0 * n1));
+    const long fx = i + (long)(n0 + 1) * (j + (long)n1 * k);
+    const long fy = i + (long)n0 * (j + (long)(n1 + 1) * k);
+    double D = (u[fx + 1] - u[fx]) / dx0 + (v[fy + n0] - v[fy]) / dx1;
+    if (DIM == 3) D = D + (w[q + (long)n0 * n1] - w[q]) / dx2;
+    rh[q] = macrhs ? -(D - macrhs[q]) : -D;
+}
+
+// covered coarse cells <- mean of their 2^dim fine cells
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_cell_restrict(MlGeom G, double *xc, const double *xf)
+{
+    const int m[3] = {G.nr[0] >> 1, G.nr[1] >> 1, (DIM == 3) ? G.nr[2] >> 1 : 1};
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)m[0] * m[1] * m[2]) return;
+    const int l[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    double s = 0.0;
+    for (int k = 0; k < (DIM == 3 ? 2 : 1); ++k)
+        for (int j = 0; j < 2; ++j)
+            for (int i = 0; i < 2; ++i)
+                s += xf[nat_cell(G.nr, 2 * l[0] + i, 2 * l[1] + j, (DIM == 3) ? 2 * l[2] + k : 0)];
+    xc[nat_cell(G.nc, (G.rlo[0] >> 1) + l[0], (G.rlo[1] >> 1) + l[1], (DIM == 3) ? (G.rlo[2] >> 1) + l[2] : 0)] =
+        s / (DIM == 3 ? 8 : 4);
+}
+
+// the composite vector whose sum is the volume-weighted sum of rh over the
+// composite cells (fine-cell units): 2^dim rh on uncovered coarse cells, 0
+// on covered ones, then the fine cells
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_comp_vec(MlGeom G, const double *rhc, const double *rhf, double *out)
+{
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
+    if (q >= ncc + nfc) return;
+    if (q < ncc) {
+        const int c[3] = {(int)(q % G.nc[0]), (int)((q / G.nc[0]) % G.nc[1]), (int)(q / ((long)G.nc[0] * G.nc[1]))};
+        out[q] = ml_covered(G, c) ? 0.0 : (DIM == 3 ? 8.0 : 4.0) * rhc[q];
+    } else {
+        out[q] = rhf[q - ncc];
+    }
+}
+
+// ------------------------------------------------------ composite residual
+// r = rh - A phi on the uncovered coarse cells (into rc; covered cells left
+// alone) and the fine cells (into rf), max |r| into *nrm.  fine_only: the
+// fine cells only.
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double *rf, unsigned long long *nrm,
+                                                   int fine_only)
+{
+    const MlGeom &G = S.G;
+    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
+    long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (fine_only) q += ncc;
+    double a = 0.0;
+    if (q < ncc) {
+        const int c[3] = {(int)(q % G.nc[0]), (int)((q / G.nc[0]) % G.nc[1]), (int)(q / ((long)G.nc[0] * G.nc[1]))};
+        if (!ml_covered(G, c)) {
+            double Ap = 0.0;
+            for (int d = 0; d < DIM; ++d)
+                Ap -= (ml_coarse_flux(S, c, d, 1) - ml_coarse_flux(S, c, d, 0)) * G.ihc[d];
+            const double r = S.rhc[q] - Ap;
+            rc[q] = r;
+            a = fabs(r);
+        }
+    } else if (q < ncc + nfc) {
+        const long p = q - ncc;
+        const int f[3] = {G.rlo[0] + (int)(p % G.nr[0]), G.rlo[1] + (int)((p / G.nr[0]) % G.nr[1]),
+                          G.rlo[2] + (int)(p / ((long)G.nr[0] * G.nr[1]))};
+        double Ap = 0.0;
+        for (int d = 0; d < DIM; ++d) Ap -= (ml_fine_flux(S, f, d, 1) - ml_fine_flux(S, f, d, 0)) * G.ihf[d];
+        const double r = S.rhf[p] - Ap;
+        rf[p] = r;
+        a = fabs(r);
+    }
+    if (!isfinite(a)) a = __builtin_huge_val();
+    a = wave_max(a);
+    if ((threadIdx.x & 63) == 0 && a > 0.0) atomicMax(nrm, (unsigned long long)__double_as_longlong(a));
+}
+
+// the coarse correction e (whole coarse level) added to phi on both levels:
+// coarse cells directly, fine cells through the linear interpolation of e
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_prolong(MlGeom G, const double *e, double *phc, double *phf)
+{
+    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q < ncc) {
+        phc[q] += e[q];
+    } else if (q < ncc + nfc) {
+        const long p = q - ncc;
+        const int f[3] = {G.rlo[0] + (int)(p % G.nr[0]), G.rlo[1] + (int)((p / G.nr[0]) % G.nr[1]),
+                          G.rlo[2] + (int)(p / ((long)G.nr[0] * G.nr[1]))};
+        const int C[3] = {f[0] >> 1, f[1] >> 1, f[2] >> 1};
+        double sig[3];
+        for (int t = 0; t < 3; ++t) sig[t] = (f[t] & 1) ? 0.25 : -0.25;
+        phf[p] += ml_lin(G, e, C, -1, sig);
+    }
+}
+
+__global__ void __launch_bounds__(MTPB) k_ml_axpy(long n, const double *x, double *y)
+{
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q < n) y[q] += x[q];
+}
+
+// ------------------------------------------------------------------ mkumac
+// fine faces: U -= beta G phi with the composite flux (C-F ghosts at the
+// region's C-F faces, the physical rule on its physical sides)
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_mkumac_fine(MlState S, double *u0, double *u1, double *u2)
+{
+    const MlGeom &G = S.G;
+    const int d = blockIdx.y;
+    double *u = d == 0 ? u0 : (d == 1 ? u1 : u2);
+    int m[3];
+    for (int e = 0; e < 3; ++e) m[e] = G.nr[e] + (e == d);
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)m[0] * m[1] * m[2]) return;
+    int r[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    // the face is the low face of cell r, or the high face of the last cell
+    int s = 0;
+    if (r[d] == G.nr[d]) { r[d] -= 1; s = 1; }
+    const int f[3] = {G.rlo[0] + r[0], G.rlo[1] + r[1], G.rlo[2] + r[2]};
+    u[q] -= ml_fine_flux(S, f, d, s);
+}
+
+// coarse faces: U -= beta G phi with the single-level boundary rule (faces
+// under the region are overwritten by the edge restriction afterwards)
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_mkumac_coarse(MlState S, double *u0, double *u1, double *u2)
+{
+    const MlGeom &G = S.G;
+    const int d = blockIdx.y;
+    double *u = d == 0 ? u0 : (d == 1 ? u1 : u2);
+    int m[3];
+    for (int e = 0; e < 3; ++e) m[e] = G.nc[e] + (e == d);
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)m[0] * m[1] * m[2]) return;
+    int c[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    int s = 0;
+    if (c[d] == G.nc[d]) { c[d] -= 1; s = 1; }
+    const double beta = S.bc[d][q];
+    const double ih = G.ihc[d];
+    const double p0 = ml_phic(S, c);
+    int nb[3] = {c[0], c[1], c[2]};
+    double F;
+    if (!ml_cnb(G, nb, d, s ? 1 : -1)) {
+        F = (G.ckind[d][s] == ELL_DIR) ? (s ? -2.0 : 2.0) * beta * ih * p0 : 0.0;
+    } else {
+        const double pn = ml_phic(S, nb);
+        F = s ? beta * ih * (pn - p0) : beta * ih * (p0 - pn);
+    }
+    u[q] -= F;
+}
+
+// U / beta0 (the level's planar profile) on a whole-level face array
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_unscale(int d, int n0, int n1, int n2, int org_r, double *u,
+                                                     const double *b0, const double *b0e)
+{
+    int m[3] = {n0, n1, n2};
+    m[d] += 1;
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)m[0] * m[1] * m[2]) return;
+    const int rd = DIM - 1;
+    const int idx[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    const int r = org_r + idx[rd];
+    u[q] = u[q] / ((d == rd) ? b0e[r] : b0[r]);
+}
+
+// whole-level arrays -> the caller's boxes: every valid face and cell
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_scatter(MlScatter A)
+{
+    const int b = blockIdx.y, d = blockIdx.z;   // d == DIM: the cells (phi)
+    const BoxDev B = A.boxes[b];
+    int e[3];
+    for (int t = 0; t < 3; ++t) e[t] = (t < DIM) ? B.hi[t] - B.lo[t] + 1 + (t == d) : 1;
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)e[0] * e[1] * e[2]) return;
+    const int g[3] = {B.lo[0] + (int)(q % e[0]), B.lo[1] + (int)((q / e[0]) % e[1]),
+                      (DIM == 3) ? B.lo[2] + (int)(q / ((long)e[0] * e[1])) : 0};
+    const int r[3] = {g[0] - A.org[0], g[1] - A.org[1], g[2] - A.org[2]};
+    if (d == DIM) {
+        A.phi[b][ml_boxoff(B, A.ng_phi, -1, DIM, g[0], g[1], g[2])] = A.phi_in[nat_cell(A.n, r[0], r[1], r[2])];
+    } else {
+        A.umac[b * DIM + d][ml_boxoff(B, A.ng_u, d, DIM, g[0], g[1], g[2])] =
+            A.u[d][nat_face(A.n, d, r[0], r[1], r[2])];
+    }
+}
+
+// fine ghost faces outside the region (inside the domain, or its periodic
+// image) from the final coarse faces: a fine face on a coarse face takes its
+// value, one between two coarse faces their mean; transversely the coarse
+// face containing it (FBoxLib's create_umac_grown is not in the reference
+// tree; this is its plain form)
+template <int DIM>
+__global__ void __launch_bounds__(MTPB) k_ml_grown(MlGrown A)
+{
+    const int b = blockIdx.y, d = blockIdx.z;
+    const BoxDev B = A.boxes[b];
+    const int ng = A.ng_u;
+    int e[3];
+    for (int t = 0; t < 3; ++t) e[t] = (t < DIM) ? B.hi[t] - B.lo[t] + 1 + 2 * ng + (t == d) : 1;
+    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    if (q >= (long)e[0] * e[1] * e[2]) return;
+    const int g[3] = {B.lo[0] - ng + (int)(q % e[0]), B.lo[1] - ng + (int)((q / e[0]) % e[1]),
+                      (DIM == 3) ? B.lo[2] - ng + (int)(q / ((long)e[0] * e[1])) : 0};
+    // only faces beyond the region (in cell-index terms, a face index of the
+    // region's high end is still the region's)
+    bool inside = true;
+    for (int t = 0; t < DIM; ++t) {
+        const int top = A.G.rlo[t] + A.G.nr[t] - 1 + (t == d);
+        if (g[t] < A.G.rlo[t] || g[t] > top) inside = false;
+    }
+    if (inside) return;
+    // map into the fine domain (periodic wrap); beyond a wall: physbc's
+    int w[3];
+    for (int t = 0; t < DIM; ++t) {
+        const int nf = 2 * A.G.nc[t];
+        int x = g[t];
+        const int top = nf - 1 + (t == d);
+        if (x < 0 || x > top) {
+            if (A.G.ckind[t][0] != ELL_PER) return;
+            x = ((x % nf) + nf) % nf;
+        }
+        w[t] = x;
+    }
+    if (DIM == 2) w[2] = 0;
+    int c[3] = {w[0] >> 1, w[1] >> 1, w[2] >> 1};
+    double v;
+    if ((w[d] & 1) == 0) {
+        c[d] = w[d] >> 1;
+        v = A.uc[d][nat_face(A.G.nc, d, c[0], c[1], c[2])];
+    } else {
+        c[d] = w[d] >> 1;
+        const double lo = A.uc[d][nat_face(A.G.nc, d, c[0], c[1], c[2])];
+        c[d] += 1;
+        const double hi = A.uc[d][nat_face(A.G.nc, d, c[0], c[1], c[2])];
+        v = 0.5 * (lo + hi);
+    }
+    A.umac[b * DIM + d][ml_boxoff(B, ng, d, DIM, g[0], g[1], g[2])] = v;
+}
+
+// ----------------------------------------------------------------- launchers
+#define ML_DIM(dim, ...)                        \
+    do {                                        \
+        if ((dim) == 3) {                       \
+            constexpr int D_ = 3;               \
+            __VA_ARGS__;                        \
+        } else {                                \
+            constexpr int D_ = 2;               \
+            __VA_ARGS__;                        \
+        }                                       \
+    } while (0)
+
+void launch_ml_gather(int dim, const MlGather &A, int nboxes, int maxcells, hipStream_t s)
+{
+    ML_DIM(dim, hipLaunchKernelGGL(k_ml_gather<D_>, dim3(mblk(maxcells), nboxes), dim3(MTPB), 0, s, A));
+}
+
+void launch_ml_edge_restrict(const MlGeom &G, double *const uc[3], const double *const uf[3], hipStream_t s)
+{
+    long m = 1;
+    for (int e = 0; e < G.dim; ++e) m *= (G.nr[e] >> 1) + 1;
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_edge_restrict<D_>, dim3(mblk(m), G.dim), dim3(MTPB), 0, s, G, uc[0],
+                                     uc[1], uc[2], uf[0], uf[1], uf[2]));
+}
+
+void launch_ml_div(int dim, const int n[3], const double dx[3], const double *const u[3], const double *macrhs,
+                   double *rh, hipStream_t s)
+{
+    const long nc = (long)n[0] * n[1] * n[2];
+    ML_DIM(dim, hipLaunchKernelGGL(k_ml_div<D_>, dim3(mblk(nc)), dim3(MTPB), 0, s, n[0], n[1], n[2], dx[0], dx[1],
+                                   dx[2], u[0], u[1], u[2], macrhs, rh));
+}
+
+void launch_ml_cell_restrict(const MlGeom &G, double *xc, const double *xf, hipStream_t s)
+{
+    long m = 1;
+    for (int e = 0; e < G.dim; ++e) m *= G.nr[e] >> 1;
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_cell_restrict<D_>, dim3(mblk(m)), dim3(MTPB), 0, s, G, xc, xf));
+}
+
+void launch_ml_comp_vec(const MlGeom &G, const double *rhc, const double *rhf, double *out, hipStream_t s)
+{
+    const long n = (long)G.nc[0] * G.nc[1] * G.nc[2] + (long)G.nr[0] * G.nr[1] * G.nr[2];
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_comp_vec<D_>, dim3(mblk(n)), dim3(MTPB), 0, s, G, rhc, rhf, out));
+}
+
+void launch_ml_resid(const MlState &S, double *rc, double *rf, unsigned long long *nrm, bool fine_only,
+                     hipStream_t s)
+{
+    const MlGeom &G = S.G;
+    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
+    const long n = fine_only ? nfc : ncc + nfc;
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_resid<D_>, dim3(mblk(n)), dim3(MTPB), 0, s, S, rc, rf, nrm,
+                                     fine_only ? 1 : 0));
+}
+
+void launch_ml_prolong(const MlGeom &G, const double *e, double *phc, double *phf, hipStream_t s)
+{
+    const long n = (long)G.nc[0] * G.nc[1] * G.nc[2] + (long)G.nr[0] * G.nr[1] * G.nr[2];
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_prolong<D_>, dim3(mblk(n)), dim3(MTPB), 0, s, G, e, phc, phf));
+}
+
+void launch_ml_axpy(long n, const double *x, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ml_axpy, dim3(mblk(n)), dim3(MTPB), 0, s, n, x, y);
+}
+
+void launch_ml_mkumac(const MlState &S, double *const uc[3], double *const uf[3], hipStream_t s)
+{
+    const MlGeom &G = S.G;
+    long mf = 0, mc = 0;
+    for (int d = 0; d < G.dim; ++d) {
+        long a = 1, b = 1;
+        for (int e = 0; e < 3; ++e) {
+            a *= G.nr[e] + (e == d);
+            b *= G.nc[e] + (e == d);
+        }
+        mf = a > mf ? a : mf;
+        mc = b > mc ? b : mc;
+    }
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_mkumac_fine<D_>, dim3(mblk(mf), G.dim), dim3(MTPB), 0, s, S, uf[0], uf[1],
+                                     uf[2]));
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_mkumac_coarse<D_>, dim3(mblk(mc), G.dim), dim3(MTPB), 0, s, S, uc[0],
+                                     uc[1], uc[2]));
+}
+
+void launch_ml_unscale(int dim, const int n[3], int org_r, double *const u[3], const double *b0, const double *b0e,
+                       hipStream_t s)
+{
+    for (int d = 0; d < dim; ++d) {
+        long m = 1;
+        for (int e = 0; e < 3; ++e) m *= n[e] + (e == d);
+        ML_DIM(dim, hipLaunchKernelGGL(k_ml_unscale<D_>, dim3(mblk(m)), dim3(MTPB), 0, s, d, n[0], n[1], n[2], org_r,
+                                       u[d], b0, b0e));
+    }
+}
+
+void launch_ml_scatter(int dim, const MlScatter &A, int nboxes, int maxfaces, hipStream_t s)
+{
+    ML_DIM(dim, hipLaunchKernelGGL(k_ml_scatter<D_>, dim3(mblk(maxfaces), nboxes, dim + 1), dim3(MTPB), 0, s, A));
+}
+
+void launch_ml_grown(int dim, const MlGrown &A, int nboxes, int maxgrown, hipStream_t s)
+{
+    ML_DIM(dim, hipLaunchKernelGGL(k_ml_grown<D_>, dim3(mblk(maxgrown), nboxes, dim), dim3(MTPB), 0, s, A));
+}
+
+}  // namespace mac

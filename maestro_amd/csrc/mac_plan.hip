@@ -2277,3 +2277,438 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
     }
     return src;
 }
+
+// ================================================== two-level projection
+// macproject with nlevs = 2 (Source/macproject.f90:51-328): both levels'
+// beta0 U, the edge restriction of the fine faces onto the coarse ones
+// (:357-361), rh per level and its cell restriction (:389-404), beta per
+// level (:178-193), the composite solve to min(eps_rel 10, 1e-8) (:217),
+// mkumac per level with the composite fluxes, the edge restriction again
+// (:829-833), U / beta0, and the ghost faces: the coarse level's from its
+// boxes, periodic images and physical rules, the fine level's from the fine
+// boxes, the coarse faces beyond the refined region (create_umac_grown,
+// :256-275) and the physical rules.
+//
+// The composite solve (FBoxLib ml_cc_solve is absent from the reference
+// tree) is a fast adaptive composite (FAC) iteration on the composite
+// residual: each step restricts the fine residual onto the covered coarse
+// cells, corrects with a few V-cycles of the coarse level's plan (whole
+// coarse domain) added on both levels, then corrects the fine level with a
+// few V-cycles of the fine plan (the refined rectangle, its coarse-fine
+// sides Dirichlet).  The composite residual alone decides convergence, so
+// the answer is the composite operator's whatever the inner cycles do.
+struct mac_ml_plan {
+    int dim = 0;
+    mac_plan *P[2] = {nullptr, nullptr};   // coarse layout; fine boxes over the region
+    MlGeom G;
+    double dxc[3] = {1, 1, 1}, dxf[3] = {1, 1, 1};
+    int cphys[3][2];
+    double *uc[3] = {nullptr, nullptr, nullptr}, *uf[3] = {nullptr, nullptr, nullptr};
+    double *bcf[3] = {nullptr, nullptr, nullptr}, *bff[3] = {nullptr, nullptr, nullptr};
+    double *rhc = nullptr, *rhf = nullptr, *phc = nullptr, *phf = nullptr;
+    double *mrc = nullptr, *mrf = nullptr, *zero = nullptr, *cvec = nullptr;
+    void *block = nullptr;
+    int maxgrown = 0;
+    mac_opts opts;
+};
+
+static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve
+static constexpr int ML_SUB_MAXIT = 2;       // ... and its cycle limit
+
+extern "C" int mac_ml_plan_destroy(mac_ml_plan *M)
+{
+    if (!M) return MAC_OK;
+    if (M->P[0] && M->P[0]->stream) (void)hipStreamSynchronize(M->P[0]->stream);
+    if (M->block) (void)hipFree(M->block);
+    if (M->P[1]) {
+        M->P[1]->stream = nullptr;   // the coarse plan's stream, destroyed with it
+        mac_plan_destroy(M->P[1]);
+    }
+    if (M->P[0]) mac_plan_destroy(M->P[0]);
+    delete M;
+    return MAC_OK;
+}
+
+extern "C" int mac_ml_plan_create(int nlevs, const mac_layout *lays, const mac_opts *opts, mac_ml_plan **out)
+{
+    if (!out || !lays) return MAC_ERR_INVALID;
+    *out = nullptr;
+    if (nlevs != 2) return MAC_ERR_INVALID;
+    const mac_layout &L0 = lays[0], &L1 = lays[1];
+    const int dim = L0.dim;
+    if ((dim != 2 && dim != 3) || L1.dim != dim) return MAC_ERR_LAYOUT;
+    if (L1.nboxes < 1 || !L1.box_lo || !L1.box_hi) return MAC_ERR_LAYOUT;
+    mac_ml_plan *M = new mac_ml_plan();
+    M->dim = dim;
+    if (opts) M->opts = *opts;
+    else mac_default_opts(&M->opts);
+    MlGeom &G = M->G;
+    memset(&G, 0, sizeof G);
+    G.dim = dim;
+    int rlo[3] = {0, 0, 0}, rhi[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+        if (d < dim) {
+            // both domains start at 0; the fine one is the coarse one refined by 2
+            if (L0.domain_lo[d] != 0 || L1.domain_lo[d] != 0 || L1.domain_hi[d] != 2 * L0.domain_hi[d] + 1 ||
+                std::fabs(L1.dx[d] * 2 - L0.dx[d]) > 1e-12 * L0.dx[d]) {
+                delete M;
+                return MAC_ERR_LAYOUT;
+            }
+            for (int s = 0; s < 2; ++s)
+                if (L0.phys_bc[d][s] != L1.phys_bc[d][s]) { delete M; return MAC_ERR_LAYOUT; }
+            G.nc[d] = L0.domain_hi[d] + 1;
+            M->dxc[d] = L0.dx[d];
+            M->dxf[d] = L1.dx[d];
+            G.ihc[d] = 1.0 / L0.dx[d];
+            G.ihf[d] = 1.0 / L1.dx[d];
+            rlo[d] = L1.box_lo[d];
+            rhi[d] = L1.box_hi[d];
+            for (int b = 1; b < L1.nboxes; ++b) {
+                rlo[d] = std::min(rlo[d], L1.box_lo[3 * b + d]);
+                rhi[d] = std::max(rhi[d], L1.box_hi[3 * b + d]);
+            }
+            // the region covers whole coarse cells (the fine plan checks that
+            // the boxes tile it)
+            if ((rlo[d] & 1) || !(rhi[d] & 1)) { delete M; return MAC_ERR_LAYOUT; }
+            for (int s = 0; s < 2; ++s) {
+                M->cphys[d][s] = L0.phys_bc[d][s];
+                G.ckind[d][s] = ell_kind(L0.phys_bc[d][s]);
+            }
+        } else {
+            G.nc[d] = 1;
+            G.ihc[d] = G.ihf[d] = 0.0;
+            M->cphys[d][0] = M->cphys[d][1] = MAC_PERIODIC;
+            G.ckind[d][0] = G.ckind[d][1] = ELL_PER;
+        }
+        G.rlo[d] = rlo[d];
+        G.nr[d] = rhi[d] - rlo[d] + 1;
+    }
+    // the fine plan: the fine boxes over the region; its coarse-fine sides
+    // Dirichlet (the inner correction solves), periodic where the region
+    // spans a periodic direction, physical where it meets the domain's side
+    mac_layout F = L1;
+    for (int d = 0; d < dim; ++d) {
+        F.domain_lo[d] = rlo[d];
+        F.domain_hi[d] = rhi[d];
+        const bool per = G.ckind[d][0] == ELL_PER;
+        const bool touch[2] = {rlo[d] == 0, rhi[d] == 2 * G.nc[d] - 1};
+        for (int s = 0; s < 2; ++s) {
+            if (per && touch[0] && touch[1]) {
+                G.side[d][s] = ML_SIDE_PER;
+                F.phys_bc[d][s] = MAC_PERIODIC;
+            } else if (touch[s] && !per) {
+                G.side[d][s] = ML_SIDE_PHYS;
+                F.phys_bc[d][s] = L1.phys_bc[d][s];
+            } else {
+                G.side[d][s] = ML_SIDE_CF;
+                F.phys_bc[d][s] = MAC_OUTLET;
+            }
+        }
+    }
+    int rc;
+    mac_opts po = M->opts;
+    po.device_ptrs = 1;
+    if ((rc = mac_plan_create(&L0, &po, &M->P[0]))) { mac_ml_plan_destroy(M); return rc; }
+    if ((rc = mac_plan_create(&F, &po, &M->P[1]))) { mac_ml_plan_destroy(M); return rc; }
+    // one stream for both levels
+    (void)hipStreamSynchronize(M->P[1]->stream);
+    (void)hipStreamDestroy(M->P[1]->stream);
+    M->P[1]->stream = M->P[0]->stream;
+    // whole-level arrays
+    const long ncc = ncell(G.nc), nfc = ncell(G.nr);
+    long nfc_c[3], nfc_f[3];
+    for (int d = 0; d < 3; ++d) {
+        int a[3] = {G.nc[0], G.nc[1], G.nc[2]}, b[3] = {G.nr[0], G.nr[1], G.nr[2]};
+        a[d] += 1;
+        b[d] += 1;
+        nfc_c[d] = (d < dim) ? ncell(a) : 0;
+        nfc_f[d] = (d < dim) ? ncell(b) : 0;
+    }
+    auto up = [](long n) { return (size_t)((n + 31) / 32 * 32); };
+    size_t tot = 0;
+    for (int d = 0; d < dim; ++d) tot += 2 * up(nfc_c[d]) + 2 * up(nfc_f[d]);
+    const long zmax = std::max({nfc_c[0], nfc_c[1], nfc_c[2], nfc_f[0], nfc_f[1], nfc_f[2]});
+    tot += 3 * up(ncc) + 3 * up(nfc) + up(zmax) + up(ncc + nfc);
+    if (hipMalloc(&M->block, tot * sizeof(double)) != hipSuccess) { mac_ml_plan_destroy(M); return MAC_ERR_NOMEM; }
+    double *p = (double *)M->block;
+    auto take = [&](long n) { double *q = p; p += up(n); return q; };
+    for (int d = 0; d < dim; ++d) {
+        M->uc[d] = take(nfc_c[d]);
+        M->bcf[d] = take(nfc_c[d]);
+        M->uf[d] = take(nfc_f[d]);
+        M->bff[d] = take(nfc_f[d]);
+    }
+    M->rhc = take(ncc); M->phc = take(ncc); M->mrc = take(ncc);
+    M->rhf = take(nfc); M->phf = take(nfc); M->mrf = take(nfc);
+    M->zero = take(zmax);
+    M->cvec = take(ncc + nfc);
+    if (hipMemset(M->zero, 0, sizeof(double) * zmax) != hipSuccess) { mac_ml_plan_destroy(M); return MAC_ERR_HIP; }
+    // the largest grown face box of a fine box (ghost faces of the fine level)
+    for (const BoxDev &B : M->P[1]->boxes)
+        for (int d = 0; d < dim; ++d) {
+            long v = 1;
+            for (int e = 0; e < dim; ++e) v *= B.hi[e] - B.lo[e] + 1 + 2 * L1.ng_umac + (e == d);
+            M->maxgrown = (int)std::max<long>(M->maxgrown, v);
+        }
+    *out = M;
+    return MAC_OK;
+}
+
+// copy of a plan's solver options for the inner correction solves
+static void ml_inner_opts(mac_ml_plan *M, mac_plan *P)
+{
+    P->opts = M->opts;
+    P->opts.device_ptrs = 1;
+    P->opts.eps_rel = ML_SUB_EPS;
+    P->opts.eps_abs = -1.0;
+    P->opts.max_iter = ML_SUB_MAXIT;
+    P->opts.verbose = 0;
+    P->planar = false;
+}
+
+// the composite residual's max norm (read back; synchronises the stream)
+static int ml_norm(mac_ml_plan *M, const MlState &S, bool fine_only, double *out, int *bad)
+{
+    mac_plan *P0 = M->P[0];
+    hipStream_t s = P0->stream;
+    HIPCHK(hipMemsetAsync(P0->nrm_dev, 0, sizeof(unsigned long long), s));
+    launch_ml_resid(S, P0->H64.lev[0].rhs, M->P[1]->H64.lev[0].rhs, P0->nrm_dev, fine_only, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(P0->nrm_host, P0->nrm_dev, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    if (bad) HIPCHK(hipMemcpyAsync(P0->nrm_host + 1, P0->err_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    memcpy(out, P0->nrm_host, sizeof(double));
+    if (bad) {
+        int e;
+        memcpy(&e, P0->nrm_host + 1, sizeof(int));
+        *bad = e;
+    }
+    return MAC_OK;
+}
+
+extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double *const *const *phi,
+                              const double *const *const *rho, const double *const *const *macrhs,
+                              const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *st)
+{
+    if (!M || !umac || !phi || !rho || !umac[0] || !umac[1] || !phi[0] || !phi[1] || !rho[0] || !rho[1])
+        return MAC_ERR_INVALID;
+    if (!M->opts.device_ptrs) return MAC_ERR_INVALID;
+    const bool b0 = beta0_1d != nullptr;
+    if (b0 && (!beta0_1d_edge || !beta0_1d[0] || !beta0_1d[1] || !beta0_1d_edge[0] || !beta0_1d_edge[1]))
+        return MAC_ERR_INVALID;
+    if (!b0 && beta0_1d_edge) return MAC_ERR_INVALID;
+    const bool mr = macrhs != nullptr;
+    if (mr && (!macrhs[0] || !macrhs[1])) return MAC_ERR_INVALID;
+    if (M->opts.precision < MAC_PREC_FP64 || M->opts.precision > MAC_PREC_MIXED_BF16) return MAC_ERR_INVALID;
+    if (st) memset(st, 0, sizeof(*st));
+    mac_plan *P0 = M->P[0], *P1 = M->P[1];
+    const MlGeom &G = M->G;
+    const int dim = M->dim;
+    const int rd = dim - 1;
+    hipStream_t s = P0->stream;
+    HIPCHK(hipSetDevice(P0->device));
+    HIPCHK(hipEventRecord(P0->ev[0], s));
+    HIPCHK(hipMemsetAsync(P0->err_dev, 0, sizeof(int), s));
+    // ---- pointer tables and the gather of each level
+    const long ncc = ncell(G.nc), nfc = ncell(G.nr);
+    for (int l = 0; l < 2; ++l) {
+        mac_plan *P = M->P[l];
+        const int nb = P->nboxes;
+        std::vector<double *> &tab = P->tab_host;
+        for (int q = 0; q < nb * dim; ++q) tab[q] = umac[l][q];
+        for (int b = 0; b < nb; ++b) {
+            tab[nb * dim + b] = phi[l][b];
+            tab[nb * dim + nb + b] = (double *)rho[l][b];
+            tab[nb * dim + 2 * nb + b] = mr ? (double *)macrhs[l][b] : nullptr;
+        }
+        HIPCHK(hipMemcpyAsync(P->tab_dev, tab.data(), sizeof(double *) * tab.size(), hipMemcpyHostToDevice, s));
+        MlGather A;
+        A.boxes = P->boxes_dev;
+        for (int d = 0; d < 3; ++d) {
+            A.org[d] = l ? G.rlo[d] : 0;
+            A.n[d] = l ? G.nr[d] : G.nc[d];
+        }
+        A.ng_u = P->ng_u; A.ng_rho = P->ng_rho; A.ng_rhs = P->ng_rhs; A.ng_phi = P->ng_phi;
+        double **td = P->tab_dev;
+        A.umac = td;
+        A.phi = td + nb * dim;
+        A.rho = td + nb * dim + nb;
+        A.macrhs = mr ? td + nb * dim + 2 * nb : nullptr;
+        A.b0 = b0 ? beta0_1d[l] : nullptr;
+        A.b0e = b0 ? beta0_1d_edge[l] : nullptr;
+        A.alt = M->opts.use_alt_energy_fix;
+        for (int d = 0; d < 3; ++d) {
+            A.u[d] = l ? M->uf[d] : M->uc[d];
+            A.beta[d] = l ? M->bff[d] : M->bcf[d];
+        }
+        A.rhs = l ? M->mrf : M->mrc;
+        A.phi_out = l ? M->phf : M->phc;
+        A.err = P0->err_dev;
+        launch_ml_gather(dim, A, nb, P->maxcells, s);
+    }
+    HIPCHK(hipGetLastError());
+    // ---- rh: edge restriction, divergence per level, cell restriction
+    launch_ml_edge_restrict(G, M->uc, M->uf, s);
+    launch_ml_div(dim, G.nc, M->dxc, M->uc, mr ? M->mrc : nullptr, M->rhc, s);
+    launch_ml_div(dim, G.nr, M->dxf, M->uf, mr ? M->mrf : nullptr, M->rhf, s);
+    launch_ml_cell_restrict(G, M->rhc, M->rhf, s);
+    bool singular = true;
+    for (int d = 0; d < dim; ++d)
+        for (int q = 0; q < 2; ++q)
+            if (G.ckind[d][q] == ELL_DIR) singular = false;
+    int rc;
+    if (singular) {
+        // the volume-weighted composite mean (compensated) off both levels
+        long nunc = ncc;
+        {
+            long cov = 1;
+            for (int d = 0; d < dim; ++d) cov *= G.nr[d] / 2;
+            nunc -= cov;
+        }
+        const long n = ncc + nfc, np = (n + 255) / 256;
+        if ((rc = ensure_part(P0, np))) return rc;
+        launch_ml_comp_vec(G, M->rhc, M->rhf, M->cvec, s);
+        launch_partial_sum<double>(M->cvec, n, P0->part, s);
+        launch_mean(P0->part, np, (double)nunc * (dim == 3 ? 8.0 : 4.0) + (double)nfc, P0->mean_dev, s);
+        launch_sub_mean<double>(M->rhc, ncc, P0->mean_dev, s);
+        launch_sub_mean<double>(M->rhf, nfc, P0->mean_dev, s);
+    }
+    // max |rh| over both levels (covered coarse cells hold fine means)
+    HIPCHK(hipMemsetAsync(P0->nrm_dev + 2, 0, sizeof(unsigned long long), s));
+    launch_absmax(M->rhc, ncc, P0->nrm_dev + 2, s);
+    launch_absmax(M->rhf, nfc, P0->nrm_dev + 2, s);
+    // ---- the inner solvers' level 0: folded weights from beta (zero U, zero
+    // warm start); each step then writes its right side and clears phi
+    for (int l = 0; l < 2; ++l) {
+        mac_plan *P = M->P[l];
+        LevelIO IO;
+        IO.ux = IO.uy = IO.uz = M->zero;
+        IO.bx = l ? M->bff[0] : M->bcf[0];
+        IO.by = l ? M->bff[1] : M->bcf[1];
+        IO.bz = (dim == 3) ? (l ? M->bff[2] : M->bcf[2]) : nullptr;
+        IO.macrhs = nullptr;
+        IO.phi = M->zero;
+        for (int d = 0; d < 3; ++d) IO.dx[d] = l ? M->dxf[d] : M->dxc[d];
+        IO.bc = P->bc;
+        const long n0 = ncell(P->n);
+        if ((rc = ensure_part(P, (n0 + 255) / 256))) return rc;
+        launch_level_setup(dim, IO, P->H64.lev[0], P->part, s);
+        ml_inner_opts(M, P);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P0->ev[1], s));
+    // ---- composite solve
+    MlState S;
+    S.G = G;
+    for (int d = 0; d < 3; ++d) {
+        S.bc[d] = M->bcf[d];
+        S.bf[d] = M->bff[d];
+    }
+    S.rhc = M->rhc; S.rhf = M->rhf; S.phc = M->phc; S.phf = M->phf;
+    const double eps = std::min(M->opts.eps_rel * 10.0, 1e-8);   // macproject.f90:217, nlevs = 2
+    double r0 = 0, r = 0;
+    int bad = 0;
+    if ((rc = ml_norm(M, S, false, &r0, &bad))) return rc;
+    if (bad || !std::isfinite(r0)) return MAC_ERR_NONFINITE;
+    r = r0;
+    auto done = [&](double x) { return x <= eps * r0 || (M->opts.eps_abs > 0 && x <= M->opts.eps_abs) || x == 0.0; };
+    int it = 0, inner = 0;
+    rc = MAC_OK;
+    while (!done(r) && it < M->opts.max_iter) {
+        // coarse correction: the fine residual restricted onto the covered cells
+        Lev<double> &C0 = P0->H64.lev[0];
+        launch_ml_cell_restrict(G, C0.rhs, P1->H64.lev[0].rhs, s);
+        if (P0->singular) {
+            if ((rc = remove_mean(P0, C0.rhs, ncc))) return rc;
+        }
+        HIPCHK(hipMemsetAsync(C0.phi, 0, sizeof(double) * ncc, s));
+        mac_stats is;
+        int src = solve(P0, &is);
+        if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+        inner += is.iters;
+        launch_ml_prolong(G, C0.phi, M->phc, M->phf, s);
+        // fine correction on the residual left by it
+        Lev<double> &F0 = P1->H64.lev[0];
+        HIPCHK(hipMemsetAsync(P0->nrm_dev, 0, sizeof(unsigned long long), s));
+        launch_ml_resid(S, C0.rhs, F0.rhs, P0->nrm_dev, true, s);
+        HIPCHK(hipMemsetAsync(F0.phi, 0, sizeof(double) * nfc, s));
+        src = solve(P1, &is);
+        if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+        inner += is.iters;
+        launch_ml_axpy(nfc, F0.phi, M->phf, s);
+        if ((rc = ml_norm(M, S, false, &r, nullptr))) return rc;
+        ++it;
+        if (M->opts.verbose) fprintf(stderr, "maestro_mac: composite step %d  res/res0 = %.3e\n", it, r / r0);
+        if (!std::isfinite(r)) return MAC_ERR_NONFINITE;
+    }
+    const bool conv = done(r);
+    HIPCHK(hipEventRecord(P0->ev[2], s));
+    // ---- phi on covered coarse cells, mkumac, edge restriction, U / beta0
+    launch_ml_cell_restrict(G, M->phc, M->phf, s);
+    launch_ml_mkumac(S, M->uc, M->uf, s);
+    launch_ml_edge_restrict(G, M->uc, M->uf, s);
+    if (b0) {
+        launch_ml_unscale(dim, G.nc, 0, M->uc, beta0_1d[0], beta0_1d_edge[0], s);
+        launch_ml_unscale(dim, G.nr, G.rlo[rd], M->uf, beta0_1d[1], beta0_1d_edge[1], s);
+    }
+    // ---- the caller's boxes, then their ghost faces
+    for (int l = 0; l < 2; ++l) {
+        mac_plan *P = M->P[l];
+        const int nb = P->nboxes;
+        MlScatter A;
+        A.boxes = P->boxes_dev;
+        for (int d = 0; d < 3; ++d) {
+            A.org[d] = l ? G.rlo[d] : 0;
+            A.n[d] = l ? G.nr[d] : G.nc[d];
+            A.u[d] = l ? M->uf[d] : M->uc[d];
+        }
+        A.ng_u = P->ng_u; A.ng_phi = P->ng_phi;
+        A.umac = P->tab_dev;
+        A.phi = P->tab_dev + nb * dim;
+        A.phi_in = l ? M->phf : M->phc;
+        launch_ml_scatter(dim, A, nb, std::max(P->maxfaces, P->maxcells), s);
+        CorrectArgs C;
+        memset(&C, 0, sizeof C);
+        C.dim = dim; C.nboxes = nb; C.boxes = P->boxes_dev;
+        for (int d = 0; d < 3; ++d) {
+            C.dlo[d] = P->dlo[d]; C.n[d] = P->n[d]; C.dx[d] = P->dx[d];
+            C.mapg[d] = P->mapg[d]; C.mapn[d] = P->mapn[d];
+            for (int q = 0; q < 2; ++q)
+                C.phys[d][q] = (l && G.side[d][q] == ML_SIDE_CF) ? MAC_INTERIOR : P->phys[d][q];
+        }
+        C.bc = P->bc;
+        C.ng_u = P->ng_u;
+        C.umac = P->tab_dev;
+        C.boxmap = P->boxmap_dev;
+        C.maxplane = P->maxplane;
+        launch_fill_ghosts(C, P->maxghost, true, false, s);
+        if (l == 1 && P->ng_u > 0) {
+            MlGrown Gr;
+            Gr.G = G;
+            Gr.boxes = P->boxes_dev;
+            Gr.ng_u = P->ng_u;
+            Gr.umac = P->tab_dev;
+            for (int d = 0; d < 3; ++d) Gr.uc[d] = M->uc[d];
+            launch_ml_grown(dim, Gr, nb, M->maxgrown, s);
+        }
+        launch_fill_ghosts(C, P->maxghost, false, true, s);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P0->ev[3], s));
+    HIPCHK(hipMemcpyAsync(P0->nrm_host + 2, P0->nrm_dev + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (st) {
+        st->iters = it;
+        st->inner_cycles = inner;
+        st->converged = conv;
+        st->nlevels = 2;
+        memcpy(&st->rhs_norm, P0->nrm_host + 2, sizeof(double));
+        st->res0 = r0;
+        st->res = r;
+        st->t_setup_ms = ev_ms(P0, 0, 1);
+        st->t_solve_ms = ev_ms(P0, 1, 2);
+        st->t_correct_ms = ev_ms(P0, 2, 3);
+        st->t_total_ms = ev_ms(P0, 0, 3);
+    }
+    return conv ? MAC_OK : MAC_ERR_NOT_CONVERGED;
+}

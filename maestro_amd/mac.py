@@ -39,7 +39,8 @@ SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_
            "mac_project", "mac_project_noalloc", "mac_project_level", "mac_adjoint_level", "mac_version",
            "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
-           "mac_plan_profile", "mac_plan_profile_read")
+           "mac_plan_profile", "mac_plan_profile_read", "mac_ml_plan_create", "mac_ml_plan_destroy",
+           "mac_ml_project")
 PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5,
         "gsrb64_zero": 6, "gsrb32_zero": 7, "restrict64": 8, "restrict32": 9, "prolong64": 10,
         "prolong32": 11, "ir_update": 12}
@@ -148,14 +149,19 @@ def lib():
         L.mac_debug_get.argtypes = [vp, ip, ip, vp]
         L.mac_debug_set.argtypes = [vp, ip, ip, vp]
         L.mac_debug_op.argtypes = [vp, ip, ip, ip]
+        L.mac_ml_plan_create.argtypes = [ip, C.POINTER(CLayout), C.POINTER(COpts), C.POINTER(vp)]
+        L.mac_ml_plan_destroy.argtypes = [vp]
+        L.mac_ml_project.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
         L.mac_plan_profile.argtypes = [vp, ip]
         L.mac_plan_profile_read.argtypes = [vp, ip, C.POINTER(C.c_long), C.POINTER(C.c_double)]
         for f in ("mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
-                  "mac_plan_profile", "mac_plan_profile_read"):
+                  "mac_plan_profile", "mac_plan_profile_read", "mac_ml_plan_create", "mac_ml_plan_destroy",
+           "mac_ml_project"):
             getattr(L, f).restype = ip
         for f in ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
                   "mac_project", "mac_project_level", "mac_adjoint_level", "mac_cell_to_edge",
-                  "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc"):
+                  "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
+                  "mac_ml_plan_create", "mac_ml_plan_destroy", "mac_ml_project"):
             getattr(L, f).restype = ip
         _lib = L
     return _lib
@@ -524,6 +530,94 @@ class MacPlan:
         rc = lib().mac_adjoint_level(self._h, *args, C.byref(st))
         if rc and (check or rc != 4):
             raise MacError(rc, "mac_adjoint_level")
+        d = st.as_dict()
+        d["rc"] = rc
+        return d
+
+
+class MacMLPlan:
+    """Two-level plan (``mac_ml_plan_create``): ``layouts`` = [coarse, fine],
+    the fine level's boxes tiling one rectangle of whole coarse cells.  The
+    arrays of ``project`` are device tensors (per level, per box)."""
+
+    def __init__(self, layouts, opts: COpts | None = None, **kw):
+        if len(layouts) != 2:
+            raise ValueError("two levels")
+        self.layouts = list(layouts)
+        self.opts = opts or default_opts(**kw)
+        self.opts.device_ptrs = 1
+        self._c = (CLayout * 2)(*[l.to_c() for l in self.layouts])
+        h = C.c_void_p()
+        rc = lib().mac_ml_plan_create(2, self._c, C.byref(self.opts), C.byref(h))
+        if rc:
+            raise MacError(rc, "mac_ml_plan_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mac_ml_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def project(self, umac, phi, rho, macrhs=None, beta0_1d=None, beta0_1d_edge=None, check=True):
+        """macproject on two levels.  Per level l: umac[l] a list per box of
+        ``dim`` face tensors (or a PtrTable of the flattened list), phi[l],
+        rho[l] and macrhs[l] per-box lists, beta0_1d[l] / beta0_1d_edge[l]
+        level l's planar profile.  Returns the stats dict."""
+        dim = self.layouts[0].dim
+        keep = []
+
+        def table(xs, l, per_box, name):
+            nb = len(self.layouts[l].boxes)
+            if isinstance(xs, PtrTable):
+                t = xs
+            elif per_box == 1:
+                t = list(xs)
+            else:
+                if any(len(box) != per_box for box in xs):
+                    raise ValueError(f"{name}[{l}]: {per_box} arrays per box")
+                t = [a for box in xs for a in box]
+            if len(t) != nb * per_box:
+                raise ValueError(f"{name}[{l}]: {nb * per_box} arrays for {nb} boxes, got {len(t)}")
+            items = t.items if isinstance(t, PtrTable) else t
+            if not all(_is_torch(x) for x in items):
+                raise ValueError("mac_ml_project takes device tensors only")
+            a = _ptr_array(t)
+            keep.append(a)
+            return C.cast(a, C.c_void_p).value
+
+        def per_level(xs, per_box, name):
+            if xs is None:
+                return None
+            arr = (C.c_void_p * 2)(*[table(xs[l], l, per_box, name) for l in range(2)])
+            keep.append(arr)
+            return arr
+
+        def vec(xs):
+            if xs is None:
+                return None
+            if not all(_is_torch(x) for x in xs):
+                raise ValueError("mac_ml_project takes device tensors only")
+            arr = (C.c_void_p * 2)(*[_ptr(x).value for x in xs])
+            keep.append(arr)
+            return arr
+
+        u = per_level(umac, dim, "umac")
+        p = per_level(phi, 1, "phi")
+        r = per_level(rho, 1, "rho")
+        m = per_level(macrhs, 1, "macrhs")
+        b0, b0e = vec(beta0_1d), vec(beta0_1d_edge)
+        import torch
+        torch.cuda.current_stream().synchronize()
+        st = CStats()
+        rc = lib().mac_ml_project(self._h, u, p, r, m, b0, b0e, C.byref(st))
+        if rc and (check or rc != 4):
+            raise MacError(rc, "mac_ml_project")
         d = st.as_dict()
         d["rc"] = rc
         return d

@@ -1,0 +1,219 @@
+"""Two-level (AMR) projection on the GPU (mac_ml_project) against the
+oracle's sparse direct composite solve, on the reference's two-level grid
+(gr0_2d_2lev_coarse, Maestro-tests.ini:245-257) and boxes touching walls,
+outlets and periodic seams; the fine ghost faces outside the refined region
+against the coarse faces they come from."""
+import numpy as np
+import pytest
+
+from maestro_amd import boxes as BX
+from maestro_amd import workloads as W
+from tests.test_ml_oracle import REGIONS, random_two_level
+
+P, WALL, OUT = W.PERIODIC, W.SLIP_WALL, W.OUTLET
+
+pytestmark = pytest.mark.gpu
+
+
+def chunks(lo, hi, size, dim):
+    """Boxes of at most ``size`` cells tiling [lo, hi] (inclusive)."""
+    axes = []
+    for d in range(dim):
+        a, cur = [], lo[d]
+        while cur <= hi[d]:
+            a.append((cur, min(cur + size - 1, hi[d])))
+            cur += size
+        axes.append(a)
+    out = []
+    for idx in np.ndindex(*[len(a) for a in reversed(axes)]):
+        idx = tuple(reversed(idx))
+        out.append(([axes[d][idx[d]][0] for d in range(dim)], [axes[d][idx[d]][1] for d in range(dim)]))
+    return out
+
+
+def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-13, precision=0):
+    import torch
+    from maestro_amd import mac
+    dim = tl.dim
+    bc = [list(tl.bc[d]) for d in range(dim)]
+    nc = tl.nc[:dim]
+    cb = chunks([0] * dim, [n - 1 for n in nc], box_c, dim)
+    fb = chunks(tl.rlo[:dim], tl.rhi[:dim], box_f, dim)
+    rel = [([l[d] - tl.rlo[d] for d in range(dim)], [h[d] - tl.rlo[d] for d in range(dim)]) for l, h in fb]
+    # the region as a domain of its own for the split: periodic only where it
+    # spans a periodic direction (rho's ghosts come from the region array)
+    rbc = [[P, P] if tl.side[d][0] == "per" else [WALL, WALL] for d in range(dim)]
+    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    lv = []
+    for boxes, faces, rho, b, mr in ((cb, fc, rc, bc, None if macrhs is None else macrhs[0]),
+                                     (rel, ff, rf, rbc, None if macrhs is None else macrhs[1])):
+        u = [[cv(BX.split_faces(faces[d], d, lo, hi, 1, dim, b)) for d in range(dim)] for lo, hi in boxes]
+        ph = [cv(np.zeros(BX.box_shape(lo, hi, 1, dim))) for lo, hi in boxes]
+        r = [cv(BX.split_cells(rho, lo, hi, 1, dim, b, glob_ghost=1)) for lo, hi in boxes]
+        m = None if mr is None else [cv(BX.split_cells(mr, lo, hi, 0, dim, b)) for lo, hi in boxes]
+        lv.append((u, ph, r, m))
+    dxc = [1.0 / nc[0]] * dim
+    lays = [mac.Layout(dim=dim, boxes=cb, domain_lo=(0,) * dim, domain_hi=tuple(n - 1 for n in nc), dx=tuple(dxc),
+                       phys_bc=bc, ng_macrhs=0),
+            mac.Layout(dim=dim, boxes=fb, domain_lo=(0,) * dim, domain_hi=tuple(2 * n - 1 for n in nc),
+                       dx=tuple(h / 2 for h in dxc), phys_bc=bc, ng_macrhs=0)]
+    plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=eps, use_alt_energy_fix=alt, precision=precision))
+    b0 = b0e = None
+    if kw:
+        b0 = [cv(kw["beta0"]), cv(kw["beta0_f"])]
+        b0e = [cv(kw["beta0_edge"]), cv(kw["beta0_edge_f"])]
+    st = plan.project([lv[0][0], lv[1][0]], [lv[0][1], lv[1][1]], [lv[0][2], lv[1][2]],
+                      None if macrhs is None else [lv[0][3], lv[1][3]], b0, b0e)
+    torch.cuda.synchronize()
+    out = {"stats": st, "umac": [lv[0][0], lv[1][0]], "boxes": [cb, fb]}
+    out["faces_c"] = [BX.gather_faces([box[d].cpu().numpy() for box in lv[0][0]], cb, d, 1, nc, dim)
+                      for d in range(dim)]
+    out["faces_f"] = [BX.gather_faces([box[d].cpu().numpy() for box in lv[1][0]], rel, d, 1, tl.nr[:dim], dim)
+                      for d in range(dim)]
+    out["phi_c"] = gather_cells([a.cpu().numpy() for a in lv[0][1]], cb, 1, nc, dim)
+    out["phi_f"] = gather_cells([a.cpu().numpy() for a in lv[1][1]], rel, 1, tl.nr[:dim], dim)
+    plan.close()
+    return out
+
+
+def gather_cells(arrs, boxes, ng, n, dim):
+    """Valid cells of box arrays (any box shape) into a level array."""
+    out = np.zeros((n[2] if dim == 3 else 1, n[1], n[0]))
+    for a, (lo, hi) in zip(arrs, boxes):
+        v = a[tuple(slice(ng, s - ng) for s in a.shape)]
+        if dim == 3:
+            out[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = v
+        else:
+            out[0, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = v
+    return out
+
+
+def composite_mean(tl, pc, pf):
+    w = tl.volume_weights()
+    x = tl.pack(pc, pf)
+    return float(np.dot(w, x) / w.sum())
+
+
+def compare(tl, g, r, tol=1e-8):
+    dim = tl.dim
+    for lev in ("c", "f"):
+        for d in range(dim):
+            a, b = g["faces_" + lev][d], r["faces_" + lev][d]
+            assert np.abs(a - b).max() <= tol * max(1.0, np.abs(b).max()), (lev, d, np.abs(a - b).max())
+    pc, pf = g["phi_c"], g["phi_f"]
+    oc, of = r["phi_c"], r["phi_f"]
+    if tl.singular():
+        m = composite_mean(tl, pc, pf)
+        pc, pf = pc - m, pf - m
+        mo = composite_mean(tl, oc, of)
+        oc, of = oc - mo, of - mo
+    scale = max(np.abs(oc).max(), np.abs(of).max())
+    cov = tl.covered_mask()
+    assert np.abs(pf - of).max() <= tol * scale
+    assert np.abs(pc - oc)[~cov].max() <= tol * scale
+    # covered coarse cells: the mean of their fine cells
+    assert np.abs(pc - tl.restrict_cells(pc, pf)).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("alt", [1, 0])
+@pytest.mark.parametrize("dim,n,bc,lo,hi", REGIONS)
+def test_ml_projection_matches_oracle(oracle_lib, dim, n, bc, lo, hi, alt):
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 23, beta0=bc[dim - 1][0] != P)
+    r = M.macproject2(tl, fc, ff, rc, rf, alt=bool(alt), **kw)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=4 if dim == 3 else 8, box_f=4 if dim == 3 else 8, alt=alt)
+    st = g["stats"]
+    print(dim, n, bc, lo, hi, alt, "composite steps", st["iters"], "inner cycles", st["inner_cycles"], flush=True)
+    assert st["rc"] == 0 and st["converged"]
+    assert st["res"] <= 1e-12 * st["res0"]
+    compare(tl, g, r)
+
+
+@pytest.mark.parametrize("kind", ["per", "wall"])
+def test_ml_reference_two_level_grid(oracle_lib, kind):
+    """gr0_2d_2lev_coarse at 64^2: the left half refined over all of y, with
+    the periodic and the slip-wall boundaries of inputs_2d_{per,wall}_mac;
+    a macrhs on both levels; the default tolerance (1e-9 for two levels,
+    Source/macproject.f90:217)."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    n = 64
+    bc = [[P, P], [P, P]] if kind == "per" else [[WALL, WALL], [WALL, WALL]]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, 2, n, bc, [0, 0], [n - 1, 2 * n - 1], 5, beta0=kind == "wall")
+    for d in range(2):
+        for s, sl in ((0, 0), (1, -1)):
+            if bc[d][s] == WALL:
+                idx = [slice(None)] * 3
+                idx[2 - d] = sl
+                fc[d][tuple(idx)] = 0.0
+                if tl.side[d][s] == "phys":
+                    ff[d][tuple(idx)] = 0.0
+    rng = np.random.default_rng(3)
+    mc = rng.standard_normal((1, n, n))
+    mf = rng.standard_normal((1, 2 * n, n))
+    mc = tl.restrict_cells(mc, mf)
+    r = M.macproject2(tl, fc, ff, rc, rf, macrhs_c=mc, macrhs_f=mf, alt=True, **kw)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=32, box_f=32, macrhs=(mc, mf), eps=1e-10)
+    st = g["stats"]
+    print(kind, "composite steps", st["iters"], "inner cycles", st["inner_cycles"], flush=True)
+    assert st["rc"] == 0
+    assert st["res"] <= 1e-9 * st["res0"]
+    compare(tl, g, r, tol=1e-6)
+
+
+def test_ml_mixed_precision_inner_cycles(oracle_lib):
+    """The inner correction solves in the mixed precision of the plan: the
+    composite residual stays FP64, so the answer is the same."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, lo, hi = 3, 16, [[P, P], [P, P], [WALL, WALL]], [8, 8, 0], [23, 23, 15]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 29)
+    for s, sl in ((0, 0), (1, -1)):
+        idx = [slice(None)] * 3
+        idx[0] = sl
+        fc[2][tuple(idx)] = 0.0
+        if tl.side[2][s] == "phys":
+            ff[2][tuple(idx)] = 0.0
+    r = M.macproject2(tl, fc, ff, rc, rf, alt=True, **kw)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, precision=1)
+    assert g["stats"]["rc"] == 0
+    compare(tl, g, r)
+
+
+def test_ml_fine_ghost_faces(oracle_lib):
+    """Fine ghost faces beyond the refined region (inside the domain) take
+    the final coarse faces: a fine face on a coarse face its value, one
+    between two coarse faces their mean, transversely the coarse face that
+    contains it."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, lo, hi = 2, 16, [[P, P], [WALL, WALL]], [8, 10], [19, 21]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 31)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=4)
+    uc = g["faces_c"]
+    checked = 0
+    for (blo, bhi), box in zip(g["boxes"][1], g["umac"][1]):
+        for d in range(dim):
+            a = box[d].cpu().numpy()
+            for jj in range(a.shape[0]):
+                for ii in range(a.shape[1]):
+                    gi, gj = blo[0] - 1 + ii, blo[1] - 1 + jj
+                    top = [tl.rhi[0] + (d == 0), tl.rhi[1] + (d == 1)]
+                    if tl.rlo[0] <= gi <= top[0] and tl.rlo[1] <= gj <= top[1]:
+                        continue
+                    w = [gi % (2 * n), gj]
+                    if not (0 <= gj <= 2 * n - 1 + (d == 1)):
+                        continue
+                    if w[d] % 2 == 0:
+                        c = [w[0] // 2, w[1] // 2]
+                        c[d] = w[d] // 2
+                        exp = uc[d][0, c[1], c[0]]
+                    else:
+                        c = [w[0] // 2, w[1] // 2]
+                        c2 = list(c)
+                        c2[d] += 1
+                        exp = 0.5 * (uc[d][0, c[1], c[0]] + uc[d][0, c2[1], c2[0]])
+                    assert abs(a[jj, ii] - exp) <= 1e-14 * max(1.0, abs(exp)), (d, gi, gj)
+                    checked += 1
+    assert checked > 0
