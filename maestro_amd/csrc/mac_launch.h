@@ -206,6 +206,9 @@ void launch_ml_comp_vec(const MlGeom &G, const double *rhc, const double *rhf, d
 void launch_ml_resid(const MlState &S, double *rc, double *rf, unsigned long long *nrm, bool fine_only,
                      hipStream_t s);
 void launch_ml_prolong(const MlGeom &G, const double *e, double *phc, double *phf, hipStream_t s);
+// the fine correction plan's folded weights on the region's coarse-fine
+// sides (level 0 of its hierarchy) scaled by f
+void launch_ml_cf_weights(const MlGeom &G, const Lev<double> &L, double f, hipStream_t s);
 void launch_ml_axpy(long n, const double *x, double *y, hipStream_t s);
 void launch_ml_mkumac(const MlState &S, double *const uc[3], double *const uf[3], hipStream_t s);
 void launch_ml_unscale(int dim, const int n[3], int org_r, double *const u[3], const double *b0, const double *b0e,

@@ -2314,6 +2314,7 @@ struct mac_ml_plan {
 
 static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve
 static constexpr int ML_SUB_MAXIT = 2;       // ... and its cycle limit
+static constexpr double ML_CF_WEIGHT = 4.0 / 15.0;   // see k_ml_cf_weights
 
 extern "C" int mac_ml_plan_destroy(mac_ml_plan *M)
 {
@@ -2593,6 +2594,7 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
         const long n0 = ncell(P->n);
         if ((rc = ensure_part(P, (n0 + 255) / 256))) return rc;
         launch_level_setup(dim, IO, P->H64.lev[0], P->part, s);
+        if (l == 1) launch_ml_cf_weights(G, P->H64.lev[0], ML_CF_WEIGHT, s);
         ml_inner_opts(M, P);
     }
     HIPCHK(hipGetLastError());
