@@ -346,6 +346,88 @@ def time_slab_share(torch, steps=5, warmup=2):
     return out
 
 
+def time_two_level(torch, steps=3, warmup=1):
+    """The two-level projection (mac_ml_project): a 128^3 coarse level (64
+    boxes of 32^3, x/y periodic, z walls, isothermal beta0 of contrast 1e4)
+    with the central 64 x 64 x 64 coarse cells at the bottom wall refined by
+    2 (128^3 fine cells in 64 boxes of 32^3), U ~ U[-1,1], rho = beta0 (1 +
+    0.01 N(0,1)), to the two-level tolerance min(1e-10 * 10, 1e-8) = 1e-9.
+    Device arrays; median of the timed calls."""
+    from maestro_amd import mac
+    from maestro_amd import workloads as W
+    from maestro_amd import boxes as BX
+    rng = np.random.default_rng(1234 + 7)
+    nc, box = 128, 32
+    rlo, rhi = (64, 64, 0), (191, 191, 127)
+    nr = [rhi[d] - rlo[d] + 1 for d in range(3)]
+    bc = [[W.PERIODIC, W.PERIODIC], [W.PERIODIC, W.PERIODIC], [W.SLIP_WALL, W.SLIP_WALL]]
+    lnc = np.log(1e4)
+    out = {"workload": "two-level MAC projection: 128^3 coarse (64 boxes of 32^3, x/y periodic, z walls, "
+                       "beta0 contrast 1e4) with the central 64x64x64 coarse cells at the bottom wall refined "
+                       "by 2 (128^3 fine cells, 64 boxes of 32^3); rel residual 1e-9 (two levels)"}
+    levels = []
+    for l, (n, org) in enumerate(((nc, (0, 0, 0)), (2 * nc, rlo))):
+        h = 1.0 / n
+        ext = [nc] * 3 if l == 0 else nr
+        zc = (np.arange(n) + 0.5) * h
+        ze = np.arange(n + 1) * h
+        b0, b0e = np.exp(-lnc * zc), np.exp(-lnc * ze)
+        faces = []
+        for d in range(3):
+            shp = [ext[2], ext[1], ext[0]]
+            shp[2 - d] += 1
+            u = rng.uniform(-1, 1, shp)
+            if d == 2 and org[2] == 0:
+                u[0] = 0.0
+            if d == 2 and l == 0:
+                u[-1] = 0.0
+            faces.append(u)
+        zg = (np.arange(-1, ext[2] + 1) + org[2] + 0.5) * h
+        rho = np.exp(-lnc * np.clip(zg, 0.5 * h, 1 - 0.5 * h))[:, None, None] * \
+            (1 + 0.01 * rng.standard_normal((ext[2] + 2, ext[1] + 2, ext[0] + 2)))
+        if l == 0:
+            rho = W._fill_rho_ghosts(rho[1:-1, 1:-1, 1:-1], bc, 3)
+        levels.append((ext, org, faces, rho, b0, b0e))
+    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    lays, args = [], []
+    for l, (ext, org, faces, rho, b0, b0e) in enumerate(levels):
+        rel = W.box_list(ext, box)
+        boxes = [([lo[d] + org[d] for d in range(3)], [hi[d] + org[d] for d in range(3)]) for lo, hi in rel]
+        sbc = bc if l == 0 else [[W.SLIP_WALL, W.SLIP_WALL]] * 3
+        du = [[cv(BX.split_faces(faces[d], d, lo, hi, 1, 3, sbc)) for d in range(3)] for lo, hi in rel]
+        dp = [cv(np.zeros(BX.box_shape(lo, hi, 1, 3))) for lo, hi in rel]
+        dr = [cv(BX.split_cells(rho, lo, hi, 1, 3, sbc, glob_ghost=1)) for lo, hi in rel]
+        n = nc * (1 + l)
+        lays.append(mac.Layout(dim=3, boxes=boxes, domain_lo=(0, 0, 0), domain_hi=(n - 1,) * 3,
+                               dx=(1.0 / n,) * 3, phys_bc=bc))
+        args.append((du, dp, dr, cv(b0), cv(b0e)))
+    plan = mac.MacMLPlan(lays, mac.default_opts())
+    u0 = [[[a.clone() for a in b] for b in x[0]] for x in args]
+    ts, st = [], None
+    for k in range(warmup + steps):
+        for l in range(2):
+            for b, b_ in zip(args[l][0], u0[l]):
+                for a, a0 in zip(b, b_):
+                    a.copy_(a0)
+            for a in args[l][1]:
+                a.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = plan.project([x[0] for x in args], [x[1] for x in args], [x[2] for x in args], None,
+                          [x[3] for x in args], [x[4] for x in args])
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    plan.close()
+    tm = float(np.median(ts[warmup:]))
+    out.update({"ms": round(1e3 * tm, 3), "composite_steps": st["iters"], "inner_vcycles": st["inner_cycles"],
+                "res_over_res0": st["res"] / st["res0"], "cells": nc ** 3 + int(np.prod(nr)),
+                "setup_ms": round(st["t_setup_ms"], 3), "solve_ms": round(st["t_solve_ms"], 3),
+                "correct_ms": round(st["t_correct_ms"], 3)})
+    print(f"# two-level 128^3 + 128^3: {1e3 * tm:.2f} ms, {st['iters']} composite steps", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def time_adjoint_256(torch, steps=3, warmup=1):
     """The 256^3 differentiable workload: C1's inputs, L = sum(w phi) +
     1/2 ||U_p||^2 with w ~ N(0, 1), mixed-precision forward (the projection
@@ -440,7 +522,8 @@ def main():
     ap.add_argument("--no-producers", action="store_true",
                     help="skip timing make_macrhs / make_s0mac and the Helmholtz mac_solve on the bench layout")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip timing the 512^3 star (mixed V/F, FP64) and the 256^3 adjoint")
+                    help="skip timing the 512^3 star (mixed V/F, FP64), the 256^3 adjoint, the slab share "
+                         "and the two-level projection")
     ap.add_argument("--headline", default="mixed", choices=("mixed", "mixed_gauss_seidel", "mixed_bf16"),
                     help="the mixed path whose throughput is `value`")
     ap.add_argument("--inner-cycles", type=int, default=0,
@@ -536,6 +619,7 @@ def main():
     c3 = None if args.no_configs else time_star_512(torch)
     c4 = None if args.no_configs else time_adjoint_256(torch)
     slab = None if args.no_configs else time_slab_share(torch)
+    two_level = None if args.no_configs else time_two_level(torch)
 
     head = args.headline
     mixed, fp64 = results[head], results["fp64"]
@@ -652,6 +736,7 @@ def main():
         "c3_star_512": c3,
         "c4_adjoint_256": c4,
         "slab_share_256x256x32": slab,
+        "two_level_128": two_level,
     }
     print(json.dumps(line))
 

@@ -347,9 +347,9 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
  * phi_1 (C: the coarse value across the face moved to the ghost's
  * transverse position with linear slopes), coarse cells there take the mean
  * of the fine fluxes.  Solved by fast adaptive composite iteration with the
- * two levels' V-cycles to min(eps_rel * 10, 1e-8) relative (:217).  Device
- * pointers only (opts.device_ptrs = 1); beta0_cart_edge (spherical) is not
- * taken.  Per level l: umac[l][box*dim + d], phi[l][box], rho[l][box]
+ * two levels' V-cycles to min(eps_rel * 10, 1e-8) relative (:217).  Host
+ * pointers (opts.device_ptrs = 0) are copied through one device allocation
+ * per call; beta0_cart_edge (spherical) is not taken.  Per level l: umac[l][box*dim + d], phi[l][box], rho[l][box]
  * (one filled ghost layer), macrhs[l][box] or macrhs NULL, beta0_1d[l]
  * and beta0_1d_edge[l] over level l's whole radial extent or both NULL.
  * Returned: both levels' umac (ghost faces filled: the coarse level as by
@@ -359,6 +359,7 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
 typedef struct mac_ml_plan mac_ml_plan;
 int mac_ml_plan_create(int nlevs, const mac_layout *layouts, const mac_opts *opts, mac_ml_plan **plan);
 int mac_ml_plan_destroy(mac_ml_plan *plan);
+int mac_ml_plan_set_opts(mac_ml_plan *plan, const mac_opts *opts);
 int mac_ml_project(mac_ml_plan *plan, double *const *const *umac, double *const *const *phi,
                    const double *const *const *rho, const double *const *const *macrhs,
                    const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *stats);

@@ -2312,8 +2312,9 @@ struct mac_ml_plan {
     mac_opts opts;
 };
 
-static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve
+static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve (mixed)
 static constexpr int ML_SUB_MAXIT = 2;       // ... and its cycle limit
+static constexpr int ML_FP64_CYCLES = 1;     // FP64 inner solves: V-cycles per correction, no norms
 static constexpr double ML_CF_WEIGHT = 4.0 / 15.0;   // see k_ml_cf_weights
 
 extern "C" int mac_ml_plan_destroy(mac_ml_plan *M)
@@ -2455,6 +2456,13 @@ extern "C" int mac_ml_plan_create(int nlevs, const mac_layout *lays, const mac_o
     return MAC_OK;
 }
 
+extern "C" int mac_ml_plan_set_opts(mac_ml_plan *M, const mac_opts *o)
+{
+    if (!M || !o) return MAC_ERR_INVALID;
+    M->opts = *o;
+    return MAC_OK;
+}
+
 // copy of a plan's solver options for the inner correction solves
 static void ml_inner_opts(mac_ml_plan *M, mac_plan *P)
 {
@@ -2465,6 +2473,23 @@ static void ml_inner_opts(mac_ml_plan *M, mac_plan *P)
     P->opts.max_iter = ML_SUB_MAXIT;
     P->opts.verbose = 0;
     P->planar = false;
+}
+
+// one inner correction solve on level 0 of P (right side written, phi
+// cleared): FP64, a fixed number of V-cycles without residual norms (no
+// host round trip); the mixed precisions, refinement steps to ML_SUB_EPS
+static int ml_inner(mac_ml_plan *M, mac_plan *P, int *cycles)
+{
+    if (M->opts.precision == MAC_PREC_FP64) {
+        for (int c = 0; c < ML_FP64_CYCLES; ++c) vcycle<double>(P, P->H64);
+        *cycles += ML_FP64_CYCLES;
+        return MAC_OK;
+    }
+    mac_stats is;
+    const int src = solve(P, &is);
+    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
+    *cycles += is.inner_cycles;
+    return MAC_OK;
 }
 
 // the composite residual's max norm (read back; synchronises the stream)
@@ -2487,13 +2512,10 @@ static int ml_norm(mac_ml_plan *M, const MlState &S, bool fine_only, double *out
     return MAC_OK;
 }
 
-extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double *const *const *phi,
-                              const double *const *const *rho, const double *const *const *macrhs,
-                              const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *st)
+static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *const *const *phi,
+                          const double *const *const *rho, const double *const *const *macrhs,
+                          const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *st)
 {
-    if (!M || !umac || !phi || !rho || !umac[0] || !umac[1] || !phi[0] || !phi[1] || !rho[0] || !rho[1])
-        return MAC_ERR_INVALID;
-    if (!M->opts.device_ptrs) return MAC_ERR_INVALID;
     const bool b0 = beta0_1d != nullptr;
     if (b0 && (!beta0_1d_edge || !beta0_1d[0] || !beta0_1d[1] || !beta0_1d_edge[0] || !beta0_1d_edge[1]))
         return MAC_ERR_INVALID;
@@ -2596,6 +2618,11 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
         launch_level_setup(dim, IO, P->H64.lev[0], P->part, s);
         if (l == 1) launch_ml_cf_weights(G, P->H64.lev[0], ML_CF_WEIGHT, s);
         ml_inner_opts(M, P);
+        if (M->opts.precision == MAC_PREC_FP64) {
+            // the weights stay for the whole call: the hierarchy once
+            set_omega(P->H64, sweep_omega(P));
+            coarsen_all<double>(P, P->H64);
+        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(P0->ev[1], s));
@@ -2624,19 +2651,14 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
             if ((rc = remove_mean(P0, C0.rhs, ncc))) return rc;
         }
         HIPCHK(hipMemsetAsync(C0.phi, 0, sizeof(double) * ncc, s));
-        mac_stats is;
-        int src = solve(P0, &is);
-        if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
-        inner += is.iters;
+        if ((rc = ml_inner(M, P0, &inner))) return rc;
         launch_ml_prolong(G, C0.phi, M->phc, M->phf, s);
         // fine correction on the residual left by it
         Lev<double> &F0 = P1->H64.lev[0];
         HIPCHK(hipMemsetAsync(P0->nrm_dev, 0, sizeof(unsigned long long), s));
         launch_ml_resid(S, C0.rhs, F0.rhs, P0->nrm_dev, true, s);
         HIPCHK(hipMemsetAsync(F0.phi, 0, sizeof(double) * nfc, s));
-        src = solve(P1, &is);
-        if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
-        inner += is.iters;
+        if ((rc = ml_inner(M, P1, &inner))) return rc;
         launch_ml_axpy(nfc, F0.phi, M->phf, s);
         if ((rc = ml_norm(M, S, false, &r, nullptr))) return rc;
         ++it;
@@ -2713,4 +2735,94 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
         st->t_total_ms = ev_ms(P0, 0, 3);
     }
     return conv ? MAC_OK : MAC_ERR_NOT_CONVERGED;
+}
+
+// host arrays: each level's per-box arrays (ghosts included) and planar
+// profiles copied into one device allocation per call, the device path run
+// on them, umac and phi copied back
+extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double *const *const *phi,
+                              const double *const *const *rho, const double *const *const *macrhs,
+                              const double *const *beta0_1d, const double *const *beta0_1d_edge, mac_stats *st)
+{
+    if (!M || !umac || !phi || !rho || !umac[0] || !umac[1] || !phi[0] || !phi[1] || !rho[0] || !rho[1])
+        return MAC_ERR_INVALID;
+    const bool b0 = beta0_1d != nullptr;
+    if (b0 && (!beta0_1d_edge || !beta0_1d[0] || !beta0_1d[1] || !beta0_1d_edge[0] || !beta0_1d_edge[1]))
+        return MAC_ERR_INVALID;
+    if (!b0 && beta0_1d_edge) return MAC_ERR_INVALID;
+    const bool mr = macrhs != nullptr;
+    if (mr && (!macrhs[0] || !macrhs[1])) return MAC_ERR_INVALID;
+    if (M->opts.device_ptrs) return ml_project_dev(M, umac, phi, rho, macrhs, beta0_1d, beta0_1d_edge, st);
+    const int dim = M->dim;
+    hipStream_t s = M->P[0]->stream;
+    HIPCHK(hipSetDevice(M->P[0]->device));
+    // sizes (doubles) of every array, in one list per level
+    struct Arr { const double *host; double *dev; size_t n; bool back; };
+    std::vector<Arr> arrs;
+    std::vector<double *> du[2], dp[2], dr[2], dm[2];
+    auto cells = [&](const BoxDev &B, int ng, int fd) {
+        size_t v = 1;
+        for (int e = 0; e < dim; ++e) v *= (size_t)(B.hi[e] - B.lo[e] + 1 + 2 * ng + (e == fd));
+        return v;
+    };
+    size_t tot = 0;
+    for (int l = 0; l < 2; ++l) {
+        mac_plan *P = M->P[l];
+        const int nb = P->nboxes;
+        for (int b = 0; b < nb; ++b) {
+            const BoxDev &B = P->boxes[b];
+            for (int d = 0; d < dim; ++d) arrs.push_back({umac[l][b * dim + d], nullptr, cells(B, P->ng_u, d), true});
+            arrs.push_back({phi[l][b], nullptr, cells(B, P->ng_phi, -1), true});
+            arrs.push_back({rho[l][b], nullptr, cells(B, P->ng_rho, -1), false});
+            if (mr) arrs.push_back({macrhs[l][b], nullptr, cells(B, P->ng_rhs, -1), false});
+        }
+        if (b0) {
+            const int nr = (l ? 2 : 1) * M->G.nc[dim - 1];
+            arrs.push_back({beta0_1d[l], nullptr, (size_t)nr, false});
+            arrs.push_back({beta0_1d_edge[l], nullptr, (size_t)nr + 1, false});
+        }
+    }
+    for (Arr &a : arrs) {
+        if (!a.host) return MAC_ERR_INVALID;
+        tot += (a.n + 31) / 32 * 32;
+    }
+    double *blk = nullptr;
+    if (hipMalloc(&blk, tot * sizeof(double)) != hipSuccess) return MAC_ERR_NOMEM;
+    size_t off = 0;
+    int rc = MAC_OK;
+    for (Arr &a : arrs) {
+        a.dev = blk + off;
+        off += (a.n + 31) / 32 * 32;
+        if (hipMemcpyAsync(a.dev, a.host, a.n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess) rc = MAC_ERR_HIP;
+    }
+    if (rc == MAC_OK) {
+        size_t q = 0;
+        const double *db0[2] = {nullptr, nullptr}, *db0e[2] = {nullptr, nullptr};
+        for (int l = 0; l < 2; ++l) {
+            const int nb = M->P[l]->nboxes;
+            for (int b = 0; b < nb; ++b) {
+                for (int d = 0; d < dim; ++d) du[l].push_back(arrs[q++].dev);
+                dp[l].push_back(arrs[q++].dev);
+                dr[l].push_back(arrs[q++].dev);
+                if (mr) dm[l].push_back(arrs[q++].dev);
+            }
+            if (b0) {
+                db0[l] = arrs[q++].dev;
+                db0e[l] = arrs[q++].dev;
+            }
+        }
+        double *const *u2[2] = {du[0].data(), du[1].data()};
+        double *const *p2[2] = {dp[0].data(), dp[1].data()};
+        const double *const *r2[2] = {dr[0].data(), dr[1].data()};
+        const double *const *m2[2] = {dm[0].data(), dm[1].data()};
+        rc = ml_project_dev(M, u2, p2, r2, mr ? m2 : nullptr, b0 ? db0 : nullptr, b0 ? db0e : nullptr, st);
+        if (rc == MAC_OK || rc == MAC_ERR_NOT_CONVERGED)
+            for (Arr &a : arrs)
+                if (a.back && hipMemcpyAsync((void *)a.host, a.dev, a.n * sizeof(double), hipMemcpyDeviceToHost, s) !=
+                                  hipSuccess)
+                    rc = MAC_ERR_HIP;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && rc == MAC_OK) rc = MAC_ERR_HIP;
+    (void)hipFree(blk);
+    return rc;
 }

@@ -40,7 +40,7 @@ SYMBOLS = ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_
            "mac_cell_to_edge", "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
            "mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
            "mac_plan_profile", "mac_plan_profile_read", "mac_ml_plan_create", "mac_ml_plan_destroy",
-           "mac_ml_project")
+           "mac_ml_project", "mac_ml_plan_set_opts")
 PROF = {"gsrb64": 0, "gsrb32": 1, "resid64": 2, "setup": 3, "correct": 4, "coarse": 5,
         "gsrb64_zero": 6, "gsrb32_zero": 7, "restrict64": 8, "restrict32": 9, "prolong64": 10,
         "prolong32": 11, "ir_update": 12}
@@ -151,17 +151,18 @@ def lib():
         L.mac_debug_op.argtypes = [vp, ip, ip, ip]
         L.mac_ml_plan_create.argtypes = [ip, C.POINTER(CLayout), C.POINTER(COpts), C.POINTER(vp)]
         L.mac_ml_plan_destroy.argtypes = [vp]
+        L.mac_ml_plan_set_opts.argtypes = [vp, C.POINTER(COpts)]
         L.mac_ml_project.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.POINTER(CStats)]
         L.mac_plan_profile.argtypes = [vp, ip]
         L.mac_plan_profile_read.argtypes = [vp, ip, C.POINTER(C.c_long), C.POINTER(C.c_double)]
         for f in ("mac_debug_level_dims", "mac_debug_get", "mac_debug_set", "mac_debug_op",
                   "mac_plan_profile", "mac_plan_profile_read", "mac_ml_plan_create", "mac_ml_plan_destroy",
-           "mac_ml_project"):
+           "mac_ml_project", "mac_ml_plan_set_opts"):
             getattr(L, f).restype = ip
         for f in ("mac_default_opts", "mac_plan_create", "mac_plan_destroy", "mac_plan_set_opts",
                   "mac_project", "mac_project_level", "mac_adjoint_level", "mac_cell_to_edge",
                   "mac_make_macrhs", "mac_make_s0mac", "mac_solve", "mac_ell_bc",
-                  "mac_ml_plan_create", "mac_ml_plan_destroy", "mac_ml_project"):
+                  "mac_ml_plan_create", "mac_ml_plan_destroy", "mac_ml_project", "mac_ml_plan_set_opts"):
             getattr(L, f).restype = ip
         _lib = L
     return _lib
@@ -538,7 +539,8 @@ class MacPlan:
 class MacMLPlan:
     """Two-level plan (``mac_ml_plan_create``): ``layouts`` = [coarse, fine],
     the fine level's boxes tiling one rectangle of whole coarse cells.  The
-    arrays of ``project`` are device tensors (per level, per box)."""
+    arrays of ``project`` are all device tensors or all host numpy arrays
+    (per level, per box)."""
 
     def __init__(self, layouts, opts: COpts | None = None, **kw):
         if len(layouts) != 2:
@@ -571,6 +573,7 @@ class MacMLPlan:
         level l's planar profile.  Returns the stats dict."""
         dim = self.layouts[0].dim
         keep = []
+        kinds = set()
 
         def table(xs, l, per_box, name):
             nb = len(self.layouts[l].boxes)
@@ -585,8 +588,7 @@ class MacMLPlan:
             if len(t) != nb * per_box:
                 raise ValueError(f"{name}[{l}]: {nb * per_box} arrays for {nb} boxes, got {len(t)}")
             items = t.items if isinstance(t, PtrTable) else t
-            if not all(_is_torch(x) for x in items):
-                raise ValueError("mac_ml_project takes device tensors only")
+            kinds.update(_is_torch(x) for x in items)
             a = _ptr_array(t)
             keep.append(a)
             return C.cast(a, C.c_void_p).value
@@ -601,8 +603,7 @@ class MacMLPlan:
         def vec(xs):
             if xs is None:
                 return None
-            if not all(_is_torch(x) for x in xs):
-                raise ValueError("mac_ml_project takes device tensors only")
+            kinds.update(_is_torch(x) for x in xs)
             arr = (C.c_void_p * 2)(*[_ptr(x).value for x in xs])
             keep.append(arr)
             return arr
@@ -612,8 +613,17 @@ class MacMLPlan:
         r = per_level(rho, 1, "rho")
         m = per_level(macrhs, 1, "macrhs")
         b0, b0e = vec(beta0_1d), vec(beta0_1d_edge)
-        import torch
-        torch.cuda.current_stream().synchronize()
+        if len(kinds) != 1:
+            raise ValueError("mac_ml_project: all arrays must be device tensors or all host arrays")
+        dev = kinds.pop()
+        if bool(self.opts.device_ptrs) != dev:
+            self.opts.device_ptrs = int(dev)
+            rc = lib().mac_ml_plan_set_opts(self._h, C.byref(self.opts))
+            if rc:
+                raise MacError(rc, "mac_ml_plan_set_opts")
+        if dev:
+            import torch
+            torch.cuda.current_stream().synchronize()
         st = CStats()
         rc = lib().mac_ml_project(self._h, u, p, r, m, b0, b0e, C.byref(st))
         if rc and (check or rc != 4):

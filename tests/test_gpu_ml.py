@@ -31,7 +31,7 @@ def chunks(lo, hi, size, dim):
     return out
 
 
-def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-13, precision=0):
+def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-13, precision=0, host=False):
     import torch
     from maestro_amd import mac
     dim = tl.dim
@@ -43,7 +43,11 @@ def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps
     # the region as a domain of its own for the split: periodic only where it
     # spans a periodic direction (rho's ghosts come from the region array)
     rbc = [[P, P] if tl.side[d][0] == "per" else [WALL, WALL] for d in range(dim)]
-    cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if host:
+        cv = lambda a: np.array(a, dtype=np.float64, order="C", copy=True)
+    else:
+        cv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    npy = (lambda a: a) if host else (lambda a: a.cpu().numpy())
     lv = []
     for boxes, faces, rho, b, mr in ((cb, fc, rc, bc, None if macrhs is None else macrhs[0]),
                                      (rel, ff, rf, rbc, None if macrhs is None else macrhs[1])):
@@ -66,12 +70,13 @@ def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps
                       None if macrhs is None else [lv[0][3], lv[1][3]], b0, b0e)
     torch.cuda.synchronize()
     out = {"stats": st, "umac": [lv[0][0], lv[1][0]], "boxes": [cb, fb]}
-    out["faces_c"] = [BX.gather_faces([box[d].cpu().numpy() for box in lv[0][0]], cb, d, 1, nc, dim)
+    out["faces_c"] = [BX.gather_faces([npy(box[d]) for box in lv[0][0]], cb, d, 1, nc, dim)
                       for d in range(dim)]
-    out["faces_f"] = [BX.gather_faces([box[d].cpu().numpy() for box in lv[1][0]], rel, d, 1, tl.nr[:dim], dim)
+    out["faces_f"] = [BX.gather_faces([npy(box[d]) for box in lv[1][0]], rel, d, 1, tl.nr[:dim], dim)
                       for d in range(dim)]
-    out["phi_c"] = gather_cells([a.cpu().numpy() for a in lv[0][1]], cb, 1, nc, dim)
-    out["phi_f"] = gather_cells([a.cpu().numpy() for a in lv[1][1]], rel, 1, tl.nr[:dim], dim)
+    out["phi_c"] = gather_cells([npy(a) for a in lv[0][1]], cb, 1, nc, dim)
+    out["phi_f"] = gather_cells([npy(a) for a in lv[1][1]], rel, 1, tl.nr[:dim], dim)
+    out["ghosts"] = [[npy(a) for box in lv[l][0] for a in box] for l in range(2)]
     plan.close()
     return out
 
@@ -217,3 +222,22 @@ def test_ml_fine_ghost_faces(oracle_lib):
                     assert abs(a[jj, ii] - exp) <= 1e-14 * max(1.0, abs(exp)), (d, gi, gj)
                     checked += 1
     assert checked > 0
+
+
+def test_ml_host_arrays_match_device(oracle_lib):
+    """Host arrays (copied in and out by the library) give the device
+    path's result bit for bit, ghost faces included."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, lo, hi = REGIONS[5]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 37)
+    rng = np.random.default_rng(8)
+    mf = rng.standard_normal((tl.nr[2], tl.nr[1], tl.nr[0]))
+    mc = tl.restrict_cells(rng.standard_normal((n, n, n)), mf)
+    a = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=4, box_f=4, macrhs=(mc, mf))
+    b = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=4, box_f=4, macrhs=(mc, mf), host=True)
+    assert a["stats"]["iters"] == b["stats"]["iters"]
+    for l in range(2):
+        for x, y in zip(a["ghosts"][l], b["ghosts"][l]):
+            assert np.array_equal(x, y)
+    assert np.array_equal(a["phi_c"], b["phi_c"]) and np.array_equal(a["phi_f"], b["phi_f"])
