@@ -401,7 +401,7 @@ def time_two_level(torch, steps=3, warmup=1):
         lays.append(mac.Layout(dim=3, boxes=boxes, domain_lo=(0, 0, 0), domain_hi=(n - 1,) * 3,
                                dx=(1.0 / n,) * 3, phys_bc=bc))
         args.append((du, dp, dr, cv(b0), cv(b0e)))
-    plan = mac.MacMLPlan(lays, mac.default_opts())
+    plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=1e-9))
     u0 = [[[a.clone() for a in b] for b in x[0]] for x in args]
     ts, st = [], None
     for k in range(warmup + steps):

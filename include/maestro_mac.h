@@ -347,7 +347,8 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
  * phi_1 (C: the coarse value across the face moved to the ghost's
  * transverse position with linear slopes), coarse cells there take the mean
  * of the fine fluxes.  Solved by fast adaptive composite iteration with the
- * two levels' V-cycles to min(eps_rel * 10, 1e-8) relative (:217).  Host
+ * two levels' V-cycles to opts.eps_rel relative (MAESTRO passes
+ * min(eps_mac * 10^(nlevs-1), eps_mac_max) = 1e-9 for two levels, :217).  Host
  * pointers (opts.device_ptrs = 0) are copied through one device allocation
  * per call; beta0_cart_edge (spherical) is not taken.  Per level l: umac[l][box*dim + d], phi[l][box], rho[l][box]
  * (one filled ghost layer), macrhs[l][box] or macrhs NULL, beta0_1d[l]

@@ -73,7 +73,9 @@ def build_fortran(verbose=False):
     _run([fc, "-O2", "-fPIC", "-shared", "-J", FDIR, mod, "-o", shim, f"-L{HERE}",
           "-lmaestro_mac", f"-Wl,-rpath,{HERE}"], cwd=FDIR)
     for src, out in ((drv, exe), (os.path.join(FDIR, "mac_multigrid_driver.f90"),
-                                  os.path.join(FDIR, "mac_multigrid_driver"))):
+                                  os.path.join(FDIR, "mac_multigrid_driver")),
+                     (os.path.join(FDIR, "macproject_ml_driver.f90"),
+                      os.path.join(FDIR, "macproject_ml_driver"))):
         if os.path.exists(src):
             _run([fc, "-O2", "-J", FDIR, src, "-o", out, f"-L{FDIR}", "-lmacproject_hip",
                   f"-L{HERE}", "-lmaestro_mac", f"-Wl,-rpath,{FDIR}", f"-Wl,-rpath,{HERE}",

@@ -2282,7 +2282,7 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
 // macproject with nlevs = 2 (Source/macproject.f90:51-328): both levels'
 // beta0 U, the edge restriction of the fine faces onto the coarse ones
 // (:357-361), rh per level and its cell restriction (:389-404), beta per
-// level (:178-193), the composite solve to min(eps_rel 10, 1e-8) (:217),
+// level (:178-193), the composite solve (:227-229),
 // mkumac per level with the composite fluxes, the edge restriction again
 // (:829-833), U / beta0, and the ghost faces: the coarse level's from its
 // boxes, periodic images and physical rules, the fine level's from the fine
@@ -2634,7 +2634,7 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
         S.bf[d] = M->bff[d];
     }
     S.rhc = M->rhc; S.rhf = M->rhf; S.phc = M->phc; S.phf = M->phf;
-    const double eps = std::min(M->opts.eps_rel * 10.0, 1e-8);   // macproject.f90:217, nlevs = 2
+    const double eps = M->opts.eps_rel;   // the caller's min(eps_mac 10^(nlevs-1), eps_mac_max), :217
     double r0 = 0, r = 0;
     int bad = 0;
     if ((rc = ml_norm(M, S, false, &r0, &bad))) return rc;

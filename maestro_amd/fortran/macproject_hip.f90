@@ -40,8 +40,14 @@ module mac_types_module
      integer :: nboxes = 0
      integer, allocatable :: lo(:,:), hi(:,:)   ! (3, nboxes), inclusive, 0-based
      integer :: pd_lo(3) = 0, pd_hi(3) = 0     ! problem domain
+     ! level 2 of a two-level hierarchy (nlevel = 2, refinement ratio 2): its
+     ! boxes, in level-2 cell indices; its domain is the level-1 one refined
+     integer :: nboxes2 = 0
+     integer, allocatable :: lo2(:,:), hi2(:,:)
      type(c_ptr) :: plan = c_null_ptr
      integer :: plan_ng(5) = -1
+     type(c_ptr) :: mlplan = c_null_ptr
+     integer :: mlplan_ng(5) = -1
   end type ml_layout
 
   type bc_tower
@@ -78,6 +84,34 @@ contains
        mf%fabs(b)%p = 0.0_dp_t
     end do
   end subroutine multifab_build
+
+  ! multifab_build on level lev (1 or 2) of the layout
+  subroutine multifab_build_level(mf, mla, lev, ng, nodal_dir)
+    type(multifab), intent(inout) :: mf
+    type(ml_layout), intent(in) :: mla
+    integer, intent(in) :: lev, ng, nodal_dir
+    integer :: b, l(3), h(3), d, nb
+    if (lev == 1) then
+       call multifab_build(mf, mla, ng, nodal_dir)
+       return
+    end if
+    nb = mla%nboxes2
+    mf%ng = ng
+    mf%nodal_dir = nodal_dir
+    if (allocated(mf%fabs)) deallocate(mf%fabs)
+    allocate(mf%fabs(nb))
+    do b = 1, nb
+       l = 0
+       h = 0
+       do d = 1, mla%dim
+          l(d) = mla%lo2(d,b) - ng
+          h(d) = mla%hi2(d,b) + ng
+          if (d == nodal_dir) h(d) = h(d) + 1
+       end do
+       allocate(mf%fabs(b)%p(l(1):h(1), l(2):h(2), l(3):h(3)))
+       mf%fabs(b)%p = 0.0_dp_t
+    end do
+  end subroutine multifab_build_level
 
 end module mac_types_module
 
@@ -161,6 +195,28 @@ module macproject_module
        type(c_ptr), value :: plan, umac, phi, rho, macrhs, b0, b0e, cart
        type(c_mac_stats), intent(out) :: st
      end function mac_project
+     integer(c_int) function mac_ml_plan_create(nlevs, lays, o, plan) bind(C, name="mac_ml_plan_create")
+       import :: c_int, c_ptr, c_mac_layout, c_mac_opts
+       integer(c_int), value :: nlevs
+       type(c_mac_layout), intent(in) :: lays(*)
+       type(c_mac_opts), intent(in) :: o
+       type(c_ptr), intent(out) :: plan
+     end function mac_ml_plan_create
+     integer(c_int) function mac_ml_plan_destroy(plan) bind(C, name="mac_ml_plan_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: plan
+     end function mac_ml_plan_destroy
+     integer(c_int) function mac_ml_plan_set_opts(plan, o) bind(C, name="mac_ml_plan_set_opts")
+       import :: c_int, c_ptr, c_mac_opts
+       type(c_ptr), value :: plan
+       type(c_mac_opts), intent(in) :: o
+     end function mac_ml_plan_set_opts
+     integer(c_int) function mac_ml_project(plan, umac, phi, rho, macrhs, b0, b0e, st) &
+          bind(C, name="mac_ml_project")
+       import :: c_int, c_ptr, c_mac_stats
+       type(c_ptr), value :: plan, umac, phi, rho, macrhs, b0, b0e
+       type(c_mac_stats), intent(out) :: st
+     end function mac_ml_project
   end interface
 
 contains
@@ -198,7 +254,12 @@ contains
     dm = mla%dim
     nb = mla%nboxes
 
-    if (nlevs /= 1) call fail('only single-level projections are supported, nlevs = ', nlevs)
+    if (nlevs == 2) then
+       call macproject_2lev(mla, umac, phi, rho, dx, the_bc_tower, macrhs, beta0_1d, beta0_1d_edge, &
+                            beta0_cart_edge)
+       return
+    end if
+    if (nlevs /= 1) call fail('one or two levels are supported, nlevs = ', nlevs)
     ! option combinations, Source/macproject.f90:99-109
     if (present(beta0_1d) .and. present(beta0_cart_edge)) &
          call fail('CANT HAVE 1D and 3D DIV_COEFF IN MACPROJECT', 1)
@@ -328,7 +389,160 @@ contains
     if (c_associated(mla%plan)) rc = mac_plan_destroy(mla%plan)
     mla%plan = c_null_ptr
     mla%plan_ng = -1
+    if (c_associated(mla%mlplan)) rc = mac_ml_plan_destroy(mla%mlplan)
+    mla%mlplan = c_null_ptr
+    mla%mlplan_ng = -1
   end subroutine macproject_release
+
+  ! macproject with nlevs = 2 (Source/macproject.f90:51-328, the nlevs > 1
+  ! branches) through mac_ml_project: level 1 on mla's boxes, level 2 on its
+  ! level-2 boxes; dx(n,:) and beta0_1d(n,0:nr_n-1) per level (nr_2 = 2 nr_1)
+  subroutine macproject_2lev(mla, umac, phi, rho, dx, the_bc_tower, &
+                             macrhs, beta0_1d, beta0_1d_edge, beta0_cart_edge)
+    type(ml_layout), intent(inout) :: mla
+    type(multifab), intent(inout), target :: umac(:,:)
+    type(multifab), intent(inout), target :: phi(:)
+    type(multifab), intent(in), target :: rho(:)
+    real(dp_t), intent(in) :: dx(:,:)
+    type(bc_tower), intent(in) :: the_bc_tower
+    type(multifab), intent(in), optional, target :: macrhs(:)
+    real(dp_t), intent(in), optional, target :: beta0_1d(:,0:)
+    real(dp_t), intent(in), optional, target :: beta0_1d_edge(:,0:)
+    type(multifab), intent(in), optional, target :: beta0_cart_edge(:,:)
+
+    type(c_mac_opts) :: o
+    type(c_mac_stats) :: st
+    type(c_mac_layout), target :: lays(2)
+    integer(c_int), allocatable, target :: blo1(:), bhi1(:), blo2(:), bhi2(:)
+    type(c_ptr), allocatable, target :: pu1(:), pu2(:), pp1(:), pp2(:), pr1(:), pr2(:), pm1(:), pm2(:)
+    type(c_ptr), target :: tu(2), tp(2), tr(2), tm(2), tb(2), tbe(2)
+    real(dp_t), allocatable, target :: b01(:), b0e1(:), b02(:), b0e2(:)
+    type(c_ptr) :: cm, cb0, cb0e
+    integer :: dm, nb1, nb2, b, d, rc, ng(5), nr, lev
+
+    dm = mla%dim
+    nb1 = mla%nboxes
+    nb2 = mla%nboxes2
+    if (mla%nlevel /= 2 .or. nb2 < 1) call fail('two-level projection without level-2 boxes, nlevel = ', &
+                                                 mla%nlevel)
+    if (present(beta0_cart_edge)) call fail('beta0_cart_edge with two levels is not supported', 2)
+    if (present(beta0_1d) .neqv. present(beta0_1d_edge)) &
+         call fail('beta0_1d and beta0_1d_edge go together', 1)
+
+    ng = [umac(1,1)%ng, phi(1)%ng, rho(1)%ng, 0, 0]
+    if (present(macrhs)) ng(4) = macrhs(1)%ng
+
+    rc = mac_default_opts(o)
+    o%eps_rel = min(eps_mac * mac_level_factor**(2-1), eps_mac_max)
+    o%eps_abs = -1.0_dp_t
+    o%use_alt_energy_fix = merge(1, 0, use_alt_energy_fix)
+    o%precision = mac_precision
+    o%verbose = mg_verbose
+    o%cycle_type = mg_cycle_type
+    o%nub = mg_bottom_nu
+    if (mg_bottom_solver /= -1 .and. mg_bottom_solver /= 4) o%bottom_solver = mg_bottom_solver
+    o%omega = mac_omega
+    o%device_ptrs = 0
+
+    if (.not. c_associated(mla%mlplan) .or. any(mla%mlplan_ng /= ng)) then
+       if (c_associated(mla%mlplan)) rc = mac_ml_plan_destroy(mla%mlplan)
+       allocate(blo1(3*nb1), bhi1(3*nb1), blo2(3*nb2), bhi2(3*nb2))
+       do b = 1, nb1
+          blo1(3*(b-1)+1:3*b) = mla%lo(:,b)
+          bhi1(3*(b-1)+1:3*b) = mla%hi(:,b)
+       end do
+       do b = 1, nb2
+          blo2(3*(b-1)+1:3*b) = mla%lo2(:,b)
+          bhi2(3*(b-1)+1:3*b) = mla%hi2(:,b)
+       end do
+       do lev = 1, 2
+          lays(lev)%dim = dm
+          lays(lev)%dx = 1.0_dp_t
+          lays(lev)%dx(1:dm) = dx(lev,1:dm)
+          do d = 1, 3
+             lays(lev)%phys_bc(1,d) = the_bc_tower%phys_bc(d,1)
+             lays(lev)%phys_bc(2,d) = the_bc_tower%phys_bc(d,2)
+          end do
+          lays(lev)%domain_lo = 0
+          lays(lev)%ng_umac = ng(1)
+          lays(lev)%ng_phi = ng(2)
+          lays(lev)%ng_rho = ng(3)
+          lays(lev)%ng_macrhs = ng(4)
+          lays(lev)%ng_beta0_cart = 0
+       end do
+       lays(1)%nboxes = nb1
+       lays(1)%box_lo = c_loc(blo1)
+       lays(1)%box_hi = c_loc(bhi1)
+       lays(1)%domain_hi = mla%pd_hi
+       lays(2)%nboxes = nb2
+       lays(2)%box_lo = c_loc(blo2)
+       lays(2)%box_hi = c_loc(bhi2)
+       lays(2)%domain_hi = 0
+       lays(2)%domain_hi(1:dm) = 2 * mla%pd_hi(1:dm) + 1
+       rc = mac_ml_plan_create(2_c_int, lays, o, mla%mlplan)
+       if (rc /= 0) call fail('mac_ml_plan_create failed, code ', rc)
+       mla%mlplan_ng = ng
+    else
+       rc = mac_ml_plan_set_opts(mla%mlplan, o)
+       if (rc /= 0) call fail('mac_ml_plan_set_opts failed, code ', rc)
+    end if
+
+    allocate(pu1(nb1*dm), pp1(nb1), pr1(nb1), pu2(nb2*dm), pp2(nb2), pr2(nb2))
+    do b = 1, nb1
+       do d = 1, dm
+          pu1((b-1)*dm + d) = c_loc(umac(1,d)%fabs(b)%p)
+       end do
+       pp1(b) = c_loc(phi(1)%fabs(b)%p)
+       pr1(b) = c_loc(rho(1)%fabs(b)%p)
+    end do
+    do b = 1, nb2
+       do d = 1, dm
+          pu2((b-1)*dm + d) = c_loc(umac(2,d)%fabs(b)%p)
+       end do
+       pp2(b) = c_loc(phi(2)%fabs(b)%p)
+       pr2(b) = c_loc(rho(2)%fabs(b)%p)
+    end do
+    tu = [c_loc(pu1), c_loc(pu2)]
+    tp = [c_loc(pp1), c_loc(pp2)]
+    tr = [c_loc(pr1), c_loc(pr2)]
+    cm = c_null_ptr
+    if (present(macrhs)) then
+       allocate(pm1(nb1), pm2(nb2))
+       do b = 1, nb1
+          pm1(b) = c_loc(macrhs(1)%fabs(b)%p)
+       end do
+       do b = 1, nb2
+          pm2(b) = c_loc(macrhs(2)%fabs(b)%p)
+       end do
+       tm = [c_loc(pm1), c_loc(pm2)]
+       cm = c_loc(tm)
+    end if
+    cb0 = c_null_ptr
+    cb0e = c_null_ptr
+    if (present(beta0_1d)) then
+       nr = mla%pd_hi(dm) - mla%pd_lo(dm) + 1
+       allocate(b01(0:nr-1), b0e1(0:nr), b02(0:2*nr-1), b0e2(0:2*nr))
+       b01 = beta0_1d(1,0:nr-1)
+       b0e1 = beta0_1d_edge(1,0:nr)
+       b02 = beta0_1d(2,0:2*nr-1)
+       b0e2 = beta0_1d_edge(2,0:2*nr)
+       tb = [c_loc(b01), c_loc(b02)]
+       tbe = [c_loc(b0e1), c_loc(b0e2)]
+       cb0 = c_loc(tb)
+       cb0e = c_loc(tbe)
+    end if
+
+    rc = mac_ml_project(mla%mlplan, c_loc(tu), c_loc(tp), c_loc(tr), cm, cb0, cb0e, st)
+    last_stats = st
+    if (mg_verbose > 0) then
+       write(*,'(a,es15.8)') 'F90mg: Initial rhs                  = ', st%rhs_norm
+       write(*,'(a,es15.8)') 'F90mg: Initial error (error0)       = ', st%res0
+       write(*,'(a,i4,a,es15.8)') 'F90mg: Final Iter. ', st%iters, ' error/error0 = ', &
+            st%res / max(st%res0, tiny(1.0_dp_t))
+    end if
+    if (rc /= 0) call fail('mac_ml_project failed, code ', rc)
+
+  end subroutine macproject_2lev
 
 end module macproject_module
 

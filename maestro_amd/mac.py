@@ -546,7 +546,9 @@ class MacMLPlan:
         if len(layouts) != 2:
             raise ValueError("two levels")
         self.layouts = list(layouts)
-        self.opts = opts or default_opts(**kw)
+        # the relative tolerance of a two-level projection: min(eps_mac *
+        # 10^(nlevs-1), eps_mac_max) = 1e-9 (Source/macproject.f90:217)
+        self.opts = opts or default_opts(**{"eps_rel": 1e-9, **kw})
         self.opts.device_ptrs = 1
         self._c = (CLayout * 2)(*[l.to_c() for l in self.layouts])
         h = C.c_void_p()

@@ -214,3 +214,70 @@ def test_fortran_mac_multigrid_matches_binding_and_oracle(gpu, oracle_lib, tmp_p
     assert rel_err(got, truth) <= max(1e-8, 2.0 * e_ref)
     if prec == 0:
         assert rel_err(got, same) <= 1e-8
+
+
+ML_DRIVER = os.path.join(ROOT, "maestro_amd", "fortran", "macproject_ml_driver")
+
+
+@pytest.mark.parametrize("case", [3, 5])
+def test_fortran_two_level_matches_binding(oracle_lib, tmp_path, case):
+    """macproject with nlevs = 2 from Fortran (the level-2 boxes in the
+    ml_layout, beta0_1d(2, 0:) per level): the same library call as the
+    Python binding's host-array path, so the same bits."""
+    from oracle import ml_oracle as M
+    from .test_ml_oracle import REGIONS, random_two_level
+    from .test_gpu_ml import ml_box_arrays, run_ml_gpu
+    O = oracle_lib
+    dim, n, bc, lo, hi = REGIONS[case]
+    use_b0 = bc[dim - 1][0] != W.PERIODIC
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 41, beta0=use_b0)
+    box = 4 if dim == 3 else 8
+    cb, fb, rel, lv = ml_box_arrays(tl, fc, ff, rc, rf, box, box)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    b3 = [list(bc[d]) for d in range(dim)] + [[-1, -1]] * (3 - dim)
+    hdr = [dim, len(cb), len(fb), 1, 1, 1, 1, int(use_b0), 0, 1]
+    dhi = [n - 1] * dim + [0] * (3 - dim)
+    pbc = [b3[d][0] for d in range(3)] + [b3[d][1] for d in range(3)]
+    with open(fin, "wb") as f:
+        np.array(hdr + dhi + pbc, dtype=np.int32).tofile(f)
+        for boxes in (cb, fb):
+            for l, h in boxes:
+                np.array(list(l) + [0] * (3 - dim) + list(h) + [0] * (3 - dim), dtype=np.int32).tofile(f)
+        np.array([1.0 / n] * dim + [1.0] * (3 - dim), dtype=np.float64).tofile(f)
+        for u, ph, r, _ in lv:
+            for b in range(len(ph)):
+                for d in range(dim):
+                    u[b][d].tofile(f)
+                ph[b].tofile(f)
+                r[b].tofile(f)
+        if use_b0:
+            for key in ("beta0", "beta0_edge", "beta0_f", "beta0_edge_f"):
+                np.asarray(kw[key], dtype=np.float64).tofile(f)
+    r = subprocess.run([ML_DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(fout, dtype=np.uint8)
+    off = 0
+    outs = []
+    for u, ph, _, _ in lv:
+        lev = []
+        for b in range(len(ph)):
+            for a in list(u[b]) + [ph[b]]:
+                nbytes = a.size * 8
+                lev.append(raw[off:off + nbytes].view(np.float64).reshape(a.shape))
+                off += nbytes
+        outs.append(lev)
+    iters, conv = raw[off:off + 8].view(np.int32)
+    assert conv == 1
+    # the binding's host path on the same arrays
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw if use_b0 else {}, box_c=box, box_f=box, eps=1e-9, host=True)
+    for l in range(2):
+        ref = []
+        nbx = len(lv[l][1])
+        for b in range(nbx):
+            ref += g["ghosts"][l][b * dim:(b + 1) * dim]
+        got_u = [x for b in range(nbx) for x in outs[l][b * (dim + 1):b * (dim + 1) + dim]]
+        for x, y in zip(got_u, ref):
+            assert np.array_equal(x, y)
+    dc, df = M.composite_divergence(tl, g["faces_c"], g["faces_f"], **(kw if use_b0 else {}))
+    assert max(np.abs(dc).max(), np.abs(df).max()) < 1e-6
