@@ -2,7 +2,8 @@
  * maestro_mac.h -- C ABI of the MI355X (gfx950) MAC-projection library
  * (libmaestro_mac.so).
  *
- * This library stands in for MAESTRO's single-level MAC projection:
+ * This library stands in for MAESTRO's MAC projection (one level, or two
+ * through mac_ml_project):
  *   macproject(mla, umac, phi, rho, dx, the_bc_tower
  *              [, macrhs][, beta0_1d, beta0_1d_edge][, beta0_cart_edge])
  *     Source/macproject.f90:51-76

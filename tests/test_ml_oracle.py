@@ -68,8 +68,8 @@ def test_composite_operator_exact_on_reproduced_fields(oracle_lib, dim):
     yc, yf = apply(tl, A, Xc ** 2, Xf ** 2)
     mc, mf = interior_mask(tl, 0, 2) & ~cov, interior_mask(tl, 1, 2)
     assert np.abs(yc[mc] + 2).max() < 1e-9 and np.abs(yf[mf] + 2).max() < 1e-9
-    # the weights are pinned: moving a twentieth from the coarse weight to
-    # the first fine one breaks the result
+    # the weights are pinned: moving a twentieth between the coarse weight
+    # and the first fine one breaks the result
     saved = (M.W_COARSE, M.W_FINE0)
     try:
         M.W_COARSE, M.W_FINE0 = saved[0] + 0.05, saved[1] - 0.05
