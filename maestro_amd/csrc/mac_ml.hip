@@ -336,39 +336,89 @@ __global__ void __launch_bounds__(MTPB) k_ml_comp_vec(MlGeom G, const double *rh
 // ------------------------------------------------------ composite residual
 // r = rh - A phi on the uncovered coarse cells (into rc; covered cells left
 // alone) and the fine cells (into rf), max |r| into *nrm.  fine_only: the
-// fine cells only.
+// fine cells only.  Cells whose six neighbours are plain cells of their own
+// level (no wall, wrap, covered cell or region boundary) take the 7-point
+// form directly; the others go through the flux functions.  32-bit index
+// arithmetic (a level holds fewer than 2^31 cells).
 template <int DIM>
 __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double *rf, unsigned long long *nrm,
                                                    int fine_only)
 {
     const MlGeom &G = S.G;
-    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
-    long q = (long)blockIdx.x * MTPB + threadIdx.x;
-    if (fine_only) q += ncc;
+    const int ncc = G.nc[0] * G.nc[1] * G.nc[2], nfc = G.nr[0] * G.nr[1] * G.nr[2];
+    const int qend = ncc + nfc;
+    double amax = 0.0;
+    // grid-stride over the cells: one norm atomic per block, not per wave
+    for (int q = blockIdx.x * MTPB + threadIdx.x + (fine_only ? ncc : 0); q < qend; q += gridDim.x * MTPB) {
     double a = 0.0;
     if (q < ncc) {
-        const int c[3] = {(int)(q % G.nc[0]), (int)((q / G.nc[0]) % G.nc[1]), (int)(q / ((long)G.nc[0] * G.nc[1]))};
+        const int t = q / G.nc[0];
+        const int c[3] = {q - t * G.nc[0], t % G.nc[1], t / G.nc[1]};
         if (!ml_covered(G, c)) {
+            bool plain = true;
+            for (int d = 0; d < DIM; ++d) {
+                if (c[d] == 0 || c[d] == G.nc[d] - 1) plain = false;
+                // a covered neighbour along d: the cell lies just outside the
+                // region's coarse range in d and inside it transversely
+                const int lo = G.rlo[d] >> 1, hi = (G.rlo[d] + G.nr[d] - 1) >> 1;
+                if (c[d] == lo - 1 || c[d] == hi + 1) plain = false;
+            }
             double Ap = 0.0;
-            for (int d = 0; d < DIM; ++d)
-                Ap -= (ml_coarse_flux(S, c, d, 1) - ml_coarse_flux(S, c, d, 0)) * G.ihc[d];
+            if (plain) {
+                const double pc = S.phc[q];
+                const int st[3] = {1, G.nc[0], G.nc[0] * G.nc[1]};
+                for (int d = 0; d < DIM; ++d) {
+                    const long f0 = nat_face(G.nc, d, c[0], c[1], c[2]);
+                    const long f1 = f0 + ((d == 0) ? 1 : (d == 1 ? (G.nc[0]) : (long)G.nc[0] * (G.nc[1])));
+                    const double bl = S.bc[d][f0], bh = S.bc[d][f1];
+                    Ap -= (bh * G.ihc[d] * (S.phc[q + st[d]] - pc) - bl * G.ihc[d] * (pc - S.phc[q - st[d]])) *
+                          G.ihc[d];
+                }
+            } else {
+                for (int d = 0; d < DIM; ++d)
+                    Ap -= (ml_coarse_flux(S, c, d, 1) - ml_coarse_flux(S, c, d, 0)) * G.ihc[d];
+            }
             const double r = S.rhc[q] - Ap;
             rc[q] = r;
             a = fabs(r);
         }
     } else if (q < ncc + nfc) {
-        const long p = q - ncc;
-        const int f[3] = {G.rlo[0] + (int)(p % G.nr[0]), G.rlo[1] + (int)((p / G.nr[0]) % G.nr[1]),
-                          G.rlo[2] + (int)(p / ((long)G.nr[0] * G.nr[1]))};
+        const int p = q - ncc;
+        const int t = p / G.nr[0];
+        const int r3[3] = {p - t * G.nr[0], t % G.nr[1], t / G.nr[1]};
+        bool plain = true;
+        for (int d = 0; d < DIM; ++d)
+            if (r3[d] == 0 || r3[d] == G.nr[d] - 1) plain = false;
         double Ap = 0.0;
-        for (int d = 0; d < DIM; ++d) Ap -= (ml_fine_flux(S, f, d, 1) - ml_fine_flux(S, f, d, 0)) * G.ihf[d];
+        if (plain) {
+            const double pc = S.phf[p];
+            const int st[3] = {1, G.nr[0], G.nr[0] * G.nr[1]};
+            for (int d = 0; d < DIM; ++d) {
+                const long f0 = nat_face(G.nr, d, r3[0], r3[1], r3[2]);
+                const long f1 = f0 + ((d == 0) ? 1 : (d == 1 ? (G.nr[0]) : (long)G.nr[0] * (G.nr[1])));
+                const double bl = S.bf[d][f0], bh = S.bf[d][f1];
+                Ap -= (bh * G.ihf[d] * (S.phf[p + st[d]] - pc) - bl * G.ihf[d] * (pc - S.phf[p - st[d]])) * G.ihf[d];
+            }
+        } else {
+            const int f[3] = {G.rlo[0] + r3[0], G.rlo[1] + r3[1], G.rlo[2] + r3[2]};
+            for (int d = 0; d < DIM; ++d) Ap -= (ml_fine_flux(S, f, d, 1) - ml_fine_flux(S, f, d, 0)) * G.ihf[d];
+        }
         const double r = S.rhf[p] - Ap;
         rf[p] = r;
         a = fabs(r);
     }
     if (!isfinite(a)) a = __builtin_huge_val();
-    a = wave_max(a);
-    if ((threadIdx.x & 63) == 0 && a > 0.0) atomicMax(nrm, (unsigned long long)__double_as_longlong(a));
+    amax = fmax(amax, a);
+    }
+    amax = wave_max(amax);
+    __shared__ double wm[MTPB / 64];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = amax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wm[0];
+        for (int w = 1; w < MTPB / 64; ++w) m = fmax(m, wm[w]);
+        if (m > 0.0) atomicMax(nrm, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // the coarse correction e (whole coarse level) added to phi on both levels:
@@ -376,14 +426,14 @@ __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double
 template <int DIM>
 __global__ void __launch_bounds__(MTPB) k_ml_prolong(MlGeom G, const double *e, double *phc, double *phf)
 {
-    const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
-    const long q = (long)blockIdx.x * MTPB + threadIdx.x;
+    const int ncc = G.nc[0] * G.nc[1] * G.nc[2], nfc = G.nr[0] * G.nr[1] * G.nr[2];
+    const int q = blockIdx.x * MTPB + threadIdx.x;
     if (q < ncc) {
         phc[q] += e[q];
     } else if (q < ncc + nfc) {
-        const long p = q - ncc;
-        const int f[3] = {G.rlo[0] + (int)(p % G.nr[0]), G.rlo[1] + (int)((p / G.nr[0]) % G.nr[1]),
-                          G.rlo[2] + (int)(p / ((long)G.nr[0] * G.nr[1]))};
+        const int p = q - ncc;
+        const int t = p / G.nr[0];
+        const int f[3] = {G.rlo[0] + p - t * G.nr[0], G.rlo[1] + t % G.nr[1], G.rlo[2] + t / G.nr[1]};
         const int C[3] = {f[0] >> 1, f[1] >> 1, f[2] >> 1};
         double sig[3];
         for (int t = 0; t < 3; ++t) sig[t] = (f[t] & 1) ? 0.25 : -0.25;
@@ -617,7 +667,8 @@ void launch_ml_resid(const MlState &S, double *rc, double *rf, unsigned long lon
     const MlGeom &G = S.G;
     const long ncc = (long)G.nc[0] * G.nc[1] * G.nc[2], nfc = (long)G.nr[0] * G.nr[1] * G.nr[2];
     const long n = fine_only ? nfc : ncc + nfc;
-    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_resid<D_>, dim3(mblk(n)), dim3(MTPB), 0, s, S, rc, rf, nrm,
+    const unsigned nb = std::min<unsigned>(mblk(n), 2048u);
+    ML_DIM(G.dim, hipLaunchKernelGGL(k_ml_resid<D_>, dim3(nb), dim3(MTPB), 0, s, S, rc, rf, nrm,
                                      fine_only ? 1 : 0));
 }
 
