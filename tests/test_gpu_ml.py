@@ -55,7 +55,8 @@ def ml_box_arrays(tl, fc, ff, rc, rf, box_c, box_f, macrhs=None):
     return cb, fb, rel, lv
 
 
-def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-12, precision=0, host=False):
+def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-12, precision=0, host=False,
+               omega=0.0, plan_out=None):
     import torch
     from maestro_amd import mac
     dim = tl.dim
@@ -74,7 +75,8 @@ def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps
                        phys_bc=bc, ng_macrhs=0),
             mac.Layout(dim=dim, boxes=fb, domain_lo=(0,) * dim, domain_hi=tuple(2 * n - 1 for n in nc),
                        dx=tuple(h / 2 for h in dxc), phys_bc=bc, ng_macrhs=0)]
-    plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=eps, use_alt_energy_fix=alt, precision=precision))
+    plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=eps, use_alt_energy_fix=alt, precision=precision,
+                                                omega=omega))
     b0 = b0e = None
     if kw:
         b0 = [cv(kw["beta0"]), cv(kw["beta0_f"])]
@@ -180,8 +182,10 @@ def test_ml_reference_two_level_grid(oracle_lib, kind):
     compare(tl, g, r, tol=1e-6)
 
 
-def test_ml_mixed_precision_inner_cycles(oracle_lib):
-    """The inner correction solves in the mixed precision of the plan: the
+@pytest.mark.parametrize("precision,omega", [(1, 0.0), (2, 0.0), (0, 1.25), (1, 1.25)])
+def test_ml_mixed_precision_inner_cycles(oracle_lib, precision, omega):
+    """The inner correction solves in the plan's precision (FP32 or bfloat16
+    storage inside FP64 refinement) and with over-relaxed sweeps: the
     composite residual stays FP64, so the answer is the same."""
     from oracle import ml_oracle as M
     O = oracle_lib
@@ -194,7 +198,7 @@ def test_ml_mixed_precision_inner_cycles(oracle_lib):
         if tl.side[2][s] == "phys":
             ff[2][tuple(idx)] = 0.0
     r = M.macproject2(tl, fc, ff, rc, rf, alt=True, **kw)
-    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, precision=1)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, precision=precision, omega=omega)
     assert g["stats"]["rc"] == 0
     compare(tl, g, r)
 
@@ -254,3 +258,34 @@ def test_ml_host_arrays_match_device(oracle_lib):
         for x, y in zip(a["ghosts"][l], b["ghosts"][l]):
             assert np.array_equal(x, y)
     assert np.array_equal(a["phi_c"], b["phi_c"]) and np.array_equal(a["phi_f"], b["phi_f"])
+
+
+def test_ml_size_properties(oracle_lib):
+    """A 64^3 coarse level with a 64^3-fine-cell block at the bottom wall
+    (2.1 M cells, too many for the direct solve): the composite divergence
+    of beta0 U is at the solver tolerance on every composite cell after the
+    projection, a second projection leaves U within the same tolerance, and
+    the walls' normal faces stay zero."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, lo, hi = 3, 64, [[P, P], [P, P], [WALL, WALL]], [32, 32, 0], [95, 95, 63]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 43)
+    for s_, sl in ((0, 0), (1, -1)):
+        idx = [slice(None)] * 3
+        idx[0] = sl
+        fc[2][tuple(idx)] = 0.0
+        if tl.side[2][s_] == "phys":
+            ff[2][tuple(idx)] = 0.0
+    d0c, d0f = M.composite_divergence(tl, fc, ff, **kw)
+    scale = max(np.abs(d0c).max(), np.abs(d0f).max())
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=32, box_f=32, eps=1e-10)
+    st = g["stats"]
+    print("64^3 + 64^3: composite steps", st["iters"], flush=True)
+    assert st["rc"] == 0
+    dc, df = M.composite_divergence(tl, g["faces_c"], g["faces_f"], **kw)
+    assert max(np.abs(dc).max(), np.abs(df).max()) < 1e-8 * scale
+    assert np.abs(g["faces_c"][2][0]).max() == 0.0 and np.abs(g["faces_f"][2][0]).max() == 0.0
+    g2 = run_ml_gpu(tl, g["faces_c"], g["faces_f"], rc, rf, kw, box_c=32, box_f=32, eps=1e-10)
+    for lev in ("faces_c", "faces_f"):
+        for d in range(dim):
+            assert np.abs(g2[lev][d] - g[lev][d]).max() < 1e-7 * max(1.0, np.abs(g[lev][d]).max())
