@@ -111,3 +111,43 @@ def test_pointer_table_holds_checked_pointers():
         mac.PtrTable([np.zeros((4, 4))[:, ::2]])
     with pytest.raises(ValueError):
         mac.PtrTable([torch.zeros(4, 4, dtype=torch.float64).t()])
+
+
+def _ml_layouts(fine_boxes, dhi_c=(15, 15), bc_f=None, dx_f=None):
+    P = mac.PERIODIC
+    bc = [[P, P], [P, P]]
+    c = mac.Layout(dim=2, boxes=mac.Layout.uniform((16, 16), 8, (1 / 16, 1 / 16), bc).boxes, domain_lo=(0, 0),
+                   domain_hi=dhi_c, dx=(1 / 16, 1 / 16), phys_bc=bc)
+    f = mac.Layout(dim=2, boxes=fine_boxes, domain_lo=(0, 0), domain_hi=(31, 31),
+                   dx=dx_f or (1 / 32, 1 / 32), phys_bc=bc_f or bc)
+    return c, f
+
+
+@pytest.mark.parametrize("case,code", [
+    ("region_not_whole_coarse_cells", 2),   # the refined rectangle starts at an odd fine index
+    ("fine_domain_not_refined", 2),         # the fine domain is not the coarse one refined by 2
+    ("bc_mismatch", 2),                     # the two levels' physical BCs differ
+    ("dx_not_halved", 2),                   # the fine cell size is not half the coarse one
+    ("three_levels", 1),                    # nlevs != 2
+])
+def test_two_level_layouts_rejected(case, code):
+    """mac_ml_plan_create refuses layouts the composite solve does not take,
+    before any device work (so on the CPU too)."""
+    W = mac.SLIP_WALL
+    boxes = [([8, 8], [15, 15]), ([16, 8], [23, 15])]
+    kw = {}
+    if case == "region_not_whole_coarse_cells":
+        boxes = [([9, 8], [16, 15])]
+    elif case == "fine_domain_not_refined":
+        kw["dhi_c"] = (14, 15)
+    elif case == "bc_mismatch":
+        kw["bc_f"] = [[W, W], [mac.PERIODIC, mac.PERIODIC]]
+    elif case == "dx_not_halved":
+        kw["dx_f"] = (1 / 16, 1 / 16)
+    c, f = _ml_layouts(boxes, **kw)
+    arr = (mac.CLayout * 2)(c.to_c(), f.to_c())
+    h = C.c_void_p()
+    nlev = 3 if case == "three_levels" else 2
+    rc = mac.lib().mac_ml_plan_create(nlev, arr, C.byref(mac.default_opts()), C.byref(h))
+    assert rc == code
+    assert not h.value

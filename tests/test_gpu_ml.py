@@ -56,7 +56,7 @@ def ml_box_arrays(tl, fc, ff, rc, rf, box_c, box_f, macrhs=None):
 
 
 def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps=1e-12, precision=0, host=False,
-               omega=0.0, plan_out=None):
+               omega=0.0, max_iter=200, check=True):
     import torch
     from maestro_amd import mac
     dim = tl.dim
@@ -76,13 +76,13 @@ def run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=8, macrhs=None, alt=1, eps
             mac.Layout(dim=dim, boxes=fb, domain_lo=(0,) * dim, domain_hi=tuple(2 * n - 1 for n in nc),
                        dx=tuple(h / 2 for h in dxc), phys_bc=bc, ng_macrhs=0)]
     plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=eps, use_alt_energy_fix=alt, precision=precision,
-                                                omega=omega))
+                                                omega=omega, max_iter=max_iter))
     b0 = b0e = None
     if kw:
         b0 = [cv(kw["beta0"]), cv(kw["beta0_f"])]
         b0e = [cv(kw["beta0_edge"]), cv(kw["beta0_edge_f"])]
     st = plan.project([lv[0][0], lv[1][0]], [lv[0][1], lv[1][1]], [lv[0][2], lv[1][2]],
-                      None if macrhs is None else [lv[0][3], lv[1][3]], b0, b0e)
+                      None if macrhs is None else [lv[0][3], lv[1][3]], b0, b0e, check=check)
     torch.cuda.synchronize()
     out = {"stats": st, "umac": [lv[0][0], lv[1][0]], "boxes": [cb, fb]}
     out["faces_c"] = [BX.gather_faces([npy(box[d]) for box in lv[0][0]], cb, d, 1, nc, dim)
@@ -289,3 +289,24 @@ def test_ml_size_properties(oracle_lib):
     for lev in ("faces_c", "faces_f"):
         for d in range(dim):
             assert np.abs(g2[lev][d] - g[lev][d]).max() < 1e-7 * max(1.0, np.abs(g[lev][d]).max())
+
+
+def test_ml_error_paths(oracle_lib):
+    """A NaN in the fine level's rho returns MAC_ERR_NONFINITE; a composite
+    step limit below what the tolerance needs returns MAC_ERR_NOT_CONVERGED
+    with the stats filled (macproject's error contract, Source/
+    macproject.f90:99-109 and the solver's iteration limit)."""
+    from maestro_amd import mac
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, lo, hi = REGIONS[1]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 47)
+    bad = rf.copy()
+    bad[bad.shape[0] // 2, 3, 3] = np.nan
+    with pytest.raises(mac.MacError) as e:
+        run_ml_gpu(tl, fc, ff, rc, bad, kw)
+    assert e.value.code == 3
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, max_iter=2, check=False)
+    st = g["stats"]
+    assert st["rc"] == 4 and st["iters"] == 2 and not st["converged"]
+    assert 0 < st["res"] < st["res0"]
