@@ -67,42 +67,65 @@ W_COARSE, W_FINE0, W_FINE1 = 8.0 / 15.0, 2.0 / 3.0, -1.0 / 5.0
 
 class TwoLevel:
     """Coarse domain of ``nc`` cells (spacing ``hc``, physical BCs ``bc``) and
-    one refined rectangle, fine cell indices ``rlo``..``rhi`` inclusive (even
-    and odd respectively: it covers whole coarse cells)."""
+    a fine level made of the boxes ``boxes`` (fine cell indices, inclusive
+    (lo, hi) pairs, lo even and hi odd: each covers whole coarse cells; any
+    union, disjoint boxes).  ``rlo``/``rhi`` alone give one box.  Fine
+    arrays cover the boxes' bounding rectangle (``rlo``..``rhi``); its cells
+    outside the boxes (holes) carry no unknown."""
 
-    def __init__(self, dim, nc, hc, bc, rlo, rhi):
+    def __init__(self, dim, nc, hc, bc, rlo=None, rhi=None, boxes=None):
         self.dim = dim
         self.nc = [int(x) for x in nc] + [1] * (3 - len(nc))
         self.hc = [float(x) for x in hc] + [1.0] * (3 - len(hc))
         self.hf = [h / 2 for h in self.hc]
         self.bc = [list(bc[d]) if d < dim else [O.PERIODIC, O.PERIODIC] for d in range(3)]
-        self.rlo = [int(x) for x in rlo] + [0] * (3 - len(rlo))
-        self.rhi = [int(x) for x in rhi] + [0] * (3 - len(rhi))
+        if boxes is None:
+            boxes = [(list(rlo), list(rhi))]
+        self.boxes = [([int(x) for x in lo] + [0] * (3 - len(lo)), [int(x) for x in hi] + [0] * (3 - len(hi)))
+                      for lo, hi in boxes]
+        self.rlo = [min(b[0][d] for b in self.boxes) for d in range(3)]
+        self.rhi = [max(b[1][d] for b in self.boxes) for d in range(3)]
         self.nr = [self.rhi[d] - self.rlo[d] + 1 for d in range(3)]
         self.kind = [[O.lib().orc_ell_bc(self.bc[d][s]) for s in range(2)] for d in range(3)]
-        for d in range(dim):
-            nf = 2 * self.nc[d]
-            assert self.rlo[d] % 2 == 0 and self.rhi[d] % 2 == 1, "region must cover whole coarse cells"
-            assert 0 <= self.rlo[d] <= self.rhi[d] < nf
-        # class of each region side: "per" (spans a periodic direction),
-        # "phys" (on a non-periodic domain side) or "cf" (coarse-fine)
-        self.side = [["per", "per"] for _ in range(3)]
-        for d in range(dim):
+        for lo, hi in self.boxes:
+            for d in range(dim):
+                nf = 2 * self.nc[d]
+                assert lo[d] % 2 == 0 and hi[d] % 2 == 1, "boxes must cover whole coarse cells"
+                assert 0 <= lo[d] <= hi[d] < nf
+        # fine cells of the boxes over the bounding rectangle, and the
+        # covered coarse cells over the coarse domain
+        self.mask = np.zeros((self.nr[2], self.nr[1], self.nr[0]), dtype=bool)
+        self.cmask = np.zeros((self.nc[2], self.nc[1], self.nc[0]), dtype=bool)
+        for lo, hi in self.boxes:
+            sl = tuple(slice(lo[d] - self.rlo[d], hi[d] - self.rlo[d] + 1) for d in (2, 1, 0))
+            assert not self.mask[sl].any(), "boxes overlap"
+            self.mask[sl] = True
+            self.cmask[tuple(slice(lo[d] // 2, hi[d] // 2 + 1) for d in (2, 1, 0))] = True
+        self.ncc = self.nc[0] * self.nc[1] * self.nc[2]
+        self.nfc = self.nr[0] * self.nr[1] * self.nr[2]
+
+    @property
+    def side(self):
+        """Class of each side of the bounding rectangle: "per" (it spans a
+        periodic direction), "phys" (on a non-periodic domain side) or
+        "cf" (inside the domain, or against a periodic seam it does not
+        span)."""
+        out = [["per", "per"] for _ in range(3)]
+        for d in range(self.dim):
             per = self.kind[d][0] == O.ELL_PER
             touch = (self.rlo[d] == 0, self.rhi[d] == 2 * self.nc[d] - 1)
             for s in range(2):
                 if per and touch[0] and touch[1]:
-                    self.side[d][s] = "per"
+                    out[d][s] = "per"
                 elif touch[s] and not per:
-                    self.side[d][s] = "phys"
+                    out[d][s] = "phys"
                 else:
-                    self.side[d][s] = "cf"
-        self.ncc = self.nc[0] * self.nc[1] * self.nc[2]
-        self.nfc = self.nr[0] * self.nr[1] * self.nr[2]
+                    out[d][s] = "cf"
+        return out
 
     # ------------------------------------------------------------ indexing
     def covered(self, c):
-        return all(self.rlo[d] // 2 <= c[d] <= self.rhi[d] // 2 for d in range(self.dim))
+        return bool(self.cmask[c[2], c[1], c[0]])
 
     def cid(self, c):
         return c[0] + self.nc[0] * (c[1] + self.nc[1] * c[2])
@@ -112,7 +135,22 @@ class TwoLevel:
         return self.ncc + r[0] + self.nr[0] * (r[1] + self.nr[1] * r[2])
 
     def in_region(self, f):
-        return all(self.rlo[d] <= f[d] <= self.rhi[d] for d in range(self.dim))
+        if not all(self.rlo[d] <= f[d] <= self.rhi[d] for d in range(self.dim)):
+            return False
+        return bool(self.mask[f[2] - self.rlo[2], f[1] - self.rlo[1], f[0] - self.rlo[0]])
+
+    def fine_nb(self, f, d, step):
+        """Fine neighbour of f along d in the fine domain (periodic wrap), or
+        None beyond a non-periodic side."""
+        q = list(f)
+        q[d] += step
+        nf = 2 * self.nc[d]
+        if 0 <= q[d] < nf:
+            return q
+        if self.kind[d][0] == O.ELL_PER:
+            q[d] %= nf
+            return q
+        return None
 
     def coarse_nb(self, c, d, step):
         """Coarse neighbour of c along d (periodic wrap) or None beyond a wall."""
@@ -126,12 +164,7 @@ class TwoLevel:
         return None
 
     def covered_mask(self):
-        m = np.zeros((self.nc[2], self.nc[1], self.nc[0]), dtype=bool)
-        sl = [slice(None)] * 3
-        for d in range(self.dim):
-            sl[2 - d] = slice(self.rlo[d] // 2, self.rhi[d] // 2 + 1)
-        m[tuple(sl)] = True
-        return m
+        return self.cmask.copy()
 
     # ------------------------------------------------------------- fluxes
     # Each flux is a dict {unknown: coefficient} of beta d(phi)/dx_d at a
@@ -146,6 +179,7 @@ class TwoLevel:
         inward = 1 if s == 0 else -1
         f1 = list(f0)
         f1[d] += inward
+        assert self.in_region(f1)
         C = [f0[e] // 2 for e in range(3)]
         C = self.coarse_nb(C, d, -inward)
         assert C is not None and not self.covered(C)
@@ -154,7 +188,13 @@ class TwoLevel:
             if t == d:
                 continue
             sig = -0.25 if f0[t] % 2 == 0 else 0.25
+            # transverse neighbours that are composite unknowns (not beyond a
+            # wall, not covered by the fine level)
             cp, cm = self.coarse_nb(C, t, 1), self.coarse_nb(C, t, -1)
+            if cp is not None and self.covered(cp):
+                cp = None
+            if cm is not None and self.covered(cm):
+                cm = None
             if cp is not None and cm is not None:
                 self._acc(cint, {self.cid(cp): 0.5, self.cid(cm): -0.5}, sig)
             elif cp is not None:
@@ -174,20 +214,15 @@ class TwoLevel:
         fr[d] += s
         beta = bf[d][fr[2], fr[1], fr[0]]
         h = self.hf[d]
-        nb = list(f)
-        nb[d] += 1 if s else -1
-        if self.in_region(nb):
-            lo, hi = (f, nb) if s else (nb, f)
-            return {self.fid(hi): beta / h, self.fid(lo): -beta / h}
-        cls = self.side[d][s]
-        if cls == "per":
-            nb[d] = self.rlo[d] + (nb[d] - self.rlo[d]) % self.nr[d]
-            lo, hi = (f, nb) if s else (nb, f)
-            return {self.fid(hi): beta / h, self.fid(lo): -beta / h}
-        if cls == "phys":
+        nb = self.fine_nb(f, d, 1 if s else -1)
+        if nb is None:
+            # a physical side of the domain
             if self.kind[d][s] == O.ELL_DIR:
                 return {self.fid(f): (-2.0 if s else 2.0) * beta / h}
             return {}
+        if self.in_region(nb):
+            lo, hi = (f, nb) if s else (nb, f)
+            return {self.fid(hi): beta / h, self.fid(lo): -beta / h}
         g = self.cf_ghost(f, d, s)
         out = {}
         if s == 0:
@@ -228,6 +263,12 @@ class TwoLevel:
         return out
 
     # ----------------------------------------------------------- operator
+    def fine_in_mask(self):
+        """Fine cells of the boxes, in bounding-rectangle order."""
+        for f in self.fine_cells():
+            if self.in_region(f):
+                yield f
+
     def cells(self):
         nz = self.nc[2]
         for K in range(nz):
@@ -263,6 +304,9 @@ class TwoLevel:
                 put(r, self.coarse_flux(bc_, bf, c, d, 0), 1.0 / self.hc[d])
         for f in self.fine_cells():
             r = self.fid(f)
+            if not self.in_region(f):
+                put(r, {r: 1.0}, 1.0)     # a hole of the bounding rectangle
+                continue
             for d in range(self.dim):
                 put(r, self.fine_flux(bf, f, d, 1), -1.0 / self.hf[d])
                 put(r, self.fine_flux(bf, f, d, 0), 1.0 / self.hf[d])
@@ -284,70 +328,98 @@ class TwoLevel:
     def volume_weights(self):
         """Composite cell volumes in fine-cell units (covered coarse cells 0)."""
         wc = np.where(self.covered_mask(), 0.0, float(2 ** self.dim)).ravel()
-        return np.concatenate([wc, np.ones(self.nfc)])
+        return np.concatenate([wc, np.where(self.mask, 1.0, 0.0).ravel()])
 
     # ------------------------------------------------------ level transfers
+    def _block_mean(self, a, fz, fy, fx):
+        """Means over blocks of fz x fy x fx of a (nz, ny, nx) array."""
+        nz, ny, nx = a.shape
+        return a.reshape(nz // fz, fz, ny // fy, fy, nx // fx, fx).mean(axis=(1, 3, 5))
+
+    def _cbox(self, extra=(0, 0, 0)):
+        """Slices of the bounding rectangle's coarse range (plus extra cells
+        per direction) in (z, y, x) order."""
+        return tuple(slice(self.rlo[d] // 2, self.rhi[d] // 2 + 1 + extra[d]) if d < self.dim else slice(None)
+                     for d in (2, 1, 0))
+
     def restrict_cells(self, xc, xf):
         """ml_cc_restriction: covered coarse cells <- mean of their fine cells."""
         out = np.array(xc, dtype=np.float64, copy=True)
-        nz, ny, nx = xf.shape
-        if self.dim == 3:
-            m = xf.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).mean(axis=(1, 3, 5))
-            out[self.rlo[2] // 2:self.rhi[2] // 2 + 1, self.rlo[1] // 2:self.rhi[1] // 2 + 1,
-                self.rlo[0] // 2:self.rhi[0] // 2 + 1] = m
-        else:
-            m = xf.reshape(ny // 2, 2, nx // 2, 2).mean(axis=(1, 3))
-            out[0, self.rlo[1] // 2:self.rhi[1] // 2 + 1, self.rlo[0] // 2:self.rhi[0] // 2 + 1] = m
+        f = (2 if self.dim == 3 else 1, 2, 2)
+        m = self._block_mean(np.asarray(xf, dtype=np.float64), *f)
+        sl = self._cbox()
+        cov = self.cmask[sl]
+        sub = out[sl]
+        sub[cov] = m[cov]
+        out[sl] = sub
         return out
 
+    def face_under_fine(self, d):
+        """Coarse faces of direction d (over the bounding rectangle's coarse
+        face range) with a covered cell on at least one side."""
+        sl = self._cbox(tuple(1 if e == d else 0 for e in range(3)))
+        shp = [x.stop - x.start if x.start is not None else n for x, n in zip(sl, self.cmask.shape)]
+        m = np.zeros(shp, dtype=bool)
+        ax = 2 - d
+        lo = self.rlo[d] // 2
+        for side in (-1, 0):
+            idx = np.arange(shp[ax]) + lo + side
+            per = self.kind[d][0] == O.ELL_PER
+            ok = (idx >= 0) & (idx < self.nc[d])
+            if per:
+                idx = np.mod(idx, self.nc[d])
+                ok[:] = True
+            idx = np.clip(idx, 0, self.nc[d] - 1)
+            csl = list(self._cbox())
+            csl[ax] = idx
+            c = self.cmask[tuple(csl) if ax != 0 else (idx,) + tuple(csl[1:])]
+            c = np.where(np.expand_dims(ok, tuple(e for e in range(3) if e != ax)), c, False)
+            m |= c
+        return sl, m
+
     def restrict_faces(self, fc, ff):
-        """ml_edge_restriction: coarse faces under the region (its boundary
-        faces included) <- mean of the 2^(dim-1) fine faces."""
+        """ml_edge_restriction: coarse faces with a covered cell on either side
+        (faces of the fine level's boundary included) <- mean of the
+        2^(dim-1) fine faces; on a periodic seam the face's periodic image
+        follows where only one of the two images lies under the fine level."""
         out = [np.array(x, dtype=np.float64, copy=True) for x in fc]
         for d in range(self.dim):
             a = ff[d][:, :, ::2] if d == 0 else (ff[d][:, ::2, :] if d == 1 else ff[d][::2])
-            if self.dim == 3:
-                nz, ny, nx = a.shape
-                fz = 1 if d == 2 else 2
-                fy = 1 if d == 1 else 2
-                fx = 1 if d == 0 else 2
-                m = a.reshape(nz // fz, fz, ny // fy, fy, nx // fx, fx).mean(axis=(1, 3, 5))
-            else:
-                _, ny, nx = a.shape
-                fy = 1 if d == 1 else 2
-                fx = 1 if d == 0 else 2
-                m = a.reshape(1, ny // fy, fy, nx // fx, fx).mean(axis=(2, 4))
-            sl = [slice(None)] * 3
-            for e in range(self.dim):
-                lo, hi = self.rlo[e] // 2, self.rhi[e] // 2 + 1 + (e == d)
-                sl[2 - e] = slice(lo, hi)
-            out[d][tuple(sl)] = m
-            # a region against a periodic seam (not spanning the direction)
-            # also sets the face's periodic image, so both cells of the seam
-            # see the same face (FBoxLib's own handling is unpinned)
-            if self.kind[d][0] == O.ELL_PER and self.side[d][0] == "cf":
-                src, dst = None, None
-                if self.rlo[d] == 0:
-                    src, dst = 0, self.nc[d]
-                elif self.rhi[d] == 2 * self.nc[d] - 1:
-                    src, dst = self.nc[d], 0
-                if src is not None:
-                    a, b = list(sl), list(sl)
-                    a[2 - d], b[2 - d] = src, dst
-                    out[d][tuple(b)] = out[d][tuple(a)]
+            f = (1 if d == 2 else (2 if self.dim == 3 else 1), 1 if d == 1 else 2, 1 if d == 0 else 2)
+            m = self._block_mean(np.asarray(a, dtype=np.float64), *f)
+            sl, under = self.face_under_fine(d)
+            sub = out[d][sl]
+            sub[under] = m[under]
+            out[d][sl] = sub
+            if self.kind[d][0] == O.ELL_PER:
+                ax = 2 - d
+                n = self.nc[d]
+                full = np.zeros(out[d].shape, dtype=bool)
+                fsub = full[sl]
+                fsub[under] = True
+                full[sl] = fsub
+                lo_i = [slice(None)] * 3
+                hi_i = [slice(None)] * 3
+                lo_i[ax], hi_i[ax] = 0, n
+                w0, wn = full[tuple(lo_i)], full[tuple(hi_i)]
+                v0, vn = out[d][tuple(lo_i)].copy(), out[d][tuple(hi_i)].copy()
+                out[d][tuple(hi_i)] = np.where(w0 & ~wn, v0, vn)
+                out[d][tuple(lo_i)] = np.where(wn & ~w0, vn, v0)
         return out
 
     def fine_face_fluxes(self, bf, x):
-        """beta G phi on every face of the region (composite fluxes)."""
+        """beta G phi on every face of the fine level (composite fluxes); 0 on
+        faces of the bounding rectangle between two holes."""
         out = []
         for d in range(self.dim):
             shp = [self.nr[2], self.nr[1], self.nr[0]]
             shp[2 - d] += 1
             F = np.zeros(shp)
-            for f in self.fine_cells():
+            for f in self.fine_in_mask():
                 r = [f[e] - self.rlo[e] for e in range(3)]
                 F[r[2], r[1], r[0]] = sum(v * x[k] for k, v in self.fine_flux(bf, f, d, 0).items())
-                if f[d] == self.rhi[d]:
+                nb = self.fine_nb(f, d, 1)
+                if r[d] == self.nr[d] - 1 or nb is None or not self.in_region(nb):
                     rr = list(r)
                     rr[d] += 1
                     F[rr[2], rr[1], rr[0]] = sum(v * x[k] for k, v in self.fine_flux(bf, f, d, 1).items())
@@ -393,19 +465,19 @@ def macproject2(tl: TwoLevel, faces_c, faces_f, rho_c, rho_f, macrhs_c=None, mac
             bc_ = O.scale_faces_1d(gc, bc_, beta0, beta0_edge, 1)
             bf = O.scale_faces_1d(gf, bf, b0f, b0ef, 1)
     b = tl.pack(rh_c, rh_f)
-    cov = tl.covered_mask().ravel()
-    b[: tl.ncc][cov] = 0.0
+    off = np.concatenate([tl.covered_mask().ravel(), ~tl.mask.ravel()])
+    b[off] = 0.0
     wv = tl.volume_weights()
     mean = 0.0
     if tl.singular():
         mean = math.fsum(wv * b) / wv.sum()
         b = b - mean
-        b[: tl.ncc][cov] = 0.0
+        b[off] = 0.0
     A = tl.matrix(bc_, bf)
     if tl.singular():
-        # pin the first fine cell, then shift to composite mean zero
+        # pin the first fine cell of the boxes, then shift to composite mean zero
         A = A.tolil()
-        k = tl.ncc
+        k = tl.ncc + int(np.argmax(tl.mask.ravel()))
         A.rows[k] = [k]
         A.data[k] = [1.0]
         A = A.tocsr()
@@ -417,7 +489,7 @@ def macproject2(tl: TwoLevel, faces_c, faces_f, rho_c, rho_f, macrhs_c=None, mac
     else:
         x = spla.spsolve(A.tocsc(), b)
     res = b - A @ x
-    res[: tl.ncc][cov] = 0.0
+    res[off] = 0.0
     phic, phif = tl.unpack(x.copy())
     phic = tl.restrict_cells(phic, phif)
     x[: tl.ncc] = phic.ravel()
@@ -450,4 +522,5 @@ def composite_divergence(tl: TwoLevel, faces_c, faces_f, beta0=None, beta0_edge=
     dc = -O.divumac(gc, *fc[:2], fc[2] if dim == 3 else None)
     df = -O.divumac(gf, *ff[:2], ff[2] if dim == 3 else None)
     dc[tl.covered_mask()] = 0.0
+    df[~tl.mask] = 0.0
     return dc, df
