@@ -3187,6 +3187,16 @@ __global__ void __launch_bounds__(TPB) k_fill_ghost(CorrectArgs A)
     if (cell[d] == A.n[d]) cell[d] -= 1;
     int mi = cell[0] / A.mapg[0], mj = cell[1] / A.mapg[1], mk = (DIM == 3) ? cell[2] / A.mapg[2] : 0;
     int ob = A.boxmap[mi + A.mapn[0] * (mj + A.mapn[1] * mk)];
+    if (ob < 0) {
+        // a level whose boxes leave holes (the fine level of a two-level
+        // projection): the face may end the box on its low side, else no
+        // box holds it
+        if (cell[d] == 0) return;
+        cell[d] -= 1;
+        mi = cell[0] / A.mapg[0]; mj = cell[1] / A.mapg[1]; mk = (DIM == 3) ? cell[2] / A.mapg[2] : 0;
+        ob = A.boxmap[mi + A.mapn[0] * (mj + A.mapn[1] * mk)];
+        if (ob < 0) return;
+    }
     const BoxDev OB = A.boxes[ob];
     int gi = G[0] + A.dlo[0], gj = G[1] + A.dlo[1], gk = (DIM == 3) ? G[2] + A.dlo[2] : 0;
     double v = A.umac[ob * DIM + d][boxoff(OB, ng, d, DIM, gi, gj, gk)];

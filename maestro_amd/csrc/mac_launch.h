@@ -159,6 +159,9 @@ struct MlGeom {
     double ihc[3], ihf[3];   // 1/h of the coarse and the fine level
     int ckind[3][2];         // elliptic kind of each physical side (ELL_*)
     int side[3][2];          // ML_SIDE_* of each side of the region
+    // the fine level's boxes: per coarse cell of the region (nr/2 per
+    // direction, x fastest), 1 where a fine box covers it; null: all
+    const unsigned char *cov;
 };
 // whole-level arrays the composite operator reads
 struct MlState {
@@ -180,6 +183,9 @@ struct MlGather {
     int alt;
     double *u[3], *beta[3], *rhs, *phi_out;
     int *err;
+    // the fine level: the region's covered-cell mask (see MlGeom), so a box
+    // also writes its high face where no box follows; null on the coarse level
+    const unsigned char *cov;
 };
 struct MlScatter {
     const BoxDev *boxes;
@@ -209,6 +215,11 @@ void launch_ml_prolong(const MlGeom &G, const double *e, double *phc, double *ph
 // the fine correction plan's folded weights on the region's coarse-fine
 // sides (level 0 of its hierarchy) scaled by f
 void launch_ml_cf_weights(const MlGeom &G, const Lev<double> &L, double f, hipStream_t s);
+// the fine correction plan's level 0 over a region with holes (cells no
+// fine box covers): alpha 0 on the boxes' cells and a large value on the
+// holes (their corrections stay zero), the faces between a box cell and a
+// hole scaled by f
+void launch_ml_holes(const MlGeom &G, const Lev<double> &L, double f, hipStream_t s);
 void launch_ml_axpy(long n, const double *x, double *y, hipStream_t s);
 void launch_ml_mkumac(const MlState &S, double *const uc[3], double *const uf[3], hipStream_t s);
 void launch_ml_unscale(int dim, const int n[3], int org_r, double *const u[3], const double *b0, const double *b0e,

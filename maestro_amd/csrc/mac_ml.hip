@@ -74,16 +74,42 @@ __device__ __forceinline__ bool ml_cnb(const MlGeom &G, int c[3], int d, int ste
     c[d] = (c[d] + G.nc[d]) % G.nc[d];
     return true;
 }
+// whether the region cell (region-relative coarse index q[], inside the
+// region) is covered by a fine box
+__device__ __forceinline__ bool ml_cov_at(const MlGeom &G, int q0, int q1, int q2)
+{
+    if (!G.cov) return true;
+    return G.cov[q0 + (G.nr[0] >> 1) * (q1 + (G.nr[1] >> 1) * q2)] != 0;
+}
 __device__ __forceinline__ bool ml_covered(const MlGeom &G, const int c[3])
 {
-    for (int d = 0; d < G.dim; ++d)
-        if (c[d] < (G.rlo[d] >> 1) || c[d] > ((G.rlo[d] + G.nr[d] - 1) >> 1)) return false;
-    return true;
+    int q[3] = {0, 0, 0};
+    for (int d = 0; d < G.dim; ++d) {
+        q[d] = c[d] - (G.rlo[d] >> 1);
+        if (q[d] < 0 || q[d] >= (G.nr[d] >> 1)) return false;
+    }
+    return ml_cov_at(G, q[0], q[1], q[2]);
 }
+// whether fine cell f (global fine index) belongs to a fine box
 __device__ __forceinline__ bool ml_in_region(const MlGeom &G, const int f[3])
 {
-    for (int d = 0; d < G.dim; ++d)
-        if (f[d] < G.rlo[d] || f[d] >= G.rlo[d] + G.nr[d]) return false;
+    int q[3] = {0, 0, 0};
+    for (int d = 0; d < G.dim; ++d) {
+        const int r = f[d] - G.rlo[d];
+        if (r < 0 || r >= G.nr[d]) return false;
+        q[d] = r >> 1;
+    }
+    return ml_cov_at(G, q[0], q[1], q[2]);
+}
+// fine neighbour of f along d in the fine domain (periodic wrap); false
+// beyond a non-periodic side
+__device__ __forceinline__ bool ml_fnb(const MlGeom &G, int f[3], int d, int step)
+{
+    f[d] += step;
+    const int nf = 2 * G.nc[d];
+    if (f[d] >= 0 && f[d] < nf) return true;
+    if (G.ckind[d][0] != ELL_PER) return false;
+    f[d] = (f[d] + nf) % nf;
     return true;
 }
 __device__ __forceinline__ double ml_phic(const MlState &S, const int c[3])
@@ -96,15 +122,23 @@ __device__ __forceinline__ double ml_phif(const MlState &S, const int f[3])
 }
 
 // the coarse value of c moved by sig[t] coarse cells along each transverse
-// direction t != d (d = -1: every direction), linear slopes of x
-__device__ double ml_lin(const MlGeom &G, const double *x, const int c[3], int d, const double sig[3])
+// direction t != d (d = -1: every direction), linear slopes of x (central,
+// one-sided where a neighbour is missing)
+__device__ double ml_lin(const MlGeom &G, const double *x, const int c[3], int d, const double sig[3],
+                         bool uncovered_only)
 {
     const double xc = x[nat_cell(G.nc, c[0], c[1], c[2])];
     double v = xc;
     for (int t = 0; t < G.dim; ++t) {
         if (t == d) continue;
         int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
-        const bool hp = ml_cnb(G, cp, t, 1), hm = ml_cnb(G, cm, t, -1);
+        bool hp = ml_cnb(G, cp, t, 1), hm = ml_cnb(G, cm, t, -1);
+        // the interface ghost uses composite unknowns only: a transverse
+        // neighbour under the fine level (a re-entrant corner) is skipped
+        if (uncovered_only) {
+            hp = hp && !ml_covered(G, cp);
+            hm = hm && !ml_covered(G, cm);
+        }
         double sl = 0.0;
         if (hp && hm) sl = 0.5 * (x[nat_cell(G.nc, cp[0], cp[1], cp[2])] - x[nat_cell(G.nc, cm[0], cm[1], cm[2])]);
         else if (hp) sl = x[nat_cell(G.nc, cp[0], cp[1], cp[2])] - xc;
@@ -124,16 +158,14 @@ __device__ double ml_fine_flux(const MlState &S, const int f[3], int d, int s)
     const double beta = S.bf[d][nat_face(G.nr, d, fr[0], fr[1], fr[2])];
     const double ih = G.ihf[d];
     int nb[3] = {f[0], f[1], f[2]};
-    nb[d] += s ? 1 : -1;
     const double p0 = ml_phif(S, f);
+    if (!ml_fnb(G, nb, d, s ? 1 : -1)) {
+        // a physical side of the domain
+        if (G.ckind[d][s] == ELL_DIR) return (s ? -2.0 : 2.0) * beta * ih * p0;
+        return 0.0;
+    }
     if (!ml_in_region(G, nb)) {
-        const int cls = G.side[d][s];
-        if (cls == ML_SIDE_PER) {
-            nb[d] = G.rlo[d] + (nb[d] - G.rlo[d] + G.nr[d]) % G.nr[d];
-        } else if (cls == ML_SIDE_PHYS) {
-            if (G.ckind[d][s] == ELL_DIR) return (s ? -2.0 : 2.0) * beta * ih * p0;
-            return 0.0;
-        } else {
+        {
             // coarse-fine face
             const int inward = s ? -1 : 1;
             int f1[3] = {f[0], f[1], f[2]};
@@ -142,7 +174,7 @@ __device__ double ml_fine_flux(const MlState &S, const int f[3], int d, int s)
             ml_cnb(G, C, d, -inward);
             double sig[3];
             for (int t = 0; t < 3; ++t) sig[t] = (f[t] & 1) ? 0.25 : -0.25;
-            const double ci = ml_lin(G, S.phc, C, d, sig);
+            const double ci = ml_lin(G, S.phc, C, d, sig, true);
             const double g = dfma(W_COARSE, ci, dfma(W_FINE0, p0, W_FINE1 * ml_phif(S, f1)));
             return s ? beta * ih * (g - p0) : beta * ih * (p0 - g);
         }
@@ -212,8 +244,16 @@ __global__ void __launch_bounds__(MTPB) k_ml_gather(MlGather A)
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
         const double *u = A.umac[b * DIM + d];
+        // the high face too where the cell ends the level's rectangle, or
+        // ends its box with no fine box after it (a face on a hole)
+        bool last = r[d] == A.n[d] - 1;
+        if (!last && A.cov && g[d] == B.hi[d]) {
+            int q[3] = {r[0] >> 1, r[1] >> 1, r[2] >> 1};
+            q[d] = (r[d] + 1) >> 1;
+            last = A.cov[q[0] + (A.n[0] >> 1) * (q[1] + (A.n[1] >> 1) * q[2])] == 0;
+        }
         for (int s = 0; s < 2; ++s) {
-            if (s == 1 && r[d] != A.n[d] - 1) break;
+            if (s == 1 && !last) break;
             int f[3] = {g[0], g[1], g[2]};
             f[d] += s;
             int nb[3] = {g[0], g[1], g[2]};
@@ -273,6 +313,15 @@ __global__ void __launch_bounds__(MTPB) k_ml_edge_restrict(MlGeom G, double *uc0
     }
     const double v = s / cnt;
     int c[3] = {(G.rlo[0] >> 1) + l[0], (G.rlo[1] >> 1) + l[1], (DIM == 3) ? (G.rlo[2] >> 1) + l[2] : 0};
+    // only faces with a covered coarse cell on a side (the low one through
+    // the periodic wrap)
+    if (G.cov) {
+        int hi_c[3] = {c[0], c[1], c[2]}, lo_c[3] = {c[0], c[1], c[2]};
+        bool cov = false;
+        if (hi_c[d] < G.nc[d]) cov = ml_covered(G, hi_c);
+        if (!cov && ml_cnb(G, lo_c, d, -1)) cov = ml_covered(G, lo_c);
+        if (!cov) return;
+    }
     uc[nat_face(G.nc, d, c[0], c[1], c[2])] = v;
     if (G.ckind[d][0] == ELL_PER && G.side[d][0] == ML_SIDE_CF) {
         if (c[d] == 0) c[d] = G.nc[d];
@@ -307,6 +356,7 @@ __global__ void __launch_bounds__(MTPB) k_ml_cell_restrict(MlGeom G, double *xc,
     const long q = (long)blockIdx.x * MTPB + threadIdx.x;
     if (q >= (long)m[0] * m[1] * m[2]) return;
     const int l[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
+    if (!ml_cov_at(G, l[0], l[1], l[2])) return;
     double s = 0.0;
     for (int k = 0; k < (DIM == 3 ? 2 : 1); ++k)
         for (int j = 0; j < 2; ++j)
@@ -329,7 +379,10 @@ __global__ void __launch_bounds__(MTPB) k_ml_comp_vec(MlGeom G, const double *rh
         const int c[3] = {(int)(q % G.nc[0]), (int)((q / G.nc[0]) % G.nc[1]), (int)(q / ((long)G.nc[0] * G.nc[1]))};
         out[q] = ml_covered(G, c) ? 0.0 : (DIM == 3 ? 8.0 : 4.0) * rhc[q];
     } else {
-        out[q] = rhf[q - ncc];
+        const long p = q - ncc;
+        const int f[3] = {G.rlo[0] + (int)(p % G.nr[0]), G.rlo[1] + (int)((p / G.nr[0]) % G.nr[1]),
+                          G.rlo[2] + (int)(p / ((long)G.nr[0] * G.nr[1]))};
+        out[q] = ml_in_region(G, f) ? rhf[p] : 0.0;
     }
 }
 
@@ -357,11 +410,14 @@ __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double
         if (!ml_covered(G, c)) {
             bool plain = true;
             for (int d = 0; d < DIM; ++d) {
-                if (c[d] == 0 || c[d] == G.nc[d] - 1) plain = false;
-                // a covered neighbour along d: the cell lies just outside the
-                // region's coarse range in d and inside it transversely
-                const int lo = G.rlo[d] >> 1, hi = (G.rlo[d] + G.nr[d] - 1) >> 1;
-                if (c[d] == lo - 1 || c[d] == hi + 1) plain = false;
+                if (c[d] == 0 || c[d] == G.nc[d] - 1) {
+                    plain = false;
+                    break;
+                }
+                int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
+                cp[d] += 1;
+                cm[d] -= 1;
+                if (ml_covered(G, cp) || ml_covered(G, cm)) plain = false;
             }
             double Ap = 0.0;
             if (plain) {
@@ -386,9 +442,19 @@ __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double
         const int p = q - ncc;
         const int t = p / G.nr[0];
         const int r3[3] = {p - t * G.nr[0], t % G.nr[1], t / G.nr[1]};
+        const int fg[3] = {G.rlo[0] + r3[0], G.rlo[1] + r3[1], G.rlo[2] + r3[2]};
+        if (!ml_in_region(G, fg)) continue;   // a hole of the region
         bool plain = true;
-        for (int d = 0; d < DIM; ++d)
-            if (r3[d] == 0 || r3[d] == G.nr[d] - 1) plain = false;
+        for (int d = 0; d < DIM; ++d) {
+            if (r3[d] == 0 || r3[d] == G.nr[d] - 1) {
+                plain = false;
+                break;
+            }
+            int fp[3] = {fg[0], fg[1], fg[2]}, fm[3] = {fg[0], fg[1], fg[2]};
+            fp[d] += 1;
+            fm[d] -= 1;
+            if (!ml_in_region(G, fp) || !ml_in_region(G, fm)) plain = false;
+        }
         double Ap = 0.0;
         if (plain) {
             const double pc = S.phf[p];
@@ -437,7 +503,7 @@ __global__ void __launch_bounds__(MTPB) k_ml_prolong(MlGeom G, const double *e, 
         const int C[3] = {f[0] >> 1, f[1] >> 1, f[2] >> 1};
         double sig[3];
         for (int t = 0; t < 3; ++t) sig[t] = (f[t] & 1) ? 0.25 : -0.25;
-        phf[p] += ml_lin(G, e, C, -1, sig);
+        if (ml_in_region(G, f)) phf[p] += ml_lin(G, e, C, -1, sig, false);
     }
 }
 
@@ -476,6 +542,45 @@ __global__ void __launch_bounds__(MTPB) k_ml_cf_weights(MlGeom G, Lev<double> L,
     }
 }
 
+// Holes of the region in the fine correction plan's level 0: alpha 0 on
+// the boxes' cells and, on a hole, 1e4 times the largest diagonal its faces
+// could give (so its correction stays ~1e-4 of its neighbours'); the face
+// between a box cell and a hole scaled by f (the interface weight, as on the
+// region's own coarse-fine sides).  Each thread owns its cell's low faces
+// (and the high slab faces across a periodic wrap).
+__global__ void __launch_bounds__(MTPB) k_ml_holes(MlGeom G, Lev<double> L, double f)
+{
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int q = blockIdx.x * MTPB + threadIdx.x;
+    if (q >= nx * ny * nz) return;
+    const int t = q / nx;
+    const int r[3] = {q - t * nx, t % ny, t / ny};
+    const int dim = G.dim;
+    auto inb = [&](int a, int b, int c) { return ml_cov_at(G, a >> 1, b >> 1, c >> 1); };
+    const bool in = inb(r[0], r[1], r[2]);
+    const int n[3] = {nx, ny, nz};
+    const int per[3] = {L.perx, L.pery, L.perz};
+    double *lo_w[3] = {L.bx, L.by, L.bz};
+    double wmax = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        int m[3] = {r[0], r[1], r[2]};
+        if (r[d] > 0 || per[d]) {
+            m[d] = (r[d] + n[d] - 1) % n[d];
+            if (inb(m[0], m[1], m[2]) != in) lo_w[d][q] *= f;
+        }
+        wmax = fmax(wmax, lo_w[d][q]);
+        if (r[d] == n[d] - 1 && per[d]) {
+            int p[3] = {r[0], r[1], r[2]};
+            p[d] = 0;
+            double *sl = (d == 0) ? &L.bxh[r[1] + (long)ny * r[2]]
+                                  : (d == 1 ? &L.byh[r[0] + (long)nx * r[2]] : &L.bzh[r[0] + (long)nx * r[1]]);
+            if (inb(p[0], p[1], p[2]) != in) *sl *= f;
+        }
+    }
+    const double ih2 = (double)L.ih2x + (double)L.ih2y + (double)L.ih2z;
+    L.alpha[q] = in ? 0.0 : 1e4 * 2.0 * ih2 * fmax(wmax, 1.0);
+}
+
 // ------------------------------------------------------------------ mkumac
 // fine faces: U -= beta G phi with the composite flux (C-F ghosts at the
 // region's C-F faces, the physical rule on its physical sides)
@@ -490,11 +595,14 @@ __global__ void __launch_bounds__(MTPB) k_ml_mkumac_fine(MlState S, double *u0, 
     const long q = (long)blockIdx.x * MTPB + threadIdx.x;
     if (q >= (long)m[0] * m[1] * m[2]) return;
     int r[3] = {(int)(q % m[0]), (int)((q / m[0]) % m[1]), (int)(q / ((long)m[0] * m[1]))};
-    // the face is the low face of cell r, or the high face of the last cell
-    int s = 0;
-    if (r[d] == G.nr[d]) { r[d] -= 1; s = 1; }
-    const int f[3] = {G.rlo[0] + r[0], G.rlo[1] + r[1], G.rlo[2] + r[2]};
-    u[q] -= ml_fine_flux(S, f, d, s);
+    // the face's fine cells: the flux from the one in a fine box (the high
+    // one's low face, else the low one's high face); faces between holes of
+    // the region are no faces of the level
+    int fh[3] = {G.rlo[0] + r[0], G.rlo[1] + r[1], G.rlo[2] + r[2]};
+    int fl[3] = {fh[0], fh[1], fh[2]};
+    fl[d] -= 1;
+    if (r[d] < G.nr[d] && ml_in_region(G, fh)) u[q] -= ml_fine_flux(S, fh, d, 0);
+    else if (r[d] > 0 && ml_in_region(G, fl)) u[q] -= ml_fine_flux(S, fl, d, 1);
 }
 
 // coarse faces: U -= beta G phi with the single-level boundary rule (faces
@@ -579,14 +687,6 @@ __global__ void __launch_bounds__(MTPB) k_ml_grown(MlGrown A)
     if (q >= (long)e[0] * e[1] * e[2]) return;
     const int g[3] = {B.lo[0] - ng + (int)(q % e[0]), B.lo[1] - ng + (int)((q / e[0]) % e[1]),
                       (DIM == 3) ? B.lo[2] - ng + (int)(q / ((long)e[0] * e[1])) : 0};
-    // only faces beyond the region (in cell-index terms, a face index of the
-    // region's high end is still the region's)
-    bool inside = true;
-    for (int t = 0; t < DIM; ++t) {
-        const int top = A.G.rlo[t] + A.G.nr[t] - 1 + (t == d);
-        if (g[t] < A.G.rlo[t] || g[t] > top) inside = false;
-    }
-    if (inside) return;
     // map into the fine domain (periodic wrap); beyond a wall: physbc's
     int w[3];
     for (int t = 0; t < DIM; ++t) {
@@ -600,6 +700,13 @@ __global__ void __launch_bounds__(MTPB) k_ml_grown(MlGrown A)
         w[t] = x;
     }
     if (DIM == 2) w[2] = 0;
+    // a face of a fine cell is a face of the fine level (the box images
+    // fill it); only the others take the coarse faces
+    {
+        int hi_f[3] = {w[0], w[1], w[2]}, lo_f[3] = {w[0], w[1], w[2]};
+        if (hi_f[d] < 2 * A.G.nc[d] && ml_in_region(A.G, hi_f)) return;
+        if (ml_fnb(A.G, lo_f, d, -1) && ml_in_region(A.G, lo_f)) return;
+    }
     int c[3] = {w[0] >> 1, w[1] >> 1, w[2] >> 1};
     double v;
     if ((w[d] & 1) == 0) {
@@ -682,6 +789,12 @@ void launch_ml_cf_weights(const MlGeom &G, const Lev<double> &L, double f, hipSt
 {
     const long plane = std::max({(long)L.ny * L.nz, (long)L.nx * L.nz, (long)L.nx * L.ny});
     hipLaunchKernelGGL(k_ml_cf_weights, dim3(mblk(plane), 2 * G.dim), dim3(MTPB), 0, s, G, L, f);
+}
+
+void launch_ml_holes(const MlGeom &G, const Lev<double> &L, double f, hipStream_t s)
+{
+    const long n = (long)L.nx * L.ny * L.nz;
+    hipLaunchKernelGGL(k_ml_holes, dim3(mblk(n)), dim3(MTPB), 0, s, G, L, f);
 }
 
 void launch_ml_axpy(long n, const double *x, double *y, hipStream_t s)

@@ -831,7 +831,16 @@ static int remove_mean(mac_plan *P, double *x, long n)
 }
 
 // ---------------------------------------------------------------- plan
+// allow_holes: the boxes need not cover the domain (the owner map keeps -1
+// where none does); for the fine level of a two-level projection only
+static int plan_create(const mac_layout *lay, const mac_opts *opts, mac_plan **out, bool allow_holes);
+
 extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_plan **out)
+{
+    return plan_create(lay, opts, out, false);
+}
+
+static int plan_create(const mac_layout *lay, const mac_opts *opts, mac_plan **out, bool allow_holes)
 {
     if (!lay || !out) return MAC_ERR_INVALID;
     *out = nullptr;
@@ -892,7 +901,7 @@ extern "C" int mac_plan_create(const mac_layout *lay, const mac_opts *opts, mac_
         covered += v;
         P->boxes[b] = B;
     }
-    if (covered != ncell(P->n)) { delete P; return MAC_ERR_LAYOUT; }
+    if (covered != ncell(P->n) && !(allow_holes && covered < ncell(P->n))) { delete P; return MAC_ERR_LAYOUT; }
     // owner map at the gcd resolution; also detects overlaps
     for (int d = 0; d < 3; ++d) {
         if (d >= P->dim || g[d] == 0) g[d] = 1;
@@ -2299,7 +2308,10 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
 // the answer is the composite operator's whatever the inner cycles do.
 struct mac_ml_plan {
     int dim = 0;
-    mac_plan *P[2] = {nullptr, nullptr};   // coarse layout; fine boxes over the region
+    mac_plan *P[2] = {nullptr, nullptr};   // coarse layout; the fine correction solver (one box: the region)
+    mac_plan *PF = nullptr;                 // the caller's fine boxes over the region (tables, ghost fill)
+    unsigned char *cov = nullptr;           // the region's covered coarse cells (device)
+    bool holes = false;                     // the fine boxes leave part of the region uncovered
     MlGeom G;
     double dxc[3] = {1, 1, 1}, dxf[3] = {1, 1, 1};
     int cphys[3][2];
@@ -2315,13 +2327,16 @@ struct mac_ml_plan {
 static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve (mixed)
 static constexpr int ML_SUB_MAXIT = 2;       // ... and its cycle limit
 static constexpr int ML_FP64_CYCLES = 1;     // FP64 inner solves: V-cycles per correction, no norms
-static constexpr double ML_CF_WEIGHT = 4.0 / 15.0;   // see k_ml_cf_weights
+static constexpr double ML_CF_WEIGHT = 4.0 / 15.0;   // see k_ml_cf_weights (folded: 2 beta -> 8/15 beta)
+static constexpr double ML_HOLE_WEIGHT = 8.0 / 15.0; // the same interface weight on an unfolded face
 
 extern "C" int mac_ml_plan_destroy(mac_ml_plan *M)
 {
     if (!M) return MAC_OK;
     if (M->P[0] && M->P[0]->stream) (void)hipStreamSynchronize(M->P[0]->stream);
     if (M->block) (void)hipFree(M->block);
+    if (M->cov) (void)hipFree(M->cov);
+    if (M->PF) mac_plan_destroy(M->PF);
     if (M->P[1]) {
         M->P[1]->stream = nullptr;   // the coarse plan's stream, destroyed with it
         mac_plan_destroy(M->P[1]);
@@ -2365,13 +2380,13 @@ extern "C" int mac_ml_plan_create(int nlevs, const mac_layout *lays, const mac_o
             G.ihf[d] = 1.0 / L1.dx[d];
             rlo[d] = L1.box_lo[d];
             rhi[d] = L1.box_hi[d];
-            for (int b = 1; b < L1.nboxes; ++b) {
+            for (int b = 0; b < L1.nboxes; ++b) {
                 rlo[d] = std::min(rlo[d], L1.box_lo[3 * b + d]);
                 rhi[d] = std::max(rhi[d], L1.box_hi[3 * b + d]);
+                // every box covers whole coarse cells (the fine box plan
+                // checks that the boxes are disjoint and inside the domain)
+                if ((L1.box_lo[3 * b + d] & 1) || !(L1.box_hi[3 * b + d] & 1)) { delete M; return MAC_ERR_LAYOUT; }
             }
-            // the region covers whole coarse cells (the fine plan checks that
-            // the boxes tile it)
-            if ((rlo[d] & 1) || !(rhi[d] & 1)) { delete M; return MAC_ERR_LAYOUT; }
             for (int s = 0; s < 2; ++s) {
                 M->cphys[d][s] = L0.phys_bc[d][s];
                 G.ckind[d][s] = ell_kind(L0.phys_bc[d][s]);
@@ -2411,7 +2426,44 @@ extern "C" int mac_ml_plan_create(int nlevs, const mac_layout *lays, const mac_o
     mac_opts po = M->opts;
     po.device_ptrs = 1;
     if ((rc = mac_plan_create(&L0, &po, &M->P[0]))) { mac_ml_plan_destroy(M); return rc; }
-    if ((rc = mac_plan_create(&F, &po, &M->P[1]))) { mac_ml_plan_destroy(M); return rc; }
+    // the caller's fine boxes over the region (holes allowed): box tables,
+    // owner map and ghost fill of the fine level
+    if ((rc = plan_create(&F, &po, &M->PF, true))) { mac_ml_plan_destroy(M); return rc; }
+    // the fine correction solver: one box, the region
+    {
+        mac_layout R = F;
+        int one_lo[3] = {rlo[0], rlo[1], rlo[2]}, one_hi[3] = {rhi[0], rhi[1], rhi[2]};
+        R.nboxes = 1;
+        R.box_lo = one_lo;
+        R.box_hi = one_hi;
+        if ((rc = mac_plan_create(&R, &po, &M->P[1]))) { mac_ml_plan_destroy(M); return rc; }
+    }
+    // the covered coarse cells of the region
+    {
+        const int cn[3] = {G.nr[0] / 2, std::max(G.nr[1] / 2, 1), std::max(G.nr[2] / 2, 1)};
+        std::vector<unsigned char> cov((size_t)cn[0] * cn[1] * cn[2], 0);
+        long ncov = 0;
+        for (int b = 0; b < L1.nboxes; ++b) {
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            for (int d = 0; d < dim; ++d) {
+                lo[d] = (L1.box_lo[3 * b + d] - rlo[d]) / 2;
+                hi[d] = (L1.box_hi[3 * b + d] - rlo[d]) / 2;
+            }
+            for (int k = lo[2]; k <= hi[2]; ++k)
+                for (int j = lo[1]; j <= hi[1]; ++j)
+                    for (int i = lo[0]; i <= hi[0]; ++i) {
+                        cov[i + (size_t)cn[0] * (j + (size_t)cn[1] * k)] = 1;
+                        ++ncov;
+                    }
+        }
+        M->holes = ncov < (long)cov.size();
+        if (hipMalloc(&M->cov, cov.size()) != hipSuccess ||
+            hipMemcpy(M->cov, cov.data(), cov.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            mac_ml_plan_destroy(M);
+            return MAC_ERR_HIP;
+        }
+        G.cov = M->holes ? M->cov : nullptr;   // no mask: every region cell covered
+    }
     // one stream for both levels
     (void)hipStreamSynchronize(M->P[1]->stream);
     (void)hipStreamDestroy(M->P[1]->stream);
@@ -2444,9 +2496,11 @@ extern "C" int mac_ml_plan_create(int nlevs, const mac_layout *lays, const mac_o
     M->rhf = take(nfc); M->phf = take(nfc); M->mrf = take(nfc);
     M->zero = take(zmax);
     M->cvec = take(ncc + nfc);
-    if (hipMemset(M->zero, 0, sizeof(double) * zmax) != hipSuccess) { mac_ml_plan_destroy(M); return MAC_ERR_HIP; }
+    // everything zero once: the fine arrays' entries on holes of the region
+    // are never written (zero faces and coefficients there)
+    if (hipMemset(M->block, 0, tot * sizeof(double)) != hipSuccess) { mac_ml_plan_destroy(M); return MAC_ERR_HIP; }
     // the largest grown face box of a fine box (ghost faces of the fine level)
-    for (const BoxDev &B : M->P[1]->boxes)
+    for (const BoxDev &B : M->PF->boxes)
         for (int d = 0; d < dim; ++d) {
             long v = 1;
             for (int e = 0; e < dim; ++e) v *= B.hi[e] - B.lo[e] + 1 + 2 * L1.ng_umac + (e == d);
@@ -2535,7 +2589,7 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
     // ---- pointer tables and the gather of each level
     const long ncc = ncell(G.nc), nfc = ncell(G.nr);
     for (int l = 0; l < 2; ++l) {
-        mac_plan *P = M->P[l];
+        mac_plan *P = l ? M->PF : M->P[0];
         const int nb = P->nboxes;
         std::vector<double *> &tab = P->tab_host;
         for (int q = 0; q < nb * dim; ++q) tab[q] = umac[l][q];
@@ -2567,6 +2621,7 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
         A.rhs = l ? M->mrf : M->mrc;
         A.phi_out = l ? M->phf : M->phc;
         A.err = P0->err_dev;
+        A.cov = l ? G.cov : nullptr;
         launch_ml_gather(dim, A, nb, P->maxcells, s);
     }
     HIPCHK(hipGetLastError());
@@ -2617,6 +2672,13 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
         if ((rc = ensure_part(P, (n0 + 255) / 256))) return rc;
         launch_level_setup(dim, IO, P->H64.lev[0], P->part, s);
         if (l == 1) launch_ml_cf_weights(G, P->H64.lev[0], ML_CF_WEIGHT, s);
+        if (l == 1 && M->holes) {
+            // holes of the region: pinned near zero through alpha, their
+            // faces with the boxes' cells weighted as interface faces
+            if ((rc = set_alpha<double>(P, P->H64, true))) return rc;
+            launch_ml_holes(G, P->H64.lev[0], ML_HOLE_WEIGHT, s);
+            P->helm = true;
+        }
         ml_inner_opts(M, P);
         if (M->opts.precision == MAC_PREC_FP64) {
             // the weights stay for the whole call: the hierarchy once
@@ -2677,7 +2739,7 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
     }
     // ---- the caller's boxes, then their ghost faces
     for (int l = 0; l < 2; ++l) {
-        mac_plan *P = M->P[l];
+        mac_plan *P = l ? M->PF : M->P[0];
         const int nb = P->nboxes;
         MlScatter A;
         A.boxes = P->boxes_dev;
@@ -2767,7 +2829,7 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
     };
     size_t tot = 0;
     for (int l = 0; l < 2; ++l) {
-        mac_plan *P = M->P[l];
+        mac_plan *P = l ? M->PF : M->P[0];
         const int nb = P->nboxes;
         for (int b = 0; b < nb; ++b) {
             const BoxDev &B = P->boxes[b];
@@ -2799,7 +2861,7 @@ extern "C" int mac_ml_project(mac_ml_plan *M, double *const *const *umac, double
         size_t q = 0;
         const double *db0[2] = {nullptr, nullptr}, *db0e[2] = {nullptr, nullptr};
         for (int l = 0; l < 2; ++l) {
-            const int nb = M->P[l]->nboxes;
+            const int nb = (l ? M->PF : M->P[0])->nboxes;
             for (int b = 0; b < nb; ++b) {
                 for (int d = 0; d < dim; ++d) du[l].push_back(arrs[q++].dev);
                 dp[l].push_back(arrs[q++].dev);
