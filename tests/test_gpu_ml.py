@@ -219,42 +219,51 @@ def test_ml_mixed_precision_inner_cycles(oracle_lib, precision, omega):
     compare(tl, g, r)
 
 
-def test_ml_fine_ghost_faces(oracle_lib):
-    """Fine ghost faces beyond the refined region (inside the domain) take
-    the final coarse faces: a fine face on a coarse face its value, one
-    between two coarse faces their mean, transversely the coarse face that
-    contains it."""
+@pytest.mark.parametrize("boxes", [None, UNIONS[0][3]])
+def test_ml_fine_ghost_faces(oracle_lib, boxes):
+    """Fine ghost faces: a face of another fine box (the boxes' own images,
+    across holes of an L too) takes that box's value; a face beyond the fine
+    level (inside the domain) takes the final coarse faces: a fine face on a
+    coarse face its value, one between two coarse faces their mean,
+    transversely the coarse face that contains it."""
     from oracle import ml_oracle as M
     O = oracle_lib
-    dim, n, bc, lo, hi = 2, 16, [[P, P], [WALL, WALL]], [8, 10], [19, 21]
-    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 31)
+    dim, n, bc = 2, 16, [[P, P], [WALL, WALL]]
+    lo, hi = ([8, 10], [19, 21]) if boxes is None else ([8, 8], [23, 23])
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 31, boxes=boxes)
     g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=8, box_f=4)
-    uc = g["faces_c"]
-    checked = 0
+    uc, uf = g["faces_c"], g["faces_f"]
+    checked = [0, 0]
     for (blo, bhi), box in zip(g["boxes"][1], g["umac"][1]):
         for d in range(dim):
             a = box[d].cpu().numpy()
             for jj in range(a.shape[0]):
                 for ii in range(a.shape[1]):
                     gi, gj = blo[0] - 1 + ii, blo[1] - 1 + jj
-                    top = [tl.rhi[0] + (d == 0), tl.rhi[1] + (d == 1)]
-                    if tl.rlo[0] <= gi <= top[0] and tl.rlo[1] <= gj <= top[1]:
-                        continue
-                    w = [gi % (2 * n), gj]
+                    if blo[0] <= gi <= bhi[0] + (d == 0) and blo[1] <= gj <= bhi[1] + (d == 1):
+                        continue                          # a valid face of this box
                     if not (0 <= gj <= 2 * n - 1 + (d == 1)):
-                        continue
-                    if w[d] % 2 == 0:
+                        continue                          # beyond the walls: the physical rule
+                    w = [gi % (2 * n), gj]
+                    # a fine cell on either side: a face of the fine level
+                    cells = [tuple(w), (w[0] - (d == 0), w[1] - (d == 1))]
+                    if any(tl.in_region([c[0] % (2 * n), c[1], 0]) for c in cells if c[1] >= 0):
+                        r = [w[0] - tl.rlo[0], w[1] - tl.rlo[1]]
+                        exp = uf[d][0, r[1], r[0]]
+                        checked[0] += 1
+                    elif w[d] % 2 == 0:
                         c = [w[0] // 2, w[1] // 2]
                         c[d] = w[d] // 2
                         exp = uc[d][0, c[1], c[0]]
+                        checked[1] += 1
                     else:
                         c = [w[0] // 2, w[1] // 2]
                         c2 = list(c)
                         c2[d] += 1
                         exp = 0.5 * (uc[d][0, c[1], c[0]] + uc[d][0, c2[1], c2[0]])
+                        checked[1] += 1
                     assert abs(a[jj, ii] - exp) <= 1e-14 * max(1.0, abs(exp)), (d, gi, gj)
-                    checked += 1
-    assert checked > 0
+    assert checked[0] > 0 and checked[1] > 0
 
 
 def test_ml_host_arrays_match_device(oracle_lib):
