@@ -341,8 +341,8 @@ int mac_solve(mac_plan *plan, const mac_solve_in *in, mac_stats *stats);
  *   divumac :357-361 and :401-406, :217, :256-275, mkumac :829-833)
  * layouts[0]: the coarse level, boxes tiling its domain (which starts at
  * 0); layouts[1]: the fine level, domain the coarse one refined by 2, same
- * physical BCs and ghost conventions, boxes tiling one rectangle that
- * covers whole coarse cells.  The composite problem (FBoxLib ml_cc_solve,
+ * physical BCs and ghost conventions, disjoint boxes that each cover whole
+ * coarse cells (any union: an L, separate patches).  The composite problem (FBoxLib ml_cc_solve,
  * absent from the reference tree; Docs/mg/mg.tex:9-19,124-156): fine cells
  * next to a coarse-fine face see the ghost value 8/15 C + 2/3 phi_0 - 1/5
  * phi_1 (C: the coarse value across the face moved to the ghost's

@@ -19,8 +19,10 @@
 //                   fine region from the coarse faces (create_umac_grown)
 //
 // Whole-level arrays are C order, x fastest, no ghosts: coarse cells nc,
-// coarse faces nc + e_d; fine arrays cover the refined rectangle only
-// (region cells nr, faces nr + e_d), whose lowest fine cell is rlo.
+// coarse faces nc + e_d; fine arrays cover the region, the fine boxes'
+// bounding rectangle (region cells nr, faces nr + e_d), whose lowest fine
+// cell is rlo; MlGeom::cov marks the region's coarse cells a box covers
+// (the others are holes: no unknowns, never written).
 //
 // The composite operator at a C-F face: the fine cell f0 next to it sees
 // the ghost value

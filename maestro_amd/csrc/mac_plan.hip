@@ -2303,8 +2303,8 @@ extern "C" int mac_solve(mac_plan *P, const mac_solve_in *in, mac_stats *st)
 // residual: each step restricts the fine residual onto the covered coarse
 // cells, corrects with a few V-cycles of the coarse level's plan (whole
 // coarse domain) added on both levels, then corrects the fine level with a
-// few V-cycles of the fine plan (the refined rectangle, its coarse-fine
-// sides Dirichlet).  The composite residual alone decides convergence, so
+// few V-cycles of the fine plan (the region: the fine boxes' bounding
+// rectangle, its coarse-fine sides Dirichlet, its holes held near zero).  The composite residual alone decides convergence, so
 // the answer is the composite operator's whatever the inner cycles do.
 struct mac_ml_plan {
     int dim = 0;

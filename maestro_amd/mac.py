@@ -538,7 +538,7 @@ class MacPlan:
 
 class MacMLPlan:
     """Two-level plan (``mac_ml_plan_create``): ``layouts`` = [coarse, fine],
-    the fine level's boxes tiling one rectangle of whole coarse cells.  The
+    the fine level's boxes disjoint and each of whole coarse cells.  The
     arrays of ``project`` are all device tensors or all host numpy arrays
     (per level, per box)."""
 
