@@ -416,10 +416,17 @@ __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double
                     plain = false;
                     break;
                 }
-                int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
-                cp[d] += 1;
-                cm[d] -= 1;
-                if (ml_covered(G, cp) || ml_covered(G, cm)) plain = false;
+                if (G.cov) {
+                    int cp[3] = {c[0], c[1], c[2]}, cm[3] = {c[0], c[1], c[2]};
+                    cp[d] += 1;
+                    cm[d] -= 1;
+                    if (ml_covered(G, cp) || ml_covered(G, cm)) plain = false;
+                } else {
+                    // one rectangle: a covered neighbour along d means the
+                    // cell lies just outside its coarse range in d
+                    const int lo = G.rlo[d] >> 1, hi = (G.rlo[d] + G.nr[d] - 1) >> 1;
+                    if (c[d] == lo - 1 || c[d] == hi + 1) plain = false;
+                }
             }
             double Ap = 0.0;
             if (plain) {
@@ -452,10 +459,12 @@ __global__ void __launch_bounds__(MTPB) k_ml_resid(MlState S, double *rc, double
                 plain = false;
                 break;
             }
-            int fp[3] = {fg[0], fg[1], fg[2]}, fm[3] = {fg[0], fg[1], fg[2]};
-            fp[d] += 1;
-            fm[d] -= 1;
-            if (!ml_in_region(G, fp) || !ml_in_region(G, fm)) plain = false;
+            if (G.cov) {
+                int fp[3] = {fg[0], fg[1], fg[2]}, fm[3] = {fg[0], fg[1], fg[2]};
+                fp[d] += 1;
+                fm[d] -= 1;
+                if (!ml_in_region(G, fp) || !ml_in_region(G, fm)) plain = false;
+            }
         }
         double Ap = 0.0;
         if (plain) {
