@@ -401,30 +401,37 @@ def time_two_level(torch, steps=3, warmup=1):
         lays.append(mac.Layout(dim=3, boxes=boxes, domain_lo=(0, 0, 0), domain_hi=(n - 1,) * 3,
                                dx=(1.0 / n,) * 3, phys_bc=bc))
         args.append((du, dp, dr, cv(b0), cv(b0e)))
-    plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=1e-9))
     u0 = [[[a.clone() for a in b] for b in x[0]] for x in args]
-    ts, st = [], None
-    for k in range(warmup + steps):
-        for l in range(2):
-            for b, b_ in zip(args[l][0], u0[l]):
-                for a, a0 in zip(b, b_):
-                    a.copy_(a0)
-            for a in args[l][1]:
-                a.zero_()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        st = plan.project([x[0] for x in args], [x[1] for x in args], [x[2] for x in args], None,
-                          [x[3] for x in args], [x[4] for x in args])
-        torch.cuda.synchronize()
-        ts.append(time.perf_counter() - t)
-    plan.close()
-    tm = float(np.median(ts[warmup:]))
-    out.update({"ms": round(1e3 * tm, 3), "composite_steps": st["iters"], "inner_vcycles": st["inner_cycles"],
-                "res_over_res0": st["res"] / st["res0"], "cells": nc ** 3 + int(np.prod(nr)),
-                "setup_ms": round(st["t_setup_ms"], 3), "solve_ms": round(st["t_solve_ms"], 3),
-                "correct_ms": round(st["t_correct_ms"], 3)})
-    print(f"# two-level 128^3 + 128^3: {1e3 * tm:.2f} ms, {st['iters']} composite steps", file=sys.stderr,
-          flush=True)
+    out["cells"] = nc ** 3 + int(np.prod(nr))
+    # FP64 inner V-cycles (the headline of this leg), then FP32 inner cycles
+    # inside FP64 refinement: the composite residual is FP64 in both
+    for name, prec in (("fp64", mac.PREC_FP64), ("mixed", mac.PREC_MIXED)):
+        plan = mac.MacMLPlan(lays, mac.default_opts(eps_rel=1e-9, precision=prec))
+        ts, st = [], None
+        for k in range(warmup + steps):
+            for l in range(2):
+                for b, b_ in zip(args[l][0], u0[l]):
+                    for a, a0 in zip(b, b_):
+                        a.copy_(a0)
+                for a in args[l][1]:
+                    a.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st = plan.project([x[0] for x in args], [x[1] for x in args], [x[2] for x in args], None,
+                              [x[3] for x in args], [x[4] for x in args])
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        plan.close()
+        tm = float(np.median(ts[warmup:]))
+        r = {"ms": round(1e3 * tm, 3), "composite_steps": st["iters"], "inner_vcycles": st["inner_cycles"],
+             "res_over_res0": st["res"] / st["res0"], "setup_ms": round(st["t_setup_ms"], 3),
+             "solve_ms": round(st["t_solve_ms"], 3), "correct_ms": round(st["t_correct_ms"], 3)}
+        if name == "fp64":
+            out.update(r)
+        else:
+            out["mixed_inner"] = r
+        print(f"# two-level 128^3 + 128^3 {name}: {1e3 * tm:.2f} ms, {st['iters']} composite steps",
+              file=sys.stderr, flush=True)
     return out
 
 

@@ -2324,8 +2324,6 @@ struct mac_ml_plan {
     mac_opts opts;
 };
 
-static constexpr double ML_SUB_EPS = 1e-2;   // relative target of each inner correction solve (mixed)
-static constexpr int ML_SUB_MAXIT = 2;       // ... and its cycle limit
 static constexpr int ML_FP64_CYCLES = 1;     // FP64 inner solves: V-cycles per correction, no norms
 static constexpr double ML_CF_WEIGHT = 4.0 / 15.0;   // see k_ml_cf_weights (folded: 2 beta -> 8/15 beta)
 static constexpr double ML_HOLE_WEIGHT = 8.0 / 15.0; // the same interface weight on an unfolded face
@@ -2517,21 +2515,21 @@ extern "C" int mac_ml_plan_set_opts(mac_ml_plan *M, const mac_opts *o)
     return MAC_OK;
 }
 
-// copy of a plan's solver options for the inner correction solves
+// the plan's solver options for the inner correction cycles (cycle shape,
+// sweeps, bottom solver, over-relaxation) from the two-level call's
 static void ml_inner_opts(mac_ml_plan *M, mac_plan *P)
 {
     P->opts = M->opts;
     P->opts.device_ptrs = 1;
-    P->opts.eps_rel = ML_SUB_EPS;
-    P->opts.eps_abs = -1.0;
-    P->opts.max_iter = ML_SUB_MAXIT;
     P->opts.verbose = 0;
     P->planar = false;
 }
 
 // one inner correction solve on level 0 of P (right side written, phi
-// cleared): FP64, a fixed number of V-cycles without residual norms (no
-// host round trip); the mixed precisions, refinement steps to ML_SUB_EPS
+// cleared), without residual norms (no host round trip): FP64, one V-cycle
+// of the FP64 hierarchy; the mixed precisions, one FP32 V-cycle from zero on
+// the FP32 copy of the right side, widened back into the FP64 phi (the
+// composite residual stays FP64, so the answer does not change)
 static int ml_inner(mac_ml_plan *M, mac_plan *P, int *cycles)
 {
     if (M->opts.precision == MAC_PREC_FP64) {
@@ -2539,10 +2537,25 @@ static int ml_inner(mac_ml_plan *M, mac_plan *P, int *cycles)
         *cycles += ML_FP64_CYCLES;
         return MAC_OK;
     }
-    mac_stats is;
-    const int src = solve(P, &is);
-    if (src != MAC_OK && src != MAC_ERR_NOT_CONVERGED) return src;
-    *cycles += is.inner_cycles;
+    const long n = ncell(P->n);
+    Lev<double> &L = P->H64.lev[0];
+    launch_convert<double, float>(L.rhs, P->H32.lev[0].rhs, n, P->stream);
+    vcycle<float>(P, P->H32, true);
+    launch_convert<float, double>(P->H32.lev[0].phi, L.phi, n, P->stream);
+    *cycles += 1;
+    return MAC_OK;
+}
+
+// the FP32 hierarchy of an inner solver plan from its FP64 level 0 (weights,
+// and alpha where the holes set it), once per call
+static int ml_inner_f32(mac_plan *P)
+{
+    int rc;
+    if ((rc = build_hier<float>(P, P->H32))) return rc;
+    set_omega(P->H32, sweep_omega(P));
+    if ((rc = set_alpha<float>(P, P->H32, P->helm && P->H64.alpha_on))) return rc;
+    launch_weights_to_f32(P->dim, P->H64.lev[0], P->H32.lev[0], P->stream);
+    coarsen_all<float>(P, P->H32);
     return MAC_OK;
 }
 
@@ -2680,10 +2693,12 @@ static int ml_project_dev(mac_ml_plan *M, double *const *const *umac, double *co
             P->helm = true;
         }
         ml_inner_opts(M, P);
+        // the weights stay for the whole call: the hierarchy once
         if (M->opts.precision == MAC_PREC_FP64) {
-            // the weights stay for the whole call: the hierarchy once
             set_omega(P->H64, sweep_omega(P));
             coarsen_all<double>(P, P->H64);
+        } else if ((rc = ml_inner_f32(P))) {
+            return rc;
         }
     }
     HIPCHK(hipGetLastError());
