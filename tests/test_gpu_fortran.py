@@ -219,18 +219,25 @@ def test_fortran_mac_multigrid_matches_binding_and_oracle(gpu, oracle_lib, tmp_p
 ML_DRIVER = os.path.join(ROOT, "maestro_amd", "fortran", "macproject_ml_driver")
 
 
-@pytest.mark.parametrize("case", [3, 5])
+@pytest.mark.parametrize("case", [3, 5, "L"])
 def test_fortran_two_level_matches_binding(oracle_lib, tmp_path, case):
     """macproject with nlevs = 2 from Fortran (the level-2 boxes in the
     ml_layout, beta0_1d(2, 0:) per level): the same library call as the
     Python binding's host-array path, so the same bits."""
     from oracle import ml_oracle as M
-    from .test_ml_oracle import REGIONS, random_two_level
+    from .test_ml_oracle import REGIONS, UNIONS, random_two_level
     from .test_gpu_ml import ml_box_arrays, run_ml_gpu
     O = oracle_lib
-    dim, n, bc, lo, hi = REGIONS[case]
+    boxes = None
+    if case == "L":
+        # an L of two fine boxes (level 2 as a union of boxes)
+        dim, n, bc, boxes = UNIONS[0]
+        lo = [min(b[0][d] for b in boxes) for d in range(dim)]
+        hi = [max(b[1][d] for b in boxes) for d in range(dim)]
+    else:
+        dim, n, bc, lo, hi = REGIONS[case]
     use_b0 = bc[dim - 1][0] != W.PERIODIC
-    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 41, beta0=use_b0)
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 41, beta0=use_b0, boxes=boxes)
     box = 4 if dim == 3 else 8
     cb, fb, rel, lv = ml_box_arrays(tl, fc, ff, rc, rf, box, box)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
