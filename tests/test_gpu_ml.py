@@ -354,3 +354,20 @@ def test_ml_box_unions_match_oracle(oracle_lib, dim, n, bc, boxes, alt):
     print(dim, n, boxes, alt, "composite steps", st["iters"], "inner cycles", st["inner_cycles"], flush=True)
     assert st["rc"] == 0 and st["converged"]
     compare(tl, g, r)
+
+
+@pytest.mark.parametrize("case", [0, 2])
+def test_ml_box_union_mixed_inner_cycles(oracle_lib, case):
+    """A union of fine boxes with FP32 inner cycles: the holes' alpha and the
+    interface weights in the FP32 hierarchy too; the answer is the oracle's."""
+    from oracle import ml_oracle as M
+    O = oracle_lib
+    dim, n, bc, boxes = UNIONS[case]
+    lo = [min(b[0][d] for b in boxes) for d in range(dim)]
+    hi = [max(b[1][d] for b in boxes) for d in range(dim)]
+    tl, fc, ff, rc, rf, kw = random_two_level(M, O, dim, n, bc, lo, hi, 59, beta0=bc[dim - 1][0] != P, boxes=boxes)
+    r = M.macproject2(tl, fc, ff, rc, rf, alt=True, **kw)
+    g = run_ml_gpu(tl, fc, ff, rc, rf, kw, box_c=4 if dim == 3 else 8, box_f=4, precision=1)
+    print(boxes, "mixed composite steps", g["stats"]["iters"], flush=True)
+    assert g["stats"]["rc"] == 0
+    compare(tl, g, r)
